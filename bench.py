@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 graph engine on directed R-MAT scale 26, edge factor 16.
+
+One step = one PageRank solve (FP32 ranks, Kahan gather, FP64 residual/dangling
+reductions) until the L1 change of an iteration drops below 1e-7, at most 100
+iterations.  The graph (~1.07e9 samples, deduplicated, seeded relabel) is generated on the
+GPU and stays in HBM; the timed region starts with every input resident.
+
+value        whole-job PageRank GTEPS = edges * iterations / wall time of the K solves
+roofline     the gather kernel (pr_gather_kernel): algorithmic bytes per launch
+             (8 E + (o + 16) V, o = offset width) / mean launch time from HIP events on the
+             engine stream, against 8 TB/s; traffic = measured HBM bytes per launch from the
+             rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/ when present
+cpu_baseline GraphMP's OpenMP engine shape (oracle restatement, edges in RAM) on the same
+             graph, timed over a bounded number of iterations
+extra        SSSP (frontier Bellman-Ford and delta-stepping, delta=64) on the same graph with
+             uint32 weights in [1,255], CC on the symmetrised graph, and the FP64 a-posteriori
+             error bound of the PageRank result.
+Only one GPU is used; --gpus N > 1 exits.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def load_traffic(scale, ef):
+    """HBM bytes per gather launch from the committed rocprofv3 PMC summary, if any."""
+    p = os.path.join(ROOT, "profiles", f"pr_gather_rmat_s{scale}_ef{ef}_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--tol", type=float, default=1e-7)
+    ap.add_argument("--max-iter", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
+    ap.add_argument("--no-extra", action="store_true", help="skip the SSSP and CC measurements")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the engine runs on a single MI355X)", file=sys.stderr)
+        return 2
+
+    import graphee_amd as gmp
+
+    scale, ef = args.scale, args.edge_factor
+    seed = 1000 + scale
+    eng = gmp.Engine(0)
+    t0 = time.perf_counter()
+    eng.generate_rmat(scale, ef, seed, weighted=True, num_shards=16)
+    V, E, P, _ = eng.info()
+    log(f"generated R-MAT s{scale} ef{ef}: V={V} E={E} in {time.perf_counter() - t0:.1f} s")
+
+    def solve():
+        return eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, want_ranks=False)
+
+    for i in range(args.warmup):
+        _, st = solve()
+        log(f"warmup {i}: {st.iterations} iterations, {st.ms_total:.1f} ms")
+    iters = 0
+    kms = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, st = solve()
+        iters += st.iterations
+        kms += st.ms_kernel
+        launches += st.launches
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    gteps = E * iters / wall / 1e9
+    o = 4 if E < 2**32 else 8
+    bytes_launch = 8 * E + (o + 16) * V
+    avg_launch_s = kms / 1e3 / max(launches, 1)
+    achieved = bytes_launch / avg_launch_s / 1e9
+    traffic = load_traffic(scale, ef)
+    log(f"pagerank: {iters / args.steps:.0f} iterations/solve, {ms_step:.1f} ms/solve, {gteps:.1f} GTEPS, "
+        f"gather {avg_launch_s * 1e3:.3f} ms/launch = {achieved:.0f} GB/s")
+    result = {
+        "metric": "pagerank_gteps_rmat_s26",
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic R-MAT generated on the GPU (a=0.57 b=0.19 c=0.19, seeded relabel, dedup)",
+        "config": {"workload": f"pagerank-rmat-s{scale}-ef{ef}-tol{args.tol:g}",
+                   "vertices": V, "edges": E, "iterations_per_step": iters / args.steps,
+                   "damping": 0.85, "max_iter": args.max_iter},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "pr_gather_kernel", "algorithmic_bytes_per_launch": bytes_launch,
+                     "avg_launch_ms": round(avg_launch_s * 1e3, 4)},
+        "cpu_baseline": None,
+        "extra": {},
+    }
+
+    if not args.no_cpu:
+        import oracle
+        ranks, _ = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol)
+        g = eng.download()
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        l1, linf = oracle.pr_step_residual(g["off"], g["idx"], g["outdeg"], ranks, threads=threads)
+        result["extra"]["pagerank_fp64_check"] = {
+            "step_residual_l1": l1, "step_residual_linf": linf, "l1_error_bound": l1 / 0.15,
+            "method": "one FP64 CPU step from the GPU ranks; ||x-x*||_1 <= ||T(x)-x||_1/(1-d)"}
+        log(f"FP64 check: ||T(x)-x||_1={l1:.3e} -> L1 error <= {l1 / 0.15:.3e}, Linf {linf:.3e}")
+        msg0 = np.full(V, 1.0 / V, dtype=np.float32)
+        _, st1 = oracle.graphmp_baseline(oracle.APP_PAGERANK, g["off"], g["idx"], g["bounds"], msg0, 1,
+                                         outdeg=g["outdeg"], threads=threads)
+        n = int(max(1, min(args.max_iter, args.cpu_seconds / max(st1.seconds, 1e-3))))
+        _, stc = oracle.graphmp_baseline(oracle.APP_PAGERANK, g["off"], g["idx"], g["bounds"], msg0, n,
+                                         outdeg=g["outdeg"], threads=threads)
+        cpu_gteps = stc.edges / stc.seconds / 1e9
+        result["cpu_baseline"] = {
+            "value": round(cpu_gteps, 4), "unit": "GTEPS", "cores": stc.threads, "kind": "port",
+            "sample": f"{stc.iterations} PageRank iterations of the GraphMP OpenMP engine restatement "
+                      f"(16 shards, guided schedule, edges in RAM) on the same s{scale} graph, "
+                      f"{stc.seconds:.1f} s"}
+        log(f"cpu baseline: {cpu_gteps:.3f} GTEPS on {stc.threads} threads ({stc.iterations} iterations)")
+        del g
+
+    if not args.no_extra:
+        src = eng.pick_source(seed)
+        for name, method in (("sssp_bf", gmp.SSSP_BF), ("sssp_delta64", gmp.SSSP_DELTA)):
+            eng.sssp(src, method=method, delta=64, want_dist=False)  # builds the out-CSR once
+            t0 = time.perf_counter()
+            d, st = eng.sssp(src, method=method, delta=64)
+            t = time.perf_counter() - t0
+            reached = None
+            if d is not None:
+                reached = int((d != 0xFFFFFFFF).sum())
+            result["extra"][name] = {"ms": round(st.ms_total, 3), "wall_ms": round(t * 1e3, 3),
+                                     "edges_relaxed": st.edges, "work_gteps": round(st.edges / st.ms_total / 1e6, 3),
+                                     "iterations": st.iterations, "source": src, "reached": reached}
+            log(f"{name}: {st.ms_total:.1f} ms, {st.iterations} iterations, {st.edges} relaxations")
+        eng.generate_rmat(scale, ef, seed, undirected=True, num_shards=16)
+        Vs, Es, _, _ = eng.info()
+        eng.cc(want_labels=False)
+        lab, st = eng.cc()
+        ncomp = int((lab == np.arange(Vs, dtype=np.uint32)).sum())
+        result["extra"]["cc"] = {"ms": round(st.ms_total, 3), "edges_symmetric": Es, "iterations": st.iterations,
+                                 "gteps": round(Es / st.ms_total / 1e6, 3), "components": ncomp}
+        log(f"cc: {st.ms_total:.1f} ms, {st.iterations} iterations, {ncomp} components")
+
+    eng.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,262 @@
+"""ctypes binding of libgmp.so, the gfx950 vertex-centric graph engine.
+
+The library is the product: HIP kernels plus the C ABI declared in include/gmp.h.  This
+module only loads it and converts NumPy arrays to plain pointers.  It never falls back
+to a CPU implementation: when libgmp.so is missing, `lib()` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgmp.so")
+
+OK, EINVAL, EIO, EFORMAT, EHIP, ENOMEM, ESTATE, EUNSUPPORTED = range(8)
+APP_PAGERANK, APP_SSSP, APP_CC, APP_SSSP_W = 1, 2, 3, 4
+SSSP_BF, SSSP_DELTA, SSSP_LEARNED = 0, 1, 2
+PR_NO_KAHAN, PR_COMPRESSED = 1, 2
+GRAPH_OFF64 = 1
+
+# every entry point include/gmp.h declares
+SYMBOLS = (
+    "gmp_last_error", "gmp_abi_version", "gmp_create", "gmp_destroy", "gmp_load_csr",
+    "gmp_load_shards", "gmp_generate_rmat", "gmp_graph_info", "gmp_graph_download",
+    "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_sssp", "gmp_cc",
+    "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards",
+)
+
+
+class GmpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[gmp error {code}] {msg}")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32), ("launches", ctypes.c_int32),
+        ("ms_total", ctypes.c_double), ("ms_kernel", ctypes.c_double),
+        ("edges", ctypes.c_uint64), ("vertices", ctypes.c_uint64),
+        ("residual", ctypes.c_double), ("dangling", ctypes.c_double),
+        ("changed", ctypes.c_uint64), ("bytes", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RmatParams(ctypes.Structure):
+    _fields_ = [
+        ("scale", ctypes.c_uint32), ("edge_factor", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+        ("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double),
+        ("undirected", ctypes.c_uint32), ("weighted", ctypes.c_uint32),
+        ("num_shards", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+    ]
+
+
+class PrParams(ctypes.Structure):
+    _fields_ = [
+        ("damping", ctypes.c_float), ("max_iter", ctypes.c_int32), ("tol", ctypes.c_double),
+        ("seed_vertex", ctypes.c_int64), ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+    ]
+
+
+class SsspParams(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int32), ("delta", ctypes.c_uint32),
+        ("unit_weights", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+_P = ctypes.c_void_p
+
+
+def lib():
+    """Loads libgmp.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "or python graphee_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    u32, u64, i32, i64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int64
+    sig = {
+        "gmp_last_error": ([], ctypes.c_char_p),
+        "gmp_abi_version": ([], ctypes.c_int),
+        "gmp_create": ([i32, ctypes.POINTER(_P)], ctypes.c_int),
+        "gmp_destroy": ([_P], ctypes.c_int),
+        "gmp_load_csr": ([_P, u32, u64, _P, _P, _P, u32, _P], ctypes.c_int),
+        "gmp_load_shards": ([_P, ctypes.c_char_p, u32, u32], ctypes.c_int),
+        "gmp_generate_rmat": ([_P, ctypes.POINTER(RmatParams)], ctypes.c_int),
+        "gmp_graph_info": ([_P, _P, _P, _P, _P], ctypes.c_int),
+        "gmp_graph_download": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
+        "gmp_pick_source": ([_P, u64, u32, _P], ctypes.c_int),
+        "gmp_run_app": ([_P, i32, i32, _P, _P, _P, _P, ctypes.POINTER(Stats)], ctypes.c_int),
+        "gmp_pagerank": ([_P, ctypes.POINTER(PrParams), _P, ctypes.POINTER(Stats)], ctypes.c_int),
+        "gmp_sssp": ([_P, u32, ctypes.POINTER(SsspParams), _P, ctypes.POINTER(Stats)], ctypes.c_int),
+        "gmp_cc": ([_P, _P, ctypes.POINTER(Stats)], ctypes.c_int),
+        "gmp_partition_bounds": ([_P, u32, u32, _P], ctypes.c_int),
+        "gmp_npy_read_u32": ([ctypes.c_char_p, _P, u64, _P], ctypes.c_int),
+        "gmp_npy_write_i32": ([ctypes.c_char_p, _P, u64], ctypes.c_int),
+        "gmp_write_shards": ([ctypes.c_char_p, u32, _P, _P, _P, u32, _P, _P], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def check(rc):
+    if rc != OK:
+        raise GmpError(rc, lib().gmp_last_error().decode(errors="replace"))
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ---- host-side format utilities (no GPU) ----------------------------------------------
+
+def partition_bounds(indeg, P):
+    indeg = np.ascontiguousarray(indeg, dtype=np.uint64)
+    b = np.zeros(P + 1, dtype=np.uint32)
+    check(lib().gmp_partition_bounds(_ptr(indeg), indeg.size, P, _ptr(b)))
+    return b
+
+
+def npy_read_u32(path):
+    n = ctypes.c_uint64(0)
+    check(lib().gmp_npy_read_u32(os.fsencode(path), None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, dtype=np.uint32)
+    check(lib().gmp_npy_read_u32(os.fsencode(path), _ptr(out), out.size, ctypes.byref(n)))
+    return out
+
+
+def npy_write_i32(path, data):
+    data = np.ascontiguousarray(data, dtype=np.int32)
+    check(lib().gmp_npy_write_i32(os.fsencode(path), _ptr(data), data.size))
+
+
+def write_shards(dir_, off, idx, bounds, weights=None, vertex_out=None):
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    V = off.size - 1
+    bounds = _u32(bounds)
+    w = None if weights is None else _u32(weights)
+    vo = None if vertex_out is None else np.ascontiguousarray(vertex_out, dtype=np.int32)
+    check(lib().gmp_write_shards(os.fsencode(dir_), V, _ptr(off), _ptr(_u32(idx)), _ptr(w),
+                                 bounds.size - 1, _ptr(bounds), _ptr(vo)))
+
+
+# ---- engine --------------------------------------------------------------------------
+
+class Engine:
+    """One engine per GPU: the graph stays in HBM between runs."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        check(lib().gmp_create(device, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().gmp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # graph
+    def load_csr(self, off, idx, weights=None, P=1, bounds=None):
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        idx = _u32(idx)
+        V = off.size - 1
+        w = None if weights is None else _u32(weights)
+        b = None if bounds is None else _u32(bounds)
+        check(lib().gmp_load_csr(self._h, V, int(off[-1]), _ptr(off), _ptr(idx), _ptr(w), P, _ptr(b)))
+
+    def load_shards(self, dir_, V, P):
+        check(lib().gmp_load_shards(self._h, os.fsencode(dir_), V, P))
+
+    def generate_rmat(self, scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, undirected=False,
+                      weighted=False, num_shards=1, off64=False):
+        p = RmatParams(scale, edge_factor, seed, a, b, c, int(undirected), int(weighted), num_shards,
+                       GRAPH_OFF64 if off64 else 0)
+        check(lib().gmp_generate_rmat(self._h, ctypes.byref(p)))
+
+    def info(self):
+        V, E, P, w = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().gmp_graph_info(self._h, ctypes.byref(V), ctypes.byref(E), ctypes.byref(P),
+                                   ctypes.byref(w)))
+        return V.value, E.value, P.value, bool(w.value)
+
+    def download(self, weights=False):
+        V, E, P, weighted = self.info()
+        off = np.zeros(V + 1, dtype=np.uint64)
+        idx = np.zeros(E, dtype=np.uint32)
+        w = np.zeros(E, dtype=np.uint32) if (weights and weighted) else None
+        bounds = np.zeros(P + 1, dtype=np.uint32)
+        od = np.zeros(V, dtype=np.uint32)
+        check(lib().gmp_graph_download(self._h, _ptr(off), _ptr(idx), _ptr(w), _ptr(bounds), _ptr(od)))
+        return {"off": off.astype(np.int64), "idx": idx, "w": w, "bounds": bounds, "outdeg": od}
+
+    def pick_source(self, seed, salt=3):
+        v = ctypes.c_uint32()
+        check(lib().gmp_pick_source(self._h, seed, salt, ctypes.byref(v)))
+        return v.value
+
+    # apps
+    def run_app(self, app, msg_init, max_iter, vertex_out=None):
+        V = self.info()[0]
+        dt = np.float32 if app == APP_PAGERANK else (np.uint32 if app == APP_SSSP_W else np.int32)
+        m0 = np.ascontiguousarray(msg_init, dtype=dt)
+        if m0.size != V:
+            raise ValueError("msg_init must have V entries")
+        msg = np.zeros(V, dtype=dt)
+        val = np.zeros(V, dtype=dt)
+        vo = None if vertex_out is None else np.ascontiguousarray(vertex_out, dtype=np.int32)
+        st = Stats()
+        check(lib().gmp_run_app(self._h, app, max_iter, _ptr(m0), _ptr(vo), _ptr(msg), _ptr(val),
+                                ctypes.byref(st)))
+        return msg, val, st
+
+    def pagerank(self, damping=0.85, max_iter=20, tol=0.0, seed_vertex=-1, kahan=True,
+                 compressed=False, want_ranks=True):
+        V = self.info()[0]
+        flags = (0 if kahan else PR_NO_KAHAN) | (PR_COMPRESSED if compressed else 0)
+        p = PrParams(damping, max_iter, tol, seed_vertex, flags, 0)
+        out = np.zeros(V, dtype=np.float32) if want_ranks else None
+        st = Stats()
+        check(lib().gmp_pagerank(self._h, ctypes.byref(p), _ptr(out), ctypes.byref(st)))
+        return out, st
+
+    def sssp(self, source, method=SSSP_BF, delta=64, unit_weights=False, want_dist=True):
+        V = self.info()[0]
+        p = SsspParams(method, delta, int(unit_weights), 0)
+        out = np.zeros(V, dtype=np.uint32) if want_dist else None
+        st = Stats()
+        check(lib().gmp_sssp(self._h, source, ctypes.byref(p), _ptr(out), ctypes.byref(st)))
+        return out, st
+
+    def cc(self, want_labels=True):
+        V = self.info()[0]
+        out = np.zeros(V, dtype=np.uint32) if want_labels else None
+        st = Stats()
+        check(lib().gmp_cc(self._h, _ptr(out), ctypes.byref(st)))
+        return out, st

@@ -1,0 +1,229 @@
+// C ABI of the engine: handle lifetime, graph loading, app entry points.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gmp_internal.hpp"
+
+using gmp::Engine;
+using gmp::fail;
+
+namespace {
+
+Engine *check(gmp_engine *h) {
+  if (!h) fail(GMP_EINVAL, "null engine handle");
+  return reinterpret_cast<Engine *>(h);
+}
+
+Engine *check_loaded(gmp_engine *h) {
+  Engine *e = check(h);
+  if (!e->loaded) fail(GMP_ESTATE, "no graph loaded");
+  GMP_HIP(hipSetDevice(e->device));
+  return e;
+}
+
+std::string join(const char *dir, const std::string &name) {
+  std::string d(dir);
+  if (!d.empty() && d.back() != '/') d.push_back('/');
+  return d + name;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gmp_create(int32_t device, gmp_engine **out) {
+  GMP_API_BEGIN
+  if (!out) fail(GMP_EINVAL, "gmp_create: out is null");
+  *out = nullptr;
+  int n = 0;
+  GMP_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) fail(GMP_EINVAL, "gmp_create: device ordinal out of range");
+  GMP_HIP(hipSetDevice(device));
+  auto *e = new Engine();
+  try {
+    e->device = device;
+    int cus = 0;
+    GMP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    e->num_cus = cus > 0 ? cus : 256;
+    GMP_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    GMP_HIP(hipHostMalloc(&e->h_pinned, 256, hipHostMallocDefault));
+  } catch (...) {
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->h_pinned) (void)hipHostFree(e->h_pinned);
+    delete e;
+    throw;
+  }
+  *out = reinterpret_cast<gmp_engine *>(e);
+  GMP_API_END
+}
+
+int gmp_destroy(gmp_engine *h) {
+  GMP_API_BEGIN
+  if (!h) return GMP_OK;
+  Engine *e = reinterpret_cast<Engine *>(h);
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->h_pinned) (void)hipHostFree(e->h_pinned);
+  hipStream_t s = e->stream;
+  delete e;
+  if (s) (void)hipStreamDestroy(s);
+  GMP_API_END
+}
+
+int gmp_load_csr(gmp_engine *h, uint32_t V, uint64_t E, const uint64_t *row_off,
+                 const uint32_t *col_idx, const uint32_t *weights, uint32_t P,
+                 const uint32_t *shard_bounds) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  GMP_HIP(hipSetDevice(e->device));
+  if (V == 0 || V == 0xFFFFFFFFu) fail(GMP_EINVAL, "gmp_load_csr: V must be in [1, 2^32-2]");
+  if (!row_off || (E && !col_idx)) fail(GMP_EINVAL, "gmp_load_csr: null pointer");
+  if (row_off[0] != 0 || row_off[V] != E) fail(GMP_EINVAL, "gmp_load_csr: row_off must run from 0 to E");
+  for (uint32_t r = 0; r < V; r++)
+    if (row_off[r + 1] < row_off[r]) fail(GMP_EINVAL, "gmp_load_csr: row_off must be nondecreasing");
+  for (uint64_t k = 0; k < E; k++)
+    if (col_idx[k] >= V) fail(GMP_EINVAL, "gmp_load_csr: column id " + std::to_string(col_idx[k]) + " >= V");
+  e->loaded = false;
+  gmp::graph_upload(*e, V, E, row_off, col_idx, weights, false);
+  gmp::graph_finish(*e, P, shard_bounds);
+  e->loaded = true;
+  GMP_API_END
+}
+
+int gmp_load_shards(gmp_engine *h, const char *dir, uint32_t V, uint32_t P) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  GMP_HIP(hipSetDevice(e->device));
+  if (!dir) fail(GMP_EINVAL, "gmp_load_shards: dir is null");
+  if (V == 0 || V == 0xFFFFFFFFu || P == 0) fail(GMP_EINVAL, "gmp_load_shards: V and P must be positive");
+  std::vector<uint64_t> off;
+  off.reserve((size_t)V + 1);
+  off.push_back(0);
+  std::vector<uint32_t> idx, wts, bounds(P + 1, 0), buf, wbuf;
+  bool weighted = false;
+  for (uint32_t p = 0; p < P; p++) {
+    const std::string path = join(dir, std::to_string(p) + ".edge.npy");
+    gmp::npy_read_u32(path, buf);
+    // header [edge_num_raw, len(indices), len(indptr), start_id, end_id] (Comp.h:37-42)
+    if (buf.size() < 5) fail(GMP_EFORMAT, path + ": shorter than the 5-word shard header");
+    const uint64_t nnz = buf[1], nptr = buf[2], start = buf[3], end = buf[4];
+    if ((int32_t)buf[1] < 0 || (int32_t)buf[2] < 1 || (int32_t)buf[3] < 0 || (int32_t)buf[4] < 0)
+      fail(GMP_EFORMAT, path + ": negative field in shard header");
+    if (buf.size() != 5 + nnz + nptr) fail(GMP_EFORMAT, path + ": length != 5 + len(indices) + len(indptr)");
+    if (end < start || nptr != end - start + 1) fail(GMP_EFORMAT, path + ": len(indptr) != end - start + 1");
+    if (start != off.size() - 1) fail(GMP_EFORMAT, path + ": shard does not start where the previous one ended");
+    if (end > V) fail(GMP_EFORMAT, path + ": end_id beyond the vertex count");
+    const uint32_t *ind = buf.data() + 5, *ptr = buf.data() + 5 + nnz;
+    if (ptr[0] != 0 || ptr[nptr - 1] != nnz) fail(GMP_EFORMAT, path + ": indptr must run from 0 to len(indices)");
+    for (uint64_t i = 0; i + 1 < nptr; i++)
+      if (ptr[i + 1] < ptr[i]) fail(GMP_EFORMAT, path + ": indptr must be nondecreasing");
+    for (uint64_t k = 0; k < nnz; k++)
+      if (ind[k] >= V) fail(GMP_EFORMAT, path + ": source id " + std::to_string(ind[k]) + " >= V");
+    const uint64_t base = idx.size();
+    idx.insert(idx.end(), ind, ind + nnz);
+    for (uint64_t i = 1; i < nptr; i++) off.push_back(base + ptr[i]);
+    bounds[p] = (uint32_t)start;
+    bounds[p + 1] = (uint32_t)end;
+    const std::string wpath = join(dir, std::to_string(p) + ".weight.npy");
+    if (FILE *f = std::fopen(wpath.c_str(), "rb")) {
+      std::fclose(f);
+      if (p > 0 && !weighted) fail(GMP_EFORMAT, wpath + ": weights present for some shards only");
+      weighted = true;
+      gmp::npy_read_u32(wpath, wbuf);
+      if (wbuf.size() != nnz) fail(GMP_EFORMAT, wpath + ": weight count != len(indices)");
+      wts.insert(wts.end(), wbuf.begin(), wbuf.end());
+    } else if (weighted) {
+      fail(GMP_EFORMAT, wpath + ": weights present for some shards only");
+    }
+  }
+  if (off.size() != (size_t)V + 1) fail(GMP_EFORMAT, "shards do not cover all V vertices");
+  e->loaded = false;
+  gmp::graph_upload(*e, V, idx.size(), off.data(), idx.data(), weighted ? wts.data() : nullptr, false);
+  gmp::graph_finish(*e, P, bounds.data());
+  e->loaded = true;
+  GMP_API_END
+}
+
+int gmp_generate_rmat(gmp_engine *h, const gmp_rmat_params *p) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  if (!p) fail(GMP_EINVAL, "gmp_generate_rmat: params are null");
+  GMP_HIP(hipSetDevice(e->device));
+  e->loaded = false;
+  gmp::graph_generate_rmat(*e, *p);
+  gmp::graph_finish(*e, p->num_shards, nullptr);
+  e->loaded = true;
+  GMP_API_END
+}
+
+int gmp_graph_info(gmp_engine *h, uint32_t *V, uint64_t *E, uint32_t *P, uint32_t *weighted) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  if (V) *V = e->g.V;
+  if (E) *E = e->g.E;
+  if (P) *P = e->g.P;
+  if (weighted) *weighted = e->g.weighted ? 1u : 0u;
+  GMP_API_END
+}
+
+int gmp_graph_download(gmp_engine *h, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights,
+                       uint32_t *shard_bounds, uint32_t *out_degree) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  gmp::graph_download(*e, row_off, col_idx, weights, out_degree);
+  if (shard_bounds) std::memcpy(shard_bounds, e->g.bounds.data(), e->g.bounds.size() * 4);
+  GMP_API_END
+}
+
+int gmp_pick_source(gmp_engine *h, uint64_t seed, uint32_t salt, uint32_t *vertex) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  if (!vertex) fail(GMP_EINVAL, "gmp_pick_source: vertex is null");
+  const uint32_t V = e->g.V;
+  std::vector<uint32_t> od(V);
+  GMP_HIP(hipMemcpy(od.data(), e->g.out_deg.p, (size_t)V * 4, hipMemcpyDeviceToHost));
+  for (uint64_t t = 0; t < (1u << 20); t++) {
+    const uint32_t v = (uint32_t)(gmp::mix64(seed + salt + (t + 1) * gmp::kGamma) % V);
+    if (od[v] > 0) {
+      *vertex = v;
+      return GMP_OK;
+    }
+  }
+  fail(GMP_EUNSUPPORTED, "gmp_pick_source: no vertex with out-degree > 0");
+  GMP_API_END
+}
+
+int gmp_run_app(gmp_engine *h, int32_t app, int32_t max_iter, const void *msg_init,
+                const int32_t *vertex_out, void *msg_out, void *value_out, gmp_stats *st) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  gmp::run_compat(*e, app, max_iter, msg_init, vertex_out, msg_out, value_out, st);
+  GMP_API_END
+}
+
+int gmp_pagerank(gmp_engine *h, const gmp_pr_params *p, float *rank_out, gmp_stats *st) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  if (!p) fail(GMP_EINVAL, "gmp_pagerank: params are null");
+  gmp::run_pagerank(*e, *p, rank_out, st);
+  GMP_API_END
+}
+
+int gmp_sssp(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, uint32_t *dist_out,
+             gmp_stats *st) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  gmp_sssp_params d{GMP_SSSP_BF, 0, 0, 0};
+  gmp::run_sssp(*e, source, p ? *p : d, dist_out, st);
+  GMP_API_END
+}
+
+int gmp_cc(gmp_engine *h, uint32_t *label_out, gmp_stats *st) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  gmp::run_cc(*e, label_out, st);
+  GMP_API_END
+}
+
+}  // extern "C"

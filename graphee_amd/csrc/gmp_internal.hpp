@@ -1,0 +1,191 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gmp.h"
+
+namespace gmp {
+
+// Error carried from anywhere inside the engine up to the C ABI, where it becomes a
+// status code plus the thread's last-error text.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+#define GMP_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      ::gmp::fail(e_ == hipErrorOutOfMemory ? GMP_ENOMEM : GMP_EHIP,                    \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                   \
+  } while (0)
+
+#define GMP_CHECK_LAUNCH() GMP_HIP(hipGetLastError())
+
+// Owning device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t n) {
+    if (n <= bytes && p) return;
+    release();
+    if (n == 0) return;
+    GMP_HIP(hipMalloc(&p, n));
+    bytes = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// PageRank gather plan: wave-sized tasks over the in-CSR.
+//   STREAM task  {row0, nrows}: rows [row0,row0+nrows) whose edges (<= stream_cap in total)
+//                are staged through LDS and reduced one row per lane.
+//   CHUNK task   {hub, k}:      edges [k*chunk, (k+1)*chunk) of hub row hubs[hub], reduced by
+//                the whole wave; rows with one chunk are applied in place, longer rows
+//                leave per-chunk partials for the hub-combine kernel.
+constexpr uint32_t kTaskChunkBit = 0x80000000u;
+struct HubRow {
+  uint32_t row;
+  uint32_t nchunks;
+  uint32_t part_base;
+  uint32_t pad;
+};
+struct GatherPlan {
+  uint32_t stream_cap = 1024;  // edges per STREAM task (LDS floats per wave)
+  uint32_t row_cap = 512;      // rows per STREAM task
+  uint32_t chunk = 2048;       // edges per CHUNK task
+  uint32_t ntasks = 0, nhubs = 0, nmulti = 0, nparts = 0;
+  DevBuf tasks;   // uint2[ntasks]
+  DevBuf hubs;    // HubRow[nhubs]
+  DevBuf multi;   // uint32[nmulti]: hub ids with nchunks > 1
+  bool built = false;
+};
+
+struct Graph {
+  uint32_t V = 0;
+  uint64_t E = 0;
+  bool off64 = false;
+  bool weighted = false;
+  DevBuf in_off, in_idx, in_w;     // destination rows
+  DevBuf out_off, out_idx, out_w;  // source rows (built on demand)
+  bool has_out = false;
+  DevBuf out_deg;                  // uint32[V]
+  uint32_t P = 1;
+  std::vector<uint32_t> bounds;    // P+1
+  std::vector<uint64_t> h_in_off;  // host copy of the in-CSR offsets (planning)
+  GatherPlan plan;
+  bool symmetric = false;
+};
+
+struct Engine {
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  Graph g;
+  bool loaded = false;
+  // scratch reused across runs
+  DevBuf s_a, s_b, s_c, s_d, s_e, s_f, s_part, s_scal;
+  void *h_pinned = nullptr;  // small pinned host buffer for per-iteration readbacks
+};
+
+void set_last_error(const std::string &m);
+const char *last_error();
+
+// Wraps a C ABI body: internal errors become status codes plus the last-error text.
+#define GMP_API_BEGIN try {
+#define GMP_API_END                                                                    \
+  return GMP_OK;                                                                       \
+  }                                                                                    \
+  catch (const ::gmp::Error &e) {                                                      \
+    ::gmp::set_last_error(e.what());                                                   \
+    return e.code;                                                                     \
+  }                                                                                    \
+  catch (const std::bad_alloc &) {                                                     \
+    ::gmp::set_last_error("host allocation failed");                                   \
+    return GMP_ENOMEM;                                                                 \
+  }                                                                                    \
+  catch (const std::exception &e) {                                                    \
+    ::gmp::set_last_error(e.what());                                                   \
+    return GMP_EINVAL;                                                                 \
+  }
+
+// ---- format.cpp (host) ----
+void npy_read_u32(const std::string &path, std::vector<uint32_t> &out);
+void npy_write_i32(const std::string &path, const int32_t *data, uint64_t n);
+void npy_write_u32(const std::string &path, const uint32_t *data, uint64_t n);
+void partition_bounds(const uint64_t *indeg, uint32_t V, uint32_t P, uint32_t *bounds);
+
+// ---- graph_build.hip ----
+void graph_upload(Engine &e, uint32_t V, uint64_t E, const uint64_t *row_off,
+                  const uint32_t *col_idx, const uint32_t *weights, bool force_off64);
+void graph_generate_rmat(Engine &e, const gmp_rmat_params &p);
+void graph_finish(Engine &e, uint32_t P, const uint32_t *bounds);  // out-degree, shards
+void graph_build_out(Engine &e);                                   // transpose
+void graph_download(Engine &e, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights,
+                    uint32_t *out_degree);
+
+// ---- plan.cpp (host) ----
+void build_gather_plan_host(const uint64_t *off, uint32_t V, GatherPlan &plan,
+                            std::vector<uint32_t> &tasks, std::vector<HubRow> &hubs,
+                            std::vector<uint32_t> &multi);
+void ensure_plan(Engine &e);
+
+// ---- pagerank.hip ----
+void run_pagerank(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st);
+void run_compat(Engine &e, int app, int max_iter, const void *msg_init, const int32_t *vout,
+                void *msg_out, void *value_out, gmp_stats *st);
+
+// ---- traverse.hip ----
+void run_sssp(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist_out,
+              gmp_stats *st);
+void run_cc(Engine &e, uint32_t *label_out, gmp_stats *st);
+
+// ---- shared constants of the synthetic-input definition ----
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Elapsed-time helper on the engine stream.
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() {
+    GMP_HIP(hipEventCreate(&a));
+    GMP_HIP(hipEventCreate(&b));
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  float ms() const {
+    float t = 0;
+    GMP_HIP(hipEventElapsedTime(&t, a, b));
+    return t;
+  }
+};
+
+}  // namespace gmp

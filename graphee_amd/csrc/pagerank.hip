@@ -1,0 +1,513 @@
+// PageRank on gfx950 and the GraphMP-compatible synchronous apps.
+//
+// Standard PageRank (gmp_pagerank): per iteration
+//   r'[v] = base + d * sum_{u->v} c[u]   (+ extra at the seed for personalised PageRank)
+//   c'[v] = r'[v] / outdeg[v]  (0 for dangling v)
+// with base = ((1-d) + d*D)/N, D = dangling mass of r.  Ranks and contributions are FP32;
+// the per-vertex gather is Kahan-compensated; the dangling mass and the L1 change are
+// FP64 reductions with a fixed order (per-workgroup partials, one finalising workgroup).
+// One iteration is three launches: the fused gather/apply over the wave task plan, the
+// combine of multi-chunk hub rows, and the FP64 finalise that prepares the next base.
+//
+// Compat apps (gmp_run_app) reproduce GraphMP's per-vertex rules bit for bit:
+//   pagerank.cpp:42-48  rel = sum in source order of Msg[u]/Out[u] in FP32, then
+//                       (float)(rel*0.85 + 1.0/N) in FP64 (no FMA contraction);
+//   sssp.cpp:144-150    min(Msg[v], Msg[u]+1) in int32;
+//   cc.cpp:238-244      min(Msg[v], Msg[u]) in int32;
+// under the synchronous loop of GraphPS.h:256-386 (changed count against VertexValue,
+// stop when nothing changed and step > 1).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "gmp_internal.hpp"
+#include "wave.hpp"
+
+namespace gmp {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kFinalBlock = 256;
+
+struct PrScal {
+  float base;    // added to every vertex
+  float extra;   // added to the seed vertex only
+  double resid;  // L1 change of the last iteration
+  double dang;   // dangling mass of the current ranks
+};
+
+template <class OffT>
+struct PrArgs {
+  const OffT *off;
+  const uint32_t *idx;
+  const uint32_t *outdeg;
+  const float *contrib_in;
+  const float *rank_in;
+  float *rank_out;
+  float *contrib_out;
+  const uint32_t *tasks;
+  uint32_t ntasks;
+  const HubRow *hubs;
+  const uint32_t *multi;
+  uint32_t nmulti;
+  float2 *hub_part;
+  const PrScal *sc;
+  float d;
+  uint32_t seed;
+  uint32_t stream_cap;
+  uint32_t chunk;
+  double *part_resid;
+  double *part_dang;
+};
+
+struct Acc {
+  double resid = 0, dang = 0;
+};
+
+template <class OffT>
+__device__ inline void pr_apply(const PrArgs<OffT> &a, uint32_t r, float s, float base, float extra,
+                                Acc &acc) {
+  float v = fmaf(a.d, s, base);
+  if (r == a.seed) v += extra;
+  a.rank_out[r] = v;
+  const uint32_t od = a.outdeg[r];
+  a.contrib_out[r] = od ? v / (float)od : 0.0f;
+  if (!od) acc.dang += (double)v;
+  acc.resid += fabs((double)v - (double)a.rank_in[r]);
+}
+
+// Deterministic block reduction of the two FP64 accumulators into partials[blockIdx.x].
+__device__ inline void block_store_acc(Acc acc, double *pr, double *pd) {
+  __shared__ double sr[kWavesPerBlock], sd[kWavesPerBlock];
+  double r = wave_reduce_sum(acc.resid), d = wave_reduce_sum(acc.dang);
+  const uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    sr[w] = r;
+    sd[w] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tr = 0, td = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) {
+      tr += sr[i];
+      td += sd[i];
+    }
+    pr[blockIdx.x] = tr;
+    pd[blockIdx.x] = td;
+  }
+}
+
+template <class OffT, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
+  extern __shared__ float lds[];
+  const uint32_t lane = lane_id(), wv = wave_id();
+  float *buf = lds + wv * a.stream_cap;
+  const float base = a.sc->base, extra = a.sc->extra;
+  Acc acc;
+  const uint32_t nw = gridDim.x * kWavesPerBlock;
+  for (uint32_t t = blockIdx.x * kWavesPerBlock + wv; t < a.ntasks; t += nw) {
+    const uint32_t x = a.tasks[2 * t], y = a.tasks[2 * t + 1];
+    if (!(y & kTaskChunkBit)) {
+      // STREAM: stage the gathered contributions of rows [x, x+y) in LDS, one row per lane.
+      const uint64_t e0 = a.off[x];
+      const uint32_t n = (uint32_t)((uint64_t)a.off[x + y] - e0);
+      for (uint32_t j = lane; j < n; j += kWave) buf[j] = a.contrib_in[a.idx[e0 + j]];
+      wave_sync();
+      for (uint32_t i = lane; i < y; i += kWave) {
+        const uint32_t r = x + i;
+        const uint32_t b = (uint32_t)((uint64_t)a.off[r] - e0), en = (uint32_t)((uint64_t)a.off[r + 1] - e0);
+        float s = 0, c = 0;
+        if (KAHAN) {
+          for (uint32_t k = b; k < en; k++) kahan_add(s, c, buf[k]);
+        } else {
+          for (uint32_t k = b; k < en; k++) s += buf[k];
+        }
+        pr_apply(a, r, s - c, base, extra, acc);
+      }
+      wave_sync();
+    } else {
+      // CHUNK of a hub row: lane-strided compensated sum, then a wave reduction.
+      const HubRow h = a.hubs[x];
+      const uint32_t k = y & ~kTaskChunkBit;
+      const uint64_t rb = a.off[h.row], re = a.off[h.row + 1];
+      const uint64_t e0 = rb + (uint64_t)k * a.chunk;
+      const uint64_t e1 = min(e0 + a.chunk, re);
+      float s = 0, c = 0;
+      if (KAHAN) {
+        for (uint64_t j = e0 + lane; j < e1; j += kWave) kahan_add(s, c, a.contrib_in[a.idx[j]]);
+      } else {
+        for (uint64_t j = e0 + lane; j < e1; j += kWave) s += a.contrib_in[a.idx[j]];
+      }
+      float hi = s, lo = -c;
+      wave_reduce_dd(hi, lo);
+      if (lane == 0) {
+        if (h.nchunks == 1) pr_apply(a, h.row, hi, base, extra, acc);
+        else a.hub_part[h.part_base + k] = make_float2(hi, lo);
+      }
+    }
+  }
+  block_store_acc(acc, a.part_resid, a.part_dang);
+}
+
+// One wave per multi-chunk hub: combine its chunk partials in chunk order, then apply.
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void pr_hub_combine_kernel(PrArgs<OffT> a) {
+  const uint32_t lane = lane_id(), wv = wave_id();
+  const float base = a.sc->base, extra = a.sc->extra;
+  Acc acc;
+  const uint32_t nw = gridDim.x * kWavesPerBlock;
+  for (uint32_t t = blockIdx.x * kWavesPerBlock + wv; t < a.nmulti; t += nw) {
+    const HubRow h = a.hubs[a.multi[t]];
+    float hi = 0, lo = 0;
+    for (uint32_t k = lane; k < h.nchunks; k += kWave) {
+      const float2 p = a.hub_part[h.part_base + k];
+      dd_add(hi, lo, p.x, p.y);
+    }
+    wave_reduce_dd(hi, lo);
+    if (lane == 0) pr_apply(a, h.row, hi, base, extra, acc);
+  }
+  block_store_acc(acc, a.part_resid, a.part_dang);
+}
+
+// r = teleport vector, c = r / outdeg, dangling partials.
+__global__ __launch_bounds__(kBlock) void pr_init_kernel(uint32_t V, uint32_t seed, const uint32_t *outdeg,
+                                                         float *rank, float *contrib, double *part_resid,
+                                                         double *part_dang) {
+  Acc acc;
+  const float r0 = seed == 0xFFFFFFFFu ? (float)(1.0 / (double)V) : 0.0f;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+    const float r = (seed != 0xFFFFFFFFu && v == seed) ? 1.0f : r0;
+    rank[v] = r;
+    const uint32_t od = outdeg[v];
+    contrib[v] = od ? r / (float)od : 0.0f;
+    if (!od) acc.dang += (double)r;
+  }
+  block_store_acc(acc, part_resid, part_dang);
+}
+
+// Sums partials in a fixed order (n1 from the gather, n2 from the hub combine), records
+// the iteration's residual and dangling mass, and prepares the next base.
+__global__ __launch_bounds__(kFinalBlock) void pr_finalize_kernel(const double *pr1, const double *pd1, uint32_t n1,
+                                                                  const double *pr2, const double *pd2, uint32_t n2,
+                                                                  uint32_t V, float d, int personalised,
+                                                                  PrScal *sc, double *hist, int32_t it) {
+  __shared__ double sr[kFinalBlock], sd[kFinalBlock];
+  double r = 0, dd = 0;
+  for (uint32_t i = threadIdx.x; i < n1; i += kFinalBlock) {
+    r += pr1[i];
+    dd += pd1[i];
+  }
+  for (uint32_t i = threadIdx.x; i < n2; i += kFinalBlock) {
+    r += pr2[i];
+    dd += pd2[i];
+  }
+  sr[threadIdx.x] = r;
+  sd[threadIdx.x] = dd;
+  __syncthreads();
+  for (int s = kFinalBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      sd[threadIdx.x] += sd[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double D = sd[0];
+    const double tele = (1.0 - (double)d) + (double)d * D;
+    sc->resid = sr[0];
+    sc->dang = D;
+    if (personalised) {
+      sc->base = 0.0f;
+      sc->extra = (float)tele;
+    } else {
+      sc->base = (float)(tele / (double)V);
+      sc->extra = 0.0f;
+    }
+    if (hist && it >= 0) {
+      hist[2 * it] = sr[0];
+      hist[2 * it + 1] = D;
+    }
+  }
+}
+
+// ---- GraphMP-compatible synchronous step (one thread per destination) ----
+template <class OffT, int APP>
+__global__ __launch_bounds__(kBlock) void compat_step_kernel(const OffT *off, const uint32_t *idx,
+                                                             const uint32_t *w, const int32_t *vout, uint32_t V,
+                                                             double invN, const void *msg_v, void *msgnew_v,
+                                                             void *value_v, unsigned long long *changed) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+    const uint64_t k0 = off[v], k1 = off[v + 1];
+    bool ch;
+    if (APP == GMP_APP_PAGERANK) {
+      const float *msg = (const float *)msg_v;
+      float rel = 0.0f;
+      for (uint64_t k = k0; k < k1; k++) {
+        const uint32_t u = idx[k];
+        const int32_t o = vout[u];
+        rel += msg[u] / (float)(o == 0 ? 1 : o);
+      }
+      rel = (float)(__dadd_rn(__dmul_rn((double)rel, 0.85), invN));
+      ((float *)msgnew_v)[v] = rel;
+      float *val = (float *)value_v;
+      ch = rel != val[v];
+      if (ch) val[v] = rel;
+    } else if (APP == GMP_APP_SSSP || APP == GMP_APP_CC) {
+      const int32_t *msg = (const int32_t *)msg_v;
+      int32_t best = msg[v];
+      for (uint64_t k = k0; k < k1; k++) {
+        int32_t t = msg[idx[k]];
+        if (APP == GMP_APP_SSSP) t = (int32_t)((uint32_t)t + 1u);
+        if (best > t) best = t;
+      }
+      ((int32_t *)msgnew_v)[v] = best;
+      int32_t *val = (int32_t *)value_v;
+      ch = best != val[v];
+      if (ch) val[v] = best;
+    } else {
+      const uint32_t *msg = (const uint32_t *)msg_v;
+      uint32_t best = msg[v];
+      for (uint64_t k = k0; k < k1; k++) {
+        const uint64_t t = (uint64_t)msg[idx[k]] + (uint64_t)w[k];
+        if (t < (uint64_t)best) best = (uint32_t)t;
+      }
+      ((uint32_t *)msgnew_v)[v] = best;
+      uint32_t *val = (uint32_t *)value_v;
+      ch = best != val[v];
+      if (ch) val[v] = best;
+    }
+    const uint64_t m = __ballot(ch);
+    if (lane_id() == 0 && m) atomicAdd(changed, (unsigned long long)__popcll(m));
+  }
+}
+
+template <class OffT>
+void launch_compat(Engine &e, int app, uint32_t grid, double invN, const void *msg, void *msgnew,
+                   void *value, const int32_t *vout, unsigned long long *changed) {
+  Graph &g = e.g;
+  const OffT *off = g.in_off.as<OffT>();
+  const uint32_t *idx = g.in_idx.as<uint32_t>();
+  const uint32_t *w = g.in_w.as<uint32_t>();
+  switch (app) {
+    case GMP_APP_PAGERANK:
+      compat_step_kernel<OffT, GMP_APP_PAGERANK><<<grid, kBlock, 0, e.stream>>>(off, idx, w, vout, g.V, invN, msg,
+                                                                                msgnew, value, changed);
+      break;
+    case GMP_APP_SSSP:
+      compat_step_kernel<OffT, GMP_APP_SSSP><<<grid, kBlock, 0, e.stream>>>(off, idx, w, vout, g.V, invN, msg, msgnew,
+                                                                            value, changed);
+      break;
+    case GMP_APP_CC:
+      compat_step_kernel<OffT, GMP_APP_CC><<<grid, kBlock, 0, e.stream>>>(off, idx, w, vout, g.V, invN, msg, msgnew,
+                                                                          value, changed);
+      break;
+    default:
+      compat_step_kernel<OffT, GMP_APP_SSSP_W><<<grid, kBlock, 0, e.stream>>>(off, idx, w, vout, g.V, invN, msg,
+                                                                              msgnew, value, changed);
+      break;
+  }
+  GMP_CHECK_LAUNCH();
+}
+
+template <class OffT>
+void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st) {
+  Graph &g = e.g;
+  ensure_plan(e);
+  GatherPlan &pl = g.plan;
+  const uint32_t V = g.V;
+  const bool personalised = p.seed_vertex >= 0;
+  const uint32_t seed = personalised ? (uint32_t)p.seed_vertex : 0xFFFFFFFFu;
+  const uint32_t G = std::max(1, e.num_cus * 8);
+  const uint32_t Gh = pl.nmulti ? std::min<uint32_t>((pl.nmulti + kWavesPerBlock - 1) / kWavesPerBlock, G) : 0;
+  const int max_iter = std::max(0, p.max_iter);
+
+  // buffers: 2 x rank, 2 x contrib, hub partials, FP64 partials, scalars, history
+  e.s_a.alloc((size_t)V * 4);
+  e.s_b.alloc((size_t)V * 4);
+  e.s_c.alloc((size_t)V * 4);
+  e.s_d.alloc((size_t)V * 4);
+  e.s_e.alloc(std::max<size_t>(pl.nparts, 1) * sizeof(float2));
+  e.s_part.alloc((size_t)(G + std::max<uint32_t>(Gh, 1)) * 2 * sizeof(double));
+  e.s_scal.alloc(sizeof(PrScal) + (size_t)(max_iter + 1) * 2 * sizeof(double));
+  float *rank[2] = {e.s_a.as<float>(), e.s_b.as<float>()};
+  float *contrib[2] = {e.s_c.as<float>(), e.s_d.as<float>()};
+  double *pr1 = e.s_part.as<double>(), *pd1 = pr1 + G;
+  double *pr2 = pd1 + G, *pd2 = pr2 + std::max<uint32_t>(Gh, 1);
+  PrScal *sc = e.s_scal.as<PrScal>();
+  double *hist = reinterpret_cast<double *>(e.s_scal.as<char>() + sizeof(PrScal));
+
+  std::vector<hipEvent_t> ev(2 * (size_t)max_iter + 2);
+  for (auto &x : ev) GMP_HIP(hipEventCreate(&x));
+  struct EvGuard {
+    std::vector<hipEvent_t> &v;
+    ~EvGuard() {
+      for (auto x : v) (void)hipEventDestroy(x);
+    }
+  } evg{ev};
+
+  PrArgs<OffT> a{};
+  a.off = g.in_off.as<OffT>();
+  a.idx = g.in_idx.as<uint32_t>();
+  a.outdeg = g.out_deg.as<uint32_t>();
+  a.tasks = pl.tasks.as<uint32_t>();
+  a.ntasks = pl.ntasks;
+  a.hubs = pl.hubs.as<HubRow>();
+  a.multi = pl.multi.as<uint32_t>();
+  a.nmulti = pl.nmulti;
+  a.hub_part = e.s_e.as<float2>();
+  a.sc = sc;
+  a.d = p.damping;
+  a.seed = seed;
+  a.stream_cap = pl.stream_cap;
+  a.chunk = pl.chunk;
+
+  GMP_HIP(hipEventRecord(ev[0], e.stream));
+  pr_init_kernel<<<G, kBlock, 0, e.stream>>>(V, seed, a.outdeg, rank[0], contrib[0], pr1, pd1);
+  GMP_CHECK_LAUNCH();
+  pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, G, pr2, pd2, 0, V, p.damping, personalised, sc,
+                                                      nullptr, -1);
+  GMP_CHECK_LAUNCH();
+  const size_t lds = (size_t)kWavesPerBlock * pl.stream_cap * sizeof(float);
+  int cur = 0, it = 0;
+  double resid = 0, dang = 0;
+  const bool kahan = !(p.flags & GMP_PR_NO_KAHAN);
+  for (it = 0; it < max_iter; it++) {
+    a.contrib_in = contrib[cur];
+    a.rank_in = rank[cur];
+    a.rank_out = rank[cur ^ 1];
+    a.contrib_out = contrib[cur ^ 1];
+    a.part_resid = pr1;
+    a.part_dang = pd1;
+    GMP_HIP(hipEventRecord(ev[2 + 2 * it], e.stream));
+    if (kahan) pr_gather_kernel<OffT, true><<<G, kBlock, lds, e.stream>>>(a);
+    else pr_gather_kernel<OffT, false><<<G, kBlock, lds, e.stream>>>(a);
+    GMP_CHECK_LAUNCH();
+    GMP_HIP(hipEventRecord(ev[3 + 2 * it], e.stream));
+    if (Gh) {
+      a.part_resid = pr2;
+      a.part_dang = pd2;
+      pr_hub_combine_kernel<OffT><<<Gh, kBlock, 0, e.stream>>>(a);
+      GMP_CHECK_LAUNCH();
+    }
+    pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, G, pr2, pd2, Gh, V, p.damping, personalised, sc,
+                                                        hist, it);
+    GMP_CHECK_LAUNCH();
+    cur ^= 1;
+    if (p.tol > 0) {
+      double h[2];
+      GMP_HIP(hipMemcpyAsync(h, hist + 2 * it, sizeof h, hipMemcpyDeviceToHost, e.stream));
+      GMP_HIP(hipStreamSynchronize(e.stream));
+      resid = h[0];
+      if (resid < p.tol) {
+        it++;
+        break;
+      }
+    }
+  }
+  GMP_HIP(hipEventRecord(ev[1], e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  double kms = 0;
+  for (int i = 0; i < it; i++) {
+    float t = 0;
+    GMP_HIP(hipEventElapsedTime(&t, ev[2 + 2 * i], ev[3 + 2 * i]));
+    kms += t;
+  }
+  float tot = 0;
+  GMP_HIP(hipEventElapsedTime(&tot, ev[0], ev[1]));
+  if (it > 0) {
+    double h[2];
+    GMP_HIP(hipMemcpy(h, hist + 2 * (it - 1), sizeof h, hipMemcpyDeviceToHost));
+    resid = h[0];
+    dang = h[1];
+  }
+  if (rank_out) GMP_HIP(hipMemcpy(rank_out, rank[cur], (size_t)V * 4, hipMemcpyDeviceToHost));
+  if (st) {
+    st->iterations = it;
+    st->launches = it;
+    st->ms_total = tot;
+    st->ms_kernel = kms;
+    st->edges = g.E * (uint64_t)it;
+    st->vertices = (uint64_t)V * it;
+    st->residual = resid;
+    st->dangling = dang;
+    st->changed = 0;
+    const uint64_t o = g.off64 ? 8 : 4;
+    st->bytes = (uint64_t)it * (8 * g.E + (o + 16) * (uint64_t)V);
+  }
+}
+
+template <class OffT>
+void compat_impl(Engine &e, int app, int max_iter, const void *msg_init, const int32_t *vout,
+                 void *msg_out, void *value_out, gmp_stats *st) {
+  Graph &g = e.g;
+  const uint32_t V = g.V;
+  if (app == GMP_APP_SSSP_W && !g.weighted) fail(GMP_EUNSUPPORTED, "weighted SSSP needs a weighted graph");
+  e.s_a.alloc((size_t)V * 4);
+  e.s_b.alloc((size_t)V * 4);
+  e.s_c.alloc((size_t)V * 4);
+  e.s_d.alloc(std::max<size_t>((size_t)V * 4, 8));
+  e.s_scal.alloc(8);
+  void *msg[2] = {e.s_a.p, e.s_b.p};
+  void *value = e.s_c.p;
+  int32_t *dvout = nullptr;
+  if (app == GMP_APP_PAGERANK) {
+    if (!vout) fail(GMP_EINVAL, "pagerank needs vertex_out");
+    dvout = e.s_d.as<int32_t>();
+    GMP_HIP(hipMemcpyAsync(dvout, vout, (size_t)V * 4, hipMemcpyHostToDevice, e.stream));
+  }
+  GMP_HIP(hipMemcpyAsync(msg[0], msg_init, (size_t)V * 4, hipMemcpyHostToDevice, e.stream));
+  GMP_HIP(hipMemsetAsync(value, 0, (size_t)V * 4, e.stream));
+  unsigned long long *changed = e.s_scal.as<unsigned long long>();
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((V + kBlock - 1) / kBlock, e.num_cus * 16));
+  const double invN = 1.0 / (double)V;
+  EventPair tot;
+  GMP_HIP(hipEventRecord(tot.a, e.stream));
+  int cur = 0, step = 0;
+  unsigned long long nch = 0;
+  for (step = 0; step < max_iter; step++) {
+    GMP_HIP(hipMemsetAsync(changed, 0, 8, e.stream));
+    launch_compat<OffT>(e, app, grid, invN, msg[cur], msg[cur ^ 1], value, dvout, changed);
+    GMP_HIP(hipMemcpyAsync(&nch, changed, 8, hipMemcpyDeviceToHost, e.stream));
+    GMP_HIP(hipStreamSynchronize(e.stream));
+    cur ^= 1;
+    if (nch == 0 && step > 1) {
+      step++;
+      break;
+    }
+  }
+  GMP_HIP(hipEventRecord(tot.b, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  if (msg_out) GMP_HIP(hipMemcpy(msg_out, msg[cur], (size_t)V * 4, hipMemcpyDeviceToHost));
+  if (value_out) GMP_HIP(hipMemcpy(value_out, value, (size_t)V * 4, hipMemcpyDeviceToHost));
+  if (st) {
+    std::memset(st, 0, sizeof *st);
+    st->iterations = step;
+    st->launches = step;
+    st->ms_total = tot.ms();
+    st->edges = g.E * (uint64_t)step;
+    st->vertices = (uint64_t)V * step;
+    st->changed = nch;
+  }
+}
+
+}  // namespace
+
+void run_pagerank(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st) {
+  if (!(p.damping >= 0.0f && p.damping <= 1.0f)) fail(GMP_EINVAL, "damping must be in [0, 1]");
+  if (p.seed_vertex >= (int64_t)e.g.V) fail(GMP_EINVAL, "seed vertex out of range");
+  if (p.flags & GMP_PR_COMPRESSED) fail(GMP_EUNSUPPORTED, "compressed gather is not built in this library");
+  if (e.g.off64) pagerank_impl<uint64_t>(e, p, rank_out, st);
+  else pagerank_impl<uint32_t>(e, p, rank_out, st);
+}
+
+void run_compat(Engine &e, int app, int max_iter, const void *msg_init, const int32_t *vout,
+                void *msg_out, void *value_out, gmp_stats *st) {
+  if (app < GMP_APP_PAGERANK || app > GMP_APP_SSSP_W) fail(GMP_EINVAL, "unknown app id");
+  if (!msg_init) fail(GMP_EINVAL, "msg_init is required");
+  if (e.g.off64) compat_impl<uint64_t>(e, app, max_iter, msg_init, vout, msg_out, value_out, st);
+  else compat_impl<uint32_t>(e, app, max_iter, msg_init, vout, msg_out, value_out, st);
+}
+
+}  // namespace gmp

@@ -1,0 +1,161 @@
+/*
+ * gmp.h -- C ABI of the gfx950 vertex-centric graph engine (libgmp.so).
+ *
+ * The engine replaces GraphMP's shared-memory engine (cap-ntu/Graphee, GraphMP/system/):
+ * a destination-interval CSR held in HBM replaces the per-iteration shard reads, and
+ * hand-written HIP kernels replace the OpenMP per-shard gather.  Every entry point takes
+ * plain host pointers and sizes; device memory never crosses this boundary.  All entry
+ * points return GMP_OK (0) or a GMP_E* code and never throw; gmp_last_error() gives the
+ * calling thread's last message.  An engine handle is not thread-safe: callers serialise
+ * calls on one handle (the engine runs everything on one HIP stream per handle).
+ */
+#ifndef GMP_H_
+#define GMP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMP_ABI_VERSION 1
+
+/* status codes */
+#define GMP_OK 0
+#define GMP_EINVAL 1       /* bad argument (null pointer, size out of range) */
+#define GMP_EIO 2          /* file missing / short read / write failure */
+#define GMP_EFORMAT 3      /* malformed .npy or shard header */
+#define GMP_EHIP 4         /* HIP runtime error */
+#define GMP_ENOMEM 5       /* device or host allocation failed */
+#define GMP_ESTATE 6       /* call out of order (e.g. run before a graph is loaded) */
+#define GMP_EUNSUPPORTED 7 /* feature not available for this graph */
+
+/* GraphMP-compatible apps for gmp_run_app: the per-vertex rules of
+ * GraphMP/app/pagerank.cpp:42-59 (float), sssp.cpp:144-162 (int32, +1 per edge) and
+ * cc.cpp:238-255 (int32 min label), plus the weighted SSSP generalisation (uint32,
+ * saturating add of the per-edge weight). */
+#define GMP_APP_PAGERANK 1
+#define GMP_APP_SSSP 2
+#define GMP_APP_CC 3
+#define GMP_APP_SSSP_W 4
+
+/* gmp_sssp methods */
+#define GMP_SSSP_BF 0      /* frontier Bellman-Ford (selective scheduling) */
+#define GMP_SSSP_DELTA 1   /* fixed-delta stepping */
+#define GMP_SSSP_LEARNED 2 /* delta chosen per bucket epoch by the trained MLP */
+
+/* gmp_pr_params.flags */
+#define GMP_PR_NO_KAHAN 1u   /* plain FP32 gather sum instead of the compensated one */
+#define GMP_PR_COMPRESSED 2u /* gather from the delta/varint neighbour lists */
+
+typedef struct gmp_engine gmp_engine;
+
+typedef struct gmp_stats {
+  int32_t iterations;  /* iterations executed */
+  int32_t launches;    /* launches of the dominant kernel */
+  double ms_total;     /* device time of the run, HIP events on the engine stream */
+  double ms_kernel;    /* device time of the dominant kernel, summed over its launches */
+  uint64_t edges;      /* edges gathered or relaxed */
+  uint64_t vertices;   /* vertex updates applied */
+  double residual;     /* PageRank: L1 change of the last iteration (FP64 reduction) */
+  double dangling;     /* PageRank: dangling mass entering the last iteration (FP64) */
+  uint64_t changed;    /* vertices changed in the last iteration */
+  uint64_t bytes;      /* algorithmic bytes of the run (uncompressed yardstick) */
+} gmp_stats;
+
+typedef struct gmp_rmat_params {
+  uint32_t scale;       /* V = 2^scale */
+  uint32_t edge_factor; /* samples = edge_factor * V */
+  uint64_t seed;
+  double a, b, c;       /* quadrant probabilities, d = 1 - a - b - c */
+  uint32_t undirected;  /* 1: add the reverse of every edge (symmetrised graph) */
+  uint32_t weighted;    /* 1: attach uint32 weights in [1,255] */
+  uint32_t num_shards;  /* destination-interval shards, equal in-edges per shard */
+  uint32_t flags;       /* GMP_GRAPH_OFF64: keep 64-bit row offsets even when E < 2^32 */
+} gmp_rmat_params;
+
+#define GMP_GRAPH_OFF64 1u
+
+typedef struct gmp_pr_params {
+  float damping;       /* 0.85 */
+  int32_t max_iter;    /* iteration cap */
+  double tol;          /* > 0: stop once the L1 change of an iteration is below tol */
+  int64_t seed_vertex; /* -1: uniform teleport; else personalised to this vertex */
+  uint32_t flags;      /* GMP_PR_* */
+  uint32_t reserved;
+} gmp_pr_params;
+
+typedef struct gmp_sssp_params {
+  int32_t method;      /* GMP_SSSP_* */
+  uint32_t delta;      /* bucket width for GMP_SSSP_DELTA (0 -> 64) */
+  uint32_t unit_weights; /* 1: every edge weighs 1 (ignores stored weights) */
+  uint32_t reserved;
+} gmp_sssp_params;
+
+/* ---- engine lifetime ---------------------------------------------------------------
+ * Replace init_workers()/GraphPS<T>::init (GraphMP/system/Global.h:336-362,
+ * GraphMP/system/GraphPS.h:77-151): one engine per GPU, no MPI. */
+const char *gmp_last_error(void);
+int gmp_abi_version(void);
+int gmp_create(int32_t device, gmp_engine **out);
+int gmp_destroy(gmp_engine *h);
+
+/* ---- graph loading -----------------------------------------------------------------
+ * In-CSR = destination rows with ascending, deduplicated source ids, the layout of
+ * GraphMP's shards (GraphMP/tools/format_spark.py:137-157, GraphMP/system/Comp.h:21-50).
+ * row_off has V+1 entries (global offsets); shard_bounds has P+1 entries (NULL: the
+ * equal-in-edge partitioner of format_spark.py:85-108 with P shards, P=0 -> 1).
+ * weights may be NULL. */
+int gmp_load_csr(gmp_engine *h, uint32_t V, uint64_t E, const uint64_t *row_off,
+                 const uint32_t *col_idx, const uint32_t *weights, uint32_t P,
+                 const uint32_t *shard_bounds);
+/* Reads dir/{p}.edge.npy for p in [0,P) (cnpy rules, GraphMP/system/cnpy.h:73-173, shard
+ * header [edge_num_raw, len(indices), len(indptr), start, end], Comp.h:37-42), optional
+ * dir/{p}.weight.npy (uint32 aligned with indices).  Replaces load_edge
+ * (GraphMP/system/Global.h:129-255): the shards are read once and stay in HBM. */
+int gmp_load_shards(gmp_engine *h, const char *dir, uint32_t V, uint32_t P);
+/* Builds the R-MAT graph on the device (definition in oracle/rmat_ref.py's docstring). */
+int gmp_generate_rmat(gmp_engine *h, const gmp_rmat_params *p);
+int gmp_graph_info(gmp_engine *h, uint32_t *V, uint64_t *E, uint32_t *P, uint32_t *weighted);
+/* Copies the device graph back; any output may be NULL.  row_off: V+1, col_idx/weights: E,
+ * shard_bounds: P+1, out_degree: V. */
+int gmp_graph_download(gmp_engine *h, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights,
+                       uint32_t *shard_bounds, uint32_t *out_degree);
+/* Seeded source pick (definition in oracle/rmat_ref.py): first vertex with out-degree > 0. */
+int gmp_pick_source(gmp_engine *h, uint64_t seed, uint32_t salt, uint32_t *vertex);
+
+/* ---- GraphMP-compatible run --------------------------------------------------------
+ * GraphPS<T>::run (GraphMP/system/GraphPS.h:171-387) with the app's _comp
+ * (GraphPS.h:25-34): msg_init is init_vertex()'s _VertexMsg (4-byte elements), the run is
+ * synchronous (MsgNew written from Msg, swapped each step) and stops when no vertex
+ * changed and step > 1, or after max_iter steps.  vertex_out is _VertexOut (pagerank
+ * only, zeros read as 1).  msg_out / value_out receive _VertexMsg / _VertexValue. */
+int gmp_run_app(gmp_engine *h, int32_t app, int32_t max_iter, const void *msg_init,
+                const int32_t *vertex_out, void *msg_out, void *value_out, gmp_stats *st);
+
+/* ---- engine apps -------------------------------------------------------------------- */
+int gmp_pagerank(gmp_engine *h, const gmp_pr_params *p, float *rank_out, gmp_stats *st);
+int gmp_sssp(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, uint32_t *dist_out,
+             gmp_stats *st);
+/* Connected components of a symmetric graph: label = minimum vertex id in the component
+ * (the fixed point of GraphMP/app/cc.cpp's min-label rule). */
+int gmp_cc(gmp_engine *h, uint32_t *label_out, gmp_stats *st);
+
+/* ---- host-side format utilities (no GPU needed) -----------------------------------
+ * Equal-in-edge partitioner, format_spark.py:85-108. */
+int gmp_partition_bounds(const uint64_t *indeg, uint32_t V, uint32_t P, uint32_t *bounds);
+/* 1-D little-endian .npy reader under cnpy's rules (cnpy.h:73-126); *n_out = element
+ * count, word size must be 4.  buf may be NULL to query the size. */
+int gmp_npy_read_u32(const char *path, uint32_t *buf, uint64_t cap, uint64_t *n_out);
+int gmp_npy_write_i32(const char *path, const int32_t *data, uint64_t n);
+/* Writes dir/{p}.edge.npy for every shard (empty shards included), dir/{p}.weight.npy when
+ * weights != NULL, and dir/vertexout.npy when vertex_out != NULL. */
+int gmp_write_shards(const char *dir, uint32_t V, const uint64_t *row_off,
+                     const uint32_t *col_idx, const uint32_t *weights, uint32_t P,
+                     const uint32_t *bounds, const int32_t *vertex_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GMP_H_ */

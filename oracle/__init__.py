@@ -1,0 +1,158 @@
+"""CPU oracle for the gfx950 graph engine.  TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as
+the checker (never as the thing measured or shipped).  gmp_oracle.c restates GraphMP's
+per-app update rules and engine loop; rmat_ref.py restates the synthetic-input
+definition.  Build with `make -C oracle` (or __graft_entry__.build()).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from . import rmat_ref  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libgmp_oracle.so")
+_lib = None
+
+APP_PAGERANK = 1
+APP_SSSP = 2
+APP_CC = 3
+APP_SSSP_W = 4
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+
+
+class BaselineStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("seconds", ctypes.c_double), ("edges", ctypes.c_double)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = ctypes.CDLL(_SO)
+        L.orc_partition_bounds.argtypes = [_P, _I64, _I32, _P]
+        L.orc_app_sync.argtypes = [_I32, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P]
+        L.orc_pr_f64.argtypes = [_I64, _P, _P, _P, ctypes.c_double, _I32, ctypes.c_double,
+                                 _I64, _P, _P, _P, _P]
+        L.orc_sssp_dijkstra.argtypes = [_I64, _P, _P, _P, ctypes.c_uint32, _P]
+        L.orc_cc_unionfind.argtypes = [_I64, _P, _P, _P]
+        L.orc_graphmp_baseline.argtypes = [_I32, _I64, _I32, _P, _P, _P, _P, _P, ctypes.c_float,
+                                           _I32, _I32, _P, _P]
+        L.orc_pr_f64_step_residual.argtypes = [_I64, _P, _P, _P, ctypes.c_double, _P, _I32, _P, _P]
+        for f in ("orc_partition_bounds", "orc_app_sync", "orc_pr_f64", "orc_sssp_dijkstra",
+                  "orc_cc_unionfind", "orc_graphmp_baseline", "orc_pr_f64_step_residual"):
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with code {rc}")
+
+
+def partition_bounds(indeg, P):
+    indeg = np.ascontiguousarray(indeg, dtype=np.int64)
+    b = np.zeros(P + 1, dtype=np.int64)
+    _check(lib().orc_partition_bounds(_ptr(indeg), indeg.size, P, _ptr(b)), "partition")
+    return b
+
+
+def app_sync(app, off, idx, msg_init, max_iter, w=None, vout=None):
+    """GraphMP synchronous run(); returns (msg, value, iterations, changed_per_iter)."""
+    V = off.size - 1
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    dt = np.float32 if app == APP_PAGERANK else (np.uint32 if app == APP_SSSP_W else np.int32)
+    msg = np.array(msg_init, dtype=dt, copy=True)
+    value = np.zeros(V, dtype=dt)
+    it = ctypes.c_int32(0)
+    changed = np.zeros(max(max_iter, 1), dtype=np.int64)
+    wv = None if w is None else np.ascontiguousarray(w, dtype=np.uint32)
+    vo = None if vout is None else np.ascontiguousarray(vout, dtype=np.int32)
+    _check(lib().orc_app_sync(app, V, _ptr(off), _ptr(idx), _ptr(wv), _ptr(vo), max_iter,
+                              _ptr(msg), _ptr(value), ctypes.byref(it), _ptr(changed)), "app_sync")
+    return msg, value, it.value, changed[: it.value]
+
+
+def pr_f64(off, idx, outdeg, d=0.85, max_iter=100, tol=0.0, seed=-1, x0=None):
+    V = off.size - 1
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    od = np.ascontiguousarray(outdeg, dtype=np.uint32)
+    x = np.zeros(V, dtype=np.float64)
+    x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+    it = ctypes.c_int32(0)
+    r = ctypes.c_double(0)
+    _check(lib().orc_pr_f64(V, _ptr(off), _ptr(idx), _ptr(od), d, max_iter, tol, seed,
+                            _ptr(x0a), _ptr(x), ctypes.byref(it), ctypes.byref(r)), "pr_f64")
+    return x, it.value, r.value
+
+
+def dijkstra(ooff, oidx, ow, src):
+    V = ooff.size - 1
+    ooff = np.ascontiguousarray(ooff, dtype=np.int64)
+    oidx = np.ascontiguousarray(oidx, dtype=np.uint32)
+    owv = None if ow is None else np.ascontiguousarray(ow, dtype=np.uint32)
+    dist = np.zeros(V, dtype=np.uint32)
+    _check(lib().orc_sssp_dijkstra(V, _ptr(ooff), _ptr(oidx), _ptr(owv), src, _ptr(dist)),
+           "dijkstra")
+    return dist
+
+
+def cc_unionfind(off, idx):
+    V = off.size - 1
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    lab = np.zeros(V, dtype=np.uint32)
+    _check(lib().orc_cc_unionfind(V, _ptr(off), _ptr(idx), _ptr(lab)), "unionfind")
+    return lab
+
+
+def graphmp_baseline(app, off, idx, bounds, msg_init, max_iter, w=None, outdeg=None,
+                     damping=0.85, threads=0):
+    """Timed CPU GraphMP engine; returns (msg, BaselineStats)."""
+    V = off.size - 1
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    bounds = np.ascontiguousarray(bounds, dtype=np.int64)
+    dt = np.float32 if app == APP_PAGERANK else np.uint32
+    msg = np.array(msg_init, dtype=dt, copy=True)
+    wv = None if w is None else np.ascontiguousarray(w, dtype=np.uint32)
+    od = None if outdeg is None else np.ascontiguousarray(outdeg, dtype=np.uint32)
+    st = BaselineStats()
+    _check(lib().orc_graphmp_baseline(app, V, bounds.size - 1, _ptr(bounds), _ptr(off), _ptr(idx),
+                                      _ptr(wv), _ptr(od), damping, max_iter, threads, _ptr(msg),
+                                      ctypes.byref(st)), "baseline")
+    return msg, st
+
+
+def pr_step_residual(off, idx, outdeg, x, d=0.85, threads=0):
+    """(||T(x)-x||_1, ||T(x)-x||_inf) of one FP64 step from the FP32 ranks x; the L1 distance
+    to the exact FP64 fixed point is at most the first value / (1 - d)."""
+    V = off.size - 1
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    od = np.ascontiguousarray(outdeg, dtype=np.uint32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    l1 = ctypes.c_double(0)
+    li = ctypes.c_double(0)
+    _check(lib().orc_pr_f64_step_residual(V, _ptr(off), _ptr(idx), _ptr(od), d, _ptr(x), threads,
+                                          ctypes.byref(l1), ctypes.byref(li)), "step_residual")
+    return l1.value, li.value

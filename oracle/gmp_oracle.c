@@ -1,0 +1,478 @@
+/*
+ * gmp_oracle.c -- CPU oracle for the gfx950 graph engine.  TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this
+ * library, and only as the checker (or the timed CPU baseline).  The product path
+ * (graphee_amd/) never links, loads or calls it.
+ *
+ * Every function restates one piece of GraphMP's algorithm (cap-ntu/Graphee,
+ * directory GraphMP/) in plain C, or is a textbook method that pins it:
+ *
+ *   orc_partition_bounds   GraphMP/tools/format_spark.py:85-108 equal-in-edge partitioner
+ *   orc_app_sync           GraphMP/system/GraphPS.h:211-386 run() loop (synchronous
+ *                          Msg/MsgNew swap, changed count against VertexValue, stop rule
+ *                          changed==0 && step>1) with the three per-vertex update rules
+ *                          GraphMP/app/pagerank.cpp:42-59, sssp.cpp:144-162, cc.cpp:238-255
+ *   orc_pr_f64             textbook FP64 power iteration (uniform or single-seed teleport,
+ *                          dangling mass redistributed to the teleport vector)
+ *   orc_sssp_dijkstra      textbook binary-heap Dijkstra (pins Bellman-Ford / delta-stepping)
+ *   orc_cc_unionfind       textbook union-find, label = minimum id in the component
+ *   orc_graphmp_baseline   GraphPS.h:256-386 engine shape for timing: OpenMP guided over
+ *                          destination-interval shards, per-shard pull gather, sync swap,
+ *                          selective scheduling of shards whose sources changed
+ *
+ * Pinning (which test checks which function) is listed in the test files and the design notes.
+ * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <omp.h>
+
+#define ORC_OK 0
+#define ORC_EINVAL 1
+#define ORC_ENOMEM 5
+
+/* app ids shared with the engine's compat apps */
+#define ORC_APP_PAGERANK 1 /* float, GraphMP/app/pagerank.cpp */
+#define ORC_APP_SSSP 2     /* int32, unit weights, GraphMP/app/sssp.cpp */
+#define ORC_APP_CC 3       /* int32, min label over in-edges, GraphMP/app/cc.cpp */
+#define ORC_APP_SSSP_W 4   /* uint32 weighted generalisation, saturating add */
+
+/* ------------------------------------------------------------------------------------ */
+/* format_spark.py:91-108.  vertex_vector = cumsum(indeg) with a leading 0 (:95-96);
+ * edge_per_partition = int(math.ceil(edge_num/partition_num)) (:93) -- the script is
+ * Python 2 and both operands are integers, so '/' floors before ceil sees it;
+ * boundary i (1..P-1) = argmax(vertex_vector >= edge_per_partition*i) (:99-100), the first
+ * prefix index reaching the target; the last boundary is vertex_num (:105) and a 0 is
+ * prepended (:112).  Shard p covers destinations [bounds[p], bounds[p+1]). */
+int orc_partition_bounds(const int64_t *indeg, int64_t V, int32_t P, int64_t *bounds) {
+  if (P <= 0 || V < 0) return ORC_EINVAL;
+  int64_t E = 0;
+  for (int64_t v = 0; v < V; v++) E += indeg[v];
+  int64_t epp = E / P;
+  int64_t j = 0, cum = 0; /* cum == vertex_vector[j] */
+  bounds[0] = 0;
+  for (int32_t i = 1; i < P; i++) {
+    int64_t target = epp * (int64_t)i;
+    while (cum < target) {
+      cum += indeg[j];
+      j++;
+    }
+    bounds[i] = j;
+  }
+  bounds[P] = V;
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* GraphPS<T>::run synchronous loop with the per-app update of the reference.
+ * off/idx: in-CSR over destinations (rows) with ascending sources, int64 offsets.
+ * msg: in = init_vertex() result (_VertexMsg), out = final _VertexMsg.
+ * value: out = final _VertexValue (initialised to 0 as every init_vertex does).
+ * vout: _VertexOut as loaded (pagerank only; zeros are replaced by 1, pagerank.cpp:77-80).
+ * w: per-edge weights (ORC_APP_SSSP_W only).
+ * Returns the number of executed iterations in *iters and the per-iteration changed
+ * counts (the reference's _Changed_Vertex, GraphPS.h:292) in changed[0..iters). */
+int orc_app_sync(int32_t app, int64_t V, const int64_t *off, const uint32_t *idx,
+                 const uint32_t *w, const int32_t *vout, int32_t max_iter, void *msg,
+                 void *value, int32_t *iters, int64_t *changed) {
+  if (V <= 0) return ORC_EINVAL;
+  size_t es = 4;
+  void *msgnew = malloc((size_t)V * es);
+  int32_t *out = NULL;
+  if (!msgnew) return ORC_ENOMEM;
+  memset(value, 0, (size_t)V * es);
+  memcpy(msgnew, msg, (size_t)V * es); /* GraphPS.h:248 */
+  if (app == ORC_APP_PAGERANK) {
+    out = (int32_t *)malloc((size_t)V * sizeof(int32_t));
+    if (!out) { free(msgnew); return ORC_ENOMEM; }
+    for (int64_t v = 0; v < V; v++) out[v] = vout[v] == 0 ? 1 : vout[v];
+  }
+  int32_t step;
+  for (step = 0; step < max_iter; step++) {
+    int64_t nchanged = 0;
+    if (app == ORC_APP_PAGERANK) {
+      const float *m = (const float *)msg;
+      float *mn = (float *)msgnew, *val = (float *)value;
+      for (int64_t v = 0; v < V; v++) {
+        float rel = 0;
+        for (int64_t k = off[v]; k < off[v + 1]; k++) {
+          uint32_t u = idx[k];
+          rel += m[u] / (float)out[u]; /* pagerank.cpp:46: float / int32 -> float */
+        }
+        rel = (float)((double)rel * 0.85 + 1.0 / (double)V); /* pagerank.cpp:48 */
+        mn[v] = rel;
+        if (rel != val[v]) { nchanged++; val[v] = rel; }
+      }
+    } else if (app == ORC_APP_SSSP || app == ORC_APP_CC) {
+      const int32_t *m = (const int32_t *)msg;
+      int32_t *mn = (int32_t *)msgnew, *val = (int32_t *)value;
+      for (int64_t v = 0; v < V; v++) {
+        int32_t best = m[v];
+        for (int64_t k = off[v]; k < off[v + 1]; k++) {
+          int32_t t = m[idx[k]];
+          if (app == ORC_APP_SSSP) t = (int32_t)((uint32_t)t + 1u); /* sssp.cpp:147 */
+          if (best > t) best = t;
+        }
+        mn[v] = best;
+        if (best != val[v]) { nchanged++; val[v] = best; }
+      }
+    } else if (app == ORC_APP_SSSP_W) {
+      const uint32_t *m = (const uint32_t *)msg;
+      uint32_t *mn = (uint32_t *)msgnew, *val = (uint32_t *)value;
+      for (int64_t v = 0; v < V; v++) {
+        uint32_t best = m[v];
+        for (int64_t k = off[v]; k < off[v + 1]; k++) {
+          uint64_t t = (uint64_t)m[idx[k]] + (uint64_t)w[k];
+          if (t > 0xFFFFFFFFull) t = 0xFFFFFFFFull;
+          if ((uint64_t)best > t) best = (uint32_t)t;
+        }
+        mn[v] = best;
+        if (best != val[v]) { nchanged++; val[v] = best; }
+      }
+    } else {
+      free(msgnew);
+      free(out);
+      return ORC_EINVAL;
+    }
+    memcpy(msg, msgnew, (size_t)V * es); /* GraphPS.h:334/343 */
+    if (changed) changed[step] = nchanged;
+    if (nchanged == 0 && step > 1) { step++; break; } /* GraphPS.h:383-385 */
+  }
+  *iters = step;
+  free(msgnew);
+  free(out);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* FP64 power iteration.  x_{k+1}[v] = (1-d) t[v] + d (sum_{u->v} x_k[u]/outdeg[u] + D_k t[v]),
+ * D_k = sum over dangling u of x_k[u]; t = uniform 1/V, or e_seed when seed >= 0.
+ * Stops after max_iter iterations, or as soon as the L1 change ||x_{k+1}-x_k||_1 < tol
+ * (tol > 0).  *iters = iterations executed, *resid = the last L1 change. */
+int orc_pr_f64(int64_t V, const int64_t *off, const uint32_t *idx, const uint32_t *outdeg,
+               double d, int32_t max_iter, double tol, int64_t seed, const double *x0,
+               double *x, int32_t *iters, double *resid) {
+  if (V <= 0) return ORC_EINVAL;
+  double *c = (double *)malloc((size_t)V * sizeof(double));
+  double *xn = (double *)malloc((size_t)V * sizeof(double));
+  if (!c || !xn) { free(c); free(xn); return ORC_ENOMEM; }
+  for (int64_t v = 0; v < V; v++) x[v] = x0 ? x0[v] : (seed >= 0 ? (v == seed ? 1.0 : 0.0) : 1.0 / (double)V);
+  int32_t it = 0;
+  double r = 0;
+  for (it = 0; it < max_iter;) {
+    double D = 0;
+    for (int64_t u = 0; u < V; u++) {
+      if (outdeg[u] == 0) { D += x[u]; c[u] = 0; }
+      else c[u] = x[u] / (double)outdeg[u];
+    }
+    r = 0;
+    for (int64_t v = 0; v < V; v++) {
+      double s = 0;
+      for (int64_t k = off[v]; k < off[v + 1]; k++) s += c[idx[k]];
+      double t = seed >= 0 ? (v == seed ? 1.0 : 0.0) : 1.0 / (double)V;
+      xn[v] = (1.0 - d) * t + d * (s + D * t);
+      r += fabs(xn[v] - x[v]);
+    }
+    memcpy(x, xn, (size_t)V * sizeof(double));
+    it++;
+    if (tol > 0 && r < tol) break;
+  }
+  if (iters) *iters = it;
+  if (resid) *resid = r;
+  free(c);
+  free(xn);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Dijkstra over an out-CSR (rows = sources).  w == NULL means unit weights.  Unreachable
+ * vertices get 0xFFFFFFFF; a path whose length would exceed 0xFFFFFFFE saturates there
+ * (the engine's saturating uint32 add). */
+typedef struct { uint64_t key; uint32_t v; } orc_heap_item;
+
+static void heap_push(orc_heap_item *h, int64_t *n, orc_heap_item it) {
+  int64_t i = (*n)++;
+  while (i > 0) {
+    int64_t p = (i - 1) / 2;
+    if (h[p].key <= it.key) break;
+    h[i] = h[p];
+    i = p;
+  }
+  h[i] = it;
+}
+static orc_heap_item heap_pop(orc_heap_item *h, int64_t *n) {
+  orc_heap_item top = h[0], last = h[--(*n)];
+  int64_t i = 0;
+  for (;;) {
+    int64_t l = 2 * i + 1, r = l + 1, m = i;
+    uint64_t mk = last.key;
+    if (l < *n && h[l].key < mk) { m = l; mk = h[l].key; }
+    if (r < *n && h[r].key < mk) { m = r; }
+    if (m == i) break;
+    h[i] = h[m];
+    i = m;
+  }
+  h[i] = last;
+  return top;
+}
+
+int orc_sssp_dijkstra(int64_t V, const int64_t *ooff, const uint32_t *oidx, const uint32_t *ow,
+                      uint32_t src, uint32_t *dist) {
+  if (V <= 0 || (int64_t)src >= V) return ORC_EINVAL;
+  int64_t E = ooff[V];
+  orc_heap_item *h = (orc_heap_item *)malloc((size_t)(E + 1) * sizeof(orc_heap_item));
+  if (!h) return ORC_ENOMEM;
+  for (int64_t v = 0; v < V; v++) dist[v] = 0xFFFFFFFFu;
+  int64_t n = 0;
+  dist[src] = 0;
+  orc_heap_item s0 = {0, src};
+  heap_push(h, &n, s0);
+  while (n > 0) {
+    orc_heap_item it = heap_pop(h, &n);
+    if (it.key != dist[it.v]) continue;
+    for (int64_t k = ooff[it.v]; k < ooff[it.v + 1]; k++) {
+      uint64_t nd = it.key + (ow ? ow[k] : 1u);
+      if (nd > 0xFFFFFFFEull) nd = 0xFFFFFFFEull;
+      uint32_t v = oidx[k];
+      if (nd < dist[v]) {
+        dist[v] = (uint32_t)nd;
+        orc_heap_item ni = {nd, v};
+        heap_push(h, &n, ni);
+      }
+    }
+  }
+  free(h);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Union-find over any edge set given as a CSR; label[v] = minimum vertex id in v's
+ * (weakly) connected component -- the unique fixed point of min-label propagation on
+ * a symmetric graph (GraphMP/app/cc.cpp:238-244). */
+static int64_t uf_find(int64_t *p, int64_t x) {
+  while (p[x] != x) {
+    p[x] = p[p[x]];
+    x = p[x];
+  }
+  return x;
+}
+
+int orc_cc_unionfind(int64_t V, const int64_t *off, const uint32_t *idx, uint32_t *label) {
+  if (V <= 0) return ORC_EINVAL;
+  int64_t *p = (int64_t *)malloc((size_t)V * sizeof(int64_t));
+  if (!p) return ORC_ENOMEM;
+  for (int64_t v = 0; v < V; v++) p[v] = v;
+  for (int64_t v = 0; v < V; v++)
+    for (int64_t k = off[v]; k < off[v + 1]; k++) {
+      int64_t a = uf_find(p, v), b = uf_find(p, idx[k]);
+      if (a < b) p[b] = a;
+      else if (b < a) p[a] = b;
+    }
+  for (int64_t v = 0; v < V; v++) label[v] = (uint32_t)uf_find(p, v);
+  free(p);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* CPU baseline: GraphMP's engine shape with the edges resident in RAM (its cache-level-0
+ * best case).  GraphPS.h:256-386: per step, shards in shuffled order under
+ * `omp parallel for schedule(guided)` (:263-278), each running the app's per-shard pull
+ * gather (pagerank.cpp:42-59 etc.) into MsgNew; then the O(V) swap (:338-345); shards
+ * none of whose sources changed are skipped when fewer than BF_THRE=0.001 of the vertices
+ * changed (:318-368) -- exact per-shard source membership stands in for the bloom filter,
+ * whose false positives only add work.
+ *
+ * app: ORC_APP_PAGERANK runs the `standard` formula in FP32 (plain sum like the reference,
+ * teleport (1-d)/N, dangling mass redistributed) for a fixed max_iter;
+ * ORC_APP_SSSP_W and ORC_APP_CC run the min rules until no vertex changes.
+ * threads <= 0 uses omp_get_max_threads().  Returns iterations, wall seconds, edges
+ * gathered. */
+typedef struct {
+  int32_t iterations;
+  int32_t threads;
+  double seconds;
+  double edges;
+} orc_baseline_stats;
+
+static uint64_t orc_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static int cmp_u32(const void *a, const void *b) {
+  uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return x < y ? -1 : (x > y);
+}
+
+static int has_u32(const uint32_t *a, int64_t n, uint32_t x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) / 2;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < n && a[lo] == x;
+}
+
+int orc_graphmp_baseline(int32_t app, int64_t V, int32_t P, const int64_t *bounds,
+                         const int64_t *off, const uint32_t *idx, const uint32_t *w,
+                         const uint32_t *outdeg, float damping, int32_t max_iter,
+                         int32_t threads, void *msg, orc_baseline_stats *st) {
+  if (V <= 0 || P <= 0) return ORC_EINVAL;
+  if (threads <= 0) threads = omp_get_max_threads();
+  int64_t E = off[V];
+  void *msgnew = malloc((size_t)V * 4);
+  float *contrib = NULL;
+  uint8_t *pactive = (uint8_t *)malloc((size_t)P);
+  int32_t *order = (int32_t *)malloc((size_t)P * sizeof(int32_t));
+  /* per-shard sorted unique source sets: the exact stand-in for the per-shard bloom
+   * filters built in GraphPS.h:225-245 (built before timing, as the reference builds its
+   * filters during init) */
+  uint32_t *srcset = (uint32_t *)malloc((size_t)(E > 0 ? E : 1) * sizeof(uint32_t));
+  int64_t *ss_off = (int64_t *)malloc((size_t)(P + 1) * sizeof(int64_t));
+  uint32_t *active = (uint32_t *)malloc((size_t)V * sizeof(uint32_t));
+  if (!msgnew || !pactive || !order || !srcset || !ss_off || !active) {
+    free(msgnew); free(pactive); free(order); free(srcset); free(ss_off); free(active);
+    return ORC_ENOMEM;
+  }
+  if (app == ORC_APP_PAGERANK) contrib = (float *)malloc((size_t)V * sizeof(float));
+  ss_off[0] = 0;
+  for (int32_t p = 0; p < P; p++) {
+    int64_t a = off[bounds[p]], b = off[bounds[p + 1]], o = ss_off[p];
+    memcpy(srcset + o, idx + a, (size_t)(b - a) * sizeof(uint32_t));
+    qsort(srcset + o, (size_t)(b - a), sizeof(uint32_t), cmp_u32);
+    int64_t n = 0;
+    for (int64_t q = 0; q < b - a; q++)
+      if (n == 0 || srcset[o + q] != srcset[o + n - 1]) srcset[o + n++] = srcset[o + q];
+    ss_off[p + 1] = o + n;
+  }
+  memcpy(msgnew, msg, (size_t)V * 4);
+  memset(pactive, 1, (size_t)P);
+  double t0 = omp_get_wtime();
+  double edges = 0;
+  int32_t step;
+  uint64_t shuf = 0x5eed;
+  for (step = 0; step < max_iter; step++) {
+    for (int32_t k = 0; k < P; k++) order[k] = k;
+    for (int32_t k = P - 1; k > 0; k--) { /* std::random_shuffle, GraphPS.h:263 */
+      shuf = orc_mix64(shuf + 0x9E3779B97F4A7C15ull);
+      int32_t j = (int32_t)(shuf % (uint64_t)(k + 1));
+      int32_t t = order[k]; order[k] = order[j]; order[j] = t;
+    }
+    float base = 0, d = damping;
+    if (app == ORC_APP_PAGERANK) {
+      const float *m = (const float *)msg;
+      double D = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : D)
+      for (int64_t u = 0; u < V; u++) {
+        if (outdeg[u]) contrib[u] = m[u] / (float)outdeg[u];
+        else { contrib[u] = 0; D += m[u]; }
+      }
+      base = (float)((1.0 - damping) / (double)V + damping * D / (double)V);
+    }
+    int64_t nchanged = 0;
+    double e_it = 0;
+#pragma omp parallel for num_threads(threads) schedule(guided) reduction(+ : nchanged, e_it)
+    for (int32_t k = 0; k < P; k++) {
+      int32_t p = order[k];
+      if (!pactive[p]) continue;
+      int64_t s = bounds[p], e = bounds[p + 1];
+      e_it += (double)(off[e] - off[s]);
+      if (app == ORC_APP_PAGERANK) {
+        const float *m = (const float *)msg;
+        float *mn = (float *)msgnew;
+        for (int64_t v = s; v < e; v++) {
+          float rel = 0;
+          for (int64_t q = off[v]; q < off[v + 1]; q++) rel += contrib[idx[q]];
+          rel = base + d * rel;
+          mn[v] = rel;
+          if (rel != m[v]) nchanged++;
+        }
+      } else {
+        const uint32_t *m = (const uint32_t *)msg;
+        uint32_t *mn = (uint32_t *)msgnew;
+        for (int64_t v = s; v < e; v++) {
+          uint32_t best = m[v];
+          for (int64_t q = off[v]; q < off[v + 1]; q++) {
+            uint64_t t = (uint64_t)m[idx[q]] + (app == ORC_APP_SSSP_W ? (uint64_t)w[q] : 0u);
+            if (t < best) best = (uint32_t)t;
+          }
+          mn[v] = best;
+          if (best != m[v]) nchanged++;
+        }
+      }
+    }
+    edges += e_it;
+    /* GraphPS.h:316-368: collect changed vertices while swapping when the changed ratio
+     * is below BF_THRE, then keep only shards holding a changed source */
+    double ratio = (double)nchanged / (double)V;
+    uint32_t *mo = (uint32_t *)msg;
+    const uint32_t *mnew = (const uint32_t *)msgnew;
+    if (ratio < 0.001) {
+      int64_t na = 0;
+      for (int64_t v = 0; v < V; v++) {
+        if (mo[v] != mnew[v]) active[na++] = (uint32_t)v;
+        mo[v] = mnew[v];
+      }
+#pragma omp parallel for num_threads(threads) schedule(static)
+      for (int32_t p = 0; p < P; p++) {
+        uint8_t act = 0;
+        for (int64_t a = 0; a < na && !act; a++)
+          act = (uint8_t)has_u32(srcset + ss_off[p], ss_off[p + 1] - ss_off[p], active[a]);
+        pactive[p] = act;
+      }
+    } else {
+#pragma omp parallel for num_threads(threads) schedule(static)
+      for (int64_t v = 0; v < V; v++) mo[v] = mnew[v];
+      memset(pactive, 1, (size_t)P);
+    }
+    if (app != ORC_APP_PAGERANK && nchanged == 0) { step++; break; }
+  }
+  double t1 = omp_get_wtime();
+  if (st) {
+    st->iterations = step;
+    st->threads = threads;
+    st->seconds = t1 - t0;
+    st->edges = edges;
+  }
+  free(msgnew); free(contrib); free(pactive); free(order); free(srcset); free(ss_off); free(active);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* A-posteriori FP64 check of an FP32 PageRank vector x (uniform teleport): one FP64 step
+ * y = T(x) of the standard formula, returning ||y - x||_1 and ||y - x||_inf.  Since T is
+ * a contraction with factor d in L1, ||x - x*||_1 <= ||T(x) - x||_1 / (1 - d) bounds the
+ * distance to the exact fixed point. */
+int orc_pr_f64_step_residual(int64_t V, const int64_t *off, const uint32_t *idx,
+                             const uint32_t *outdeg, double d, const float *x, int32_t threads,
+                             double *l1, double *linf) {
+  if (V <= 0) return ORC_EINVAL;
+  if (threads <= 0) threads = omp_get_max_threads();
+  double *c = (double *)malloc((size_t)V * sizeof(double));
+  if (!c) return ORC_ENOMEM;
+  double D = 0, s1 = 0, mx = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : D)
+  for (int64_t u = 0; u < V; u++) {
+    if (outdeg[u] == 0) { D += x[u]; c[u] = 0; }
+    else c[u] = (double)x[u] / (double)outdeg[u];
+  }
+  const double t = 1.0 / (double)V;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096) reduction(+ : s1) reduction(max : mx)
+  for (int64_t v = 0; v < V; v++) {
+    double s = 0;
+    for (int64_t k = off[v]; k < off[v + 1]; k++) s += c[idx[k]];
+    double y = (1.0 - d) * t + d * (s + D * t);
+    double e = fabs(y - (double)x[v]);
+    s1 += e;
+    if (e > mx) mx = e;
+  }
+  *l1 = s1;
+  *linf = mx;
+  free(c);
+  return ORC_OK;
+}

@@ -1,0 +1,242 @@
+"""Parity of the HIP engine (through the C ABI) against the CPU oracle.
+
+Integer results (SSSP distances, CC labels, the GraphMP-compatible apps) must match bit
+for bit; standard PageRank must be within 1e-6 L1 of the FP64 oracle (the FP32 ranks
+carry about 1e-7 relative rounding per vertex)."""
+import sys
+
+import numpy as np
+import pytest
+
+import graphee_amd as gmp
+import oracle
+from oracle import rmat_ref
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+PR_L1_TOL = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gen(engine, scale, ef=16, seed=None, **kw):
+    seed = 1000 + scale if seed is None else seed
+    say(f"generate R-MAT scale={scale} ef={ef} seed={seed} {kw}")
+    engine.generate_rmat(scale, ef, seed, **kw)
+    return engine.download(weights=kw.get("weighted", False))
+
+
+# ---- graph construction ----------------------------------------------------------------
+
+@pytest.mark.parametrize("scale,undirected,weighted,off64",
+                         [(8, False, False, False), (11, False, True, False), (10, True, False, False),
+                          (9, False, True, True)])
+def test_generator_matches_definition(engine, scale, undirected, weighted, off64):
+    g = gen(engine, scale, 16, undirected=undirected, weighted=weighted, off64=off64, num_shards=4)
+    V, off, idx = rmat_ref.in_csr(scale, 16, 1000 + scale, symmetric=undirected)
+    np.testing.assert_array_equal(g["off"], off)
+    np.testing.assert_array_equal(g["idx"], idx)
+    np.testing.assert_array_equal(g["outdeg"], graphs.out_degree(V, idx))
+    np.testing.assert_array_equal(g["bounds"], oracle.partition_bounds(np.diff(off), 4))
+    if weighted:
+        rows = np.repeat(np.arange(V, dtype=np.uint64), np.diff(off))
+        np.testing.assert_array_equal(g["w"], rmat_ref.weights(idx, rows, 1000 + scale))
+
+
+def test_pick_source_matches_definition(engine):
+    g = gen(engine, 10)
+    assert engine.pick_source(1010, 3) == rmat_ref.pick_source(1 << 10, g["outdeg"], 1010, 3)
+
+
+# ---- standard PageRank -----------------------------------------------------------------
+
+@pytest.mark.parametrize("kahan", [True, False])
+def test_pagerank_20_iterations_vs_fp64(engine, kahan):
+    g = gen(engine, 14)
+    say(f"pagerank 20 iterations kahan={kahan}")
+    r, st = engine.pagerank(max_iter=20, kahan=kahan)
+    x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
+    l1 = np.abs(r.astype(np.float64) - x).sum()
+    assert l1 < PR_L1_TOL, l1
+    assert st.iterations == 20 and st.edges == 20 * g["idx"].size
+    assert abs(st.dangling - x[g["outdeg"] == 0].sum()) < 1e-3
+
+
+def test_pagerank_tolerance_stop_vs_fp64(engine):
+    g = gen(engine, 13)
+    say("pagerank until L1 change < 1e-7")
+    r, st = engine.pagerank(max_iter=100, tol=1e-7)
+    x, it, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=100, tol=1e-7)
+    assert abs(st.iterations - it) <= 1 and st.residual < 1e-7
+    assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+    assert abs(r.astype(np.float64).sum() - 1.0) < 1e-5
+
+
+def test_personalised_pagerank_vs_fp64(engine):
+    g = gen(engine, 12)
+    seed = int(np.argmax(g["outdeg"]))
+    say(f"personalised pagerank seed={seed}")
+    r, _ = engine.pagerank(max_iter=30, seed_vertex=seed)
+    x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=30, seed=seed)
+    assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+
+
+def test_pagerank_multichunk_hub_rows(engine):
+    """Rows longer than one chunk take the partial + combine path."""
+    V = 40000
+    rng = np.random.default_rng(5)
+    src = np.concatenate([np.arange(1, V), rng.integers(0, V, 100000), np.arange(2, 9000)])
+    dst = np.concatenate([np.zeros(V - 1, np.int64), rng.integers(0, V, 100000), np.ones(8998, np.int64)])
+    off, idx = graphs.csr_from_edges(V, src, dst)
+    say(f"pagerank hub rows: in-degree of 0 = {off[1]}, of 1 = {off[2] - off[1]}")
+    engine.load_csr(off, idx)
+    r, _ = engine.pagerank(max_iter=25)
+    x, _, _ = oracle.pr_f64(off, idx, graphs.out_degree(V, idx), max_iter=25)
+    assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+
+
+# ---- GraphMP-compatible apps: bit-exact ------------------------------------------------
+
+def test_compat_pagerank_bit_exact(engine):
+    g = gen(engine, 11)
+    V = g["outdeg"].size
+    vout = g["outdeg"].astype(np.int32)
+    msg0 = np.full(V, 1.0 / V, dtype=np.float32)
+    say("compat pagerank (pagerank.cpp rule), 200 steps max")
+    msg, val, st = engine.run_app(gmp.APP_PAGERANK, msg0, 200, vertex_out=vout)
+    want, wval, it, changed = oracle.app_sync(oracle.APP_PAGERANK, g["off"], g["idx"], msg0, 200, vout=vout)
+    np.testing.assert_array_equal(msg.view(np.uint32), want.view(np.uint32))
+    np.testing.assert_array_equal(val.view(np.uint32), wval.view(np.uint32))
+    assert st.iterations == it and st.changed == changed[-1]
+
+
+@pytest.mark.parametrize("app", [gmp.APP_SSSP, gmp.APP_CC, gmp.APP_SSSP_W])
+def test_compat_min_apps_bit_exact(engine, app):
+    g = gen(engine, 12, weighted=(app == gmp.APP_SSSP_W))
+    V = g["outdeg"].size
+    if app == gmp.APP_CC:
+        msg0 = np.arange(V, dtype=np.int32)
+    elif app == gmp.APP_SSSP:
+        msg0 = np.full(V, 10000, dtype=np.int32)
+        msg0[1] = 0
+    else:
+        msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+        msg0[engine.pick_source(1012)] = 0
+    say(f"compat app {app}")
+    msg, val, st = engine.run_app(app, msg0, 1000)
+    want, wval, it, _ = oracle.app_sync(app, g["off"], g["idx"], msg0, 1000, w=g["w"])
+    np.testing.assert_array_equal(msg, want)
+    np.testing.assert_array_equal(val, wval)
+    assert st.iterations == it
+
+
+# ---- SSSP and CC -----------------------------------------------------------------------
+
+@pytest.mark.parametrize("method,delta", [(gmp.SSSP_BF, 0), (gmp.SSSP_DELTA, 64), (gmp.SSSP_DELTA, 1),
+                                          (gmp.SSSP_DELTA, 1000)])
+def test_sssp_matches_dijkstra(engine, method, delta):
+    g = gen(engine, 13, weighted=True)
+    V = g["outdeg"].size
+    src = engine.pick_source(1013)
+    say(f"sssp method={method} delta={delta} src={src}")
+    d, st = engine.sssp(src, method=method, delta=delta)
+    ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+    np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, ow, src))
+    assert st.edges > 0
+
+
+def test_sssp_unit_weights_is_bfs(engine):
+    g = gen(engine, 12)
+    V = g["outdeg"].size
+    src = engine.pick_source(1012)
+    say(f"sssp unit weights src={src}")
+    d, _ = engine.sssp(src, method=gmp.SSSP_DELTA, delta=2)
+    ooff, oidx, _ = graphs.transpose(V, g["off"], g["idx"])
+    np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, None, src))
+
+
+@pytest.mark.parametrize("scale", [10, 14])
+def test_cc_matches_unionfind(engine, scale):
+    g = gen(engine, scale, undirected=True)
+    say(f"cc scale={scale}")
+    lab, st = engine.cc()
+    np.testing.assert_array_equal(lab, oracle.cc_unionfind(g["off"], g["idx"]))
+    assert st.iterations >= 1
+
+
+def test_off64_offsets_path(engine):
+    g = gen(engine, 11, weighted=True, off64=True)
+    V = g["outdeg"].size
+    say("off64: pagerank, sssp")
+    r, _ = engine.pagerank(max_iter=15)
+    x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=15)
+    assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+    src = engine.pick_source(1011)
+    d, _ = engine.sssp(src, method=gmp.SSSP_DELTA, delta=64)
+    ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+    np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, ow, src))
+
+
+# ---- shard files and edge cases --------------------------------------------------------
+
+def test_load_shards_roundtrip(engine, tmp_path):
+    g = gen(engine, 11, weighted=True, num_shards=8)
+    V = g["outdeg"].size
+    gmp.write_shards(tmp_path, g["off"], g["idx"], g["bounds"], weights=g["w"])
+    r1, _ = engine.pagerank(max_iter=10)
+    src = engine.pick_source(5)
+    d1, _ = engine.sssp(src)
+    say("load shards from", tmp_path)
+    engine.load_shards(tmp_path, V, 8)
+    h = engine.download(weights=True)
+    np.testing.assert_array_equal(h["off"], g["off"])
+    np.testing.assert_array_equal(h["idx"], g["idx"])
+    np.testing.assert_array_equal(h["w"], g["w"])
+    np.testing.assert_array_equal(h["bounds"], g["bounds"])
+    r2, _ = engine.pagerank(max_iter=10)
+    d2, _ = engine.sssp(src)
+    np.testing.assert_array_equal(r1, r2)
+    np.testing.assert_array_equal(d1, d2)
+
+
+def test_load_shards_missing_file_is_an_error(engine, tmp_path):
+    off, idx = graphs.random_graph(100, 300, seed=1)
+    gmp.write_shards(tmp_path, off, idx, np.array([0, 50, 100], dtype=np.uint32))
+    (tmp_path / "1.edge.npy").unlink()
+    with pytest.raises(gmp.GmpError) as ei:
+        engine.load_shards(tmp_path, 100, 2)
+    assert ei.value.code == gmp.EIO
+
+
+def test_edgeless_and_tiny_graphs(engine):
+    say("edgeless graph")
+    V = 5
+    engine.load_csr(np.zeros(V + 1, dtype=np.uint64), np.zeros(0, dtype=np.uint32))
+    r, _ = engine.pagerank(max_iter=3)
+    np.testing.assert_allclose(r, 1.0 / V, rtol=1e-6)
+    d, _ = engine.sssp(2)
+    np.testing.assert_array_equal(d, [0xFFFFFFFF, 0xFFFFFFFF, 0, 0xFFFFFFFF, 0xFFFFFFFF])
+    lab, _ = engine.cc()
+    np.testing.assert_array_equal(lab, np.arange(V))
+    say("single edge")
+    engine.load_csr(np.array([0, 0, 1], dtype=np.uint64), np.array([0], dtype=np.uint32))
+    d, _ = engine.sssp(0)
+    np.testing.assert_array_equal(d, [0, 1])
+
+
+def test_argument_errors(engine):
+    engine.load_csr(np.array([0, 1, 1], dtype=np.uint64), np.array([1], dtype=np.uint32))
+    with pytest.raises(gmp.GmpError) as ei:
+        engine.sssp(7)
+    assert ei.value.code == gmp.EINVAL
+    with pytest.raises(gmp.GmpError) as ei:
+        engine.load_csr(np.array([0, 1, 1], dtype=np.uint64), np.array([9], dtype=np.uint32))
+    assert ei.value.code == gmp.EINVAL
+    with gmp.Engine(0) as fresh:
+        with pytest.raises(gmp.GmpError) as ei:
+            fresh.pagerank()
+        assert ei.value.code == gmp.ESTATE

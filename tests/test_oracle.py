@@ -1,0 +1,278 @@
+"""Pins the CPU oracle (oracle/) against independent methods: scipy's graph routines,
+dense linear algebra, closed forms on tiny graphs and brute force.  CPU only.
+
+The reference ships no tests, golden vectors or fixtures for this path, and it cannot be
+compiled here (its headers need MPI, glog, ZeroMQ and snappy), so these checks are what
+anchor the oracle.  Each check is chosen so that a plausible slip in the oracle (a dropped
+teleport term, a wrong damping placement, a transposed operand, an off-by-one in the
+partitioner or the stop rule) makes it fail.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csg
+
+import oracle
+from oracle import rmat_ref
+from tests import graphs
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---- partitioner (format_spark.py:91-108) ---------------------------------------------
+
+def _argmax_partition(indeg, P):
+    """The script's own steps, in NumPy: cumsum with a leading 0, floor(E/P) (Python 2
+    integer division), boundary i = argmax(prefix >= epp*i), last boundary = V."""
+    vv = np.append(0, np.cumsum(indeg))
+    epp = int(indeg.sum()) // P
+    b = [0] + [int(np.argmax(vv >= epp * i)) for i in range(1, P)] + [indeg.size]
+    return np.array(b, dtype=np.int64)
+
+
+@pytest.mark.parametrize("seed,V,P", [(0, 100, 4), (1, 1000, 16), (2, 37, 37), (3, 500, 1), (4, 10, 64)])
+def test_partition_matches_script_steps(seed, V, P):
+    rng = np.random.default_rng(seed)
+    indeg = rng.zipf(1.7, V).clip(0, 5000) * (rng.random(V) < 0.7)
+    got = oracle.partition_bounds(indeg, P)
+    np.testing.assert_array_equal(got, _argmax_partition(indeg, P))
+    assert np.all(np.diff(got) >= 0) and got[0] == 0 and got[-1] == V
+
+
+def test_partition_balances_in_edges():
+    indeg = np.full(1000, 16, dtype=np.int64)
+    b = oracle.partition_bounds(indeg, 10)
+    np.testing.assert_array_equal(b, np.arange(0, 1001, 100))
+
+
+# ---- GraphMP synchronous apps ----------------------------------------------------------
+
+def test_pagerank_graphmp_fixed_point_matches_linear_solve():
+    V = 200
+    off, idx = graphs.random_graph(V, 1500, seed=5)
+    vout = graphs.out_degree(V, idx).astype(np.int32)
+    msg0 = np.full(V, 1.0 / V, dtype=np.float32)
+    msg, val, iters, changed = oracle.app_sync(oracle.APP_PAGERANK, off, idx, msg0, 500, vout=vout)
+    assert iters < 500 and changed[-1] == 0
+    # r = 0.85 * A^T diag(1/max(out,1)) r + 1/N   (pagerank.cpp:46-48, :77-80)
+    A = graphs.scipy_adj(V, off, idx).toarray()  # A[u, v] = 1 for u -> v
+    o = np.maximum(vout, 1).astype(np.float64)
+    M = (A / o[:, None]).T
+    x = np.linalg.solve(np.eye(V) - 0.85 * M, np.full(V, 1.0 / V))
+    np.testing.assert_allclose(msg, x, rtol=2e-5)
+    np.testing.assert_array_equal(msg, val)
+
+
+def test_pagerank_graphmp_exact_first_step():
+    # V = 4, every contribution a power of two, so the FP32 sum is exact in any order
+    off, idx = graphs.csr_from_edges(4, [0, 1, 2, 3, 0], [1, 2, 3, 0, 2])
+    vout = np.array([2, 1, 1, 1], dtype=np.int32)
+    msg0 = np.array([0.5, 0.25, 0.125, 0.0625], dtype=np.float32)
+    msg, _, _, _ = oracle.app_sync(oracle.APP_PAGERANK, off, idx, msg0, 1, vout=vout)
+    s = [0.0625, 0.25, 0.25 + 0.25, 0.125]  # in-sums: 0<-3, 1<-0(/2), 2<-{0(/2),1}, 3<-2
+    want = np.array([np.float32(x * 0.85 + 1.0 / 4) for x in s], dtype=np.float32)
+    np.testing.assert_array_equal(msg, want)
+
+
+def test_stop_rule_counts_three_steps_on_empty_graph():
+    # GraphPS.h:383: break only when nothing changed AND step > 1.  Without edges step 0
+    # changes every vertex (VertexValue starts at 0), steps 1 and 2 change nothing.
+    off = np.zeros(9, dtype=np.int64)
+    idx = np.zeros(0, dtype=np.uint32)
+    msg0 = np.full(8, 1.0 / 8, dtype=np.float32)
+    msg, _, iters, changed = oracle.app_sync(oracle.APP_PAGERANK, off, idx, msg0, 50,
+                                             vout=np.zeros(8, np.int32))
+    assert iters == 3
+    np.testing.assert_array_equal(changed, [8, 0, 0])
+    np.testing.assert_array_equal(msg, np.float32(1.0 / 8))
+
+
+def test_sssp_compat_is_bfs_levels():
+    V = 400
+    off, idx = graphs.random_graph(V, 1200, seed=7)
+    msg0 = np.full(V, 10000, dtype=np.int32)  # GPS_INF, sssp.cpp:175-176
+    msg0[1] = 0
+    msg, _, _, _ = oracle.app_sync(oracle.APP_SSSP, off, idx, msg0, 1000)
+    A = graphs.scipy_adj(V, off, idx)
+    d = csg.shortest_path(A, directed=True, unweighted=True, indices=1)
+    want = np.where(np.isinf(d), 10000, d).astype(np.int32)
+    np.testing.assert_array_equal(msg, want)
+
+
+def test_sssp_weighted_matches_scipy_dijkstra():
+    V = 300
+    off, idx = graphs.random_graph(V, 2000, seed=8)
+    w = graphs.edge_weights(off, idx, seed=9)
+    msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+    msg0[3] = 0
+    msg, _, _, _ = oracle.app_sync(oracle.APP_SSSP_W, off, idx, msg0, 1000, w=w)
+    A = graphs.scipy_adj(V, off, idx, w)
+    d = csg.dijkstra(A, directed=True, indices=3)
+    want = np.where(np.isinf(d), 0xFFFFFFFF, d).astype(np.uint32)
+    np.testing.assert_array_equal(msg, want)
+
+
+def test_cc_compat_directed_is_min_ancestor():
+    V = 120
+    off, idx = graphs.random_graph(V, 150, seed=11)
+    msg0 = np.arange(V, dtype=np.int32)
+    msg, _, _, _ = oracle.app_sync(oracle.APP_CC, off, idx, msg0, 1000)
+    reach = np.isfinite(csg.shortest_path(graphs.scipy_adj(V, off, idx), directed=True, unweighted=True))
+    want = np.array([np.flatnonzero(reach[:, v]).min() for v in range(V)], dtype=np.int32)
+    np.testing.assert_array_equal(msg, want)
+
+
+# ---- textbook methods ------------------------------------------------------------------
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_dijkstra_matches_scipy(seed):
+    V = 500
+    off, idx = graphs.random_graph(V, 4000, seed=seed)
+    w = graphs.edge_weights(off, idx, seed=seed + 100)
+    ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+    got = oracle.dijkstra(ooff, oidx, ow, 0)
+    d = csg.dijkstra(graphs.scipy_adj(V, off, idx, w), directed=True, indices=0)
+    np.testing.assert_array_equal(got, np.where(np.isinf(d), 0xFFFFFFFF, d).astype(np.uint32))
+
+
+def test_unionfind_matches_scipy_components():
+    V = 2000
+    off, idx = graphs.random_graph(V, 1500, seed=3)
+    off, idx = graphs.symmetrize(V, off, idx)
+    lab = oracle.cc_unionfind(off, idx)
+    n, comp = csg.connected_components(graphs.scipy_adj(V, off, idx), directed=False)
+    mins = np.full(n, V)
+    np.minimum.at(mins, comp, np.arange(V))
+    np.testing.assert_array_equal(lab, mins[comp])
+
+
+@pytest.mark.parametrize("seed_vertex", [-1, 17])
+def test_pr_f64_matches_dense_solve(seed_vertex):
+    V, d = 150, 0.85
+    off, idx = graphs.random_graph(V, 600, seed=21)
+    od = graphs.out_degree(V, idx)
+    x, iters, r = oracle.pr_f64(off, idx, od, d=d, max_iter=1000, tol=1e-14, seed=seed_vertex)
+    assert r < 1e-14
+    t = np.full(V, 1.0 / V) if seed_vertex < 0 else np.eye(V)[seed_vertex]
+    A = graphs.scipy_adj(V, off, idx).toarray()
+    P = np.zeros((V, V))
+    for u in range(V):  # column u: where u's mass goes (dangling -> teleport vector)
+        P[:, u] = A[u] / od[u] if od[u] else t
+    want = np.linalg.solve(np.eye(V) - d * P, (1 - d) * t)
+    np.testing.assert_allclose(x, want, rtol=1e-10, atol=1e-14)
+    assert abs(x.sum() - 1.0) < 1e-12
+
+
+def test_pr_f64_cycle_is_uniform():
+    V = 64
+    off, idx = graphs.csr_from_edges(V, np.arange(V), (np.arange(V) + 1) % V)
+    x, _, _ = oracle.pr_f64(off, idx, graphs.out_degree(V, idx), max_iter=5)
+    np.testing.assert_allclose(x, 1.0 / V, rtol=0, atol=1e-15)
+
+
+# ---- CPU baseline engine ---------------------------------------------------------------
+
+def test_baseline_apps_match_textbook():
+    V = 5000
+    off, idx = graphs.random_graph(V, 20000, seed=31)
+    od = graphs.out_degree(V, idx)
+    bounds = oracle.partition_bounds(np.diff(off), 7)
+    # PageRank standard, FP32 vs FP64 power iteration
+    msg0 = np.full(V, 1.0 / V, dtype=np.float32)
+    pr, st = oracle.graphmp_baseline(oracle.APP_PAGERANK, off, idx, bounds, msg0, 20, outdeg=od)
+    x, _, _ = oracle.pr_f64(off, idx, od, max_iter=20)
+    assert np.abs(pr - x).sum() < 1e-5 and st.iterations == 20 and st.edges == 20 * idx.size
+    # weighted SSSP (selective scheduling kicks in once < 0.1% of vertices change)
+    w = graphs.edge_weights(off, idx, seed=32)
+    msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+    msg0[0] = 0
+    many = oracle.partition_bounds(np.diff(off), 200)
+    dist, st = oracle.graphmp_baseline(oracle.APP_SSSP_W, off, idx, many, msg0, 10000, w=w)
+    ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+    np.testing.assert_array_equal(dist, oracle.dijkstra(ooff, oidx, ow, 0))
+    assert st.edges < st.iterations * idx.size  # some shards were skipped
+    # CC on the symmetrised graph
+    soff, sidx = graphs.symmetrize(V, off, idx)
+    sb = oracle.partition_bounds(np.diff(soff), 7)
+    lab, _ = oracle.graphmp_baseline(oracle.APP_CC, soff, sidx, sb, np.arange(V, dtype=np.uint32), 10000)
+    np.testing.assert_array_equal(lab, oracle.cc_unionfind(soff, sidx))
+
+
+# ---- synthetic-input definition --------------------------------------------------------
+
+@pytest.mark.parametrize("scale", [1, 2, 5, 12, 17])
+def test_perm_is_bijection(scale):
+    x = np.arange(1 << scale, dtype=np.uint64)
+    y = rmat_ref.perm(x, scale, seed=99)
+    assert np.unique(y).size == x.size and y.max() < (1 << scale)
+
+
+def test_rmat_quadrant_frequencies():
+    # asymmetric parameters so a swapped b/c or src/dst bit would show
+    a, b, c = 0.5, 0.3, 0.15
+    src, dst = rmat_ref.raw_edges(10, 64, seed=4, a=a, b=b, c=c, permute=False)
+    top_s = (src >> np.uint64(9)) & np.uint64(1)
+    top_d = (dst >> np.uint64(9)) & np.uint64(1)
+    n = src.size
+    assert abs((top_s == 0).mean() - (a + b)) < 0.01
+    assert abs((top_d == 0).mean() - (a + c)) < 0.01
+    assert abs(((top_s == 1) & (top_d == 1)).mean() - (1 - a - b - c)) < 0.01
+    assert n == 64 << 10
+
+
+def test_rmat_csr_properties():
+    V, off, idx = rmat_ref.in_csr(12, 16, seed=1012)
+    rows = np.repeat(np.arange(V), np.diff(off))
+    assert np.all(rows != idx)                                      # no self-loops
+    key = rows.astype(np.int64) * V + idx
+    assert np.all(np.diff(key) > 0)                                 # sorted, deduplicated
+    V2, off2, idx2 = rmat_ref.in_csr(12, 16, seed=1012, symmetric=True)
+    A = graphs.scipy_adj(V2, off2, idx2)
+    assert (A != A.T).nnz == 0
+    w = rmat_ref.weights(idx, rows, seed=1012)
+    assert w.min() >= 1 and w.max() <= 255
+
+
+# ---- hand-derived known answers --------------------------------------------------------
+
+def test_golden_tiny_graphs():
+    with open(os.path.join(GOLDEN, "tiny_graphs.json")) as f:
+        cases = json.load(f)
+    for case in cases:
+        V = case["V"]
+        src, dst = np.array(case["edges"], dtype=np.int64).T
+        off, idx = graphs.csr_from_edges(V, src, dst)
+        if case["app"] == "sssp":
+            w = None
+            if "weights" in case:
+                lut = {(s, d): x for (s, d), x in zip(case["edges"], case["weights"])}
+                rows = np.repeat(np.arange(V), np.diff(off))
+                w = np.array([lut[(int(s), int(d))] for s, d in zip(idx, rows)], dtype=np.uint32)
+            ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+            got = oracle.dijkstra(ooff, oidx, ow, case["source"])
+            np.testing.assert_array_equal(got.astype(np.int64), case["expect"], err_msg=case["name"])
+        elif case["app"] == "cc":
+            got = oracle.cc_unionfind(*graphs.symmetrize(V, off, idx))
+            np.testing.assert_array_equal(got, case["expect"], err_msg=case["name"])
+        elif case["app"] == "pagerank":
+            x, _, _ = oracle.pr_f64(off, idx, graphs.out_degree(V, idx), d=case["damping"],
+                                    max_iter=2000, tol=1e-15)
+            np.testing.assert_allclose(x, case["expect"], rtol=1e-12, err_msg=case["name"])
+
+
+def test_step_residual_bounds_distance_to_fixed_point():
+    V, d = 300, 0.85
+    off, idx = graphs.random_graph(V, 1500, seed=41)
+    od = graphs.out_degree(V, idx)
+    xs, _, _ = oracle.pr_f64(off, idx, od, d=d, max_iter=2000, tol=1e-15)
+    for k in (3, 10, 40):
+        xk, _, _ = oracle.pr_f64(off, idx, od, d=d, max_iter=k)
+        x32 = xk.astype(np.float32)
+        l1, linf = oracle.pr_step_residual(off, idx, od, x32, d=d)
+        true = np.abs(x32.astype(np.float64) - xs).sum()
+        assert true <= l1 / (1 - d) + 1e-12
+        assert l1 >= (1 - d) * true - 1e-12   # ||T(x)-x|| >= (1-d)||x-x*|| as well
+        assert 0 < linf <= l1
