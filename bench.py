@@ -45,6 +45,24 @@ def load_traffic(scale, ef):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
+def copy_calibration(nbytes):
+    """One hipMemcpy device-to-device of nbytes (reads nbytes, writes nbytes)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {nbytes >> 20} MiB device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +75,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--no-extra", action="store_true", help="skip the SSSP and CC measurements")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size "
+                         "dispatch that scales the HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -68,6 +89,8 @@ def main():
     scale, ef = args.scale, args.edge_factor
     seed = 1000 + scale
     eng = gmp.Engine(0)
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib << 20)
     t0 = time.perf_counter()
     eng.generate_rmat(scale, ef, seed, weighted=True, num_shards=16)
     V, E, P, _ = eng.info()

@@ -17,12 +17,14 @@ APP_PAGERANK, APP_SSSP, APP_CC, APP_SSSP_W = 1, 2, 3, 4
 SSSP_BF, SSSP_DELTA, SSSP_LEARNED = 0, 1, 2
 PR_NO_KAHAN, PR_COMPRESSED = 1, 2
 GRAPH_OFF64 = 1
+OPT_REORDER, OPT_STREAM_CAP, OPT_ROW_CAP, OPT_CHUNK, OPT_WG_PER_CU, OPT_VEC_MIN = 1, 2, 3, 4, 5, 6
 
 # every entry point include/gmp.h declares
 SYMBOLS = (
     "gmp_last_error", "gmp_abi_version", "gmp_create", "gmp_destroy", "gmp_load_csr",
     "gmp_load_shards", "gmp_generate_rmat", "gmp_graph_info", "gmp_graph_download",
     "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_sssp", "gmp_cc",
+    "gmp_set_option", "gmp_get_option",
     "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards",
 )
 
@@ -94,6 +96,8 @@ def lib():
         "gmp_graph_info": ([_P, _P, _P, _P, _P], ctypes.c_int),
         "gmp_graph_download": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
         "gmp_pick_source": ([_P, u64, u32, _P], ctypes.c_int),
+        "gmp_set_option": ([_P, i32, i64], ctypes.c_int),
+        "gmp_get_option": ([_P, i32, _P], ctypes.c_int),
         "gmp_run_app": ([_P, i32, i32, _P, _P, _P, _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_pagerank": ([_P, ctypes.POINTER(PrParams), _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_sssp": ([_P, u32, ctypes.POINTER(SsspParams), _P, ctypes.POINTER(Stats)], ctypes.c_int),
@@ -215,6 +219,14 @@ class Engine:
         od = np.zeros(V, dtype=np.uint32)
         check(lib().gmp_graph_download(self._h, _ptr(off), _ptr(idx), _ptr(w), _ptr(bounds), _ptr(od)))
         return {"off": off.astype(np.int64), "idx": idx, "w": w, "bounds": bounds, "outdeg": od}
+
+    def set_option(self, key, value):
+        check(lib().gmp_set_option(self._h, key, int(value)))
+
+    def get_option(self, key):
+        v = ctypes.c_int64()
+        check(lib().gmp_get_option(self._h, key, ctypes.byref(v)))
+        return v.value
 
     def pick_source(self, seed, salt=3):
         v = ctypes.c_uint32()

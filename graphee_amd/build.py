@@ -53,7 +53,27 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_apps(verbose)
     return LIB
+
+
+APPS = os.path.join(HERE, "apps")
+BIN = os.path.join(HERE, "bin")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_apps(verbose=True):
+    """GraphMP-style apps (host C++ over the C ABI) -> graphee_amd/bin/gmp_<app>."""
+    os.makedirs(BIN, exist_ok=True)
+    for src in sorted(f for f in os.listdir(APPS) if f.endswith(".cpp")):
+        out = os.path.join(BIN, "gmp_" + src[:-4])
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", os.path.join(APPS, src), f"-L{HERE}",
+               "-lgmp", "-Wl,-rpath,$ORIGIN/..", "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"built {out}")
 
 
 if __name__ == "__main__":

@@ -71,6 +71,66 @@ int gmp_destroy(gmp_engine *h) {
   GMP_API_END
 }
 
+int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  auto in = [&](int64_t lo, int64_t hi, const char *name) {
+    if (value < lo || value > hi)
+      fail(GMP_EINVAL, std::string("gmp_set_option: ") + name + " must be in [" + std::to_string(lo) + ", " +
+                           std::to_string(hi) + "]");
+  };
+  switch (key) {
+    case GMP_OPT_REORDER:
+      in(0, 1, "reorder");
+      if ((int)value != e->opt.reorder) {
+        e->opt.reorder = (int)value;
+        e->g.work_ready = false;
+        e->g.wout_ready = false;
+        e->g.plan.built = false;
+      }
+      break;
+    case GMP_OPT_STREAM_CAP:
+      in(64, 4096, "stream_cap");
+      e->opt.stream_cap = (uint32_t)value;
+      break;
+    case GMP_OPT_ROW_CAP:
+      in(1, 4096, "row_cap");
+      e->opt.row_cap = (uint32_t)value;
+      break;
+    case GMP_OPT_CHUNK:
+      in(64, 1 << 20, "chunk");
+      e->opt.chunk = (uint32_t)value;
+      break;
+    case GMP_OPT_WG_PER_CU:
+      in(1, 64, "wg_per_cu");
+      e->opt.wg_per_cu = (uint32_t)value;
+      break;
+    case GMP_OPT_VEC_MIN:
+      in(1, 4096, "vec_min");
+      e->opt.vec_min = (uint32_t)value;
+      break;
+    default:
+      fail(GMP_EINVAL, "gmp_set_option: unknown key " + std::to_string(key));
+  }
+  GMP_API_END
+}
+
+int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  if (!value) fail(GMP_EINVAL, "gmp_get_option: value is null");
+  switch (key) {
+    case GMP_OPT_REORDER: *value = e->opt.reorder; break;
+    case GMP_OPT_STREAM_CAP: *value = e->opt.stream_cap; break;
+    case GMP_OPT_ROW_CAP: *value = e->opt.row_cap; break;
+    case GMP_OPT_CHUNK: *value = e->opt.chunk; break;
+    case GMP_OPT_WG_PER_CU: *value = e->opt.wg_per_cu; break;
+    case GMP_OPT_VEC_MIN: *value = e->opt.vec_min; break;
+    default: fail(GMP_EINVAL, "gmp_get_option: unknown key " + std::to_string(key));
+  }
+  GMP_API_END
+}
+
 int gmp_load_csr(gmp_engine *h, uint32_t V, uint64_t E, const uint64_t *row_off,
                  const uint32_t *col_idx, const uint32_t *weights, uint32_t P,
                  const uint32_t *shard_bounds) {

@@ -76,6 +76,7 @@ struct GatherPlan {
   uint32_t stream_cap = 1024;  // edges per STREAM task (LDS floats per wave)
   uint32_t row_cap = 512;      // rows per STREAM task
   uint32_t chunk = 2048;       // edges per CHUNK task
+  uint32_t vec_min = 64;       // rows with more edges are reduced by a whole wave
   uint32_t ntasks = 0, nhubs = 0, nmulti = 0, nparts = 0;
   DevBuf tasks;   // uint2[ntasks]
   DevBuf hubs;    // HubRow[nhubs]
@@ -83,20 +84,64 @@ struct GatherPlan {
   bool built = false;
 };
 
+// One CSR in device memory; off is uint32[V+1] or uint64[V+1] (Graph::off64).
+struct Csr {
+  DevBuf off, idx, w;
+  bool valid = false;
+};
+
+// Non-owning view handed to the kernel launchers.
+struct CsrView {
+  const void *off = nullptr;
+  const uint32_t *idx = nullptr;
+  const uint32_t *w = nullptr;
+  template <class OffT> const OffT *offs() const { return static_cast<const OffT *>(off); }
+};
+
 struct Graph {
   uint32_t V = 0;
   uint64_t E = 0;
   bool off64 = false;
   bool weighted = false;
-  DevBuf in_off, in_idx, in_w;     // destination rows
-  DevBuf out_off, out_idx, out_w;  // source rows (built on demand)
-  bool has_out = false;
-  DevBuf out_deg;                  // uint32[V]
-  uint32_t P = 1;
-  std::vector<uint32_t> bounds;    // P+1
-  std::vector<uint64_t> h_in_off;  // host copy of the in-CSR offsets (planning)
-  GatherPlan plan;
   bool symmetric = false;
+  // layout as loaded: destination rows, sources in shard-file order
+  Csr in;
+  Csr out;                          // its transpose (built on demand)
+  DevBuf out_deg;                   // uint32[V]
+  uint32_t P = 1;
+  std::vector<uint32_t> bounds;     // P+1
+  std::vector<uint64_t> h_in_off;   // host copy of the in-CSR offsets
+  // work layout: inside every destination interval the vertices are renumbered by
+  // descending out-degree (ties by id), so the most-gathered sources share cache lines.
+  // With reordering off the work layout is the loaded one.
+  bool work_ready = false;
+  bool work_identity = true;
+  Csr win, wout;                    // work in/out CSR (unused when identity)
+  bool wout_ready = false;
+  DevBuf o2n, n2o;                  // uint32[V]: work id of an original id and back
+  DevBuf wdeg;                      // uint32[V]: out-degree by work id
+  std::vector<uint64_t> h_win_off;  // host copy of the work in-CSR offsets (planning)
+  GatherPlan plan;                  // over the work in-CSR
+
+  CsrView view(const Csr &c) const { return CsrView{c.off.p, c.idx.as<uint32_t>(), c.w.as<uint32_t>()}; }
+  CsrView work_in() const { return work_identity ? view(in) : view(win); }
+  // a symmetric unweighted graph is its own transpose
+  CsrView work_out() const {
+    if (symmetric && !weighted) return work_in();
+    return work_identity ? view(out) : view(wout);
+  }
+  const uint32_t *work_deg() const { return work_identity ? out_deg.as<uint32_t>() : wdeg.as<uint32_t>(); }
+  const std::vector<uint64_t> &work_h_off() const { return work_identity ? h_in_off : h_win_off; }
+};
+
+// Tunables (gmp_set_option); the autotuner writes these.
+struct Options {
+  int reorder = 1;             // build the degree-ordered work layout
+  uint32_t stream_cap = 1024;  // gather plan: edges per STREAM task
+  uint32_t row_cap = 512;      // gather plan: rows per STREAM task
+  uint32_t chunk = 2048;       // gather plan: edges per hub CHUNK task
+  uint32_t vec_min = 64;       // gather plan: rows above this many edges go wave-wide
+  uint32_t wg_per_cu = 8;      // gather grid = wg_per_cu * CUs
 };
 
 struct Engine {
@@ -104,6 +149,7 @@ struct Engine {
   int num_cus = 256;
   hipStream_t stream = nullptr;
   Graph g;
+  Options opt;
   bool loaded = false;
   // scratch reused across runs
   DevBuf s_a, s_b, s_c, s_d, s_e, s_f, s_part, s_scal;
@@ -142,7 +188,12 @@ void graph_upload(Engine &e, uint32_t V, uint64_t E, const uint64_t *row_off,
                   const uint32_t *col_idx, const uint32_t *weights, bool force_off64);
 void graph_generate_rmat(Engine &e, const gmp_rmat_params &p);
 void graph_finish(Engine &e, uint32_t P, const uint32_t *bounds);  // out-degree, shards
-void graph_build_out(Engine &e);                                   // transpose
+void graph_build_out(Engine &e);                                   // transpose of `in`
+void ensure_work(Engine &e);                                       // degree-ordered layout
+void ensure_work_out(Engine &e);                                   // its transpose
+// out[v] = x[o2n[v]] for 4-byte values (work ids -> original ids); device pointers.
+void map_to_original(Engine &e, const uint32_t *x_work, uint32_t *out);
+uint32_t work_id(Engine &e, uint32_t original);
 void graph_download(Engine &e, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights,
                     uint32_t *out_degree);
 

@@ -1,11 +1,12 @@
-// Device-resident graph construction: upload, R-MAT generation, CSR assembly, transpose.
+// Device-resident graph construction: upload, R-MAT generation, CSR assembly, transpose,
+// and the degree-ordered work layout.
 //
 // The whole graph lives in HBM as a destination-row CSR (the layout of GraphMP's shards,
 // GraphMP/tools/format_spark.py:137-157) with row offsets in uint32 when E < 2^32 and uint64
-// otherwise, uint32 column ids and optional uint32 weights.  Generation and transposition
-// are sort-based on the device: one 64-bit key per edge (row << shift | column), an LSD radix
-// sort over exactly 2*shift bits, a unique pass, and a single kernel that turns the sorted
-// keys into row offsets.
+// otherwise, uint32 column ids and optional uint32 weights.  Generation, transposition and
+// renumbering are sort-based on the device: one 64-bit key per edge (row << shift | column),
+// an LSD radix sort over exactly 2*shift bits, and one kernel that turns sorted keys into
+// row offsets.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -25,11 +26,14 @@ inline uint32_t grid_for(uint64_t n, uint32_t block = kBlock, uint32_t cap = 1u 
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
 }
 
-inline uint32_t bits_for(uint32_t V) {
+inline uint32_t bits_for(uint64_t n) {
   uint32_t b = 0;
-  while (b < 32 && (1ull << b) < V) b++;
+  while (b < 63 && (1ull << b) < n) b++;
   return std::max<uint32_t>(b, 1);
 }
+
+#define GRID_STRIDE(i, n) \
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
 
 struct RmatConsts {
   uint64_t base;
@@ -54,8 +58,7 @@ __device__ inline uint32_t perm_dev(uint64_t x, const RmatConsts &c) {
 // self-loop, so they sort to the end and the unique pass leaves at most one to drop.
 __global__ void rmat_keys_kernel(RmatConsts c, uint64_t m, uint64_t *keys, int undirected) {
   const uint64_t sentinel = (c.mask << c.scale) | c.mask;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m;
-       i += (uint64_t)gridDim.x * blockDim.x) {
+  GRID_STRIDE(i, m) {
     uint64_t st = c.base ^ i;
     uint64_t src = 0, dst = 0;
     for (uint32_t k = 0; k < c.half; k++) {
@@ -81,59 +84,71 @@ __global__ void rmat_keys_kernel(RmatConsts c, uint64_t m, uint64_t *keys, int u
 
 // off[r] = first j with row(key[j]) >= r, for r in [0, V]; key rows are nondecreasing.
 template <class OffT>
-__global__ void offsets_from_keys_kernel(const uint64_t *keys, uint64_t E, uint32_t shift,
-                                         uint32_t V, OffT *off) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j <= E;
-       j += (uint64_t)gridDim.x * blockDim.x) {
+__global__ void offsets_from_keys_kernel(const uint64_t *keys, uint64_t E, uint32_t shift, uint32_t V, OffT *off) {
+  GRID_STRIDE(j, E + 1) {
     int64_t r = j < E ? (int64_t)(keys[j] >> shift) : (int64_t)V;
     int64_t rp = j > 0 ? (int64_t)(keys[j - 1] >> shift) : -1;
     for (int64_t rr = rp + 1; rr <= r; rr++) off[rr] = (OffT)j;
   }
 }
 
-__global__ void cols_from_keys_kernel(const uint64_t *keys, uint64_t E, uint64_t mask,
-                                      uint32_t *idx) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < E;
-       j += (uint64_t)gridDim.x * blockDim.x)
-    idx[j] = (uint32_t)(keys[j] & mask);
+__global__ void cols_from_keys_kernel(const uint64_t *keys, uint64_t E, uint64_t mask, uint32_t *idx) {
+  GRID_STRIDE(j, E) idx[j] = (uint32_t)(keys[j] & mask);
 }
 
-__global__ void rmat_weights_kernel(const uint64_t *keys, uint64_t E, uint32_t shift,
-                                    uint64_t mask, uint64_t seed2, uint32_t *w) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < E;
-       j += (uint64_t)gridDim.x * blockDim.x) {
+__global__ void rmat_weights_kernel(const uint64_t *keys, uint64_t E, uint32_t shift, uint64_t mask, uint64_t seed2,
+                                    uint32_t *w) {
+  GRID_STRIDE(j, E) {
     uint64_t dst = keys[j] >> shift, src = keys[j] & mask;
     w[j] = 1u + (uint32_t)(mix64(seed2 ^ (src << 32) ^ dst) % 255ull);
   }
 }
 
 __global__ void out_degree_kernel(const uint32_t *idx, uint64_t E, uint32_t *deg) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < E;
-       j += (uint64_t)gridDim.x * blockDim.x)
-    atomicAdd(&deg[idx[j]], 1u);
+  GRID_STRIDE(j, E) atomicAdd(&deg[idx[j]], 1u);
 }
 
-// rowid[off[r]] = r for every non-empty row (unique writer per position); a max-scan
-// then labels every edge with its row.
+// rowid[off[r]] = r for every non-empty row (unique writer per position); a max-scan then
+// labels every edge with its row.
 template <class OffT>
 __global__ void mark_rows_kernel(const OffT *off, uint32_t V, uint32_t *rowid) {
-  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < V;
-       r += (uint64_t)gridDim.x * blockDim.x)
-    if (off[r + 1] > off[r]) rowid[off[r]] = (uint32_t)r;
+  GRID_STRIDE(r, V) if (off[r + 1] > off[r]) rowid[off[r]] = (uint32_t)r;
 }
 
-__global__ void transpose_keys_kernel(const uint32_t *idx, const uint32_t *rowid, uint64_t E,
-                                      uint32_t shift, uint64_t *keys) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < E;
-       j += (uint64_t)gridDim.x * blockDim.x)
-    keys[j] = ((uint64_t)idx[j] << shift) | rowid[j];
+// key = map(col) << shift | map(row); map == nullptr is the identity.
+__global__ void edge_keys_kernel(const uint32_t *idx, const uint32_t *rowid, uint64_t E, uint32_t shift,
+                                 const uint32_t *map, int col_major, uint64_t *keys) {
+  GRID_STRIDE(j, E) {
+    uint64_t r = rowid[j], c = idx[j];
+    if (map) {
+      r = map[r];
+      c = map[c];
+    }
+    keys[j] = col_major ? ((c << shift) | r) : ((r << shift) | c);
+  }
 }
 
-template <class OffT>
-__global__ void widen_offsets_kernel(const OffT *off, uint32_t n, uint64_t *out) {
-  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n;
-       r += (uint64_t)gridDim.x * blockDim.x)
-    out[r] = (uint64_t)off[r];
+// Work-layout order key: shard, then out-degree descending (stable sort keeps id order).
+__global__ void order_keys_kernel(const uint32_t *outdeg, const uint32_t *bounds, uint32_t P, uint32_t V,
+                                  uint64_t *keys, uint32_t *vals) {
+  GRID_STRIDE(v, V) {
+    uint32_t lo = 0, hi = P - 1;  // shard p with bounds[p] <= v < bounds[p+1]
+    while (lo < hi) {
+      uint32_t mid = (lo + hi + 1) >> 1;
+      if (bounds[mid] <= v) lo = mid;
+      else hi = mid - 1;
+    }
+    keys[v] = ((uint64_t)lo << 32) | (uint64_t)(0xFFFFFFFFu - outdeg[v]);
+    vals[v] = (uint32_t)v;
+  }
+}
+
+__global__ void scatter_inverse_kernel(const uint32_t *n2o, uint32_t V, uint32_t *o2n) {
+  GRID_STRIDE(i, V) o2n[n2o[i]] = (uint32_t)i;
+}
+
+__global__ void gather_u32_kernel(const uint32_t *x, const uint32_t *map, uint32_t n, uint32_t *out) {
+  GRID_STRIDE(i, n) out[i] = x[map[i]];
 }
 
 uint32_t thresh(double x) {
@@ -143,48 +158,115 @@ uint32_t thresh(double x) {
   return (uint32_t)t;
 }
 
-// Sorted 64-bit keys (row << shift | col), already unique -> CSR arrays in g (in or out side).
-void csr_from_sorted_keys(Engine &e, const uint64_t *keys, uint64_t E, uint32_t shift, uint32_t V,
-                          bool off64, DevBuf &off, DevBuf &idx) {
-  off.alloc((size_t)(V + 1) * (off64 ? 8 : 4));
-  idx.alloc(std::max<uint64_t>(E, 1) * 4);
+// Sorted unique keys (row << shift | col) -> CSR offsets and column ids.
+void csr_from_sorted_keys(Engine &e, const uint64_t *keys, uint64_t E, uint32_t shift, uint32_t V, bool off64,
+                          Csr &c) {
+  c.off.alloc((size_t)(V + 1) * (off64 ? 8 : 4));
+  c.idx.alloc(std::max<uint64_t>(E, 1) * 4);
   uint32_t g = grid_for(E + 1);
   if (off64)
-    offsets_from_keys_kernel<uint64_t><<<g, kBlock, 0, e.stream>>>(keys, E, shift, V, off.as<uint64_t>());
+    offsets_from_keys_kernel<uint64_t><<<g, kBlock, 0, e.stream>>>(keys, E, shift, V, c.off.as<uint64_t>());
   else
-    offsets_from_keys_kernel<uint32_t><<<g, kBlock, 0, e.stream>>>(keys, E, shift, V, off.as<uint32_t>());
+    offsets_from_keys_kernel<uint32_t><<<g, kBlock, 0, e.stream>>>(keys, E, shift, V, c.off.as<uint32_t>());
   GMP_CHECK_LAUNCH();
   if (E) {
-    cols_from_keys_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(keys, E, (1ull << shift) - 1, idx.as<uint32_t>());
+    cols_from_keys_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(keys, E, (1ull << shift) - 1, c.idx.as<uint32_t>());
     GMP_CHECK_LAUNCH();
+  }
+  c.valid = true;
+}
+
+// Row id of every edge of a CSR (mark + inclusive max-scan).
+void edge_rows(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, DevBuf &rowid, DevBuf &tmp) {
+  rowid.alloc(std::max<uint64_t>(E, 1) * 4);
+  if (!E) return;
+  GMP_HIP(hipMemsetAsync(rowid.p, 0, E * 4, e.stream));
+  if (off64) mark_rows_kernel<uint64_t><<<grid_for(V), kBlock, 0, e.stream>>>(src.offs<uint64_t>(), V, rowid.as<uint32_t>());
+  else mark_rows_kernel<uint32_t><<<grid_for(V), kBlock, 0, e.stream>>>(src.offs<uint32_t>(), V, rowid.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  size_t tb = 0;
+  GMP_HIP(rocprim::inclusive_scan(nullptr, tb, rowid.as<uint32_t>(), rowid.as<uint32_t>(), E,
+                                  rocprim::maximum<uint32_t>(), e.stream));
+  tmp.alloc(tb);
+  GMP_HIP(rocprim::inclusive_scan(tmp.p, tb, rowid.as<uint32_t>(), rowid.as<uint32_t>(), E,
+                                  rocprim::maximum<uint32_t>(), e.stream));
+}
+
+// Re-keys every edge of `src` (optionally renumbering vertices through `map` and/or
+// swapping rows and columns), sorts, and builds `dst` -- weights travel with their edge.
+void rebuild_csr(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, bool weighted,
+                 const uint32_t *map, bool transpose, Csr &dst) {
+  const uint32_t shift = bits_for(V);
+  DevBuf rowid, kA, kB, wA, wB, tmp;
+  edge_rows(e, src, V, E, off64, rowid, tmp);
+  kA.alloc(std::max<uint64_t>(E, 1) * 8);
+  kB.alloc(std::max<uint64_t>(E, 1) * 8);
+  if (E) {
+    edge_keys_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(src.idx, rowid.as<uint32_t>(), E, shift, map,
+                                                          transpose ? 1 : 0, kA.as<uint64_t>());
+    GMP_CHECK_LAUNCH();
+  }
+  rowid.release();
+  rocprim::double_buffer<uint64_t> dk(kA.as<uint64_t>(), kB.as<uint64_t>());
+  if (E) {
+    size_t tb = 0;
+    if (weighted) {
+      wA.alloc(E * 4);
+      wB.alloc(E * 4);
+      GMP_HIP(hipMemcpyAsync(wA.p, src.w, E * 4, hipMemcpyDeviceToDevice, e.stream));
+      rocprim::double_buffer<uint32_t> dv(wA.as<uint32_t>(), wB.as<uint32_t>());
+      GMP_HIP(rocprim::radix_sort_pairs(nullptr, tb, dk, dv, E, 0, 2 * shift, e.stream));
+      tmp.alloc(tb);
+      GMP_HIP(rocprim::radix_sort_pairs(tmp.p, tb, dk, dv, E, 0, 2 * shift, e.stream));
+      if (dv.current() == wA.as<uint32_t>()) std::swap(wA, wB);
+      dst.w = std::move(wB);
+    } else {
+      GMP_HIP(rocprim::radix_sort_keys(nullptr, tb, dk, E, 0, 2 * shift, e.stream));
+      tmp.alloc(tb);
+      GMP_HIP(rocprim::radix_sort_keys(tmp.p, tb, dk, E, 0, 2 * shift, e.stream));
+    }
+  }
+  csr_from_sorted_keys(e, dk.current(), E, shift, V, off64, dst);
+  GMP_HIP(hipStreamSynchronize(e.stream));
+}
+
+void download_offsets(Engine &e, const Csr &c, uint32_t V, bool off64, std::vector<uint64_t> &h) {
+  h.resize((size_t)V + 1);
+  if (off64) {
+    GMP_HIP(hipMemcpy(h.data(), c.off.p, (size_t)(V + 1) * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<uint32_t> o32((size_t)V + 1);
+    GMP_HIP(hipMemcpy(o32.data(), c.off.p, (size_t)(V + 1) * 4, hipMemcpyDeviceToHost));
+    for (size_t r = 0; r <= V; r++) h[r] = o32[r];
   }
 }
 
 }  // namespace
 
-void graph_upload(Engine &e, uint32_t V, uint64_t E, const uint64_t *row_off,
-                  const uint32_t *col_idx, const uint32_t *weights, bool force_off64) {
+void graph_upload(Engine &e, uint32_t V, uint64_t E, const uint64_t *row_off, const uint32_t *col_idx,
+                  const uint32_t *weights, bool force_off64) {
   Graph &g = e.g;
   g = Graph();
   g.V = V;
   g.E = E;
   g.off64 = force_off64 || E > 0xFFFFFFFFull;
   g.weighted = weights != nullptr;
-  g.in_off.alloc((size_t)(V + 1) * (g.off64 ? 8 : 4));
+  g.in.off.alloc((size_t)(V + 1) * (g.off64 ? 8 : 4));
   if (g.off64) {
-    GMP_HIP(hipMemcpyAsync(g.in_off.p, row_off, (size_t)(V + 1) * 8, hipMemcpyHostToDevice, e.stream));
+    GMP_HIP(hipMemcpyAsync(g.in.off.p, row_off, (size_t)(V + 1) * 8, hipMemcpyHostToDevice, e.stream));
   } else {
-    std::vector<uint32_t> o32(V + 1);
+    std::vector<uint32_t> o32((size_t)V + 1);
     for (uint64_t r = 0; r <= V; r++) o32[r] = (uint32_t)row_off[r];
-    GMP_HIP(hipMemcpyAsync(g.in_off.p, o32.data(), (size_t)(V + 1) * 4, hipMemcpyHostToDevice, e.stream));
+    GMP_HIP(hipMemcpyAsync(g.in.off.p, o32.data(), (size_t)(V + 1) * 4, hipMemcpyHostToDevice, e.stream));
     GMP_HIP(hipStreamSynchronize(e.stream));
   }
-  g.in_idx.alloc(std::max<uint64_t>(E, 1) * 4);
-  if (E) GMP_HIP(hipMemcpyAsync(g.in_idx.p, col_idx, E * 4, hipMemcpyHostToDevice, e.stream));
+  g.in.idx.alloc(std::max<uint64_t>(E, 1) * 4);
+  if (E) GMP_HIP(hipMemcpyAsync(g.in.idx.p, col_idx, E * 4, hipMemcpyHostToDevice, e.stream));
   if (weights) {
-    g.in_w.alloc(std::max<uint64_t>(E, 1) * 4);
-    if (E) GMP_HIP(hipMemcpyAsync(g.in_w.p, weights, E * 4, hipMemcpyHostToDevice, e.stream));
+    g.in.w.alloc(std::max<uint64_t>(E, 1) * 4);
+    if (E) GMP_HIP(hipMemcpyAsync(g.in.w.p, weights, E * 4, hipMemcpyHostToDevice, e.stream));
   }
+  g.in.valid = true;
   g.h_in_off.assign(row_off, row_off + V + 1);
   GMP_HIP(hipStreamSynchronize(e.stream));
 }
@@ -223,11 +305,11 @@ void graph_generate_rmat(Engine &e, const gmp_rmat_params &p) {
   uint64_t *uniq = (sorted == kA.as<uint64_t>()) ? kB.as<uint64_t>() : kA.as<uint64_t>();
   cnt.alloc(8);
   size_t tb2 = 0;
-  GMP_HIP(rocprim::unique(nullptr, tb2, sorted, uniq, cnt.as<uint64_t>(), nkeys,
-                          rocprim::equal_to<uint64_t>(), e.stream));
+  GMP_HIP(rocprim::unique(nullptr, tb2, sorted, uniq, cnt.as<uint64_t>(), nkeys, rocprim::equal_to<uint64_t>(),
+                          e.stream));
   tmp.alloc(tb2);
-  GMP_HIP(rocprim::unique(tmp.p, tb2, sorted, uniq, cnt.as<uint64_t>(), nkeys,
-                          rocprim::equal_to<uint64_t>(), e.stream));
+  GMP_HIP(rocprim::unique(tmp.p, tb2, sorted, uniq, cnt.as<uint64_t>(), nkeys, rocprim::equal_to<uint64_t>(),
+                          e.stream));
   uint64_t nuniq = 0, last = 0;
   GMP_HIP(hipMemcpyAsync(&nuniq, cnt.p, 8, hipMemcpyDeviceToHost, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
@@ -240,26 +322,18 @@ void graph_generate_rmat(Engine &e, const gmp_rmat_params &p) {
   g.E = nuniq;
   g.off64 = (p.flags & GMP_GRAPH_OFF64) || nuniq > 0xFFFFFFFFull;
   g.symmetric = p.undirected != 0;
-  csr_from_sorted_keys(e, uniq, nuniq, p.scale, V, g.off64, g.in_off, g.in_idx);
+  csr_from_sorted_keys(e, uniq, nuniq, p.scale, V, g.off64, g.in);
   if (p.weighted) {
     g.weighted = true;
-    g.in_w.alloc(std::max<uint64_t>(nuniq, 1) * 4);
+    g.in.w.alloc(std::max<uint64_t>(nuniq, 1) * 4);
     if (nuniq) {
       rmat_weights_kernel<<<grid_for(nuniq), kBlock, 0, e.stream>>>(uniq, nuniq, p.scale, c.mask, p.seed + 2,
-                                                                   g.in_w.as<uint32_t>());
+                                                                   g.in.w.as<uint32_t>());
       GMP_CHECK_LAUNCH();
     }
   }
   GMP_HIP(hipStreamSynchronize(e.stream));
-  // host copy of the offsets for partitioning and gather planning
-  g.h_in_off.resize((size_t)V + 1);
-  if (g.off64) {
-    GMP_HIP(hipMemcpy(g.h_in_off.data(), g.in_off.p, (size_t)(V + 1) * 8, hipMemcpyDeviceToHost));
-  } else {
-    std::vector<uint32_t> o32((size_t)V + 1);
-    GMP_HIP(hipMemcpy(o32.data(), g.in_off.p, (size_t)(V + 1) * 4, hipMemcpyDeviceToHost));
-    for (size_t r = 0; r <= V; r++) g.h_in_off[r] = o32[r];
-  }
+  download_offsets(e, g.in, V, g.off64, g.h_in_off);
 }
 
 void graph_finish(Engine &e, uint32_t P, const uint32_t *bounds) {
@@ -267,7 +341,7 @@ void graph_finish(Engine &e, uint32_t P, const uint32_t *bounds) {
   g.out_deg.alloc((size_t)std::max<uint32_t>(g.V, 1) * 4);
   GMP_HIP(hipMemsetAsync(g.out_deg.p, 0, (size_t)g.V * 4, e.stream));
   if (g.E) {
-    out_degree_kernel<<<grid_for(g.E), kBlock, 0, e.stream>>>(g.in_idx.as<uint32_t>(), g.E, g.out_deg.as<uint32_t>());
+    out_degree_kernel<<<grid_for(g.E), kBlock, 0, e.stream>>>(g.in.idx.as<uint32_t>(), g.E, g.out_deg.as<uint32_t>());
     GMP_CHECK_LAUNCH();
   }
   if (P == 0) P = 1;
@@ -288,70 +362,94 @@ void graph_finish(Engine &e, uint32_t P, const uint32_t *bounds) {
 
 void graph_build_out(Engine &e) {
   Graph &g = e.g;
-  if (g.has_out) return;
-  if (g.symmetric && !g.weighted) {
-    // a symmetric graph is its own transpose: share the in-CSR
-    g.has_out = true;
-    return;
-  }
-  const uint32_t shift = bits_for(g.V);
-  const uint64_t E = g.E;
-  DevBuf rowid, kA, kB, wB, tmp;
-  rowid.alloc(std::max<uint64_t>(E, 1) * 4);
-  kA.alloc(std::max<uint64_t>(E, 1) * 8);
-  kB.alloc(std::max<uint64_t>(E, 1) * 8);
-  if (E) {
-    GMP_HIP(hipMemsetAsync(rowid.p, 0, E * 4, e.stream));
-    if (g.off64)
-      mark_rows_kernel<uint64_t><<<grid_for(g.V), kBlock, 0, e.stream>>>(g.in_off.as<uint64_t>(), g.V, rowid.as<uint32_t>());
-    else
-      mark_rows_kernel<uint32_t><<<grid_for(g.V), kBlock, 0, e.stream>>>(g.in_off.as<uint32_t>(), g.V, rowid.as<uint32_t>());
-    GMP_CHECK_LAUNCH();
-    size_t tb = 0;
-    GMP_HIP(rocprim::inclusive_scan(nullptr, tb, rowid.as<uint32_t>(), rowid.as<uint32_t>(), E,
-                                    rocprim::maximum<uint32_t>(), e.stream));
-    tmp.alloc(tb);
-    GMP_HIP(rocprim::inclusive_scan(tmp.p, tb, rowid.as<uint32_t>(), rowid.as<uint32_t>(), E,
-                                    rocprim::maximum<uint32_t>(), e.stream));
-    transpose_keys_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(g.in_idx.as<uint32_t>(), rowid.as<uint32_t>(), E,
-                                                               shift, kA.as<uint64_t>());
-    GMP_CHECK_LAUNCH();
-  }
-  rocprim::double_buffer<uint64_t> dk(kA.as<uint64_t>(), kB.as<uint64_t>());
-  if (E) {
-    size_t tb = 0;
-    if (g.weighted) {
-      wB.alloc(E * 4);
-      // the in-CSR weights are consumed as the value buffer's first half
-      DevBuf wA;
-      wA.alloc(E * 4);
-      GMP_HIP(hipMemcpyAsync(wA.p, g.in_w.p, E * 4, hipMemcpyDeviceToDevice, e.stream));
-      rocprim::double_buffer<uint32_t> dv(wA.as<uint32_t>(), wB.as<uint32_t>());
-      GMP_HIP(rocprim::radix_sort_pairs(nullptr, tb, dk, dv, E, 0, 2 * shift, e.stream));
-      tmp.alloc(tb);
-      GMP_HIP(rocprim::radix_sort_pairs(tmp.p, tb, dk, dv, E, 0, 2 * shift, e.stream));
-      if (dv.current() == wA.as<uint32_t>()) std::swap(wA, wB);
-      g.out_w = std::move(wB);
-      GMP_HIP(hipStreamSynchronize(e.stream));
-    } else {
-      GMP_HIP(rocprim::radix_sort_keys(nullptr, tb, dk, E, 0, 2 * shift, e.stream));
-      tmp.alloc(tb);
-      GMP_HIP(rocprim::radix_sort_keys(tmp.p, tb, dk, E, 0, 2 * shift, e.stream));
-    }
-  }
-  csr_from_sorted_keys(e, dk.current(), E, shift, g.V, g.off64, g.out_off, g.out_idx);
-  GMP_HIP(hipStreamSynchronize(e.stream));
-  g.has_out = true;
+  if (g.out.valid) return;
+  rebuild_csr(e, g.view(g.in), g.V, g.E, g.off64, g.weighted, nullptr, true, g.out);
 }
 
-void graph_download(Engine &e, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights,
-                    uint32_t *out_degree) {
+void ensure_work(Engine &e) {
+  Graph &g = e.g;
+  if (g.work_ready) return;
+  g.work_identity = !e.opt.reorder;
+  g.wout_ready = false;
+  if (g.work_identity) {
+    g.work_ready = true;
+    return;
+  }
+  const uint32_t V = g.V;
+  // order: per destination interval, out-degree descending, ties by id
+  DevBuf keys, vals, kB, vB, dbounds, tmp;
+  keys.alloc((size_t)V * 8);
+  kB.alloc((size_t)V * 8);
+  vals.alloc((size_t)V * 4);
+  vB.alloc((size_t)V * 4);
+  dbounds.alloc((g.bounds.size()) * 4);
+  GMP_HIP(hipMemcpyAsync(dbounds.p, g.bounds.data(), g.bounds.size() * 4, hipMemcpyHostToDevice, e.stream));
+  order_keys_kernel<<<grid_for(V), kBlock, 0, e.stream>>>(g.out_deg.as<uint32_t>(), dbounds.as<uint32_t>(), g.P, V,
+                                                         keys.as<uint64_t>(), vals.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  rocprim::double_buffer<uint64_t> dk(keys.as<uint64_t>(), kB.as<uint64_t>());
+  rocprim::double_buffer<uint32_t> dv(vals.as<uint32_t>(), vB.as<uint32_t>());
+  const uint32_t endbit = 32 + bits_for((uint64_t)g.P + 1);
+  size_t tb = 0;
+  GMP_HIP(rocprim::radix_sort_pairs(nullptr, tb, dk, dv, (size_t)V, 0, endbit, e.stream));
+  tmp.alloc(tb);
+  GMP_HIP(rocprim::radix_sort_pairs(tmp.p, tb, dk, dv, (size_t)V, 0, endbit, e.stream));
+  g.n2o.alloc((size_t)V * 4);
+  g.o2n.alloc((size_t)V * 4);
+  GMP_HIP(hipMemcpyAsync(g.n2o.p, dv.current(), (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+  scatter_inverse_kernel<<<grid_for(V), kBlock, 0, e.stream>>>(g.n2o.as<uint32_t>(), V, g.o2n.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  g.wdeg.alloc((size_t)V * 4);
+  gather_u32_kernel<<<grid_for(V), kBlock, 0, e.stream>>>(g.out_deg.as<uint32_t>(), g.n2o.as<uint32_t>(), V,
+                                                         g.wdeg.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  keys.release();
+  kB.release();
+  vals.release();
+  vB.release();
+  rebuild_csr(e, g.view(g.in), V, g.E, g.off64, g.weighted, g.o2n.as<uint32_t>(), false, g.win);
+  download_offsets(e, g.win, V, g.off64, g.h_win_off);
+  g.plan.built = false;
+  g.work_ready = true;
+}
+
+void ensure_work_out(Engine &e) {
+  Graph &g = e.g;
+  ensure_work(e);
+  if (g.wout_ready) return;
+  if (g.work_identity) {
+    if (!(g.symmetric && !g.weighted)) graph_build_out(e);
+  } else if (!(g.symmetric && !g.weighted)) {
+    rebuild_csr(e, g.view(g.win), g.V, g.E, g.off64, g.weighted, nullptr, true, g.wout);
+  }
+  g.wout_ready = true;
+}
+
+void map_to_original(Engine &e, const uint32_t *x_work, uint32_t *out) {
+  Graph &g = e.g;
+  if (g.work_identity) {
+    GMP_HIP(hipMemcpyAsync(out, x_work, (size_t)g.V * 4, hipMemcpyDeviceToDevice, e.stream));
+    return;
+  }
+  gather_u32_kernel<<<grid_for(g.V), kBlock, 0, e.stream>>>(x_work, g.o2n.as<uint32_t>(), g.V, out);
+  GMP_CHECK_LAUNCH();
+}
+
+uint32_t work_id(Engine &e, uint32_t original) {
+  Graph &g = e.g;
+  if (g.work_identity) return original;
+  uint32_t w = 0;
+  GMP_HIP(hipMemcpy(&w, g.o2n.as<uint32_t>() + original, 4, hipMemcpyDeviceToHost));
+  return w;
+}
+
+void graph_download(Engine &e, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights, uint32_t *out_degree) {
   Graph &g = e.g;
   if (row_off) std::memcpy(row_off, g.h_in_off.data(), (size_t)(g.V + 1) * 8);
-  if (col_idx && g.E) GMP_HIP(hipMemcpy(col_idx, g.in_idx.p, g.E * 4, hipMemcpyDeviceToHost));
+  if (col_idx && g.E) GMP_HIP(hipMemcpy(col_idx, g.in.idx.p, g.E * 4, hipMemcpyDeviceToHost));
   if (weights) {
     if (!g.weighted) fail(GMP_EUNSUPPORTED, "graph has no weights");
-    if (g.E) GMP_HIP(hipMemcpy(weights, g.in_w.p, g.E * 4, hipMemcpyDeviceToHost));
+    if (g.E) GMP_HIP(hipMemcpy(weights, g.in.w.p, g.E * 4, hipMemcpyDeviceToHost));
   }
   if (out_degree && g.V) GMP_HIP(hipMemcpy(out_degree, g.out_deg.p, (size_t)g.V * 4, hipMemcpyDeviceToHost));
 }

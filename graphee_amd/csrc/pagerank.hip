@@ -71,11 +71,12 @@ __device__ inline void pr_apply(const PrArgs<OffT> &a, uint32_t r, float s, floa
                                 Acc &acc) {
   float v = fmaf(a.d, s, base);
   if (r == a.seed) v += extra;
-  a.rank_out[r] = v;
-  const uint32_t od = a.outdeg[r];
+  const uint32_t od = __builtin_nontemporal_load(&a.outdeg[r]);
+  const float old = __builtin_nontemporal_load(&a.rank_in[r]);
+  __builtin_nontemporal_store(v, &a.rank_out[r]);
   a.contrib_out[r] = od ? v / (float)od : 0.0f;
   if (!od) acc.dang += (double)v;
-  acc.resid += fabs((double)v - (double)a.rank_in[r]);
+  acc.resid += fabs((double)v - (double)old);
 }
 
 // Deterministic block reduction of the two FP64 accumulators into partials[blockIdx.x].
@@ -113,7 +114,20 @@ __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
       // STREAM: stage the gathered contributions of rows [x, x+y) in LDS, one row per lane.
       const uint64_t e0 = a.off[x];
       const uint32_t n = (uint32_t)((uint64_t)a.off[x + y] - e0);
-      for (uint32_t j = lane; j < n; j += kWave) buf[j] = a.contrib_in[a.idx[e0 + j]];
+      // four independent id loads, then four gathers per lane and round (memory-level parallelism)
+      uint32_t j = lane;
+      for (; j + 3 * kWave < n; j += 4 * kWave) {
+        const uint32_t i0 = __builtin_nontemporal_load(&a.idx[e0 + j]);
+        const uint32_t i1 = __builtin_nontemporal_load(&a.idx[e0 + j + kWave]);
+        const uint32_t i2 = __builtin_nontemporal_load(&a.idx[e0 + j + 2 * kWave]);
+        const uint32_t i3 = __builtin_nontemporal_load(&a.idx[e0 + j + 3 * kWave]);
+        const float c0 = a.contrib_in[i0], c1 = a.contrib_in[i1], c2 = a.contrib_in[i2], c3 = a.contrib_in[i3];
+        buf[j] = c0;
+        buf[j + kWave] = c1;
+        buf[j + 2 * kWave] = c2;
+        buf[j + 3 * kWave] = c3;
+      }
+      for (; j < n; j += kWave) buf[j] = a.contrib_in[__builtin_nontemporal_load(&a.idx[e0 + j])];
       wave_sync();
       for (uint32_t i = lane; i < y; i += kWave) {
         const uint32_t r = x + i;
@@ -135,10 +149,29 @@ __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
       const uint64_t e0 = rb + (uint64_t)k * a.chunk;
       const uint64_t e1 = min(e0 + a.chunk, re);
       float s = 0, c = 0;
-      if (KAHAN) {
-        for (uint64_t j = e0 + lane; j < e1; j += kWave) kahan_add(s, c, a.contrib_in[a.idx[j]]);
-      } else {
-        for (uint64_t j = e0 + lane; j < e1; j += kWave) s += a.contrib_in[a.idx[j]];
+      uint64_t j = e0 + lane;
+      for (; j + 3 * kWave < e1; j += 4 * kWave) {
+        const uint32_t i0 = __builtin_nontemporal_load(&a.idx[j]);
+        const uint32_t i1 = __builtin_nontemporal_load(&a.idx[j + kWave]);
+        const uint32_t i2 = __builtin_nontemporal_load(&a.idx[j + 2 * kWave]);
+        const uint32_t i3 = __builtin_nontemporal_load(&a.idx[j + 3 * kWave]);
+        const float c0 = a.contrib_in[i0], c1 = a.contrib_in[i1], c2 = a.contrib_in[i2], c3 = a.contrib_in[i3];
+        if (KAHAN) {
+          kahan_add(s, c, c0);
+          kahan_add(s, c, c1);
+          kahan_add(s, c, c2);
+          kahan_add(s, c, c3);
+        } else {
+          s += c0;
+          s += c1;
+          s += c2;
+          s += c3;
+        }
+      }
+      for (; j < e1; j += kWave) {
+        const float x = a.contrib_in[__builtin_nontemporal_load(&a.idx[j])];
+        if (KAHAN) kahan_add(s, c, x);
+        else s += x;
       }
       float hi = s, lo = -c;
       wave_reduce_dd(hi, lo);
@@ -287,9 +320,9 @@ template <class OffT>
 void launch_compat(Engine &e, int app, uint32_t grid, double invN, const void *msg, void *msgnew,
                    void *value, const int32_t *vout, unsigned long long *changed) {
   Graph &g = e.g;
-  const OffT *off = g.in_off.as<OffT>();
-  const uint32_t *idx = g.in_idx.as<uint32_t>();
-  const uint32_t *w = g.in_w.as<uint32_t>();
+  const OffT *off = g.in.off.as<OffT>();
+  const uint32_t *idx = g.in.idx.as<uint32_t>();
+  const uint32_t *w = g.in.w.as<uint32_t>();
   switch (app) {
     case GMP_APP_PAGERANK:
       compat_step_kernel<OffT, GMP_APP_PAGERANK><<<grid, kBlock, 0, e.stream>>>(off, idx, w, vout, g.V, invN, msg,
@@ -318,8 +351,8 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   GatherPlan &pl = g.plan;
   const uint32_t V = g.V;
   const bool personalised = p.seed_vertex >= 0;
-  const uint32_t seed = personalised ? (uint32_t)p.seed_vertex : 0xFFFFFFFFu;
-  const uint32_t G = std::max(1, e.num_cus * 8);
+  const uint32_t seed = personalised ? work_id(e, (uint32_t)p.seed_vertex) : 0xFFFFFFFFu;
+  const uint32_t G = std::max<uint32_t>(1, (uint32_t)e.num_cus * e.opt.wg_per_cu);
   const uint32_t Gh = pl.nmulti ? std::min<uint32_t>((pl.nmulti + kWavesPerBlock - 1) / kWavesPerBlock, G) : 0;
   const int max_iter = std::max(0, p.max_iter);
 
@@ -348,9 +381,10 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   } evg{ev};
 
   PrArgs<OffT> a{};
-  a.off = g.in_off.as<OffT>();
-  a.idx = g.in_idx.as<uint32_t>();
-  a.outdeg = g.out_deg.as<uint32_t>();
+  const CsrView wv = g.work_in();
+  a.off = wv.offs<OffT>();
+  a.idx = wv.idx;
+  a.outdeg = g.work_deg();
   a.tasks = pl.tasks.as<uint32_t>();
   a.ntasks = pl.ntasks;
   a.hubs = pl.hubs.as<HubRow>();
@@ -422,7 +456,12 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     resid = h[0];
     dang = h[1];
   }
-  if (rank_out) GMP_HIP(hipMemcpy(rank_out, rank[cur], (size_t)V * 4, hipMemcpyDeviceToHost));
+  if (rank_out) {
+    e.s_f.alloc((size_t)V * 4);
+    map_to_original(e, reinterpret_cast<const uint32_t *>(rank[cur]), e.s_f.as<uint32_t>());
+    GMP_HIP(hipMemcpyAsync(rank_out, e.s_f.p, (size_t)V * 4, hipMemcpyDeviceToHost, e.stream));
+    GMP_HIP(hipStreamSynchronize(e.stream));
+  }
   if (st) {
     st->iterations = it;
     st->launches = it;

@@ -5,7 +5,7 @@
 //   * consecutive low-degree rows are packed into STREAM tasks holding at most stream_cap
 //     edges and row_cap rows: the wave streams their column ids coalesced, stages the
 //     gathered values in LDS and reduces one row per lane;
-//   * a row with more than stream_cap edges becomes a hub split into CHUNK tasks of
+//   * a row with more than vec_min edges becomes a hub split into CHUNK tasks of
 //     `chunk` edges, each reduced by the whole wave with cross-lane reductions; rows of
 //     one chunk are finished by that wave, longer rows leave per-chunk partials that a
 //     second kernel combines in chunk order.
@@ -24,6 +24,8 @@ void build_gather_plan_host(const uint64_t *off, uint32_t V, GatherPlan &plan,
   hubs.clear();
   multi.clear();
   const uint64_t cap = plan.stream_cap, rcap = plan.row_cap, chunk = plan.chunk;
+  // rows above vec_min edges are reduced by a whole wave; below, by one lane from LDS
+  const uint64_t vmin = std::min<uint64_t>(plan.vec_min, cap);
   uint32_t parts = 0;
   uint32_t r0 = 0, nr = 0;
   uint64_t acc = 0;
@@ -37,7 +39,7 @@ void build_gather_plan_host(const uint64_t *off, uint32_t V, GatherPlan &plan,
   };
   for (uint32_t r = 0; r < V; r++) {
     uint64_t deg = off[r + 1] - off[r];
-    if (deg > cap) {
+    if (deg > vmin) {
       close_stream();
       uint64_t nch = (deg + chunk - 1) / chunk;
       if (nch > 0x7fffffffull) fail(GMP_EUNSUPPORTED, "row too long for the gather plan");
@@ -71,12 +73,19 @@ void build_gather_plan_host(const uint64_t *off, uint32_t V, GatherPlan &plan,
 }
 
 void ensure_plan(Engine &e) {
+  ensure_work(e);
   Graph &g = e.g;
   GatherPlan &pl = g.plan;
-  if (pl.built) return;
+  if (pl.built && pl.stream_cap == e.opt.stream_cap && pl.row_cap == e.opt.row_cap && pl.chunk == e.opt.chunk &&
+      pl.vec_min == e.opt.vec_min)
+    return;
+  pl.stream_cap = e.opt.stream_cap;
+  pl.row_cap = e.opt.row_cap;
+  pl.chunk = e.opt.chunk;
+  pl.vec_min = e.opt.vec_min;
   std::vector<uint32_t> tasks, multi;
   std::vector<HubRow> hubs;
-  build_gather_plan_host(g.h_in_off.data(), g.V, pl, tasks, hubs, multi);
+  build_gather_plan_host(g.work_h_off().data(), g.V, pl, tasks, hubs, multi);
   pl.tasks.alloc(std::max<size_t>(tasks.size(), 2) * 4);
   pl.hubs.alloc(std::max<size_t>(hubs.size(), 1) * sizeof(HubRow));
   pl.multi.alloc(std::max<size_t>(multi.size(), 1) * 4);

@@ -172,8 +172,20 @@ __global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
-__global__ void iota_u32_kernel(uint32_t *p, uint32_t n) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+// CC labels start as the original vertex ids (so the minimum is taken over original ids
+// whatever the work numbering).
+__global__ void init_labels_kernel(uint32_t *p, const uint32_t *n2o, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = n2o ? n2o[i] : i;
+}
+
+// Maps a work-id result back to original ids and copies it to the host.
+void finish_output(Engine &e, const uint32_t *x_work, uint32_t *host_out) {
+  if (!host_out) return;
+  e.s_f.alloc((size_t)e.g.V * 4);
+  map_to_original(e, x_work, e.s_f.as<uint32_t>());
+  GMP_HIP(hipMemcpyAsync(host_out, e.s_f.p, (size_t)e.g.V * 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
 }
 
 struct TraverseBufs {
@@ -231,16 +243,18 @@ void read_counters(Engine &e, Counters *dev, Counters &h) {
 template <class OffT>
 void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist_out, gmp_stats *st) {
   Graph &g = e.g;
-  graph_build_out(e);
+  ensure_work_out(e);
   const bool unit = p.unit_weights || !g.weighted;
   const int mode = unit ? kModeUnit : kModeWeighted;
   const bool delta_mode = p.method == GMP_SSSP_DELTA || p.method == GMP_SSSP_LEARNED;
   const uint32_t delta = p.delta ? p.delta : 64u;
   const uint32_t V = g.V;
+  src = work_id(e, src);
   TraverseBufs b = alloc_traverse(e, delta_mode);
-  const OffT *ooff = g.has_out && g.out_off.p ? g.out_off.as<OffT>() : g.in_off.as<OffT>();
-  const uint32_t *oidx = g.out_idx.p ? g.out_idx.as<uint32_t>() : g.in_idx.as<uint32_t>();
-  const uint32_t *ow = g.out_w.p ? g.out_w.as<uint32_t>() : g.in_w.as<uint32_t>();
+  const CsrView ov = g.work_out();
+  const OffT *ooff = ov.offs<OffT>();
+  const uint32_t *oidx = ov.idx;
+  const uint32_t *ow = ov.w;
 
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
@@ -309,7 +323,7 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist
   GMP_HIP(hipEventRecord(tot.b, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
   read_counters(e, b.cnt, h);
-  if (dist_out) GMP_HIP(hipMemcpy(dist_out, b.dist, (size_t)V * 4, hipMemcpyDeviceToHost));
+  finish_output(e, b.dist, dist_out);
   if (st) {
     std::memset(st, 0, sizeof *st);
     st->iterations = iters;
@@ -326,14 +340,16 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist
 template <class OffT>
 void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
   Graph &g = e.g;
-  graph_build_out(e);
+  ensure_work_out(e);
   const uint32_t V = g.V;
   TraverseBufs b = alloc_traverse(e, false);
-  const OffT *ooff = g.out_off.p ? g.out_off.as<OffT>() : g.in_off.as<OffT>();
-  const uint32_t *oidx = g.out_idx.p ? g.out_idx.as<uint32_t>() : g.in_idx.as<uint32_t>();
+  const CsrView ov = g.work_out();
+  const OffT *ooff = ov.offs<OffT>();
+  const uint32_t *oidx = ov.idx;
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
-  iota_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.dist, V);
+  init_labels_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.dist, g.work_identity ? nullptr : g.n2o.as<uint32_t>(),
+                                                              V);
   fill_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.stamp, V, 0u);
   GMP_CHECK_LAUNCH();
   GMP_HIP(hipMemsetAsync(b.cnt, 0, sizeof(Counters), e.stream));
@@ -364,7 +380,7 @@ void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
   }
   GMP_HIP(hipEventRecord(tot.b, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
-  if (label_out) GMP_HIP(hipMemcpy(label_out, b.dist, (size_t)V * 4, hipMemcpyDeviceToHost));
+  finish_output(e, b.dist, label_out);
   if (st) {
     std::memset(st, 0, sizeof *st);
     st->iterations = iters;

@@ -101,6 +101,18 @@ int gmp_abi_version(void);
 int gmp_create(int32_t device, gmp_engine **out);
 int gmp_destroy(gmp_engine *h);
 
+/* ---- tunables -----------------------------------------------------------------------
+ * GraphMP fixes its knobs at compile time (GraphMP/system/Global.h:45-58); the engine takes
+ * them at run time.  Changing GMP_OPT_REORDER drops the work layout (rebuilt on next use). */
+#define GMP_OPT_REORDER 1    /* 1: degree-ordered work layout (default), 0: as loaded */
+#define GMP_OPT_STREAM_CAP 2 /* gather plan: edges per low-degree task, 64..4096 */
+#define GMP_OPT_ROW_CAP 3    /* gather plan: rows per low-degree task, 1..4096 */
+#define GMP_OPT_CHUNK 4      /* gather plan: edges per hub chunk, 64..2^20 */
+#define GMP_OPT_WG_PER_CU 5  /* gather grid: workgroups per compute unit, 1..64 */
+#define GMP_OPT_VEC_MIN 6    /* gather plan: rows above this many edges are reduced wave-wide */
+int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
+int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
+
 /* ---- graph loading -----------------------------------------------------------------
  * In-CSR = destination rows with ascending, deduplicated source ids, the layout of
  * GraphMP's shards (GraphMP/tools/format_spark.py:137-157, GraphMP/system/Comp.h:21-50).

@@ -181,6 +181,63 @@ def test_off64_offsets_path(engine):
     np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, ow, src))
 
 
+# ---- layouts and tunables change nothing but speed -------------------------------------
+
+@pytest.mark.parametrize("reorder", [0, 1])
+def test_results_independent_of_layout(engine, reorder):
+    engine.set_option(gmp.OPT_REORDER, reorder)
+    try:
+        g = gen(engine, 12, weighted=True, num_shards=8)
+        V = g["outdeg"].size
+        say(f"layout reorder={reorder}")
+        r, _ = engine.pagerank(max_iter=20)
+        x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
+        assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+        seed = int(np.argmax(g["outdeg"]))
+        r, _ = engine.pagerank(max_iter=20, seed_vertex=seed)
+        x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20, seed=seed)
+        assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+        src = engine.pick_source(77)
+        ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+        want = oracle.dijkstra(ooff, oidx, ow, src)
+        for method in (gmp.SSSP_BF, gmp.SSSP_DELTA):
+            d, _ = engine.sssp(src, method=method, delta=64)
+            np.testing.assert_array_equal(d, want)
+        h = gen(engine, 12, undirected=True, num_shards=8)
+        lab, _ = engine.cc()
+        np.testing.assert_array_equal(lab, oracle.cc_unionfind(h["off"], h["idx"]))
+        # the compat apps always run on the loaded layout (reference summation order)
+        msg0 = np.arange(V, dtype=np.int32)
+        msg, _, _ = engine.run_app(gmp.APP_CC, msg0, 1000)
+        want, _, _, _ = oracle.app_sync(oracle.APP_CC, h["off"], h["idx"], msg0, 1000)
+        np.testing.assert_array_equal(msg, want)
+    finally:
+        engine.set_option(gmp.OPT_REORDER, 1)
+
+
+@pytest.mark.parametrize("cap,rows,chunk,wg", [(64, 1, 64, 1), (4096, 4096, 1 << 20, 64), (256, 16, 512, 3)])
+def test_plan_tunables_keep_results(engine, cap, rows, chunk, wg):
+    g = gen(engine, 13)
+    x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=10)
+    for k, v in ((gmp.OPT_STREAM_CAP, cap), (gmp.OPT_ROW_CAP, rows), (gmp.OPT_CHUNK, chunk), (gmp.OPT_WG_PER_CU, wg)):
+        engine.set_option(k, v)
+    try:
+        say(f"plan cap={cap} rows={rows} chunk={chunk} wg={wg}")
+        r, _ = engine.pagerank(max_iter=10)
+        assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+    finally:
+        for k, v in ((gmp.OPT_STREAM_CAP, 1024), (gmp.OPT_ROW_CAP, 512), (gmp.OPT_CHUNK, 2048), (gmp.OPT_WG_PER_CU, 8)):
+            engine.set_option(k, v)
+
+
+def test_option_range_checks(engine):
+    for k, v in ((gmp.OPT_STREAM_CAP, 8192), (gmp.OPT_REORDER, 2), (gmp.OPT_WG_PER_CU, 0), (99, 1)):
+        with pytest.raises(gmp.GmpError) as ei:
+            engine.set_option(k, v)
+        assert ei.value.code == gmp.EINVAL
+    assert engine.get_option(gmp.OPT_CHUNK) == 2048
+
+
 # ---- shard files and edge cases --------------------------------------------------------
 
 def test_load_shards_roundtrip(engine, tmp_path):
