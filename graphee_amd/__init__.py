@@ -24,7 +24,8 @@ SYMBOLS = (
     "gmp_last_error", "gmp_abi_version", "gmp_create", "gmp_destroy", "gmp_load_csr",
     "gmp_load_shards", "gmp_generate_rmat", "gmp_graph_info", "gmp_graph_download",
     "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_sssp", "gmp_cc",
-    "gmp_set_option", "gmp_get_option",
+    "gmp_set_option", "gmp_get_option", "gmp_batch_generate", "gmp_batch_info", "gmp_batch_download",
+    "gmp_batch_sssp", "gmp_batch_ppr",
     "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards",
 )
 
@@ -97,6 +98,11 @@ def lib():
         "gmp_graph_download": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
         "gmp_pick_source": ([_P, u64, u32, _P], ctypes.c_int),
         "gmp_set_option": ([_P, i32, i64], ctypes.c_int),
+        "gmp_batch_generate": ([_P, u32, _P], ctypes.c_int),
+        "gmp_batch_info": ([_P, u32, _P, _P], ctypes.c_int),
+        "gmp_batch_download": ([_P, u32, _P, _P, _P], ctypes.c_int),
+        "gmp_batch_sssp": ([_P, u32, _P, _P, _P, ctypes.POINTER(Stats)], ctypes.c_int),
+        "gmp_batch_ppr": ([_P, u32, _P, _P, ctypes.c_float, i32, _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_get_option": ([_P, i32, _P], ctypes.c_int),
         "gmp_run_app": ([_P, i32, i32, _P, _P, _P, _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_pagerank": ([_P, ctypes.POINTER(PrParams), _P, ctypes.POINTER(Stats)], ctypes.c_int),
@@ -232,6 +238,54 @@ class Engine:
         v = ctypes.c_uint32()
         check(lib().gmp_pick_source(self._h, seed, salt, ctypes.byref(v)))
         return v.value
+
+    # batched multi-instance runs
+    def batch_generate(self, specs):
+        """specs: list of dicts with scale, edge_factor, seed (weighted graphs)."""
+        arr = (RmatParams * len(specs))()
+        for i, s in enumerate(specs):
+            arr[i] = RmatParams(s["scale"], s["edge_factor"], s["seed"], s.get("a", 0.57), s.get("b", 0.19),
+                                s.get("c", 0.19), int(s.get("undirected", False)), int(s.get("weighted", True)),
+                                1, 0)
+        check(lib().gmp_batch_generate(self._h, len(specs), ctypes.cast(arr, ctypes.c_void_p)))
+
+    def batch_info(self, g):
+        V, E = ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().gmp_batch_info(self._h, g, ctypes.byref(V), ctypes.byref(E)))
+        return V.value, E.value
+
+    def batch_download(self, g):
+        V, E = self.batch_info(g)
+        off = np.zeros(V + 1, dtype=np.uint64)
+        idx = np.zeros(E, dtype=np.uint32)
+        w = np.zeros(E, dtype=np.uint32)
+        check(lib().gmp_batch_download(self._h, g, _ptr(off), _ptr(idx), _ptr(w)))
+        return off.astype(np.int64), idx, w
+
+    def _split(self, graphs, flat, dtype):
+        out, pos = [], 0
+        for g in graphs:
+            V = self.batch_info(int(g))[0]
+            out.append(flat[pos:pos + V])
+            pos += V
+        return out
+
+    def batch_sssp(self, graphs, sources):
+        graphs, sources = _u32(graphs), _u32(sources)
+        total = sum(self.batch_info(int(g))[0] for g in graphs)
+        out = np.zeros(total, dtype=np.uint32)
+        st = Stats()
+        check(lib().gmp_batch_sssp(self._h, graphs.size, _ptr(graphs), _ptr(sources), _ptr(out), ctypes.byref(st)))
+        return self._split(graphs, out, np.uint32), st
+
+    def batch_ppr(self, graphs, seeds, damping=0.85, iters=30):
+        graphs, seeds = _u32(graphs), _u32(seeds)
+        total = sum(self.batch_info(int(g))[0] for g in graphs)
+        out = np.zeros(total, dtype=np.float32)
+        st = Stats()
+        check(lib().gmp_batch_ppr(self._h, graphs.size, _ptr(graphs), _ptr(seeds), damping, iters, _ptr(out),
+                                  ctypes.byref(st)))
+        return self._split(graphs, out, np.float32), st
 
     # apps
     def run_app(self, app, msg_init, max_iter, vertex_out=None):

@@ -279,6 +279,66 @@ int gmp_sssp(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, uint32_t 
   GMP_API_END
 }
 
+int gmp_batch_generate(gmp_engine *h, uint32_t ngraphs, const gmp_rmat_params *params) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  if (!params) fail(GMP_EINVAL, "gmp_batch_generate: params are null");
+  GMP_HIP(hipSetDevice(e->device));
+  gmp::batch_generate(*e, ngraphs, params);
+  GMP_API_END
+}
+
+int gmp_batch_info(gmp_engine *h, uint32_t graph, uint32_t *V, uint64_t *E) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  const gmp::BatchGraphs &b = e->batch;
+  if (!b.ready) fail(GMP_ESTATE, "no batch loaded");
+  if (graph + 1 >= b.vbase.size()) fail(GMP_EINVAL, "gmp_batch_info: graph out of range");
+  if (V) *V = b.vbase[graph + 1] - b.vbase[graph];
+  if (E) *E = b.ebase[graph + 1] - b.ebase[graph];
+  GMP_API_END
+}
+
+int gmp_batch_download(gmp_engine *h, uint32_t graph, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  GMP_HIP(hipSetDevice(e->device));
+  const gmp::BatchGraphs &b = e->batch;
+  if (!b.ready) fail(GMP_ESTATE, "no batch loaded");
+  if (graph + 1 >= b.vbase.size()) fail(GMP_EINVAL, "gmp_batch_download: graph out of range");
+  const uint32_t v0 = b.vbase[graph], V = b.vbase[graph + 1] - v0;
+  const uint64_t e0 = b.ebase[graph], E = b.ebase[graph + 1] - e0;
+  if (row_off)
+    for (uint32_t r = 0; r <= V; r++) row_off[r] = b.h_off[v0 + r] - e0;
+  if (col_idx && E) {
+    GMP_HIP(hipMemcpy(col_idx, b.in.idx.as<uint32_t>() + e0, E * 4, hipMemcpyDeviceToHost));
+    for (uint64_t k = 0; k < E; k++) col_idx[k] -= v0;
+  }
+  if (weights) {
+    if (!b.weighted) fail(GMP_EUNSUPPORTED, "batch graphs have no weights");
+    if (E) GMP_HIP(hipMemcpy(weights, b.in.w.as<uint32_t>() + e0, E * 4, hipMemcpyDeviceToHost));
+  }
+  GMP_API_END
+}
+
+int gmp_batch_sssp(gmp_engine *h, uint32_t n, const uint32_t *graph, const uint32_t *source, uint32_t *dist_out,
+                   gmp_stats *st) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  GMP_HIP(hipSetDevice(e->device));
+  gmp::batch_sssp(*e, n, graph, source, dist_out, st);
+  GMP_API_END
+}
+
+int gmp_batch_ppr(gmp_engine *h, uint32_t n, const uint32_t *graph, const uint32_t *seed, float damping,
+                  int32_t iters, float *rank_out, gmp_stats *st) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  GMP_HIP(hipSetDevice(e->device));
+  gmp::batch_ppr(*e, n, graph, seed, damping, iters, rank_out, st);
+  GMP_API_END
+}
+
 int gmp_cc(gmp_engine *h, uint32_t *label_out, gmp_stats *st) {
   GMP_API_BEGIN
   Engine *e = check_loaded(h);

@@ -137,11 +137,25 @@ struct Graph {
 // Tunables (gmp_set_option); the autotuner writes these.
 struct Options {
   int reorder = 1;             // build the degree-ordered work layout
-  uint32_t stream_cap = 1024;  // gather plan: edges per STREAM task
+  uint32_t stream_cap = 2048;  // gather plan: edges per STREAM task
   uint32_t row_cap = 512;      // gather plan: rows per STREAM task
-  uint32_t chunk = 2048;       // gather plan: edges per hub CHUNK task
-  uint32_t vec_min = 64;       // gather plan: rows above this many edges go wave-wide
-  uint32_t wg_per_cu = 8;      // gather grid = wg_per_cu * CUs
+  uint32_t chunk = 4096;       // gather plan: edges per hub CHUNK task
+  uint32_t vec_min = 128;      // gather plan: rows above this many edges go wave-wide
+  uint32_t wg_per_cu = 16;     // gather grid = wg_per_cu * CUs
+};
+
+// Many graphs resident at once as one disjoint union (graph k owns union ids
+// [vbase[k], vbase[k+1])), for batched multi-instance runs.
+struct BatchGraphs {
+  std::vector<uint32_t> vbase;        // G+1
+  std::vector<uint64_t> ebase;        // G+1
+  std::vector<uint64_t> h_off;        // union in-CSR offsets (host copy)
+  Csr in, out;                        // union in-CSR (offsets always uint64) and its transpose
+  DevBuf outdeg;                      // uint32[Vtot]
+  uint32_t V = 0;
+  uint64_t E = 0;
+  bool weighted = false;
+  bool ready = false;
 };
 
 struct Engine {
@@ -149,6 +163,7 @@ struct Engine {
   int num_cus = 256;
   hipStream_t stream = nullptr;
   Graph g;
+  BatchGraphs batch;
   Options opt;
   bool loaded = false;
   // scratch reused across runs
@@ -194,6 +209,10 @@ void ensure_work_out(Engine &e);                                   // its transp
 // out[v] = x[o2n[v]] for 4-byte values (work ids -> original ids); device pointers.
 void map_to_original(Engine &e, const uint32_t *x_work, uint32_t *out);
 uint32_t work_id(Engine &e, uint32_t original);
+// Transpose of a CSR (weights travel with their edge); rows of dst = columns of src.
+void transpose_csr(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, bool weighted, Csr &dst);
+// deg[v] = number of entries equal to v in idx[0..E)
+void count_degrees(Engine &e, const uint32_t *idx, uint64_t E, uint32_t V, uint32_t *deg);
 void graph_download(Engine &e, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights,
                     uint32_t *out_degree);
 
@@ -212,6 +231,13 @@ void run_compat(Engine &e, int app, int max_iter, const void *msg_init, const in
 void run_sssp(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist_out,
               gmp_stats *st);
 void run_cc(Engine &e, uint32_t *label_out, gmp_stats *st);
+
+// ---- batch.hip ----
+void batch_generate(Engine &e, uint32_t ngraphs, const gmp_rmat_params *params);
+void batch_sssp(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *source, uint32_t *dist_out,
+                gmp_stats *st);
+void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *seed, float damping, int32_t iters,
+               float *rank_out, gmp_stats *st);
 
 // ---- shared constants of the synthetic-input definition ----
 constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;

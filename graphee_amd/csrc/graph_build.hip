@@ -425,6 +425,18 @@ void ensure_work_out(Engine &e) {
   g.wout_ready = true;
 }
 
+void transpose_csr(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, bool weighted, Csr &dst) {
+  rebuild_csr(e, src, V, E, off64, weighted, nullptr, true, dst);
+}
+
+void count_degrees(Engine &e, const uint32_t *idx, uint64_t E, uint32_t V, uint32_t *deg) {
+  GMP_HIP(hipMemsetAsync(deg, 0, (size_t)V * 4, e.stream));
+  if (E) {
+    out_degree_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(idx, E, deg);
+    GMP_CHECK_LAUNCH();
+  }
+}
+
 void map_to_original(Engine &e, const uint32_t *x_work, uint32_t *out) {
   Graph &g = e.g;
   if (g.work_identity) {

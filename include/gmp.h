@@ -154,6 +154,21 @@ int gmp_sssp(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, uint32_t 
  * (the fixed point of GraphMP/app/cc.cpp's min-label rule). */
 int gmp_cc(gmp_engine *h, uint32_t *label_out, gmp_stats *st);
 
+/* ---- batched multi-instance runs ----------------------------------------------------
+ * No reference counterpart (GraphMP runs one app on one graph per process).  A batch keeps
+ * many graphs resident at once (independent of the graph loaded above); instances name a
+ * graph and a source/seed.  Each kind runs as persistent launches replayed from a HIP graph.
+ * dist_out / rank_out receive the instances' vectors back to back (instance i has as many
+ * entries as its graph has vertices), in instance order. */
+int gmp_batch_generate(gmp_engine *h, uint32_t ngraphs, const gmp_rmat_params *params);
+int gmp_batch_info(gmp_engine *h, uint32_t graph, uint32_t *V, uint64_t *E);
+/* In-CSR of one batch graph with local ids (row_off: V+1, col_idx/weights: E). */
+int gmp_batch_download(gmp_engine *h, uint32_t graph, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights);
+int gmp_batch_sssp(gmp_engine *h, uint32_t n, const uint32_t *graph, const uint32_t *source, uint32_t *dist_out,
+                   gmp_stats *st);
+int gmp_batch_ppr(gmp_engine *h, uint32_t n, const uint32_t *graph, const uint32_t *seed, float damping,
+                  int32_t iters, float *rank_out, gmp_stats *st);
+
 /* ---- host-side format utilities (no GPU needed) -----------------------------------
  * Equal-in-edge partitioner, format_spark.py:85-108. */
 int gmp_partition_bounds(const uint64_t *indeg, uint32_t V, uint32_t P, uint32_t *bounds);

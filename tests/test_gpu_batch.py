@@ -1,0 +1,61 @@
+"""Batched multi-instance runs (many graphs resident, many SSSP sources and personalised
+PageRank seeds in one launch sequence) against the oracle, instance by instance."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(engine, specs):
+    print("batch", specs, flush=True)
+    engine.batch_generate(specs)
+    return [engine.batch_download(g) for g in range(len(specs))]
+
+
+SPECS = [dict(scale=8, edge_factor=8, seed=5000), dict(scale=11, edge_factor=32, seed=5001),
+         dict(scale=9, edge_factor=16, seed=5002), dict(scale=12, edge_factor=8, seed=5003),
+         dict(scale=6, edge_factor=4, seed=5004)]
+
+
+def test_batch_graphs_match_single_generation(engine):
+    parts = _batch(engine, SPECS)
+    from oracle import rmat_ref
+    for (off, idx, w), s in zip(parts, SPECS):
+        V, roff, ridx = rmat_ref.in_csr(s["scale"], s["edge_factor"], s["seed"])
+        np.testing.assert_array_equal(off, roff)
+        np.testing.assert_array_equal(idx, ridx)
+        rows = np.repeat(np.arange(V, dtype=np.uint64), np.diff(off))
+        np.testing.assert_array_equal(w, rmat_ref.weights(idx, rows, s["seed"]))
+
+
+def test_batch_sssp_matches_dijkstra(engine):
+    parts = _batch(engine, SPECS)
+    rng = np.random.default_rng(1)
+    gsel = np.array([1, 1, 1, 0, 3, 3, 2, 4, 1, 3, 3, 3, 0, 2, 1, 1, 4, 3, 1, 1], dtype=np.uint32)  # uneven
+    srcs = np.array([rng.integers(0, parts[g][0].size - 1) for g in gsel], dtype=np.uint32)
+    dists, st = engine.batch_sssp(gsel, srcs)
+    assert st.iterations > 0 and st.launches > 0
+    for i, (g, s) in enumerate(zip(gsel, srcs)):
+        off, idx, w = parts[g]
+        V = off.size - 1
+        ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+        np.testing.assert_array_equal(dists[i], oracle.dijkstra(ooff, oidx, ow, int(s)), err_msg=f"instance {i}")
+
+
+def test_batch_ppr_matches_fp64(engine):
+    parts = _batch(engine, SPECS)
+    rng = np.random.default_rng(2)
+    # 70 instances on graph 2 (more than one 64-wide group), a few elsewhere
+    gsel = np.array([2] * 70 + [0, 1, 1, 3, 4], dtype=np.uint32)
+    seeds = np.array([rng.integers(0, parts[g][0].size - 1) for g in gsel], dtype=np.uint32)
+    ranks, st = engine.batch_ppr(gsel, seeds, damping=0.85, iters=30)
+    assert st.iterations == 30
+    for i, (g, s) in enumerate(zip(gsel, seeds)):
+        off, idx, _ = parts[g]
+        V = off.size - 1
+        x, _, _ = oracle.pr_f64(off, idx, graphs.out_degree(V, idx), d=0.85, max_iter=30, seed=int(s))
+        l1 = np.abs(ranks[i].astype(np.float64) - x).sum()
+        assert l1 < 1e-6, (i, l1)

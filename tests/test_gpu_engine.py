@@ -219,14 +219,16 @@ def test_results_independent_of_layout(engine, reorder):
 def test_plan_tunables_keep_results(engine, cap, rows, chunk, wg):
     g = gen(engine, 13)
     x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=10)
-    for k, v in ((gmp.OPT_STREAM_CAP, cap), (gmp.OPT_ROW_CAP, rows), (gmp.OPT_CHUNK, chunk), (gmp.OPT_WG_PER_CU, wg)):
+    keys = (gmp.OPT_STREAM_CAP, gmp.OPT_ROW_CAP, gmp.OPT_CHUNK, gmp.OPT_WG_PER_CU, gmp.OPT_VEC_MIN)
+    saved = [engine.get_option(k) for k in keys]
+    for k, v in zip(keys, (cap, rows, chunk, wg, max(1, cap // 8))):
         engine.set_option(k, v)
     try:
         say(f"plan cap={cap} rows={rows} chunk={chunk} wg={wg}")
         r, _ = engine.pagerank(max_iter=10)
         assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
     finally:
-        for k, v in ((gmp.OPT_STREAM_CAP, 1024), (gmp.OPT_ROW_CAP, 512), (gmp.OPT_CHUNK, 2048), (gmp.OPT_WG_PER_CU, 8)):
+        for k, v in zip(keys, saved):
             engine.set_option(k, v)
 
 
@@ -235,7 +237,7 @@ def test_option_range_checks(engine):
         with pytest.raises(gmp.GmpError) as ei:
             engine.set_option(k, v)
         assert ei.value.code == gmp.EINVAL
-    assert engine.get_option(gmp.OPT_CHUNK) == 2048
+    assert engine.get_option(gmp.OPT_CHUNK) == 4096
 
 
 # ---- shard files and edge cases --------------------------------------------------------
