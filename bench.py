@@ -63,6 +63,39 @@ def copy_calibration(nbytes):
     log(f"copy calibration: {nbytes >> 20} MiB device-to-device")
 
 
+def batched_workload(seed=5000):
+    """32 R-MAT graphs with scale uniform in 14..22 and edge factor uniform in 8..32 (weights
+    1..255), 256 SSSP sources and 256 PPR seeds placed on graphs with Zipf-skewed odds."""
+    rng = np.random.default_rng(seed)
+    specs = [dict(scale=int(rng.integers(14, 23)), edge_factor=int(rng.integers(8, 33)), seed=seed + g)
+             for g in range(32)]
+    odds = 1.0 / np.arange(1, 33) ** 1.1
+    odds /= odds.sum()
+    gs = rng.choice(32, size=256, p=odds).astype(np.uint32)
+    gp = rng.choice(32, size=256, p=odds[::-1]).astype(np.uint32)
+    return specs, gs, gp, rng
+
+
+def run_batched(eng):
+    specs, gs, gp, rng = batched_workload()
+    t0 = time.perf_counter()
+    eng.batch_generate(specs)
+    gen_s = time.perf_counter() - t0
+    sizes = [eng.batch_info(g) for g in range(32)]
+    srcs = np.array([rng.integers(0, sizes[g][0]) for g in gs], dtype=np.uint32)
+    seeds = np.array([rng.integers(0, sizes[g][0]) for g in gp], dtype=np.uint32)
+    eng.batch_sssp(gs[:4], srcs[:4])  # warm the kernels
+    _, st = eng.batch_sssp(gs, srcs)
+    sssp = {"instances": 256, "ms": round(st.ms_total, 3), "rounds": st.iterations, "graph_launches": st.launches,
+            "edges_relaxed": st.edges, "work_gteps": round(st.edges / st.ms_total / 1e6, 3)}
+    _, st2 = eng.batch_ppr(gp, seeds, damping=0.85, iters=30)
+    ppr = {"instances": 256, "iterations": 30, "ms": round(st2.ms_total, 3),
+           "instance_edges": st2.edges, "gteps": round(st2.edges / st2.ms_total / 1e6, 3)}
+    log(f"batched: {sum(s[1] for s in sizes)} edges in 32 graphs (built in {gen_s:.1f} s); "
+        f"SSSP x256 {st.ms_total:.1f} ms, PPR x256 {st2.ms_total:.1f} ms")
+    return {"graphs": 32, "total_edges": int(sum(s[1] for s in sizes)), "sssp": sssp, "ppr": ppr}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +225,9 @@ def main():
         result["extra"]["cc"] = {"ms": round(st.ms_total, 3), "edges_symmetric": Es, "iterations": st.iterations,
                                  "gteps": round(Es / st.ms_total / 1e6, 3), "components": ncomp}
         log(f"cc: {st.ms_total:.1f} ms, {st.iterations} iterations, {ncomp} components")
+
+    if not args.no_extra:
+        result["extra"]["batched"] = run_batched(eng)
 
     eng.close()
     print(json.dumps(result), flush=True)

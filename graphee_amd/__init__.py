@@ -42,7 +42,7 @@ class Stats(ctypes.Structure):
         ("ms_total", ctypes.c_double), ("ms_kernel", ctypes.c_double),
         ("edges", ctypes.c_uint64), ("vertices", ctypes.c_uint64),
         ("residual", ctypes.c_double), ("dangling", ctypes.c_double),
-        ("changed", ctypes.c_uint64), ("bytes", ctypes.c_uint64),
+        ("changed", ctypes.c_uint64), ("bytes", ctypes.c_uint64), ("index_bytes", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -98,6 +98,13 @@ struct CsrView {
   template <class OffT> const OffT *offs() const { return static_cast<const OffT *>(off); }
 };
 
+// Delta + varint column ids of the work in-CSR (decoded inside the gather).
+struct PackedCols {
+  DevBuf bytes, gbyte, gmask;
+  uint64_t nbytes = 0, ngroups = 0;
+  bool valid = false;
+};
+
 struct Graph {
   uint32_t V = 0;
   uint64_t E = 0;
@@ -122,6 +129,7 @@ struct Graph {
   DevBuf wdeg;                      // uint32[V]: out-degree by work id
   std::vector<uint64_t> h_win_off;  // host copy of the work in-CSR offsets (planning)
   GatherPlan plan;                  // over the work in-CSR
+  PackedCols packed;                // compressed work in-CSR columns (built on demand)
 
   CsrView view(const Csr &c) const { return CsrView{c.off.p, c.idx.as<uint32_t>(), c.w.as<uint32_t>()}; }
   CsrView work_in() const { return work_identity ? view(in) : view(win); }
@@ -221,6 +229,9 @@ void build_gather_plan_host(const uint64_t *off, uint32_t V, GatherPlan &plan,
                             std::vector<uint32_t> &tasks, std::vector<HubRow> &hubs,
                             std::vector<uint32_t> &multi);
 void ensure_plan(Engine &e);
+
+// ---- compress.hip ----
+void ensure_packed(Engine &e);
 
 // ---- pagerank.hip ----
 void run_pagerank(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st);

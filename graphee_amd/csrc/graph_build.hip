@@ -371,6 +371,8 @@ void ensure_work(Engine &e) {
   if (g.work_ready) return;
   g.work_identity = !e.opt.reorder;
   g.wout_ready = false;
+  g.packed.valid = false;
+  g.plan.built = false;
   if (g.work_identity) {
     g.work_ready = true;
     return;

@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "gmp_internal.hpp"
+#include "varint.hpp"
 #include "wave.hpp"
 
 namespace gmp {
@@ -60,6 +61,8 @@ struct PrArgs {
   uint32_t chunk;
   double *part_resid;
   double *part_dang;
+  Packed pk;    // compressed column ids (COMP kernels)
+  uint64_t E;
 };
 
 struct Acc {
@@ -100,11 +103,70 @@ __device__ inline void block_store_acc(Acc acc, double *pr, double *pd) {
   }
 }
 
-template <class OffT, bool KAHAN>
+template <bool KAHAN>
+__device__ inline void acc_add(float &s, float &c, float x) {
+  if (KAHAN) kahan_add(s, c, x);
+  else s += x;
+}
+
+// Gathers contributions of edges [e0, e1) in 64-edge groups aligned to the global edge index
+// (lane k takes the edges with e % 64 == k), so the plain and the compressed column streams
+// assign every edge to the same lane: the two paths give bit-identical sums.  STREAM tasks
+// store each value at buf[e - e0]; CHUNK tasks accumulate per lane.
+template <class OffT, bool KAHAN, bool COMP, bool TO_LDS>
+__device__ inline void gather_range(const PrArgs<OffT> &a, uint64_t e0, uint64_t e1, float *buf, uint32_t *scratch,
+                                    float &s, float &c) {
+  const uint32_t lane = lane_id();
+  if (e1 <= e0) return;
+  if (COMP) {
+    for (uint64_t g = e0 / kVarintGroup; g * kVarintGroup < e1; g++) {
+      const uint32_t src = varint_decode_group(a.pk, g, a.E, scratch);
+      const uint64_t e = g * kVarintGroup + lane;
+      if (e >= e0 && e < e1) {
+        const float x = a.contrib_in[src];
+        if (TO_LDS) buf[e - e0] = x;
+        else acc_add<KAHAN>(s, c, x);
+      }
+    }
+    return;
+  }
+  uint64_t j = (e0 & ~(uint64_t)(kWave - 1)) + lane;
+  // four independent id loads, then four gathers per lane and round (memory-level parallelism)
+  for (; j + 3 * kWave < e1; j += 4 * kWave) {
+    const bool ok0 = j >= e0, ok1 = j + kWave >= e0;
+    const uint32_t i0 = ok0 ? __builtin_nontemporal_load(&a.idx[j]) : 0u;
+    const uint32_t i1 = ok1 ? __builtin_nontemporal_load(&a.idx[j + kWave]) : 0u;
+    const uint32_t i2 = __builtin_nontemporal_load(&a.idx[j + 2 * kWave]);
+    const uint32_t i3 = __builtin_nontemporal_load(&a.idx[j + 3 * kWave]);
+    const float c0 = ok0 ? a.contrib_in[i0] : 0.0f, c1 = ok1 ? a.contrib_in[i1] : 0.0f;
+    const float c2 = a.contrib_in[i2], c3 = a.contrib_in[i3];
+    if (TO_LDS) {
+      if (ok0) buf[j - e0] = c0;
+      if (ok1) buf[j + kWave - e0] = c1;
+      buf[j + 2 * kWave - e0] = c2;
+      buf[j + 3 * kWave - e0] = c3;
+    } else {
+      if (ok0) acc_add<KAHAN>(s, c, c0);
+      if (ok1) acc_add<KAHAN>(s, c, c1);
+      acc_add<KAHAN>(s, c, c2);
+      acc_add<KAHAN>(s, c, c3);
+    }
+  }
+  for (; j < e1; j += kWave) {
+    if (j < e0) continue;
+    const float x = a.contrib_in[__builtin_nontemporal_load(&a.idx[j])];
+    if (TO_LDS) buf[j - e0] = x;
+    else acc_add<KAHAN>(s, c, x);
+  }
+}
+
+template <class OffT, bool KAHAN, bool COMP>
 __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
   extern __shared__ float lds[];
   const uint32_t lane = lane_id(), wv = wave_id();
-  float *buf = lds + wv * a.stream_cap;
+  const uint32_t per_wave = a.stream_cap + (COMP ? kVarintScratchWords : 0u);
+  float *buf = lds + wv * per_wave;
+  uint32_t *scratch = reinterpret_cast<uint32_t *>(buf + a.stream_cap);
   const float base = a.sc->base, extra = a.sc->extra;
   Acc acc;
   const uint32_t nw = gridDim.x * kWavesPerBlock;
@@ -112,67 +174,27 @@ __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
     const uint32_t x = a.tasks[2 * t], y = a.tasks[2 * t + 1];
     if (!(y & kTaskChunkBit)) {
       // STREAM: stage the gathered contributions of rows [x, x+y) in LDS, one row per lane.
-      const uint64_t e0 = a.off[x];
-      const uint32_t n = (uint32_t)((uint64_t)a.off[x + y] - e0);
-      // four independent id loads, then four gathers per lane and round (memory-level parallelism)
-      uint32_t j = lane;
-      for (; j + 3 * kWave < n; j += 4 * kWave) {
-        const uint32_t i0 = __builtin_nontemporal_load(&a.idx[e0 + j]);
-        const uint32_t i1 = __builtin_nontemporal_load(&a.idx[e0 + j + kWave]);
-        const uint32_t i2 = __builtin_nontemporal_load(&a.idx[e0 + j + 2 * kWave]);
-        const uint32_t i3 = __builtin_nontemporal_load(&a.idx[e0 + j + 3 * kWave]);
-        const float c0 = a.contrib_in[i0], c1 = a.contrib_in[i1], c2 = a.contrib_in[i2], c3 = a.contrib_in[i3];
-        buf[j] = c0;
-        buf[j + kWave] = c1;
-        buf[j + 2 * kWave] = c2;
-        buf[j + 3 * kWave] = c3;
-      }
-      for (; j < n; j += kWave) buf[j] = a.contrib_in[__builtin_nontemporal_load(&a.idx[e0 + j])];
+      const uint64_t e0 = a.off[x], e1 = a.off[x + y];
+      float s0 = 0, c0 = 0;
+      gather_range<OffT, KAHAN, COMP, true>(a, e0, e1, buf, scratch, s0, c0);
       wave_sync();
       for (uint32_t i = lane; i < y; i += kWave) {
         const uint32_t r = x + i;
         const uint32_t b = (uint32_t)((uint64_t)a.off[r] - e0), en = (uint32_t)((uint64_t)a.off[r + 1] - e0);
         float s = 0, c = 0;
-        if (KAHAN) {
-          for (uint32_t k = b; k < en; k++) kahan_add(s, c, buf[k]);
-        } else {
-          for (uint32_t k = b; k < en; k++) s += buf[k];
-        }
+        for (uint32_t k = b; k < en; k++) acc_add<KAHAN>(s, c, buf[k]);
         pr_apply(a, r, s - c, base, extra, acc);
       }
       wave_sync();
     } else {
-      // CHUNK of a hub row: lane-strided compensated sum, then a wave reduction.
+      // CHUNK of a hub row: per-lane compensated sums, then a wave reduction.
       const HubRow h = a.hubs[x];
       const uint32_t k = y & ~kTaskChunkBit;
       const uint64_t rb = a.off[h.row], re = a.off[h.row + 1];
       const uint64_t e0 = rb + (uint64_t)k * a.chunk;
       const uint64_t e1 = min(e0 + a.chunk, re);
       float s = 0, c = 0;
-      uint64_t j = e0 + lane;
-      for (; j + 3 * kWave < e1; j += 4 * kWave) {
-        const uint32_t i0 = __builtin_nontemporal_load(&a.idx[j]);
-        const uint32_t i1 = __builtin_nontemporal_load(&a.idx[j + kWave]);
-        const uint32_t i2 = __builtin_nontemporal_load(&a.idx[j + 2 * kWave]);
-        const uint32_t i3 = __builtin_nontemporal_load(&a.idx[j + 3 * kWave]);
-        const float c0 = a.contrib_in[i0], c1 = a.contrib_in[i1], c2 = a.contrib_in[i2], c3 = a.contrib_in[i3];
-        if (KAHAN) {
-          kahan_add(s, c, c0);
-          kahan_add(s, c, c1);
-          kahan_add(s, c, c2);
-          kahan_add(s, c, c3);
-        } else {
-          s += c0;
-          s += c1;
-          s += c2;
-          s += c3;
-        }
-      }
-      for (; j < e1; j += kWave) {
-        const float x = a.contrib_in[__builtin_nontemporal_load(&a.idx[j])];
-        if (KAHAN) kahan_add(s, c, x);
-        else s += x;
-      }
+      gather_range<OffT, KAHAN, COMP, false>(a, e0, e1, buf, scratch, s, c);
       float hi = s, lo = -c;
       wave_reduce_dd(hi, lo);
       if (lane == 0) {
@@ -403,7 +425,13 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, G, pr2, pd2, 0, V, p.damping, personalised, sc,
                                                       nullptr, -1);
   GMP_CHECK_LAUNCH();
-  const size_t lds = (size_t)kWavesPerBlock * pl.stream_cap * sizeof(float);
+  const bool comp = (p.flags & GMP_PR_COMPRESSED) != 0;
+  if (comp) {
+    ensure_packed(e);
+    a.pk = Packed{g.packed.bytes.as<uint8_t>(), g.packed.gbyte.as<uint64_t>(), g.packed.gmask.as<uint64_t>()};
+    a.E = g.E;
+  }
+  const size_t lds = (size_t)kWavesPerBlock * (pl.stream_cap + (comp ? kVarintScratchWords : 0u)) * sizeof(float);
   int cur = 0, it = 0;
   double resid = 0, dang = 0;
   const bool kahan = !(p.flags & GMP_PR_NO_KAHAN);
@@ -415,8 +443,13 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     a.part_resid = pr1;
     a.part_dang = pd1;
     GMP_HIP(hipEventRecord(ev[2 + 2 * it], e.stream));
-    if (kahan) pr_gather_kernel<OffT, true><<<G, kBlock, lds, e.stream>>>(a);
-    else pr_gather_kernel<OffT, false><<<G, kBlock, lds, e.stream>>>(a);
+    if (comp) {
+      if (kahan) pr_gather_kernel<OffT, true, true><<<G, kBlock, lds, e.stream>>>(a);
+      else pr_gather_kernel<OffT, false, true><<<G, kBlock, lds, e.stream>>>(a);
+    } else {
+      if (kahan) pr_gather_kernel<OffT, true, false><<<G, kBlock, lds, e.stream>>>(a);
+      else pr_gather_kernel<OffT, false, false><<<G, kBlock, lds, e.stream>>>(a);
+    }
     GMP_CHECK_LAUNCH();
     GMP_HIP(hipEventRecord(ev[3 + 2 * it], e.stream));
     if (Gh) {
@@ -474,6 +507,7 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     st->changed = 0;
     const uint64_t o = g.off64 ? 8 : 4;
     st->bytes = (uint64_t)it * (8 * g.E + (o + 16) * (uint64_t)V);
+    st->index_bytes = comp ? g.packed.nbytes + 16 * g.packed.ngroups : 4 * g.E;
   }
 }
 
@@ -536,7 +570,6 @@ void compat_impl(Engine &e, int app, int max_iter, const void *msg_init, const i
 void run_pagerank(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st) {
   if (!(p.damping >= 0.0f && p.damping <= 1.0f)) fail(GMP_EINVAL, "damping must be in [0, 1]");
   if (p.seed_vertex >= (int64_t)e.g.V) fail(GMP_EINVAL, "seed vertex out of range");
-  if (p.flags & GMP_PR_COMPRESSED) fail(GMP_EUNSUPPORTED, "compressed gather is not built in this library");
   if (e.g.off64) pagerank_impl<uint64_t>(e, p, rank_out, st);
   else pagerank_impl<uint32_t>(e, p, rank_out, st);
 }

@@ -62,6 +62,7 @@ typedef struct gmp_stats {
   double dangling;     /* PageRank: dangling mass entering the last iteration (FP64) */
   uint64_t changed;    /* vertices changed in the last iteration */
   uint64_t bytes;      /* algorithmic bytes of the run (uncompressed yardstick) */
+  uint64_t index_bytes; /* PageRank: bytes of the column-id stream read per iteration */
 } gmp_stats;
 
 typedef struct gmp_rmat_params {

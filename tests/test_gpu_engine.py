@@ -232,6 +232,36 @@ def test_plan_tunables_keep_results(engine, cap, rows, chunk, wg):
             engine.set_option(k, v)
 
 
+@pytest.mark.parametrize("reorder", [0, 1])
+def test_compressed_gather_is_bit_identical(engine, reorder):
+    """Delta/varint column ids decoded in the gather give exactly the plain path's ranks."""
+    engine.set_option(gmp.OPT_REORDER, reorder)
+    try:
+        g = gen(engine, 13, off64=bool(reorder))
+        say(f"compressed gather reorder={reorder}")
+        r0, s0 = engine.pagerank(max_iter=20)
+        r1, s1 = engine.pagerank(max_iter=20, compressed=True)
+        np.testing.assert_array_equal(r1.view(np.uint32), r0.view(np.uint32))
+        assert s1.index_bytes < s0.index_bytes == 4 * g["idx"].size
+        # rows in arbitrary order (restarts), hubs spanning many groups, E not a multiple of 64
+        V = 30011
+        rng = np.random.default_rng(9)
+        src = np.concatenate([rng.integers(0, V, 200001), np.arange(V)])
+        dst = np.concatenate([rng.integers(0, 50, 200001), rng.integers(0, V, V)])
+        off, idx = graphs.csr_from_edges(V, src, dst)
+        for r in range(0, V, 7):  # scramble some rows
+            seg = idx[off[r]:off[r + 1]]
+            idx[off[r]:off[r + 1]] = seg[::-1]
+        engine.load_csr(off, idx)
+        r0, _ = engine.pagerank(max_iter=12)
+        r1, _ = engine.pagerank(max_iter=12, compressed=True)
+        np.testing.assert_array_equal(r1.view(np.uint32), r0.view(np.uint32))
+        x, _, _ = oracle.pr_f64(off, idx, graphs.out_degree(V, idx), max_iter=12)
+        assert np.abs(r1.astype(np.float64) - x).sum() < PR_L1_TOL
+    finally:
+        engine.set_option(gmp.OPT_REORDER, 1)
+
+
 def test_option_range_checks(engine):
     for k, v in ((gmp.OPT_STREAM_CAP, 8192), (gmp.OPT_REORDER, 2), (gmp.OPT_WG_PER_CU, 0), (99, 1)):
         with pytest.raises(gmp.GmpError) as ei:
