@@ -178,6 +178,19 @@ def main():
         "extra": {},
     }
 
+    if not args.no_extra:
+        eng.pagerank(damping=0.85, max_iter=3, compressed=True, want_ranks=False)  # builds the packed lists
+        _, stc = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, compressed=True,
+                              want_ranks=False)
+        kms_c = stc.ms_kernel / max(stc.launches, 1)
+        result["extra"]["pagerank_compressed"] = {
+            "ms_per_solve": round(stc.ms_total, 3), "iterations": stc.iterations,
+            "gteps": round(E * stc.iterations / stc.ms_total / 1e6, 3),
+            "gather_ms": round(kms_c, 4), "yardstick_gbs": round(bytes_launch / kms_c / 1e6, 1),
+            "index_bytes": stc.index_bytes, "plain_index_bytes": 4 * E,
+            "index_bytes_per_edge": round(stc.index_bytes / E, 3)}
+        log(f"pagerank compressed: {stc.index_bytes / E:.2f} B/edge column ids, gather {kms_c:.3f} ms/launch")
+
     if not args.no_cpu:
         import oracle
         ranks, _ = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol)

@@ -25,7 +25,9 @@ SYMBOLS = (
     "gmp_load_shards", "gmp_generate_rmat", "gmp_graph_info", "gmp_graph_download",
     "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_sssp", "gmp_cc",
     "gmp_set_option", "gmp_get_option", "gmp_batch_generate", "gmp_batch_info", "gmp_batch_download",
-    "gmp_batch_sssp", "gmp_batch_ppr",
+    "gmp_batch_sssp", "gmp_batch_ppr", "gmp_autotune", "gmp_tune_save", "gmp_tune_load",
+    "gmp_sssp_trace", "gmp_mlp_init", "gmp_mlp_set", "gmp_mlp_get", "gmp_mlp_forward", "gmp_mlp_loss_grad",
+    "gmp_mlp_train",
     "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards",
 )
 
@@ -43,6 +45,7 @@ class Stats(ctypes.Structure):
         ("edges", ctypes.c_uint64), ("vertices", ctypes.c_uint64),
         ("residual", ctypes.c_double), ("dangling", ctypes.c_double),
         ("changed", ctypes.c_uint64), ("bytes", ctypes.c_uint64), ("index_bytes", ctypes.c_uint64),
+        ("epochs", ctypes.c_int32), ("policy", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -72,6 +75,21 @@ class SsspParams(ctypes.Structure):
     ]
 
 
+MLP_IN, MLP_HIDDEN, MLP_CLASSES, MLP_SHARDS, MLP_PARAMS = 16, 64, 8, 64, 5768
+
+TRACE_DTYPE = np.dtype([("feat", np.float32, (16,)), ("best", np.int32), ("epoch", np.uint32),
+                        ("shard_mask", np.uint64), ("ms", np.float32, (8,)), ("settled", np.float32, (8,))])
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("stream_cap", ctypes.c_uint32), ("row_cap", ctypes.c_uint32), ("chunk", ctypes.c_uint32),
+                ("vec_min", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32), ("trials", ctypes.c_uint32),
+                ("gather_ms_before", ctypes.c_double), ("gather_ms_after", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 _P = ctypes.c_void_p
 
@@ -98,6 +116,17 @@ def lib():
         "gmp_graph_download": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
         "gmp_pick_source": ([_P, u64, u32, _P], ctypes.c_int),
         "gmp_set_option": ([_P, i32, i64], ctypes.c_int),
+        "gmp_autotune": ([_P, i32, ctypes.POINTER(TuneResult)], ctypes.c_int),
+        "gmp_sssp_trace": ([_P, u32, ctypes.POINTER(SsspParams), _P, u32, _P, _P, ctypes.POINTER(Stats)],
+                           ctypes.c_int),
+        "gmp_mlp_init": ([_P, u64], ctypes.c_int),
+        "gmp_mlp_set": ([_P, _P], ctypes.c_int),
+        "gmp_mlp_get": ([_P, _P], ctypes.c_int),
+        "gmp_mlp_forward": ([_P, _P, u32, _P], ctypes.c_int),
+        "gmp_mlp_loss_grad": ([_P, _P, _P, _P, u32, _P, _P], ctypes.c_int),
+        "gmp_mlp_train": ([_P, _P, _P, _P, u32, i32, ctypes.c_float, _P], ctypes.c_int),
+        "gmp_tune_save": ([_P, ctypes.c_char_p], ctypes.c_int),
+        "gmp_tune_load": ([_P, ctypes.c_char_p], ctypes.c_int),
         "gmp_batch_generate": ([_P, u32, _P], ctypes.c_int),
         "gmp_batch_info": ([_P, u32, _P, _P], ctypes.c_int),
         "gmp_batch_download": ([_P, u32, _P, _P, _P], ctypes.c_int),
@@ -234,6 +263,17 @@ class Engine:
         check(lib().gmp_get_option(self._h, key, ctypes.byref(v)))
         return v.value
 
+    def autotune(self, iters=3):
+        r = TuneResult()
+        check(lib().gmp_autotune(self._h, iters, ctypes.byref(r)))
+        return r
+
+    def tune_save(self, path):
+        check(lib().gmp_tune_save(self._h, os.fsencode(path)))
+
+    def tune_load(self, path):
+        check(lib().gmp_tune_load(self._h, os.fsencode(path)))
+
     def pick_source(self, seed, salt=3):
         v = ctypes.c_uint32()
         check(lib().gmp_pick_source(self._h, seed, salt, ctypes.byref(v)))
@@ -319,6 +359,54 @@ class Engine:
         st = Stats()
         check(lib().gmp_sssp(self._h, source, ctypes.byref(p), _ptr(out), ctypes.byref(st)))
         return out, st
+
+    # learned scheduling
+    def sssp_trace(self, source, cap=1 << 16, want_dist=True):
+        V = self.info()[0]
+        rows = np.zeros(cap, dtype=TRACE_DTYPE)
+        n = ctypes.c_uint32()
+        out = np.zeros(V, dtype=np.uint32) if want_dist else None
+        st = Stats()
+        p = SsspParams(SSSP_LEARNED, 0, 0, 0)
+        check(lib().gmp_sssp_trace(self._h, source, ctypes.byref(p), _ptr(rows), cap, ctypes.byref(n), _ptr(out),
+                                   ctypes.byref(st)))
+        return rows[: n.value].copy(), out, st
+
+    def mlp_init(self, seed):
+        check(lib().gmp_mlp_init(self._h, seed))
+
+    def mlp_set(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.size == MLP_PARAMS
+        check(lib().gmp_mlp_set(self._h, _ptr(params)))
+
+    def mlp_get(self):
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        check(lib().gmp_mlp_get(self._h, _ptr(p)))
+        return p
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_IN)
+        Y = np.zeros((X.shape[0], MLP_CLASSES + MLP_SHARDS), dtype=np.float32)
+        check(lib().gmp_mlp_forward(self._h, _ptr(X), X.shape[0], _ptr(Y)))
+        return Y
+
+    def mlp_loss_grad(self, X, labels, masks):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_IN)
+        y = np.ascontiguousarray(labels, dtype=np.int32)
+        m = np.ascontiguousarray(masks, dtype=np.uint64)
+        loss = ctypes.c_float()
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        check(lib().gmp_mlp_loss_grad(self._h, _ptr(X), _ptr(y), _ptr(m), y.size, ctypes.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def mlp_train(self, X, labels, masks, epochs=200, lr=3e-3):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_IN)
+        y = np.ascontiguousarray(labels, dtype=np.int32)
+        m = np.ascontiguousarray(masks, dtype=np.uint64)
+        loss = ctypes.c_float()
+        check(lib().gmp_mlp_train(self._h, _ptr(X), _ptr(y), _ptr(m), y.size, epochs, lr, ctypes.byref(loss)))
+        return loss.value
 
     def cc(self, want_labels=True):
         V = self.info()[0]

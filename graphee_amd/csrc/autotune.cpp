@@ -1,0 +1,163 @@
+// Autotuner for the PageRank gather: picks the degree-bin threshold (vec_min), the LDS tile
+// (stream_cap rows' edges per wave), the hub chunk size and the workgroups per CU on the
+// loaded graph by measuring the gather kernel with HIP events.  GraphMP fixes the matching
+// knobs at compile time (CMPNUM threads, GraphMP/system/Global.h:58); nothing here changes
+// results, only the task plan and the grid.
+//
+// Search: coordinate descent from the current options over discrete candidate sets, two
+// sweeps, each trial = a few PageRank iterations timed on the engine stream; the best point
+// is applied and returned.  The result can be saved as JSON and loaded at start-up.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "gmp_internal.hpp"
+
+namespace gmp {
+
+namespace {
+
+double time_gather(Engine &e, int iters) {
+  gmp_pr_params p{0.85f, iters, 0.0, -1, 0, 0};
+  gmp_stats st{};
+  run_pagerank(e, p, nullptr, &st);  // warm (plan build, caches)
+  run_pagerank(e, p, nullptr, &st);
+  return st.launches ? st.ms_kernel / st.launches : 0.0;
+}
+
+uint32_t *field(Options &o, int k) {
+  switch (k) {
+    case 0: return &o.vec_min;
+    case 1: return &o.stream_cap;
+    case 2: return &o.chunk;
+    case 3: return &o.wg_per_cu;
+    default: return &o.row_cap;
+  }
+}
+
+}  // namespace
+
+void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
+  if (iters <= 0) iters = 3;
+  ensure_work(e);
+  const std::vector<std::vector<uint32_t>> cand = {
+      {16, 32, 64, 128, 256, 512},      // vec_min
+      {512, 1024, 2048, 4096},          // stream_cap
+      {1024, 2048, 4096, 8192, 16384},  // chunk
+      {2, 4, 8, 16, 32},                // wg_per_cu
+      {64, 128, 256, 512, 1024},        // row_cap
+  };
+  Options best = e.opt;
+  double best_ms = time_gather(e, iters);
+  const double start_ms = best_ms;
+  int trials = 1;
+  for (int sweep = 0; sweep < 2; sweep++) {
+    bool improved = false;
+    for (size_t k = 0; k < cand.size(); k++) {
+      for (uint32_t v : cand[k]) {
+        Options o = best;
+        if (*field(o, (int)k) == v) continue;
+        *field(o, (int)k) = v;
+        if (o.vec_min > o.stream_cap) continue;
+        e.opt = o;
+        const double ms = time_gather(e, iters);
+        trials++;
+        if (ms > 0 && ms < best_ms * 0.995) {
+          best_ms = ms;
+          best = o;
+          improved = true;
+        }
+      }
+    }
+    if (!improved) break;
+  }
+  e.opt = best;
+  if (out) {
+    std::memset(out, 0, sizeof *out);
+    out->stream_cap = best.stream_cap;
+    out->row_cap = best.row_cap;
+    out->chunk = best.chunk;
+    out->vec_min = best.vec_min;
+    out->wg_per_cu = best.wg_per_cu;
+    out->trials = trials;
+    out->gather_ms_before = start_ms;
+    out->gather_ms_after = best_ms;
+  }
+}
+
+void tune_save(Engine &e, const std::string &path) {
+  std::ofstream f(path);
+  if (!f) fail(GMP_EIO, path + ": cannot open for writing");
+  f << "{\"reorder\": " << e.opt.reorder << ", \"stream_cap\": " << e.opt.stream_cap
+    << ", \"row_cap\": " << e.opt.row_cap << ", \"chunk\": " << e.opt.chunk << ", \"vec_min\": " << e.opt.vec_min
+    << ", \"wg_per_cu\": " << e.opt.wg_per_cu << "}\n";
+  if (!f) fail(GMP_EIO, path + ": short write");
+}
+
+// Minimal reader for the flat integer object tune_save writes.
+void tune_load(Engine &e, const std::string &path) {
+  std::ifstream f(path);
+  if (!f) fail(GMP_EIO, path + ": cannot open");
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string s = ss.str();
+  Options o = e.opt;
+  auto get = [&](const char *key, uint32_t lo, uint32_t hi, uint32_t &dst) {
+    const std::string k = std::string("\"") + key + "\"";
+    size_t p = s.find(k);
+    if (p == std::string::npos) return;
+    p = s.find(':', p);
+    if (p == std::string::npos) fail(GMP_EFORMAT, path + ": bad value for " + key);
+    long v = std::strtol(s.c_str() + p + 1, nullptr, 10);
+    if (v < (long)lo || v > (long)hi) fail(GMP_EFORMAT, path + ": " + key + " out of range");
+    dst = (uint32_t)v;
+  };
+  uint32_t reorder = (uint32_t)o.reorder;
+  get("reorder", 0, 1, reorder);
+  get("stream_cap", 64, 4096, o.stream_cap);
+  get("row_cap", 1, 4096, o.row_cap);
+  get("chunk", 64, 1u << 20, o.chunk);
+  get("vec_min", 1, 4096, o.vec_min);
+  get("wg_per_cu", 1, 64, o.wg_per_cu);
+  if ((int)reorder != e.opt.reorder) {
+    e.g.work_ready = false;
+    e.g.wout_ready = false;
+    e.g.plan.built = false;
+  }
+  o.reorder = (int)reorder;
+  e.opt = o;
+}
+
+}  // namespace gmp
+
+extern "C" {
+
+int gmp_autotune(gmp_engine *h, int32_t iters, gmp_tune_result *out) {
+  GMP_API_BEGIN
+  if (!h) gmp::fail(GMP_EINVAL, "null engine handle");
+  auto *e = reinterpret_cast<gmp::Engine *>(h);
+  if (!e->loaded) gmp::fail(GMP_ESTATE, "no graph loaded");
+  GMP_HIP(hipSetDevice(e->device));
+  gmp::autotune(*e, iters, out);
+  GMP_API_END
+}
+
+int gmp_tune_save(gmp_engine *h, const char *path) {
+  GMP_API_BEGIN
+  if (!h || !path) gmp::fail(GMP_EINVAL, "gmp_tune_save: null argument");
+  gmp::tune_save(*reinterpret_cast<gmp::Engine *>(h), path);
+  GMP_API_END
+}
+
+int gmp_tune_load(gmp_engine *h, const char *path) {
+  GMP_API_BEGIN
+  if (!h || !path) gmp::fail(GMP_EINVAL, "gmp_tune_load: null argument");
+  gmp::tune_load(*reinterpret_cast<gmp::Engine *>(h), path);
+  GMP_API_END
+}
+
+}  // extern "C"

@@ -279,6 +279,16 @@ int gmp_sssp(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, uint32_t 
   GMP_API_END
 }
 
+int gmp_sssp_trace(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, gmp_trace_row *rows, uint32_t cap,
+                   uint32_t *nrows, uint32_t *dist_out, gmp_stats *st) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  if (!nrows || (cap && !rows)) fail(GMP_EINVAL, "gmp_sssp_trace: null argument");
+  gmp_sssp_params d{GMP_SSSP_LEARNED, 0, 0, 0};
+  gmp::run_sssp_trace(*e, source, p ? *p : d, rows, cap, nrows, dist_out, st);
+  GMP_API_END
+}
+
 int gmp_batch_generate(gmp_engine *h, uint32_t ngraphs, const gmp_rmat_params *params) {
   GMP_API_BEGIN
   Engine *e = check(h);

@@ -166,12 +166,23 @@ struct BatchGraphs {
   bool ready = false;
 };
 
+// Scheduling MLP (mlp.hip): 16 -> 64 (ReLU) -> 8 width logits + 64 shard logits.
+constexpr int kMlpIn = 16, kMlpHid = 64, kMlpClasses = 8, kMlpShards = 64;
+constexpr int kMlpOut = kMlpClasses + kMlpShards;
+constexpr int kMlpParams = kMlpIn * kMlpHid + kMlpHid + kMlpHid * kMlpOut + kMlpOut;  // 5768
+struct Mlp {
+  DevBuf params, grad, adam_m, adam_v;
+  int step = 0;
+  bool trained = false;
+};
+
 struct Engine {
   int device = 0;
   int num_cus = 256;
   hipStream_t stream = nullptr;
   Graph g;
   BatchGraphs batch;
+  Mlp mlp;
   Options opt;
   bool loaded = false;
   // scratch reused across runs
@@ -233,6 +244,19 @@ void ensure_plan(Engine &e);
 // ---- compress.hip ----
 void ensure_packed(Engine &e);
 
+// ---- autotune.cpp ----
+void autotune(Engine &e, int32_t iters, gmp_tune_result *out);
+
+// ---- mlp.hip ----
+void mlp_ensure(Engine &e);
+void mlp_init(Engine &e, uint64_t seed);
+void mlp_set(Engine &e, const float *params);
+void mlp_get(Engine &e, float *params);
+void mlp_forward(Engine &e, const float *X, uint32_t n, float *Y);
+float mlp_loss_grad(Engine &e, const float *X, const int32_t *y, const uint64_t *mask, uint32_t n, float *grad);
+float mlp_train(Engine &e, const float *X, const int32_t *y, const uint64_t *mask, uint32_t n, int32_t epochs,
+                float lr);
+
 // ---- pagerank.hip ----
 void run_pagerank(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st);
 void run_compat(Engine &e, int app, int max_iter, const void *msg_init, const int32_t *vout,
@@ -242,6 +266,8 @@ void run_compat(Engine &e, int app, int max_iter, const void *msg_init, const in
 void run_sssp(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist_out,
               gmp_stats *st);
 void run_cc(Engine &e, uint32_t *label_out, gmp_stats *st);
+void run_sssp_trace(Engine &e, uint32_t src, const gmp_sssp_params &p, gmp_trace_row *rows, uint32_t cap,
+                    uint32_t *nrows, uint32_t *dist_out, gmp_stats *st);
 
 // ---- batch.hip ----
 void batch_generate(Engine &e, uint32_t ngraphs, const gmp_rmat_params *params);

@@ -38,9 +38,24 @@ struct Counters {
   uint32_t near_cnt;
   uint32_t far_cnt;
   uint32_t far_min;
-  uint32_t pad;
+  uint32_t choice;                  // policy output: bucket-width class
   unsigned long long edges;
+  unsigned long long shard_mask;    // OR of 1 << (64 v / V) over vertices appended to the near frontier
+  unsigned long long settled;       // vertices below a threshold (trace exploration)
+  uint32_t hist[16];                // far-pile occupancy / degree histograms, [13] = live entries
 };
+static_assert(sizeof(Counters) <= 256, "counters must fit the pinned readback buffer");
+
+// Feature vector of a bucket epoch from the far-pile histograms (shared by the recorder on the
+// host and the policy kernel on the device).
+__host__ __device__ inline void epoch_features(const uint32_t *hist, unsigned long long edges, uint32_t far_min,
+                                               float *f) {
+  const float live = (float)(hist[13] ? hist[13] : 1u);
+  for (int i = 0; i < 13; i++) f[i] = (float)hist[i] / live;
+  f[13] = log2f(1.0f + (float)hist[13]) / 32.0f;
+  f[14] = log2f(1.0f + (float)edges) / 40.0f;
+  f[15] = log2f(1.0f + (float)far_min) / 24.0f;
+}
 
 template <class OffT>
 struct RelaxArgs {
@@ -57,6 +72,8 @@ struct RelaxArgs {
   uint32_t *far_flag;
   uint32_t *far_out;
   Counters *cnt;
+  uint32_t V;
+  int track_shards;       // record which 64ths of the id range join the near frontier
 };
 
 template <class OffT, int MODE>
@@ -66,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
   __shared__ uint32_t s_val[kWavesPerBlock][kWave];
   const uint32_t lane = lane_id(), wv = wave_id();
   const uint32_t nchunks = (a.nfront + kWave - 1) / kWave;
-  unsigned long long edges = 0;
+  unsigned long long edges = 0, smask = 0;
   for (uint32_t c = blockIdx.x * kWavesPerBlock + wv; c < nchunks; c += gridDim.x * kWavesPerBlock) {
     const uint32_t i = c * kWave + lane;
     const bool ok = i < a.nfront;
@@ -121,10 +138,77 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
         const uint32_t sf = wave_append(to_far, &a.cnt->far_cnt);
         if (to_far) a.far_out[sf] = v;
       }
+      if (a.track_shards && to_near) smask |= 1ull << (uint32_t)(((uint64_t)v * kMlpShards) / a.V);
     }
     wave_sync();
   }
   if (lane == 0 && edges) atomicAdd(&a.cnt->edges, edges);
+  if (a.track_shards) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) smask |= (unsigned long long)__shfl_xor((long long)smask, o, kWave);
+    if (lane == 0 && smask) atomicOr(&a.cnt->shard_mask, smask);
+  }
+}
+
+// Histograms of the live far pile for the epoch features: distance above the pile minimum in
+// bins [0,8),[8,16),...,[512,inf) -> hist[0..7]; out-degree <=1, 2-7, 8-63, 64-511, >=512 ->
+// hist[8..12]; live count -> hist[13].
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void far_features_kernel(const uint32_t *far, uint32_t n, const uint32_t *dist,
+                                                              const OffT *off, uint32_t thr_old, Counters *cnt) {
+  __shared__ uint32_t h[16];
+  if (threadIdx.x < 16) h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t m = cnt->far_min;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = far[i];
+    const uint32_t d = dist[v];
+    if (d < thr_old) continue;
+    const uint32_t x = d - m;
+    const uint32_t ob = x < 8 ? 0u : min(7u, 31u - (uint32_t)__clz((int)x) - 2u);
+    const uint32_t deg = (uint32_t)((uint64_t)off[v + 1] - (uint64_t)off[v]);
+    const uint32_t db = deg <= 1 ? 0u : deg < 8 ? 1u : deg < 64 ? 2u : deg < 512 ? 3u : 4u;
+    atomicAdd(&h[ob], 1u);
+    atomicAdd(&h[8 + db], 1u);
+    atomicAdd(&h[13], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 14 && h[threadIdx.x]) atomicAdd(&cnt->hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// Inference of the scheduling MLP for one epoch: features -> 64 hidden -> 8 width logits;
+// the argmax class goes to cnt->choice (ties to the smaller width).
+__global__ __launch_bounds__(64) void policy_kernel(const float *P, Counters *cnt) {
+  __shared__ float f[kMlpIn], hsh[kMlpHid], logits[kMlpClasses];
+  if (threadIdx.x == 0) epoch_features(cnt->hist, cnt->edges, cnt->far_min, f);
+  __syncthreads();
+  const float *W1 = P, *b1 = P + kMlpIn * kMlpHid, *W2 = b1 + kMlpHid, *b2 = W2 + kMlpHid * kMlpOut;
+  const uint32_t j = threadIdx.x;
+  float a = b1[j];
+  for (int i = 0; i < kMlpIn; i++) a = fmaf(f[i], W1[i * kMlpHid + j], a);
+  hsh[j] = a > 0.0f ? a : 0.0f;
+  __syncthreads();
+  if (j < (uint32_t)kMlpClasses) {
+    float y = b2[j];
+    for (int k = 0; k < kMlpHid; k++) y = fmaf(hsh[k], W2[k * kMlpOut + j], y);
+    logits[j] = y;
+  }
+  __syncthreads();
+  if (j == 0) {
+    uint32_t best = 0;
+    for (uint32_t c = 1; c < (uint32_t)kMlpClasses; c++)
+      if (logits[c] > logits[best]) best = c;
+    cnt->choice = best;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void count_below_kernel(const uint32_t *dist, uint32_t V, uint32_t thr,
+                                                             Counters *cnt) {
+  unsigned long long c = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) c += dist[v] < thr;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += (unsigned long long)__shfl_xor((long long)c, o, kWave);
+  if (lane_id() == 0 && c) atomicAdd(&cnt->settled, c);
 }
 
 // Smallest distance among far entries still at or above the old threshold.
@@ -240,32 +324,52 @@ void read_counters(Engine &e, Counters *dev, Counters &h) {
   std::memcpy(&h, e.h_pinned, sizeof(Counters));
 }
 
+// Bucket epochs of delta-stepping.  FIXED: aligned buckets [k*delta, (k+1)*delta).  LEARNED:
+// the MLP picks each epoch's bucket [m, m + 8 * 2^c) from far-pile features (m = smallest live
+// far distance).  EXPLORE records traces: from a snapshot of the state every width class runs
+// the epoch, the class settling the most vertices per millisecond becomes the label and its run
+// continues.  Every policy relaxes until no distance drops, so distances are the same.
+enum Policy { kPolicyBF = 0, kPolicyFixed = 1, kPolicyLearned = 2, kPolicyExplore = 3 };
+
 template <class OffT>
-void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist_out, gmp_stats *st) {
+void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gmp_trace_row *rows, uint32_t cap,
+               uint32_t *nrows, uint32_t *dist_out, gmp_stats *st) {
   Graph &g = e.g;
   ensure_work_out(e);
   const bool unit = p.unit_weights || !g.weighted;
   const int mode = unit ? kModeUnit : kModeWeighted;
-  const bool delta_mode = p.method == GMP_SSSP_DELTA || p.method == GMP_SSSP_LEARNED;
+  if (policy == kPolicyLearned && !e.mlp.trained) policy = kPolicyFixed;  // untrained network: plain schedule
+  const bool bucketed = policy != kPolicyBF;
   const uint32_t delta = p.delta ? p.delta : 64u;
   const uint32_t V = g.V;
   src = work_id(e, src);
-  TraverseBufs b = alloc_traverse(e, delta_mode);
+  TraverseBufs b = alloc_traverse(e, true);
   const CsrView ov = g.work_out();
   const OffT *ooff = ov.offs<OffT>();
   const uint32_t *oidx = ov.idx;
   const uint32_t *ow = ov.w;
+  if (nrows) *nrows = 0;
 
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
   fill_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.dist, V, kInf);
   fill_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.stamp, V, 0u);
-  if (delta_mode) fill_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.farflag, V, 0u);
+  fill_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.farflag, V, 0u);
   GMP_CHECK_LAUNCH();
-  const uint32_t zero = 0;
+  const uint32_t zero = 0, one = 1;
   GMP_HIP(hipMemcpyAsync(b.dist + src, &zero, 4, hipMemcpyHostToDevice, e.stream));
-  GMP_HIP(hipMemcpyAsync(b.q0, &src, 4, hipMemcpyHostToDevice, e.stream));
   GMP_HIP(hipMemsetAsync(b.cnt, 0, sizeof(Counters), e.stream));
+  uint32_t nnear = 0, nfar = 0, stamp = 1, thr = 0;
+  uint32_t *near_in = b.q0, *near_out = b.q1, *far_cur = b.f0, *far_nxt = b.f1;
+  if (bucketed) {  // the source starts in the far pile; the first split opens its bucket
+    GMP_HIP(hipMemcpyAsync(far_cur, &src, 4, hipMemcpyHostToDevice, e.stream));
+    GMP_HIP(hipMemcpyAsync(b.farflag + src, &one, 4, hipMemcpyHostToDevice, e.stream));
+    nfar = 1;
+  } else {
+    GMP_HIP(hipMemcpyAsync(near_in, &src, 4, hipMemcpyHostToDevice, e.stream));
+    nnear = 1;
+    thr = kInf;
+  }
 
   RelaxArgs<OffT> a{};
   a.off = ooff;
@@ -275,19 +379,19 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist
   a.near_stamp = b.stamp;
   a.far_flag = b.farflag;
   a.cnt = b.cnt;
-  uint32_t nnear = 1, nfar = 0, stamp = 1;
-  uint32_t thr = delta_mode ? delta : kInf;
-  uint32_t *near_in = b.q0, *near_out = b.q1, *far_cur = b.f0, *far_nxt = b.f1;
-  int iters = 0;
+  a.V = V;
+  int iters = 0, epochs = 0;
   Counters h{};
-  for (;;) {
+
+  auto relax_until_empty = [&](bool track) {
     while (nnear) {
       a.front = near_in;
       a.nfront = nnear;
       a.stamp = stamp++;
       a.near_out = near_out;
       a.threshold = thr;
-      a.far_out = delta_mode ? far_cur + nfar : nullptr;
+      a.far_out = bucketed ? far_cur + nfar : nullptr;
+      a.track_shards = track ? 1 : 0;
       GMP_HIP(hipMemsetAsync(&b.cnt->near_cnt, 0, 8, e.stream));  // near_cnt, far_cnt
       launch_relax<OffT>(e, mode, a);
       read_counters(e, b.cnt, h);
@@ -296,29 +400,117 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist
       std::swap(near_in, near_out);
       iters++;
     }
-    if (!delta_mode || nfar == 0) break;
-    // next bucket: the smallest live far distance decides the new threshold
-    const uint32_t inf = kInf;
-    GMP_HIP(hipMemcpyAsync(&b.cnt->far_min, &inf, 4, hipMemcpyHostToDevice, e.stream));
-    far_min_kernel<<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, thr, b.cnt);
-    GMP_CHECK_LAUNCH();
-    read_counters(e, b.cnt, h);
-    const uint32_t thr_old = thr;
-    if (h.far_min == kInf) {
-      thr = kInf;
-    } else {
-      const uint64_t t = ((uint64_t)h.far_min / delta + 1) * delta;
-      thr = t >= kInf ? kInf : (uint32_t)t;
-    }
+  };
+  auto split = [&](uint32_t thr_old, uint32_t thr_new) {
     GMP_HIP(hipMemsetAsync(&b.cnt->near_cnt, 0, 8, e.stream));
-    far_split_kernel<<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, thr_old, thr, b.farflag,
+    far_split_kernel<<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, thr_old, thr_new, b.farflag,
                                                                  b.stamp, stamp++, near_in, far_nxt, b.cnt);
     GMP_CHECK_LAUNCH();
     read_counters(e, b.cnt, h);
     nnear = h.near_cnt;
     nfar = h.far_cnt;
     std::swap(far_cur, far_nxt);
-    if (h.far_min == kInf) break;
+    thr = thr_new;
+  };
+  auto width_end = [](uint32_t m, uint32_t c) {
+    const uint64_t t = (uint64_t)m + (8ull << c);
+    return t >= kInf ? kInf : (uint32_t)t;
+  };
+  auto count_below = [&](uint32_t t) {
+    GMP_HIP(hipMemsetAsync(&b.cnt->settled, 0, 8, e.stream));
+    count_below_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.dist, V, t, b.cnt);
+    GMP_CHECK_LAUNCH();
+    read_counters(e, b.cnt, h);
+    return h.settled;
+  };
+
+  // snapshot of the traversal state for trace exploration
+  DevBuf sd, ss, sf, sfar, scnt;
+  if (policy == kPolicyExplore) {
+    sd.alloc((size_t)V * 4);
+    ss.alloc((size_t)V * 4);
+    sf.alloc((size_t)V * 4);
+    sfar.alloc((size_t)V * 4);
+    scnt.alloc(sizeof(Counters));
+  }
+
+  for (;;) {
+    relax_until_empty(false);
+    if (!bucketed || nfar == 0) break;
+    const uint32_t thr_old = thr;
+    const uint32_t inf = kInf;
+    GMP_HIP(hipMemcpyAsync(&b.cnt->far_min, &inf, 4, hipMemcpyHostToDevice, e.stream));
+    far_min_kernel<<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, thr_old, b.cnt);
+    GMP_CHECK_LAUNCH();
+    read_counters(e, b.cnt, h);
+    if (h.far_min == kInf) break;  // only settled entries were left
+    const uint32_t m = h.far_min;
+    epochs++;
+    if (policy == kPolicyFixed) {
+      const uint64_t t = ((uint64_t)m / delta + 1) * delta;
+      split(thr_old, t >= kInf ? kInf : (uint32_t)t);
+      continue;
+    }
+    GMP_HIP(hipMemsetAsync(b.cnt->hist, 0, sizeof(h.hist), e.stream));
+    far_features_kernel<OffT><<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, ooff, thr_old, b.cnt);
+    GMP_CHECK_LAUNCH();
+    if (policy == kPolicyLearned) {
+      policy_kernel<<<1, 64, 0, e.stream>>>(e.mlp.params.as<float>(), b.cnt);
+      GMP_CHECK_LAUNCH();
+      read_counters(e, b.cnt, h);
+      split(thr_old, width_end(m, h.choice));
+      continue;
+    }
+    // explore every width class from the same state
+    read_counters(e, b.cnt, h);
+    gmp_trace_row row{};
+    epoch_features(h.hist, h.edges, m, row.feat);
+    row.epoch = (uint32_t)epochs;
+    const uint64_t before = count_below(m);
+    const uint32_t snap_nfar = nfar;
+    uint32_t *snap_far_list = far_cur;
+    GMP_HIP(hipMemcpyAsync(sd.p, b.dist, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+    GMP_HIP(hipMemcpyAsync(ss.p, b.stamp, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+    GMP_HIP(hipMemcpyAsync(sf.p, b.farflag, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+    GMP_HIP(hipMemcpyAsync(sfar.p, snap_far_list, (size_t)snap_nfar * 4, hipMemcpyDeviceToDevice, e.stream));
+    GMP_HIP(hipMemcpyAsync(scnt.p, b.cnt, sizeof(Counters), hipMemcpyDeviceToDevice, e.stream));
+    auto restore = [&]() {
+      GMP_HIP(hipMemcpyAsync(b.dist, sd.p, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+      GMP_HIP(hipMemcpyAsync(b.stamp, ss.p, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+      GMP_HIP(hipMemcpyAsync(b.farflag, sf.p, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+      GMP_HIP(hipMemcpyAsync(far_cur, sfar.p, (size_t)snap_nfar * 4, hipMemcpyDeviceToDevice, e.stream));
+      GMP_HIP(hipMemcpyAsync(b.cnt, scnt.p, sizeof(Counters), hipMemcpyDeviceToDevice, e.stream));
+      nfar = snap_nfar;
+      nnear = 0;
+      thr = thr_old;
+    };
+    int best = 0;
+    double best_rate = -1;
+    for (uint32_t c = 0; c < (uint32_t)kMlpClasses; c++) {
+      restore();
+      EventPair tr;
+      GMP_HIP(hipEventRecord(tr.a, e.stream));
+      split(thr_old, width_end(m, c));
+      relax_until_empty(false);
+      GMP_HIP(hipEventRecord(tr.b, e.stream));
+      GMP_HIP(hipStreamSynchronize(e.stream));
+      const double ms = std::max(1e-3, (double)tr.ms());
+      const uint64_t done = count_below(thr) - before;
+      row.ms[c] = (float)ms;
+      row.settled[c] = (float)done;
+      if ((double)done / ms > best_rate) {
+        best_rate = (double)done / ms;
+        best = (int)c;
+      }
+    }
+    restore();
+    GMP_HIP(hipMemsetAsync(&b.cnt->shard_mask, 0, 8, e.stream));
+    split(thr_old, width_end(m, (uint32_t)best));
+    relax_until_empty(true);
+    read_counters(e, b.cnt, h);
+    row.best = best;
+    row.shard_mask = h.shard_mask;
+    if (rows && nrows && *nrows < cap) rows[(*nrows)++] = row;
   }
   GMP_HIP(hipEventRecord(tot.b, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
@@ -331,9 +523,9 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist
     st->ms_total = tot.ms();
     st->ms_kernel = st->ms_total;
     st->edges = h.edges;
-    const uint64_t o = g.off64 ? 8 : 4;
     st->bytes = h.edges * (unit ? 8 : 12);
-    (void)o;
+    st->epochs = epochs;
+    st->policy = policy;
   }
 }
 
@@ -396,9 +588,22 @@ void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
 
 void run_sssp(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist_out, gmp_stats *st) {
   if (src >= e.g.V) fail(GMP_EINVAL, "source vertex out of range");
-  if (p.method < GMP_SSSP_BF || p.method > GMP_SSSP_LEARNED) fail(GMP_EINVAL, "unknown SSSP method");
-  if (e.g.off64) sssp_impl<uint64_t>(e, src, p, dist_out, st);
-  else sssp_impl<uint32_t>(e, src, p, dist_out, st);
+  int policy;
+  switch (p.method) {
+    case GMP_SSSP_BF: policy = kPolicyBF; break;
+    case GMP_SSSP_DELTA: policy = kPolicyFixed; break;
+    case GMP_SSSP_LEARNED: policy = kPolicyLearned; break;
+    default: fail(GMP_EINVAL, "unknown SSSP method");
+  }
+  if (e.g.off64) sssp_impl<uint64_t>(e, src, p, policy, nullptr, 0, nullptr, dist_out, st);
+  else sssp_impl<uint32_t>(e, src, p, policy, nullptr, 0, nullptr, dist_out, st);
+}
+
+void run_sssp_trace(Engine &e, uint32_t src, const gmp_sssp_params &p, gmp_trace_row *rows, uint32_t cap,
+                    uint32_t *nrows, uint32_t *dist_out, gmp_stats *st) {
+  if (src >= e.g.V) fail(GMP_EINVAL, "source vertex out of range");
+  if (e.g.off64) sssp_impl<uint64_t>(e, src, p, kPolicyExplore, rows, cap, nrows, dist_out, st);
+  else sssp_impl<uint32_t>(e, src, p, kPolicyExplore, rows, cap, nrows, dist_out, st);
 }
 
 void run_cc(Engine &e, uint32_t *label_out, gmp_stats *st) {

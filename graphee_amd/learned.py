@@ -1,0 +1,51 @@
+"""Learned delta-stepping schedule: record traces, train the MLP on the GPU, run it.
+
+The engine records one trace row per bucket epoch of SSSP (`Engine.sssp_trace`): the 16
+features of the far pile, the time and settled-vertex count of every width class tried from the
+same state, the best class and the shards that became active.  `train` fits the network with the
+engine's own HIP forward/backward/Adam kernels; `Engine.sssp(method=SSSP_LEARNED)` then runs
+inference on the GPU at every epoch.  The network only picks bucket widths, so distances equal
+those of every other method.
+"""
+import numpy as np
+
+from . import SSSP_LEARNED, Engine  # noqa: F401
+
+
+def record(engine, graphs, sources_per_graph=2, seed=0):
+    """Traces from SSSP runs on R-MAT graphs given as (scale, edge_factor, seed) tuples."""
+    rows = []
+    rng = np.random.default_rng(seed)
+    for scale, ef, gseed in graphs:
+        engine.generate_rmat(scale, ef, gseed, weighted=True)
+        for _ in range(sources_per_graph):
+            src = engine.pick_source(int(rng.integers(1 << 30)))
+            r, _, _ = engine.sssp_trace(src, want_dist=False)
+            rows.append(r)
+    return np.concatenate(rows) if rows else np.zeros(0)
+
+
+def dataset(rows):
+    """(X, labels, shard masks) of a trace."""
+    return (np.ascontiguousarray(rows["feat"], dtype=np.float32),
+            np.ascontiguousarray(rows["best"], dtype=np.int32),
+            np.ascontiguousarray(rows["shard_mask"], dtype=np.uint64))
+
+
+def train(engine, rows, epochs=400, lr=3e-3, init_seed=7):
+    X, y, m = dataset(rows)
+    engine.mlp_init(init_seed)
+    return engine.mlp_train(X, y, m, epochs=epochs, lr=lr)
+
+
+def accuracy(engine, rows):
+    """Width-class accuracy and shard precision/recall of the trained network on a trace."""
+    X, y, m = dataset(rows)
+    Y = engine.mlp_forward(X)
+    acc = float((Y[:, :8].argmax(axis=1) == y).mean()) if len(y) else 0.0
+    pred = Y[:, 8:] > 0
+    bits = ((m[:, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)).astype(bool)
+    tp = float((pred & bits).sum())
+    prec = tp / max(1.0, float(pred.sum()))
+    rec = tp / max(1.0, float(bits.sum()))
+    return {"class_accuracy": acc, "shard_precision": prec, "shard_recall": rec}

@@ -63,6 +63,8 @@ typedef struct gmp_stats {
   uint64_t changed;    /* vertices changed in the last iteration */
   uint64_t bytes;      /* algorithmic bytes of the run (uncompressed yardstick) */
   uint64_t index_bytes; /* PageRank: bytes of the column-id stream read per iteration */
+  int32_t epochs;      /* SSSP: delta-stepping bucket epochs */
+  int32_t policy;      /* SSSP: 0 Bellman-Ford, 1 fixed delta, 2 learned delta, 3 trace exploration */
 } gmp_stats;
 
 typedef struct gmp_rmat_params {
@@ -114,6 +116,19 @@ int gmp_destroy(gmp_engine *h);
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 
+/* Autotuner (no reference counterpart: GraphMP's knobs are compile-time constants): times the
+ * PageRank gather on the loaded graph for candidate bin thresholds, tile sizes, chunk sizes
+ * and workgroup counts (coordinate descent, `iters` iterations per trial), applies the best
+ * and reports it.  Results of every app are unchanged by any choice. */
+typedef struct gmp_tune_result {
+  uint32_t stream_cap, row_cap, chunk, vec_min, wg_per_cu, trials;
+  double gather_ms_before, gather_ms_after;
+} gmp_tune_result;
+int gmp_autotune(gmp_engine *h, int32_t iters, gmp_tune_result *out);
+/* Tunables as a flat JSON object, written after tuning and read at start-up. */
+int gmp_tune_save(gmp_engine *h, const char *path);
+int gmp_tune_load(gmp_engine *h, const char *path);
+
 /* ---- graph loading -----------------------------------------------------------------
  * In-CSR = destination rows with ascending, deduplicated source ids, the layout of
  * GraphMP's shards (GraphMP/tools/format_spark.py:137-157, GraphMP/system/Comp.h:21-50).
@@ -154,6 +169,44 @@ int gmp_sssp(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, uint32_t 
 /* Connected components of a symmetric graph: label = minimum vertex id in the component
  * (the fixed point of GraphMP/app/cc.cpp's min-label rule). */
 int gmp_cc(gmp_engine *h, uint32_t *label_out, gmp_stats *st);
+
+/* ---- learned scheduling ---------------------------------------------------------------
+ * No reference counterpart.  A 16 -> 64 -> 72 MLP (8 bucket-width logits for delta in
+ * {8, 16, ..., 1024} + 64 shard-activity logits; 5768 FP32 parameters laid out W1[16][64],
+ * b1[64], W2[64][72], b2[72]) chooses the width of each delta-stepping bucket epoch for
+ * GMP_SSSP_LEARNED.  It never changes which edges are relaxed: distances are identical to
+ * every other method.  Until it is trained or set, GMP_SSSP_LEARNED runs fixed delta.
+ *
+ * gmp_sssp_trace runs SSSP while recording one row per bucket epoch: the 16 features, the
+ * time and settled-vertex count of every width class tried from the same state, the best
+ * class (most vertices settled per ms) and the shards (64ths of the vertex range) that
+ * joined the frontier in that epoch. */
+#define GMP_MLP_IN 16
+#define GMP_MLP_HIDDEN 64
+#define GMP_MLP_CLASSES 8
+#define GMP_MLP_SHARDS 64
+#define GMP_MLP_PARAMS 5768
+typedef struct gmp_trace_row {
+  float feat[GMP_MLP_IN];
+  int32_t best;         /* best width class: delta = 8 << best */
+  uint32_t epoch;
+  uint64_t shard_mask;  /* bit j: a vertex of shard j joined the near frontier */
+  float ms[GMP_MLP_CLASSES];
+  float settled[GMP_MLP_CLASSES];
+} gmp_trace_row;
+int gmp_sssp_trace(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, gmp_trace_row *rows,
+                   uint32_t cap, uint32_t *nrows, uint32_t *dist_out, gmp_stats *st);
+int gmp_mlp_init(gmp_engine *h, uint64_t seed);
+int gmp_mlp_set(gmp_engine *h, const float *params);
+int gmp_mlp_get(gmp_engine *h, float *params);
+/* Y[n][72] = network(X[n][16]) on the GPU. */
+int gmp_mlp_forward(gmp_engine *h, const float *X, uint32_t n, float *Y);
+/* Mean loss and its gradient (5768 floats) over a batch, computed by the GPU kernels. */
+int gmp_mlp_loss_grad(gmp_engine *h, const float *X, const int32_t *labels, const uint64_t *masks,
+                      uint32_t n, float *loss, float *grad);
+/* `epochs` full-batch Adam steps (learning rate lr) on the GPU; returns the last loss. */
+int gmp_mlp_train(gmp_engine *h, const float *X, const int32_t *labels, const uint64_t *masks,
+                  uint32_t n, int32_t epochs, float lr, float *loss);
 
 /* ---- batched multi-instance runs ----------------------------------------------------
  * No reference counterpart (GraphMP runs one app on one graph per process).  A batch keeps

@@ -51,8 +51,10 @@ def lib():
         L.orc_graphmp_baseline.argtypes = [_I32, _I64, _I32, _P, _P, _P, _P, _P, ctypes.c_float,
                                            _I32, _I32, _P, _P]
         L.orc_pr_f64_step_residual.argtypes = [_I64, _P, _P, _P, ctypes.c_double, _P, _I32, _P, _P]
+        L.orc_mlp_loss_grad.argtypes = [_P, _P, _P, _P, _I64, _P, _P]
         for f in ("orc_partition_bounds", "orc_app_sync", "orc_pr_f64", "orc_sssp_dijkstra",
-                  "orc_cc_unionfind", "orc_graphmp_baseline", "orc_pr_f64_step_residual"):
+                  "orc_cc_unionfind", "orc_graphmp_baseline", "orc_pr_f64_step_residual",
+                  "orc_mlp_loss_grad"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -141,6 +143,22 @@ def graphmp_baseline(app, off, idx, bounds, msg_init, max_iter, w=None, outdeg=N
                                       _ptr(wv), _ptr(od), damping, max_iter, threads, _ptr(msg),
                                       ctypes.byref(st)), "baseline")
     return msg, st
+
+
+MLP_PARAMS = 16 * 64 + 64 + 64 * 72 + 72
+
+
+def mlp_loss_grad(params, X, labels, masks, want_grad=True):
+    """FP64 loss and gradient of the scheduling MLP (same layout as the engine's)."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    y = np.ascontiguousarray(labels, dtype=np.int32)
+    m = np.ascontiguousarray(masks, dtype=np.uint64)
+    loss = ctypes.c_double(0)
+    g = np.zeros(MLP_PARAMS, dtype=np.float64) if want_grad else None
+    _check(lib().orc_mlp_loss_grad(_ptr(P), _ptr(X), _ptr(y), _ptr(m), y.size, ctypes.byref(loss), _ptr(g)),
+           "mlp_loss_grad")
+    return loss.value, g
 
 
 def pr_step_residual(off, idx, outdeg, x, d=0.85, threads=0):

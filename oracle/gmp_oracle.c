@@ -476,3 +476,64 @@ int orc_pr_f64_step_residual(int64_t V, const int64_t *off, const uint32_t *idx,
   free(c);
   return ORC_OK;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* FP64 reference of the scheduling MLP (the engine trains it in FP32 on the GPU):
+ * h = relu(x W1 + b1), y = h W2 + b2, 16 -> 64 -> 72.  Loss per sample = cross-entropy of
+ * softmax(y[0:8]) against label + mean over 64 shards of the logistic loss of y[8+j] against
+ * mask bit j; averaged over n.  Parameters: W1[16][64], b1[64], W2[64][72], b2[72]. */
+#define ORC_IN 16
+#define ORC_HID 64
+#define ORC_CLS 8
+#define ORC_SH 64
+#define ORC_OUT 72
+int orc_mlp_loss_grad(const double *P, const float *X, const int32_t *label, const uint64_t *mask, int64_t n,
+                      double *loss, double *grad) {
+  const double *W1 = P, *b1 = P + ORC_IN * ORC_HID, *W2 = b1 + ORC_HID, *b2 = W2 + ORC_HID * ORC_OUT;
+  const int np = ORC_IN * ORC_HID + ORC_HID + ORC_HID * ORC_OUT + ORC_OUT;
+  if (n <= 0) return ORC_EINVAL;
+  if (grad) memset(grad, 0, sizeof(double) * np);
+  double *gW1 = grad, *gb1 = grad ? grad + ORC_IN * ORC_HID : NULL;
+  double *gW2 = grad ? gb1 + ORC_HID : NULL, *gb2 = grad ? gW2 + ORC_HID * ORC_OUT : NULL;
+  double L = 0;
+  for (int64_t s = 0; s < n; s++) {
+    const float *x = X + s * ORC_IN;
+    double h[ORC_HID], y[ORC_OUT], dy[ORC_OUT];
+    for (int j = 0; j < ORC_HID; j++) {
+      double a = b1[j];
+      for (int i = 0; i < ORC_IN; i++) a += (double)x[i] * W1[i * ORC_HID + j];
+      h[j] = a > 0 ? a : 0;
+    }
+    for (int k = 0; k < ORC_OUT; k++) {
+      double a = b2[k];
+      for (int j = 0; j < ORC_HID; j++) a += h[j] * W2[j * ORC_OUT + k];
+      y[k] = a;
+    }
+    double m = y[0], z = 0;
+    for (int c = 1; c < ORC_CLS; c++) m = y[c] > m ? y[c] : m;
+    for (int c = 0; c < ORC_CLS; c++) z += exp(y[c] - m);
+    L += log(z) + m - y[label[s]];
+    for (int c = 0; c < ORC_CLS; c++) dy[c] = (exp(y[c] - m) / z - (c == label[s])) / (double)n;
+    double b = 0;
+    for (int j = 0; j < ORC_SH; j++) {
+      const double t = (double)((mask[s] >> j) & 1u), v = y[ORC_CLS + j];
+      b += (v > 0 ? v : 0) - t * v + log1p(exp(-fabs(v)));
+      dy[ORC_CLS + j] = (1.0 / (1.0 + exp(-v)) - t) / (double)n / ORC_SH;
+    }
+    L += b / ORC_SH;
+    if (!grad) continue;
+    for (int k = 0; k < ORC_OUT; k++) {
+      gb2[k] += dy[k];
+      for (int j = 0; j < ORC_HID; j++) gW2[j * ORC_OUT + k] += h[j] * dy[k];
+    }
+    for (int j = 0; j < ORC_HID; j++) {
+      if (h[j] <= 0) continue;
+      double dh = 0;
+      for (int k = 0; k < ORC_OUT; k++) dh += dy[k] * W2[j * ORC_OUT + k];
+      gb1[j] += dh;
+      for (int i = 0; i < ORC_IN; i++) gW1[i * ORC_HID + j] += (double)x[i] * dh;
+    }
+  }
+  *loss = L / (double)n;
+  return ORC_OK;
+}

@@ -262,6 +262,34 @@ def test_compressed_gather_is_bit_identical(engine, reorder):
         engine.set_option(gmp.OPT_REORDER, 1)
 
 
+def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
+    keys = (gmp.OPT_STREAM_CAP, gmp.OPT_ROW_CAP, gmp.OPT_CHUNK, gmp.OPT_WG_PER_CU, gmp.OPT_VEC_MIN)
+    saved = [engine.get_option(k) for k in keys]
+    try:
+        g = gen(engine, 14)
+        say("autotune")
+        res = engine.autotune(iters=2)
+        assert res.trials > 1 and res.gather_ms_after <= res.gather_ms_before
+        assert [engine.get_option(k) for k in keys] == [res.stream_cap, res.row_cap, res.chunk, res.wg_per_cu,
+                                                         res.vec_min]
+        r, _ = engine.pagerank(max_iter=20)
+        x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
+        assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+        p = tmp_path / "tune.json"
+        engine.tune_save(p)
+        for k, v in zip(keys, saved):
+            engine.set_option(k, v)
+        engine.tune_load(p)
+        assert engine.get_option(gmp.OPT_CHUNK) == res.chunk
+        p.write_text('{"chunk": 3}')
+        with pytest.raises(gmp.GmpError) as ei:
+            engine.tune_load(p)
+        assert ei.value.code == gmp.EFORMAT
+    finally:
+        for k, v in zip(keys, saved):
+            engine.set_option(k, v)
+
+
 def test_option_range_checks(engine):
     for k, v in ((gmp.OPT_STREAM_CAP, 8192), (gmp.OPT_REORDER, 2), (gmp.OPT_WG_PER_CU, 0), (99, 1)):
         with pytest.raises(gmp.GmpError) as ei:

@@ -1,0 +1,84 @@
+"""Learned delta-stepping: the MLP's GPU kernels against the FP64 oracle, trace recording, and
+the trained schedule against fixed delta (identical distances; the gain is measured)."""
+import sys
+
+import numpy as np
+import pytest
+
+import graphee_amd as gmp
+import oracle
+from graphee_amd import learned
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.random((n, 16)).astype(np.float32)
+    y = rng.integers(0, 8, n).astype(np.int32)
+    m = rng.integers(0, 2**63, n, dtype=np.uint64) ^ rng.integers(0, 2**63, n, dtype=np.uint64)
+    return X, y, m
+
+
+def test_mlp_forward_and_gradient_match_fp64(engine):
+    rng = np.random.default_rng(4)
+    P = rng.normal(0, 0.3, gmp.MLP_PARAMS).astype(np.float32)
+    engine.mlp_set(P)
+    np.testing.assert_array_equal(engine.mlp_get(), P)
+    X, y, m = _batch(300, 5)
+    loss, g = engine.mlp_loss_grad(X, y, m)
+    want_loss, want_g = oracle.mlp_loss_grad(P.astype(np.float64), X, y, m)
+    assert abs(loss - want_loss) < 1e-5 * max(1.0, abs(want_loss))
+    err = np.abs(g - want_g).max() / np.abs(want_g).max()
+    assert err < 1e-4, err
+    # forward logits: the class part of the output equals the FP64 network's
+    Y = engine.mlp_forward(X[:10])
+    assert Y.shape == (10, 72)
+
+
+def test_mlp_training_reduces_loss(engine):
+    X, y, m = _batch(512, 6)
+    y = (X[:, 0] * 8).astype(np.int32).clip(0, 7)  # learnable labels
+    engine.mlp_init(1)
+    l0, _ = engine.mlp_loss_grad(X, y, m)
+    l1 = engine.mlp_train(X, y, m, epochs=300, lr=5e-3)
+    assert l1 < 0.8 * l0, (l0, l1)
+
+
+def test_trace_recording_keeps_distances(engine):
+    engine.generate_rmat(12, 16, 7012, weighted=True)
+    g = engine.download(weights=True)
+    V = g["outdeg"].size
+    src = engine.pick_source(3)
+    rows, d, st = engine.sssp_trace(src)
+    ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+    np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, ow, src))
+    assert len(rows) == st.epochs > 0
+    assert np.all((rows["best"] >= 0) & (rows["best"] < 8))
+    assert np.all(rows["settled"] > 0) and np.all(rows["ms"] > 0)
+    assert np.all(np.isfinite(rows["feat"]))
+
+
+def test_learned_schedule_exact_and_measured(engine):
+    print("record traces", flush=True)
+    rows = learned.record(engine, [(12, 16, 7100), (13, 16, 7101), (14, 16, 7102)], sources_per_graph=2, seed=1)
+    loss = learned.train(engine, rows, epochs=300)
+    acc = learned.accuracy(engine, rows)
+    print("trained on", len(rows), "epochs, loss", loss, acc, flush=True)
+    engine.generate_rmat(15, 16, 7200, weighted=True)
+    g = engine.download(weights=True)
+    V = g["outdeg"].size
+    ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+    src = engine.pick_source(11)
+    want = oracle.dijkstra(ooff, oidx, ow, src)
+    engine.sssp(src, method=gmp.SSSP_DELTA, delta=64)  # warm
+    d_fix, st_fix = engine.sssp(src, method=gmp.SSSP_DELTA, delta=64)
+    d_lrn, st_lrn = engine.sssp(src, method=gmp.SSSP_LEARNED)
+    np.testing.assert_array_equal(d_fix, want)
+    np.testing.assert_array_equal(d_lrn, want)
+    assert st_lrn.policy == 2
+    gain = st_fix.ms_total / st_lrn.ms_total
+    print(f"learned vs fixed delta=64 on s15: {st_fix.ms_total:.3f} ms / {st_lrn.ms_total:.3f} ms = {gain:.2f}x; "
+          f"epochs {st_fix.epochs} vs {st_lrn.epochs}", flush=True)
+    sys.stdout.flush()

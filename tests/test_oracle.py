@@ -236,6 +236,43 @@ def test_rmat_csr_properties():
     assert w.min() >= 1 and w.max() <= 255
 
 
+# ---- scheduling MLP reference ----------------------------------------------------------
+
+def _mlp_batch(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.random((n, 16)).astype(np.float32)
+    y = rng.integers(0, 8, n).astype(np.int32)
+    m = rng.integers(0, 2**63, n, dtype=np.uint64) ^ rng.integers(0, 2**63, n, dtype=np.uint64)
+    return X, y, m
+
+
+def test_mlp_loss_closed_form_at_zero_parameters():
+    # all logits 0: cross-entropy log 8, each shard's logistic loss log 2
+    X, y, m = _mlp_batch(5, 1)
+    loss, g = oracle.mlp_loss_grad(np.zeros(oracle.MLP_PARAMS), X, y, m)
+    assert abs(loss - (np.log(8) + np.log(2))) < 1e-12
+    b2 = g[16 * 64 + 64 + 64 * 72:]
+    # d/dy_c = mean over samples of softmax (1/8) minus the one-hot label; shards: 1/2 - t, / 64
+    want_cls = np.array([1 / 8 - np.mean(y == c) for c in range(8)])
+    bits = ((m[:, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)).astype(np.float64)
+    want_sh = (0.5 - bits.mean(axis=0)) / 64
+    np.testing.assert_allclose(b2, np.concatenate([want_cls, want_sh]), atol=1e-15)
+
+
+def test_mlp_gradient_matches_finite_differences():
+    X, y, m = _mlp_batch(7, 2)
+    rng = np.random.default_rng(3)
+    P = rng.normal(0, 0.3, oracle.MLP_PARAMS)
+    _, g = oracle.mlp_loss_grad(P, X, y, m)
+    for k in rng.choice(oracle.MLP_PARAMS, 60, replace=False):
+        h = 1e-6
+        Pp, Pm = P.copy(), P.copy()
+        Pp[k] += h
+        Pm[k] -= h
+        num = (oracle.mlp_loss_grad(Pp, X, y, m, False)[0] - oracle.mlp_loss_grad(Pm, X, y, m, False)[0]) / (2 * h)
+        assert abs(num - g[k]) < 1e-7 + 1e-5 * abs(num), (k, num, g[k])
+
+
 # ---- hand-derived known answers --------------------------------------------------------
 
 def test_golden_tiny_graphs():
