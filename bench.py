@@ -76,6 +76,50 @@ def batched_workload(seed=5000):
     return specs, gs, gp, rng
 
 
+def run_learned(eng, eval_scale, nsources=8, train_scales=(22, 23, 24)):
+    """Learned delta-stepping: traces from SSSP runs at train_scales, MLP trained on the GPU,
+    then fixed delta=64 vs the learned schedule on R-MAT eval_scale for nsources seeded sources
+    (distances must be identical)."""
+    from graphee_amd import learned
+    t0 = time.perf_counter()
+    rows = learned.record(eng, [(s, 16, 1000 + s) for s in train_scales], sources_per_graph=2, seed=27)
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = learned.train(eng, rows, epochs=400)
+    t_train = time.perf_counter() - t0
+    acc = learned.accuracy(eng, rows)
+    log(f"learned: {len(rows)} trace rows in {t_rec:.1f} s, trained in {t_train:.2f} s, loss {loss:.3f}, {acc}")
+    eng.generate_rmat(eval_scale, 16, 1000 + eval_scale, weighted=True, num_shards=16)
+    V, E, _, _ = eng.info()
+    rng = np.random.default_rng(2700 + eval_scale)
+    srcs = [eng.pick_source(int(rng.integers(1 << 30))) for _ in range(nsources)]
+    eng.sssp(srcs[0], method=gmp_const("SSSP_DELTA"), delta=64, want_dist=False)  # builds layouts
+    t_fix = t_lrn = 0.0
+    same = True
+    widths = []
+    for s in srcs:
+        d0, s0 = eng.sssp(s, method=gmp_const("SSSP_DELTA"), delta=64)
+        d1, s1 = eng.sssp(s, method=gmp_const("SSSP_LEARNED"))
+        same = same and bool(np.array_equal(d0, d1))
+        t_fix += s0.ms_total
+        t_lrn += s1.ms_total
+        widths.append(s1.epochs)
+    log(f"learned s{eval_scale}: fixed-64 {t_fix:.1f} ms vs learned {t_lrn:.1f} ms over {nsources} sources "
+        f"-> {t_fix / t_lrn:.2f}x, identical distances: {same}")
+    return {"eval_scale": eval_scale, "vertices": V, "edges": E, "sources": nsources,
+            "fixed_delta64_ms": round(t_fix, 3), "learned_ms": round(t_lrn, 3),
+            "speedup": round(t_fix / t_lrn, 3), "identical_distances": same,
+            "learned_epochs": widths, "trace_rows": int(len(rows)), "train_scales": list(train_scales),
+            "trace_seconds": round(t_rec, 1), "train_seconds": round(t_train, 2), "train_loss": round(loss, 4),
+            "train_class_accuracy": round(acc["class_accuracy"], 3),
+            "shard_precision": round(acc["shard_precision"], 3), "shard_recall": round(acc["shard_recall"], 3)}
+
+
+def gmp_const(name):
+    import graphee_amd
+    return getattr(graphee_amd, name)
+
+
 def run_batched(eng):
     specs, gs, gp, rng = batched_workload()
     t0 = time.perf_counter()
@@ -108,6 +152,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--no-extra", action="store_true", help="skip the SSSP and CC measurements")
+    ap.add_argument("--learn-scale", type=int, default=27, help="R-MAT scale of the learned-SSSP evaluation")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size "
                          "dispatch that scales the HBM byte counters under rocprofv3 --pmc)")
@@ -241,6 +286,7 @@ def main():
 
     if not args.no_extra:
         result["extra"]["batched"] = run_batched(eng)
+        result["extra"]["learned_sssp"] = run_learned(eng, args.learn_scale)
 
     eng.close()
     print(json.dumps(result), flush=True)

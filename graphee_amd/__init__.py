@@ -78,7 +78,7 @@ class SsspParams(ctypes.Structure):
 MLP_IN, MLP_HIDDEN, MLP_CLASSES, MLP_SHARDS, MLP_PARAMS = 16, 64, 8, 64, 5768
 
 TRACE_DTYPE = np.dtype([("feat", np.float32, (16,)), ("best", np.int32), ("epoch", np.uint32),
-                        ("shard_mask", np.uint64), ("ms", np.float32, (8,)), ("settled", np.float32, (8,))])
+                        ("shard_mask", np.uint64), ("ms", np.float32, (8,)), ("relaxed", np.float32, (8,))])
 
 
 class TuneResult(ctypes.Structure):

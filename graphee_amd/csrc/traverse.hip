@@ -434,83 +434,104 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
     scnt.alloc(sizeof(Counters));
   }
 
-  for (;;) {
-    relax_until_empty(false);
-    if (!bucketed || nfar == 0) break;
-    const uint32_t thr_old = thr;
+  // smallest live far distance, or kInf when only settled entries are left
+  auto far_min = [&]() {
     const uint32_t inf = kInf;
     GMP_HIP(hipMemcpyAsync(&b.cnt->far_min, &inf, 4, hipMemcpyHostToDevice, e.stream));
-    far_min_kernel<<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, thr_old, b.cnt);
+    far_min_kernel<<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, thr, b.cnt);
     GMP_CHECK_LAUNCH();
     read_counters(e, b.cnt, h);
-    if (h.far_min == kInf) break;  // only settled entries were left
-    const uint32_t m = h.far_min;
+    return h.far_min;
+  };
+  // the rest of the run with one constant width class (trace rollouts)
+  auto run_constant = [&](uint32_t m, uint32_t c) {
+    for (;;) {
+      split(thr, width_end(m, c));
+      relax_until_empty(false);
+      if (nfar == 0) return;
+      m = far_min();
+      if (m == kInf) return;
+    }
+  };
+  // explored epochs: the first four, then powers of two
+  auto checkpoint = [](int ep) { return ep <= 4 || (ep & (ep - 1)) == 0; };
+  uint32_t cur_class = 3;  // width 64 until the first exploration
+
+  relax_until_empty(false);
+  while (bucketed && nfar) {
+    const uint32_t thr_old = thr;
+    const uint32_t m = far_min();
+    if (m == kInf) break;  // only settled entries were left
     epochs++;
     if (policy == kPolicyFixed) {
       const uint64_t t = ((uint64_t)m / delta + 1) * delta;
       split(thr_old, t >= kInf ? kInf : (uint32_t)t);
+      relax_until_empty(false);
       continue;
     }
-    GMP_HIP(hipMemsetAsync(b.cnt->hist, 0, sizeof(h.hist), e.stream));
-    far_features_kernel<OffT><<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, ooff, thr_old, b.cnt);
-    GMP_CHECK_LAUNCH();
+    const bool explore = policy == kPolicyExplore && checkpoint(epochs);
+    if (policy == kPolicyLearned || explore) {
+      GMP_HIP(hipMemsetAsync(b.cnt->hist, 0, sizeof(h.hist), e.stream));
+      far_features_kernel<OffT><<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, ooff, thr_old,
+                                                                            b.cnt);
+      GMP_CHECK_LAUNCH();
+    }
     if (policy == kPolicyLearned) {
       policy_kernel<<<1, 64, 0, e.stream>>>(e.mlp.params.as<float>(), b.cnt);
       GMP_CHECK_LAUNCH();
       read_counters(e, b.cnt, h);
-      split(thr_old, width_end(m, h.choice));
-      continue;
+      cur_class = h.choice;
     }
-    // explore every width class from the same state
-    read_counters(e, b.cnt, h);
     gmp_trace_row row{};
-    epoch_features(h.hist, h.edges, m, row.feat);
-    row.epoch = (uint32_t)epochs;
-    const uint64_t before = count_below(m);
-    const uint32_t snap_nfar = nfar;
-    uint32_t *snap_far_list = far_cur;
-    GMP_HIP(hipMemcpyAsync(sd.p, b.dist, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
-    GMP_HIP(hipMemcpyAsync(ss.p, b.stamp, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
-    GMP_HIP(hipMemcpyAsync(sf.p, b.farflag, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
-    GMP_HIP(hipMemcpyAsync(sfar.p, snap_far_list, (size_t)snap_nfar * 4, hipMemcpyDeviceToDevice, e.stream));
-    GMP_HIP(hipMemcpyAsync(scnt.p, b.cnt, sizeof(Counters), hipMemcpyDeviceToDevice, e.stream));
-    auto restore = [&]() {
-      GMP_HIP(hipMemcpyAsync(b.dist, sd.p, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
-      GMP_HIP(hipMemcpyAsync(b.stamp, ss.p, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
-      GMP_HIP(hipMemcpyAsync(b.farflag, sf.p, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
-      GMP_HIP(hipMemcpyAsync(far_cur, sfar.p, (size_t)snap_nfar * 4, hipMemcpyDeviceToDevice, e.stream));
-      GMP_HIP(hipMemcpyAsync(b.cnt, scnt.p, sizeof(Counters), hipMemcpyDeviceToDevice, e.stream));
-      nfar = snap_nfar;
-      nnear = 0;
-      thr = thr_old;
-    };
-    int best = 0;
-    double best_rate = -1;
-    for (uint32_t c = 0; c < (uint32_t)kMlpClasses; c++) {
-      restore();
-      EventPair tr;
-      GMP_HIP(hipEventRecord(tr.a, e.stream));
-      split(thr_old, width_end(m, c));
-      relax_until_empty(false);
-      GMP_HIP(hipEventRecord(tr.b, e.stream));
-      GMP_HIP(hipStreamSynchronize(e.stream));
-      const double ms = std::max(1e-3, (double)tr.ms());
-      const uint64_t done = count_below(thr) - before;
-      row.ms[c] = (float)ms;
-      row.settled[c] = (float)done;
-      if ((double)done / ms > best_rate) {
-        best_rate = (double)done / ms;
-        best = (int)c;
+    if (explore) {
+      // time every width class from this state to the end of the run
+      read_counters(e, b.cnt, h);
+      epoch_features(h.hist, h.edges, m, row.feat);
+      row.epoch = (uint32_t)epochs;
+      const uint32_t snap_nfar = nfar;
+      GMP_HIP(hipMemcpyAsync(sd.p, b.dist, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+      GMP_HIP(hipMemcpyAsync(ss.p, b.stamp, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+      GMP_HIP(hipMemcpyAsync(sf.p, b.farflag, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+      GMP_HIP(hipMemcpyAsync(sfar.p, far_cur, (size_t)snap_nfar * 4, hipMemcpyDeviceToDevice, e.stream));
+      GMP_HIP(hipMemcpyAsync(scnt.p, b.cnt, sizeof(Counters), hipMemcpyDeviceToDevice, e.stream));
+      auto restore = [&]() {
+        GMP_HIP(hipMemcpyAsync(b.dist, sd.p, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+        GMP_HIP(hipMemcpyAsync(b.stamp, ss.p, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+        GMP_HIP(hipMemcpyAsync(b.farflag, sf.p, (size_t)V * 4, hipMemcpyDeviceToDevice, e.stream));
+        GMP_HIP(hipMemcpyAsync(far_cur, sfar.p, (size_t)snap_nfar * 4, hipMemcpyDeviceToDevice, e.stream));
+        GMP_HIP(hipMemcpyAsync(b.cnt, scnt.p, sizeof(Counters), hipMemcpyDeviceToDevice, e.stream));
+        nfar = snap_nfar;
+        nnear = 0;
+        thr = thr_old;
+      };
+      double best_ms = 0;
+      const unsigned long long snap_edges = h.edges;
+      for (uint32_t c = 0; c < (uint32_t)kMlpClasses; c++) {
+        restore();
+        EventPair tr;
+        GMP_HIP(hipEventRecord(tr.a, e.stream));
+        run_constant(m, c);
+        GMP_HIP(hipEventRecord(tr.b, e.stream));
+        GMP_HIP(hipStreamSynchronize(e.stream));
+        read_counters(e, b.cnt, h);
+        row.ms[c] = std::max(1e-3f, tr.ms());
+        row.relaxed[c] = (float)(h.edges - snap_edges);  // relaxations the rollout needed
+        if (c == 0 || row.ms[c] < best_ms) {
+          best_ms = row.ms[c];
+          cur_class = c;
+        }
       }
+      restore();
+      row.best = (int32_t)cur_class;
+      GMP_HIP(hipMemsetAsync(&b.cnt->shard_mask, 0, 8, e.stream));
     }
-    restore();
-    GMP_HIP(hipMemsetAsync(&b.cnt->shard_mask, 0, 8, e.stream));
-    split(thr_old, width_end(m, (uint32_t)best));
-    relax_until_empty(true);
-    read_counters(e, b.cnt, h);
-    row.best = best;
-    row.shard_mask = h.shard_mask;
-    if (rows && nrows && *nrows < cap) rows[(*nrows)++] = row;
+    split(thr_old, width_end(m, cur_class));
+    relax_until_empty(explore);
+    if (explore) {
+      read_counters(e, b.cnt, h);
+      row.shard_mask = h.shard_mask;
+      if (rows && nrows && *nrows < cap) rows[(*nrows)++] = row;
+    }
   }
   GMP_HIP(hipEventRecord(tot.b, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));

@@ -177,9 +177,10 @@ int gmp_cc(gmp_engine *h, uint32_t *label_out, gmp_stats *st);
  * GMP_SSSP_LEARNED.  It never changes which edges are relaxed: distances are identical to
  * every other method.  Until it is trained or set, GMP_SSSP_LEARNED runs fixed delta.
  *
- * gmp_sssp_trace runs SSSP while recording one row per bucket epoch: the 16 features, the
- * time and settled-vertex count of every width class tried from the same state, the best
- * class (most vertices settled per ms) and the shards (64ths of the vertex range) that
+ * gmp_sssp_trace runs SSSP while recording a row at explored bucket epochs (the first four,
+ * then every power of two): the 16 features, and for every width class the device time and
+ * relaxations of finishing the run from that state with that constant width; the best class
+ * (fastest finish), which the run then uses, and the shards (64ths of the vertex range) that
  * joined the frontier in that epoch. */
 #define GMP_MLP_IN 16
 #define GMP_MLP_HIDDEN 64
@@ -191,8 +192,8 @@ typedef struct gmp_trace_row {
   int32_t best;         /* best width class: delta = 8 << best */
   uint32_t epoch;
   uint64_t shard_mask;  /* bit j: a vertex of shard j joined the near frontier */
-  float ms[GMP_MLP_CLASSES];
-  float settled[GMP_MLP_CLASSES];
+  float ms[GMP_MLP_CLASSES];      /* time to finish with each constant width class */
+  float relaxed[GMP_MLP_CLASSES]; /* relaxations to finish with each class */
 } gmp_trace_row;
 int gmp_sssp_trace(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, gmp_trace_row *rows,
                    uint32_t cap, uint32_t *nrows, uint32_t *dist_out, gmp_stats *st);
