@@ -275,6 +275,24 @@ def main():
                                      "edges_relaxed": st.edges, "work_gteps": round(st.edges / st.ms_total / 1e6, 3),
                                      "iterations": st.iterations, "source": src, "reached": reached}
             log(f"{name}: {st.ms_total:.1f} ms, {st.iterations} iterations, {st.edges} relaxations")
+            if name == "sssp_delta64":
+                gpu_dist, gpu_ms = d, st.ms_total
+        if not args.no_cpu:
+            # full GraphMP-engine SSSP on the host (synchronous weighted rule, selective scheduling)
+            import oracle
+            g = eng.download(weights=True)
+            msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+            msg0[src] = 0
+            threads = min(len(os.sched_getaffinity(0)), 16)
+            dist, stc = oracle.graphmp_baseline(oracle.APP_SSSP_W, g["off"], g["idx"], g["bounds"], msg0, 10000,
+                                                w=g["w"], threads=threads)
+            result["extra"]["sssp_cpu_baseline"] = {
+                "seconds": round(stc.seconds, 3), "threads": stc.threads, "sweeps": stc.iterations,
+                "edges_gathered": stc.edges, "identical_to_gpu": bool(np.array_equal(dist, gpu_dist)),
+                "gpu_delta64_speedup": round(stc.seconds * 1e3 / gpu_ms, 2)}
+            log(f"sssp cpu baseline: {stc.seconds:.2f} s ({stc.iterations} sweeps) -> GPU delta-64 is "
+                f"{stc.seconds * 1e3 / gpu_ms:.1f}x; identical: {np.array_equal(dist, gpu_dist)}")
+            del g, dist
         eng.generate_rmat(scale, ef, seed, undirected=True, num_shards=16)
         Vs, Es, _, _ = eng.info()
         eng.cc(want_labels=False)
@@ -283,6 +301,19 @@ def main():
         result["extra"]["cc"] = {"ms": round(st.ms_total, 3), "edges_symmetric": Es, "iterations": st.iterations,
                                  "gteps": round(Es / st.ms_total / 1e6, 3), "components": ncomp}
         log(f"cc: {st.ms_total:.1f} ms, {st.iterations} iterations, {ncomp} components")
+        if not args.no_cpu:
+            import oracle
+            g = eng.download()
+            threads = min(len(os.sched_getaffinity(0)), 16)
+            labc, stc = oracle.graphmp_baseline(oracle.APP_CC, g["off"], g["idx"], g["bounds"],
+                                                np.arange(Vs, dtype=np.uint32), 10000, threads=threads)
+            result["extra"]["cc_cpu_baseline"] = {
+                "seconds": round(stc.seconds, 3), "threads": stc.threads, "sweeps": stc.iterations,
+                "edges_gathered": stc.edges, "identical_to_gpu": bool(np.array_equal(labc, lab)),
+                "gpu_speedup": round(stc.seconds * 1e3 / st.ms_total, 2)}
+            log(f"cc cpu baseline: {stc.seconds:.2f} s ({stc.iterations} sweeps) -> GPU is "
+                f"{stc.seconds * 1e3 / st.ms_total:.1f}x; identical: {np.array_equal(labc, lab)}")
+            del g, labc
 
     if not args.no_extra:
         result["extra"]["batched"] = run_batched(eng)

@@ -109,6 +109,10 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       in(1, 4096, "vec_min");
       e->opt.vec_min = (uint32_t)value;
       break;
+    case GMP_OPT_STRIP_LOG:
+      in(0, 31, "strip_log");
+      e->opt.strip_log = (uint32_t)value;
+      break;
     default:
       fail(GMP_EINVAL, "gmp_set_option: unknown key " + std::to_string(key));
   }
@@ -126,6 +130,7 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_CHUNK: *value = e->opt.chunk; break;
     case GMP_OPT_WG_PER_CU: *value = e->opt.wg_per_cu; break;
     case GMP_OPT_VEC_MIN: *value = e->opt.vec_min; break;
+    case GMP_OPT_STRIP_LOG: *value = e->opt.strip_log; break;
     default: fail(GMP_EINVAL, "gmp_get_option: unknown key " + std::to_string(key));
   }
   GMP_API_END

@@ -98,6 +98,25 @@ struct CsrView {
   template <class OffT> const OffT *offs() const { return static_cast<const OffT *>(off); }
 };
 
+// Source-strip layout of the work in-CSR for the PageRank gather: edges sorted by
+// (strip of the source, destination, source) where a strip is 2^wlog consecutive work ids.
+// A "pair" is the run of edges of one destination inside one strip; the gather sums pairs
+// strip by strip (so only one strip's contributions are live in L2 at a time) and a row-block
+// combine adds each row's pair partials in strip order.
+struct StripCsr {
+  uint32_t wlog = 0, nstrips = 0, nblk = 0;
+  uint64_t npairs = 0;
+  DevBuf pair_off;                 // uint32[npairs+1] edge offsets
+  DevBuf pair_row;                 // uint32[npairs] destination of each pair
+  DevBuf idx;                      // uint32[E] sources
+  DevBuf blk;                      // uint32[(nblk+1) * nstrips]: first pair of (row block, strip)
+  DevBuf partial;                  // float2[npairs]: per-iteration pair partials
+  std::vector<uint64_t> h_pair_off;
+  GatherPlan plan;
+  bool valid = false;
+};
+constexpr uint32_t kCombineRows = 1024;  // rows per combine block
+
 // Delta + varint column ids of the work in-CSR (decoded inside the gather).
 struct PackedCols {
   DevBuf bytes, gbyte, gmask;
@@ -130,6 +149,7 @@ struct Graph {
   std::vector<uint64_t> h_win_off;  // host copy of the work in-CSR offsets (planning)
   GatherPlan plan;                  // over the work in-CSR
   PackedCols packed;                // compressed work in-CSR columns (built on demand)
+  StripCsr strips;                  // source-strip layout (built on demand)
 
   CsrView view(const Csr &c) const { return CsrView{c.off.p, c.idx.as<uint32_t>(), c.w.as<uint32_t>()}; }
   CsrView work_in() const { return work_identity ? view(in) : view(win); }
@@ -150,6 +170,7 @@ struct Options {
   uint32_t chunk = 4096;       // gather plan: edges per hub CHUNK task
   uint32_t vec_min = 128;      // gather plan: rows above this many edges go wave-wide
   uint32_t wg_per_cu = 16;     // gather grid = wg_per_cu * CUs
+  uint32_t strip_log = 20;     // source strips of 2^strip_log ids for PageRank (0: row gather)
 };
 
 // Many graphs resident at once as one disjoint union (graph k owns union ids
@@ -240,9 +261,17 @@ void build_gather_plan_host(const uint64_t *off, uint32_t V, GatherPlan &plan,
                             std::vector<uint32_t> &tasks, std::vector<HubRow> &hubs,
                             std::vector<uint32_t> &multi);
 void ensure_plan(Engine &e);
+void plan_for(Engine &e, GatherPlan &pl, const uint64_t *off, uint32_t nrows);
 
 // ---- compress.hip ----
 void ensure_packed(Engine &e);
+
+// ---- strip.hip ----
+// Builds e.g.strips for e.opt.strip_log; returns false when the layout does not apply (strips
+// off, 64-bit offsets, or ids too wide for the sort key).
+bool ensure_strips(Engine &e);
+// Row id of every edge of a CSR (device array of E entries).
+void edge_row_ids(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, DevBuf &rowid);
 
 // ---- autotune.cpp ----
 void autotune(Engine &e, int32_t iters, gmp_tune_result *out);

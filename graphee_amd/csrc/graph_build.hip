@@ -372,26 +372,29 @@ void ensure_work(Engine &e) {
   g.work_identity = !e.opt.reorder;
   g.wout_ready = false;
   g.packed.valid = false;
+  g.strips.valid = false;
   g.plan.built = false;
   if (g.work_identity) {
     g.work_ready = true;
     return;
   }
   const uint32_t V = g.V;
-  // order: per destination interval, out-degree descending, ties by id
+  // order: out-degree descending over the whole graph, ties by id (one interval: the work
+  // layout's hottest sources then form one dense prefix for the source-strip gather)
   DevBuf keys, vals, kB, vB, dbounds, tmp;
   keys.alloc((size_t)V * 8);
   kB.alloc((size_t)V * 8);
   vals.alloc((size_t)V * 4);
   vB.alloc((size_t)V * 4);
-  dbounds.alloc((g.bounds.size()) * 4);
-  GMP_HIP(hipMemcpyAsync(dbounds.p, g.bounds.data(), g.bounds.size() * 4, hipMemcpyHostToDevice, e.stream));
-  order_keys_kernel<<<grid_for(V), kBlock, 0, e.stream>>>(g.out_deg.as<uint32_t>(), dbounds.as<uint32_t>(), g.P, V,
+  const uint32_t whole[2] = {0, V};
+  dbounds.alloc(8);
+  GMP_HIP(hipMemcpyAsync(dbounds.p, whole, 8, hipMemcpyHostToDevice, e.stream));
+  order_keys_kernel<<<grid_for(V), kBlock, 0, e.stream>>>(g.out_deg.as<uint32_t>(), dbounds.as<uint32_t>(), 1u, V,
                                                          keys.as<uint64_t>(), vals.as<uint32_t>());
   GMP_CHECK_LAUNCH();
   rocprim::double_buffer<uint64_t> dk(keys.as<uint64_t>(), kB.as<uint64_t>());
   rocprim::double_buffer<uint32_t> dv(vals.as<uint32_t>(), vB.as<uint32_t>());
-  const uint32_t endbit = 32 + bits_for((uint64_t)g.P + 1);
+  const uint32_t endbit = 32;
   size_t tb = 0;
   GMP_HIP(rocprim::radix_sort_pairs(nullptr, tb, dk, dv, (size_t)V, 0, endbit, e.stream));
   tmp.alloc(tb);
@@ -425,6 +428,12 @@ void ensure_work_out(Engine &e) {
     rebuild_csr(e, g.view(g.win), g.V, g.E, g.off64, g.weighted, nullptr, true, g.wout);
   }
   g.wout_ready = true;
+}
+
+void edge_row_ids(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, DevBuf &rowid) {
+  DevBuf tmp;
+  edge_rows(e, src, V, E, off64, rowid, tmp);
+  GMP_HIP(hipStreamSynchronize(e.stream));
 }
 
 void transpose_csr(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, bool weighted, Csr &dst) {

@@ -62,6 +62,10 @@ struct PrArgs {
   double *part_resid;
   double *part_dang;
   Packed pk;    // compressed column ids (COMP kernels)
+  float2 *partial;         // PART kernels: hi/lo sum of every (destination, strip) pair
+  const uint32_t *pair_row;  // strip combine: destination of each pair
+  const uint32_t *blk;       // strip combine: first pair of (row block, strip)
+  uint32_t nstrips, nblk, V;
   uint64_t E;
 };
 
@@ -160,7 +164,9 @@ __device__ inline void gather_range(const PrArgs<OffT> &a, uint64_t e0, uint64_t
   }
 }
 
-template <class OffT, bool KAHAN, bool COMP>
+// PART = false: rows are destinations and every row is applied.  PART = true: rows are the
+// (destination, strip) pairs of the strip layout and every row leaves its hi/lo partial.
+template <class OffT, bool KAHAN, bool COMP, bool PART = false>
 __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
   extern __shared__ float lds[];
   const uint32_t lane = lane_id(), wv = wave_id();
@@ -183,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
         const uint32_t b = (uint32_t)((uint64_t)a.off[r] - e0), en = (uint32_t)((uint64_t)a.off[r + 1] - e0);
         float s = 0, c = 0;
         for (uint32_t k = b; k < en; k++) acc_add<KAHAN>(s, c, buf[k]);
-        pr_apply(a, r, s - c, base, extra, acc);
+        if (PART) a.partial[r] = make_float2(s, -c);
+        else pr_apply(a, r, s - c, base, extra, acc);
       }
       wave_sync();
     } else {
@@ -198,7 +205,10 @@ __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
       float hi = s, lo = -c;
       wave_reduce_dd(hi, lo);
       if (lane == 0) {
-        if (h.nchunks == 1) pr_apply(a, h.row, hi, base, extra, acc);
+        if (h.nchunks == 1) {
+          if (PART) a.partial[h.row] = make_float2(hi, lo);
+          else pr_apply(a, h.row, hi, base, extra, acc);
+        }
         else a.hub_part[h.part_base + k] = make_float2(hi, lo);
       }
     }
@@ -206,8 +216,9 @@ __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
   block_store_acc(acc, a.part_resid, a.part_dang);
 }
 
-// One wave per multi-chunk hub: combine its chunk partials in chunk order, then apply.
-template <class OffT>
+// One wave per multi-chunk hub: combine its chunk partials in chunk order, then apply (or, in
+// the strip layout, store the pair's partial).
+template <class OffT, bool PART = false>
 __global__ __launch_bounds__(kBlock) void pr_hub_combine_kernel(PrArgs<OffT> a) {
   const uint32_t lane = lane_id(), wv = wave_id();
   const float base = a.sc->base, extra = a.sc->extra;
@@ -221,7 +232,39 @@ __global__ __launch_bounds__(kBlock) void pr_hub_combine_kernel(PrArgs<OffT> a) 
       dd_add(hi, lo, p.x, p.y);
     }
     wave_reduce_dd(hi, lo);
-    if (lane == 0) pr_apply(a, h.row, hi, base, extra, acc);
+    if (lane == 0) {
+      if (PART) a.partial[h.row] = make_float2(hi, lo);
+      else pr_apply(a, h.row, hi, base, extra, acc);
+    }
+  }
+  if (!PART) block_store_acc(acc, a.part_resid, a.part_dang);
+}
+
+// Strip layout: per block of kCombineRows destinations, add every row's pair partials in
+// strip order (each strip holds at most one pair per row, so no two threads touch one row's
+// accumulator within a strip), then apply the rows.
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void pr_combine_apply_kernel(PrArgs<OffT> a) {
+  __shared__ float2 accum[kCombineRows];
+  const float base = a.sc->base, extra = a.sc->extra;
+  Acc acc;
+  for (uint32_t blk = blockIdx.x; blk < a.nblk; blk += gridDim.x) {
+    const uint32_t row0 = blk * kCombineRows;
+    for (uint32_t i = threadIdx.x; i < kCombineRows; i += kBlock) accum[i] = make_float2(0.0f, 0.0f);
+    __syncthreads();
+    for (uint32_t s = 0; s < a.nstrips; s++) {
+      const uint32_t p0 = a.blk[(size_t)blk * a.nstrips + s], p1 = a.blk[(size_t)(blk + 1) * a.nstrips + s];
+      for (uint32_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
+        const uint32_t r = a.pair_row[p] - row0;
+        const float2 v = a.partial[p];
+        float2 t = accum[r];
+        dd_add(t.x, t.y, v.x, v.y);
+        accum[r] = t;
+      }
+      __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < kCombineRows && row0 + i < a.V; i += kBlock) pr_apply(a, row0 + i, accum[i].x, base, extra, acc);
+    __syncthreads();
   }
   block_store_acc(acc, a.part_resid, a.part_dang);
 }
@@ -384,12 +427,12 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   e.s_c.alloc((size_t)V * 4);
   e.s_d.alloc((size_t)V * 4);
   e.s_e.alloc(std::max<size_t>(pl.nparts, 1) * sizeof(float2));
-  e.s_part.alloc((size_t)(G + std::max<uint32_t>(Gh, 1)) * 2 * sizeof(double));
+  e.s_part.alloc((size_t)G * 4 * sizeof(double));
   e.s_scal.alloc(sizeof(PrScal) + (size_t)(max_iter + 1) * 2 * sizeof(double));
   float *rank[2] = {e.s_a.as<float>(), e.s_b.as<float>()};
   float *contrib[2] = {e.s_c.as<float>(), e.s_d.as<float>()};
   double *pr1 = e.s_part.as<double>(), *pd1 = pr1 + G;
-  double *pr2 = pd1 + G, *pd2 = pr2 + std::max<uint32_t>(Gh, 1);
+  double *pr2 = pd1 + G, *pd2 = pr2 + G;
   PrScal *sc = e.s_scal.as<PrScal>();
   double *hist = reinterpret_cast<double *>(e.s_scal.as<char>() + sizeof(PrScal));
 
@@ -431,35 +474,86 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     a.pk = Packed{g.packed.bytes.as<uint8_t>(), g.packed.gbyte.as<uint64_t>(), g.packed.gmask.as<uint64_t>()};
     a.E = g.E;
   }
-  const size_t lds = (size_t)kWavesPerBlock * (pl.stream_cap + (comp ? kVarintScratchWords : 0u)) * sizeof(float);
+  // strip layout (default for uncompressed PageRank on graphs with 32-bit offsets)
+  const bool strips = !comp && ensure_strips(e);
+  PrArgs<uint32_t> ap{};
+  uint32_t Ghs = 0, Gc = 0;
+  if (strips) {
+    StripCsr &S = g.strips;
+    S.partial.alloc(std::max<uint64_t>(S.npairs, 1) * sizeof(float2));
+    e.s_e.alloc(std::max<size_t>(std::max(pl.nparts, S.plan.nparts), 1) * sizeof(float2));
+    ap.off = S.pair_off.as<uint32_t>();
+    ap.idx = S.idx.as<uint32_t>();
+    ap.outdeg = a.outdeg;
+    ap.tasks = S.plan.tasks.as<uint32_t>();
+    ap.ntasks = S.plan.ntasks;
+    ap.hubs = S.plan.hubs.as<HubRow>();
+    ap.multi = S.plan.multi.as<uint32_t>();
+    ap.nmulti = S.plan.nmulti;
+    ap.hub_part = e.s_e.as<float2>();
+    ap.sc = sc;
+    ap.d = p.damping;
+    ap.seed = seed;
+    ap.stream_cap = S.plan.stream_cap;
+    ap.chunk = S.plan.chunk;
+    ap.partial = S.partial.as<float2>();
+    ap.pair_row = S.pair_row.as<uint32_t>();
+    ap.blk = S.blk.as<uint32_t>();
+    ap.nstrips = S.nstrips;
+    ap.nblk = S.nblk;
+    ap.V = V;
+    Ghs = S.plan.nmulti ? std::min<uint32_t>((S.plan.nmulti + kWavesPerBlock - 1) / kWavesPerBlock, G) : 0;
+    Gc = std::min<uint32_t>(S.nblk, G);
+  }
+  const uint32_t cap_lds = strips ? ap.stream_cap : pl.stream_cap;
+  const size_t lds = (size_t)kWavesPerBlock * (cap_lds + (comp ? kVarintScratchWords : 0u)) * sizeof(float);
   int cur = 0, it = 0;
   double resid = 0, dang = 0;
   const bool kahan = !(p.flags & GMP_PR_NO_KAHAN);
   for (it = 0; it < max_iter; it++) {
-    a.contrib_in = contrib[cur];
-    a.rank_in = rank[cur];
-    a.rank_out = rank[cur ^ 1];
-    a.contrib_out = contrib[cur ^ 1];
+    a.contrib_in = ap.contrib_in = contrib[cur];
+    a.rank_in = ap.rank_in = rank[cur];
+    a.rank_out = ap.rank_out = rank[cur ^ 1];
+    a.contrib_out = ap.contrib_out = contrib[cur ^ 1];
     a.part_resid = pr1;
     a.part_dang = pd1;
     GMP_HIP(hipEventRecord(ev[2 + 2 * it], e.stream));
-    if (comp) {
-      if (kahan) pr_gather_kernel<OffT, true, true><<<G, kBlock, lds, e.stream>>>(a);
-      else pr_gather_kernel<OffT, false, true><<<G, kBlock, lds, e.stream>>>(a);
-    } else {
-      if (kahan) pr_gather_kernel<OffT, true, false><<<G, kBlock, lds, e.stream>>>(a);
-      else pr_gather_kernel<OffT, false, false><<<G, kBlock, lds, e.stream>>>(a);
-    }
-    GMP_CHECK_LAUNCH();
-    GMP_HIP(hipEventRecord(ev[3 + 2 * it], e.stream));
-    if (Gh) {
-      a.part_resid = pr2;
-      a.part_dang = pd2;
-      pr_hub_combine_kernel<OffT><<<Gh, kBlock, 0, e.stream>>>(a);
+    uint32_t nfin1 = G, nfin2 = Gh;
+    if (strips) {
+      ap.part_resid = pr2;  // the pair gather applies nothing; its accumulators stay zero
+      ap.part_dang = pd2;
+      if (kahan) pr_gather_kernel<uint32_t, true, false, true><<<G, kBlock, lds, e.stream>>>(ap);
+      else pr_gather_kernel<uint32_t, false, false, true><<<G, kBlock, lds, e.stream>>>(ap);
       GMP_CHECK_LAUNCH();
+      if (Ghs) {
+        pr_hub_combine_kernel<uint32_t, true><<<Ghs, kBlock, 0, e.stream>>>(ap);
+        GMP_CHECK_LAUNCH();
+      }
+      ap.part_resid = pr1;
+      ap.part_dang = pd1;
+      pr_combine_apply_kernel<uint32_t><<<Gc, kBlock, 0, e.stream>>>(ap);
+      GMP_CHECK_LAUNCH();
+      nfin1 = Gc;
+      nfin2 = 0;
+    } else {
+      if (comp) {
+        if (kahan) pr_gather_kernel<OffT, true, true><<<G, kBlock, lds, e.stream>>>(a);
+        else pr_gather_kernel<OffT, false, true><<<G, kBlock, lds, e.stream>>>(a);
+      } else {
+        if (kahan) pr_gather_kernel<OffT, true, false><<<G, kBlock, lds, e.stream>>>(a);
+        else pr_gather_kernel<OffT, false, false><<<G, kBlock, lds, e.stream>>>(a);
+      }
+      GMP_CHECK_LAUNCH();
+      if (Gh) {
+        a.part_resid = pr2;
+        a.part_dang = pd2;
+        pr_hub_combine_kernel<OffT><<<Gh, kBlock, 0, e.stream>>>(a);
+        GMP_CHECK_LAUNCH();
+      }
     }
-    pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, G, pr2, pd2, Gh, V, p.damping, personalised, sc,
-                                                        hist, it);
+    GMP_HIP(hipEventRecord(ev[3 + 2 * it], e.stream));
+    pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, nfin1, pr2, pd2, nfin2, V, p.damping, personalised,
+                                                        sc, hist, it);
     GMP_CHECK_LAUNCH();
     cur ^= 1;
     if (p.tol > 0) {

@@ -72,10 +72,8 @@ void build_gather_plan_host(const uint64_t *off, uint32_t V, GatherPlan &plan,
   plan.nparts = parts;
 }
 
-void ensure_plan(Engine &e) {
-  ensure_work(e);
-  Graph &g = e.g;
-  GatherPlan &pl = g.plan;
+// (Re)builds `pl` over `nrows` rows with offsets `off` when the plan options changed.
+void plan_for(Engine &e, GatherPlan &pl, const uint64_t *off, uint32_t nrows) {
   if (pl.built && pl.stream_cap == e.opt.stream_cap && pl.row_cap == e.opt.row_cap && pl.chunk == e.opt.chunk &&
       pl.vec_min == e.opt.vec_min)
     return;
@@ -85,7 +83,7 @@ void ensure_plan(Engine &e) {
   pl.vec_min = e.opt.vec_min;
   std::vector<uint32_t> tasks, multi;
   std::vector<HubRow> hubs;
-  build_gather_plan_host(g.work_h_off().data(), g.V, pl, tasks, hubs, multi);
+  build_gather_plan_host(off, nrows, pl, tasks, hubs, multi);
   pl.tasks.alloc(std::max<size_t>(tasks.size(), 2) * 4);
   pl.hubs.alloc(std::max<size_t>(hubs.size(), 1) * sizeof(HubRow));
   pl.multi.alloc(std::max<size_t>(multi.size(), 1) * 4);
@@ -97,6 +95,11 @@ void ensure_plan(Engine &e) {
     GMP_HIP(hipMemcpyAsync(pl.multi.p, multi.data(), multi.size() * 4, hipMemcpyHostToDevice, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
   pl.built = true;
+}
+
+void ensure_plan(Engine &e) {
+  ensure_work(e);
+  plan_for(e, e.g.plan, e.g.work_h_off().data(), e.g.V);
 }
 
 }  // namespace gmp

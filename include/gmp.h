@@ -113,6 +113,7 @@ int gmp_destroy(gmp_engine *h);
 #define GMP_OPT_CHUNK 4      /* gather plan: edges per hub chunk, 64..2^20 */
 #define GMP_OPT_WG_PER_CU 5  /* gather grid: workgroups per compute unit, 1..64 */
 #define GMP_OPT_VEC_MIN 6    /* gather plan: rows above this many edges are reduced wave-wide */
+#define GMP_OPT_STRIP_LOG 7  /* PageRank: source strips of 2^k ids (default 20; 0 = row gather) */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 

@@ -234,8 +234,9 @@ def test_plan_tunables_keep_results(engine, cap, rows, chunk, wg):
 
 @pytest.mark.parametrize("reorder", [0, 1])
 def test_compressed_gather_is_bit_identical(engine, reorder):
-    """Delta/varint column ids decoded in the gather give exactly the plain path's ranks."""
+    """Delta/varint column ids decoded in the gather give exactly the plain row gather's ranks."""
     engine.set_option(gmp.OPT_REORDER, reorder)
+    engine.set_option(gmp.OPT_STRIP_LOG, 0)
     try:
         g = gen(engine, 13, off64=bool(reorder))
         say(f"compressed gather reorder={reorder}")
@@ -260,6 +261,39 @@ def test_compressed_gather_is_bit_identical(engine, reorder):
         assert np.abs(r1.astype(np.float64) - x).sum() < PR_L1_TOL
     finally:
         engine.set_option(gmp.OPT_REORDER, 1)
+        engine.set_option(gmp.OPT_STRIP_LOG, 20)
+
+
+@pytest.mark.parametrize("strip_log", [0, 4, 9, 12, 20])
+def test_source_strips_keep_results(engine, strip_log):
+    """The strip-major gather (pair partials + row-block combine) matches FP64 at every strip
+    width, including one-vertex-wide and single strips, and is bit-reproducible."""
+    engine.set_option(gmp.OPT_STRIP_LOG, strip_log)
+    try:
+        for scale, kw in ((13, {}), (12, dict(undirected=True)), (11, dict(off64=True))):
+            g = gen(engine, scale, **kw)
+            say(f"strip_log={strip_log} scale={scale} {kw}")
+            r, st = engine.pagerank(max_iter=20)
+            r2, _ = engine.pagerank(max_iter=20)
+            np.testing.assert_array_equal(r.view(np.uint32), r2.view(np.uint32))
+            x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
+            assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+            seed = int(np.argmax(g["outdeg"]))
+            r, _ = engine.pagerank(max_iter=20, seed_vertex=seed)
+            x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20, seed=seed)
+            assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+        # hubs split over many chunks inside one strip
+        V = 40000
+        rng = np.random.default_rng(5)
+        src = np.concatenate([np.arange(1, V), rng.integers(0, V, 100000), np.arange(2, 9000)])
+        dst = np.concatenate([np.zeros(V - 1, np.int64), rng.integers(0, V, 100000), np.ones(8998, np.int64)])
+        off, idx = graphs.csr_from_edges(V, src, dst)
+        engine.load_csr(off, idx)
+        r, _ = engine.pagerank(max_iter=25)
+        x, _, _ = oracle.pr_f64(off, idx, graphs.out_degree(V, idx), max_iter=25)
+        assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+    finally:
+        engine.set_option(gmp.OPT_STRIP_LOG, 20)
 
 
 def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
