@@ -17,7 +17,8 @@ APP_PAGERANK, APP_SSSP, APP_CC, APP_SSSP_W = 1, 2, 3, 4
 SSSP_BF, SSSP_DELTA, SSSP_LEARNED = 0, 1, 2
 PR_NO_KAHAN, PR_COMPRESSED = 1, 2
 GRAPH_OFF64 = 1
-OPT_REORDER, OPT_STREAM_CAP, OPT_ROW_CAP, OPT_CHUNK, OPT_WG_PER_CU, OPT_VEC_MIN, OPT_STRIP_LOG = 1, 2, 3, 4, 5, 6, 7
+OPT_REORDER, OPT_STREAM_CAP, OPT_ROW_CAP, OPT_CHUNK, OPT_WG_PER_CU, OPT_VEC_MIN = 1, 2, 3, 4, 5, 6
+OPT_STRIP_LOG, OPT_BIN_LOG, OPT_BIN_TASK_LOG = 7, 8, 9
 
 # every entry point include/gmp.h declares
 SYMBOLS = (

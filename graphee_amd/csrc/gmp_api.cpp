@@ -110,8 +110,19 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       e->opt.vec_min = (uint32_t)value;
       break;
     case GMP_OPT_STRIP_LOG:
-      in(0, 31, "strip_log");
+      if (value != 0) in(8, 15, "strip_log");
+      if ((uint32_t)value != e->opt.strip_log) e->g.strips.valid = false;
       e->opt.strip_log = (uint32_t)value;
+      break;
+    case GMP_OPT_BIN_LOG:
+      in(8, 14, "bin_log");
+      if ((uint32_t)value != e->opt.bin_log) e->g.strips.valid = false;
+      e->opt.bin_log = (uint32_t)value;
+      break;
+    case GMP_OPT_BIN_TASK_LOG:
+      in(8, 24, "bin_task_log");
+      if ((uint32_t)value != e->opt.bin_task_log) e->g.strips.valid = false;
+      e->opt.bin_task_log = (uint32_t)value;
       break;
     default:
       fail(GMP_EINVAL, "gmp_set_option: unknown key " + std::to_string(key));
@@ -131,6 +142,8 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_WG_PER_CU: *value = e->opt.wg_per_cu; break;
     case GMP_OPT_VEC_MIN: *value = e->opt.vec_min; break;
     case GMP_OPT_STRIP_LOG: *value = e->opt.strip_log; break;
+    case GMP_OPT_BIN_LOG: *value = e->opt.bin_log; break;
+    case GMP_OPT_BIN_TASK_LOG: *value = e->opt.bin_task_log; break;
     default: fail(GMP_EINVAL, "gmp_get_option: unknown key " + std::to_string(key));
   }
   GMP_API_END

@@ -104,18 +104,23 @@ struct CsrView {
 // strip by strip (so only one strip's contributions are live in L2 at a time) and a row-block
 // combine adds each row's pair partials in strip order.
 struct StripCsr {
-  uint32_t wlog = 0, nstrips = 0, nblk = 0;
-  uint64_t npairs = 0;
-  DevBuf pair_off;                 // uint32[npairs+1] edge offsets
-  DevBuf pair_row;                 // uint32[npairs] destination of each pair
-  DevBuf idx;                      // uint32[E] sources
-  DevBuf blk;                      // uint32[(nblk+1) * nstrips]: first pair of (row block, strip)
-  DevBuf partial;                  // float2[npairs]: per-iteration pair partials
-  std::vector<uint64_t> h_pair_off;
-  GatherPlan plan;
+  uint32_t wlog = 0, rlog = 0;  // strips of 2^wlog sources, bins of 2^rlog destinations
+  uint32_t ns = 0, nbins = 0;   // strips up to the last one holding an edge; destination bins
+  uint64_t nseg = 0;            // segments = messages per iteration
+  DevBuf code;                  // uint16[E+pad]: source offset in its strip, bit 15 ends a segment
+  DevBuf wtask;                 // uint32x4[nwt]: {first edge, end edge, first segment, strip}
+  DevBuf gtask;                 // uint32x4[ngt]: {strip, first wave task, end wave task, 0}
+  DevBuf seg_row;               // uint16[nseg]: destination offset inside its bin
+  DevBuf runs;                  // uint32[(nbins+1)*ns]: [b*ns+s] = first segment of strip s in bin >= b
+  DevBuf btask;                 // uint32x4[nbt]: {bin, first message, end message, partial slot or ~0}
+  DevBuf split;                 // uint32x4[nsplit]: {bin, first slot, slots, 0}
+  DevBuf msg;                   // float[nseg]: the iteration's segment sums
+  DevBuf bpart;                 // uint64[nslots << rlog]: fixed-point sums of split bins
+  uint32_t nwt = 0, ngt = 0, nbt = 0, nsplit = 0, nslots = 0;
+  uint32_t lds1 = 0, lds2 = 0;  // dynamic LDS bytes of the gather and bin kernels
+  uint32_t g1 = 0, g2 = 0, g3 = 0;  // grids of the gather, bin and split-apply kernels
   bool valid = false;
 };
-constexpr uint32_t kCombineRows = 1024;  // rows per combine block
 
 // Delta + varint column ids of the work in-CSR (decoded inside the gather).
 struct PackedCols {
@@ -170,7 +175,9 @@ struct Options {
   uint32_t chunk = 4096;       // gather plan: edges per hub CHUNK task
   uint32_t vec_min = 128;      // gather plan: rows above this many edges go wave-wide
   uint32_t wg_per_cu = 16;     // gather grid = wg_per_cu * CUs
-  uint32_t strip_log = 20;     // source strips of 2^strip_log ids for PageRank (0: row gather)
+  uint32_t strip_log = 15;     // PageRank source strips of 2^strip_log ids, 8..15 (0: row gather)
+  uint32_t bin_log = 14;       // PageRank destination bins of up to 2^bin_log rows, 8..14
+  uint32_t bin_task_log = 18;  // PageRank bins with more than 2^k messages are split, 8..24
 };
 
 // Many graphs resident at once as one disjoint union (graph k owns union ids
@@ -267,8 +274,8 @@ void plan_for(Engine &e, GatherPlan &pl, const uint64_t *off, uint32_t nrows);
 void ensure_packed(Engine &e);
 
 // ---- strip.hip ----
-// Builds e.g.strips for e.opt.strip_log; returns false when the layout does not apply (strips
-// off, 64-bit offsets, or ids too wide for the sort key).
+// Builds e.g.strips for e.opt.strip_log / bin_log; returns false when the layout does not apply
+// (strips off, or more than 2^32 - 1 edges).
 bool ensure_strips(Engine &e);
 // Row id of every edge of a CSR (device array of E entries).
 void edge_row_ids(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, DevBuf &rowid);

@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "gmp_internal.hpp"
+#include "pr_common.hpp"
 #include "varint.hpp"
 #include "wave.hpp"
 
@@ -31,13 +32,6 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kFinalBlock = 256;
-
-struct PrScal {
-  float base;    // added to every vertex
-  float extra;   // added to the seed vertex only
-  double resid;  // L1 change of the last iteration
-  double dang;   // dangling mass of the current ranks
-};
 
 template <class OffT>
 struct PrArgs {
@@ -61,51 +55,9 @@ struct PrArgs {
   uint32_t chunk;
   double *part_resid;
   double *part_dang;
-  Packed pk;    // compressed column ids (COMP kernels)
-  float2 *partial;         // PART kernels: hi/lo sum of every (destination, strip) pair
-  const uint32_t *pair_row;  // strip combine: destination of each pair
-  const uint32_t *blk;       // strip combine: first pair of (row block, strip)
-  uint32_t nstrips, nblk, V;
+  Packed pk;  // compressed column ids (COMP kernels)
   uint64_t E;
 };
-
-struct Acc {
-  double resid = 0, dang = 0;
-};
-
-template <class OffT>
-__device__ inline void pr_apply(const PrArgs<OffT> &a, uint32_t r, float s, float base, float extra,
-                                Acc &acc) {
-  float v = fmaf(a.d, s, base);
-  if (r == a.seed) v += extra;
-  const uint32_t od = __builtin_nontemporal_load(&a.outdeg[r]);
-  const float old = __builtin_nontemporal_load(&a.rank_in[r]);
-  __builtin_nontemporal_store(v, &a.rank_out[r]);
-  a.contrib_out[r] = od ? v / (float)od : 0.0f;
-  if (!od) acc.dang += (double)v;
-  acc.resid += fabs((double)v - (double)old);
-}
-
-// Deterministic block reduction of the two FP64 accumulators into partials[blockIdx.x].
-__device__ inline void block_store_acc(Acc acc, double *pr, double *pd) {
-  __shared__ double sr[kWavesPerBlock], sd[kWavesPerBlock];
-  double r = wave_reduce_sum(acc.resid), d = wave_reduce_sum(acc.dang);
-  const uint32_t w = threadIdx.x >> 6;
-  if (lane_id() == 0) {
-    sr[w] = r;
-    sd[w] = d;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double tr = 0, td = 0;
-    for (int i = 0; i < kWavesPerBlock; i++) {
-      tr += sr[i];
-      td += sd[i];
-    }
-    pr[blockIdx.x] = tr;
-    pd[blockIdx.x] = td;
-  }
-}
 
 template <bool KAHAN>
 __device__ inline void acc_add(float &s, float &c, float x) {
@@ -164,9 +116,8 @@ __device__ inline void gather_range(const PrArgs<OffT> &a, uint64_t e0, uint64_t
   }
 }
 
-// PART = false: rows are destinations and every row is applied.  PART = true: rows are the
-// (destination, strip) pairs of the strip layout and every row leaves its hi/lo partial.
-template <class OffT, bool KAHAN, bool COMP, bool PART = false>
+// Rows are destinations; every row is applied except multi-chunk hubs (pr_hub_combine_kernel).
+template <class OffT, bool KAHAN, bool COMP>
 __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
   extern __shared__ float lds[];
   const uint32_t lane = lane_id(), wv = wave_id();
@@ -174,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
   float *buf = lds + wv * per_wave;
   uint32_t *scratch = reinterpret_cast<uint32_t *>(buf + a.stream_cap);
   const float base = a.sc->base, extra = a.sc->extra;
-  Acc acc;
+  PrAcc acc;
   const uint32_t nw = gridDim.x * kWavesPerBlock;
   for (uint32_t t = blockIdx.x * kWavesPerBlock + wv; t < a.ntasks; t += nw) {
     const uint32_t x = a.tasks[2 * t], y = a.tasks[2 * t + 1];
@@ -189,8 +140,7 @@ __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
         const uint32_t b = (uint32_t)((uint64_t)a.off[r] - e0), en = (uint32_t)((uint64_t)a.off[r + 1] - e0);
         float s = 0, c = 0;
         for (uint32_t k = b; k < en; k++) acc_add<KAHAN>(s, c, buf[k]);
-        if (PART) a.partial[r] = make_float2(s, -c);
-        else pr_apply(a, r, s - c, base, extra, acc);
+        pr_apply(a, r, s - c, base, extra, acc);
       }
       wave_sync();
     } else {
@@ -205,24 +155,20 @@ __global__ __launch_bounds__(kBlock) void pr_gather_kernel(PrArgs<OffT> a) {
       float hi = s, lo = -c;
       wave_reduce_dd(hi, lo);
       if (lane == 0) {
-        if (h.nchunks == 1) {
-          if (PART) a.partial[h.row] = make_float2(hi, lo);
-          else pr_apply(a, h.row, hi, base, extra, acc);
-        }
+        if (h.nchunks == 1) pr_apply(a, h.row, hi, base, extra, acc);
         else a.hub_part[h.part_base + k] = make_float2(hi, lo);
       }
     }
   }
-  block_store_acc(acc, a.part_resid, a.part_dang);
+  block_store_acc<kWavesPerBlock>(acc, a.part_resid, a.part_dang);
 }
 
-// One wave per multi-chunk hub: combine its chunk partials in chunk order, then apply (or, in
-// the strip layout, store the pair's partial).
-template <class OffT, bool PART = false>
+// One wave per multi-chunk hub: combine its chunk partials in chunk order, then apply.
+template <class OffT>
 __global__ __launch_bounds__(kBlock) void pr_hub_combine_kernel(PrArgs<OffT> a) {
   const uint32_t lane = lane_id(), wv = wave_id();
   const float base = a.sc->base, extra = a.sc->extra;
-  Acc acc;
+  PrAcc acc;
   const uint32_t nw = gridDim.x * kWavesPerBlock;
   for (uint32_t t = blockIdx.x * kWavesPerBlock + wv; t < a.nmulti; t += nw) {
     const HubRow h = a.hubs[a.multi[t]];
@@ -232,48 +178,16 @@ __global__ __launch_bounds__(kBlock) void pr_hub_combine_kernel(PrArgs<OffT> a) 
       dd_add(hi, lo, p.x, p.y);
     }
     wave_reduce_dd(hi, lo);
-    if (lane == 0) {
-      if (PART) a.partial[h.row] = make_float2(hi, lo);
-      else pr_apply(a, h.row, hi, base, extra, acc);
-    }
+    if (lane == 0) pr_apply(a, h.row, hi, base, extra, acc);
   }
-  if (!PART) block_store_acc(acc, a.part_resid, a.part_dang);
-}
-
-// Strip layout: per block of kCombineRows destinations, add every row's pair partials in
-// strip order (each strip holds at most one pair per row, so no two threads touch one row's
-// accumulator within a strip), then apply the rows.
-template <class OffT>
-__global__ __launch_bounds__(kBlock) void pr_combine_apply_kernel(PrArgs<OffT> a) {
-  __shared__ float2 accum[kCombineRows];
-  const float base = a.sc->base, extra = a.sc->extra;
-  Acc acc;
-  for (uint32_t blk = blockIdx.x; blk < a.nblk; blk += gridDim.x) {
-    const uint32_t row0 = blk * kCombineRows;
-    for (uint32_t i = threadIdx.x; i < kCombineRows; i += kBlock) accum[i] = make_float2(0.0f, 0.0f);
-    __syncthreads();
-    for (uint32_t s = 0; s < a.nstrips; s++) {
-      const uint32_t p0 = a.blk[(size_t)blk * a.nstrips + s], p1 = a.blk[(size_t)(blk + 1) * a.nstrips + s];
-      for (uint32_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
-        const uint32_t r = a.pair_row[p] - row0;
-        const float2 v = a.partial[p];
-        float2 t = accum[r];
-        dd_add(t.x, t.y, v.x, v.y);
-        accum[r] = t;
-      }
-      __syncthreads();
-    }
-    for (uint32_t i = threadIdx.x; i < kCombineRows && row0 + i < a.V; i += kBlock) pr_apply(a, row0 + i, accum[i].x, base, extra, acc);
-    __syncthreads();
-  }
-  block_store_acc(acc, a.part_resid, a.part_dang);
+  block_store_acc<kWavesPerBlock>(acc, a.part_resid, a.part_dang);
 }
 
 // r = teleport vector, c = r / outdeg, dangling partials.
 __global__ __launch_bounds__(kBlock) void pr_init_kernel(uint32_t V, uint32_t seed, const uint32_t *outdeg,
                                                          float *rank, float *contrib, double *part_resid,
                                                          double *part_dang) {
-  Acc acc;
+  PrAcc acc;
   const float r0 = seed == 0xFFFFFFFFu ? (float)(1.0 / (double)V) : 0.0f;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
     const float r = (seed != 0xFFFFFFFFu && v == seed) ? 1.0f : r0;
@@ -282,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void pr_init_kernel(uint32_t V, uint32_t se
     contrib[v] = od ? r / (float)od : 0.0f;
     if (!od) acc.dang += (double)r;
   }
-  block_store_acc(acc, part_resid, part_dang);
+  block_store_acc<kWavesPerBlock>(acc, part_resid, part_dang);
 }
 
 // Sums partials in a fixed order (n1 from the gather, n2 from the hub combine), records
@@ -421,18 +335,26 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   const uint32_t Gh = pl.nmulti ? std::min<uint32_t>((pl.nmulti + kWavesPerBlock - 1) / kWavesPerBlock, G) : 0;
   const int max_iter = std::max(0, p.max_iter);
 
+  const bool comp = (p.flags & GMP_PR_COMPRESSED) != 0;
+  const bool kahan = !(p.flags & GMP_PR_NO_KAHAN);
+  if (comp) ensure_packed(e);
+  // source-strip layout: the default for compensated, uncompressed PageRank on graphs whose
+  // edges fit 32-bit indices; the row gather serves the other cases
+  const bool strips = !comp && kahan && ensure_strips(e);
+  const uint32_t np = std::max(G, strips ? strip_partials(e) : 0u);
+
   // buffers: 2 x rank, 2 x contrib, hub partials, FP64 partials, scalars, history
   e.s_a.alloc((size_t)V * 4);
   e.s_b.alloc((size_t)V * 4);
   e.s_c.alloc((size_t)V * 4);
   e.s_d.alloc((size_t)V * 4);
   e.s_e.alloc(std::max<size_t>(pl.nparts, 1) * sizeof(float2));
-  e.s_part.alloc((size_t)G * 4 * sizeof(double));
+  e.s_part.alloc((size_t)np * 4 * sizeof(double));
   e.s_scal.alloc(sizeof(PrScal) + (size_t)(max_iter + 1) * 2 * sizeof(double));
   float *rank[2] = {e.s_a.as<float>(), e.s_b.as<float>()};
   float *contrib[2] = {e.s_c.as<float>(), e.s_d.as<float>()};
-  double *pr1 = e.s_part.as<double>(), *pd1 = pr1 + G;
-  double *pr2 = pd1 + G, *pd2 = pr2 + G;
+  double *pr1 = e.s_part.as<double>(), *pd1 = pr1 + np;
+  double *pr2 = pd1 + np, *pd2 = pr2 + np;
   PrScal *sc = e.s_scal.as<PrScal>();
   double *hist = reinterpret_cast<double *>(e.s_scal.as<char>() + sizeof(PrScal));
 
@@ -468,73 +390,26 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, G, pr2, pd2, 0, V, p.damping, personalised, sc,
                                                       nullptr, -1);
   GMP_CHECK_LAUNCH();
-  const bool comp = (p.flags & GMP_PR_COMPRESSED) != 0;
   if (comp) {
-    ensure_packed(e);
     a.pk = Packed{g.packed.bytes.as<uint8_t>(), g.packed.gbyte.as<uint64_t>(), g.packed.gmask.as<uint64_t>()};
     a.E = g.E;
   }
-  // strip layout (default for uncompressed PageRank on graphs with 32-bit offsets)
-  const bool strips = !comp && ensure_strips(e);
-  PrArgs<uint32_t> ap{};
-  uint32_t Ghs = 0, Gc = 0;
-  if (strips) {
-    StripCsr &S = g.strips;
-    S.partial.alloc(std::max<uint64_t>(S.npairs, 1) * sizeof(float2));
-    e.s_e.alloc(std::max<size_t>(std::max(pl.nparts, S.plan.nparts), 1) * sizeof(float2));
-    ap.off = S.pair_off.as<uint32_t>();
-    ap.idx = S.idx.as<uint32_t>();
-    ap.outdeg = a.outdeg;
-    ap.tasks = S.plan.tasks.as<uint32_t>();
-    ap.ntasks = S.plan.ntasks;
-    ap.hubs = S.plan.hubs.as<HubRow>();
-    ap.multi = S.plan.multi.as<uint32_t>();
-    ap.nmulti = S.plan.nmulti;
-    ap.hub_part = e.s_e.as<float2>();
-    ap.sc = sc;
-    ap.d = p.damping;
-    ap.seed = seed;
-    ap.stream_cap = S.plan.stream_cap;
-    ap.chunk = S.plan.chunk;
-    ap.partial = S.partial.as<float2>();
-    ap.pair_row = S.pair_row.as<uint32_t>();
-    ap.blk = S.blk.as<uint32_t>();
-    ap.nstrips = S.nstrips;
-    ap.nblk = S.nblk;
-    ap.V = V;
-    Ghs = S.plan.nmulti ? std::min<uint32_t>((S.plan.nmulti + kWavesPerBlock - 1) / kWavesPerBlock, G) : 0;
-    Gc = std::min<uint32_t>(S.nblk, G);
-  }
-  const uint32_t cap_lds = strips ? ap.stream_cap : pl.stream_cap;
-  const size_t lds = (size_t)kWavesPerBlock * (cap_lds + (comp ? kVarintScratchWords : 0u)) * sizeof(float);
+  const size_t lds = (size_t)kWavesPerBlock * (pl.stream_cap + (comp ? kVarintScratchWords : 0u)) * sizeof(float);
   int cur = 0, it = 0;
   double resid = 0, dang = 0;
-  const bool kahan = !(p.flags & GMP_PR_NO_KAHAN);
   for (it = 0; it < max_iter; it++) {
-    a.contrib_in = ap.contrib_in = contrib[cur];
-    a.rank_in = ap.rank_in = rank[cur];
-    a.rank_out = ap.rank_out = rank[cur ^ 1];
-    a.contrib_out = ap.contrib_out = contrib[cur ^ 1];
+    a.contrib_in = contrib[cur];
+    a.rank_in = rank[cur];
+    a.rank_out = rank[cur ^ 1];
+    a.contrib_out = contrib[cur ^ 1];
     a.part_resid = pr1;
     a.part_dang = pd1;
     GMP_HIP(hipEventRecord(ev[2 + 2 * it], e.stream));
     uint32_t nfin1 = G, nfin2 = Gh;
     if (strips) {
-      ap.part_resid = pr2;  // the pair gather applies nothing; its accumulators stay zero
-      ap.part_dang = pd2;
-      if (kahan) pr_gather_kernel<uint32_t, true, false, true><<<G, kBlock, lds, e.stream>>>(ap);
-      else pr_gather_kernel<uint32_t, false, false, true><<<G, kBlock, lds, e.stream>>>(ap);
-      GMP_CHECK_LAUNCH();
-      if (Ghs) {
-        pr_hub_combine_kernel<uint32_t, true><<<Ghs, kBlock, 0, e.stream>>>(ap);
-        GMP_CHECK_LAUNCH();
-      }
-      ap.part_resid = pr1;
-      ap.part_dang = pd1;
-      pr_combine_apply_kernel<uint32_t><<<Gc, kBlock, 0, e.stream>>>(ap);
-      GMP_CHECK_LAUNCH();
-      nfin1 = Gc;
-      nfin2 = 0;
+      const PrStripIO io{a.outdeg, contrib[cur], rank[cur], rank[cur ^ 1], contrib[cur ^ 1], sc, p.damping, seed,
+                         pr1, pd1, pr2, pd2};
+      strip_iteration(e, io, nfin1, nfin2);
     } else {
       if (comp) {
         if (kahan) pr_gather_kernel<OffT, true, true><<<G, kBlock, lds, e.stream>>>(a);
@@ -601,7 +476,7 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     st->changed = 0;
     const uint64_t o = g.off64 ? 8 : 4;
     st->bytes = (uint64_t)it * (8 * g.E + (o + 16) * (uint64_t)V);
-    st->index_bytes = comp ? g.packed.nbytes + 16 * g.packed.ngroups : 4 * g.E;
+    st->index_bytes = comp ? g.packed.nbytes + 16 * g.packed.ngroups : strips ? 2 * g.E : 4 * g.E;
   }
 }
 

@@ -1,0 +1,95 @@
+// Pieces shared by the PageRank kernels of pagerank.hip (row gather) and strip.hip (source
+// strips): the per-iteration scalars, the update of one vertex, the deterministic FP64 block
+// reductions, and the exact fixed-point form of a contribution.
+#pragma once
+
+#include "gmp_internal.hpp"
+#include "wave.hpp"
+
+namespace gmp {
+
+struct PrScal {
+  float base;    // added to every vertex
+  float extra;   // added to the seed vertex only
+  double resid;  // L1 change of the last iteration
+  double dang;   // dangling mass of the current ranks
+};
+
+struct PrAcc {
+  double resid = 0, dang = 0;
+};
+
+// r'[v] = d*s + base (+ extra at the seed); c'[v] = r'[v] / outdeg[v].  `A` carries outdeg,
+// rank_in, rank_out, contrib_out, d and seed.
+template <class A>
+__device__ inline void pr_apply(const A &a, uint32_t r, float s, float base, float extra, PrAcc &acc) {
+  float v = fmaf(a.d, s, base);
+  if (r == a.seed) v += extra;
+  const uint32_t od = __builtin_nontemporal_load(&a.outdeg[r]);
+  const float old = __builtin_nontemporal_load(&a.rank_in[r]);
+  __builtin_nontemporal_store(v, &a.rank_out[r]);
+  a.contrib_out[r] = od ? v / (float)od : 0.0f;
+  if (!od) acc.dang += (double)v;
+  acc.resid += fabs((double)v - (double)old);
+}
+
+// Deterministic reduction of the two FP64 accumulators of a block of WAVES wavefronts into
+// pr[blockIdx.x], pd[blockIdx.x].
+template <int WAVES>
+__device__ inline void block_store_acc(PrAcc acc, double *pr, double *pd) {
+  __shared__ double sr[WAVES], sd[WAVES];
+  double r = wave_reduce_sum(acc.resid), d = wave_reduce_sum(acc.dang);
+  const uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    sr[w] = r;
+    sd[w] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tr = 0, td = 0;
+    for (int i = 0; i < WAVES; i++) {
+      tr += sr[i];
+      td += sd[i];
+    }
+    pr[blockIdx.x] = tr;
+    pd[blockIdx.x] = td;
+  }
+}
+
+// Fixed point with 61 fraction bits: a non-negative float x < 4 maps to floor(x * 2^61), which
+// is exact for x >= 2^-38 (all 24 mantissa bits land above bit 0).  Sums of such integers are
+// exact and independent of order, so unordered accumulation stays deterministic.
+constexpr int kFixBits = 61;
+
+__device__ inline unsigned long long fix_from_float(float x) {
+  const uint32_t bits = __float_as_uint(x);
+  const uint32_t ex = bits >> 23;  // x >= 0: no sign bit
+  if (ex == 0) return 0ull;        // zero or subnormal (< 2^-126): below the fixed-point grid
+  const unsigned long long m = (bits & 0x7FFFFFu) | 0x800000u;
+  if (ex >= 128) return 1ull << 62;  // x >= 2 never occurs for a contribution; saturate
+  // x = m * 2^(ex-150); x * 2^61 = m * 2^(ex-89)
+  return ex >= 89 ? m << (ex - 89) : m >> min(89u - ex, 63u);
+}
+
+__device__ inline double fix_to_double(unsigned long long s) { return (double)s * 0x1p-61; }
+
+// One PageRank iteration over the source-strip layout (strip.hip): what it reads and writes.
+struct PrStripIO {
+  const uint32_t *outdeg;
+  const float *contrib_in;
+  const float *rank_in;
+  float *rank_out;
+  float *contrib_out;
+  const PrScal *sc;
+  float d;
+  uint32_t seed;
+  double *pr1, *pd1;  // FP64 residual/dangling partials of the bin kernel, one per block
+  double *pr2, *pd2;  // and of the split-bin kernel
+};
+
+// FP64 partials that strip_iteration may write into each of pr1, pd1, pr2, pd2.
+uint32_t strip_partials(Engine &e);
+// Launches one iteration on e.stream; n1 and n2 receive the partial counts written.
+void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2);
+
+}  // namespace gmp

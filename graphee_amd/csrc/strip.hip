@@ -1,27 +1,53 @@
-// Source-strip layout of the PageRank in-CSR (see StripCsr).
+// Source-strip layout of the PageRank in-CSR and the iteration that runs over it.
 //
-// Why: with the whole graph resident, a PageRank iteration is bound by its random gather of
-// contributions.  Gathering row by row touches every source at once, so every cold source
-// costs an L2 miss and a full cache line.  Sorting the edges by (source strip, destination,
-// source) lets the gather walk one strip of 2^wlog consecutive (degree-ordered) sources at a
-// time -- a strip's contributions fit one XCD's L2 -- while every (destination, strip) pair
-// leaves one FP32 hi/lo partial that a row-block combine adds in strip order.
+// With the whole graph resident, a row-by-row PageRank gather is bound by its random reads of
+// source contributions: every cold source costs a cache-line miss, and even L2 hits cost one
+// L1 fill per edge.  Here the edges are sorted by (source strip, destination, source), strips
+// being 2^wlog consecutive work ids (the degree order puts the hubs in the first strips):
 //
-// Build: one 64-bit key per edge, strip << 2b | dst << b | src (b = id bits), an LSD radix
-// sort, head flags where (strip, dst) changes, an exclusive scan into pair ids, and binary
-// searches for the first pair of every (row block, strip).
+//   pr_strip_gather_kernel  stages one strip's contributions in LDS and streams the strip's
+//                           16-bit source offsets (8 per lane per round).  Every run of edges
+//                           with one destination -- a segment -- leaves one FP32 message, its
+//                           sum taken in FP64 (lane-sequential, then a segmented wave scan).
+//                           Messages go through L2 with plain stores, which merges the lanes'
+//                           scattered writes into whole lines.
+//   pr_bin_kernel           owns a bin of 2^rlog destinations: its messages form one run per
+//                           strip; they are added into LDS accumulators in an exact 61-bit
+//                           fixed point with integer atomics -- exact sums do not depend on
+//                           the order of the adds, so the result is deterministic -- and the
+//                           rank update is applied.
+//   pr_split_apply_kernel   finishes bins whose messages were split over several workgroups
+//                           (their slices leave fixed-point partials).
+//
+// Segments end where the strip or destination changes and at every wave-task boundary, so
+// each wave task starts with a fresh segment.  Bit 15 of an edge's code marks a segment's
+// last edge, hence strips of at most 2^15 sources.
+//
+// Build (once per graph and layout options): a 64-bit key per edge, strip << (b + wlog) |
+// dst << wlog | source offset (b = id bits), an LSD radix sort, segment ends, an exclusive
+// scan into segment ids, and binary searches for the first segment of every (bin, strip).
 #include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
 
-#include "gmp_internal.hpp"
+#include "pr_common.hpp"
 
 namespace gmp {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;                    // build kernels, split apply
+constexpr int kIterBlock = 1024;               // gather and bin kernels
+constexpr int kIterWaves = kIterBlock / kWave;
+constexpr uint32_t kRound = 8 * kWave;         // edges per wave per round (8 per lane)
+constexpr uint32_t kWaveTaskEdges = 16 * kRound;  // edges per wave task
+constexpr int kAhead = 4;                      // rounds of codes in flight per wave
+constexpr int kUnroll = 8;                     // messages in flight per thread (bin kernel)
+constexpr uint32_t kChunk = 256;               // bin kernel: messages per chunk-index entry
+constexpr uint32_t kGroupWaveTasks = 256;      // wave tasks per workgroup task (one staging)
+constexpr uint32_t kCodePad = kRound;          // codes read past the last edge stay in bounds
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 #define GS64(i, n) \
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
@@ -36,140 +62,615 @@ inline uint32_t bits_for(uint64_t n) {
   return std::max<uint32_t>(b, 1);
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+struct StripArgs {
+  // vertex update (pr_apply)
+  const uint32_t *outdeg;
+  const float *contrib_in;
+  const float *rank_in;
+  float *rank_out;
+  float *contrib_out;
+  const PrScal *sc;
+  float d;
+  uint32_t seed;
+  // layout
+  const uint16_t *code;
+  const uint4 *wtask;
+  const uint4 *gtask;
+  uint32_t ngt;
+  float *msg;
+  const uint16_t *seg_row;
+  const uint32_t *runs;
+  uint32_t ns;
+  const uint4 *btask;
+  uint32_t nbt;
+  unsigned long long *bpart;
+  const uint4 *split;
+  uint32_t nsplit;
+  uint32_t V, wlog, rlog;
+  double *pr1, *pd1, *pr2, *pd2;
+};
+
+// ---------------------------------------------------------------- build kernels
+
 __global__ void strip_keys_kernel(const uint32_t *idx, const uint32_t *rowid, uint64_t E, uint32_t b, uint32_t wlog,
                                   uint64_t *keys) {
+  const uint64_t wmask = (1ull << wlog) - 1;
   GS64(j, E) {
     const uint64_t src = idx[j], dst = rowid[j];
-    keys[j] = ((src >> wlog) << (2 * b)) | (dst << b) | src;
+    keys[j] = ((src >> wlog) << (b + wlog)) | (dst << wlog) | (src & wmask);
   }
 }
 
-__global__ void head_flags_kernel(const uint64_t *keys, uint64_t E, uint32_t b, uint32_t *flag) {
-  GS64(j, E) flag[j] = (j == 0 || (keys[j] >> b) != (keys[j - 1] >> b)) ? 1u : 0u;
+// off[s] = first edge whose strip >= s, s in [0, ns]
+__global__ void strip_offsets_kernel(const uint64_t *keys, uint64_t E, uint32_t sh, uint32_t ns, uint32_t *off) {
+  GS64(i, E + 1) {
+    const int64_t s = i < E ? (int64_t)(keys[i] >> sh) : (int64_t)ns;
+    const int64_t sp = i > 0 ? (int64_t)(keys[i - 1] >> sh) : -1;
+    for (int64_t q = sp + 1; q <= s; q++) off[q] = (uint32_t)i;
+  }
 }
 
-__global__ void pairs_from_keys_kernel(const uint64_t *keys, const uint32_t *pid, uint64_t E, uint32_t b,
-                                       uint32_t *pair_off, uint32_t *pair_row, uint32_t *pair_strip, uint32_t *idx) {
-  const uint64_t mask = (1ull << b) - 1;
+// code = source offset in the strip | 0x8000 when the next edge has another (strip, dst)
+__global__ void codes_kernel(const uint64_t *keys, uint64_t E, uint32_t wlog, uint16_t *code) {
+  const uint64_t wmask = (1ull << wlog) - 1;
   GS64(j, E) {
     const uint64_t k = keys[j];
-    idx[j] = (uint32_t)(k & mask);
-    if (j == 0 || (k >> b) != (keys[j - 1] >> b)) {
-      const uint32_t p = pid[j];
-      pair_off[p] = (uint32_t)j;
-      pair_row[p] = (uint32_t)((k >> b) & mask);
-      pair_strip[p] = (uint32_t)(k >> (2 * b));
+    const bool end = j + 1 == E || (keys[j + 1] >> wlog) != (k >> wlog);
+    code[j] = (uint16_t)((k & wmask) | (end ? 0x8000u : 0u));
+  }
+}
+
+// every wave task starts a segment: end one at the edge before it
+__global__ void task_ends_kernel(const uint4 *wt, uint32_t n, uint16_t *code) {
+  GS64(t, n) {
+    const uint32_t e0 = wt[t].x;
+    if (e0 > 0) code[e0 - 1] |= (uint16_t)0x8000u;
+  }
+}
+
+__global__ void end_flags_kernel(const uint16_t *code, uint64_t E, uint32_t *flag) {
+  GS64(j, E) flag[j] = code[j] >> 15;
+}
+
+__global__ void task_seg_kernel(uint4 *wt, uint32_t n, const uint32_t *segid) {
+  GS64(t, n) wt[t].z = segid[wt[t].x];
+}
+
+// at each segment's last edge: the destination inside its bin, and the full destination
+__global__ void seg_rows_kernel(const uint64_t *keys, const uint16_t *code, const uint32_t *segid, uint64_t E,
+                                uint32_t wlog, uint32_t b, uint32_t rmask, uint16_t *seg_row, uint32_t *seg_full) {
+  const uint64_t bmask = (1ull << b) - 1;
+  GS64(j, E) {
+    if (code[j] & 0x8000u) {
+      const uint32_t row = (uint32_t)((keys[j] >> wlog) & bmask);
+      const uint32_t p = segid[j];
+      seg_row[p] = (uint16_t)(row & rmask);
+      seg_full[p] = row;
     }
   }
 }
 
-// strip_base[s] = first pair whose strip >= s, s in [0, nstrips]
-__global__ void strip_base_kernel(const uint32_t *pair_strip, uint64_t P, uint32_t nstrips, uint32_t *base) {
-  GS64(i, P + 1) {
-    const int64_t s = i < P ? (int64_t)pair_strip[i] : (int64_t)nstrips;
-    const int64_t sp = i > 0 ? (int64_t)pair_strip[i - 1] : -1;
-    for (int64_t q = sp + 1; q <= s; q++) base[q] = (uint32_t)i;
-  }
+// sseg[s] = first segment of strip s (an empty strip gets its successor's), sseg[ns] = nseg
+__global__ void strip_seg_kernel(const uint32_t *soff, const uint32_t *segid, uint32_t ns, uint64_t E,
+                                 uint32_t nseg, uint32_t *sseg) {
+  GS64(s, (uint64_t)ns + 1) sseg[s] = (s < ns && soff[s] < E) ? segid[soff[s]] : nseg;
 }
 
-// blk[k * nstrips + s] = first pair of strip s whose row >= k * rows_per_block, k in [0, nblk]
-__global__ void block_table_kernel(const uint32_t *pair_row, const uint32_t *strip_base, uint32_t nstrips,
-                                   uint32_t nblk, uint32_t rows_per_block, uint32_t *blk) {
-  GS64(t, (uint64_t)(nblk + 1) * nstrips) {
-    const uint32_t s = (uint32_t)(t % nstrips), k = (uint32_t)(t / nstrips);
-    const uint64_t target = (uint64_t)k * rows_per_block;
-    uint32_t lo = strip_base[s], hi = strip_base[s + 1];
+// runs[bb*ns + s] = first segment of strip s whose destination >= bb << rlog, bb in [0, nbins]
+__global__ void runs_kernel(const uint32_t *seg_full, const uint32_t *sseg, uint32_t ns, uint32_t nbins,
+                            uint32_t rlog, uint32_t *runs) {
+  GS64(t, (uint64_t)(nbins + 1) * ns) {
+    const uint32_t s = (uint32_t)(t % ns), bb = (uint32_t)(t / ns);
+    const uint64_t target = (uint64_t)bb << rlog;
+    uint32_t lo = sseg[s], hi = sseg[s + 1];
     while (lo < hi) {
       const uint32_t mid = lo + (hi - lo) / 2;
-      if ((uint64_t)pair_row[mid] < target) lo = mid + 1;
+      if ((uint64_t)seg_full[mid] < target) lo = mid + 1;
       else hi = mid;
     }
-    blk[t] = lo;
+    runs[t] = lo;
   }
+}
+
+// cnt[bb] = messages of bin bb (one wave per bin)
+__global__ void bin_count_kernel(const uint32_t *runs, uint32_t ns, uint32_t nbins, uint32_t *cnt) {
+  const uint32_t lane = lane_id();
+  const uint32_t wpb = blockDim.x / kWave;
+  for (uint32_t bb = blockIdx.x * wpb + threadIdx.x / kWave; bb < nbins; bb += gridDim.x * wpb) {
+    uint32_t c = 0;
+    for (uint32_t s = lane; s < ns; s += kWave) c += runs[(size_t)(bb + 1) * ns + s] - runs[(size_t)bb * ns + s];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o, kWave);
+    if (lane == 0) cnt[bb] = c;
+  }
+}
+
+// ---------------------------------------------------------------- iteration kernels
+
+// Codes of round r of a wave task (8 per lane); zero outside [e0, e1).
+__device__ inline u32x4 load_round(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1) {
+  const uint32_t q = b0 + r * kRound + 8 * lane_id();
+  u32x4 cv = {0u, 0u, 0u, 0u};
+  if (q < e1 && q + 8 > e0) cv = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.code + q));
+  return cv;
+}
+
+// One round: 8 consecutive edges per lane from q.  FULL: every lane's 8 edges lie inside
+// [e0, e1) or entirely outside it (only the first and last rounds of a strip can cut a lane).
+// The messages of the segments that end in the round are placed in the wave's LDS buffer mb
+// in segment order, then stored with coalesced writes; the sum of the segment left open stays
+// in `carry`.
+template <bool FULL>
+__device__ inline void strip_round(const StripArgs &a, const float *sv, float *mb, u32x4 cv, uint32_t q, uint32_t e0,
+                                   uint32_t e1, uint64_t upto, uint32_t &seg, double &carry) {
+  const uint32_t lane = lane_id();
+  const bool live = q < e1 && q + 8 > e0;
+  float x[8];
+  uint32_t ends = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t c = (cv[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+    const float v = sv[c & 0x7FFFu];
+    if (FULL) {
+      x[k] = v;  // a dead lane (past e1) holds zero codes: no ends; its sum is dropped below
+      ends |= (c >> 15) << k;
+    } else {
+      const bool ok = q + k >= e0 && q + k < e1;
+      x[k] = ok ? v : 0.0f;
+      if (ok && (c & 0x8000u)) ends |= 1u << k;
+    }
+  }
+  const uint32_t cnt = (uint32_t)__popc(ends);
+  const uint32_t incl = wave_scan_dpp(cnt);
+  const uint32_t total = read_lane_u32(incl, kWave - 1);
+  const uint32_t o0 = incl - cnt;  // this lane's first message within the round
+  double run = 0.0, first = 0.0;
+  uint32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    run += (double)x[k];
+    const bool e = (ends >> k) & 1u;
+    if (e && n == 0) first = run;
+    // an end writes its message slot; any other edge writes the free slot `total` (some edge
+    // of the round is not an end whenever this happens, so total < kRound)
+    mb[e ? o0 + n : total] = (float)run;
+    n += e ? 1u : 0u;
+    if (e) run = 0.0;
+  }
+  if (FULL && !live) run = 0.0;
+  // the open sum through each lane; the previous round's carry joins where no segment
+  // started since the round began
+  const bool head = ends != 0;
+  double open = wave_segmented_scan_dpp(run, head);
+  if (!(__ballot(head) & upto)) open += carry;
+  double cin = dpp_f64<kDppWaveShr1>(open);
+  if (lane == 0) cin = carry;
+  if (head) mb[o0] = (float)(cin + first);
+  carry = read_lane_f64(open, kWave - 1);
+  wave_sync();
+  for (uint32_t i = lane; i < total; i += kWave) a.msg[seg + i] = mb[i];
+  wave_sync();
+  seg += total;
+}
+
+// Messages of edges [e0, e1) whose first segment has index seg; every segment ends inside.
+// The code loads run kAhead rounds ahead of their use.
+__device__ inline void strip_wave_task(const StripArgs &a, const float *sv, float *mb, uint32_t e0, uint32_t e1,
+                                       uint32_t seg) {
+  const uint32_t lane = lane_id();
+  const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
+  const uint32_t b0 = e0 & ~7u;
+  const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
+  double carry = 0.0;  // sum of the segment left open by the previous round
+  u32x4 buf[kAhead];
+#pragma unroll
+  for (int i = 0; i < kAhead; i++) buf[i] = load_round(a, b0, i, e0, e1);
+  for (uint32_t r = 0; r < nr; r += kAhead) {
+#pragma unroll
+    for (int i = 0; i < kAhead; i++) {
+      if (r + i < nr) {
+        const u32x4 cv = buf[i];
+        buf[i] = load_round(a, b0, r + i + kAhead, e0, e1);
+        const uint32_t base = b0 + (r + i) * kRound;
+        const uint32_t q = base + 8 * lane;
+        // lanes are cut only where e0 or e1 is not a multiple of 8 inside this round
+        const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
+        if (cut) strip_round<false>(a, sv, mb, cv, q, e0, e1, upto, seg, carry);
+        else strip_round<true>(a, sv, mb, cv, q, e0, e1, upto, seg, carry);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a) {
+  extern __shared__ float sv[];  // contributions of the task's strip, then a message buffer per wave
+  const uint32_t W = 1u << a.wlog;
+  const uint32_t wv = wave_id();
+  float *mb = sv + W + wv * kRound;
+  for (uint32_t t = blockIdx.x; t < a.ngt; t += gridDim.x) {
+    const uint4 gt = a.gtask[t];
+    const uint32_t v0 = gt.x << a.wlog;
+    const uint32_t n = min(W, a.V - v0);
+    __syncthreads();  // the previous task's readers are done
+    if (n == W && (v0 & 3u) == 0) {
+      // whole strip: 16-byte loads, four in flight per thread before their stores
+      const u32x4 *src = reinterpret_cast<const u32x4 *>(a.contrib_in + v0);
+      u32x4 *dst = reinterpret_cast<u32x4 *>(sv);
+      const uint32_t n4 = W / 4;
+      uint32_t i = threadIdx.x;
+      for (; i + 3 * kIterBlock < n4; i += 4 * kIterBlock) {
+        const u32x4 t0 = src[i], t1 = src[i + kIterBlock], t2 = src[i + 2 * kIterBlock], t3 = src[i + 3 * kIterBlock];
+        dst[i] = t0;
+        dst[i + kIterBlock] = t1;
+        dst[i + 2 * kIterBlock] = t2;
+        dst[i + 3 * kIterBlock] = t3;
+      }
+      for (; i < n4; i += kIterBlock) dst[i] = src[i];
+    } else {
+      for (uint32_t i = threadIdx.x; i < n; i += kIterBlock) sv[i] = a.contrib_in[v0 + i];
+    }
+    __syncthreads();
+    for (uint32_t w = gt.y + wv; w < gt.z; w += kIterWaves) {
+      const uint4 wt = a.wtask[w];
+      strip_wave_task(a, sv, mb, wt.x, wt.y, wt.z);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kIterBlock) void pr_bin_kernel(StripArgs a) {
+  extern __shared__ unsigned long long acc[];  // 2^rlog accumulators, then the bin's run table
+  __shared__ uint32_t wtot[kIterWaves];
+  const uint32_t R = 1u << a.rlog, ns = a.ns;
+  uint32_t *rbase = reinterpret_cast<uint32_t *>(acc + R);  // first message of each run
+  uint32_t *pre = rbase + ns;                               // exclusive prefix of run lengths, ns+1
+  uint32_t *cidx = pre + ns + 1;  // run holding message bt.y + c*kChunk, c <= slice/kChunk
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wv = wave_id();
+  const float base = a.sc->base, extra = a.sc->extra;
+  const uint32_t per = (ns + kIterBlock - 1) / kIterBlock;
+  const uint32_t s0 = min(tid * per, ns), s1 = min(s0 + per, ns);
+  PrAcc pacc;
+  for (uint32_t t = blockIdx.x; t < a.nbt; t += gridDim.x) {
+    const uint4 bt = a.btask[t];
+    const uint32_t b = bt.x;
+    __syncthreads();  // the previous task is done with the LDS
+    for (uint32_t i = tid; i < R; i += kIterBlock) acc[i] = 0ull;
+    uint32_t sum = 0;
+    for (uint32_t s = s0; s < s1; s++) {
+      const uint32_t r0 = a.runs[(size_t)b * ns + s], r1 = a.runs[(size_t)(b + 1) * ns + s];
+      rbase[s] = r0;
+      pre[s] = r1 - r0;
+      sum += r1 - r0;
+    }
+    const uint32_t incl = wave_inclusive_scan(sum);
+    if (lane == kWave - 1) wtot[wv] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (uint32_t w = 0; w < wv; w++) run += wtot[w];
+    for (uint32_t s = s0; s < s1; s++) {
+      const uint32_t l = pre[s];
+      pre[s] = run;
+      run += l;
+    }
+    if (tid == kIterBlock - 1) pre[ns] = run;
+    __syncthreads();
+    // chunk index of the slice: every kChunk-th message's run bounds the searches below
+    const uint32_t nch = (bt.z - bt.y + kChunk - 1) / kChunk;
+    for (uint32_t s = s0; s < s1; s++) {
+      const uint32_t p0 = max(pre[s], bt.y), p1 = min(pre[s + 1], bt.z);
+      for (uint32_t c = (p0 - bt.y + kChunk - 1) / kChunk; p0 < p1 && bt.y + c * kChunk < p1; c++) cidx[c] = s;
+    }
+    if (tid == 0) cidx[nch] = ns - 1;
+    __syncthreads();
+    for (uint32_t jb = bt.y + tid; jb < bt.z; jb += kIterBlock * kUnroll) {
+      uint32_t m[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const uint32_t j = jb + u * kIterBlock;
+        m[u] = 0xFFFFFFFFu;
+        if (j < bt.z) {
+          // the run holding message j: the last r with pre[r] <= j (never an empty run)
+          const uint32_t c = (j - bt.y) / kChunk;
+          uint32_t lo = cidx[c], hi = cidx[c + 1];
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= j) lo = mid;
+            else hi = mid - 1;
+          }
+          m[u] = rbase[lo] + (j - pre[lo]);
+        }
+      }
+      float x[kUnroll];
+      uint32_t r[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (m[u] != 0xFFFFFFFFu) {
+          x[u] = __builtin_nontemporal_load(&a.msg[m[u]]);
+          r[u] = __builtin_nontemporal_load(&a.seg_row[m[u]]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++)
+        if (m[u] != 0xFFFFFFFFu) atomicAdd(&acc[r[u]], fix_from_float(x[u]));
+    }
+    __syncthreads();
+    if (bt.w == kNoSlot) {
+      const uint32_t v0 = b << a.rlog;
+      for (uint32_t i = tid; i < R && v0 + i < a.V; i += kIterBlock)
+        pr_apply(a, v0 + i, (float)fix_to_double(acc[i]), base, extra, pacc);
+    } else {
+      unsigned long long *dst = a.bpart + ((size_t)bt.w << a.rlog);
+      for (uint32_t i = tid; i < R; i += kIterBlock) dst[i] = acc[i];
+    }
+  }
+  block_store_acc<kIterWaves>(pacc, a.pr1, a.pd1);
+}
+
+__global__ __launch_bounds__(kBlock) void pr_split_apply_kernel(StripArgs a) {
+  const float base = a.sc->base, extra = a.sc->extra;
+  const uint32_t R = 1u << a.rlog;
+  PrAcc pacc;
+  for (uint32_t t = blockIdx.x; t < a.nsplit; t += gridDim.x) {
+    const uint4 sp = a.split[t];
+    const uint32_t v0 = sp.x << a.rlog;
+    for (uint32_t i = threadIdx.x; i < R && v0 + i < a.V; i += kBlock) {
+      unsigned long long s = 0;
+      for (uint32_t k = 0; k < sp.z; k++) s += a.bpart[((size_t)(sp.y + k) << a.rlog) + i];
+      pr_apply(a, v0 + i, (float)fix_to_double(s), base, extra, pacc);
+    }
+  }
+  block_store_acc<kBlock / kWave>(pacc, a.pr2, a.pd2);
+}
+
+int lds_limit(Engine &e) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, e.device) != hipSuccess || v <= 0)
+    v = 65536;
+  return v;
 }
 
 }  // namespace
+
+uint32_t strip_partials(Engine &e) { return 2u * (uint32_t)std::max(1, e.num_cus); }
 
 bool ensure_strips(Engine &e) {
   ensure_work(e);
   Graph &g = e.g;
   StripCsr &S = g.strips;
   const uint32_t wlog = e.opt.strip_log;
-  if (wlog == 0 || g.off64 || g.E == 0) return false;
+  if (wlog == 0 || g.E == 0 || g.E >= 0xFFFFFFFFull - kCodePad) return false;
+  if (S.valid) return true;
   const uint32_t V = g.V;
   const uint64_t E = g.E;
   const uint32_t b = bits_for(V);
-  const uint32_t nstrips = (uint32_t)(((uint64_t)V + (1ull << std::min<uint32_t>(wlog, 40)) - 1) >> std::min<uint32_t>(wlog, 40));
-  const uint32_t sbits = bits_for((uint64_t)nstrips + 1);
-  if (2 * b + sbits > 64) return false;
-  if (!(S.valid && S.wlog == wlog)) {
-    S = StripCsr();
-    const CsrView w = g.work_in();
-    DevBuf rowid, kA, kB, tmp, flag;
-    edge_row_ids(e, w, V, E, g.off64, rowid);
-    kA.alloc(E * 8);
-    kB.alloc(E * 8);
-    strip_keys_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(w.idx, rowid.as<uint32_t>(), E, b, wlog, kA.as<uint64_t>());
-    GMP_CHECK_LAUNCH();
-    rowid.release();
-    rocprim::double_buffer<uint64_t> dk(kA.as<uint64_t>(), kB.as<uint64_t>());
-    size_t tb = 0;
-    GMP_HIP(rocprim::radix_sort_keys(nullptr, tb, dk, E, 0, 2 * b + sbits, e.stream));
-    tmp.alloc(tb);
-    GMP_HIP(rocprim::radix_sort_keys(tmp.p, tb, dk, E, 0, 2 * b + sbits, e.stream));
-    const uint64_t *keys = dk.current();
-    flag.alloc(E * 4);
-    head_flags_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(keys, E, b, flag.as<uint32_t>());
-    GMP_CHECK_LAUNCH();
-    // exclusive scan of the head flags into pair ids, in place
-    DevBuf pid;
-    pid.alloc(E * 4);
-    size_t tb2 = 0;
-    GMP_HIP(rocprim::exclusive_scan(nullptr, tb2, flag.as<uint32_t>(), pid.as<uint32_t>(), 0u, E,
-                                    rocprim::plus<uint32_t>(), e.stream));
-    tmp.alloc(tb2);
-    GMP_HIP(rocprim::exclusive_scan(tmp.p, tb2, flag.as<uint32_t>(), pid.as<uint32_t>(), 0u, E,
-                                    rocprim::plus<uint32_t>(), e.stream));
-    uint32_t last_pid = 0, last_flag = 0;
-    GMP_HIP(hipMemcpyAsync(&last_pid, pid.as<uint32_t>() + E - 1, 4, hipMemcpyDeviceToHost, e.stream));
-    GMP_HIP(hipMemcpyAsync(&last_flag, flag.as<uint32_t>() + E - 1, 4, hipMemcpyDeviceToHost, e.stream));
-    GMP_HIP(hipStreamSynchronize(e.stream));
-    flag.release();
-    const uint64_t P = (uint64_t)last_pid + last_flag;
-    S.npairs = P;
-    S.pair_off.alloc((P + 1) * 4);
-    S.pair_row.alloc(std::max<uint64_t>(P, 1) * 4);
-    S.idx.alloc(E * 4);
-    DevBuf pstrip, sbase;
-    pstrip.alloc(std::max<uint64_t>(P, 1) * 4);
-    pairs_from_keys_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(keys, pid.as<uint32_t>(), E, b,
-                                                                S.pair_off.as<uint32_t>(), S.pair_row.as<uint32_t>(),
-                                                                pstrip.as<uint32_t>(), S.idx.as<uint32_t>());
-    GMP_CHECK_LAUNCH();
-    const uint32_t e32 = (uint32_t)E;
-    GMP_HIP(hipMemcpyAsync(S.pair_off.as<uint32_t>() + P, &e32, 4, hipMemcpyHostToDevice, e.stream));
-    GMP_HIP(hipStreamSynchronize(e.stream));
-    kA.release();
-    kB.release();
-    pid.release();
-    sbase.alloc(((size_t)nstrips + 1) * 4);
-    strip_base_kernel<<<grid_for(P + 1), kBlock, 0, e.stream>>>(pstrip.as<uint32_t>(), P, nstrips, sbase.as<uint32_t>());
-    GMP_CHECK_LAUNCH();
-    S.nstrips = nstrips;
-    S.nblk = (V + kCombineRows - 1) / kCombineRows;
-    S.blk.alloc(((size_t)S.nblk + 1) * nstrips * 4);
-    block_table_kernel<<<grid_for((uint64_t)(S.nblk + 1) * nstrips), kBlock, 0, e.stream>>>(
-        S.pair_row.as<uint32_t>(), sbase.as<uint32_t>(), nstrips, S.nblk, kCombineRows, S.blk.as<uint32_t>());
-    GMP_CHECK_LAUNCH();
-    std::vector<uint32_t> h32(P + 1);
-    GMP_HIP(hipMemcpyAsync(h32.data(), S.pair_off.p, (P + 1) * 4, hipMemcpyDeviceToHost, e.stream));
-    GMP_HIP(hipStreamSynchronize(e.stream));
-    S.h_pair_off.assign(h32.begin(), h32.end());
-    S.wlog = wlog;
-    S.valid = true;
+  const uint32_t nsmax = (uint32_t)(((uint64_t)V + (1ull << wlog) - 1) >> wlog);
+  const uint32_t sbits = bits_for(nsmax);
+  const size_t ldsmax = (size_t)lds_limit(e);
+  const size_t ldsmax2 = ldsmax - 512;  // the bin kernel's static LDS (scan and FP64 partials)
+  const size_t mbuf = (size_t)kIterWaves * kRound * 4;  // per-wave message buffers
+  if (sbits + b + wlog > 64 || (4ull << wlog) + mbuf > ldsmax) return false;
+
+  S = StripCsr();
+  const CsrView w = g.work_in();
+  DevBuf rowid, kA, kB, tmp;
+  edge_row_ids(e, w, V, E, g.off64, rowid);
+  kA.alloc(E * 8);
+  kB.alloc(E * 8);
+  strip_keys_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(w.idx, rowid.as<uint32_t>(), E, b, wlog, kA.as<uint64_t>());
+  GMP_CHECK_LAUNCH();
+  rowid.release();
+  rocprim::double_buffer<uint64_t> dk(kA.as<uint64_t>(), kB.as<uint64_t>());
+  size_t tb = 0;
+  GMP_HIP(rocprim::radix_sort_keys(nullptr, tb, dk, E, 0, sbits + b + wlog, e.stream));
+  tmp.alloc(tb);
+  GMP_HIP(rocprim::radix_sort_keys(tmp.p, tb, dk, E, 0, sbits + b + wlog, e.stream));
+  const uint64_t *keys = dk.current();
+  uint64_t last = 0;
+  GMP_HIP(hipMemcpyAsync(&last, keys + E - 1, 8, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  const uint32_t sh = b + wlog;
+  const uint32_t ns = (uint32_t)(last >> sh) + 1;
+
+  // bins: as many rows as the LDS holds next to the run table
+  uint32_t rlog = std::min<uint32_t>(e.opt.bin_log, b);
+  auto lds2 = [&](uint32_t r) {
+    return ((size_t)8 << r) + (size_t)(2 * (size_t)ns + 1) * 4 + ((size_t)(1u << e.opt.bin_task_log) / kChunk + 2) * 4;
+  };
+  while (rlog > 4 && lds2(rlog) > ldsmax2) rlog--;
+  if (lds2(rlog) > ldsmax2) return false;
+  const uint32_t R = 1u << rlog;
+  const uint32_t nbins = (uint32_t)(((uint64_t)V + R - 1) >> rlog);
+
+  // strip edge ranges -> wave tasks and workgroup tasks
+  DevBuf soff;
+  soff.alloc(((size_t)ns + 1) * 4);
+  strip_offsets_kernel<<<grid_for(E + 1), kBlock, 0, e.stream>>>(keys, E, sh, ns, soff.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  std::vector<uint32_t> hso((size_t)ns + 1);
+  GMP_HIP(hipMemcpyAsync(hso.data(), soff.p, hso.size() * 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  std::vector<uint4> wt, gt;
+  for (uint32_t s = 0; s < ns; s++) {
+    const uint32_t e0 = hso[s], e1 = hso[s + 1];
+    if (e0 == e1) continue;
+    const uint32_t w0 = (uint32_t)wt.size();
+    // task boundaries inside a strip fall on multiples of 8, so only the strip's own ends can
+    // cut a lane's 8 edges
+    for (uint64_t x = e0; x < e1;) {
+      const uint64_t nx = std::min<uint64_t>(((x & ~7ull) + kWaveTaskEdges), e1);
+      wt.push_back(make_uint4((uint32_t)x, (uint32_t)nx, 0u, s));
+      x = nx;
+    }
+    for (uint32_t k = w0; k < (uint32_t)wt.size(); k += kGroupWaveTasks)
+      gt.push_back(make_uint4(s, k, std::min<uint32_t>(k + kGroupWaveTasks, (uint32_t)wt.size()), 0u));
   }
-  plan_for(e, S.plan, S.h_pair_off.data(), (uint32_t)S.npairs);
+  // heaviest workgroup tasks first: the static round robin then balances well
+  std::stable_sort(gt.begin(), gt.end(), [&](const uint4 &p, const uint4 &q) {
+    return wt[p.z - 1].y - wt[p.y].x > wt[q.z - 1].y - wt[q.y].x;
+  });
+  S.nwt = (uint32_t)wt.size();
+  S.ngt = (uint32_t)gt.size();
+  S.wtask.alloc((size_t)S.nwt * sizeof(uint4));
+  S.gtask.alloc((size_t)S.ngt * sizeof(uint4));
+  GMP_HIP(hipMemcpyAsync(S.wtask.p, wt.data(), (size_t)S.nwt * sizeof(uint4), hipMemcpyHostToDevice, e.stream));
+  GMP_HIP(hipMemcpyAsync(S.gtask.p, gt.data(), (size_t)S.ngt * sizeof(uint4), hipMemcpyHostToDevice, e.stream));
+
+  // codes and segment ids
+  S.code.alloc((E + kCodePad) * 2);
+  GMP_HIP(hipMemsetAsync(S.code.p, 0, (E + kCodePad) * 2, e.stream));
+  codes_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(keys, E, wlog, S.code.as<uint16_t>());
+  GMP_CHECK_LAUNCH();
+  task_ends_kernel<<<grid_for(S.nwt), kBlock, 0, e.stream>>>(S.wtask.as<uint4>(), S.nwt, S.code.as<uint16_t>());
+  GMP_CHECK_LAUNCH();
+  DevBuf flag, segid;
+  flag.alloc(E * 4);
+  segid.alloc(E * 4);
+  end_flags_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(S.code.as<uint16_t>(), E, flag.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  size_t tb2 = 0;
+  GMP_HIP(rocprim::exclusive_scan(nullptr, tb2, flag.as<uint32_t>(), segid.as<uint32_t>(), 0u, E,
+                                  rocprim::plus<uint32_t>(), e.stream));
+  DevBuf tmp2;
+  tmp2.alloc(tb2);
+  GMP_HIP(rocprim::exclusive_scan(tmp2.p, tb2, flag.as<uint32_t>(), segid.as<uint32_t>(), 0u, E,
+                                  rocprim::plus<uint32_t>(), e.stream));
+  uint32_t lastseg = 0;
+  GMP_HIP(hipMemcpyAsync(&lastseg, segid.as<uint32_t>() + E - 1, 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  flag.release();
+  tmp2.release();
+  const uint32_t nseg = lastseg + 1;  // the last edge always ends a segment
+  S.nseg = nseg;
+  task_seg_kernel<<<grid_for(S.nwt), kBlock, 0, e.stream>>>(S.wtask.as<uint4>(), S.nwt, segid.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+
+  // segment destinations and the (bin, strip) run table
+  DevBuf segfull, sseg;
+  S.seg_row.alloc((size_t)nseg * 2);
+  segfull.alloc((size_t)nseg * 4);
+  seg_rows_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(keys, S.code.as<uint16_t>(), segid.as<uint32_t>(), E, wlog,
+                                                        b, R - 1, S.seg_row.as<uint16_t>(), segfull.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  sseg.alloc(((size_t)ns + 1) * 4);
+  strip_seg_kernel<<<grid_for((uint64_t)ns + 1), kBlock, 0, e.stream>>>(soff.as<uint32_t>(), segid.as<uint32_t>(), ns,
+                                                                         E, nseg, sseg.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  kA.release();
+  kB.release();
+  tmp.release();
+  segid.release();
+  S.runs.alloc(((size_t)nbins + 1) * ns * 4);
+  runs_kernel<<<grid_for((uint64_t)(nbins + 1) * ns), kBlock, 0, e.stream>>>(segfull.as<uint32_t>(), sseg.as<uint32_t>(),
+                                                                            ns, nbins, rlog, S.runs.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  DevBuf bcnt;
+  bcnt.alloc((size_t)nbins * 4);
+  bin_count_kernel<<<std::max<uint32_t>(1, std::min<uint32_t>((nbins + 3) / 4, 65536)), kBlock, 0, e.stream>>>(
+      S.runs.as<uint32_t>(), ns, nbins, bcnt.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  std::vector<uint32_t> hc(nbins);
+  GMP_HIP(hipMemcpyAsync(hc.data(), bcnt.p, (size_t)nbins * 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+
+  // bin tasks: heavier bins are split into slices that leave fixed-point partials
+  const uint32_t kBinTaskMsgs = 1u << e.opt.bin_task_log;
+  std::vector<uint4> bt, sp;
+  uint32_t nslots = 0;
+  for (uint32_t bb = 0; bb < nbins; bb++) {
+    const uint32_t m = hc[bb];
+    if (m <= kBinTaskMsgs) {
+      bt.push_back(make_uint4(bb, 0u, m, kNoSlot));
+      continue;
+    }
+    const uint32_t k = (m + kBinTaskMsgs - 1) / kBinTaskMsgs;
+    for (uint32_t i = 0; i < k; i++)
+      bt.push_back(make_uint4(bb, i * kBinTaskMsgs, std::min<uint32_t>(m, (i + 1) * kBinTaskMsgs), nslots + i));
+    sp.push_back(make_uint4(bb, nslots, k, 0u));
+    nslots += k;
+  }
+  std::stable_sort(bt.begin(), bt.end(), [](const uint4 &p, const uint4 &q) { return p.z - p.y > q.z - q.y; });
+  S.nbt = (uint32_t)bt.size();
+  S.nsplit = (uint32_t)sp.size();
+  S.nslots = nslots;
+  S.btask.alloc((size_t)S.nbt * sizeof(uint4));
+  GMP_HIP(hipMemcpyAsync(S.btask.p, bt.data(), (size_t)S.nbt * sizeof(uint4), hipMemcpyHostToDevice, e.stream));
+  if (S.nsplit) {
+    S.split.alloc((size_t)S.nsplit * sizeof(uint4));
+    GMP_HIP(hipMemcpyAsync(S.split.p, sp.data(), (size_t)S.nsplit * sizeof(uint4), hipMemcpyHostToDevice, e.stream));
+    S.bpart.alloc(((size_t)nslots << rlog) * 8);
+  }
+  S.msg.alloc((size_t)nseg * 4);
+  GMP_HIP(hipStreamSynchronize(e.stream));
+
+  S.wlog = wlog;
+  S.rlog = rlog;
+  S.ns = ns;
+  S.nbins = nbins;
+  S.lds1 = (uint32_t)((4ull << wlog) + mbuf);
+  S.lds2 = (uint32_t)lds2(rlog);
+  const uint32_t cus = (uint32_t)std::max(1, e.num_cus);
+  const uint32_t per1 = (uint32_t)std::min<size_t>(2, std::max<size_t>(1, ldsmax / S.lds1));
+  const uint32_t per2 = (uint32_t)std::min<size_t>(2, std::max<size_t>(1, ldsmax / (S.lds2 + 512)));
+  S.g1 = std::max<uint32_t>(1, std::min<uint32_t>(S.ngt, cus * per1));
+  S.g2 = std::max<uint32_t>(1, std::min<uint32_t>(S.nbt, cus * per2));
+  S.g3 = std::max<uint32_t>(1, std::min<uint32_t>(S.nsplit, cus));
+  // raise the dynamic-LDS ceiling where the runtime keeps one; a refusal leaves the launch
+  // itself to report a size the device cannot hold
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&pr_strip_gather_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
+    (void)hipGetLastError();
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&pr_bin_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)S.lds2) != hipSuccess)
+    (void)hipGetLastError();
+  S.valid = true;
   return true;
+}
+
+void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2) {
+  const StripCsr &S = e.g.strips;
+  if (!S.valid) fail(GMP_ESTATE, "strip layout not built");
+  StripArgs a{};
+  a.outdeg = io.outdeg;
+  a.contrib_in = io.contrib_in;
+  a.rank_in = io.rank_in;
+  a.rank_out = io.rank_out;
+  a.contrib_out = io.contrib_out;
+  a.sc = io.sc;
+  a.d = io.d;
+  a.seed = io.seed;
+  a.code = S.code.as<uint16_t>();
+  a.wtask = S.wtask.as<uint4>();
+  a.gtask = S.gtask.as<uint4>();
+  a.ngt = S.ngt;
+  a.msg = S.msg.as<float>();
+  a.seg_row = S.seg_row.as<uint16_t>();
+  a.runs = S.runs.as<uint32_t>();
+  a.ns = S.ns;
+  a.btask = S.btask.as<uint4>();
+  a.nbt = S.nbt;
+  a.bpart = S.bpart.as<unsigned long long>();
+  a.split = S.split.as<uint4>();
+  a.nsplit = S.nsplit;
+  a.V = e.g.V;
+  a.wlog = S.wlog;
+  a.rlog = S.rlog;
+  a.pr1 = io.pr1;
+  a.pd1 = io.pd1;
+  a.pr2 = io.pr2;
+  a.pd2 = io.pd2;
+  pr_strip_gather_kernel<<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+  GMP_CHECK_LAUNCH();
+  pr_bin_kernel<<<S.g2, kIterBlock, S.lds2, e.stream>>>(a);
+  GMP_CHECK_LAUNCH();
+  n1 = S.g2;
+  n2 = 0;
+  if (S.nsplit) {
+    pr_split_apply_kernel<<<S.g3, kBlock, 0, e.stream>>>(a);
+    GMP_CHECK_LAUNCH();
+    n2 = S.g3;
+  }
 }
 
 }  // namespace gmp

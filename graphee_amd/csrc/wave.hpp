@@ -83,6 +83,68 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
   return x;
 }
 
+// ---- DPP forms (GFX9 row shifts / row broadcasts / wave shift; no LDS traffic) ----
+// dpp<CTRL, ROWS>(x): x of the lane the DPP control names; 0 where that lane is out of range
+// or the lane's row is outside ROWS.
+template <int CTRL, int ROWS = 0xF>
+__device__ inline uint32_t dpp_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xF, true);
+}
+
+template <int CTRL, int ROWS = 0xF>
+__device__ inline double dpp_f64(double x) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const uint64_t lo = dpp_u32<CTRL, ROWS>((uint32_t)b), hi = dpp_u32<CTRL, ROWS>((uint32_t)(b >> 32));
+  return __longlong_as_double((long long)((hi << 32) | lo));
+}
+
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
+
+// Inclusive prefix sum over the 64 lanes: row-local shifts by 1, 2, 4, 8, then the last lane
+// of row 0 (2) is added to row 1 (3), then lane 31 to rows 2 and 3.
+__device__ inline uint32_t wave_scan_dpp(uint32_t x) {
+  x += dpp_u32<kDppRowShr1>(x);
+  x += dpp_u32<kDppRowShr2>(x);
+  x += dpp_u32<kDppRowShr4>(x);
+  x += dpp_u32<kDppRowShr8>(x);
+  x += dpp_u32<kDppRowBcast15, 0xA>(x);
+  x += dpp_u32<kDppRowBcast31, 0xC>(x);
+  return x;
+}
+
+template <int CTRL, int ROWS = 0xF>
+__device__ inline void seg_scan_step(double &x, uint32_t &f) {
+  const double y = dpp_f64<CTRL, ROWS>(x);
+  const uint32_t fy = dpp_u32<CTRL, ROWS>(f);
+  if (!f) x += y;
+  f |= fy;
+}
+
+// Inclusive segmented sum over the 64 lanes (same step pattern as wave_scan_dpp); a lane with
+// `head` starts a new segment.  The combination tree is fixed, so FP64 results are
+// deterministic.
+__device__ inline double wave_segmented_scan_dpp(double x, bool head) {
+  uint32_t f = head ? 1u : 0u;
+  seg_scan_step<kDppRowShr1>(x, f);
+  seg_scan_step<kDppRowShr2>(x, f);
+  seg_scan_step<kDppRowShr4>(x, f);
+  seg_scan_step<kDppRowShr8>(x, f);
+  seg_scan_step<kDppRowBcast15, 0xA>(x, f);
+  seg_scan_step<kDppRowBcast31, 0xC>(x, f);
+  return x;
+}
+
+__device__ inline uint32_t read_lane_u32(uint32_t x, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
+}
+
+__device__ inline double read_lane_f64(double x, int lane) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const uint64_t lo = read_lane_u32((uint32_t)b, lane), hi = read_lane_u32((uint32_t)(b >> 32), lane);
+  return __longlong_as_double((long long)((hi << 32) | lo));
+}
+
 // Wave-aggregated append: lanes with `pred` get consecutive slots of `list`, with one
 // atomic per wave on `count` (ballot + popcount prefix).
 __device__ inline uint32_t wave_append(bool pred, uint32_t *count) {

@@ -261,39 +261,50 @@ def test_compressed_gather_is_bit_identical(engine, reorder):
         assert np.abs(r1.astype(np.float64) - x).sum() < PR_L1_TOL
     finally:
         engine.set_option(gmp.OPT_REORDER, 1)
-        engine.set_option(gmp.OPT_STRIP_LOG, 20)
+        engine.set_option(gmp.OPT_STRIP_LOG, 15)
 
 
-@pytest.mark.parametrize("strip_log", [0, 4, 9, 12, 20])
-def test_source_strips_keep_results(engine, strip_log):
-    """The strip-major gather (pair partials + row-block combine) matches FP64 at every strip
-    width, including one-vertex-wide and single strips, and is bit-reproducible."""
-    engine.set_option(gmp.OPT_STRIP_LOG, strip_log)
+STRIP_DEFAULTS = ((gmp.OPT_STRIP_LOG, 15), (gmp.OPT_BIN_LOG, 14), (gmp.OPT_BIN_TASK_LOG, 18))
+
+
+@pytest.mark.parametrize("strip_log,bin_log,task_log", [(0, 14, 18), (8, 8, 8), (11, 10, 10), (15, 14, 18),
+                                                        (15, 14, 8)])
+def test_source_strips_keep_results(engine, strip_log, bin_log, task_log):
+    """The source-strip iteration (LDS-staged strips, per-segment messages, fixed-point bin
+    sums, split bins) matches FP64 for every strip width, bin size and split size, and is
+    bit-reproducible; strip_log 0 selects the row gather."""
+    for k, v in zip((gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG), (strip_log, bin_log, task_log)):
+        engine.set_option(k, v)
     try:
         for scale, kw in ((13, {}), (12, dict(undirected=True)), (11, dict(off64=True))):
             g = gen(engine, scale, **kw)
-            say(f"strip_log={strip_log} scale={scale} {kw}")
+            say(f"strips {strip_log}/{bin_log}/{task_log} scale={scale} {kw}")
             r, st = engine.pagerank(max_iter=20)
+            # the strip layout streams 2-byte source codes, the row gather 4-byte ids
+            assert st.index_bytes == (2 if strip_log else 4) * len(g["idx"])
             r2, _ = engine.pagerank(max_iter=20)
             np.testing.assert_array_equal(r.view(np.uint32), r2.view(np.uint32))
             x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
             assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
+            assert abs(float(r.astype(np.float64).sum()) - 1.0) < 1e-5
             seed = int(np.argmax(g["outdeg"]))
             r, _ = engine.pagerank(max_iter=20, seed_vertex=seed)
             x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20, seed=seed)
             assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
-        # hubs split over many chunks inside one strip
+        # a hub with 40k in-edges from every strip, dense pairs, many tasks per strip
         V = 40000
         rng = np.random.default_rng(5)
         src = np.concatenate([np.arange(1, V), rng.integers(0, V, 100000), np.arange(2, 9000)])
         dst = np.concatenate([np.zeros(V - 1, np.int64), rng.integers(0, V, 100000), np.ones(8998, np.int64)])
         off, idx = graphs.csr_from_edges(V, src, dst)
         engine.load_csr(off, idx)
-        r, _ = engine.pagerank(max_iter=25)
+        r, st = engine.pagerank(max_iter=25)
+        assert st.index_bytes == (2 if strip_log else 4) * len(idx)
         x, _, _ = oracle.pr_f64(off, idx, graphs.out_degree(V, idx), max_iter=25)
         assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
     finally:
-        engine.set_option(gmp.OPT_STRIP_LOG, 20)
+        for k, v in STRIP_DEFAULTS:
+            engine.set_option(k, v)
 
 
 def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
