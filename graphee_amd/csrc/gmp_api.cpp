@@ -120,7 +120,7 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       e->opt.bin_log = (uint32_t)value;
       break;
     case GMP_OPT_BIN_TASK_LOG:
-      in(8, 24, "bin_task_log");
+      in(8, 22, "bin_task_log");
       if ((uint32_t)value != e->opt.bin_task_log) e->g.strips.valid = false;
       e->opt.bin_task_log = (uint32_t)value;
       break;

@@ -19,18 +19,48 @@ struct PrAcc {
   double resid = 0, dang = 0;
 };
 
-// r'[v] = d*s + base (+ extra at the seed); c'[v] = r'[v] / outdeg[v].  `A` carries outdeg,
-// rank_in, rank_out, contrib_out, d and seed.
+// r'[v] = d*s + base (+ extra at the seed); c'[v] = r'[v] / outdeg[v], given outdeg[v] = od
+// and the old rank.  `A` carries outdeg, rank_in, rank_out, contrib_out, d and seed.
 template <class A>
-__device__ inline void pr_apply(const A &a, uint32_t r, float s, float base, float extra, PrAcc &acc) {
+__device__ inline void pr_update(const A &a, uint32_t r, float s, float base, float extra, uint32_t od, float old,
+                                 PrAcc &acc) {
   float v = fmaf(a.d, s, base);
   if (r == a.seed) v += extra;
-  const uint32_t od = __builtin_nontemporal_load(&a.outdeg[r]);
-  const float old = __builtin_nontemporal_load(&a.rank_in[r]);
   __builtin_nontemporal_store(v, &a.rank_out[r]);
   a.contrib_out[r] = od ? v / (float)od : 0.0f;
   if (!od) acc.dang += (double)v;
   acc.resid += fabs((double)v - (double)old);
+}
+
+template <class A>
+__device__ inline void pr_apply(const A &a, uint32_t r, float s, float base, float extra, PrAcc &acc) {
+  const uint32_t od = __builtin_nontemporal_load(&a.outdeg[r]);
+  const float old = __builtin_nontemporal_load(&a.rank_in[r]);
+  pr_update(a, r, s, base, extra, od, old, acc);
+}
+
+// pr_apply over rows v0 + i, i = first, first + stride, ... < n, U rows' loads in flight at a
+// time; sum(i) gives the row's gathered sum.
+template <int U, class A, class F>
+__device__ inline void pr_apply_rows(const A &a, uint32_t v0, uint32_t n, uint32_t first, uint32_t stride, F sum,
+                                     float base, float extra, PrAcc &acc) {
+  for (uint32_t i0 = first; i0 < n; i0 += stride * U) {
+    uint32_t od[U];
+    float old[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t i = i0 + u * stride;
+      if (i < n) {
+        od[u] = __builtin_nontemporal_load(&a.outdeg[v0 + i]);
+        old[u] = __builtin_nontemporal_load(&a.rank_in[v0 + i]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t i = i0 + u * stride;
+      if (i < n) pr_update(a, v0 + i, sum(i), base, extra, od[u], old[u], acc);
+    }
+  }
 }
 
 // Deterministic reduction of the two FP64 accumulators of a block of WAVES wavefronts into

@@ -44,7 +44,8 @@ constexpr uint32_t kRound = 8 * kWave;         // edges per wave per round (8 pe
 constexpr uint32_t kWaveTaskEdges = 16 * kRound;  // edges per wave task
 constexpr int kAhead = 4;                      // rounds of codes in flight per wave
 constexpr int kUnroll = 8;                     // messages in flight per thread (bin kernel)
-constexpr uint32_t kChunk = 256;               // bin kernel: messages per chunk-index entry
+constexpr uint32_t kChunk = 1024;              // bin kernel: messages per chunk-index entry
+constexpr uint32_t kSplitRows = 1024;          // split apply: rows per block task
 constexpr uint32_t kGroupWaveTasks = 256;      // wave tasks per workgroup task (one staging)
 constexpr uint32_t kCodePad = kRound;          // codes read past the last edge stay in bounds
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
@@ -197,14 +198,28 @@ __device__ inline u32x4 load_round(const StripArgs &a, uint32_t b0, uint32_t r, 
   return cv;
 }
 
+// Message buffer of one wave in LDS: messages [base, base + fill) of the wave task wait there
+// until a round would overflow it, then leave with coalesced stores.
+struct MsgBuf {
+  float *mb;
+  uint32_t base, fill;
+};
+
+__device__ inline void flush_messages(const StripArgs &a, MsgBuf &B) {
+  wave_sync();
+  for (uint32_t i = lane_id(); i < B.fill; i += kWave) a.msg[B.base + i] = B.mb[i];
+  wave_sync();
+  B.base += B.fill;
+  B.fill = 0;
+}
+
 // One round: 8 consecutive edges per lane from q.  FULL: every lane's 8 edges lie inside
 // [e0, e1) or entirely outside it (only the first and last rounds of a strip can cut a lane).
-// The messages of the segments that end in the round are placed in the wave's LDS buffer mb
-// in segment order, then stored with coalesced writes; the sum of the segment left open stays
-// in `carry`.
+// The messages of the segments that end in the round go to the wave's buffer in segment
+// order; the sum of the segment left open stays in `carry`.
 template <bool FULL>
-__device__ inline void strip_round(const StripArgs &a, const float *sv, float *mb, u32x4 cv, uint32_t q, uint32_t e0,
-                                   uint32_t e1, uint64_t upto, uint32_t &seg, double &carry) {
+__device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &B, u32x4 cv, uint32_t q, uint32_t e0,
+                                   uint32_t e1, uint64_t upto, double &carry) {
   const uint32_t lane = lane_id();
   const bool live = q < e1 && q + 8 > e0;
   float x[8];
@@ -225,34 +240,31 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, float *m
   const uint32_t cnt = (uint32_t)__popc(ends);
   const uint32_t incl = wave_scan_dpp(cnt);
   const uint32_t total = read_lane_u32(incl, kWave - 1);
-  const uint32_t o0 = incl - cnt;  // this lane's first message within the round
+  // room for the round plus a free slot (needed only when some edge is not an end)
+  if (B.fill + total >= kRound) flush_messages(a, B);
+  const uint32_t o0 = B.fill + incl - cnt;  // this lane's first slot
+  const uint32_t spare = B.fill + total;    // the free slot: non-ends write there
+  const uint32_t kf = ends ? (uint32_t)__ffs(ends) - 1u : 8u;
   double run = 0.0, first = 0.0;
-  uint32_t n = 0;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     run += (double)x[k];
     const bool e = (ends >> k) & 1u;
-    if (e && n == 0) first = run;
-    // an end writes its message slot; any other edge writes the free slot `total` (some edge
-    // of the round is not an end whenever this happens, so total < kRound)
-    mb[e ? o0 + n : total] = (float)run;
-    n += e ? 1u : 0u;
+    if ((uint32_t)k == kf) first = run;
+    B.mb[e ? o0 + (uint32_t)__popc(ends & ((1u << k) - 1u)) : spare] = (float)run;
     if (e) run = 0.0;
   }
   if (FULL && !live) run = 0.0;
   // the open sum through each lane; the previous round's carry joins where no segment
   // started since the round began
   const bool head = ends != 0;
-  double open = wave_segmented_scan_dpp(run, head);
+  double open = wave_segmented_scan_dpp_short(run, head);
   if (!(__ballot(head) & upto)) open += carry;
   double cin = dpp_f64<kDppWaveShr1>(open);
   if (lane == 0) cin = carry;
-  if (head) mb[o0] = (float)(cin + first);
+  if (head) B.mb[o0] = (float)(cin + first);
   carry = read_lane_f64(open, kWave - 1);
-  wave_sync();
-  for (uint32_t i = lane; i < total; i += kWave) a.msg[seg + i] = mb[i];
-  wave_sync();
-  seg += total;
+  B.fill += total;
 }
 
 // Messages of edges [e0, e1) whose first segment has index seg; every segment ends inside.
@@ -264,6 +276,7 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
   const uint32_t b0 = e0 & ~7u;
   const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
   double carry = 0.0;  // sum of the segment left open by the previous round
+  MsgBuf B{mb, seg, 0u};
   u32x4 buf[kAhead];
 #pragma unroll
   for (int i = 0; i < kAhead; i++) buf[i] = load_round(a, b0, i, e0, e1);
@@ -277,11 +290,12 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
         const uint32_t q = base + 8 * lane;
         // lanes are cut only where e0 or e1 is not a multiple of 8 inside this round
         const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
-        if (cut) strip_round<false>(a, sv, mb, cv, q, e0, e1, upto, seg, carry);
-        else strip_round<true>(a, sv, mb, cv, q, e0, e1, upto, seg, carry);
+        if (cut) strip_round<false>(a, sv, B, cv, q, e0, e1, upto, carry);
+        else strip_round<true>(a, sv, B, cv, q, e0, e1, upto, carry);
       }
     }
   }
+  flush_messages(a, B);
 }
 
 __global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a) {
@@ -397,8 +411,9 @@ __global__ __launch_bounds__(kIterBlock) void pr_bin_kernel(StripArgs a) {
     __syncthreads();
     if (bt.w == kNoSlot) {
       const uint32_t v0 = b << a.rlog;
-      for (uint32_t i = tid; i < R && v0 + i < a.V; i += kIterBlock)
-        pr_apply(a, v0 + i, (float)fix_to_double(acc[i]), base, extra, pacc);
+      pr_apply_rows<kUnroll>(
+          a, v0, min(R, a.V - v0), tid, kIterBlock, [&](uint32_t i) { return (float)fix_to_double(acc[i]); }, base,
+          extra, pacc);
     } else {
       unsigned long long *dst = a.bpart + ((size_t)bt.w << a.rlog);
       for (uint32_t i = tid; i < R; i += kIterBlock) dst[i] = acc[i];
@@ -411,13 +426,23 @@ __global__ __launch_bounds__(kBlock) void pr_split_apply_kernel(StripArgs a) {
   const float base = a.sc->base, extra = a.sc->extra;
   const uint32_t R = 1u << a.rlog;
   PrAcc pacc;
-  for (uint32_t t = blockIdx.x; t < a.nsplit; t += gridDim.x) {
-    const uint4 sp = a.split[t];
+  const uint32_t nck = (R + kSplitRows - 1) / kSplitRows;  // row chunks per split bin
+  for (uint32_t t = blockIdx.x; t < a.nsplit * nck; t += gridDim.x) {
+    const uint4 sp = a.split[t / nck];
+    const uint32_t r0 = (t % nck) * kSplitRows;
     const uint32_t v0 = sp.x << a.rlog;
-    for (uint32_t i = threadIdx.x; i < R && v0 + i < a.V; i += kBlock) {
-      unsigned long long s = 0;
-      for (uint32_t k = 0; k < sp.z; k++) s += a.bpart[((size_t)(sp.y + k) << a.rlog) + i];
-      pr_apply(a, v0 + i, (float)fix_to_double(s), base, extra, pacc);
+    const uint32_t n = min(min(R, a.V - v0), r0 + kSplitRows);
+    for (uint32_t i0 = r0 + threadIdx.x; i0 < n; i0 += kBlock * 4) {
+      unsigned long long s[4] = {0ull, 0ull, 0ull, 0ull};
+      for (uint32_t k = 0; k < sp.z; k++) {
+        const unsigned long long *part = a.bpart + ((size_t)(sp.y + k) << a.rlog);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (i0 + u * kBlock < n) s[u] += part[i0 + u * kBlock];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (i0 + u * kBlock < n) pr_apply(a, v0 + i0 + u * kBlock, (float)fix_to_double(s[u]), base, extra, pacc);
     }
   }
   block_store_acc<kBlock / kWave>(pacc, a.pr2, a.pd2);
@@ -432,7 +457,7 @@ int lds_limit(Engine &e) {
 
 }  // namespace
 
-uint32_t strip_partials(Engine &e) { return 2u * (uint32_t)std::max(1, e.num_cus); }
+uint32_t strip_partials(Engine &e) { return 4u * (uint32_t)std::max(1, e.num_cus); }
 
 bool ensure_strips(Engine &e) {
   ensure_work(e);
@@ -615,7 +640,7 @@ bool ensure_strips(Engine &e) {
   const uint32_t per2 = (uint32_t)std::min<size_t>(2, std::max<size_t>(1, ldsmax / (S.lds2 + 512)));
   S.g1 = std::max<uint32_t>(1, std::min<uint32_t>(S.ngt, cus * per1));
   S.g2 = std::max<uint32_t>(1, std::min<uint32_t>(S.nbt, cus * per2));
-  S.g3 = std::max<uint32_t>(1, std::min<uint32_t>(S.nsplit, cus));
+  S.g3 = std::max<uint32_t>(1, std::min<uint32_t>(S.nsplit * ((R + kSplitRows - 1) / kSplitRows), cus * 4));
   // raise the dynamic-LDS ceiling where the runtime keeps one; a refusal leaves the launch
   // itself to report a size the device cannot hold
   if (hipFuncSetAttribute(reinterpret_cast<const void *>(&pr_strip_gather_kernel),

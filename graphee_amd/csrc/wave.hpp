@@ -135,6 +135,28 @@ __device__ inline double wave_segmented_scan_dpp(double x, bool head) {
   return x;
 }
 
+// The same segmented sum, skipping the steps no lane needs: after the row shift by s a lane is
+// complete when its window holds a segment start or reaches its row's first lane; the row
+// broadcasts run only when some lane of rows 1-3 has no start in its row prefix.  A skipped
+// step would leave every lane unchanged, so the result equals the full scan's.
+__device__ inline double wave_segmented_scan_dpp_short(double x, bool head) {
+  const uint32_t lane = lane_id(), rl = lane & 15u;
+  uint32_t f = head ? 1u : 0u;
+  seg_scan_step<kDppRowShr1>(x, f);
+  if (__ballot(!f && rl >= 2u)) {
+    seg_scan_step<kDppRowShr2>(x, f);
+    if (__ballot(!f && rl >= 4u)) {
+      seg_scan_step<kDppRowShr4>(x, f);
+      if (__ballot(!f && rl >= 8u)) seg_scan_step<kDppRowShr8>(x, f);
+    }
+  }
+  if (__ballot(!f && lane >= 16u)) {
+    seg_scan_step<kDppRowBcast15, 0xA>(x, f);
+    seg_scan_step<kDppRowBcast31, 0xC>(x, f);
+  }
+  return x;
+}
+
 __device__ inline uint32_t read_lane_u32(uint32_t x, int lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
 }

@@ -114,8 +114,8 @@ int gmp_destroy(gmp_engine *h);
 #define GMP_OPT_WG_PER_CU 5  /* gather grid: workgroups per compute unit, 1..64 */
 #define GMP_OPT_VEC_MIN 6    /* gather plan: rows above this many edges are reduced wave-wide */
 #define GMP_OPT_STRIP_LOG 7  /* PageRank: source strips of 2^k ids, 8..15 (default 15; 0 = row gather) */
-#define GMP_OPT_BIN_LOG 8    /* PageRank strips: destination bins of up to 2^k rows, 8..14 (default 14) */
-#define GMP_OPT_BIN_TASK_LOG 9 /* PageRank strips: messages per bin task 2^k, 8..24 (default 18) */
+#define GMP_OPT_BIN_LOG 8    /* PageRank strips: destination bins of up to 2^k rows, 8..14 (default 13) */
+#define GMP_OPT_BIN_TASK_LOG 9 /* PageRank strips: messages per bin task 2^k, 8..22 (default 20) */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 

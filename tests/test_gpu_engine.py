@@ -264,11 +264,11 @@ def test_compressed_gather_is_bit_identical(engine, reorder):
         engine.set_option(gmp.OPT_STRIP_LOG, 15)
 
 
-STRIP_DEFAULTS = ((gmp.OPT_STRIP_LOG, 15), (gmp.OPT_BIN_LOG, 14), (gmp.OPT_BIN_TASK_LOG, 18))
+STRIP_DEFAULTS = ((gmp.OPT_STRIP_LOG, 15), (gmp.OPT_BIN_LOG, 13), (gmp.OPT_BIN_TASK_LOG, 20))
 
 
-@pytest.mark.parametrize("strip_log,bin_log,task_log", [(0, 14, 18), (8, 8, 8), (11, 10, 10), (15, 14, 18),
-                                                        (15, 14, 8)])
+@pytest.mark.parametrize("strip_log,bin_log,task_log", [(0, 13, 20), (8, 8, 8), (11, 10, 10), (15, 13, 20),
+                                                        (15, 14, 8), (15, 14, 22)])
 def test_source_strips_keep_results(engine, strip_log, bin_log, task_log):
     """The source-strip iteration (LDS-staged strips, per-segment messages, fixed-point bin
     sums, split bins) matches FP64 for every strip width, bin size and split size, and is
