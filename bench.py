@@ -7,10 +7,12 @@ iterations.  The graph (~1.07e9 samples, deduplicated, seeded relabel) is genera
 GPU and stays in HBM; the timed region starts with every input resident.
 
 value        whole-job PageRank GTEPS = edges * iterations / wall time of the K solves
-roofline     the gather kernel (pr_gather_kernel): algorithmic bytes per launch
-             (8 E + (o + 16) V, o = offset width) / mean launch time from HIP events on the
-             engine stream, against 8 TB/s; traffic = measured HBM bytes per launch from the
-             rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/ when present
+roofline     one PageRank iteration, i.e. the launch sequence pr_strip_gather_kernel ->
+             pr_bin_kernel -> pr_split_apply_kernel on the engine stream: algorithmic bytes
+             (8 E + (o + 16) V, o = offset width) / mean iteration time from HIP events
+             recorded on that stream around the three launches, against 8 TB/s; traffic =
+             measured HBM bytes of the three kernels per iteration from the rocprofv3
+             FETCH_SIZE/WRITE_SIZE summary in profiles/ when present
 cpu_baseline GraphMP's OpenMP engine shape (oracle restatement, edges in RAM) on the same
              graph, timed over a bounded number of iterations
 extra        SSSP (frontier Bellman-Ford and delta-stepping, delta=64) on the same graph with
@@ -37,8 +39,8 @@ def log(*a):
 
 
 def load_traffic(scale, ef):
-    """HBM bytes per gather launch from the committed rocprofv3 PMC summary, if any."""
-    p = os.path.join(ROOT, "profiles", f"pr_gather_rmat_s{scale}_ef{ef}_traffic.json")
+    """HBM bytes per PageRank iteration from the committed rocprofv3 PMC summary, if any."""
+    p = os.path.join(ROOT, "profiles", f"pr_strip_rmat_s{scale}_ef{ef}_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -153,6 +155,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--no-extra", action="store_true", help="skip the SSSP and CC measurements")
     ap.add_argument("--learn-scale", type=int, default=27, help="R-MAT scale of the learned-SSSP evaluation")
+    ap.add_argument("--strip-log", type=int, default=15,
+                    help="PageRank source strips of 2^k ids (0: the row gather over the in-CSR)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size "
                          "dispatch that scales the HBM byte counters under rocprofv3 --pmc)")
@@ -167,6 +171,7 @@ def main():
     scale, ef = args.scale, args.edge_factor
     seed = 1000 + scale
     eng = gmp.Engine(0)
+    eng.set_option(gmp.OPT_STRIP_LOG, args.strip_log)
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib << 20)
     t0 = time.perf_counter()
@@ -198,7 +203,7 @@ def main():
     achieved = bytes_launch / avg_launch_s / 1e9
     traffic = load_traffic(scale, ef)
     log(f"pagerank: {iters / args.steps:.0f} iterations/solve, {ms_step:.1f} ms/solve, {gteps:.1f} GTEPS, "
-        f"gather {avg_launch_s * 1e3:.3f} ms/launch = {achieved:.0f} GB/s")
+        f"iteration kernels {avg_launch_s * 1e3:.3f} ms = {achieved:.0f} GB/s")
     result = {
         "metric": "pagerank_gteps_rmat_s26",
         "value": round(gteps, 3),
@@ -217,13 +222,27 @@ def main():
                    "damping": 0.85, "max_iter": args.max_iter},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "pr_gather_kernel", "algorithmic_bytes_per_launch": bytes_launch,
+                     "kernel": ("pagerank iteration: pr_strip_gather_kernel + pr_bin_kernel + "
+                                "pr_split_apply_kernel" if args.strip_log else
+                                "pagerank iteration: pr_gather_kernel + pr_hub_combine_kernel"),
+                     "algorithmic_bytes_per_launch": bytes_launch,
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4)},
         "cpu_baseline": None,
         "extra": {},
     }
 
     if not args.no_extra:
+        # the row-by-row gather over the in-CSR (strip layout off), for comparison
+        eng.set_option(gmp.OPT_STRIP_LOG, 0)
+        eng.pagerank(damping=0.85, max_iter=3, want_ranks=False)
+        _, str_ = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, want_ranks=False)
+        eng.set_option(gmp.OPT_STRIP_LOG, args.strip_log)
+        kms_r = str_.ms_kernel / max(str_.launches, 1)
+        result["extra"]["pagerank_row_gather"] = {
+            "ms_per_iteration": round(kms_r, 4), "iterations": str_.iterations,
+            "yardstick_gbs": round(bytes_launch / kms_r / 1e6, 1),
+            "strip_speedup": round(kms_r / (avg_launch_s * 1e3), 3)}
+        log(f"pagerank row gather: {kms_r:.3f} ms/iteration ({kms_r / (avg_launch_s * 1e3):.2f}x the strip path)")
         eng.pagerank(damping=0.85, max_iter=3, compressed=True, want_ranks=False)  # builds the packed lists
         _, stc = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, compressed=True,
                               want_ranks=False)

@@ -122,6 +122,16 @@ struct StripCsr {
   bool valid = false;
 };
 
+// Vertices of the work out-CSR whose out-degree exceeds a threshold, with their edge lists cut
+// into fixed-size slices, so frontier relaxation spreads a hub over many waves (traverse.hip).
+struct HubTable {
+  DevBuf ids;     // uint32[H]: hub work ids, ascending
+  DevBuf slices;  // uint2[nslices]: {hub index, slice number}
+  DevBuf flag;    // uint32[H]: stamp of the relax launch whose frontier holds the hub
+  uint32_t H = 0, nslices = 0;
+  bool valid = false;
+};
+
 // Delta + varint column ids of the work in-CSR (decoded inside the gather).
 struct PackedCols {
   DevBuf bytes, gbyte, gmask;
@@ -142,8 +152,8 @@ struct Graph {
   uint32_t P = 1;
   std::vector<uint32_t> bounds;     // P+1
   std::vector<uint64_t> h_in_off;   // host copy of the in-CSR offsets
-  // work layout: inside every destination interval the vertices are renumbered by
-  // descending out-degree (ties by id), so the most-gathered sources share cache lines.
+  // work layout: the vertices are renumbered by descending out-degree (ties by id), so the
+  // most-gathered sources share cache lines and PageRank's source strips start with the hubs.
   // With reordering off the work layout is the loaded one.
   bool work_ready = false;
   bool work_identity = true;
@@ -155,6 +165,7 @@ struct Graph {
   GatherPlan plan;                  // over the work in-CSR
   PackedCols packed;                // compressed work in-CSR columns (built on demand)
   StripCsr strips;                  // source-strip layout (built on demand)
+  HubTable hubs;                    // hub slices of the work out-CSR (built on demand)
 
   CsrView view(const Csr &c) const { return CsrView{c.off.p, c.idx.as<uint32_t>(), c.w.as<uint32_t>()}; }
   CsrView work_in() const { return work_identity ? view(in) : view(win); }

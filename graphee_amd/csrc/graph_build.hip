@@ -373,6 +373,7 @@ void ensure_work(Engine &e) {
   g.wout_ready = false;
   g.packed.valid = false;
   g.strips.valid = false;
+  g.hubs.valid = false;
   g.plan.built = false;
   if (g.work_identity) {
     g.work_ready = true;

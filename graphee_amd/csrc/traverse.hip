@@ -31,6 +31,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kInf = 0xFFFFFFFFu;
+constexpr uint32_t kHubDegree = 1024;  // vertices above this out-degree relax in slices
+constexpr uint32_t kHubSlice = 2048;   // edges per hub slice (one wave)
 
 enum RelaxMode { kModeWeighted = 0, kModeUnit = 1, kModeMinLabel = 2 };
 
@@ -74,7 +76,58 @@ struct RelaxArgs {
   Counters *cnt;
   uint32_t V;
   int track_shards;       // record which 64ths of the id range join the near frontier
+  // hubs (out-degree > kHubDegree) leave the wave chunks; hub_relax_kernel splits their edges
+  const uint32_t *hub_ids;  // ascending work ids
+  uint32_t nhubs;
+  uint32_t *hub_flag;       // = stamp when the hub is in this launch's frontier
+  const uint2 *slices;      // {hub index, slice number}
+  uint32_t nslices;
 };
+
+// One edge relaxation: du is the source's value; sets v and whether v joins the near
+// frontier (first time this launch) or the far pile.
+template <class OffT, int MODE>
+__device__ inline void relax_edge(const RelaxArgs<OffT> &a, uint64_t edge, uint32_t du, uint32_t &v, bool &to_near,
+                                  bool &to_far) {
+  v = a.idx[edge];
+  uint32_t nd;
+  if (MODE == kModeMinLabel) {
+    nd = du;
+  } else {
+    const uint64_t t = (uint64_t)du + (MODE == kModeUnit ? 1u : a.w[edge]);
+    nd = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
+  }
+  if (nd < a.dist[v]) {
+    const uint32_t old = atomicMin(&a.dist[v], nd);
+    if (nd < old) {
+      if (nd < a.threshold) to_near = atomicExch(&a.near_stamp[v], a.stamp) != a.stamp;
+      else to_far = atomicExch(&a.far_flag[v], 1u) == 0u;
+    }
+  }
+}
+
+// Wave-aggregated appends of the relaxed vertices to the near frontier and the far pile.
+template <class OffT>
+__device__ inline void relax_append(const RelaxArgs<OffT> &a, uint32_t v, bool to_near, bool to_far,
+                                    unsigned long long &smask) {
+  const uint32_t sn = wave_append(to_near, &a.cnt->near_cnt);
+  if (to_near) a.near_out[sn] = v;
+  if (a.far_out) {
+    const uint32_t sf = wave_append(to_far, &a.cnt->far_cnt);
+    if (to_far) a.far_out[sf] = v;
+  }
+  if (a.track_shards && to_near) smask |= 1ull << (uint32_t)(((uint64_t)v * kMlpShards) / a.V);
+}
+
+template <class OffT>
+__device__ inline void relax_flush_counts(const RelaxArgs<OffT> &a, unsigned long long edges, unsigned long long smask) {
+  if (lane_id() == 0 && edges) atomicAdd(&a.cnt->edges, edges);
+  if (a.track_shards) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) smask |= (unsigned long long)__shfl_xor((long long)smask, o, kWave);
+    if (lane_id() == 0 && smask) atomicOr(&a.cnt->shard_mask, smask);
+  }
+}
 
 template <class OffT, int MODE>
 __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
@@ -95,6 +148,17 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
       deg = (uint32_t)((uint64_t)a.off[u + 1] - b);
       du = a.dist[u];
       if (MODE != kModeMinLabel && du == kInf) deg = 0;
+      if (deg > kHubDegree && a.nhubs) {
+        // a hub: its edges go to hub_relax_kernel's slices, spread over many waves
+        uint32_t lo = 0, hi = a.nhubs - 1;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (a.hub_ids[mid] < u) lo = mid + 1;
+          else hi = mid;
+        }
+        a.hub_flag[lo] = a.stamp;
+        deg = 0;
+      }
     }
     const uint32_t incl = wave_inclusive_scan(deg);
     const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
@@ -115,39 +179,38 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
           if (s_start[wv][mid] <= j) lo = mid;
           else hi = mid - 1;
         }
-        const uint64_t edge = s_base[wv][lo] + (j - s_start[wv][lo]);
-        v = a.idx[edge];
-        uint32_t nd;
-        if (MODE == kModeMinLabel) {
-          nd = s_val[wv][lo];
-        } else {
-          const uint64_t t = (uint64_t)s_val[wv][lo] + (MODE == kModeUnit ? 1u : a.w[edge]);
-          nd = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
-        }
-        if (nd < a.dist[v]) {
-          const uint32_t old = atomicMin(&a.dist[v], nd);
-          if (nd < old) {
-            if (nd < a.threshold) to_near = atomicExch(&a.near_stamp[v], a.stamp) != a.stamp;
-            else to_far = atomicExch(&a.far_flag[v], 1u) == 0u;
-          }
-        }
+        relax_edge<OffT, MODE>(a, s_base[wv][lo] + (j - s_start[wv][lo]), s_val[wv][lo], v, to_near, to_far);
       }
-      const uint32_t sn = wave_append(to_near, &a.cnt->near_cnt);
-      if (to_near) a.near_out[sn] = v;
-      if (a.far_out) {
-        const uint32_t sf = wave_append(to_far, &a.cnt->far_cnt);
-        if (to_far) a.far_out[sf] = v;
-      }
-      if (a.track_shards && to_near) smask |= 1ull << (uint32_t)(((uint64_t)v * kMlpShards) / a.V);
+      relax_append(a, v, to_near, to_far, smask);
     }
     wave_sync();
   }
-  if (lane == 0 && edges) atomicAdd(&a.cnt->edges, edges);
-  if (a.track_shards) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) smask |= (unsigned long long)__shfl_xor((long long)smask, o, kWave);
-    if (lane == 0 && smask) atomicOr(&a.cnt->shard_mask, smask);
+  relax_flush_counts(a, edges, smask);
+}
+
+// The edges of the hubs flagged by this launch's relax_kernel: one wave per slice of
+// kHubSlice edges, so a hub's edge list is spread over the whole chip.
+template <class OffT, int MODE>
+__global__ __launch_bounds__(kBlock) void hub_relax_kernel(RelaxArgs<OffT> a) {
+  const uint32_t lane = lane_id(), wv = wave_id();
+  unsigned long long edges = 0, smask = 0;
+  for (uint32_t t = blockIdx.x * kWavesPerBlock + wv; t < a.nslices; t += gridDim.x * kWavesPerBlock) {
+    const uint2 sl = a.slices[t];
+    if (a.hub_flag[sl.x] != a.stamp) continue;
+    const uint32_t u = a.hub_ids[sl.x];
+    const uint64_t rb = a.off[u], re = a.off[u + 1];
+    const uint64_t e0 = rb + (uint64_t)sl.y * kHubSlice, e1 = min(e0 + kHubSlice, re);
+    const uint32_t du = a.dist[u];
+    edges += e1 - e0;
+    for (uint64_t jb = e0; jb < e1; jb += kWave) {
+      const uint64_t j = jb + lane;
+      bool to_near = false, to_far = false;
+      uint32_t v = 0;
+      if (j < e1) relax_edge<OffT, MODE>(a, j, du, v, to_near, to_far);
+      relax_append(a, v, to_near, to_far, smask);
+    }
   }
+  relax_flush_counts(a, edges, smask);
 }
 
 // Histograms of the live far pile for the epoch features: distance above the pile minimum in
@@ -307,6 +370,75 @@ inline uint32_t grid_fill(Engine &e, uint64_t n) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + kBlock - 1) / kBlock, (uint64_t)e.num_cus * 16));
 }
 
+// hub candidates: {work id, out-degree} of every vertex above kHubDegree (unordered)
+template <class OffT>
+__global__ void hub_collect_kernel(const OffT *off, uint32_t V, uint32_t cap, uint32_t *cnt, uint2 *out) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+    const uint64_t d = (uint64_t)off[v + 1] - (uint64_t)off[v];
+    if (d > kHubDegree) {
+      const uint32_t i = atomicAdd(cnt, 1u);
+      if (i < cap) out[i] = make_uint2(v, d > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)d);
+    }
+  }
+}
+
+// Builds e.g.hubs for the work out-CSR (once per layout).
+template <class OffT>
+void ensure_hubs(Engine &e) {
+  Graph &g = e.g;
+  HubTable &T = g.hubs;
+  if (T.valid) return;
+  T = HubTable();
+  const CsrView ov = g.work_out();
+  const uint32_t V = g.V;
+  const uint32_t cap = (uint32_t)std::min<uint64_t>(g.E / (kHubDegree + 1) + 1, V);
+  DevBuf cand;
+  cand.alloc((size_t)cap * sizeof(uint2) + 16);
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(cand.as<char>() + (size_t)cap * sizeof(uint2));
+  GMP_HIP(hipMemsetAsync(cnt, 0, 4, e.stream));
+  hub_collect_kernel<OffT><<<grid_fill(e, V), kBlock, 0, e.stream>>>(ov.offs<OffT>(), V, cap, cnt, cand.as<uint2>());
+  GMP_CHECK_LAUNCH();
+  uint32_t H = 0;
+  GMP_HIP(hipMemcpyAsync(&H, cnt, 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  if (H > cap) fail(GMP_ESTATE, "hub table overflow");
+  std::vector<uint2> hub(H);
+  if (H) GMP_HIP(hipMemcpy(hub.data(), cand.p, (size_t)H * sizeof(uint2), hipMemcpyDeviceToHost));
+  std::sort(hub.begin(), hub.end(), [](const uint2 &p, const uint2 &q) { return p.x < q.x; });
+  std::vector<uint32_t> ids(H);
+  std::vector<uint2> sl;
+  for (uint32_t h = 0; h < H; h++) {
+    ids[h] = hub[h].x;
+    for (uint32_t k = 0; (uint64_t)k * kHubSlice < hub[h].y; k++) sl.push_back(make_uint2(h, k));
+  }
+  T.H = H;
+  T.nslices = (uint32_t)sl.size();
+  if (H) {
+    T.ids.alloc((size_t)H * 4);
+    T.flag.alloc((size_t)H * 4);
+    T.slices.alloc(sl.size() * sizeof(uint2));
+    GMP_HIP(hipMemcpy(T.ids.p, ids.data(), (size_t)H * 4, hipMemcpyHostToDevice));
+    GMP_HIP(hipMemcpy(T.slices.p, sl.data(), sl.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  }
+  T.valid = true;
+}
+
+// Points the relax arguments at the hub table and clears its flags (stamps restart per run).
+template <class OffT>
+void attach_hubs(Engine &e, RelaxArgs<OffT> &a) {
+  ensure_hubs<OffT>(e);
+  HubTable &T = e.g.hubs;
+  a.nhubs = T.H;
+  a.nslices = T.nslices;
+  a.hub_ids = T.ids.as<uint32_t>();
+  a.hub_flag = T.flag.as<uint32_t>();
+  a.slices = T.slices.as<uint2>();
+  if (T.H) {
+    fill_u32_kernel<<<grid_fill(e, T.H), kBlock, 0, e.stream>>>(a.hub_flag, T.H, 0u);
+    GMP_CHECK_LAUNCH();
+  }
+}
+
 template <class OffT>
 void launch_relax(Engine &e, int mode, const RelaxArgs<OffT> &a) {
   const uint32_t nchunks = (a.nfront + kWave - 1) / kWave;
@@ -316,6 +448,14 @@ void launch_relax(Engine &e, int mode, const RelaxArgs<OffT> &a) {
   else if (mode == kModeUnit) relax_kernel<OffT, kModeUnit><<<grid, kBlock, 0, e.stream>>>(a);
   else relax_kernel<OffT, kModeMinLabel><<<grid, kBlock, 0, e.stream>>>(a);
   GMP_CHECK_LAUNCH();
+  if (a.nslices) {
+    const uint32_t gh = std::max<uint32_t>(
+        1, std::min<uint32_t>((a.nslices + kWavesPerBlock - 1) / kWavesPerBlock, (uint32_t)e.num_cus * 8));
+    if (mode == kModeWeighted) hub_relax_kernel<OffT, kModeWeighted><<<gh, kBlock, 0, e.stream>>>(a);
+    else if (mode == kModeUnit) hub_relax_kernel<OffT, kModeUnit><<<gh, kBlock, 0, e.stream>>>(a);
+    else hub_relax_kernel<OffT, kModeMinLabel><<<gh, kBlock, 0, e.stream>>>(a);
+    GMP_CHECK_LAUNCH();
+  }
 }
 
 void read_counters(Engine &e, Counters *dev, Counters &h) {
@@ -343,6 +483,7 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
   const uint32_t delta = p.delta ? p.delta : 64u;
   const uint32_t V = g.V;
   src = work_id(e, src);
+  ensure_hubs<OffT>(e);
   TraverseBufs b = alloc_traverse(e, true);
   const CsrView ov = g.work_out();
   const OffT *ooff = ov.offs<OffT>();
@@ -380,6 +521,7 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
   a.far_flag = b.farflag;
   a.cnt = b.cnt;
   a.V = V;
+  attach_hubs(e, a);
   int iters = 0, epochs = 0;
   Counters h{};
 
@@ -555,6 +697,7 @@ void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
   Graph &g = e.g;
   ensure_work_out(e);
   const uint32_t V = g.V;
+  ensure_hubs<OffT>(e);
   TraverseBufs b = alloc_traverse(e, false);
   const CsrView ov = g.work_out();
   const OffT *ooff = ov.offs<OffT>();
@@ -573,6 +716,7 @@ void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
   a.near_stamp = b.stamp;
   a.threshold = kInf;
   a.cnt = b.cnt;
+  attach_hubs(e, a);
   uint32_t n = V, stamp = 1;
   const uint32_t *front = nullptr;  // every vertex starts active
   uint32_t *qa = b.q0, *qb = b.q1;
