@@ -78,13 +78,13 @@ def batched_workload(seed=5000):
     return specs, gs, gp, rng
 
 
-def run_learned(eng, eval_scale, nsources=8, train_scales=(22, 23, 24)):
+def run_learned(eng, eval_scale, nsources=8, train_scales=(24, 25, 26), trace_sources=2):
     """Learned delta-stepping: traces from SSSP runs at train_scales, MLP trained on the GPU,
     then fixed delta=64 vs the learned schedule on R-MAT eval_scale for nsources seeded sources
     (distances must be identical)."""
     from graphee_amd import learned
     t0 = time.perf_counter()
-    rows = learned.record(eng, [(s, 16, 1000 + s) for s in train_scales], sources_per_graph=2, seed=27)
+    rows = learned.record(eng, [(s, 16, 1000 + s) for s in train_scales], sources_per_graph=trace_sources, seed=27)
     t_rec = time.perf_counter() - t0
     t0 = time.perf_counter()
     loss = learned.train(eng, rows, epochs=400)

@@ -128,6 +128,7 @@ struct HubTable {
   DevBuf ids;     // uint32[H]: hub work ids, ascending
   DevBuf slices;  // uint2[nslices]: {hub index, slice number}
   DevBuf flag;    // uint32[H]: stamp of the relax launch whose frontier holds the hub
+  DevBuf list;    // uint32[nslices]: slices flagged in the current launch
   uint32_t H = 0, nslices = 0;
   bool valid = false;
 };

@@ -41,6 +41,8 @@ struct Counters {
   uint32_t far_cnt;
   uint32_t far_min;
   uint32_t choice;                  // policy output: bucket-width class
+  uint32_t hub_cnt;                 // flagged hub slices of the current relax launch
+  uint32_t pad;
   unsigned long long edges;
   unsigned long long shard_mask;    // OR of 1 << (64 v / V) over vertices appended to the near frontier
   unsigned long long settled;       // vertices below a threshold (trace exploration)
@@ -82,6 +84,7 @@ struct RelaxArgs {
   uint32_t *hub_flag;       // = stamp when the hub is in this launch's frontier
   const uint2 *slices;      // {hub index, slice number}
   uint32_t nslices;
+  uint32_t *hub_list;       // flagged slices of this launch (hub_compact_kernel)
 };
 
 // One edge relaxation: du is the source's value; sets v and whether v joins the near
@@ -137,6 +140,7 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
   const uint32_t lane = lane_id(), wv = wave_id();
   const uint32_t nchunks = (a.nfront + kWave - 1) / kWave;
   unsigned long long edges = 0, smask = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt->hub_cnt = 0;  // hub_compact_kernel counts after us
   for (uint32_t c = blockIdx.x * kWavesPerBlock + wv; c < nchunks; c += gridDim.x * kWavesPerBlock) {
     const uint32_t i = c * kWave + lane;
     const bool ok = i < a.nfront;
@@ -188,15 +192,26 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
   relax_flush_counts(a, edges, smask);
 }
 
-// The edges of the hubs flagged by this launch's relax_kernel: one wave per slice of
-// kHubSlice edges, so a hub's edge list is spread over the whole chip.
+// Lists the slices of the hubs flagged by this launch's relax_kernel (one thread per slice).
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void hub_compact_kernel(RelaxArgs<OffT> a) {
+  const uint32_t n = (a.nslices + kBlock - 1) / kBlock * kBlock;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const bool f = t < a.nslices && a.hub_flag[a.slices[t].x] == a.stamp;
+    const uint32_t p = wave_append(f, &a.cnt->hub_cnt);
+    if (f) a.hub_list[p] = t;
+  }
+}
+
+// The listed hub slices: one wave per slice of kHubSlice edges, so a hub's edge list is
+// spread over the whole chip.
 template <class OffT, int MODE>
 __global__ __launch_bounds__(kBlock) void hub_relax_kernel(RelaxArgs<OffT> a) {
   const uint32_t lane = lane_id(), wv = wave_id();
+  const uint32_t nlist = a.cnt->hub_cnt;
   unsigned long long edges = 0, smask = 0;
-  for (uint32_t t = blockIdx.x * kWavesPerBlock + wv; t < a.nslices; t += gridDim.x * kWavesPerBlock) {
-    const uint2 sl = a.slices[t];
-    if (a.hub_flag[sl.x] != a.stamp) continue;
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wv; i < nlist; i += gridDim.x * kWavesPerBlock) {
+    const uint2 sl = a.slices[a.hub_list[i]];
     const uint32_t u = a.hub_ids[sl.x];
     const uint64_t rb = a.off[u], re = a.off[u + 1];
     const uint64_t e0 = rb + (uint64_t)sl.y * kHubSlice, e1 = min(e0 + kHubSlice, re);
@@ -417,6 +432,7 @@ void ensure_hubs(Engine &e) {
     T.ids.alloc((size_t)H * 4);
     T.flag.alloc((size_t)H * 4);
     T.slices.alloc(sl.size() * sizeof(uint2));
+    T.list.alloc(sl.size() * 4);
     GMP_HIP(hipMemcpy(T.ids.p, ids.data(), (size_t)H * 4, hipMemcpyHostToDevice));
     GMP_HIP(hipMemcpy(T.slices.p, sl.data(), sl.size() * sizeof(uint2), hipMemcpyHostToDevice));
   }
@@ -433,6 +449,7 @@ void attach_hubs(Engine &e, RelaxArgs<OffT> &a) {
   a.hub_ids = T.ids.as<uint32_t>();
   a.hub_flag = T.flag.as<uint32_t>();
   a.slices = T.slices.as<uint2>();
+  a.hub_list = T.list.as<uint32_t>();
   if (T.H) {
     fill_u32_kernel<<<grid_fill(e, T.H), kBlock, 0, e.stream>>>(a.hub_flag, T.H, 0u);
     GMP_CHECK_LAUNCH();
@@ -449,6 +466,8 @@ void launch_relax(Engine &e, int mode, const RelaxArgs<OffT> &a) {
   else relax_kernel<OffT, kModeMinLabel><<<grid, kBlock, 0, e.stream>>>(a);
   GMP_CHECK_LAUNCH();
   if (a.nslices) {
+    hub_compact_kernel<OffT><<<grid_fill(e, a.nslices), kBlock, 0, e.stream>>>(a);
+    GMP_CHECK_LAUNCH();
     const uint32_t gh = std::max<uint32_t>(
         1, std::min<uint32_t>((a.nslices + kWavesPerBlock - 1) / kWavesPerBlock, (uint32_t)e.num_cus * 8));
     if (mode == kModeWeighted) hub_relax_kernel<OffT, kModeWeighted><<<gh, kBlock, 0, e.stream>>>(a);
@@ -466,9 +485,10 @@ void read_counters(Engine &e, Counters *dev, Counters &h) {
 
 // Bucket epochs of delta-stepping.  FIXED: aligned buckets [k*delta, (k+1)*delta).  LEARNED:
 // the MLP picks each epoch's bucket [m, m + 8 * 2^c) from far-pile features (m = smallest live
-// far distance).  EXPLORE records traces: from a snapshot of the state every width class runs
-// the epoch, the class settling the most vertices per millisecond becomes the label and its run
-// continues.  Every policy relaxes until no distance drops, so distances are the same.
+// far distance).  EXPLORE records traces: at checkpoint epochs, from a snapshot of the state,
+// every width class runs the epoch and then the widest class runs to the end of the solve; the
+// class with the shortest such rollout is the label and the run continues with it.  Every
+// policy relaxes until no distance drops, so distances are the same.
 enum Policy { kPolicyBF = 0, kPolicyFixed = 1, kPolicyLearned = 2, kPolicyExplore = 3 };
 
 template <class OffT>
@@ -585,14 +605,15 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
     read_counters(e, b.cnt, h);
     return h.far_min;
   };
-  // the rest of the run with one constant width class (trace rollouts)
-  auto run_constant = [&](uint32_t m, uint32_t c) {
+  // trace rollout: one epoch of width class c, then the widest class to the end of the run
+  auto run_rollout = [&](uint32_t m, uint32_t c) {
     for (;;) {
       split(thr, width_end(m, c));
       relax_until_empty(false);
       if (nfar == 0) return;
       m = far_min();
       if (m == kInf) return;
+      c = (uint32_t)kMlpClasses - 1;
     }
   };
   // explored epochs: the first four, then powers of two
@@ -652,7 +673,7 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
         restore();
         EventPair tr;
         GMP_HIP(hipEventRecord(tr.a, e.stream));
-        run_constant(m, c);
+        run_rollout(m, c);
         GMP_HIP(hipEventRecord(tr.b, e.stream));
         GMP_HIP(hipStreamSynchronize(e.stream));
         read_counters(e, b.cnt, h);

@@ -1,11 +1,12 @@
 """Learned delta-stepping schedule: record traces, train the MLP on the GPU, run it.
 
-The engine records one trace row per bucket epoch of SSSP (`Engine.sssp_trace`): the 16
-features of the far pile, the time and settled-vertex count of every width class tried from the
-same state, the best class and the shards that became active.  `train` fits the network with the
-engine's own HIP forward/backward/Adam kernels; `Engine.sssp(method=SSSP_LEARNED)` then runs
-inference on the GPU at every epoch.  The network only picks bucket widths, so distances equal
-those of every other method.
+The engine records a trace row at checkpoint bucket epochs of SSSP (`Engine.sssp_trace`): the
+16 features of the far pile, and for every width class the time and relaxations of finishing the
+solve from that state (one epoch of the class, then the widest class), the fastest class and the
+shards that became active.  `train` fits the network with the engine's own HIP
+forward/backward/Adam kernels; `Engine.sssp(method=SSSP_LEARNED)` then runs inference on the GPU
+at every epoch.  The network only picks bucket widths, so distances equal those of every other
+method.
 """
 import numpy as np
 
@@ -25,10 +26,33 @@ def record(engine, graphs, sources_per_graph=2, seed=0):
     return np.concatenate(rows) if rows else np.zeros(0)
 
 
-def dataset(rows):
+def labels(rows, tol=0.1):
+    """Training label of every row, from the rollout times of all width classes.
+
+    Each row's times are taken relative to its fastest class.  Rows with identical features (the
+    first epoch of every run looks the same: the far pile holds only the source) must share one
+    decision, so they are pooled and get the class with the smallest summed relative time.  Among
+    classes within `tol` of the best, the narrowest is taken: neighbouring widths often tie on
+    small graphs, and narrow early buckets are what pays off on larger graphs (where the
+    rollouts separate them), so a wider class has to win clearly to become the label."""
+    ms = np.asarray(rows["ms"], dtype=np.float64)
+    if not len(ms):
+        return np.zeros(0, dtype=np.int32)
+    rel = ms / ms.min(axis=1, keepdims=True)
+    feat = np.ascontiguousarray(rows["feat"], dtype=np.float32)
+    _, group = np.unique(feat.view(np.dtype((np.void, feat.dtype.itemsize * feat.shape[1]))).ravel(),
+                         return_inverse=True)
+    cost = np.zeros((group.max() + 1, ms.shape[1]))
+    np.add.at(cost, group, rel)
+    cost = cost[group]
+    ok = cost <= cost.min(axis=1, keepdims=True) * (1.0 + tol)
+    return ok.argmax(axis=1).astype(np.int32)
+
+
+def dataset(rows, tol=0.1):
     """(X, labels, shard masks) of a trace."""
     return (np.ascontiguousarray(rows["feat"], dtype=np.float32),
-            np.ascontiguousarray(rows["best"], dtype=np.int32),
+            np.ascontiguousarray(labels(rows, tol)),
             np.ascontiguousarray(rows["shard_mask"], dtype=np.uint64))
 
 

@@ -182,9 +182,9 @@ int gmp_cc(gmp_engine *h, uint32_t *label_out, gmp_stats *st);
  *
  * gmp_sssp_trace runs SSSP while recording a row at explored bucket epochs (the first four,
  * then every power of two): the 16 features, and for every width class the device time and
- * relaxations of finishing the run from that state with that constant width; the best class
- * (fastest finish), which the run then uses, and the shards (64ths of the vertex range) that
- * joined the frontier in that epoch. */
+ * relaxations of finishing the run from that state with one epoch of that width followed by
+ * the widest class; the best class (fastest finish), which the run then uses, and the shards
+ * (64ths of the vertex range) that joined the frontier in that epoch. */
 #define GMP_MLP_IN 16
 #define GMP_MLP_HIDDEN 64
 #define GMP_MLP_CLASSES 8
@@ -195,7 +195,7 @@ typedef struct gmp_trace_row {
   int32_t best;         /* best width class: delta = 8 << best */
   uint32_t epoch;
   uint64_t shard_mask;  /* bit j: a vertex of shard j joined the near frontier */
-  float ms[GMP_MLP_CLASSES];      /* time to finish with each constant width class */
+  float ms[GMP_MLP_CLASSES];      /* time to finish: one epoch of each class, then the widest */
   float relaxed[GMP_MLP_CLASSES]; /* relaxations to finish with each class */
 } gmp_trace_row;
 int gmp_sssp_trace(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, gmp_trace_row *rows,
