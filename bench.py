@@ -157,6 +157,8 @@ def main():
     ap.add_argument("--learn-scale", type=int, default=27, help="R-MAT scale of the learned-SSSP evaluation")
     ap.add_argument("--strip-log", type=int, default=15,
                     help="PageRank source strips of 2^k ids (0: the row gather over the in-CSR)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the autotuner pass over the strip layout before the warm-up")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size "
                          "dispatch that scales the HBM byte counters under rocprofv3 --pmc)")
@@ -178,6 +180,14 @@ def main():
     eng.generate_rmat(scale, ef, seed, weighted=True, num_shards=16)
     V, E, P, _ = eng.info()
     log(f"generated R-MAT s{scale} ef{ef}: V={V} E={E} in {time.perf_counter() - t0:.1f} s")
+
+    tuned = None
+    if args.autotune and args.strip_log:
+        t0 = time.perf_counter()
+        tuned = eng.autotune(iters=5).as_dict()
+        tuned["seconds"] = round(time.perf_counter() - t0, 1)
+        log(f"autotune: strip 2^{tuned['strip_log']}, bin 2^{tuned['bin_log']}, task 2^{tuned['bin_task_log']}, "
+            f"{tuned['gather_ms_before']:.3f} -> {tuned['gather_ms_after']:.3f} ms/iteration in {tuned['seconds']} s")
 
     def solve():
         return eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, want_ranks=False)
@@ -223,20 +233,21 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": ("pagerank iteration: pr_strip_gather_kernel + pr_bin_kernel + "
-                                "pr_split_apply_kernel" if args.strip_log else
+                                "pr_split_apply_kernel" if eng.get_option(gmp.OPT_STRIP_LOG) else
                                 "pagerank iteration: pr_gather_kernel + pr_hub_combine_kernel"),
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4)},
         "cpu_baseline": None,
-        "extra": {},
+        "extra": {"autotune": tuned},
     }
 
     if not args.no_extra:
         # the row-by-row gather over the in-CSR (strip layout off), for comparison
+        strip_now = eng.get_option(gmp.OPT_STRIP_LOG)
         eng.set_option(gmp.OPT_STRIP_LOG, 0)
         eng.pagerank(damping=0.85, max_iter=3, want_ranks=False)
         _, str_ = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, want_ranks=False)
-        eng.set_option(gmp.OPT_STRIP_LOG, args.strip_log)
+        eng.set_option(gmp.OPT_STRIP_LOG, strip_now)
         kms_r = str_.ms_kernel / max(str_.launches, 1)
         result["extra"]["pagerank_row_gather"] = {
             "ms_per_iteration": round(kms_r, 4), "iterations": str_.iterations,

@@ -85,7 +85,9 @@ TRACE_DTYPE = np.dtype([("feat", np.float32, (16,)), ("best", np.int32), ("epoch
 class TuneResult(ctypes.Structure):
     _fields_ = [("stream_cap", ctypes.c_uint32), ("row_cap", ctypes.c_uint32), ("chunk", ctypes.c_uint32),
                 ("vec_min", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32), ("trials", ctypes.c_uint32),
-                ("gather_ms_before", ctypes.c_double), ("gather_ms_after", ctypes.c_double)]
+                ("gather_ms_before", ctypes.c_double), ("gather_ms_after", ctypes.c_double),
+                ("strip_log", ctypes.c_uint32), ("bin_log", ctypes.c_uint32), ("bin_task_log", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

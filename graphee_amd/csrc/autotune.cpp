@@ -1,6 +1,8 @@
-// Autotuner for the PageRank gather: picks the degree-bin threshold (vec_min), the LDS tile
-// (stream_cap rows' edges per wave), the hub chunk size and the workgroups per CU on the
-// loaded graph by measuring the gather kernel with HIP events.  GraphMP fixes the matching
+// Autotuner for the PageRank iteration: picks the source-strip width, the destination-bin size
+// and the bin-task size of the strip layout (or the row gather, strip width 0), and for the row
+// gather the degree-bin threshold (vec_min), the LDS tile (stream_cap rows' edges per wave), the
+// hub chunk size and the workgroups per CU, on the loaded graph, by measuring iterations with
+// HIP events.  GraphMP fixes the matching
 // knobs at compile time (CMPNUM threads, GraphMP/system/Global.h:58); nothing here changes
 // results, only the task plan and the grid.
 //
@@ -31,12 +33,23 @@ double time_gather(Engine &e, int iters) {
 
 uint32_t *field(Options &o, int k) {
   switch (k) {
-    case 0: return &o.vec_min;
-    case 1: return &o.stream_cap;
-    case 2: return &o.chunk;
-    case 3: return &o.wg_per_cu;
+    case 0: return &o.strip_log;
+    case 1: return &o.bin_log;
+    case 2: return &o.bin_task_log;
+    case 3: return &o.vec_min;
+    case 4: return &o.stream_cap;
+    case 5: return &o.chunk;
+    case 6: return &o.wg_per_cu;
     default: return &o.row_cap;
   }
+}
+constexpr int kStripKnobs = 3;  // fields 0..2 shape the strip layout, the rest the row gather
+
+// Makes o the engine's options, dropping the strip layout when its shape changes.
+void apply(Engine &e, const Options &o) {
+  if (o.strip_log != e.opt.strip_log || o.bin_log != e.opt.bin_log || o.bin_task_log != e.opt.bin_task_log)
+    e.g.strips.valid = false;
+  e.opt = o;
 }
 
 }  // namespace
@@ -45,6 +58,9 @@ void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
   if (iters <= 0) iters = 3;
   ensure_work(e);
   const std::vector<std::vector<uint32_t>> cand = {
+      {0, 13, 14, 15},                  // strip_log (0: row gather)
+      {12, 13, 14},                     // bin_log
+      {18, 19, 20, 21},                 // bin_task_log
       {16, 32, 64, 128, 256, 512},      // vec_min
       {512, 1024, 2048, 4096},          // stream_cap
       {1024, 2048, 4096, 8192, 16384},  // chunk
@@ -61,9 +77,11 @@ void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
       for (uint32_t v : cand[k]) {
         Options o = best;
         if (*field(o, (int)k) == v) continue;
+        // the strip knobs matter only with strips on, the gather knobs only without
+        if ((int)k >= kStripKnobs ? best.strip_log != 0 : ((int)k > 0 && best.strip_log == 0)) continue;
         *field(o, (int)k) = v;
         if (o.vec_min > o.stream_cap) continue;
-        e.opt = o;
+        apply(e, o);
         const double ms = time_gather(e, iters);
         trials++;
         if (ms > 0 && ms < best_ms * 0.995) {
@@ -75,9 +93,12 @@ void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
     }
     if (!improved) break;
   }
-  e.opt = best;
+  apply(e, best);
   if (out) {
     std::memset(out, 0, sizeof *out);
+    out->strip_log = best.strip_log;
+    out->bin_log = best.bin_log;
+    out->bin_task_log = best.bin_task_log;
     out->stream_cap = best.stream_cap;
     out->row_cap = best.row_cap;
     out->chunk = best.chunk;
@@ -94,7 +115,8 @@ void tune_save(Engine &e, const std::string &path) {
   if (!f) fail(GMP_EIO, path + ": cannot open for writing");
   f << "{\"reorder\": " << e.opt.reorder << ", \"stream_cap\": " << e.opt.stream_cap
     << ", \"row_cap\": " << e.opt.row_cap << ", \"chunk\": " << e.opt.chunk << ", \"vec_min\": " << e.opt.vec_min
-    << ", \"wg_per_cu\": " << e.opt.wg_per_cu << "}\n";
+    << ", \"wg_per_cu\": " << e.opt.wg_per_cu << ", \"strip_log\": " << e.opt.strip_log
+    << ", \"bin_log\": " << e.opt.bin_log << ", \"bin_task_log\": " << e.opt.bin_task_log << "}\n";
   if (!f) fail(GMP_EIO, path + ": short write");
 }
 
@@ -123,13 +145,17 @@ void tune_load(Engine &e, const std::string &path) {
   get("chunk", 64, 1u << 20, o.chunk);
   get("vec_min", 1, 4096, o.vec_min);
   get("wg_per_cu", 1, 64, o.wg_per_cu);
+  get("strip_log", 0, 15, o.strip_log);
+  get("bin_log", 8, 14, o.bin_log);
+  get("bin_task_log", 8, 22, o.bin_task_log);
+  if (o.strip_log != 0 && o.strip_log < 8) fail(GMP_EFORMAT, path + ": strip_log out of range");
   if ((int)reorder != e.opt.reorder) {
     e.g.work_ready = false;
     e.g.wout_ready = false;
     e.g.plan.built = false;
   }
   o.reorder = (int)reorder;
-  e.opt = o;
+  apply(e, o);
 }
 
 }  // namespace gmp

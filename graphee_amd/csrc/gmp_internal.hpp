@@ -188,7 +188,7 @@ struct Options {
   uint32_t vec_min = 128;      // gather plan: rows above this many edges go wave-wide
   uint32_t wg_per_cu = 16;     // gather grid = wg_per_cu * CUs
   uint32_t strip_log = 15;     // PageRank source strips of 2^strip_log ids, 8..15 (0: row gather)
-  uint32_t bin_log = 13;       // PageRank destination bins of up to 2^bin_log rows, 8..14
+  uint32_t bin_log = 14;       // PageRank destination bins of up to 2^bin_log rows, 8..14
   uint32_t bin_task_log = 20;  // PageRank bins with more than 2^k messages are split, 8..22
 };
 

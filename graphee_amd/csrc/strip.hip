@@ -243,7 +243,9 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &
   // room for the round plus a free slot (needed only when some edge is not an end)
   if (B.fill + total >= kRound) flush_messages(a, B);
   const uint32_t o0 = B.fill + incl - cnt;  // this lane's first slot
-  const uint32_t spare = B.fill + total;    // the free slot: non-ends write there
+  // non-ends write a free slot past the round's messages, one per lane where the buffer has
+  // room (lanes writing one LDS address would serialise)
+  const uint32_t spare = B.fill + total + min(lane, kRound - 1 - min(B.fill + total, kRound - 1));
   const uint32_t kf = ends ? (uint32_t)__ffs(ends) - 1u : 8u;
   double run = 0.0, first = 0.0;
 #pragma unroll

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define GMP_ABI_VERSION 1
+#define GMP_ABI_VERSION 2
 
 /* status codes */
 #define GMP_OK 0
@@ -114,7 +114,7 @@ int gmp_destroy(gmp_engine *h);
 #define GMP_OPT_WG_PER_CU 5  /* gather grid: workgroups per compute unit, 1..64 */
 #define GMP_OPT_VEC_MIN 6    /* gather plan: rows above this many edges are reduced wave-wide */
 #define GMP_OPT_STRIP_LOG 7  /* PageRank: source strips of 2^k ids, 8..15 (default 15; 0 = row gather) */
-#define GMP_OPT_BIN_LOG 8    /* PageRank strips: destination bins of up to 2^k rows, 8..14 (default 13) */
+#define GMP_OPT_BIN_LOG 8    /* PageRank strips: destination bins of up to 2^k rows, 8..14 (default 14) */
 #define GMP_OPT_BIN_TASK_LOG 9 /* PageRank strips: messages per bin task 2^k, 8..22 (default 20) */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
@@ -125,7 +125,8 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
  * and reports it.  Results of every app are unchanged by any choice. */
 typedef struct gmp_tune_result {
   uint32_t stream_cap, row_cap, chunk, vec_min, wg_per_cu, trials;
-  double gather_ms_before, gather_ms_after;
+  double gather_ms_before, gather_ms_after; /* mean PageRank iteration time, first and best */
+  uint32_t strip_log, bin_log, bin_task_log, reserved;
 } gmp_tune_result;
 int gmp_autotune(gmp_engine *h, int32_t iters, gmp_tune_result *out);
 /* Tunables as a flat JSON object, written after tuning and read at start-up. */

@@ -264,7 +264,7 @@ def test_compressed_gather_is_bit_identical(engine, reorder):
         engine.set_option(gmp.OPT_STRIP_LOG, 15)
 
 
-STRIP_DEFAULTS = ((gmp.OPT_STRIP_LOG, 15), (gmp.OPT_BIN_LOG, 13), (gmp.OPT_BIN_TASK_LOG, 20))
+STRIP_DEFAULTS = ((gmp.OPT_STRIP_LOG, 15), (gmp.OPT_BIN_LOG, 14), (gmp.OPT_BIN_TASK_LOG, 20))
 
 
 @pytest.mark.parametrize("strip_log,bin_log,task_log", [(0, 13, 20), (8, 8, 8), (11, 10, 10), (15, 13, 20),
@@ -308,7 +308,8 @@ def test_source_strips_keep_results(engine, strip_log, bin_log, task_log):
 
 
 def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
-    keys = (gmp.OPT_STREAM_CAP, gmp.OPT_ROW_CAP, gmp.OPT_CHUNK, gmp.OPT_WG_PER_CU, gmp.OPT_VEC_MIN)
+    keys = (gmp.OPT_STREAM_CAP, gmp.OPT_ROW_CAP, gmp.OPT_CHUNK, gmp.OPT_WG_PER_CU, gmp.OPT_VEC_MIN,
+            gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG)
     saved = [engine.get_option(k) for k in keys]
     try:
         g = gen(engine, 14)
@@ -316,7 +317,8 @@ def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
         res = engine.autotune(iters=2)
         assert res.trials > 1 and res.gather_ms_after <= res.gather_ms_before
         assert [engine.get_option(k) for k in keys] == [res.stream_cap, res.row_cap, res.chunk, res.wg_per_cu,
-                                                         res.vec_min]
+                                                         res.vec_min, res.strip_log, res.bin_log, res.bin_task_log]
+        assert res.strip_log in (0, 13, 14, 15)
         r, _ = engine.pagerank(max_iter=20)
         x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
         assert np.abs(r.astype(np.float64) - x).sum() < PR_L1_TOL
@@ -326,6 +328,7 @@ def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
             engine.set_option(k, v)
         engine.tune_load(p)
         assert engine.get_option(gmp.OPT_CHUNK) == res.chunk
+        assert engine.get_option(gmp.OPT_STRIP_LOG) == res.strip_log
         p.write_text('{"chunk": 3}')
         with pytest.raises(gmp.GmpError) as ei:
             engine.tune_load(p)
