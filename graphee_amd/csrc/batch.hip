@@ -18,6 +18,7 @@
 #include <numeric>
 
 #include "gmp_internal.hpp"
+#include "pr_common.hpp"
 #include "wave.hpp"
 
 namespace gmp {
@@ -74,13 +75,24 @@ __device__ inline uint32_t instance_of(const uint32_t *slot_base, uint32_t n, ui
   return lo;
 }
 
+constexpr int kRelaxUnroll = 4;    // edges per lane in flight in the relax loop
+constexpr uint32_t kLdsInst = 1024;  // instance slot bases kept in LDS up to this many instances
+
 // One round for every instance: reads queue[round&1] and appends improved slots to the other
-// queue; the last workgroup to finish flips the round and clears the consumed count.
+// queue; the last workgroup to finish flips the round and clears the consumed count.  A wave
+// takes 64 frontier slots and expands their edges kRelaxUnroll per lane at a time: the edge,
+// weight and distance loads of those edges are issued together (edges past the end reload the
+// wave's first edge and are dropped), then the improving ones are applied with atomics.
 __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
   __shared__ uint32_t s_start[kWavesPerBlock][kWave];
   __shared__ uint64_t s_base[kWavesPerBlock][kWave];
   __shared__ uint32_t s_val[kWavesPerBlock][kWave];
   __shared__ uint32_t s_sb[kWavesPerBlock][kWave];  // slot base minus union vertex base
+  __shared__ uint32_t s_islot[kLdsInst];
+  const bool lds_inst = a.ninst <= kLdsInst;
+  if (lds_inst)
+    for (uint32_t i = threadIdx.x; i < a.ninst; i += blockDim.x) s_islot[i] = a.inst_slot[i];
+  __syncthreads();
   const uint32_t round = a.count[2];
   const uint32_t cur = round & 1u, nxt = cur ^ 1u;
   const uint32_t nfront = a.count[cur];
@@ -95,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
     uint32_t deg = 0, du = kInf, sb = 0;
     if (i < nfront) {
       const uint32_t s = front[i];
-      const uint32_t inst = instance_of(a.inst_slot, a.ninst, s);
+      const uint32_t inst = lds_inst ? instance_of(s_islot, a.ninst, s) : instance_of(a.inst_slot, a.ninst, s);
       const uint32_t u = a.inst_vbase[inst] + (s - a.inst_slot[inst]);
       sb = a.inst_slot[inst] - a.inst_vbase[inst];
       b = a.off[u];
@@ -111,28 +123,68 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
     s_sb[wv][lane] = sb;
     wave_sync();
     edges += total;
-    for (uint32_t jb = 0; jb < total; jb += kWave) {
-      const uint32_t j = jb + lane;
-      bool app = false;
-      uint32_t slot = 0;
-      if (j < total) {
+    for (uint32_t jb = 0; jb < total; jb += kWave * kRelaxUnroll) {
+      uint64_t e[kRelaxUnroll];
+      uint32_t src[kRelaxUnroll];
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++) {
+        const uint32_t j = jb + lane + (uint32_t)q * kWave;
+        const uint32_t jj = j < total ? j : 0u;
+        // the frontier lane holding edge jj: the last lane whose first edge is <= jj
         uint32_t lo = 0, hi = kWave - 1;
         while (lo < hi) {
           const uint32_t mid = (lo + hi + 1) >> 1;
-          if (s_start[wv][mid] <= j) lo = mid;
+          if (s_start[wv][mid] <= jj) lo = mid;
           else hi = mid - 1;
         }
-        const uint64_t e = s_base[wv][lo] + (j - s_start[wv][lo]);
-        slot = s_sb[wv][lo] + a.idx[e];
-        const uint64_t t = (uint64_t)s_val[wv][lo] + a.w[e];
-        const uint32_t nd = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
-        if (nd < a.dist[slot]) {
-          const uint32_t old = atomicMin(&a.dist[slot], nd);
-          if (nd < old) app = atomicExch(&a.stamp[slot], round + 1) != round + 1;
+        src[q] = lo;
+        e[q] = s_base[wv][lo] + (jj - s_start[wv][lo]);
+      }
+      uint32_t to[kRelaxUnroll], wt[kRelaxUnroll];
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++) {
+        to[q] = a.idx[e[q]];
+        wt[q] = a.w[e[q]];
+      }
+      uint32_t slot[kRelaxUnroll], nd[kRelaxUnroll], cd[kRelaxUnroll];
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++) {
+        slot[q] = s_sb[wv][src[q]] + to[q];
+        const uint64_t t = (uint64_t)s_val[wv][src[q]] + wt[q];
+        nd[q] = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
+        cd[q] = a.dist[slot[q]];
+      }
+      // nd < cd (read this round) means the slot's distance drops this round, by this lane or
+      // by another one, so the slot belongs in the next frontier whichever atomicMin wins; the
+      // stamp exchange lets exactly one lane append it.  All atomics are issued before any
+      // result is read.
+      bool imp[kRelaxUnroll];
+      uint32_t old[kRelaxUnroll];
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++) {
+        imp[q] = jb + lane + (uint32_t)q * kWave < total && nd[q] < cd[q];
+        if (imp[q]) {
+          atomicMin(&a.dist[slot[q]], nd[q]);
+          old[q] = atomicExch(&a.stamp[slot[q]], round + 1);
         }
       }
-      const uint32_t pos = wave_append(app, &a.count[nxt]);
-      if (app) out[pos] = slot;
+      uint32_t app = 0;
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++)
+        if (imp[q] && old[q] != round + 1) app |= 1u << q;
+      // append the lane's improved slots: one atomic per wave
+      const uint32_t cnt = (uint32_t)__popc(app);
+      const uint32_t inc = wave_inclusive_scan(cnt);
+      const uint32_t tot = (uint32_t)__shfl((int)inc, kWave - 1, kWave);
+      uint32_t base = 0;
+      if (tot) {
+        if (lane == 0) base = atomicAdd(&a.count[nxt], tot);
+        base = (uint32_t)__shfl((int)base, 0, kWave);
+      }
+      uint32_t pos = base + inc - cnt;
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++)
+        if ((app >> q) & 1u) out[pos++] = slot[q];
     }
     wave_sync();
   }
@@ -173,22 +225,48 @@ __global__ void batch_sssp_seed_kernel(const uint32_t *slots, uint32_t n, uint32
 }
 
 // ---- personalised PageRank -------------------------------------------------------------
+//
+// A group holds up to 64 instances of one graph; lane l of a wave serves instance l % kk of
+// the group (kk = instances rounded up to a power of two) in slot l / kk, so a wave runs
+// 64 / kk slots.  Rows of in-degree up to kHubDegree are handed out in runs, one row per slot
+// at a time; a slot walks its row's in-edges with kLoads source loads in flight per lane and a
+// Kahan sum per lane, and writes the row's k ranks as one contiguous vector.  Rows above
+// kHubDegree are cut into pieces of kHubDegree edges spread over the slots of a wave; each
+// piece leaves an FP64 partial per instance, and ppr_hub_kernel adds a row's pieces in order.
+// Waves take tasks (runs and pieces of every group) from one queue, so groups of any size and
+// instance count share the chip without a cost model.  The dangling mass of the new ranks
+// (needed by the next iteration's teleport) is summed per instance in an exact 61-bit fixed
+// point: the sum does not depend on which wave took which task, so results are reproducible.
 
-struct PprGroup {       // up to 64 instances on one graph
-  uint32_t vbase;       // union vertex base
+constexpr uint32_t kHubDegree = 1024;  // rows above this are split into pieces of this size
+constexpr int kLoads = 8;              // source loads in flight per lane
+constexpr uint32_t kRunRows = 32;      // rows per slot in a row run
+constexpr uint32_t kRunEdges = 2048;   // edges per slot in a row run
+constexpr uint32_t kCols = 64;         // partials are stored 64 instances wide
+
+struct PprGroup {      // up to 64 instances on one graph
+  uint32_t vbase;      // union vertex base
   uint32_t V;
-  uint32_t k;           // instances in the group
-  uint32_t kk;          // k rounded up to a power of two
-  uint64_t rbase;       // base of the group's [V][k] block
-  uint32_t first;       // group-order instance id of column 0
+  uint32_t k;          // instances in the group
+  uint32_t kk;         // k rounded up to a power of two
+  uint64_t rbase;      // base of the group's [V][k] block of ranks
+  uint32_t first;      // group-order id of instance column 0
   uint32_t pad;
 };
 
-struct PprTask {        // a run of rows of one group
+struct PprTask {       // a run of rows (part == kNoPiece) or one piece of a hub row
   uint32_t group;
-  uint32_t row0;
-  uint32_t nrows;
-  uint32_t pad;
+  uint32_t row;        // first row of the run, or the hub row (group-local)
+  uint32_t n;          // rows of the run, or the piece number
+  uint32_t part;       // hub pieces: partial slot
+};
+constexpr uint32_t kNoPiece = 0xFFFFFFFFu;
+
+struct PprHub {        // a row above kHubDegree
+  uint32_t group;
+  uint32_t row;        // group-local
+  uint32_t part0;      // partial slots [part0, part0 + nparts)
+  uint32_t nparts;
 };
 
 struct PprArgs {
@@ -198,84 +276,170 @@ struct PprArgs {
   const PprGroup *groups;
   const PprTask *tasks;
   uint32_t ntasks;
+  uint32_t *next;       // task queue head (cleared before each gather)
+  const PprHub *hubs;
+  uint32_t nhubs;
   const float *contrib_in;
   float *rank_out;
   float *contrib_out;
-  const uint32_t *seed_local;  // per group-order instance: seed (local id)
-  const double *dang;          // per group-order instance: dangling mass of the current ranks
+  const uint32_t *seed_local;       // per group-order instance: seed (local id)
+  const unsigned long long *dang;   // per group-order instance: dangling mass of the current ranks
+  unsigned long long *dang_next;    // ... and of the new ranks (fixed point, cleared beforehand)
+  double *part;                     // [parts][kCols]: FP64 sums of hub pieces
   float d;
 };
 
-// Lane l serves instance l % kk over edge slots l / kk, l / kk + 64/kk, ...; the lanes of one
-// instance are combined by a butterfly over the edge-slot bits (Kahan per lane, hi/lo combine).
+// Kahan-sums contrib_in over the in-edges [e, e1) of a row, taking every step-th edge.  The
+// loads are unconditional -- an edge past e1 reloads edge e, a lane without an instance reads
+// column 0 -- so all kLoads of a batch are in flight together; the extra values are dropped.
+__device__ inline void ppr_sum_edges(const PprArgs &a, const PprGroup &g, uint32_t inst, bool on, uint64_t e,
+                                     uint64_t e1, uint32_t step, float &s, float &c) {
+  const float *col = a.contrib_in + g.rbase + (on ? inst : 0u);
+  for (; e < e1; e += (uint64_t)kLoads * step) {
+    uint32_t src[kLoads];
+#pragma unroll
+    for (int u = 0; u < kLoads; u++) {
+      const uint64_t j = e + (uint64_t)u * step;
+      src[u] = a.idx[j < e1 ? j : e];
+    }
+    float x[kLoads];
+#pragma unroll
+    for (int u = 0; u < kLoads; u++) x[u] = col[(uint64_t)(src[u] - g.vbase) * g.k];
+#pragma unroll
+    for (int u = 0; u < kLoads; u++)
+      if (e + (uint64_t)u * step < e1) kahan_add(s, c, x[u]);
+  }
+}
+
+// New rank of row r (group-local, out-degree od) from its gathered sum; returns the mass it
+// adds to the dangling sum, in fixed point.  tele = (1 - d) + d * (dangling mass), added at
+// the instance's seed.
+__device__ inline unsigned long long ppr_finish_row(const PprArgs &a, const PprGroup &g, uint32_t inst, uint32_t r,
+                                                    float sum, uint32_t seed, float tele, uint32_t od) {
+  float v = a.d * sum;
+  if (r == seed) v += tele;
+  const uint64_t pos = g.rbase + (uint64_t)r * g.k + inst;
+  a.rank_out[pos] = v;
+  a.contrib_out[pos] = od ? v / (float)od : 0.0f;
+  return od ? 0ull : fix_from_float(v);
+}
+
+// Adds a wave's fixed-point dangling sums (one per lane) of group g into dang_next: the slots
+// of an instance first, then one atomic per instance.
+__device__ inline void ppr_flush_dangling(const PprArgs &a, const PprGroup &g, unsigned long long &acc) {
+  const uint32_t lane = lane_id();
+  for (uint32_t o = kWave >> 1; o >= g.kk; o >>= 1) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)acc, (int)o, kWave);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(acc >> 32), (int)o, kWave);
+    acc += ((unsigned long long)hi << 32) | lo;
+  }
+  if (lane < g.k && acc) atomicAdd(&a.dang_next[g.first + lane], acc);
+  acc = 0ull;
+}
+
 __global__ __launch_bounds__(kBlock) void ppr_gather_kernel(PprArgs a) {
-  const uint32_t lane = lane_id(), wv = wave_id();
-  const uint32_t nw = gridDim.x * kWavesPerBlock;
-  for (uint32_t t = blockIdx.x * kWavesPerBlock + wv; t < a.ntasks; t += nw) {
+  const uint32_t lane = lane_id();
+  uint32_t gid = 0xFFFFFFFFu;
+  PprGroup g{};
+  uint32_t inst = 0, slot = 0, S = 1, seed = kInf;
+  bool on = false;
+  float tele = 0.0f;
+  unsigned long long dacc = 0ull;  // fixed-point dangling mass of this lane's instance
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.next, 1u);
+    t = (uint32_t)__shfl((int)t, 0, kWave);
+    if (t >= a.ntasks) break;
     const PprTask tk = a.tasks[t];
-    const PprGroup g = a.groups[tk.group];
-    const uint32_t inst = lane % g.kk, slot = lane / g.kk, step = kWave / g.kk;
-    for (uint32_t r = tk.row0; r < tk.row0 + tk.nrows; r++) {
-      const uint32_t u = g.vbase + r;
-      const uint64_t e0 = a.off[u], e1 = a.off[u + 1];
-      float s = 0, c = 0;
-      if (inst < g.k) {
-        for (uint64_t e = e0 + slot; e < e1; e += step) {
-          const uint32_t src = a.idx[e] - g.vbase;
-          kahan_add(s, c, a.contrib_in[g.rbase + (uint64_t)src * g.k + inst]);
-        }
+    if (tk.group != gid) {
+      if (gid != 0xFFFFFFFFu) ppr_flush_dangling(a, g, dacc);
+      gid = tk.group;
+      g = a.groups[gid];
+      inst = lane & (g.kk - 1);
+      slot = lane / g.kk;
+      S = kWave / g.kk;
+      on = inst < g.k;
+      seed = on ? a.seed_local[g.first + inst] : kInf;
+      tele = on ? (float)((1.0 - (double)a.d) + (double)a.d * fix_to_double(a.dang[g.first + inst])) : 0.0f;
+    }
+    if (tk.part == kNoPiece) {
+      // a slot's next row (clamped into the run, so the loads need no branch) is looked up
+      // while the current row's edges are summed
+      const uint32_t rend = tk.row + tk.n, rlast = g.vbase + rend - 1;
+      uint32_t q = min(g.vbase + tk.row + slot, rlast);
+      uint64_t e0 = a.off[q], e1 = a.off[q + 1];
+      uint32_t od = a.outdeg[q];
+      for (uint32_t r = tk.row + slot; r < rend; r += S) {
+        q = min(g.vbase + r + S, rlast);
+        const uint64_t n0 = a.off[q], n1 = a.off[q + 1];
+        const uint32_t nod = a.outdeg[q];
+        float s = 0.0f, c = 0.0f;
+        ppr_sum_edges(a, g, inst, on, e0, e1, 1, s, c);
+        if (on) dacc += ppr_finish_row(a, g, inst, r, s - c, seed, tele, od);
+        e0 = n0;
+        e1 = n1;
+        od = nod;
       }
+    } else {
+      const uint32_t u = g.vbase + tk.row;
+      const uint64_t e0 = a.off[u] + (uint64_t)tk.n * kHubDegree;
+      const uint64_t e1 = min(a.off[u + 1], e0 + kHubDegree);
+      float s = 0.0f, c = 0.0f;
+      ppr_sum_edges(a, g, inst, on, e0 + slot, e1, S, s, c);
+      // the slots' sums of one instance, combined in a fixed butterfly order
       float hi = s, lo = -c;
-      for (uint32_t off = kWave >> 1; off >= g.kk; off >>= 1) {
-        const float oh = __shfl_xor(hi, (int)off, kWave), ol = __shfl_xor(lo, (int)off, kWave);
+      for (uint32_t o = kWave >> 1; o >= g.kk; o >>= 1) {
+        const float oh = __shfl_xor(hi, (int)o, kWave), ol = __shfl_xor(lo, (int)o, kWave);
         dd_add(hi, lo, oh, ol);
       }
-      if (slot == 0 && inst < g.k) {
-        const uint32_t gi = g.first + inst;
-        float v = a.d * hi;
-        if (r == a.seed_local[gi]) v += (float)((1.0 - (double)a.d) + (double)a.d * a.dang[gi]);
-        const uint64_t pos = g.rbase + (uint64_t)r * g.k + inst;
-        a.rank_out[pos] = v;
-        const uint32_t od = a.outdeg[u];
-        a.contrib_out[pos] = od ? v / (float)od : 0.0f;
-      }
+      if (slot == 0 && on) a.part[(size_t)tk.part * kCols + inst] = (double)hi + (double)lo;
+    }
+  }
+  if (gid != 0xFFFFFFFFu) ppr_flush_dangling(a, g, dacc);
+}
+
+// One wave per hub row: its pieces' partials in piece order, then the rank update.
+__global__ __launch_bounds__(kBlock) void ppr_hub_kernel(PprArgs a) {
+  const uint32_t lane = lane_id();
+  const uint32_t nw = gridDim.x * kWavesPerBlock;
+  for (uint32_t h = blockIdx.x * kWavesPerBlock + wave_id(); h < a.nhubs; h += nw) {
+    const PprHub hb = a.hubs[h];
+    const PprGroup g = a.groups[hb.group];
+    if (lane < g.k) {
+      double s = 0.0;
+      for (uint32_t p = 0; p < hb.nparts; p++) s += a.part[(size_t)(hb.part0 + p) * kCols + lane];
+      const uint32_t gi = g.first + lane;
+      const float tele = (float)((1.0 - (double)a.d) + (double)a.d * fix_to_double(a.dang[gi]));
+      const unsigned long long dm =
+          ppr_finish_row(a, g, lane, hb.row, (float)s, a.seed_local[gi], tele, a.outdeg[g.vbase + hb.row]);
+      if (dm) atomicAdd(&a.dang_next[gi], dm);
     }
   }
 }
 
-// One block per instance: FP64 mass on the dangling vertices of its graph (fixed order).
-__global__ __launch_bounds__(kBlock) void ppr_dangling_kernel(const PprGroup *groups, const uint32_t *group_of,
-                                                              const uint32_t *dlist, const uint64_t *dbase,
-                                                              const float *rank, double *dang) {
-  __shared__ double sh[kBlock];
-  const uint32_t i = blockIdx.x;
-  const uint32_t gid = group_of[i];
-  const PprGroup g = groups[gid];
-  const uint32_t col = i - g.first;
-  double acc = 0;
-  for (uint64_t j = dbase[gid] + threadIdx.x; j < dbase[gid + 1]; j += kBlock)
-    acc += (double)rank[g.rbase + (uint64_t)dlist[j] * g.k + col];
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) dang[i] = sh[0];
+// Clears the task queue and the dangling sums the next gather accumulates into.
+__global__ void ppr_reset_kernel(uint32_t *next, unsigned long long *dang_next, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dang_next[i] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *next = 0u;
 }
 
-// One block per instance: r = e_seed, c = r / outdeg.
-__global__ void ppr_init_kernel(const PprGroup *groups, const uint32_t *group_of, const uint32_t *seed_local,
-                                const uint32_t *outdeg, float *rank, float *contrib) {
-  const uint32_t i = blockIdx.x;
-  const PprGroup g = groups[group_of[i]];
-  const uint32_t col = i - g.first;
-  for (uint32_t v = threadIdx.x; v < g.V; v += blockDim.x) {
-    const float r = v == seed_local[i] ? 1.0f : 0.0f;
-    const uint64_t pos = g.rbase + (uint64_t)v * g.k + col;
-    rank[pos] = r;
+// r = e_seed, c = r / outdeg over the group's [V][k] block (blockIdx.y = group), and the
+// dangling mass of r per instance.
+__global__ void ppr_init_kernel(const PprGroup *groups, const uint32_t *seed_local, const uint32_t *outdeg,
+                                float *rank, float *contrib, unsigned long long *dang) {
+  const PprGroup g = groups[blockIdx.y];
+  const uint64_t total = (uint64_t)g.V * g.k;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = (uint32_t)(i / g.k), col = (uint32_t)(i % g.k);
+    const float r = v == seed_local[g.first + col] ? 1.0f : 0.0f;
+    rank[g.rbase + i] = r;
     const uint32_t od = outdeg[g.vbase + v];
-    contrib[pos] = od ? r / (float)od : 0.0f;
+    contrib[g.rbase + i] = od ? r / (float)od : 0.0f;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < g.k) {
+    const uint32_t gi = g.first + threadIdx.x;
+    dang[gi] = outdeg[g.vbase + seed_local[gi]] ? 0ull : fix_from_float(1.0f);
   }
 }
 
@@ -450,19 +614,18 @@ void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *see
   }
   std::vector<PprGroup> groups;
   std::vector<uint32_t> group_of, seed_local, order;  // per group-order instance
-  std::vector<PprTask> tasks;
-  std::vector<uint32_t> dlist;
-  std::vector<uint64_t> dbase(1, 0);
-  std::vector<uint32_t> od((size_t)b.V);
-  GMP_HIP(hipMemcpy(od.data(), b.outdeg.p, (size_t)b.V * 4, hipMemcpyDeviceToHost));
+  std::vector<PprTask> pieces, runs;
+  std::vector<PprHub> hubs;
   uint64_t rtot = 0;
+  uint32_t nparts = 0;
   for (uint32_t k = 0; k < G; k++) {
     const uint32_t Vk = b.vbase[k + 1] - b.vbase[k];
-    for (size_t s0 = 0; s0 < per[k].size(); s0 += 64) {
+    const uint64_t *off = b.h_off.data() + b.vbase[k];
+    for (size_t s0 = 0; s0 < per[k].size(); s0 += kCols) {
       PprGroup g{};
       g.vbase = b.vbase[k];
       g.V = Vk;
-      g.k = (uint32_t)std::min<size_t>(64, per[k].size() - s0);
+      g.k = (uint32_t)std::min<size_t>(kCols, per[k].size() - s0);
       g.kk = 1;
       while (g.kk < g.k) g.kk <<= 1;
       g.rbase = rtot;
@@ -475,35 +638,44 @@ void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *see
         seed_local.push_back(seed[i]);
       }
       rtot += (uint64_t)Vk * g.k;
-      groups.push_back(g);
-      for (uint32_t v = 0; v < Vk; v++)
-        if (od[g.vbase + v] == 0) dlist.push_back(v);
-      dbase.push_back(dlist.size());
-      // row runs of at most 2048 edges or 256 rows
-      uint32_t r0 = 0;
-      while (r0 < Vk) {
+      // tasks: the pieces of the hub rows, and runs of the other rows sized for the slots
+      const uint32_t S = kWave / g.kk;
+      for (uint32_t r = 0; r < Vk; r++) {
+        const uint64_t d = off[r + 1] - off[r];
+        if (d <= kHubDegree) continue;
+        const uint32_t np = (uint32_t)((d + kHubDegree - 1) / kHubDegree);
+        hubs.push_back(PprHub{gid, r, nparts, np});
+        for (uint32_t q = 0; q < np; q++) pieces.push_back(PprTask{gid, r, q, nparts + q});
+        nparts += np;
+      }
+      for (uint32_t r0 = 0; r0 < Vk;) {
         uint32_t r1 = r0;
         uint64_t edges = 0;
-        while (r1 < Vk && r1 - r0 < 256) {
-          const uint64_t d = b.h_off[g.vbase + r1 + 1] - b.h_off[g.vbase + r1];
-          if (r1 > r0 && edges + d > 2048) break;
+        while (r1 < Vk && r1 - r0 < kRunRows * S) {
+          const uint64_t d = off[r1 + 1] - off[r1];
+          if (d > kHubDegree || (r1 > r0 && edges + d > (uint64_t)kRunEdges * S)) break;
           edges += d;
           r1++;
         }
-        tasks.push_back(PprTask{gid, r0, r1 - r0, 0});
-        r0 = r1;
+        if (r1 > r0) runs.push_back(PprTask{gid, r0, r1 - r0, kNoPiece});
+        r0 = r1 + (r1 < Vk && off[r1 + 1] - off[r1] > kHubDegree ? 1 : 0);
       }
+      groups.push_back(g);
     }
   }
+  // the queue: hub pieces first (the longest tasks start early), then the runs
+  std::vector<PprTask> tasks(pieces);
+  tasks.insert(tasks.end(), runs.begin(), runs.end());
   const uint32_t ng = (uint32_t)groups.size();
-  DevBuf d_groups, d_gof, d_seed, d_tasks, d_dlist, d_dbase, d_dang, rank0, rank1, con0, con1;
+  DevBuf d_groups, d_gof, d_seed, d_tasks, d_hubs, d_next, d_dang, d_part, rank0, rank1, con0, con1;
   d_groups.alloc(ng * sizeof(PprGroup));
   d_gof.alloc(n * 4);
   d_seed.alloc(n * 4);
   d_tasks.alloc(std::max<size_t>(tasks.size(), 1) * sizeof(PprTask));
-  d_dlist.alloc(std::max<size_t>(dlist.size(), 1) * 4);
-  d_dbase.alloc(dbase.size() * 8);
-  d_dang.alloc(n * 8);
+  d_hubs.alloc(std::max<size_t>(hubs.size(), 1) * sizeof(PprHub));
+  d_next.alloc(4);
+  d_dang.alloc((size_t)2 * n * 8);  // two fixed-point dangling sums per instance, ping-pong
+  d_part.alloc(std::max<size_t>(nparts, 1) * kCols * 8);
   rank0.alloc(std::max<uint64_t>(rtot, 1) * 4);
   rank1.alloc(std::max<uint64_t>(rtot, 1) * 4);
   con0.alloc(std::max<uint64_t>(rtot, 1) * 4);
@@ -511,12 +683,13 @@ void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *see
   GMP_HIP(hipMemcpyAsync(d_groups.p, groups.data(), ng * sizeof(PprGroup), hipMemcpyHostToDevice, e.stream));
   GMP_HIP(hipMemcpyAsync(d_gof.p, group_of.data(), n * 4, hipMemcpyHostToDevice, e.stream));
   GMP_HIP(hipMemcpyAsync(d_seed.p, seed_local.data(), n * 4, hipMemcpyHostToDevice, e.stream));
-  GMP_HIP(hipMemcpyAsync(d_tasks.p, tasks.data(), tasks.size() * sizeof(PprTask), hipMemcpyHostToDevice, e.stream));
-  if (!dlist.empty())
-    GMP_HIP(hipMemcpyAsync(d_dlist.p, dlist.data(), dlist.size() * 4, hipMemcpyHostToDevice, e.stream));
-  GMP_HIP(hipMemcpyAsync(d_dbase.p, dbase.data(), dbase.size() * 8, hipMemcpyHostToDevice, e.stream));
+  if (!tasks.empty())
+    GMP_HIP(hipMemcpyAsync(d_tasks.p, tasks.data(), tasks.size() * sizeof(PprTask), hipMemcpyHostToDevice, e.stream));
+  if (!hubs.empty())
+    GMP_HIP(hipMemcpyAsync(d_hubs.p, hubs.data(), hubs.size() * sizeof(PprHub), hipMemcpyHostToDevice, e.stream));
   float *rank[2] = {rank0.as<float>(), rank1.as<float>()};
   float *con[2] = {con0.as<float>(), con1.as<float>()};
+  unsigned long long *dang[2] = {d_dang.as<unsigned long long>(), d_dang.as<unsigned long long>() + n};
   PprArgs a{};
   a.off = b.in.off.as<uint64_t>();
   a.idx = b.in.idx.as<uint32_t>();
@@ -524,27 +697,37 @@ void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *see
   a.groups = d_groups.as<PprGroup>();
   a.tasks = d_tasks.as<PprTask>();
   a.ntasks = (uint32_t)tasks.size();
+  a.next = d_next.as<uint32_t>();
+  a.hubs = d_hubs.as<PprHub>();
+  a.nhubs = (uint32_t)hubs.size();
   a.seed_local = d_seed.as<uint32_t>();
-  a.dang = d_dang.as<double>();
+  a.part = d_part.as<double>();
   a.d = damping;
-  const uint32_t grid = (uint32_t)e.num_cus * 8;
+  const uint32_t gather_grid = (uint32_t)std::max(1, e.num_cus) * 8;
+  const uint32_t hub_grid = std::max<uint32_t>(1, std::min<uint32_t>((a.nhubs + kWavesPerBlock - 1) / kWavesPerBlock,
+                                                                      (uint32_t)e.num_cus * 8));
+  uint32_t init_x = 1;
+  for (const PprGroup &g : groups)
+    init_x = std::max<uint32_t>(init_x, (uint32_t)std::min<uint64_t>(((uint64_t)g.V * g.k + kBlock - 1) / kBlock, 1024));
 
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
-  ppr_init_kernel<<<n, kBlock, 0, e.stream>>>(a.groups, d_gof.as<uint32_t>(), a.seed_local, a.outdeg, rank[0],
-                                               con[0]);
+  ppr_init_kernel<<<dim3(init_x, ng), kBlock, 0, e.stream>>>(a.groups, a.seed_local, a.outdeg, rank[0], con[0],
+                                                              dang[0]);
   GMP_CHECK_LAUNCH();
   GraphExecGuard ge;
   GMP_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
   for (int it = 0; it < iters; it++) {
     const int cur = it & 1;
-    ppr_dangling_kernel<<<n, kBlock, 0, e.stream>>>(a.groups, d_gof.as<uint32_t>(), d_dlist.as<uint32_t>(),
-                                                     d_dbase.as<uint64_t>(), rank[cur], d_dang.as<double>());
     PprArgs ai = a;
     ai.contrib_in = con[cur];
     ai.rank_out = rank[cur ^ 1];
     ai.contrib_out = con[cur ^ 1];
-    ppr_gather_kernel<<<grid, kBlock, 0, e.stream>>>(ai);
+    ai.dang = dang[cur];
+    ai.dang_next = dang[cur ^ 1];
+    ppr_reset_kernel<<<1, kBlock, 0, e.stream>>>(ai.next, ai.dang_next, n);
+    ppr_gather_kernel<<<gather_grid, kBlock, 0, e.stream>>>(ai);
+    if (a.nhubs) ppr_hub_kernel<<<hub_grid, kBlock, 0, e.stream>>>(ai);
   }
   GMP_HIP(hipStreamEndCapture(e.stream, &ge.graph));
   if (iters > 0) {
