@@ -87,39 +87,89 @@ struct RelaxArgs {
   uint32_t *hub_list;       // flagged slices of this launch (hub_compact_kernel)
 };
 
-// One edge relaxation: du is the source's value; sets v and whether v joins the near
-// frontier (first time this launch) or the far pile.
+constexpr int kRelaxUnroll = 4;  // edges per lane in flight in the relax loops
+
+// Relaxes up to kRelaxUnroll edges per lane (edge[q], source value du[q]; entries with bit q
+// of `live` clear are skipped) and appends the improved targets to the near frontier (first
+// time this launch) or the far pile.  Each phase is issued for all edges before its results
+// are read: the edge and weight loads, the distance loads, the atomicMin of the improving
+// edges, then the stamp or flag exchange of those whose atomicMin lowered the distance.
 template <class OffT, int MODE>
-__device__ inline void relax_edge(const RelaxArgs<OffT> &a, uint64_t edge, uint32_t du, uint32_t &v, bool &to_near,
-                                  bool &to_far) {
-  v = a.idx[edge];
-  uint32_t nd;
-  if (MODE == kModeMinLabel) {
-    nd = du;
-  } else {
-    const uint64_t t = (uint64_t)du + (MODE == kModeUnit ? 1u : a.w[edge]);
-    nd = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
+__device__ inline void relax_batch(const RelaxArgs<OffT> &a, const uint64_t (&edge)[kRelaxUnroll],
+                                   const uint32_t (&du)[kRelaxUnroll], uint32_t live, unsigned long long &smask) {
+  uint32_t v[kRelaxUnroll], wt[kRelaxUnroll];
+#pragma unroll
+  for (int q = 0; q < kRelaxUnroll; q++) {
+    v[q] = a.idx[edge[q]];
+    wt[q] = MODE == kModeWeighted ? a.w[edge[q]] : 1u;
   }
-  if (nd < a.dist[v]) {
-    const uint32_t old = atomicMin(&a.dist[v], nd);
-    if (nd < old) {
-      if (nd < a.threshold) to_near = atomicExch(&a.near_stamp[v], a.stamp) != a.stamp;
-      else to_far = atomicExch(&a.far_flag[v], 1u) == 0u;
+  uint32_t nd[kRelaxUnroll], cd[kRelaxUnroll];
+#pragma unroll
+  for (int q = 0; q < kRelaxUnroll; q++) {
+    if (MODE == kModeMinLabel) {
+      nd[q] = du[q];
+    } else {
+      const uint64_t t = (uint64_t)du[q] + wt[q];
+      nd[q] = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
+    }
+    cd[q] = a.dist[v[q]];
+  }
+  bool tried[kRelaxUnroll];
+  uint32_t old[kRelaxUnroll];
+#pragma unroll
+  for (int q = 0; q < kRelaxUnroll; q++) {
+    tried[q] = ((live >> q) & 1u) && nd[q] < cd[q];
+    old[q] = 0u;
+    if (tried[q]) old[q] = atomicMin(&a.dist[v[q]], nd[q]);
+  }
+  // the results are first read here, after every atomicMin has been issued (an empty asm
+  // taking them keeps the compares below from being moved up next to each atomic)
+#pragma unroll
+  for (int q = 0; q < kRelaxUnroll; q++) asm volatile("" : "+v"(old[q]));
+  bool tn[kRelaxUnroll], tf[kRelaxUnroll];
+  uint32_t ex[kRelaxUnroll];
+#pragma unroll
+  for (int q = 0; q < kRelaxUnroll; q++) {
+    const bool imp = tried[q] && nd[q] < old[q];
+    tn[q] = imp && nd[q] < a.threshold;
+    tf[q] = imp && !(nd[q] < a.threshold);
+    if (tn[q]) ex[q] = atomicExch(&a.near_stamp[v[q]], a.stamp);
+    else if (tf[q]) ex[q] = atomicExch(&a.far_flag[v[q]], 1u);
+  }
+  uint32_t nmask = 0, fmask = 0;
+#pragma unroll
+  for (int q = 0; q < kRelaxUnroll; q++) {
+    if (tn[q] && ex[q] != a.stamp) nmask |= 1u << q;
+    if (tf[q] && ex[q] == 0u) fmask |= 1u << q;
+  }
+  // wave-aggregated appends: one atomic per wave and list
+  const uint32_t lane = lane_id();
+  const uint32_t nn = (uint32_t)__popc(nmask), nf = (uint32_t)__popc(fmask);
+  const uint32_t in = wave_inclusive_scan(nn);
+  const uint32_t tnn = (uint32_t)__shfl((int)in, kWave - 1, kWave);
+  if (tnn) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&a.cnt->near_cnt, tnn);
+    uint32_t pos = (uint32_t)__shfl((int)base, 0, kWave) + in - nn;
+#pragma unroll
+    for (int q = 0; q < kRelaxUnroll; q++)
+      if ((nmask >> q) & 1u) {
+        a.near_out[pos++] = v[q];
+        if (a.track_shards) smask |= 1ull << (uint32_t)(((uint64_t)v[q] * kMlpShards) / a.V);
+      }
+  }
+  if (a.far_out) {
+    const uint32_t inf = wave_inclusive_scan(nf);
+    const uint32_t tff = (uint32_t)__shfl((int)inf, kWave - 1, kWave);
+    if (tff) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&a.cnt->far_cnt, tff);
+      uint32_t pos = (uint32_t)__shfl((int)base, 0, kWave) + inf - nf;
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++)
+        if ((fmask >> q) & 1u) a.far_out[pos++] = v[q];
     }
   }
-}
-
-// Wave-aggregated appends of the relaxed vertices to the near frontier and the far pile.
-template <class OffT>
-__device__ inline void relax_append(const RelaxArgs<OffT> &a, uint32_t v, bool to_near, bool to_far,
-                                    unsigned long long &smask) {
-  const uint32_t sn = wave_append(to_near, &a.cnt->near_cnt);
-  if (to_near) a.near_out[sn] = v;
-  if (a.far_out) {
-    const uint32_t sf = wave_append(to_far, &a.cnt->far_cnt);
-    if (to_far) a.far_out[sf] = v;
-  }
-  if (a.track_shards && to_near) smask |= 1ull << (uint32_t)(((uint64_t)v * kMlpShards) / a.V);
 }
 
 template <class OffT>
@@ -171,21 +221,25 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
     s_val[wv][lane] = du;
     wave_sync();
     edges += total;
-    for (uint32_t jb = 0; jb < total; jb += kWave) {
-      const uint32_t j = jb + lane;
-      bool to_near = false, to_far = false;
-      uint32_t v = 0;
-      if (j < total) {
-        // owner = last lane whose start <= j (zero-degree lanes never win)
+    for (uint32_t jb = 0; jb < total; jb += kWave * kRelaxUnroll) {
+      uint64_t e[kRelaxUnroll];
+      uint32_t du[kRelaxUnroll], live = 0;
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++) {
+        const uint32_t j = jb + lane + (uint32_t)q * kWave;
+        if (j < total) live |= 1u << q;
+        const uint32_t jj = j < total ? j : 0u;  // past the end: the wave's first edge, skipped
+        // owner = last lane whose start <= jj (zero-degree lanes never win)
         uint32_t lo = 0, hi = kWave - 1;
         while (lo < hi) {
           const uint32_t mid = (lo + hi + 1) >> 1;
-          if (s_start[wv][mid] <= j) lo = mid;
+          if (s_start[wv][mid] <= jj) lo = mid;
           else hi = mid - 1;
         }
-        relax_edge<OffT, MODE>(a, s_base[wv][lo] + (j - s_start[wv][lo]), s_val[wv][lo], v, to_near, to_far);
+        e[q] = s_base[wv][lo] + (jj - s_start[wv][lo]);
+        du[q] = s_val[wv][lo];
       }
-      relax_append(a, v, to_near, to_far, smask);
+      relax_batch<OffT, MODE>(a, e, du, live, smask);
     }
     wave_sync();
   }
@@ -217,12 +271,17 @@ __global__ __launch_bounds__(kBlock) void hub_relax_kernel(RelaxArgs<OffT> a) {
     const uint64_t e0 = rb + (uint64_t)sl.y * kHubSlice, e1 = min(e0 + kHubSlice, re);
     const uint32_t du = a.dist[u];
     edges += e1 - e0;
-    for (uint64_t jb = e0; jb < e1; jb += kWave) {
-      const uint64_t j = jb + lane;
-      bool to_near = false, to_far = false;
-      uint32_t v = 0;
-      if (j < e1) relax_edge<OffT, MODE>(a, j, du, v, to_near, to_far);
-      relax_append(a, v, to_near, to_far, smask);
+    for (uint64_t jb = e0; jb < e1; jb += (uint64_t)kWave * kRelaxUnroll) {
+      uint64_t e[kRelaxUnroll];
+      uint32_t dv[kRelaxUnroll], live = 0;
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++) {
+        const uint64_t j = jb + lane + (uint64_t)q * kWave;
+        if (j < e1) live |= 1u << q;
+        e[q] = j < e1 ? j : e0;
+        dv[q] = du;
+      }
+      relax_batch<OffT, MODE>(a, e, dv, live, smask);
     }
   }
   relax_flush_counts(a, edges, smask);
