@@ -243,9 +243,14 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &
   // room for the round plus a free slot (needed only when some edge is not an end)
   if (B.fill + total >= kRound) flush_messages(a, B);
   const uint32_t o0 = B.fill + incl - cnt;  // this lane's first slot
-  // non-ends write a free slot past the round's messages, one per lane where the buffer has
-  // room (lanes writing one LDS address would serialise)
+  // Every edge writes slot slot0 + (ends before it).  In a lane with ends, an end writes its
+  // message and a non-end writes the slot of the lane's next message (which overwrites it
+  // later in program order) or, after the lane's last end, the first slot of the next lane
+  // with an end -- its head slot, rewritten below after the scan -- or the free slot past the
+  // round's messages.  A lane without ends writes a free slot of its own, one per lane where the
+  // buffer has room (lanes writing one LDS address would serialise).
   const uint32_t spare = B.fill + total + min(lane, kRound - 1 - min(B.fill + total, kRound - 1));
+  const uint32_t slot0 = ends ? o0 : spare;
   const uint32_t kf = ends ? (uint32_t)__ffs(ends) - 1u : 8u;
   double run = 0.0, first = 0.0;
 #pragma unroll
@@ -253,7 +258,7 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &
     run += (double)x[k];
     const bool e = (ends >> k) & 1u;
     if ((uint32_t)k == kf) first = run;
-    B.mb[e ? o0 + (uint32_t)__popc(ends & ((1u << k) - 1u)) : spare] = (float)run;
+    B.mb[slot0 + (uint32_t)__popc(ends & ((1u << k) - 1u))] = (float)run;
     if (e) run = 0.0;
   }
   if (FULL && !live) run = 0.0;
