@@ -252,15 +252,22 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &
   const uint32_t spare = B.fill + total + min(lane, kRound - 1 - min(B.fill + total, kRound - 1));
   const uint32_t slot0 = ends ? o0 : spare;
   const uint32_t kf = ends ? (uint32_t)__ffs(ends) - 1u : 8u;
+  // the sum restarts after an end: run = fma(run, keep, x) with keep = 1.0, or 0.0 after an
+  // end -- exactly run + x, or exactly x (run is finite) -- one FMA in place of two selects
+  const uint32_t nends = ~ends;
   double run = 0.0, first = 0.0;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    run += (double)x[k];
-    const bool e = (ends >> k) & 1u;
+    if (k == 0) {
+      run = (double)x[0];
+    } else {
+      const uint32_t keep_hi = (uint32_t)(-(int32_t)((nends >> (k - 1)) & 1u)) & 0x3FF00000u;
+      run = fma(run, __hiloint2double((int)keep_hi, 0), (double)x[k]);
+    }
     if ((uint32_t)k == kf) first = run;
     B.mb[slot0 + (uint32_t)__popc(ends & ((1u << k) - 1u))] = (float)run;
-    if (e) run = 0.0;
   }
+  run *= __hiloint2double((int)((uint32_t)(-(int32_t)((nends >> 7) & 1u)) & 0x3FF00000u), 0);
   if (FULL && !live) run = 0.0;
   // the open sum through each lane; the previous round's carry joins where no segment
   // started since the round began
