@@ -301,8 +301,11 @@ def main():
             reached = None
             if d is not None:
                 reached = int((d != 0xFFFFFFFF).sum())
+            # yardstick: 12 B per relaxed edge (target id, weight, target distance)
             result["extra"][name] = {"ms": round(st.ms_total, 3), "wall_ms": round(t * 1e3, 3),
                                      "edges_relaxed": st.edges, "work_gteps": round(st.edges / st.ms_total / 1e6, 3),
+                                     "yardstick_gbs": round(st.bytes / st.ms_total / 1e6, 1),
+                                     "hbm_frac": round(st.bytes / st.ms_total / 1e6 / HBM_PEAK_GBS, 4),
                                      "iterations": st.iterations, "source": src, "reached": reached}
             log(f"{name}: {st.ms_total:.1f} ms, {st.iterations} iterations, {st.edges} relaxations")
             if name == "sssp_delta64":
@@ -328,8 +331,12 @@ def main():
         eng.cc(want_labels=False)
         lab, st = eng.cc()
         ncomp = int((lab == np.arange(Vs, dtype=np.uint32)).sum())
+        # yardstick: 8 B per relaxed edge (target id, target label)
         result["extra"]["cc"] = {"ms": round(st.ms_total, 3), "edges_symmetric": Es, "iterations": st.iterations,
-                                 "gteps": round(Es / st.ms_total / 1e6, 3), "components": ncomp}
+                                 "gteps": round(Es / st.ms_total / 1e6, 3), "edges_relaxed": st.edges,
+                                 "yardstick_gbs": round(st.bytes / st.ms_total / 1e6, 1),
+                                 "hbm_frac": round(st.bytes / st.ms_total / 1e6 / HBM_PEAK_GBS, 4),
+                                 "components": ncomp}
         log(f"cc: {st.ms_total:.1f} ms, {st.iterations} iterations, {ncomp} components")
         if not args.no_cpu:
             import oracle

@@ -289,6 +289,11 @@ void ensure_packed(Engine &e);
 // Builds e.g.strips for e.opt.strip_log / bin_log; returns false when the layout does not apply
 // (strips off, or more than 2^32 - 1 edges).
 bool ensure_strips(Engine &e);
+// Connected components by synchronous min-label sweeps over the strip layout: label (V entries
+// in work order, device) receives, per vertex, the least original id that reaches it along
+// edges -- GraphMP's CC rule iterated to its fixed point.  Returns false, having launched
+// nothing, when the layout does not apply; ms is the device time of the sweeps.
+bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms);
 // Row id of every edge of a CSR (device array of E entries).
 void edge_row_ids(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, DevBuf &rowid);
 

@@ -28,6 +28,7 @@
 // scan into segment ids, and binary searches for the first segment of every (bin, strip).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include <rocprim/rocprim.hpp>
 
@@ -91,6 +92,9 @@ struct StripArgs {
   uint32_t nsplit;
   uint32_t V, wlog, rlog;
   double *pr1, *pd1, *pr2, *pd2;
+  // connected components
+  uint32_t *nlab;     // complemented labels, updated in place
+  uint32_t *changed;  // vertices whose label dropped in the sweep
 };
 
 // ---------------------------------------------------------------- build kernels
@@ -281,15 +285,69 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &
   B.fill += total;
 }
 
+// The same round for connected components: labels are complemented (nl = ~label), so a
+// segment's message is the maximum nl of its sources -- the least label -- and 0, which DPP
+// moves in past a row's start, is the neutral element.
+template <bool FULL>
+__device__ inline void cc_round(const StripArgs &a, const uint32_t *sv, MsgBuf &B, u32x4 cv, uint32_t q, uint32_t e0,
+                                uint32_t e1, uint64_t upto, uint32_t &carry) {
+  const uint32_t lane = lane_id();
+  const bool live = q < e1 && q + 8 > e0;
+  uint32_t x[8];
+  uint32_t ends = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t c = (cv[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+    const uint32_t v = sv[c & 0x7FFFu];
+    if (FULL) {
+      x[k] = v;
+      ends |= (c >> 15) << k;
+    } else {
+      const bool ok = q + k >= e0 && q + k < e1;
+      x[k] = ok ? v : 0u;
+      if (ok && (c & 0x8000u)) ends |= 1u << k;
+    }
+  }
+  const uint32_t cnt = (uint32_t)__popc(ends);
+  const uint32_t incl = wave_scan_dpp(cnt);
+  const uint32_t total = read_lane_u32(incl, kWave - 1);
+  if (B.fill + total >= kRound) flush_messages(a, B);
+  const uint32_t o0 = B.fill + incl - cnt;
+  // slots as in strip_round
+  const uint32_t spare = B.fill + total + min(lane, kRound - 1 - min(B.fill + total, kRound - 1));
+  const uint32_t slot0 = ends ? o0 : spare;
+  const uint32_t kf = ends ? (uint32_t)__ffs(ends) - 1u : 8u;
+  uint32_t *mb = reinterpret_cast<uint32_t *>(B.mb);
+  const uint32_t nends = ~ends;
+  uint32_t run = 0, first = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    run = k == 0 ? x[0] : max(run & (0u - ((nends >> (k - 1)) & 1u)), x[k]);  // restart after an end
+    if ((uint32_t)k == kf) first = run;
+    mb[slot0 + (uint32_t)__popc(ends & ((1u << k) - 1u))] = run;
+  }
+  run &= 0u - ((nends >> 7) & 1u);
+  if (!live) run = 0u;
+  const bool head = ends != 0;
+  uint32_t open = wave_segmented_max_dpp_short(run, head);
+  if (!(__ballot(head) & upto)) open = max(open, carry);
+  uint32_t cin = dpp_u32<kDppWaveShr1>(open);
+  if (lane == 0) cin = carry;
+  if (head) mb[o0] = max(cin, first);
+  carry = read_lane_u32(open, kWave - 1);
+  B.fill += total;
+}
+
 // Messages of edges [e0, e1) whose first segment has index seg; every segment ends inside.
 // The code loads run kAhead rounds ahead of their use.
+template <bool CC>
 __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, float *mb, uint32_t e0, uint32_t e1,
                                        uint32_t seg) {
   const uint32_t lane = lane_id();
   const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
   const uint32_t b0 = e0 & ~7u;
   const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
-  double carry = 0.0;  // sum of the segment left open by the previous round
+  typename std::conditional<CC, uint32_t, double>::type carry = 0;  // the open segment's value
   MsgBuf B{mb, seg, 0u};
   u32x4 buf[kAhead];
 #pragma unroll
@@ -304,17 +362,26 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
         const uint32_t q = base + 8 * lane;
         // lanes are cut only where e0 or e1 is not a multiple of 8 inside this round
         const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
-        if (cut) strip_round<false>(a, sv, B, cv, q, e0, e1, upto, carry);
-        else strip_round<true>(a, sv, B, cv, q, e0, e1, upto, carry);
+        if constexpr (CC) {
+          const uint32_t *su = reinterpret_cast<const uint32_t *>(sv);
+          if (cut) cc_round<false>(a, su, B, cv, q, e0, e1, upto, carry);
+          else cc_round<true>(a, su, B, cv, q, e0, e1, upto, carry);
+        } else {
+          if (cut) strip_round<false>(a, sv, B, cv, q, e0, e1, upto, carry);
+          else strip_round<true>(a, sv, B, cv, q, e0, e1, upto, carry);
+        }
       }
     }
   }
   flush_messages(a, B);
 }
 
-__global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a) {
-  extern __shared__ float sv[];  // contributions of the task's strip, then a message buffer per wave
+// One strip's values (contributions, or complemented labels for CC) staged in LDS, then the
+// strip's wave tasks.
+template <bool CC>
+__device__ inline void strip_gather(const StripArgs &a, float *sv) {
   const uint32_t W = 1u << a.wlog;
+  const uint32_t *vals = CC ? a.nlab : reinterpret_cast<const uint32_t *>(a.contrib_in);
   const uint32_t wv = wave_id();
   float *mb = sv + W + wv * kRound;
   for (uint32_t t = blockIdx.x; t < a.ngt; t += gridDim.x) {
@@ -324,7 +391,7 @@ __global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a
     __syncthreads();  // the previous task's readers are done
     if (n == W && (v0 & 3u) == 0) {
       // whole strip: 16-byte loads, four in flight per thread before their stores
-      const u32x4 *src = reinterpret_cast<const u32x4 *>(a.contrib_in + v0);
+      const u32x4 *src = reinterpret_cast<const u32x4 *>(vals + v0);
       u32x4 *dst = reinterpret_cast<u32x4 *>(sv);
       const uint32_t n4 = W / 4;
       uint32_t i = threadIdx.x;
@@ -337,100 +404,143 @@ __global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a
       }
       for (; i < n4; i += kIterBlock) dst[i] = src[i];
     } else {
-      for (uint32_t i = threadIdx.x; i < n; i += kIterBlock) sv[i] = a.contrib_in[v0 + i];
+      for (uint32_t i = threadIdx.x; i < n; i += kIterBlock) reinterpret_cast<uint32_t *>(sv)[i] = vals[v0 + i];
     }
     __syncthreads();
     for (uint32_t w = gt.y + wv; w < gt.z; w += kIterWaves) {
       const uint4 wt = a.wtask[w];
-      strip_wave_task(a, sv, mb, wt.x, wt.y, wt.z);
+      strip_wave_task<CC>(a, sv, mb, wt.x, wt.y, wt.z);
     }
   }
 }
 
+__global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a) {
+  extern __shared__ float sv[];  // contributions of the task's strip, then a message buffer per wave
+  strip_gather<false>(a, sv);
+}
+
+__global__ __launch_bounds__(kIterBlock) void cc_strip_gather_kernel(StripArgs a) {
+  extern __shared__ float sv[];  // complemented labels of the task's strip, then the message buffers
+  strip_gather<true>(a, sv);
+}
+
+// LDS of a bin task: 2^rlog accumulators of type ACC, then the bin's run table -- the first
+// message of each run, the exclusive prefix of run lengths (ns + 1) and the chunk index (the
+// run holding message bt.y + c * kChunk).
+template <class ACC>
+struct BinLds {
+  ACC *acc;
+  uint32_t *rbase, *pre, *cidx;
+};
+
+template <class ACC>
+__device__ inline BinLds<ACC> bin_lds(void *lds, uint32_t R, uint32_t ns) {
+  BinLds<ACC> L;
+  L.acc = reinterpret_cast<ACC *>(lds);
+  L.rbase = reinterpret_cast<uint32_t *>(L.acc + R);
+  L.pre = L.rbase + ns;
+  L.cidx = L.pre + ns + 1;
+  return L;
+}
+
+// Sets every accumulator to init and builds the run table of bin task bt (all threads of
+// the block; thread tid owns strips [s0, s1)).
+template <class ACC>
+__device__ inline void bin_prepare(const StripArgs &a, const uint4 &bt, const BinLds<ACC> &L, ACC init,
+                                   uint32_t *wtot, uint32_t s0, uint32_t s1) {
+  const uint32_t R = 1u << a.rlog, ns = a.ns, b = bt.x;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wv = wave_id();
+  __syncthreads();  // the previous task is done with the LDS
+  for (uint32_t i = tid; i < R; i += kIterBlock) L.acc[i] = init;
+  uint32_t sum = 0;
+  for (uint32_t s = s0; s < s1; s++) {
+    const uint32_t r0 = a.runs[(size_t)b * ns + s], r1 = a.runs[(size_t)(b + 1) * ns + s];
+    L.rbase[s] = r0;
+    L.pre[s] = r1 - r0;
+    sum += r1 - r0;
+  }
+  const uint32_t incl = wave_inclusive_scan(sum);
+  if (lane == kWave - 1) wtot[wv] = incl;
+  __syncthreads();
+  uint32_t run = incl - sum;
+  for (uint32_t w = 0; w < wv; w++) run += wtot[w];
+  for (uint32_t s = s0; s < s1; s++) {
+    const uint32_t l = L.pre[s];
+    L.pre[s] = run;
+    run += l;
+  }
+  if (tid == kIterBlock - 1) L.pre[ns] = run;
+  __syncthreads();
+  // chunk index of the slice: every kChunk-th message's run bounds the searches below
+  const uint32_t nch = (bt.z - bt.y + kChunk - 1) / kChunk;
+  for (uint32_t s = s0; s < s1; s++) {
+    const uint32_t p0 = max(L.pre[s], bt.y), p1 = min(L.pre[s + 1], bt.z);
+    for (uint32_t c = (p0 - bt.y + kChunk - 1) / kChunk; p0 < p1 && bt.y + c * kChunk < p1; c++) L.cidx[c] = s;
+  }
+  if (tid == 0) L.cidx[nch] = ns - 1;
+  __syncthreads();
+}
+
+// Calls add(row, message word) for every message [bt.y, bt.z) of the task, kUnroll messages
+// in flight per thread; the caller synchronises afterwards.
+template <class ACC, class F>
+__device__ inline void bin_messages(const StripArgs &a, const uint4 &bt, const BinLds<ACC> &L, F add) {
+  const uint32_t *msgw = reinterpret_cast<const uint32_t *>(a.msg);
+  for (uint32_t jb = bt.y + threadIdx.x; jb < bt.z; jb += kIterBlock * kUnroll) {
+    uint32_t m[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t j = jb + u * kIterBlock;
+      m[u] = 0xFFFFFFFFu;
+      if (j < bt.z) {
+        // the run holding message j: the last r with pre[r] <= j (never an empty run)
+        const uint32_t c = (j - bt.y) / kChunk;
+        uint32_t lo = L.cidx[c], hi = L.cidx[c + 1];
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) >> 1;
+          if (L.pre[mid] <= j) lo = mid;
+          else hi = mid - 1;
+        }
+        m[u] = L.rbase[lo] + (j - L.pre[lo]);
+      }
+    }
+    uint32_t x[kUnroll], r[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      if (m[u] != 0xFFFFFFFFu) {
+        x[u] = __builtin_nontemporal_load(&msgw[m[u]]);
+        r[u] = __builtin_nontemporal_load(&a.seg_row[m[u]]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++)
+      if (m[u] != 0xFFFFFFFFu) add(r[u], x[u]);
+  }
+}
+
 __global__ __launch_bounds__(kIterBlock) void pr_bin_kernel(StripArgs a) {
-  extern __shared__ unsigned long long acc[];  // 2^rlog accumulators, then the bin's run table
+  extern __shared__ unsigned long long lds2[];
   __shared__ uint32_t wtot[kIterWaves];
   const uint32_t R = 1u << a.rlog, ns = a.ns;
-  uint32_t *rbase = reinterpret_cast<uint32_t *>(acc + R);  // first message of each run
-  uint32_t *pre = rbase + ns;                               // exclusive prefix of run lengths, ns+1
-  uint32_t *cidx = pre + ns + 1;  // run holding message bt.y + c*kChunk, c <= slice/kChunk
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wv = wave_id();
+  const BinLds<unsigned long long> L = bin_lds<unsigned long long>(lds2, R, ns);
+  const uint32_t tid = threadIdx.x;
   const float base = a.sc->base, extra = a.sc->extra;
   const uint32_t per = (ns + kIterBlock - 1) / kIterBlock;
   const uint32_t s0 = min(tid * per, ns), s1 = min(s0 + per, ns);
   PrAcc pacc;
   for (uint32_t t = blockIdx.x; t < a.nbt; t += gridDim.x) {
     const uint4 bt = a.btask[t];
-    const uint32_t b = bt.x;
-    __syncthreads();  // the previous task is done with the LDS
-    for (uint32_t i = tid; i < R; i += kIterBlock) acc[i] = 0ull;
-    uint32_t sum = 0;
-    for (uint32_t s = s0; s < s1; s++) {
-      const uint32_t r0 = a.runs[(size_t)b * ns + s], r1 = a.runs[(size_t)(b + 1) * ns + s];
-      rbase[s] = r0;
-      pre[s] = r1 - r0;
-      sum += r1 - r0;
-    }
-    const uint32_t incl = wave_inclusive_scan(sum);
-    if (lane == kWave - 1) wtot[wv] = incl;
-    __syncthreads();
-    uint32_t run = incl - sum;
-    for (uint32_t w = 0; w < wv; w++) run += wtot[w];
-    for (uint32_t s = s0; s < s1; s++) {
-      const uint32_t l = pre[s];
-      pre[s] = run;
-      run += l;
-    }
-    if (tid == kIterBlock - 1) pre[ns] = run;
-    __syncthreads();
-    // chunk index of the slice: every kChunk-th message's run bounds the searches below
-    const uint32_t nch = (bt.z - bt.y + kChunk - 1) / kChunk;
-    for (uint32_t s = s0; s < s1; s++) {
-      const uint32_t p0 = max(pre[s], bt.y), p1 = min(pre[s + 1], bt.z);
-      for (uint32_t c = (p0 - bt.y + kChunk - 1) / kChunk; p0 < p1 && bt.y + c * kChunk < p1; c++) cidx[c] = s;
-    }
-    if (tid == 0) cidx[nch] = ns - 1;
-    __syncthreads();
-    for (uint32_t jb = bt.y + tid; jb < bt.z; jb += kIterBlock * kUnroll) {
-      uint32_t m[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-        const uint32_t j = jb + u * kIterBlock;
-        m[u] = 0xFFFFFFFFu;
-        if (j < bt.z) {
-          // the run holding message j: the last r with pre[r] <= j (never an empty run)
-          const uint32_t c = (j - bt.y) / kChunk;
-          uint32_t lo = cidx[c], hi = cidx[c + 1];
-          while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (pre[mid] <= j) lo = mid;
-            else hi = mid - 1;
-          }
-          m[u] = rbase[lo] + (j - pre[lo]);
-        }
-      }
-      float x[kUnroll];
-      uint32_t r[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-        if (m[u] != 0xFFFFFFFFu) {
-          x[u] = __builtin_nontemporal_load(&a.msg[m[u]]);
-          r[u] = __builtin_nontemporal_load(&a.seg_row[m[u]]);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kUnroll; u++)
-        if (m[u] != 0xFFFFFFFFu) atomicAdd(&acc[r[u]], fix_from_float(x[u]));
-    }
+    bin_prepare(a, bt, L, 0ull, wtot, s0, s1);
+    bin_messages(a, bt, L, [&](uint32_t r, uint32_t x) { atomicAdd(&L.acc[r], fix_from_float(__uint_as_float(x))); });
     __syncthreads();
     if (bt.w == kNoSlot) {
-      const uint32_t v0 = b << a.rlog;
+      const uint32_t v0 = bt.x << a.rlog;
       pr_apply_rows<kUnroll>(
-          a, v0, min(R, a.V - v0), tid, kIterBlock, [&](uint32_t i) { return (float)fix_to_double(acc[i]); }, base,
+          a, v0, min(R, a.V - v0), tid, kIterBlock, [&](uint32_t i) { return (float)fix_to_double(L.acc[i]); }, base,
           extra, pacc);
     } else {
       unsigned long long *dst = a.bpart + ((size_t)bt.w << a.rlog);
-      for (uint32_t i = tid; i < R; i += kIterBlock) dst[i] = acc[i];
+      for (uint32_t i = tid; i < R; i += kIterBlock) dst[i] = L.acc[i];
     }
   }
   block_store_acc<kIterWaves>(pacc, a.pr1, a.pd1);
@@ -460,6 +570,76 @@ __global__ __launch_bounds__(kBlock) void pr_split_apply_kernel(StripArgs a) {
     }
   }
   block_store_acc<kBlock / kWave>(pacc, a.pr2, a.pd2);
+}
+
+// CC bins: LDS atomicMax of the messages (exact, order-free), then nl = max(nl, bin maximum)
+// in place; split bins leave their maxima in bpart (as 32-bit words).
+__device__ inline void cc_count_changed(uint32_t n, uint32_t *changed) {
+  n = read_lane_u32(wave_scan_dpp(n), kWave - 1);
+  if (lane_id() == 0 && n) atomicAdd(changed, n);
+}
+
+__global__ __launch_bounds__(kIterBlock) void cc_bin_kernel(StripArgs a) {
+  extern __shared__ uint32_t lds2u[];
+  __shared__ uint32_t wtot[kIterWaves];
+  const uint32_t R = 1u << a.rlog, ns = a.ns;
+  const BinLds<uint32_t> L = bin_lds<uint32_t>(lds2u, R, ns);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t per = (ns + kIterBlock - 1) / kIterBlock;
+  const uint32_t s0 = min(tid * per, ns), s1 = min(s0 + per, ns);
+  uint32_t changed = 0;
+  for (uint32_t t = blockIdx.x; t < a.nbt; t += gridDim.x) {
+    const uint4 bt = a.btask[t];
+    bin_prepare(a, bt, L, 0u, wtot, s0, s1);
+    bin_messages(a, bt, L, [&](uint32_t r, uint32_t x) { atomicMax(&L.acc[r], x); });
+    __syncthreads();
+    if (bt.w == kNoSlot) {
+      const uint32_t v0 = bt.x << a.rlog, n = min(R, a.V - v0);
+      for (uint32_t i = tid; i < n; i += kIterBlock) {
+        const uint32_t old = a.nlab[v0 + i], m = max(old, L.acc[i]);
+        if (m != old) {
+          a.nlab[v0 + i] = m;
+          changed++;
+        }
+      }
+    } else {
+      uint32_t *dst = reinterpret_cast<uint32_t *>(a.bpart) + ((size_t)bt.w << a.rlog);
+      for (uint32_t i = tid; i < R; i += kIterBlock) dst[i] = L.acc[i];
+    }
+  }
+  cc_count_changed(changed, a.changed);
+}
+
+__global__ __launch_bounds__(kBlock) void cc_split_apply_kernel(StripArgs a) {
+  const uint32_t R = 1u << a.rlog;
+  const uint32_t *part = reinterpret_cast<const uint32_t *>(a.bpart);
+  const uint32_t nck = (R + kSplitRows - 1) / kSplitRows;  // row chunks per split bin
+  uint32_t changed = 0;
+  for (uint32_t t = blockIdx.x; t < a.nsplit * nck; t += gridDim.x) {
+    const uint4 sp = a.split[t / nck];
+    const uint32_t r0 = (t % nck) * kSplitRows;
+    const uint32_t v0 = sp.x << a.rlog;
+    const uint32_t n = min(min(R, a.V - v0), r0 + kSplitRows);
+    for (uint32_t i = r0 + threadIdx.x; i < n; i += kBlock) {
+      uint32_t m = a.nlab[v0 + i];
+      const uint32_t old = m;
+      for (uint32_t k = 0; k < sp.z; k++) m = max(m, part[((size_t)(sp.y + k) << a.rlog) + i]);
+      if (m != old) {
+        a.nlab[v0 + i] = m;
+        changed++;
+      }
+    }
+  }
+  cc_count_changed(changed, a.changed);
+}
+
+__global__ void cc_init_kernel(uint32_t *nl, const uint32_t *n2o, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    nl[i] = ~(n2o ? n2o[i] : i);
+}
+
+__global__ void cc_complement_kernel(uint32_t *x, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = ~x[i];
 }
 
 int lds_limit(Engine &e) {
@@ -710,6 +890,69 @@ void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2)
     GMP_CHECK_LAUNCH();
     n2 = S.g3;
   }
+}
+
+bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms) {
+  if (!ensure_strips(e)) return false;
+  const StripCsr &S = e.g.strips;
+  const uint32_t *n2o = e.g.work_identity ? nullptr : e.g.n2o.as<uint32_t>();  // built with the layout
+  const uint32_t V = e.g.V, R = 1u << S.rlog;
+  const size_t lds2 = (size_t)4 * R + (2 * (size_t)S.ns + 1) * 4 + ((size_t)(1u << e.opt.bin_task_log) / kChunk + 2) * 4;
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&cc_strip_gather_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
+    (void)hipGetLastError();
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&cc_bin_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds2) != hipSuccess)
+    (void)hipGetLastError();
+  DevBuf changed;
+  changed.alloc(4);
+  StripArgs a{};
+  a.code = S.code.as<uint16_t>();
+  a.wtask = S.wtask.as<uint4>();
+  a.gtask = S.gtask.as<uint4>();
+  a.ngt = S.ngt;
+  a.msg = S.msg.as<float>();
+  a.seg_row = S.seg_row.as<uint16_t>();
+  a.runs = S.runs.as<uint32_t>();
+  a.ns = S.ns;
+  a.btask = S.btask.as<uint4>();
+  a.nbt = S.nbt;
+  a.bpart = S.bpart.as<unsigned long long>();
+  a.split = S.split.as<uint4>();
+  a.nsplit = S.nsplit;
+  a.V = V;
+  a.wlog = S.wlog;
+  a.rlog = S.rlog;
+  a.nlab = label;
+  a.changed = changed.as<uint32_t>();
+  const uint32_t gi = (uint32_t)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, (uint64_t)e.num_cus * 8);
+  EventPair tot;
+  GMP_HIP(hipEventRecord(tot.a, e.stream));
+  cc_init_kernel<<<gi, kBlock, 0, e.stream>>>(label, n2o, V);
+  GMP_CHECK_LAUNCH();
+  sweeps = 0;
+  for (;;) {
+    GMP_HIP(hipMemsetAsync(changed.p, 0, 4, e.stream));
+    cc_strip_gather_kernel<<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+    GMP_CHECK_LAUNCH();
+    cc_bin_kernel<<<S.g2, kIterBlock, lds2, e.stream>>>(a);
+    GMP_CHECK_LAUNCH();
+    if (S.nsplit) {
+      cc_split_apply_kernel<<<S.g3, kBlock, 0, e.stream>>>(a);
+      GMP_CHECK_LAUNCH();
+    }
+    uint32_t h = 0;
+    GMP_HIP(hipMemcpyAsync(&h, changed.p, 4, hipMemcpyDeviceToHost, e.stream));
+    GMP_HIP(hipStreamSynchronize(e.stream));
+    sweeps++;
+    if (h == 0) break;
+  }
+  cc_complement_kernel<<<gi, kBlock, 0, e.stream>>>(label, V);
+  GMP_CHECK_LAUNCH();
+  GMP_HIP(hipEventRecord(tot.b, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  ms = tot.ms();
+  return true;
 }
 
 }  // namespace gmp

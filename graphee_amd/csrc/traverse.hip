@@ -775,6 +775,27 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
 template <class OffT>
 void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
   Graph &g = e.g;
+  {
+    // dense sweeps over the source-strip layout when it applies: every sweep streams the edges
+    // once (2-byte codes, labels staged in LDS), which beats frontier relaxation on graphs of
+    // small diameter
+    e.s_a.alloc(std::max<size_t>(g.V, 1) * 4);  // finish_output maps through s_f
+    int sweeps = 0;
+    double ms = 0.0;
+    if (strip_cc(e, e.s_a.as<uint32_t>(), sweeps, ms)) {
+      finish_output(e, e.s_a.as<uint32_t>(), label_out);
+      if (st) {
+        std::memset(st, 0, sizeof *st);
+        st->iterations = sweeps;
+        st->launches = sweeps;
+        st->ms_total = ms;
+        st->ms_kernel = ms;
+        st->edges = (uint64_t)sweeps * g.E;
+        st->bytes = st->edges * 8;
+      }
+      return;
+    }
+  }
   ensure_work_out(e);
   const uint32_t V = g.V;
   ensure_hubs<OffT>(e);

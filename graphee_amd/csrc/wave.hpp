@@ -157,6 +157,34 @@ __device__ inline double wave_segmented_scan_dpp_short(double x, bool head) {
   return x;
 }
 
+template <int CTRL, int ROWS = 0xF>
+__device__ inline void seg_max_step(uint32_t &x, uint32_t &f) {
+  const uint32_t y = dpp_u32<CTRL, ROWS>(x);
+  const uint32_t fy = dpp_u32<CTRL, ROWS>(f);
+  if (!f) x = max(x, y);
+  f |= fy;
+}
+
+// Inclusive segmented maximum over the 64 lanes with the same early exits as
+// wave_segmented_scan_dpp_short; 0 (what the DPP moves in past a row's start) is neutral.
+__device__ inline uint32_t wave_segmented_max_dpp_short(uint32_t x, bool head) {
+  const uint32_t lane = lane_id(), rl = lane & 15u;
+  uint32_t f = head ? 1u : 0u;
+  seg_max_step<kDppRowShr1>(x, f);
+  if (__ballot(!f && rl >= 2u)) {
+    seg_max_step<kDppRowShr2>(x, f);
+    if (__ballot(!f && rl >= 4u)) {
+      seg_max_step<kDppRowShr4>(x, f);
+      if (__ballot(!f && rl >= 8u)) seg_max_step<kDppRowShr8>(x, f);
+    }
+  }
+  if (__ballot(!f && lane >= 16u)) {
+    seg_max_step<kDppRowBcast15, 0xA>(x, f);
+    seg_max_step<kDppRowBcast31, 0xC>(x, f);
+  }
+  return x;
+}
+
 __device__ inline uint32_t read_lane_u32(uint32_t x, int lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
 }

@@ -168,6 +168,33 @@ def test_cc_matches_unionfind(engine, scale):
     assert st.iterations >= 1
 
 
+@pytest.mark.parametrize("strip_log,bin_log,task_log", [(0, 14, 20), (8, 8, 8), (11, 10, 10), (15, 14, 8)])
+def test_cc_sweeps_and_frontier_agree(engine, strip_log, bin_log, task_log):
+    """CC by min-label sweeps over the strip layout (any strip width, bin size, split size) and
+    by frontier relaxation (strip_log 0) give the union-find labels, also on a graph of many
+    components with a long path (many sweeps) and with a hub."""
+    for k, v in zip((gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG), (strip_log, bin_log, task_log)):
+        engine.set_option(k, v)
+    try:
+        g = gen(engine, 12, undirected=True)
+        say(f"cc strips {strip_log}/{bin_log}/{task_log}")
+        lab, _ = engine.cc()
+        np.testing.assert_array_equal(lab, oracle.cc_unionfind(g["off"], g["idx"]))
+        V = 30000
+        rng = np.random.default_rng(11)
+        a = np.concatenate([np.arange(0, 3000), rng.integers(5000, V, 20000), np.full(4000, 29999)])
+        b = np.concatenate([np.arange(1, 3001), rng.integers(5000, V, 20000), rng.integers(5000, V, 4000)])
+        src, dst = np.concatenate([a, b]), np.concatenate([b, a])  # symmetric
+        off, idx = graphs.csr_from_edges(V, src, dst)
+        engine.load_csr(off, idx)
+        lab, st = engine.cc()
+        np.testing.assert_array_equal(lab, oracle.cc_unionfind(off, idx))
+        assert st.iterations >= (3000 if strip_log == 0 else 2)
+    finally:
+        for k, v in STRIP_DEFAULTS:
+            engine.set_option(k, v)
+
+
 def test_off64_offsets_path(engine):
     g = gen(engine, 11, weighted=True, off64=True)
     V = g["outdeg"].size
