@@ -98,11 +98,11 @@ struct CsrView {
   template <class OffT> const OffT *offs() const { return static_cast<const OffT *>(off); }
 };
 
-// Source-strip layout of the work in-CSR for the PageRank gather: edges sorted by
-// (strip of the source, destination, source) where a strip is 2^wlog consecutive work ids.
-// A "pair" is the run of edges of one destination inside one strip; the gather sums pairs
-// strip by strip (so only one strip's contributions are live in L2 at a time) and a row-block
-// combine adds each row's pair partials in strip order.
+// Source-strip layout of the work in-CSR (strip.hip): edges sorted by (strip of the source,
+// destination, source) where a strip is 2^wlog consecutive work ids.  A segment is the run of
+// edges of one destination inside one strip (also cut at wave-task boundaries); the gather
+// stages one strip's values in LDS and leaves one message per segment, and bins of 2^rlog
+// destinations combine their messages.  PageRank sums; CC and SSSP sweeps take minima.
 struct StripCsr {
   uint32_t wlog = 0, rlog = 0;  // strips of 2^wlog sources, bins of 2^rlog destinations
   uint32_t ns = 0, nbins = 0;   // strips up to the last one holding an edge; destination bins
@@ -116,6 +116,8 @@ struct StripCsr {
   DevBuf split;                 // uint32x4[nsplit]: {bin, first slot, slots, 0}
   DevBuf msg;                   // float[nseg]: the iteration's segment sums
   DevBuf bpart;                 // uint64[nslots << rlog]: fixed-point sums of split bins
+  DevBuf wt;                    // uint32[E+pad]: edge weights in layout order (weighted SSSP sweeps)
+  bool wt_valid = false;
   uint32_t nwt = 0, ngt = 0, nbt = 0, nsplit = 0, nslots = 0;
   uint32_t lds1 = 0, lds2 = 0;  // dynamic LDS bytes of the gather and bin kernels
   uint32_t g1 = 0, g2 = 0, g3 = 0;  // grids of the gather, bin and split-apply kernels
@@ -292,8 +294,15 @@ bool ensure_strips(Engine &e);
 // Connected components by synchronous min-label sweeps over the strip layout: label (V entries
 // in work order, device) receives, per vertex, the least original id that reaches it along
 // edges -- GraphMP's CC rule iterated to its fixed point.  Returns false, having launched
-// nothing, when the layout does not apply; ms is the device time of the sweeps.
-bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms);
+// nothing, when the layout does not apply; ms is the device time of the sweeps and edges the
+// edges gathered (a sweep skips strips none of whose vertices changed in the one before).
+bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms, uint64_t &edges);
+// Bellman-Ford by synchronous sweeps over the strip layout: dist (V entries in work order,
+// device) receives the shortest-path lengths from work id src (unit weights, or the graph's
+// weights), 0xFFFFFFFF where unreached; sums saturate at 0xFFFFFFFE like the frontier
+// relaxation.  Returns false, having launched nothing, when the layout does not apply; ms and
+// edges as for strip_cc.
+bool strip_sssp(Engine &e, uint32_t src, bool unit, uint32_t *dist, int &sweeps, double &ms, uint64_t &edges);
 // Row id of every edge of a CSR (device array of E entries).
 void edge_row_ids(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, DevBuf &rowid);
 

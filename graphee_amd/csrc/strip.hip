@@ -50,6 +50,12 @@ constexpr uint32_t kSplitRows = 1024;          // split apply: rows per block ta
 constexpr uint32_t kGroupWaveTasks = 256;      // wave tasks per workgroup task (one staging)
 constexpr uint32_t kCodePad = kRound;          // codes read past the last edge stay in bounds
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kNever = 0xFFFFFFFFu;       // strip flag: no source changed yet
+
+// What a sweep over the strips computes.  PageRank sums contributions; the others keep
+// complemented values (c = ~x), so the least label or distance is the greatest c and 0 -- what
+// DPP moves in past a row's start, and the value of an unreached vertex -- is neutral.
+enum SweepMode { kSweepPR = 0, kSweepCC = 1, kSweepUnit = 2, kSweepWeighted = 3 };
 
 #define GS64(i, n) \
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
@@ -92,9 +98,13 @@ struct StripArgs {
   uint32_t nsplit;
   uint32_t V, wlog, rlog;
   double *pr1, *pd1, *pr2, *pd2;
-  // connected components
-  uint32_t *nlab;     // complemented labels, updated in place
-  uint32_t *changed;  // vertices whose label dropped in the sweep
+  // CC and SSSP sweeps
+  uint32_t *nlab;          // complemented labels or distances, updated in place
+  uint32_t *changed;       // vertices whose value dropped in the sweep
+  const uint32_t *wt;      // edge weights in layout order (weighted SSSP)
+  uint32_t *sflag;         // per strip: the last sweep that changed one of its vertices
+  uint32_t sweep;          // this sweep's number, from 1
+  unsigned long long *edges;  // edges gathered (tasks not skipped)
 };
 
 // ---------------------------------------------------------------- build kernels
@@ -285,12 +295,13 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &
   B.fill += total;
 }
 
-// The same round for connected components: labels are complemented (nl = ~label), so a
-// segment's message is the maximum nl of its sources -- the least label -- and 0, which DPP
-// moves in past a row's start, is the neutral element.
-template <bool FULL>
-__device__ inline void cc_round(const StripArgs &a, const uint32_t *sv, MsgBuf &B, u32x4 cv, uint32_t q, uint32_t e0,
-                                uint32_t e1, uint64_t upto, uint32_t &carry) {
+// The round of the CC and SSSP sweeps: the value an edge offers its destination is the source's
+// complemented label (CC) or the complement of min(distance + weight, 2^32 - 2) (SSSP; 0 from
+// an unreached source), a segment's message is the greatest offer, and the open segment's
+// value carries over lanes and rounds by a segmented maximum.
+template <bool FULL, int MODE>
+__device__ inline void min_round(const StripArgs &a, const uint32_t *sv, MsgBuf &B, u32x4 cv, u32x4 w0, u32x4 w1,
+                                 uint32_t q, uint32_t e0, uint32_t e1, uint64_t upto, uint32_t &carry) {
   const uint32_t lane = lane_id();
   const bool live = q < e1 && q + 8 > e0;
   uint32_t x[8];
@@ -298,7 +309,13 @@ __device__ inline void cc_round(const StripArgs &a, const uint32_t *sv, MsgBuf &
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const uint32_t c = (cv[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
-    const uint32_t v = sv[c & 0x7FFFu];
+    uint32_t v = sv[c & 0x7FFFu];
+    if (MODE != kSweepCC) {
+      const uint32_t w = MODE == kSweepWeighted ? (k < 4 ? w0[k] : w1[k - 4]) : 1u;
+      const uint32_t d = ~v;
+      const uint32_t t = d > 0xFFFFFFFEu - w ? 0xFFFFFFFEu : d + w;
+      v = v ? ~t : 0u;
+    }
     if (FULL) {
       x[k] = v;
       ends |= (c >> 15) << k;
@@ -338,34 +355,55 @@ __device__ inline void cc_round(const StripArgs &a, const uint32_t *sv, MsgBuf &
   B.fill += total;
 }
 
+// Weights of round r of a wave task (8 per lane, two 16-byte loads); zero outside [e0, e1).
+__device__ inline void load_round_w(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1, u32x4 &w0,
+                                    u32x4 &w1) {
+  const uint32_t q = b0 + r * kRound + 8 * lane_id();
+  w0 = u32x4{0u, 0u, 0u, 0u};
+  w1 = w0;
+  if (q < e1 && q + 8 > e0) {
+    w0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.wt + q));
+    w1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.wt + q + 4));
+  }
+}
+
 // Messages of edges [e0, e1) whose first segment has index seg; every segment ends inside.
 // The code loads run kAhead rounds ahead of their use.
-template <bool CC>
+template <int MODE>
 __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, float *mb, uint32_t e0, uint32_t e1,
                                        uint32_t seg) {
   const uint32_t lane = lane_id();
   const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
   const uint32_t b0 = e0 & ~7u;
   const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
-  typename std::conditional<CC, uint32_t, double>::type carry = 0;  // the open segment's value
+  typename std::conditional<MODE == kSweepPR, double, uint32_t>::type carry = 0;  // the open segment's value
   MsgBuf B{mb, seg, 0u};
-  u32x4 buf[kAhead];
+  u32x4 buf[kAhead], wa[kAhead], wb[kAhead];
 #pragma unroll
-  for (int i = 0; i < kAhead; i++) buf[i] = load_round(a, b0, i, e0, e1);
+  for (int i = 0; i < kAhead; i++) {
+    buf[i] = load_round(a, b0, i, e0, e1);
+    if (MODE == kSweepWeighted) load_round_w(a, b0, i, e0, e1, wa[i], wb[i]);
+  }
   for (uint32_t r = 0; r < nr; r += kAhead) {
 #pragma unroll
     for (int i = 0; i < kAhead; i++) {
       if (r + i < nr) {
         const u32x4 cv = buf[i];
         buf[i] = load_round(a, b0, r + i + kAhead, e0, e1);
+        u32x4 w0{}, w1{};
+        if (MODE == kSweepWeighted) {
+          w0 = wa[i];
+          w1 = wb[i];
+          load_round_w(a, b0, r + i + kAhead, e0, e1, wa[i], wb[i]);
+        }
         const uint32_t base = b0 + (r + i) * kRound;
         const uint32_t q = base + 8 * lane;
         // lanes are cut only where e0 or e1 is not a multiple of 8 inside this round
         const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
-        if constexpr (CC) {
+        if constexpr (MODE != kSweepPR) {
           const uint32_t *su = reinterpret_cast<const uint32_t *>(sv);
-          if (cut) cc_round<false>(a, su, B, cv, q, e0, e1, upto, carry);
-          else cc_round<true>(a, su, B, cv, q, e0, e1, upto, carry);
+          if (cut) min_round<false, MODE>(a, su, B, cv, w0, w1, q, e0, e1, upto, carry);
+          else min_round<true, MODE>(a, su, B, cv, w0, w1, q, e0, e1, upto, carry);
         } else {
           if (cut) strip_round<false>(a, sv, B, cv, q, e0, e1, upto, carry);
           else strip_round<true>(a, sv, B, cv, q, e0, e1, upto, carry);
@@ -378,14 +416,20 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
 
 // One strip's values (contributions, or complemented labels for CC) staged in LDS, then the
 // strip's wave tasks.
-template <bool CC>
+template <int MODE>
 __device__ inline void strip_gather(const StripArgs &a, float *sv) {
   const uint32_t W = 1u << a.wlog;
-  const uint32_t *vals = CC ? a.nlab : reinterpret_cast<const uint32_t *>(a.contrib_in);
+  const uint32_t *vals = MODE != kSweepPR ? a.nlab : reinterpret_cast<const uint32_t *>(a.contrib_in);
   const uint32_t wv = wave_id();
   float *mb = sv + W + wv * kRound;
   for (uint32_t t = blockIdx.x; t < a.ngt; t += gridDim.x) {
     const uint4 gt = a.gtask[t];
+    // a sweep skips strips none of whose vertices changed in the previous sweep: their
+    // messages from an earlier sweep (or the cleared ones) still hold
+    if (MODE != kSweepPR) {
+      if (a.sflag[gt.x] + 1u != a.sweep) continue;
+      if (threadIdx.x == 0) atomicAdd(a.edges, (unsigned long long)(a.wtask[gt.z - 1].y - a.wtask[gt.y].x));
+    }
     const uint32_t v0 = gt.x << a.wlog;
     const uint32_t n = min(W, a.V - v0);
     __syncthreads();  // the previous task's readers are done
@@ -409,19 +453,21 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
     __syncthreads();
     for (uint32_t w = gt.y + wv; w < gt.z; w += kIterWaves) {
       const uint4 wt = a.wtask[w];
-      strip_wave_task<CC>(a, sv, mb, wt.x, wt.y, wt.z);
+      strip_wave_task<MODE>(a, sv, mb, wt.x, wt.y, wt.z);
     }
   }
 }
 
 __global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a) {
   extern __shared__ float sv[];  // contributions of the task's strip, then a message buffer per wave
-  strip_gather<false>(a, sv);
+  strip_gather<kSweepPR>(a, sv);
 }
 
-__global__ __launch_bounds__(kIterBlock) void cc_strip_gather_kernel(StripArgs a) {
-  extern __shared__ float sv[];  // complemented labels of the task's strip, then the message buffers
-  strip_gather<true>(a, sv);
+// complemented labels or distances of the task's strip, then the message buffers
+template <int MODE>
+__global__ __launch_bounds__(kIterBlock) void min_strip_gather_kernel(StripArgs a) {
+  extern __shared__ float sv[];
+  strip_gather<MODE>(a, sv);
 }
 
 // LDS of a bin task: 2^rlog accumulators of type ACC, then the bin's run table -- the first
@@ -572,14 +618,24 @@ __global__ __launch_bounds__(kBlock) void pr_split_apply_kernel(StripArgs a) {
   block_store_acc<kBlock / kWave>(pacc, a.pr2, a.pd2);
 }
 
-// CC bins: LDS atomicMax of the messages (exact, order-free), then nl = max(nl, bin maximum)
-// in place; split bins leave their maxima in bpart (as 32-bit words).
-__device__ inline void cc_count_changed(uint32_t n, uint32_t *changed) {
+// CC and SSSP bins: LDS atomicMax of the messages (exact, order-free), then c = max(c, bin
+// maximum) in place, marking the strip of every vertex that changed; split bins leave their
+// maxima in bpart (as 32-bit words).
+__device__ inline void min_count_changed(uint32_t n, uint32_t *changed) {
   n = read_lane_u32(wave_scan_dpp(n), kWave - 1);
   if (lane_id() == 0 && n) atomicAdd(changed, n);
 }
 
-__global__ __launch_bounds__(kIterBlock) void cc_bin_kernel(StripArgs a) {
+__device__ inline void min_apply(const StripArgs &a, uint32_t v, uint32_t m, uint32_t &changed) {
+  const uint32_t old = a.nlab[v];
+  if (m > old) {
+    a.nlab[v] = m;
+    a.sflag[v >> a.wlog] = a.sweep;
+    changed++;
+  }
+}
+
+__global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
   extern __shared__ uint32_t lds2u[];
   __shared__ uint32_t wtot[kIterWaves];
   const uint32_t R = 1u << a.rlog, ns = a.ns;
@@ -595,22 +651,16 @@ __global__ __launch_bounds__(kIterBlock) void cc_bin_kernel(StripArgs a) {
     __syncthreads();
     if (bt.w == kNoSlot) {
       const uint32_t v0 = bt.x << a.rlog, n = min(R, a.V - v0);
-      for (uint32_t i = tid; i < n; i += kIterBlock) {
-        const uint32_t old = a.nlab[v0 + i], m = max(old, L.acc[i]);
-        if (m != old) {
-          a.nlab[v0 + i] = m;
-          changed++;
-        }
-      }
+      for (uint32_t i = tid; i < n; i += kIterBlock) min_apply(a, v0 + i, L.acc[i], changed);
     } else {
       uint32_t *dst = reinterpret_cast<uint32_t *>(a.bpart) + ((size_t)bt.w << a.rlog);
       for (uint32_t i = tid; i < R; i += kIterBlock) dst[i] = L.acc[i];
     }
   }
-  cc_count_changed(changed, a.changed);
+  min_count_changed(changed, a.changed);
 }
 
-__global__ __launch_bounds__(kBlock) void cc_split_apply_kernel(StripArgs a) {
+__global__ __launch_bounds__(kBlock) void min_split_apply_kernel(StripArgs a) {
   const uint32_t R = 1u << a.rlog;
   const uint32_t *part = reinterpret_cast<const uint32_t *>(a.bpart);
   const uint32_t nck = (R + kSplitRows - 1) / kSplitRows;  // row chunks per split bin
@@ -621,24 +671,27 @@ __global__ __launch_bounds__(kBlock) void cc_split_apply_kernel(StripArgs a) {
     const uint32_t v0 = sp.x << a.rlog;
     const uint32_t n = min(min(R, a.V - v0), r0 + kSplitRows);
     for (uint32_t i = r0 + threadIdx.x; i < n; i += kBlock) {
-      uint32_t m = a.nlab[v0 + i];
-      const uint32_t old = m;
+      uint32_t m = 0;
       for (uint32_t k = 0; k < sp.z; k++) m = max(m, part[((size_t)(sp.y + k) << a.rlog) + i]);
-      if (m != old) {
-        a.nlab[v0 + i] = m;
-        changed++;
-      }
+      min_apply(a, v0 + i, m, changed);
     }
   }
-  cc_count_changed(changed, a.changed);
+  min_count_changed(changed, a.changed);
 }
 
-__global__ void cc_init_kernel(uint32_t *nl, const uint32_t *n2o, uint32_t n) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    nl[i] = ~(n2o ? n2o[i] : i);
+// Starting values: CC c = ~(original id) and every strip active; SSSP c = 0 (unreached) but
+// ~0 at the source, and only the source's strip active.  Messages start at 0 (neutral), so a
+// strip no sweep has gathered yet offers nothing.
+__global__ void min_init_kernel(uint32_t *c, const uint32_t *n2o, uint32_t n, uint32_t src, uint32_t *sflag,
+                                uint32_t ns, uint32_t wlog, float *msg, uint64_t nseg) {
+  const uint32_t stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t i = i0; i < n; i += stride) c[i] = src == kNever ? ~(n2o ? n2o[i] : i) : (i == src ? ~0u : 0u);
+  for (uint32_t s = i0; s < ns; s += stride) sflag[s] = (src == kNever || s == (src >> wlog)) ? 0u : kNever;
+  uint32_t *m = reinterpret_cast<uint32_t *>(msg);
+  for (uint64_t j = i0; j < nseg; j += stride) m[j] = 0u;
 }
 
-__global__ void cc_complement_kernel(uint32_t *x, uint32_t n) {
+__global__ void complement_kernel(uint32_t *x, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = ~x[i];
 }
 
@@ -892,20 +945,68 @@ void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2)
   }
 }
 
-bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms) {
+namespace {
+
+// Weights of the work in-CSR in layout order (the same key sort as the codes, weights as
+// payload); kCodePad zero entries past the end keep the 16-byte round loads in bounds.
+void ensure_strip_weights(Engine &e) {
+  Graph &g = e.g;
+  StripCsr &S = g.strips;
+  if (S.wt_valid) return;
+  const CsrView w = g.work_in();
+  if (!w.w) fail(GMP_ESTATE, "graph has no weights");
+  const uint32_t V = g.V;
+  const uint64_t E = g.E;
+  const uint32_t b = bits_for(V), wlog = S.wlog;
+  const uint32_t sbits = bits_for(((uint64_t)V + (1ull << wlog) - 1) >> wlog);
+  DevBuf rowid, kA, kB, vB, tmp;
+  edge_row_ids(e, w, V, E, g.off64, rowid);
+  kA.alloc(E * 8);
+  kB.alloc(E * 8);
+  vB.alloc(E * 4);
+  strip_keys_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(w.idx, rowid.as<uint32_t>(), E, b, wlog, kA.as<uint64_t>());
+  GMP_CHECK_LAUNCH();
+  rowid.release();
+  S.wt.alloc((E + kCodePad) * 4);
+  GMP_HIP(hipMemcpyAsync(S.wt.p, w.w, E * 4, hipMemcpyDeviceToDevice, e.stream));
+  GMP_HIP(hipMemsetAsync(S.wt.as<uint32_t>() + E, 0, kCodePad * 4, e.stream));
+  rocprim::double_buffer<uint64_t> dk(kA.as<uint64_t>(), kB.as<uint64_t>());
+  rocprim::double_buffer<uint32_t> dv(S.wt.as<uint32_t>(), vB.as<uint32_t>());
+  size_t tb = 0;
+  GMP_HIP(rocprim::radix_sort_pairs(nullptr, tb, dk, dv, E, 0, sbits + b + wlog, e.stream));
+  tmp.alloc(tb);
+  GMP_HIP(rocprim::radix_sort_pairs(tmp.p, tb, dk, dv, E, 0, sbits + b + wlog, e.stream));
+  if (dv.current() != S.wt.as<uint32_t>())
+    GMP_HIP(hipMemcpyAsync(S.wt.p, dv.current(), E * 4, hipMemcpyDeviceToDevice, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  S.wt_valid = true;
+}
+
+// Synchronous sweeps of one min-type rule over the strip layout until nothing changes; c is
+// the complemented result in work order.
+bool strip_sweeps(Engine &e, int mode, uint32_t src, uint32_t *c, int &sweeps, double &ms, uint64_t &edges) {
   if (!ensure_strips(e)) return false;
+  if (mode == kSweepWeighted) ensure_strip_weights(e);
   const StripCsr &S = e.g.strips;
   const uint32_t *n2o = e.g.work_identity ? nullptr : e.g.n2o.as<uint32_t>();  // built with the layout
   const uint32_t V = e.g.V, R = 1u << S.rlog;
   const size_t lds2 = (size_t)4 * R + (2 * (size_t)S.ns + 1) * 4 + ((size_t)(1u << e.opt.bin_task_log) / kChunk + 2) * 4;
-  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&cc_strip_gather_kernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
+  const void *gk = mode == kSweepCC     ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepCC>)
+                   : mode == kSweepUnit ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepUnit>)
+                                        : reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepWeighted>);
+  if (hipFuncSetAttribute(gk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
     (void)hipGetLastError();
-  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&cc_bin_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&min_bin_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)lds2) != hipSuccess)
     (void)hipGetLastError();
-  DevBuf changed;
+  DevBuf changed, sflag, nedges;
   changed.alloc(4);
+  nedges.alloc(8);
+  GMP_HIP(hipMemsetAsync(nedges.p, 0, 8, e.stream));
+  // a flag for every strip of the id range: vertices past the last strip holding an edge (no
+  // out-edges in the degree order) still change and mark their strip
+  const uint32_t nsall = (uint32_t)(((uint64_t)V + (1ull << S.wlog) - 1) >> S.wlog);
+  sflag.alloc((size_t)std::max<uint32_t>(nsall, 1) * 4);
   StripArgs a{};
   a.code = S.code.as<uint16_t>();
   a.wtask = S.wtask.as<uint4>();
@@ -923,22 +1024,29 @@ bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms) {
   a.V = V;
   a.wlog = S.wlog;
   a.rlog = S.rlog;
-  a.nlab = label;
+  a.nlab = c;
   a.changed = changed.as<uint32_t>();
+  a.wt = S.wt.as<uint32_t>();
+  a.sflag = sflag.as<uint32_t>();
+  a.edges = nedges.as<unsigned long long>();
   const uint32_t gi = (uint32_t)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, (uint64_t)e.num_cus * 8);
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
-  cc_init_kernel<<<gi, kBlock, 0, e.stream>>>(label, n2o, V);
+  min_init_kernel<<<gi, kBlock, 0, e.stream>>>(c, n2o, V, mode == kSweepCC ? kNever : src, a.sflag, nsall, S.wlog,
+                                               a.msg, S.nseg);
   GMP_CHECK_LAUNCH();
   sweeps = 0;
   for (;;) {
+    a.sweep = (uint32_t)sweeps + 1;
     GMP_HIP(hipMemsetAsync(changed.p, 0, 4, e.stream));
-    cc_strip_gather_kernel<<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+    if (mode == kSweepCC) min_strip_gather_kernel<kSweepCC><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+    else if (mode == kSweepUnit) min_strip_gather_kernel<kSweepUnit><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+    else min_strip_gather_kernel<kSweepWeighted><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
     GMP_CHECK_LAUNCH();
-    cc_bin_kernel<<<S.g2, kIterBlock, lds2, e.stream>>>(a);
+    min_bin_kernel<<<S.g2, kIterBlock, lds2, e.stream>>>(a);
     GMP_CHECK_LAUNCH();
     if (S.nsplit) {
-      cc_split_apply_kernel<<<S.g3, kBlock, 0, e.stream>>>(a);
+      min_split_apply_kernel<<<S.g3, kBlock, 0, e.stream>>>(a);
       GMP_CHECK_LAUNCH();
     }
     uint32_t h = 0;
@@ -947,12 +1055,24 @@ bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms) {
     sweeps++;
     if (h == 0) break;
   }
-  cc_complement_kernel<<<gi, kBlock, 0, e.stream>>>(label, V);
+  complement_kernel<<<gi, kBlock, 0, e.stream>>>(c, V);
   GMP_CHECK_LAUNCH();
   GMP_HIP(hipEventRecord(tot.b, e.stream));
+  GMP_HIP(hipMemcpyAsync(&edges, nedges.p, 8, hipMemcpyDeviceToHost, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
   ms = tot.ms();
   return true;
+}
+
+}  // namespace
+
+bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms, uint64_t &edges) {
+  return strip_sweeps(e, kSweepCC, kNever, label, sweeps, ms, edges);
+}
+
+bool strip_sssp(Engine &e, uint32_t src, bool unit, uint32_t *dist, int &sweeps, double &ms, uint64_t &edges) {
+  if (!unit && !e.g.work_in().w) return false;
+  return strip_sweeps(e, unit ? kSweepUnit : kSweepWeighted, src, dist, sweeps, ms, edges);
 }
 
 }  // namespace gmp

@@ -554,8 +554,30 @@ template <class OffT>
 void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gmp_trace_row *rows, uint32_t cap,
                uint32_t *nrows, uint32_t *dist_out, gmp_stats *st) {
   Graph &g = e.g;
-  ensure_work_out(e);
   const bool unit = p.unit_weights || !g.weighted;
+  if (policy == kPolicyBF) {
+    // Bellman-Ford as dense sweeps over the source-strip layout when it applies (strips whose
+    // sources did not change are skipped); the frontier form below otherwise
+    ensure_work(e);
+    e.s_a.alloc(std::max<size_t>(g.V, 1) * 4);  // finish_output maps through s_f
+    int sweeps = 0;
+    double ms = 0.0;
+    uint64_t edges = 0;
+    if (strip_sssp(e, work_id(e, src), unit, e.s_a.as<uint32_t>(), sweeps, ms, edges)) {
+      finish_output(e, e.s_a.as<uint32_t>(), dist_out);
+      if (st) {
+        std::memset(st, 0, sizeof *st);
+        st->iterations = sweeps;
+        st->launches = sweeps;
+        st->ms_total = ms;
+        st->ms_kernel = ms;
+        st->edges = edges;
+        st->bytes = edges * (unit ? 8 : 12);
+      }
+      return;
+    }
+  }
+  ensure_work_out(e);
   const int mode = unit ? kModeUnit : kModeWeighted;
   if (policy == kPolicyLearned && !e.mlp.trained) policy = kPolicyFixed;  // untrained network: plain schedule
   const bool bucketed = policy != kPolicyBF;
@@ -782,7 +804,8 @@ void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
     e.s_a.alloc(std::max<size_t>(g.V, 1) * 4);  // finish_output maps through s_f
     int sweeps = 0;
     double ms = 0.0;
-    if (strip_cc(e, e.s_a.as<uint32_t>(), sweeps, ms)) {
+    uint64_t edges = 0;
+    if (strip_cc(e, e.s_a.as<uint32_t>(), sweeps, ms, edges)) {
       finish_output(e, e.s_a.as<uint32_t>(), label_out);
       if (st) {
         std::memset(st, 0, sizeof *st);
@@ -790,8 +813,8 @@ void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
         st->launches = sweeps;
         st->ms_total = ms;
         st->ms_kernel = ms;
-        st->edges = (uint64_t)sweeps * g.E;
-        st->bytes = st->edges * 8;
+        st->edges = edges;
+        st->bytes = edges * 8;
       }
       return;
     }

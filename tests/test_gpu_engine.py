@@ -149,6 +149,45 @@ def test_sssp_matches_dijkstra(engine, method, delta):
     assert st.edges > 0
 
 
+@pytest.mark.parametrize("strip_log,bin_log,task_log", [(0, 14, 20), (8, 8, 8), (11, 10, 10), (15, 14, 8)])
+def test_sssp_bellman_ford_sweeps_and_frontier_agree(engine, strip_log, bin_log, task_log):
+    """Bellman-Ford by sweeps over the strip layout (any strip width, bin size, split size;
+    weighted, unit weights and a weight that saturates) and by frontier relaxation (strip_log 0)
+    give Dijkstra's distances, and the sweeps skip strips without changes."""
+    for k, v in zip((gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG), (strip_log, bin_log, task_log)):
+        engine.set_option(k, v)
+    try:
+        g = gen(engine, 12, weighted=True)
+        V = g["outdeg"].size
+        ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+        for s in (engine.pick_source(1), engine.pick_source(2)):
+            say(f"bf strips {strip_log}/{bin_log}/{task_log} src={s}")
+            d, st = engine.sssp(s, method=gmp.SSSP_BF)
+            np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, ow, s))
+            d, _ = engine.sssp(s, method=gmp.SSSP_BF, unit_weights=True)
+            np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, None, s))
+            assert st.edges <= st.iterations * g["idx"].size
+            if strip_log and V > (1 << strip_log):
+                assert st.edges < st.iterations * g["idx"].size  # the first sweeps skip idle strips
+        # a path of 2000 edges with one weight near 2^32 (distances saturate at 2^32 - 2), plus
+        # an unreachable part
+        V = 5000
+        src_ = np.concatenate([np.arange(0, 2000), np.arange(3000, 3999)])
+        dst_ = np.concatenate([np.arange(1, 2001), np.arange(3001, 4000)])
+        w_ = np.ones(src_.size, np.uint32)
+        w_[1500] = 0xFFFFFFF0
+        order = np.lexsort((src_, dst_))
+        off, idx = graphs.csr_from_edges(V, src_, dst_)
+        engine.load_csr(off, idx, w_[order])
+        d, st = engine.sssp(0, method=gmp.SSSP_BF)
+        ooff, oidx, ow = graphs.transpose(V, off, idx, w_[order])
+        np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, ow, 0))
+        assert d[2000] == 0xFFFFFFFE and d[3500] == 0xFFFFFFFF
+    finally:
+        for k, v in STRIP_DEFAULTS:
+            engine.set_option(k, v)
+
+
 def test_sssp_unit_weights_is_bfs(engine):
     g = gen(engine, 12)
     V = g["outdeg"].size
