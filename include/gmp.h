@@ -41,7 +41,8 @@ extern "C" {
 #define GMP_APP_SSSP_W 4
 
 /* gmp_sssp methods */
-#define GMP_SSSP_BF 0      /* frontier Bellman-Ford (selective scheduling) */
+#define GMP_SSSP_BF 0      /* Bellman-Ford: sweeps over the source strips that skip unchanged
+                              strips, or frontier relaxation when strips are off */
 #define GMP_SSSP_DELTA 1   /* fixed-delta stepping */
 #define GMP_SSSP_LEARNED 2 /* delta chosen per bucket epoch by the trained MLP */
 
@@ -113,7 +114,8 @@ int gmp_destroy(gmp_engine *h);
 #define GMP_OPT_CHUNK 4      /* gather plan: edges per hub chunk, 64..2^20 */
 #define GMP_OPT_WG_PER_CU 5  /* gather grid: workgroups per compute unit, 1..64 */
 #define GMP_OPT_VEC_MIN 6    /* gather plan: rows above this many edges are reduced wave-wide */
-#define GMP_OPT_STRIP_LOG 7  /* PageRank: source strips of 2^k ids, 8..15 (default 15; 0 = row gather) */
+#define GMP_OPT_STRIP_LOG 7  /* source strips of 2^k ids, 8..15 (default 15); 0 = PageRank row
+                                gather, frontier CC and Bellman-Ford */
 #define GMP_OPT_BIN_LOG 8    /* PageRank strips: destination bins of up to 2^k rows, 8..14 (default 14) */
 #define GMP_OPT_BIN_TASK_LOG 9 /* PageRank strips: messages per bin task 2^k, 8..22 (default 20) */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
@@ -171,7 +173,9 @@ int gmp_pagerank(gmp_engine *h, const gmp_pr_params *p, float *rank_out, gmp_sta
 int gmp_sssp(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, uint32_t *dist_out,
              gmp_stats *st);
 /* Connected components of a symmetric graph: label = minimum vertex id in the component
- * (the fixed point of GraphMP/app/cc.cpp's min-label rule). */
+ * (the fixed point of GraphMP/app/cc.cpp's min-label rule).  Synchronous min-label sweeps over
+ * the source strips (st->iterations = sweeps, st->edges = edges gathered), or frontier
+ * relaxation when strips are off; the labels are the same. */
 int gmp_cc(gmp_engine *h, uint32_t *label_out, gmp_stats *st);
 
 /* ---- learned scheduling ---------------------------------------------------------------
