@@ -55,7 +55,7 @@ constexpr uint32_t kNever = 0xFFFFFFFFu;       // strip flag: no source changed 
 // What a sweep over the strips computes.  PageRank sums contributions; the others keep
 // complemented values (c = ~x), so the least label or distance is the greatest c and 0 -- what
 // DPP moves in past a row's start, and the value of an unreached vertex -- is neutral.
-enum SweepMode { kSweepPR = 0, kSweepCC = 1, kSweepUnit = 2, kSweepWeighted = 3 };
+enum SweepMode { kSweepPR = 0, kSweepCC = 1, kSweepUnit = 2, kSweepWeighted = 3, kSweepWeighted8 = 4 };
 
 #define GS64(i, n) \
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
@@ -71,6 +71,7 @@ inline uint32_t bits_for(uint64_t n) {
 }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 struct StripArgs {
   // vertex update (pr_apply)
@@ -101,7 +102,8 @@ struct StripArgs {
   // CC and SSSP sweeps
   uint32_t *nlab;          // complemented labels or distances, updated in place
   uint32_t *changed;       // vertices whose value dropped in the sweep
-  const uint32_t *wt;      // edge weights in layout order (weighted SSSP)
+  const uint32_t *wt;      // edge weights in layout order (weighted SSSP), or
+  const uint8_t *wt8;      // the same as bytes when every weight fits one
   uint32_t *sflag;         // per strip: the last sweep that changed one of its vertices
   uint32_t sweep;          // this sweep's number, from 1
   unsigned long long *edges;  // edges gathered (tasks not skipped)
@@ -311,7 +313,9 @@ __device__ inline void min_round(const StripArgs &a, const uint32_t *sv, MsgBuf 
     const uint32_t c = (cv[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
     uint32_t v = sv[c & 0x7FFFu];
     if (MODE != kSweepCC) {
-      const uint32_t w = MODE == kSweepWeighted ? (k < 4 ? w0[k] : w1[k - 4]) : 1u;
+      const uint32_t w = MODE == kSweepWeighted    ? (k < 4 ? w0[k] : w1[k - 4])
+                         : MODE == kSweepWeighted8 ? (w0[k >> 2] >> (8 * (k & 3))) & 0xFFu
+                                                   : 1u;
       const uint32_t d = ~v;
       const uint32_t t = d > 0xFFFFFFFEu - w ? 0xFFFFFFFEu : d + w;
       v = v ? ~t : 0u;
@@ -355,6 +359,18 @@ __device__ inline void min_round(const StripArgs &a, const uint32_t *sv, MsgBuf 
   B.fill += total;
 }
 
+// Byte weights of round r (8 per lane, one 8-byte load into w0.x, w0.y); zero outside [e0, e1).
+__device__ inline void load_round_w8(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1,
+                                     u32x4 &w0) {
+  const uint32_t q = b0 + r * kRound + 8 * lane_id();
+  w0 = u32x4{0u, 0u, 0u, 0u};
+  if (q < e1 && q + 8 > e0) {
+    const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(a.wt8 + q));
+    w0.x = v.x;
+    w0.y = v.y;
+  }
+}
+
 // Weights of round r of a wave task (8 per lane, two 16-byte loads); zero outside [e0, e1).
 __device__ inline void load_round_w(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1, u32x4 &w0,
                                     u32x4 &w1) {
@@ -383,6 +399,7 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
   for (int i = 0; i < kAhead; i++) {
     buf[i] = load_round(a, b0, i, e0, e1);
     if (MODE == kSweepWeighted) load_round_w(a, b0, i, e0, e1, wa[i], wb[i]);
+    if (MODE == kSweepWeighted8) load_round_w8(a, b0, i, e0, e1, wa[i]);
   }
   for (uint32_t r = 0; r < nr; r += kAhead) {
 #pragma unroll
@@ -395,6 +412,10 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
           w0 = wa[i];
           w1 = wb[i];
           load_round_w(a, b0, r + i + kAhead, e0, e1, wa[i], wb[i]);
+        }
+        if (MODE == kSweepWeighted8) {
+          w0 = wa[i];
+          load_round_w8(a, b0, r + i + kAhead, e0, e1, wa[i]);
         }
         const uint32_t base = b0 + (r + i) * kRound;
         const uint32_t q = base + 8 * lane;
@@ -489,19 +510,28 @@ __device__ inline BinLds<ACC> bin_lds(void *lds, uint32_t R, uint32_t ns) {
   return L;
 }
 
+constexpr uint32_t kRunIdle = 0x80000000u;  // run-table mark: the run's strip was not gathered
+
 // Sets every accumulator to init and builds the run table of bin task bt (all threads of
-// the block; thread tid owns strips [s0, s1)).
-template <class ACC>
-__device__ inline void bin_prepare(const StripArgs &a, const uint4 &bt, const BinLds<ACC> &L, ACC init,
-                                   uint32_t *wtot, uint32_t s0, uint32_t s1) {
+// the block; thread tid owns strips [s0, s1)).  With IDLE, runs of strips the sweep did not
+// gather are marked kRunIdle (their messages were applied when they were made, and applying a
+// maximum again changes nothing); returns the number of messages of the others in the task
+// (every message without IDLE).
+template <bool IDLE, class ACC>
+__device__ inline uint32_t bin_prepare(const StripArgs &a, const uint4 &bt, const BinLds<ACC> &L, ACC init,
+                                       uint32_t *wtot, uint32_t *live, uint32_t s0, uint32_t s1) {
   const uint32_t R = 1u << a.rlog, ns = a.ns, b = bt.x;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wv = wave_id();
   __syncthreads();  // the previous task is done with the LDS
   for (uint32_t i = tid; i < R; i += kIterBlock) L.acc[i] = init;
+  if (IDLE && tid == 0) *live = 0;
   uint32_t sum = 0;
   for (uint32_t s = s0; s < s1; s++) {
     const uint32_t r0 = a.runs[(size_t)b * ns + s], r1 = a.runs[(size_t)(b + 1) * ns + s];
-    L.rbase[s] = r0;
+    // gathered this sweep: flag t - 1; bins that already applied may have moved it to t
+    // (re-applying a strip's earlier messages is harmless), kNever + 1 wraps to 0
+    const bool idle = IDLE && a.sflag[s] + 1u < a.sweep;
+    L.rbase[s] = r0 | (idle ? kRunIdle : 0u);
     L.pre[s] = r1 - r0;
     sum += r1 - r0;
   }
@@ -525,11 +555,20 @@ __device__ inline void bin_prepare(const StripArgs &a, const uint4 &bt, const Bi
   }
   if (tid == 0) L.cidx[nch] = ns - 1;
   __syncthreads();
+  if (!IDLE) return bt.z - bt.y;
+  // messages of gathered strips inside the slice [bt.y, bt.z)
+  uint32_t in = 0;
+  for (uint32_t s = s0; s < s1; s++)
+    if (!(L.rbase[s] & kRunIdle)) in += min(L.pre[s + 1], bt.z) - min(max(L.pre[s], bt.y), min(L.pre[s + 1], bt.z));
+  if (in) atomicAdd(live, in);
+  __syncthreads();
+  return *live;
 }
 
-// Calls add(row, message word) for every message [bt.y, bt.z) of the task, kUnroll messages
-// in flight per thread; the caller synchronises afterwards.
-template <class ACC, class F>
+// Calls add(row, message word) for every message [bt.y, bt.z) of the task (with IDLE, not for
+// messages of runs marked kRunIdle), kUnroll messages in flight per thread; the caller
+// synchronises afterwards.
+template <bool IDLE, class ACC, class F>
 __device__ inline void bin_messages(const StripArgs &a, const uint4 &bt, const BinLds<ACC> &L, F add) {
   const uint32_t *msgw = reinterpret_cast<const uint32_t *>(a.msg);
   for (uint32_t jb = bt.y + threadIdx.x; jb < bt.z; jb += kIterBlock * kUnroll) {
@@ -547,7 +586,7 @@ __device__ inline void bin_messages(const StripArgs &a, const uint4 &bt, const B
           if (L.pre[mid] <= j) lo = mid;
           else hi = mid - 1;
         }
-        m[u] = L.rbase[lo] + (j - L.pre[lo]);
+        m[u] = IDLE && (L.rbase[lo] & kRunIdle) ? 0xFFFFFFFFu : L.rbase[lo] + (j - L.pre[lo]);
       }
     }
     uint32_t x[kUnroll], r[kUnroll];
@@ -566,7 +605,7 @@ __device__ inline void bin_messages(const StripArgs &a, const uint4 &bt, const B
 
 __global__ __launch_bounds__(kIterBlock) void pr_bin_kernel(StripArgs a) {
   extern __shared__ unsigned long long lds2[];
-  __shared__ uint32_t wtot[kIterWaves];
+  __shared__ uint32_t wtot[kIterWaves], live;
   const uint32_t R = 1u << a.rlog, ns = a.ns;
   const BinLds<unsigned long long> L = bin_lds<unsigned long long>(lds2, R, ns);
   const uint32_t tid = threadIdx.x;
@@ -576,8 +615,8 @@ __global__ __launch_bounds__(kIterBlock) void pr_bin_kernel(StripArgs a) {
   PrAcc pacc;
   for (uint32_t t = blockIdx.x; t < a.nbt; t += gridDim.x) {
     const uint4 bt = a.btask[t];
-    bin_prepare(a, bt, L, 0ull, wtot, s0, s1);
-    bin_messages(a, bt, L, [&](uint32_t r, uint32_t x) { atomicAdd(&L.acc[r], fix_from_float(__uint_as_float(x))); });
+    bin_prepare<false>(a, bt, L, 0ull, wtot, &live, s0, s1);
+    bin_messages<false>(a, bt, L, [&](uint32_t r, uint32_t x) { atomicAdd(&L.acc[r], fix_from_float(__uint_as_float(x))); });
     __syncthreads();
     if (bt.w == kNoSlot) {
       const uint32_t v0 = bt.x << a.rlog;
@@ -637,7 +676,7 @@ __device__ inline void min_apply(const StripArgs &a, uint32_t v, uint32_t m, uin
 
 __global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
   extern __shared__ uint32_t lds2u[];
-  __shared__ uint32_t wtot[kIterWaves];
+  __shared__ uint32_t wtot[kIterWaves], live;
   const uint32_t R = 1u << a.rlog, ns = a.ns;
   const BinLds<uint32_t> L = bin_lds<uint32_t>(lds2u, R, ns);
   const uint32_t tid = threadIdx.x;
@@ -646,8 +685,10 @@ __global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
   uint32_t changed = 0;
   for (uint32_t t = blockIdx.x; t < a.nbt; t += gridDim.x) {
     const uint4 bt = a.btask[t];
-    bin_prepare(a, bt, L, 0u, wtot, s0, s1);
-    bin_messages(a, bt, L, [&](uint32_t r, uint32_t x) { atomicMax(&L.acc[r], x); });
+    // a whole bin with no message of a gathered strip cannot change; a split bin's slice still
+    // writes its partial (zeros), which the split apply reads
+    if (bin_prepare<true>(a, bt, L, 0u, wtot, &live, s0, s1) == 0 && bt.w == kNoSlot) continue;
+    bin_messages<true>(a, bt, L, [&](uint32_t r, uint32_t x) { atomicMax(&L.acc[r], x); });
     __syncthreads();
     if (bt.w == kNoSlot) {
       const uint32_t v0 = bt.x << a.rlog, n = min(R, a.V - v0);
@@ -689,6 +730,11 @@ __global__ void min_init_kernel(uint32_t *c, const uint32_t *n2o, uint32_t n, ui
   for (uint32_t s = i0; s < ns; s += stride) sflag[s] = (src == kNever || s == (src >> wlog)) ? 0u : kNever;
   uint32_t *m = reinterpret_cast<uint32_t *>(msg);
   for (uint64_t j = i0; j < nseg; j += stride) m[j] = 0u;
+}
+
+__global__ void weights_to_bytes_kernel(const uint32_t *w, uint64_t n, uint8_t *b) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    b[i] = (uint8_t)w[i];
 }
 
 __global__ void complement_kernel(uint32_t *x, uint32_t n) {
@@ -978,7 +1024,26 @@ void ensure_strip_weights(Engine &e) {
   GMP_HIP(rocprim::radix_sort_pairs(tmp.p, tb, dk, dv, E, 0, sbits + b + wlog, e.stream));
   if (dv.current() != S.wt.as<uint32_t>())
     GMP_HIP(hipMemcpyAsync(S.wt.p, dv.current(), E * 4, hipMemcpyDeviceToDevice, e.stream));
+  // one byte per weight when every weight fits (R-MAT weights are 1..255)
+  DevBuf mx;
+  mx.alloc(4);
+  size_t tr = 0;
+  GMP_HIP(rocprim::reduce(nullptr, tr, S.wt.as<uint32_t>(), mx.as<uint32_t>(), 0u, E, rocprim::maximum<uint32_t>(),
+                          e.stream));
+  tmp.alloc(tr);
+  GMP_HIP(rocprim::reduce(tmp.p, tr, S.wt.as<uint32_t>(), mx.as<uint32_t>(), 0u, E, rocprim::maximum<uint32_t>(),
+                          e.stream));
+  uint32_t wmax = 0;
+  GMP_HIP(hipMemcpyAsync(&wmax, mx.p, 4, hipMemcpyDeviceToHost, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
+  if (wmax <= 0xFFu) {
+    S.wt8.alloc(E + kCodePad);
+    GMP_HIP(hipMemsetAsync(S.wt8.as<uint8_t>() + E, 0, kCodePad, e.stream));
+    weights_to_bytes_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(S.wt.as<uint32_t>(), E, S.wt8.as<uint8_t>());
+    GMP_CHECK_LAUNCH();
+    GMP_HIP(hipStreamSynchronize(e.stream));
+    S.wt.release();
+  }
   S.wt_valid = true;
 }
 
@@ -986,14 +1051,18 @@ void ensure_strip_weights(Engine &e) {
 // the complemented result in work order.
 bool strip_sweeps(Engine &e, int mode, uint32_t src, uint32_t *c, int &sweeps, double &ms, uint64_t &edges) {
   if (!ensure_strips(e)) return false;
-  if (mode == kSweepWeighted) ensure_strip_weights(e);
+  if (mode == kSweepWeighted) {
+    ensure_strip_weights(e);
+    if (e.g.strips.wt8.p) mode = kSweepWeighted8;
+  }
   const StripCsr &S = e.g.strips;
   const uint32_t *n2o = e.g.work_identity ? nullptr : e.g.n2o.as<uint32_t>();  // built with the layout
   const uint32_t V = e.g.V, R = 1u << S.rlog;
   const size_t lds2 = (size_t)4 * R + (2 * (size_t)S.ns + 1) * 4 + ((size_t)(1u << e.opt.bin_task_log) / kChunk + 2) * 4;
-  const void *gk = mode == kSweepCC     ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepCC>)
-                   : mode == kSweepUnit ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepUnit>)
-                                        : reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepWeighted>);
+  const void *gk = mode == kSweepCC          ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepCC>)
+                   : mode == kSweepUnit      ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepUnit>)
+                   : mode == kSweepWeighted8 ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepWeighted8>)
+                                             : reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepWeighted>);
   if (hipFuncSetAttribute(gk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
     (void)hipGetLastError();
   if (hipFuncSetAttribute(reinterpret_cast<const void *>(&min_bin_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1027,6 +1096,7 @@ bool strip_sweeps(Engine &e, int mode, uint32_t src, uint32_t *c, int &sweeps, d
   a.nlab = c;
   a.changed = changed.as<uint32_t>();
   a.wt = S.wt.as<uint32_t>();
+  a.wt8 = S.wt8.as<uint8_t>();
   a.sflag = sflag.as<uint32_t>();
   a.edges = nedges.as<unsigned long long>();
   const uint32_t gi = (uint32_t)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, (uint64_t)e.num_cus * 8);
@@ -1041,6 +1111,8 @@ bool strip_sweeps(Engine &e, int mode, uint32_t src, uint32_t *c, int &sweeps, d
     GMP_HIP(hipMemsetAsync(changed.p, 0, 4, e.stream));
     if (mode == kSweepCC) min_strip_gather_kernel<kSweepCC><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
     else if (mode == kSweepUnit) min_strip_gather_kernel<kSweepUnit><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+    else if (mode == kSweepWeighted8)
+      min_strip_gather_kernel<kSweepWeighted8><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
     else min_strip_gather_kernel<kSweepWeighted><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
     GMP_CHECK_LAUNCH();
     min_bin_kernel<<<S.g2, kIterBlock, lds2, e.stream>>>(a);
