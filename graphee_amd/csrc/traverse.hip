@@ -45,7 +45,6 @@ struct Counters {
   uint32_t pad;
   unsigned long long edges;
   unsigned long long shard_mask;    // OR of 1 << (64 v / V) over vertices appended to the near frontier
-  unsigned long long settled;       // vertices below a threshold (trace exploration)
   uint32_t hist[16];                // far-pile occupancy / degree histograms, [13] = live entries
 };
 static_assert(sizeof(Counters) <= 256, "counters must fit the pinned readback buffer");
@@ -337,15 +336,6 @@ __global__ __launch_bounds__(64) void policy_kernel(const float *P, Counters *cn
       if (logits[c] > logits[best]) best = c;
     cnt->choice = best;
   }
-}
-
-__global__ __launch_bounds__(kBlock) void count_below_kernel(const uint32_t *dist, uint32_t V, uint32_t thr,
-                                                             Counters *cnt) {
-  unsigned long long c = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) c += dist[v] < thr;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) c += (unsigned long long)__shfl_xor((long long)c, o, kWave);
-  if (lane_id() == 0 && c) atomicAdd(&cnt->settled, c);
 }
 
 // Smallest distance among far entries still at or above the old threshold.
@@ -659,14 +649,6 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
     const uint64_t t = (uint64_t)m + (8ull << c);
     return t >= kInf ? kInf : (uint32_t)t;
   };
-  auto count_below = [&](uint32_t t) {
-    GMP_HIP(hipMemsetAsync(&b.cnt->settled, 0, 8, e.stream));
-    count_below_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.dist, V, t, b.cnt);
-    GMP_CHECK_LAUNCH();
-    read_counters(e, b.cnt, h);
-    return h.settled;
-  };
-
   // snapshot of the traversal state for trace exploration
   DevBuf sd, ss, sf, sfar, scnt;
   if (policy == kPolicyExplore) {
