@@ -17,19 +17,20 @@ APP_PAGERANK, APP_SSSP, APP_CC, APP_SSSP_W = 1, 2, 3, 4
 SSSP_BF, SSSP_DELTA, SSSP_LEARNED = 0, 1, 2
 PR_NO_KAHAN, PR_COMPRESSED = 1, 2
 GRAPH_OFF64 = 1
+EDGES_WEIGHTED, EDGES_SYMMETRIZE = 1, 2
 OPT_REORDER, OPT_STREAM_CAP, OPT_ROW_CAP, OPT_CHUNK, OPT_WG_PER_CU, OPT_VEC_MIN = 1, 2, 3, 4, 5, 6
 OPT_STRIP_LOG, OPT_BIN_LOG, OPT_BIN_TASK_LOG = 7, 8, 9
 
 # every entry point include/gmp.h declares
 SYMBOLS = (
     "gmp_last_error", "gmp_abi_version", "gmp_create", "gmp_destroy", "gmp_load_csr",
-    "gmp_load_shards", "gmp_generate_rmat", "gmp_graph_info", "gmp_graph_download",
+    "gmp_load_shards", "gmp_load_edge_list", "gmp_generate_rmat", "gmp_graph_info", "gmp_graph_download",
     "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_sssp", "gmp_cc",
     "gmp_set_option", "gmp_get_option", "gmp_batch_generate", "gmp_batch_info", "gmp_batch_download",
     "gmp_batch_sssp", "gmp_batch_ppr", "gmp_autotune", "gmp_tune_save", "gmp_tune_load",
     "gmp_sssp_trace", "gmp_mlp_init", "gmp_mlp_set", "gmp_mlp_get", "gmp_mlp_forward", "gmp_mlp_loss_grad",
     "gmp_mlp_train",
-    "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards",
+    "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards", "gmp_convert_edge_list",
 )
 
 
@@ -114,6 +115,7 @@ def lib():
         "gmp_destroy": ([_P], ctypes.c_int),
         "gmp_load_csr": ([_P, u32, u64, _P, _P, _P, u32, _P], ctypes.c_int),
         "gmp_load_shards": ([_P, ctypes.c_char_p, u32, u32], ctypes.c_int),
+        "gmp_load_edge_list": ([_P, ctypes.c_char_p, u32, u32, u32], ctypes.c_int),
         "gmp_generate_rmat": ([_P, ctypes.POINTER(RmatParams)], ctypes.c_int),
         "gmp_graph_info": ([_P, _P, _P, _P, _P], ctypes.c_int),
         "gmp_graph_download": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
@@ -144,6 +146,7 @@ def lib():
         "gmp_npy_read_u32": ([ctypes.c_char_p, _P, u64, _P], ctypes.c_int),
         "gmp_npy_write_i32": ([ctypes.c_char_p, _P, u64], ctypes.c_int),
         "gmp_write_shards": ([ctypes.c_char_p, u32, _P, _P, _P, u32, _P, _P], ctypes.c_int),
+        "gmp_convert_edge_list": ([ctypes.c_char_p, ctypes.c_char_p, u32, u32, u32, _P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -198,6 +201,14 @@ def write_shards(dir_, off, idx, bounds, weights=None, vertex_out=None):
                                  bounds.size - 1, _ptr(bounds), _ptr(vo)))
 
 
+def convert_edge_list(path, dir_, V=0, P=1, weighted=False, symmetrize=False):
+    """Text edge list -> GraphMP shard directory (gmp_convert_edge_list, no GPU); returns V."""
+    flags = (EDGES_WEIGHTED if weighted else 0) | (EDGES_SYMMETRIZE if symmetrize else 0)
+    v = ctypes.c_uint32()
+    check(lib().gmp_convert_edge_list(os.fsencode(path), os.fsencode(dir_), V, P, flags, ctypes.byref(v)))
+    return v.value
+
+
 # ---- engine --------------------------------------------------------------------------
 
 class Engine:
@@ -235,6 +246,11 @@ class Engine:
 
     def load_shards(self, dir_, V, P):
         check(lib().gmp_load_shards(self._h, os.fsencode(dir_), V, P))
+
+    def load_edge_list(self, path, V=0, P=1, weighted=False, symmetrize=False):
+        """Text edge list ("src dst [weight]" per line) -> the engine's graph (gmp_load_edge_list)."""
+        flags = (EDGES_WEIGHTED if weighted else 0) | (EDGES_SYMMETRIZE if symmetrize else 0)
+        check(lib().gmp_load_edge_list(self._h, os.fsencode(path), V, P, flags))
 
     def generate_rmat(self, scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, undirected=False,
                       weighted=False, num_shards=1, off64=False):

@@ -169,6 +169,24 @@ int gmp_load_csr(gmp_engine *h, uint32_t V, uint64_t E, const uint64_t *row_off,
   GMP_API_END
 }
 
+int gmp_load_edge_list(gmp_engine *h, const char *path, uint32_t V, uint32_t P, uint32_t flags) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  GMP_HIP(hipSetDevice(e->device));
+  if (!path) fail(GMP_EINVAL, "gmp_load_edge_list: null path");
+  if (V == 0xFFFFFFFFu) fail(GMP_EINVAL, "gmp_load_edge_list: V must be below 2^32 - 1");
+  gmp::EdgeCsr g;
+  gmp::read_edge_list(path, flags, V, g);
+  const uint32_t np = P ? P : 1;
+  std::vector<uint32_t> bounds(np + 1);
+  gmp::partition_bounds(g.raw_indeg.data(), V, np, bounds.data());
+  e->loaded = false;
+  gmp::graph_upload(*e, V, g.off[V], g.off.data(), g.idx.data(), g.w.empty() ? nullptr : g.w.data(), false);
+  gmp::graph_finish(*e, np, bounds.data());
+  e->loaded = true;
+  GMP_API_END
+}
+
 int gmp_load_shards(gmp_engine *h, const char *dir, uint32_t V, uint32_t P) {
   GMP_API_BEGIN
   Engine *e = check(h);

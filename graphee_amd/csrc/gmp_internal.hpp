@@ -260,6 +260,18 @@ void npy_write_i32(const std::string &path, const int32_t *data, uint64_t n);
 void npy_write_u32(const std::string &path, const uint32_t *data, uint64_t n);
 void partition_bounds(const uint64_t *indeg, uint32_t V, uint32_t P, uint32_t *bounds);
 
+// ---- edgelist.cpp ----
+// Destination-row CSR built on the host from a text edge list, with the raw in-edge counts
+// (duplicates counted) that GraphMP's partitioner and shard headers use.
+struct EdgeCsr {
+  std::vector<uint64_t> off;        // V + 1
+  std::vector<uint32_t> idx, w;     // E (w empty without weights)
+  std::vector<uint64_t> raw_indeg;  // V
+};
+// V == 0: set to the largest id + 1.  flags: GMP_EDGES_WEIGHTED, GMP_EDGES_SYMMETRIZE.
+void read_edge_list(const std::string &path, uint32_t flags, uint32_t &V, EdgeCsr &out);
+void write_shard_files(const std::string &dir, uint32_t V, const EdgeCsr &g, uint32_t P, const uint32_t *bounds);
+
 // ---- graph_build.hip ----
 void graph_upload(Engine &e, uint32_t V, uint64_t E, const uint64_t *row_off,
                   const uint32_t *col_idx, const uint32_t *weights, bool force_off64);

@@ -149,6 +149,14 @@ int gmp_load_csr(gmp_engine *h, uint32_t V, uint64_t E, const uint64_t *row_off,
  * dir/{p}.weight.npy (uint32 aligned with indices).  Replaces load_edge
  * (GraphMP/system/Global.h:129-255): the shards are read once and stay in HBM. */
 int gmp_load_shards(gmp_engine *h, const char *dir, uint32_t V, uint32_t P);
+/* Reads a text edge list -- one "src dst [weight]" per line, non-negative integers separated by
+ * spaces or tabs; empty lines and lines starting with '#' or '%' are skipped -- and loads it the
+ * way GraphMP/tools/format_spark.py shards it: destination rows with ascending, deduplicated
+ * sources, self-loops kept (:137-157), and P partitions of equal raw in-edge count, duplicates
+ * counted (:54-108).  V = 0: the largest id + 1.  Duplicate edges keep their least weight. */
+#define GMP_EDGES_WEIGHTED 1u   /* third column: uint32 weight */
+#define GMP_EDGES_SYMMETRIZE 2u /* add the reverse of every edge (CC input) */
+int gmp_load_edge_list(gmp_engine *h, const char *path, uint32_t V, uint32_t P, uint32_t flags);
 /* Builds the R-MAT graph on the device (definition in oracle/rmat_ref.py's docstring). */
 int gmp_generate_rmat(gmp_engine *h, const gmp_rmat_params *p);
 int gmp_graph_info(gmp_engine *h, uint32_t *V, uint64_t *E, uint32_t *P, uint32_t *weighted);
@@ -244,6 +252,12 @@ int gmp_npy_write_i32(const char *path, const int32_t *data, uint64_t n);
 int gmp_write_shards(const char *dir, uint32_t V, const uint64_t *row_off,
                      const uint32_t *col_idx, const uint32_t *weights, uint32_t P,
                      const uint32_t *bounds, const int32_t *vertex_out);
+/* format_spark.py natively: the edge list of gmp_load_edge_list written as GraphMP shards --
+ * dir/{p}.edge.npy with the header [raw edge lines, nnz, rows + 1, start, end], dir/{p}.weight.npy
+ * with GMP_EDGES_WEIGHTED, dir/vertexout.npy (out-degrees of the deduplicated graph).  *V_out
+ * (may be NULL) receives V.  No GPU is used. */
+int gmp_convert_edge_list(const char *path, const char *dir, uint32_t V, uint32_t P, uint32_t flags,
+                          uint32_t *V_out);
 
 #ifdef __cplusplus
 }

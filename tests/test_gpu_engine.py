@@ -471,3 +471,33 @@ def test_argument_errors(engine):
         with pytest.raises(gmp.GmpError) as ei:
             fresh.pagerank()
         assert ei.value.code == gmp.ESTATE
+
+
+def test_load_edge_list_matches_the_converter(engine, tmp_path):
+    """gmp_load_edge_list builds on the device the graph gmp_convert_edge_list writes as shards
+    (same CSR, weights and partition), and the apps run on it."""
+    rng = np.random.default_rng(8)
+    V, E = 5000, 60000
+    src, dst, w = rng.integers(0, V, E), rng.integers(0, V, E), rng.integers(1, 200, E)
+    p = tmp_path / "g.tsv"
+    p.write_text("# edges\n" + "".join(f"{a} {b} {c}\n" for a, b, c in zip(src, dst, w)))
+    d = tmp_path / "shards"
+    d.mkdir()
+    assert gmp.convert_edge_list(p, d, V=V, P=4, weighted=True) == V
+    engine.load_edge_list(p, V=V, P=4, weighted=True)
+    g = engine.download(weights=True)
+    engine.load_shards(d, V, 4)
+    h = engine.download(weights=True)
+    for k in ("off", "idx", "w", "bounds", "outdeg"):
+        np.testing.assert_array_equal(g[k], h[k], err_msg=k)
+    off, idx = graphs.csr_from_edges(V, src, dst)
+    np.testing.assert_array_equal(g["off"], off)
+    np.testing.assert_array_equal(g["idx"], idx)
+    s = engine.pick_source(3)
+    d_, _ = engine.sssp(s, method=gmp.SSSP_BF)
+    ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+    np.testing.assert_array_equal(d_, oracle.dijkstra(ooff, oidx, ow, s))
+    engine.load_edge_list(p, V=V, P=2, symmetrize=True)
+    gs = engine.download()
+    lab, _ = engine.cc()
+    np.testing.assert_array_equal(lab, oracle.cc_unionfind(gs["off"], gs["idx"]))
