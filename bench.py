@@ -154,6 +154,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--no-extra", action="store_true", help="skip the SSSP and CC measurements")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-instance runs")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-schedule comparison")
     ap.add_argument("--learn-scale", type=int, default=27, help="R-MAT scale of the learned-SSSP evaluation")
     ap.add_argument("--strip-log", type=int, default=15,
                     help="PageRank source strips of 2^k ids (0: the row gather over the in-CSR)")
@@ -357,8 +359,9 @@ def main():
                 f"{stc.seconds * 1e3 / st.ms_total:.1f}x; identical: {np.array_equal(labc, lab)}")
             del g, labc
 
-    if not args.no_extra:
+    if not args.no_extra and not args.no_batched:
         result["extra"]["batched"] = run_batched(eng)
+    if not args.no_extra and not args.no_learned:
         result["extra"]["learned_sssp"] = run_learned(eng, args.learn_scale)
 
     eng.close()
