@@ -20,6 +20,7 @@ GRAPH_OFF64 = 1
 EDGES_WEIGHTED, EDGES_SYMMETRIZE = 1, 2
 OPT_REORDER, OPT_STREAM_CAP, OPT_ROW_CAP, OPT_CHUNK, OPT_WG_PER_CU, OPT_VEC_MIN = 1, 2, 3, 4, 5, 6
 OPT_STRIP_LOG, OPT_BIN_LOG, OPT_BIN_TASK_LOG = 7, 8, 9
+OPT_EDGE_BUDGET_MB, OPT_WINDOW_MB, OPT_WINDOW_PACKED = 10, 11, 12
 
 # every entry point include/gmp.h declares
 SYMBOLS = (
@@ -29,7 +30,7 @@ SYMBOLS = (
     "gmp_set_option", "gmp_get_option", "gmp_batch_generate", "gmp_batch_info", "gmp_batch_download",
     "gmp_batch_sssp", "gmp_batch_ppr", "gmp_autotune", "gmp_tune_save", "gmp_tune_load",
     "gmp_sssp_trace", "gmp_mlp_init", "gmp_mlp_set", "gmp_mlp_get", "gmp_mlp_forward", "gmp_mlp_loss_grad",
-    "gmp_mlp_train",
+    "gmp_mlp_train", "gmp_windows",
     "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards", "gmp_convert_edge_list",
 )
 
@@ -83,6 +84,16 @@ TRACE_DTYPE = np.dtype([("feat", np.float32, (16,)), ("best", np.int32), ("epoch
                         ("shard_mask", np.uint64), ("ms", np.float32, (8,)), ("relaxed", np.float32, (8,))])
 
 
+class WindowStats(ctypes.Structure):
+    _fields_ = [("windows", ctypes.c_uint32), ("cached", ctypes.c_uint32), ("ring_slots", ctypes.c_uint32),
+                ("packed", ctypes.c_uint32), ("host_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("streamed_bytes", ctypes.c_uint64), ("windows_run", ctypes.c_uint64),
+                ("windows_skipped", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class TuneResult(ctypes.Structure):
     _fields_ = [("stream_cap", ctypes.c_uint32), ("row_cap", ctypes.c_uint32), ("chunk", ctypes.c_uint32),
                 ("vec_min", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32), ("trials", ctypes.c_uint32),
@@ -118,6 +129,7 @@ def lib():
         "gmp_load_edge_list": ([_P, ctypes.c_char_p, u32, u32, u32], ctypes.c_int),
         "gmp_generate_rmat": ([_P, ctypes.POINTER(RmatParams)], ctypes.c_int),
         "gmp_graph_info": ([_P, _P, _P, _P, _P], ctypes.c_int),
+        "gmp_windows": ([_P, _P], ctypes.c_int),
         "gmp_graph_download": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
         "gmp_pick_source": ([_P, u64, u32, _P], ctypes.c_int),
         "gmp_set_option": ([_P, i32, i64], ctypes.c_int),
@@ -273,6 +285,12 @@ class Engine:
         od = np.zeros(V, dtype=np.uint32)
         check(lib().gmp_graph_download(self._h, _ptr(off), _ptr(idx), _ptr(w), _ptr(bounds), _ptr(od)))
         return {"off": off.astype(np.int64), "idx": idx, "w": w, "bounds": bounds, "outdeg": od}
+
+    def windows(self):
+        """Residency of the loaded graph (windows == 0: resident) and the window counters."""
+        r = WindowStats()
+        check(lib().gmp_windows(self._h, ctypes.byref(r)))
+        return r.as_dict()
 
     def set_option(self, key, value):
         check(lib().gmp_set_option(self._h, key, int(value)))

@@ -167,6 +167,7 @@ int gmp_autotune(gmp_engine *h, int32_t iters, gmp_tune_result *out) {
   if (!h) gmp::fail(GMP_EINVAL, "null engine handle");
   auto *e = reinterpret_cast<gmp::Engine *>(h);
   if (!e->loaded) gmp::fail(GMP_ESTATE, "no graph loaded");
+  if (e->g.semi) gmp::fail(GMP_EUNSUPPORTED, "gmp_autotune: the graph is semi-external");
   GMP_HIP(hipSetDevice(e->device));
   gmp::autotune(*e, iters, out);
   GMP_API_END

@@ -124,6 +124,18 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       if ((uint32_t)value != e->opt.bin_task_log) e->g.strips.valid = false;
       e->opt.bin_task_log = (uint32_t)value;
       break;
+    case GMP_OPT_EDGE_BUDGET_MB:
+      in(0, 1ll << 30, "edge_budget_mb");
+      e->opt.edge_budget_mb = (uint64_t)value;
+      break;
+    case GMP_OPT_WINDOW_MB:
+      in(1, 4096, "window_mb");
+      e->opt.window_mb = (uint32_t)value;
+      break;
+    case GMP_OPT_WINDOW_PACKED:
+      in(0, 1, "window_packed");
+      e->opt.window_packed = (int)value;
+      break;
     default:
       fail(GMP_EINVAL, "gmp_set_option: unknown key " + std::to_string(key));
   }
@@ -144,6 +156,9 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_STRIP_LOG: *value = e->opt.strip_log; break;
     case GMP_OPT_BIN_LOG: *value = e->opt.bin_log; break;
     case GMP_OPT_BIN_TASK_LOG: *value = e->opt.bin_task_log; break;
+    case GMP_OPT_EDGE_BUDGET_MB: *value = (int64_t)e->opt.edge_budget_mb; break;
+    case GMP_OPT_WINDOW_MB: *value = e->opt.window_mb; break;
+    case GMP_OPT_WINDOW_PACKED: *value = e->opt.window_packed; break;
     default: fail(GMP_EINVAL, "gmp_get_option: unknown key " + std::to_string(key));
   }
   GMP_API_END
@@ -249,6 +264,16 @@ int gmp_generate_rmat(gmp_engine *h, const gmp_rmat_params *p) {
   e->loaded = false;
   gmp::graph_generate_rmat(*e, *p);
   gmp::graph_finish(*e, p->num_shards, nullptr);
+  gmp::Graph &g = e->g;
+  if (gmp::semi_wanted(*e, g.E, g.weighted)) {
+    // generated on the device, then moved out to host memory like a loaded semi-external graph
+    std::vector<uint32_t> idx(g.E), w(g.weighted ? g.E : 0);
+    if (g.E) GMP_HIP(hipMemcpy(idx.data(), g.in.idx.p, g.E * 4, hipMemcpyDeviceToHost));
+    if (g.weighted && g.E) GMP_HIP(hipMemcpy(w.data(), g.in.w.p, g.E * 4, hipMemcpyDeviceToHost));
+    g.in.idx.release();
+    g.in.w.release();
+    gmp::semi_setup(*e, idx.data(), g.weighted ? w.data() : nullptr);
+  }
   e->loaded = true;
   GMP_API_END
 }
@@ -260,6 +285,14 @@ int gmp_graph_info(gmp_engine *h, uint32_t *V, uint64_t *E, uint32_t *P, uint32_
   if (E) *E = e->g.E;
   if (P) *P = e->g.P;
   if (weighted) *weighted = e->g.weighted ? 1u : 0u;
+  GMP_API_END
+}
+
+int gmp_windows(gmp_engine *h, gmp_window_stats *out) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  if (!out) fail(GMP_EINVAL, "gmp_windows: out is null");
+  gmp::semi_info(*e, out);
   GMP_API_END
 }
 
@@ -320,6 +353,7 @@ int gmp_sssp_trace(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, gmp
   GMP_API_BEGIN
   Engine *e = check_loaded(h);
   if (!nrows || (cap && !rows)) fail(GMP_EINVAL, "gmp_sssp_trace: null argument");
+  if (e->g.semi) fail(GMP_EUNSUPPORTED, "gmp_sssp_trace: the graph is semi-external");
   gmp_sssp_params d{GMP_SSSP_LEARNED, 0, 0, 0};
   gmp::run_sssp_trace(*e, source, p ? *p : d, rows, cap, nrows, dist_out, st);
   GMP_API_END

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -136,11 +137,60 @@ struct HubTable {
   bool valid = false;
 };
 
+// Device view of a varint-packed column stream (format in varint.hpp).
+struct Packed {
+  const uint8_t *bytes;   // padded by >= 16 bytes
+  const uint64_t *gbyte;  // G+1 byte offsets
+  const uint64_t *gmask;  // G restart masks
+};
+
 // Delta + varint column ids of the work in-CSR (decoded inside the gather).
 struct PackedCols {
   DevBuf bytes, gbyte, gmask;
   uint64_t nbytes = 0, ngroups = 0;
   bool valid = false;
+};
+
+// Semi-external graphs (window.hip): the column ids (varint-packed or plain) and weights stay in
+// pinned host memory; windows of consecutive destination rows are copied into a ring of device
+// slots on a copy stream while the compute stream works on the window before, and the windows
+// that fit into the device edge budget stay cached in device memory.  This is GraphMP's sliding
+// window over shards (GraphPS.h:256-386) with its compressed edge cache (Global.h:129-255).
+struct EdgeWindow {
+  uint32_t r0 = 0, r1 = 0;  // destination rows
+  uint64_t e0 = 0, e1 = 0;  // edges
+  uint64_t b0 = 0, b1 = 0;  // bytes of the host id stream the window reads
+  int64_t cache = -1;       // byte offset in the device cache, -1: streamed through the ring
+  uint32_t task0 = 0, ntasks = 0, hub0 = 0, nhubs = 0, multi0 = 0, nmulti = 0, nparts = 0;
+};
+struct SemiExternal {
+  bool packed = false;
+  uint32_t blk_log = 0, nblk = 0, blk_words = 0;  // activation: source blocks of 2^blk_log ids
+  uint64_t slot_bytes = 0, max_edges = 0;         // id bytes of one ring slot, edges of the largest window
+  uint32_t nslots = 0;
+  uint64_t host_bytes = 0, cache_bytes = 0, budget = 0;
+  std::vector<EdgeWindow> win;
+  uint8_t *h_ids = nullptr;  // pinned: packed bytes (+ padding) or uint32 ids
+  uint32_t *h_w = nullptr;   // pinned: weights
+  DevBuf gbyte, gmask;       // group index of the packed stream
+  DevBuf ring, wring;        // nslots slots of ids and of weights
+  DevBuf cache, wcache;      // cached windows (ids at EdgeWindow::cache, weights at 4*(e0 - first cached edge))
+  uint64_t wcache_e0 = 0;
+  DevBuf plain;              // one window's decoded ids
+  DevBuf tasks, hubs, multi; // gather plans of all windows, rows relative to the window
+  uint32_t plan_key[4] = {0, 0, 0, 0};
+  uint32_t max_parts = 0;
+  DevBuf srcblk;             // uint64[nwin * blk_words]: source blocks of every window
+  DevBuf stamp;              // uint32[nblk]: last sweep that changed a vertex of the block
+  DevBuf active;             // uint8[nwin]
+  hipStream_t copy = nullptr;
+  std::vector<hipEvent_t> ready, freed;
+  uint64_t slot_seq = 0;
+  uint64_t streamed_bytes = 0, windows_run = 0, windows_skipped = 0;  // since the load
+  SemiExternal() = default;
+  SemiExternal(const SemiExternal &) = delete;
+  SemiExternal &operator=(const SemiExternal &) = delete;
+  ~SemiExternal();
 };
 
 struct Graph {
@@ -170,6 +220,7 @@ struct Graph {
   PackedCols packed;                // compressed work in-CSR columns (built on demand)
   StripCsr strips;                  // source-strip layout (built on demand)
   HubTable hubs;                    // hub slices of the work out-CSR (built on demand)
+  std::unique_ptr<SemiExternal> semi;  // set: the edges live in host memory (window.hip)
 
   CsrView view(const Csr &c) const { return CsrView{c.off.p, c.idx.as<uint32_t>(), c.w.as<uint32_t>()}; }
   CsrView work_in() const { return work_identity ? view(in) : view(win); }
@@ -193,6 +244,9 @@ struct Options {
   uint32_t strip_log = 15;     // PageRank source strips of 2^strip_log ids, 8..15 (0: row gather)
   uint32_t bin_log = 14;       // PageRank destination bins of up to 2^bin_log rows, 8..14
   uint32_t bin_task_log = 20;  // PageRank bins with more than 2^k messages are split, 8..22
+  uint64_t edge_budget_mb = 0; // > 0: graphs whose edge data exceed it load semi-externally
+  uint32_t window_mb = 64;     // semi-external windows: id bytes per window
+  int window_packed = 1;       // semi-external windows: varint-packed ids
 };
 
 // Many graphs resident at once as one disjoint union (graph k owns union ids
@@ -289,6 +343,42 @@ void transpose_csr(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool o
 void count_degrees(Engine &e, const uint32_t *idx, uint64_t E, uint32_t V, uint32_t *deg);
 void graph_download(Engine &e, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights,
                     uint32_t *out_degree);
+
+// ---- window.hip ----
+// True when a graph of E edges (weighted or not) must load semi-externally under the options.
+bool semi_wanted(const Engine &e, uint64_t E, bool weighted);
+// Takes the host column ids (and weights) of the loaded graph, whose offsets and out-degrees are
+// already on the device, into pinned host memory and builds the windows.
+void semi_setup(Engine &e, const uint32_t *idx, const uint32_t *w);
+// Device pointers of one window: ids (plain, indexed by global edge), pk (packed, indexed by the
+// global group index), w (weights, indexed by global edge; null when not requested).
+struct WindowDev {
+  const uint32_t *idx = nullptr;
+  Packed pk{};
+  const uint32_t *w = nullptr;
+  int slot = -1;
+};
+// Makes window `w` available to work enqueued next on e.stream: a cached window at once, a
+// streamed one by a copy on the copy stream that e.stream waits for.  plain: ids decoded to
+// uint32 (always available for plain streams; packed ones are decoded on e.stream).
+WindowDev window_begin(Engine &e, uint32_t w, bool weights, bool plain);
+// Releases the ring slot of window_begin after the work enqueued so far on e.stream.
+void window_end(Engine &e, const WindowDev &d);
+// Gather plans of the windows for the current plan options.
+void window_plans(Engine &e);
+// Activation: flags[w] = 1 when a source block of window w holds a vertex changed in `sweep`;
+// returns the flags on the host (e.stream synchronised).
+const uint8_t *window_activity(Engine &e, uint32_t sweep);
+void window_stamp_init(Engine &e, uint32_t value, int64_t only_vertex);
+void semi_cc(Engine &e, uint32_t *label_out, gmp_stats *st);
+void semi_sssp(Engine &e, uint32_t src, bool unit, uint32_t *dist_out, gmp_stats *st);
+void semi_download(Engine &e, uint32_t *col_idx, uint32_t *weights);
+void semi_info(Engine &e, gmp_window_stats *out);
+// Host varint packing in the format of varint.hpp (compress.hip builds the same on the device).
+void pack_host(const uint64_t *off, uint32_t V, const uint32_t *idx, uint64_t E, std::vector<uint64_t> &gbyte,
+               std::vector<uint64_t> &gmask, uint8_t *bytes, uint64_t &nbytes, bool measure_only);
+void unpack_host(const uint8_t *bytes, const uint64_t *gbyte, const uint64_t *gmask, uint64_t E, uint64_t e0,
+                 uint64_t e1, uint32_t *out);
 
 // ---- plan.cpp (host) ----
 void build_gather_plan_host(const uint64_t *off, uint32_t V, GatherPlan &plan,

@@ -260,6 +260,14 @@ void graph_upload(Engine &e, uint32_t V, uint64_t E, const uint64_t *row_off, co
     GMP_HIP(hipMemcpyAsync(g.in.off.p, o32.data(), (size_t)(V + 1) * 4, hipMemcpyHostToDevice, e.stream));
     GMP_HIP(hipStreamSynchronize(e.stream));
   }
+  g.h_in_off.assign(row_off, row_off + V + 1);
+  if (semi_wanted(e, E, weights != nullptr)) {
+    // the edges stay in pinned host memory; windows stream them (window.hip)
+    GMP_HIP(hipStreamSynchronize(e.stream));
+    g.in.valid = true;
+    semi_setup(e, col_idx, weights);
+    return;
+  }
   g.in.idx.alloc(std::max<uint64_t>(E, 1) * 4);
   if (E) GMP_HIP(hipMemcpyAsync(g.in.idx.p, col_idx, E * 4, hipMemcpyHostToDevice, e.stream));
   if (weights) {
@@ -267,7 +275,6 @@ void graph_upload(Engine &e, uint32_t V, uint64_t E, const uint64_t *row_off, co
     if (E) GMP_HIP(hipMemcpyAsync(g.in.w.p, weights, E * 4, hipMemcpyHostToDevice, e.stream));
   }
   g.in.valid = true;
-  g.h_in_off.assign(row_off, row_off + V + 1);
   GMP_HIP(hipStreamSynchronize(e.stream));
 }
 
@@ -338,9 +345,11 @@ void graph_generate_rmat(Engine &e, const gmp_rmat_params &p) {
 
 void graph_finish(Engine &e, uint32_t P, const uint32_t *bounds) {
   Graph &g = e.g;
-  g.out_deg.alloc((size_t)std::max<uint32_t>(g.V, 1) * 4);
-  GMP_HIP(hipMemsetAsync(g.out_deg.p, 0, (size_t)g.V * 4, e.stream));
-  if (g.E) {
+  if (!g.semi) {  // semi-external graphs counted their out-degrees while setting up
+    g.out_deg.alloc((size_t)std::max<uint32_t>(g.V, 1) * 4);
+    GMP_HIP(hipMemsetAsync(g.out_deg.p, 0, (size_t)g.V * 4, e.stream));
+  }
+  if (g.E && !g.semi) {
     out_degree_kernel<<<grid_for(g.E), kBlock, 0, e.stream>>>(g.in.idx.as<uint32_t>(), g.E, g.out_deg.as<uint32_t>());
     GMP_CHECK_LAUNCH();
   }
@@ -369,7 +378,7 @@ void graph_build_out(Engine &e) {
 void ensure_work(Engine &e) {
   Graph &g = e.g;
   if (g.work_ready) return;
-  g.work_identity = !e.opt.reorder;
+  g.work_identity = !e.opt.reorder || g.semi != nullptr;
   g.wout_ready = false;
   g.packed.valid = false;
   g.strips.valid = false;
@@ -470,6 +479,12 @@ uint32_t work_id(Engine &e, uint32_t original) {
 void graph_download(Engine &e, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights, uint32_t *out_degree) {
   Graph &g = e.g;
   if (row_off) std::memcpy(row_off, g.h_in_off.data(), (size_t)(g.V + 1) * 8);
+  if (g.semi) {
+    if (weights && !g.weighted) fail(GMP_EUNSUPPORTED, "graph has no weights");
+    semi_download(e, col_idx, weights);
+    if (out_degree && g.V) GMP_HIP(hipMemcpy(out_degree, g.out_deg.p, (size_t)g.V * 4, hipMemcpyDeviceToHost));
+    return;
+  }
   if (col_idx && g.E) GMP_HIP(hipMemcpy(col_idx, g.in.idx.p, g.E * 4, hipMemcpyDeviceToHost));
   if (weights) {
     if (!g.weighted) fail(GMP_EUNSUPPORTED, "graph has no weights");

@@ -245,12 +245,16 @@ __global__ __launch_bounds__(kFinalBlock) void pr_finalize_kernel(const double *
 }
 
 // ---- GraphMP-compatible synchronous step (one thread per destination) ----
+// Rows [v0, v1); with stamp set, the block of every row whose message changes is stamped with
+// `sweep` (window activation of semi-external graphs).
 template <class OffT, int APP>
 __global__ __launch_bounds__(kBlock) void compat_step_kernel(const OffT *off, const uint32_t *idx,
-                                                             const uint32_t *w, const int32_t *vout, uint32_t V,
-                                                             double invN, const void *msg_v, void *msgnew_v,
-                                                             void *value_v, unsigned long long *changed) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+                                                             const uint32_t *w, const int32_t *vout, uint32_t v0,
+                                                             uint32_t v1, double invN, const void *msg_v,
+                                                             void *msgnew_v, void *value_v,
+                                                             unsigned long long *changed, uint32_t *stamp,
+                                                             uint32_t blk_log, uint32_t sweep) {
+  for (uint32_t v = v0 + blockIdx.x * blockDim.x + threadIdx.x; v < v1; v += gridDim.x * blockDim.x) {
     const uint64_t k0 = off[v], k1 = off[v + 1];
     bool ch;
     if (APP == GMP_APP_PAGERANK) {
@@ -275,6 +279,7 @@ __global__ __launch_bounds__(kBlock) void compat_step_kernel(const OffT *off, co
         if (best > t) best = t;
       }
       ((int32_t *)msgnew_v)[v] = best;
+      if (stamp && best != msg[v]) stamp[v >> blk_log] = sweep;
       int32_t *val = (int32_t *)value_v;
       ch = best != val[v];
       if (ch) val[v] = best;
@@ -286,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void compat_step_kernel(const OffT *off, co
         if (t < (uint64_t)best) best = (uint32_t)t;
       }
       ((uint32_t *)msgnew_v)[v] = best;
+      if (stamp && best != msg[v]) stamp[v >> blk_log] = sweep;
       uint32_t *val = (uint32_t *)value_v;
       ch = best != val[v];
       if (ch) val[v] = best;
@@ -295,37 +301,228 @@ __global__ __launch_bounds__(kBlock) void compat_step_kernel(const OffT *off, co
   }
 }
 
+struct CompatRows {
+  const uint32_t *idx, *w;  // ids and weights by global edge
+  uint32_t v0, v1;
+  uint32_t *stamp;
+  uint32_t blk_log, sweep;
+};
+
 template <class OffT>
 void launch_compat(Engine &e, int app, uint32_t grid, double invN, const void *msg, void *msgnew,
-                   void *value, const int32_t *vout, unsigned long long *changed) {
-  Graph &g = e.g;
-  const OffT *off = g.in.off.as<OffT>();
-  const uint32_t *idx = g.in.idx.as<uint32_t>();
-  const uint32_t *w = g.in.w.as<uint32_t>();
+                   void *value, const int32_t *vout, unsigned long long *changed, const CompatRows &r) {
+  const OffT *off = e.g.in.off.as<OffT>();
+#define GMP_COMPAT(APP)                                                                                      \
+  compat_step_kernel<OffT, APP><<<grid, kBlock, 0, e.stream>>>(off, r.idx, r.w, vout, r.v0, r.v1, invN, msg, \
+                                                               msgnew, value, changed, r.stamp, r.blk_log,  \
+                                                               r.sweep)
   switch (app) {
-    case GMP_APP_PAGERANK:
-      compat_step_kernel<OffT, GMP_APP_PAGERANK><<<grid, kBlock, 0, e.stream>>>(off, idx, w, vout, g.V, invN, msg,
-                                                                                msgnew, value, changed);
-      break;
-    case GMP_APP_SSSP:
-      compat_step_kernel<OffT, GMP_APP_SSSP><<<grid, kBlock, 0, e.stream>>>(off, idx, w, vout, g.V, invN, msg, msgnew,
-                                                                            value, changed);
-      break;
-    case GMP_APP_CC:
-      compat_step_kernel<OffT, GMP_APP_CC><<<grid, kBlock, 0, e.stream>>>(off, idx, w, vout, g.V, invN, msg, msgnew,
-                                                                          value, changed);
-      break;
-    default:
-      compat_step_kernel<OffT, GMP_APP_SSSP_W><<<grid, kBlock, 0, e.stream>>>(off, idx, w, vout, g.V, invN, msg,
-                                                                              msgnew, value, changed);
-      break;
+    case GMP_APP_PAGERANK: GMP_COMPAT(GMP_APP_PAGERANK); break;
+    case GMP_APP_SSSP: GMP_COMPAT(GMP_APP_SSSP); break;
+    case GMP_APP_CC: GMP_COMPAT(GMP_APP_CC); break;
+    default: GMP_COMPAT(GMP_APP_SSSP_W); break;
   }
+#undef GMP_COMPAT
   GMP_CHECK_LAUNCH();
+}
+
+// One synchronous step of a compat app over the windows of a semi-external graph.  The min
+// apps skip a window none of whose source blocks changed in the step before (the stamps of
+// step - 1): its rows would recompute their current messages, which are copied instead.
+template <class OffT>
+void compat_windows_step(Engine &e, int app, int step, uint32_t grid_cap, double invN, const void *msg, void *msgnew,
+                         void *value, const int32_t *vout, unsigned long long *changed) {
+  SemiExternal &s = *e.g.semi;
+  const bool minapp = app != GMP_APP_PAGERANK;
+  const uint32_t nwin = (uint32_t)s.win.size();
+  const uint8_t *act = nullptr;
+  if (minapp && step > 0) act = window_activity(e, (uint32_t)step);
+  for (uint32_t w = 0; w < nwin; w++) {
+    const EdgeWindow &x = s.win[w];
+    if (act && !act[w]) {
+      GMP_HIP(hipMemcpyAsync(static_cast<uint32_t *>(msgnew) + x.r0, static_cast<const uint32_t *>(msg) + x.r0,
+                             (size_t)(x.r1 - x.r0) * 4, hipMemcpyDeviceToDevice, e.stream));
+      s.windows_skipped++;
+      continue;
+    }
+    const WindowDev d = window_begin(e, w, app == GMP_APP_SSSP_W, true);
+    CompatRows r{d.idx, d.w, x.r0, x.r1, minapp ? s.stamp.as<uint32_t>() : nullptr, s.blk_log, (uint32_t)step + 1};
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((x.r1 - x.r0 + kBlock - 1) / kBlock, grid_cap));
+    launch_compat<OffT>(e, app, grid, invN, msg, msgnew, value, vout, changed, r);
+    window_end(e, d);
+  }
+}
+
+// PageRank over the windows of a semi-external graph: per iteration every window's gather
+// plan runs on its rows (row ids relative to the window, the row arrays shifted to its first
+// row, ids addressed by global edge), fed from the device cache or a ring slot; packed windows
+// are decoded inside the gather.  The gathers give the same per-row sums as the resident row
+// gather (same tasks per row, same lane per edge); the FP64 partials are summed over all windows
+// in a fixed order.
+template <class OffT>
+void pagerank_windows(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st) {
+  Graph &g = e.g;
+  SemiExternal &s = *g.semi;
+  window_plans(e);
+  const uint32_t V = g.V, nwin = (uint32_t)s.win.size();
+  const bool personalised = p.seed_vertex >= 0;
+  const uint32_t seed = personalised ? (uint32_t)p.seed_vertex : 0xFFFFFFFFu;
+  const uint32_t G = std::max<uint32_t>(1, (uint32_t)e.num_cus * e.opt.wg_per_cu);
+  const int max_iter = std::max(0, p.max_iter);
+  const bool comp = s.packed, kahan = !(p.flags & GMP_PR_NO_KAHAN);
+  std::vector<uint32_t> g1(nwin), g2(nwin);
+  uint32_t n1 = 0, n2 = 0;
+  for (uint32_t w = 0; w < nwin; w++) {
+    const EdgeWindow &x = s.win[w];
+    g1[w] = x.ntasks ? std::min<uint32_t>(G, (x.ntasks + kWavesPerBlock - 1) / kWavesPerBlock) : 0u;
+    g2[w] = x.nmulti ? std::min<uint32_t>(G, (x.nmulti + kWavesPerBlock - 1) / kWavesPerBlock) : 0u;
+    n1 += g1[w];
+    n2 += g2[w];
+  }
+  const uint32_t np1 = std::max(n1, G), np2 = std::max<uint32_t>(n2, 1);
+  e.s_a.alloc((size_t)V * 4);
+  e.s_b.alloc((size_t)V * 4);
+  e.s_c.alloc((size_t)V * 4);
+  e.s_d.alloc((size_t)V * 4);
+  e.s_e.alloc(std::max<size_t>(s.max_parts, 1) * sizeof(float2));
+  e.s_part.alloc(((size_t)np1 + np2) * 2 * sizeof(double));
+  e.s_scal.alloc(sizeof(PrScal) + (size_t)(max_iter + 1) * 2 * sizeof(double));
+  float *rank[2] = {e.s_a.as<float>(), e.s_b.as<float>()};
+  float *contrib[2] = {e.s_c.as<float>(), e.s_d.as<float>()};
+  double *pr1 = e.s_part.as<double>(), *pd1 = pr1 + np1;
+  double *pr2 = pd1 + np1, *pd2 = pr2 + np2;
+  PrScal *sc = e.s_scal.as<PrScal>();
+  double *hist = reinterpret_cast<double *>(e.s_scal.as<char>() + sizeof(PrScal));
+  const uint32_t *outdeg = g.out_deg.as<uint32_t>();
+  const OffT *off = g.in.off.as<OffT>();
+  const uint32_t *tasks = s.tasks.as<uint32_t>(), *multi = s.multi.as<uint32_t>();
+  const HubRow *hubs = s.hubs.as<HubRow>();
+
+  std::vector<hipEvent_t> ev(2 * (size_t)max_iter + 2);
+  for (auto &x : ev) GMP_HIP(hipEventCreate(&x));
+  struct EvGuard {
+    std::vector<hipEvent_t> &v;
+    ~EvGuard() {
+      for (auto x : v) (void)hipEventDestroy(x);
+    }
+  } evg{ev};
+
+  GMP_HIP(hipEventRecord(ev[0], e.stream));
+  pr_init_kernel<<<G, kBlock, 0, e.stream>>>(V, seed, outdeg, rank[0], contrib[0], pr1, pd1);
+  GMP_CHECK_LAUNCH();
+  pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, G, pr2, pd2, 0, V, p.damping, personalised, sc,
+                                                      nullptr, -1);
+  GMP_CHECK_LAUNCH();
+  const size_t lds = (size_t)kWavesPerBlock * (g.plan.stream_cap + (comp ? kVarintScratchWords : 0u)) * sizeof(float);
+  int cur = 0, it = 0;
+  double resid = 0, dang = 0;
+  for (it = 0; it < max_iter; it++) {
+    GMP_HIP(hipEventRecord(ev[2 + 2 * it], e.stream));
+    uint32_t o1 = 0, o2 = 0;
+    for (uint32_t w = 0; w < nwin; w++) {
+      const EdgeWindow &x = s.win[w];
+      const WindowDev d = window_begin(e, w, false, false);
+      PrArgs<OffT> a{};
+      a.off = off + x.r0;
+      a.idx = d.idx;
+      a.pk = d.pk;
+      a.E = g.E;
+      a.outdeg = outdeg + x.r0;
+      a.contrib_in = contrib[cur];
+      a.rank_in = rank[cur] + x.r0;
+      a.rank_out = rank[cur ^ 1] + x.r0;
+      a.contrib_out = contrib[cur ^ 1] + x.r0;
+      a.tasks = tasks + 2ull * x.task0;
+      a.ntasks = x.ntasks;
+      a.hubs = hubs + x.hub0;
+      a.multi = multi + x.multi0;
+      a.nmulti = x.nmulti;
+      a.hub_part = e.s_e.as<float2>();
+      a.sc = sc;
+      a.d = p.damping;
+      a.seed = seed - x.r0;  // matches no row of the window when the seed lies outside it
+      a.stream_cap = g.plan.stream_cap;
+      a.chunk = g.plan.chunk;
+      if (g1[w]) {
+        a.part_resid = pr1 + o1;
+        a.part_dang = pd1 + o1;
+        if (comp) {
+          if (kahan) pr_gather_kernel<OffT, true, true><<<g1[w], kBlock, lds, e.stream>>>(a);
+          else pr_gather_kernel<OffT, false, true><<<g1[w], kBlock, lds, e.stream>>>(a);
+        } else {
+          if (kahan) pr_gather_kernel<OffT, true, false><<<g1[w], kBlock, lds, e.stream>>>(a);
+          else pr_gather_kernel<OffT, false, false><<<g1[w], kBlock, lds, e.stream>>>(a);
+        }
+        GMP_CHECK_LAUNCH();
+        o1 += g1[w];
+      }
+      if (g2[w]) {
+        a.part_resid = pr2 + o2;
+        a.part_dang = pd2 + o2;
+        pr_hub_combine_kernel<OffT><<<g2[w], kBlock, 0, e.stream>>>(a);
+        GMP_CHECK_LAUNCH();
+        o2 += g2[w];
+      }
+      window_end(e, d);
+    }
+    GMP_HIP(hipEventRecord(ev[3 + 2 * it], e.stream));
+    pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, n1, pr2, pd2, n2, V, p.damping, personalised, sc,
+                                                        hist, it);
+    GMP_CHECK_LAUNCH();
+    cur ^= 1;
+    if (p.tol > 0) {
+      double h[2];
+      GMP_HIP(hipMemcpyAsync(h, hist + 2 * it, sizeof h, hipMemcpyDeviceToHost, e.stream));
+      GMP_HIP(hipStreamSynchronize(e.stream));
+      resid = h[0];
+      if (resid < p.tol) {
+        it++;
+        break;
+      }
+    }
+  }
+  GMP_HIP(hipEventRecord(ev[1], e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  double kms = 0;
+  for (int i = 0; i < it; i++) {
+    float t = 0;
+    GMP_HIP(hipEventElapsedTime(&t, ev[2 + 2 * i], ev[3 + 2 * i]));
+    kms += t;
+  }
+  float tot = 0;
+  GMP_HIP(hipEventElapsedTime(&tot, ev[0], ev[1]));
+  if (it > 0) {
+    double h[2];
+    GMP_HIP(hipMemcpy(h, hist + 2 * (it - 1), sizeof h, hipMemcpyDeviceToHost));
+    resid = h[0];
+    dang = h[1];
+  }
+  if (rank_out) GMP_HIP(hipMemcpy(rank_out, rank[cur], (size_t)V * 4, hipMemcpyDeviceToHost));
+  if (st) {
+    st->iterations = it;
+    st->launches = it;
+    st->ms_total = tot;
+    st->ms_kernel = kms;
+    st->edges = g.E * (uint64_t)it;
+    st->vertices = (uint64_t)V * it;
+    st->residual = resid;
+    st->dangling = dang;
+    st->changed = 0;
+    const uint64_t o = g.off64 ? 8 : 4;
+    st->bytes = (uint64_t)it * (8 * g.E + (o + 16) * (uint64_t)V);
+    uint64_t ib = 0;
+    for (const auto &x : s.win) ib += x.b1 - x.b0;
+    st->index_bytes = ib;
+  }
 }
 
 template <class OffT>
 void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st) {
   Graph &g = e.g;
+  if (g.semi) {
+    pagerank_windows<OffT>(e, p, rank_out, st);
+    return;
+  }
   ensure_plan(e);
   GatherPlan &pl = g.plan;
   const uint32_t V = g.V;
@@ -501,6 +698,7 @@ void compat_impl(Engine &e, int app, int max_iter, const void *msg_init, const i
   }
   GMP_HIP(hipMemcpyAsync(msg[0], msg_init, (size_t)V * 4, hipMemcpyHostToDevice, e.stream));
   GMP_HIP(hipMemsetAsync(value, 0, (size_t)V * 4, e.stream));
+  if (g.semi) window_stamp_init(e, 0u, -1);
   unsigned long long *changed = e.s_scal.as<unsigned long long>();
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((V + kBlock - 1) / kBlock, e.num_cus * 16));
   const double invN = 1.0 / (double)V;
@@ -510,7 +708,12 @@ void compat_impl(Engine &e, int app, int max_iter, const void *msg_init, const i
   unsigned long long nch = 0;
   for (step = 0; step < max_iter; step++) {
     GMP_HIP(hipMemsetAsync(changed, 0, 8, e.stream));
-    launch_compat<OffT>(e, app, grid, invN, msg[cur], msg[cur ^ 1], value, dvout, changed);
+    if (g.semi) {
+      compat_windows_step<OffT>(e, app, step, grid, invN, msg[cur], msg[cur ^ 1], value, dvout, changed);
+    } else {
+      const CompatRows all{g.in.idx.as<uint32_t>(), g.in.w.as<uint32_t>(), 0u, V, nullptr, 0u, 0u};
+      launch_compat<OffT>(e, app, grid, invN, msg[cur], msg[cur ^ 1], value, dvout, changed, all);
+    }
     GMP_HIP(hipMemcpyAsync(&nch, changed, 8, hipMemcpyDeviceToHost, e.stream));
     GMP_HIP(hipStreamSynchronize(e.stream));
     cur ^= 1;

@@ -859,6 +859,11 @@ void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
 
 void run_sssp(Engine &e, uint32_t src, const gmp_sssp_params &p, uint32_t *dist_out, gmp_stats *st) {
   if (src >= e.g.V) fail(GMP_EINVAL, "source vertex out of range");
+  if (p.method < GMP_SSSP_BF || p.method > GMP_SSSP_LEARNED) fail(GMP_EINVAL, "unknown SSSP method");
+  if (e.g.semi) {  // every method gives the same distances; windows run Bellman-Ford sweeps
+    semi_sssp(e, src, p.unit_weights != 0, dist_out, st);
+    return;
+  }
   int policy;
   switch (p.method) {
     case GMP_SSSP_BF: policy = kPolicyBF; break;
@@ -878,6 +883,10 @@ void run_sssp_trace(Engine &e, uint32_t src, const gmp_sssp_params &p, gmp_trace
 }
 
 void run_cc(Engine &e, uint32_t *label_out, gmp_stats *st) {
+  if (e.g.semi) {
+    semi_cc(e, label_out, st);
+    return;
+  }
   if (e.g.off64) cc_impl<uint64_t>(e, label_out, st);
   else cc_impl<uint32_t>(e, label_out, st);
 }

@@ -19,6 +19,7 @@
 
 #include <cstdint>
 
+#include "gmp_internal.hpp"
 #include "wave.hpp"
 
 namespace gmp {
@@ -31,12 +32,6 @@ constexpr uint32_t kVarintScratchWords = kVarintStageWords + kVarintGroup;  // s
 __host__ __device__ inline uint32_t varint_len(uint32_t v) {
   return v < (1u << 7) ? 1u : v < (1u << 14) ? 2u : v < (1u << 21) ? 3u : v < (1u << 28) ? 4u : 5u;
 }
-
-struct Packed {
-  const uint8_t *bytes;   // padded by >= 16 bytes
-  const uint64_t *gbyte;  // G+1 byte offsets
-  const uint64_t *gmask;  // G restart masks
-};
 
 // Returns the source id of edge 64*g + lane (meaningless for lanes beyond the last edge).
 // `scratch` is this wave's kVarintScratchWords words of LDS.  All 64 lanes must call it.

@@ -118,6 +118,18 @@ int gmp_destroy(gmp_engine *h);
                                 gather, frontier CC and Bellman-Ford */
 #define GMP_OPT_BIN_LOG 8    /* PageRank strips: destination bins of up to 2^k rows, 8..14 (default 14) */
 #define GMP_OPT_BIN_TASK_LOG 9 /* PageRank strips: messages per bin task 2^k, 8..22 (default 20) */
+/* Semi-external graphs, GraphMP's sliding window (GraphMP/system/GraphPS.h:256-386) with its
+ * compressed edge cache (GraphMP/system/Global.h:129-255).  Read when a graph is loaded. */
+#define GMP_OPT_EDGE_BUDGET_MB 10 /* 0 (default): graphs load whole into device memory.  n > 0: a
+                                     graph whose column ids and weights take more than n MiB loads
+                                     semi-externally -- they stay in pinned host memory, windows
+                                     of destination rows stream through a ring of device slots
+                                     and the windows that fit into n MiB stay cached on the
+                                     device; gmp_pagerank, gmp_run_app, gmp_sssp and gmp_cc run
+                                     over the windows, the other runs return GMP_EUNSUPPORTED */
+#define GMP_OPT_WINDOW_MB 11      /* semi-external: id bytes per window, 1..4096 (default 64) */
+#define GMP_OPT_WINDOW_PACKED 12  /* semi-external: 1 (default) windows hold delta + varint packed
+                                     ids (decoded in the gather), 0 plain uint32 ids */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 
@@ -162,6 +174,22 @@ int gmp_generate_rmat(gmp_engine *h, const gmp_rmat_params *p);
 int gmp_graph_info(gmp_engine *h, uint32_t *V, uint64_t *E, uint32_t *P, uint32_t *weighted);
 /* Copies the device graph back; any output may be NULL.  row_off: V+1, col_idx/weights: E,
  * shard_bounds: P+1, out_degree: V. */
+/* Residency of the loaded graph.  windows == 0: resident in device memory.  Counters run from
+ * the load: bytes copied from host to device for windows, windows gathered, windows skipped
+ * because no source of theirs changed (GraphMP's bloom-filter activation, GraphPS.h:318-351,
+ * here exact at the granularity of source blocks). */
+typedef struct gmp_window_stats {
+  uint32_t windows;        /* windows of consecutive destination rows (0: resident graph) */
+  uint32_t cached;         /* windows kept in device memory */
+  uint32_t ring_slots;     /* device slots the other windows stream through */
+  uint32_t packed;         /* 1: windows hold varint-packed ids */
+  uint64_t host_bytes;     /* edge bytes in pinned host memory (ids + weights) */
+  uint64_t device_bytes;   /* device bytes for edges: cache + ring */
+  uint64_t streamed_bytes; /* host-to-device bytes copied for windows */
+  uint64_t windows_run;
+  uint64_t windows_skipped;
+} gmp_window_stats;
+int gmp_windows(gmp_engine *h, gmp_window_stats *out);
 int gmp_graph_download(gmp_engine *h, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights,
                        uint32_t *shard_bounds, uint32_t *out_degree);
 /* Seeded source pick (definition in oracle/rmat_ref.py): first vertex with out-degree > 0. */

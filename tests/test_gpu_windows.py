@@ -1,0 +1,223 @@
+"""Semi-external graphs -- edges in pinned host memory, windows of destination rows streamed
+through device slots (GraphMP's sliding window, GraphMP/system/GraphPS.h:256-386, with its
+compressed edge cache, Global.h:129-255) -- against the resident engine and the oracle.
+
+Every app must give what it gives on the resident graph: the same PageRank per-row sums
+(the L1 bound and near-total bit equality; the FP64 partials group differently), the same
+distances and labels, and the compat apps bit for bit, iteration count and changed count
+included, even where window activation skips windows."""
+import numpy as np
+import pytest
+
+import graphee_amd as gmp
+import oracle
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _engine(budget_mb=0, window_mb=1, packed=True):
+    e = gmp.Engine(0)
+    if budget_mb:
+        e.set_option(gmp.OPT_EDGE_BUDGET_MB, budget_mb)
+        e.set_option(gmp.OPT_WINDOW_MB, window_mb)
+        e.set_option(gmp.OPT_WINDOW_PACKED, int(packed))
+    return e
+
+
+# (edge budget MiB, window MiB, packed): nothing cached / some windows cached, both streams
+CONFIGS = [(1, 1, True), (1, 1, False), (10, 1, True), (10, 1, False)]
+
+
+@pytest.mark.parametrize("budget,window,packed", CONFIGS)
+def test_windows_pagerank_matches_resident(budget, window, packed):
+    say(f"semi-external pagerank budget={budget} window={window} packed={packed}")
+    with _engine() as r, _engine(budget, window, packed) as s:
+        for e in (r, s):
+            e.generate_rmat(18, 16, 77, num_shards=4)
+        # the resident row gather over the loaded order sums every row like the windows do
+        r.set_option(gmp.OPT_REORDER, 0)
+        r.set_option(gmp.OPT_STRIP_LOG, 0)
+        assert r.windows()["windows"] == 0
+        w0 = s.windows()
+        assert w0["windows"] >= 4 and w0["packed"] == int(packed)
+        if budget == 1:
+            assert w0["cached"] == 0
+        else:
+            assert 0 < w0["cached"] < w0["windows"]
+        g, h = r.download(), s.download()
+        np.testing.assert_array_equal(g["off"], h["off"])
+        np.testing.assert_array_equal(g["idx"], h["idx"])
+        np.testing.assert_array_equal(g["outdeg"], h["outdeg"])
+        x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
+        for seed in (-1, int(np.argmax(g["outdeg"])), g["outdeg"].size - 1):
+            for kahan in (True, False):
+                a, _ = r.pagerank(max_iter=20, seed_vertex=seed, kahan=kahan)
+                b, st = s.pagerank(max_iter=20, seed_vertex=seed, kahan=kahan)
+                assert st.iterations == 20
+                d = np.abs(a.astype(np.float64) - b)
+                assert d.sum() <= 1e-6, (seed, kahan, d.sum())
+                assert np.mean(a == b) > 0.999, (seed, kahan)
+                if seed < 0 and kahan:
+                    assert np.abs(b - x).sum() < 1e-6
+        w1 = s.windows()
+        uncached = w1["windows"] - w1["cached"]
+        assert w1["windows_run"] - w0["windows_run"] == 6 * 20 * w1["windows"]
+        assert (w1["streamed_bytes"] > w0["streamed_bytes"]) == (uncached > 0)
+
+
+def test_windows_pagerank_stops_on_tolerance():
+    with _engine() as r, _engine(1, 1, True) as s:
+        for e in (r, s):
+            e.generate_rmat(17, 16, 5)
+        r.set_option(gmp.OPT_REORDER, 0)
+        r.set_option(gmp.OPT_STRIP_LOG, 0)
+        a, sa = r.pagerank(max_iter=200, tol=1e-7)
+        b, sb = s.pagerank(max_iter=200, tol=1e-7)
+        assert sa.iterations == sb.iterations < 200
+        assert np.abs(a.astype(np.float64) - b).sum() <= 1e-6
+
+
+@pytest.mark.parametrize("budget,window,packed", CONFIGS)
+def test_windows_cc_and_sssp_match_resident(budget, window, packed):
+    say(f"semi-external cc/sssp budget={budget} window={window} packed={packed}")
+    with _engine() as r, _engine(budget, window, packed) as s:
+        for e in (r, s):
+            e.generate_rmat(17, 16, 31, undirected=True)
+        assert s.windows()["windows"] >= 4
+        a, _ = r.cc()
+        b, st = s.cc()
+        np.testing.assert_array_equal(a, b)
+        assert st.iterations >= 2
+        for e in (r, s):
+            e.generate_rmat(17, 16, 32, weighted=True)
+        assert s.windows()["windows"] >= 4
+        src = r.pick_source(32)
+        assert s.pick_source(32) == src
+        g = r.download(weights=True)
+        ooff, oidx, ow = graphs.transpose(g["outdeg"].size, g["off"], g["idx"], g["w"])
+        want = oracle.dijkstra(ooff, oidx, ow, src)
+        for method in (gmp.SSSP_BF, gmp.SSSP_DELTA):
+            d, _ = s.sssp(src, method=method)
+            np.testing.assert_array_equal(d, want)
+        du, _ = r.sssp(src, unit_weights=True)
+        dv, _ = s.sssp(src, unit_weights=True)
+        np.testing.assert_array_equal(du, dv)
+
+
+def _local_graph(V, chain):
+    """Pairs in the first half of the ids, chains of `chain` vertices (edges i -> i+1) in the
+    second, a self-loop on every vertex: the sources of a window are its own rows, so windows
+    of the finished first half go idle while the chains still propagate."""
+    src, dst = [np.arange(V)], [np.arange(V)]
+    h = V // 2
+    a = np.arange(0, h, 2)
+    src += [a, a + 1]
+    dst += [a + 1, a]
+    b = np.arange(h, V)
+    b = b[(b - h) % chain != chain - 1]
+    src.append(b)
+    dst.append(b + 1)
+    return graphs.csr_from_edges(V, np.concatenate(src), np.concatenate(dst))
+
+
+@pytest.mark.parametrize("packed", [False, True])
+def test_windows_skip_idle_windows_exactly(packed):
+    V = 1 << 20
+    off, idx = _local_graph(V, 32)
+    w = graphs.edge_weights(off, idx, 9)
+    with _engine() as r, _engine(1, 1, packed) as s:
+        for e in (r, s):
+            e.load_csr(off, idx, weights=w)
+        w0 = s.windows()
+        assert w0["windows"] >= 3
+        a, _ = r.cc()
+        b, st = s.cc()
+        np.testing.assert_array_equal(a, b)
+        w1 = s.windows()
+        assert w1["windows_skipped"] > w0["windows_skipped"]
+        assert w1["windows_run"] - w0["windows_run"] < st.iterations * w0["windows"]
+        d0, _ = r.sssp(V // 2 + 3)
+        d1, _ = s.sssp(V // 2 + 3)
+        np.testing.assert_array_equal(d0, d1)
+        assert s.windows()["windows_skipped"] > w1["windows_skipped"]
+        # the compat CC app: activation must leave messages, values and the stop step unchanged
+        msg0 = np.arange(V, dtype=np.int32)
+        m0, v0, s0 = r.run_app(gmp.APP_CC, msg0, 1000)
+        w2 = s.windows()
+        m1, v1, s1 = s.run_app(gmp.APP_CC, msg0, 1000)
+        np.testing.assert_array_equal(m0, m1)
+        np.testing.assert_array_equal(v0, v1)
+        assert s0.iterations == s1.iterations and s0.changed == s1.changed
+        assert s.windows()["windows_skipped"] > w2["windows_skipped"]
+
+
+@pytest.mark.parametrize("app", [gmp.APP_PAGERANK, gmp.APP_SSSP, gmp.APP_CC, gmp.APP_SSSP_W])
+@pytest.mark.parametrize("packed", [False, True])
+def test_windows_compat_apps_bit_exact(app, packed):
+    say(f"semi-external compat app {app} packed={packed}")
+    with _engine(1, 1, packed) as s:
+        s.generate_rmat(16, 16, 12, weighted=True, num_shards=3)
+        assert s.windows()["windows"] >= 2
+        g = s.download(weights=True)
+        V = g["outdeg"].size
+        vout = None
+        if app == gmp.APP_PAGERANK:
+            msg0 = np.full(V, 1.0 / V, dtype=np.float32)
+            vout = g["outdeg"].astype(np.int32)
+            cap = 60
+        elif app == gmp.APP_CC:
+            msg0 = np.arange(V, dtype=np.int32)
+            cap = 1000
+        elif app == gmp.APP_SSSP:
+            msg0 = np.full(V, 10000, dtype=np.int32)  # sssp.cpp's "infinity"
+            msg0[s.pick_source(12)] = 0
+            cap = 1000
+        else:
+            msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+            msg0[s.pick_source(12)] = 0
+            cap = 1000
+        msg, val, st = s.run_app(app, msg0, cap, vertex_out=vout)
+        want, wval, it, changed = oracle.app_sync(app, g["off"], g["idx"], msg0, cap, w=g["w"], vout=vout)
+        np.testing.assert_array_equal(msg.view(np.uint32), want.view(np.uint32))
+        np.testing.assert_array_equal(val.view(np.uint32), wval.view(np.uint32))
+        assert st.iterations == it and st.changed == changed[-1]
+
+
+def test_windows_load_paths_and_unsupported_runs(tmp_path):
+    rng = np.random.default_rng(3)
+    V, E = 50000, 600000
+    off, idx = graphs.csr_from_edges(V, rng.integers(0, V, E), rng.integers(0, V, E))
+    lines = []
+    for v in range(V):
+        for k in range(off[v], off[v + 1]):
+            lines.append(f"{idx[k]} {v}\n")
+    p = tmp_path / "g.tsv"
+    p.write_text("".join(lines))
+    d = tmp_path / "shards"
+    d.mkdir()
+    gmp.convert_edge_list(p, d, V=V, P=5)
+    with _engine(1, 1, True) as s:
+        for load in (lambda: s.load_csr(off, idx, P=5), lambda: s.load_edge_list(p, V=V, P=5),
+                     lambda: s.load_shards(d, V, 5)):
+            load()
+            w = s.windows()
+            assert w["windows"] >= 2 and w["host_bytes"] > 0
+            h = s.download()
+            np.testing.assert_array_equal(h["off"], off)
+            np.testing.assert_array_equal(h["idx"], idx)
+            np.testing.assert_array_equal(h["outdeg"], graphs.out_degree(V, idx))
+        with pytest.raises(gmp.GmpError) as ei:
+            s.sssp_trace(0)
+        assert ei.value.code == gmp.EUNSUPPORTED
+        with pytest.raises(gmp.GmpError) as ei:
+            s.autotune(1)
+        assert ei.value.code == gmp.EUNSUPPORTED
+    # within the budget the graph stays resident
+    with _engine(64, 1, True) as s:
+        s.load_csr(off, idx)
+        assert s.windows()["windows"] == 0
