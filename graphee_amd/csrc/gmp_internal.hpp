@@ -153,7 +153,7 @@ struct PackedCols {
 
 // Semi-external graphs (window.hip): the column ids (varint-packed or plain) and weights stay in
 // pinned host memory; windows of consecutive destination rows are copied into a ring of device
-// slots on a copy stream while the compute stream works on the window before, and the windows
+// slots on copy streams while the compute stream works on the windows before, and the windows
 // that fit into the device edge budget stay cached in device memory.  This is GraphMP's sliding
 // window over shards (GraphPS.h:256-386) with its compressed edge cache (Global.h:129-255).
 struct EdgeWindow {
@@ -183,7 +183,7 @@ struct SemiExternal {
   DevBuf srcblk;             // uint64[nwin * blk_words]: source blocks of every window
   DevBuf stamp;              // uint32[nblk]: last sweep that changed a vertex of the block
   DevBuf active;             // uint8[nwin]
-  hipStream_t copy = nullptr;
+  std::vector<hipStream_t> copy;  // copy streams of the ring slots
   std::vector<hipEvent_t> ready, freed;
   uint64_t slot_seq = 0;
   uint64_t streamed_bytes = 0, windows_run = 0, windows_skipped = 0;  // since the load

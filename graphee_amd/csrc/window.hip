@@ -33,7 +33,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kInf = 0xFFFFFFFFu;
-constexpr uint32_t kRingSlots = 3;
+constexpr uint32_t kRingSlots = 4;
+constexpr uint32_t kCopyStreams = 2;              // slot k copies on stream k % kCopyStreams
 constexpr uint64_t kPackPad = 64;                 // zero bytes after the packed stream
 constexpr uint64_t kBitmapBytes = 64ull << 20;    // all windows' source-block bitmaps together
 constexpr uint64_t kStageEdges = 32ull << 20;     // ids per chunk when counting out-degrees
@@ -315,10 +316,10 @@ void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st) {
 }  // namespace
 
 SemiExternal::~SemiExternal() {
-  if (copy) (void)hipStreamSynchronize(copy);
+  for (auto c : copy) (void)hipStreamSynchronize(c);
   for (auto x : ready) (void)hipEventDestroy(x);
   for (auto x : freed) (void)hipEventDestroy(x);
-  if (copy) (void)hipStreamDestroy(copy);
+  for (auto c : copy) (void)hipStreamDestroy(c);
   if (h_ids) (void)hipHostFree(h_ids);
   if (h_w) (void)hipHostFree(h_w);
 }
@@ -527,7 +528,8 @@ void semi_setup(Engine &e, const uint32_t *idx, const uint32_t *w) {
     s.stamp.alloc((uint64_t)std::max<uint32_t>(s.nblk, 1) * 4);
     s.active.alloc(std::max<uint32_t>(nwin, 1));
   }
-  GMP_HIP(hipStreamCreateWithFlags(&s.copy, hipStreamNonBlocking));
+  s.copy.resize(kCopyStreams);
+  for (auto &c : s.copy) GMP_HIP(hipStreamCreateWithFlags(&c, hipStreamNonBlocking));
   s.ready.resize(std::max<uint32_t>(s.nslots, 1));
   s.freed.resize(std::max<uint32_t>(s.nslots, 1));
   for (auto &x : s.ready) GMP_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
@@ -596,16 +598,17 @@ WindowDev window_begin(Engine &e, uint32_t wi, bool weights, bool plain) {
   } else {
     const uint32_t k = (uint32_t)(s.slot_seq++ % s.nslots);
     uint8_t *dst = s.ring.as<uint8_t>() + (uint64_t)k * s.slot_bytes;
-    GMP_HIP(hipStreamWaitEvent(s.copy, s.freed[k], 0));
-    if (x.b1 > x.b0) GMP_HIP(hipMemcpyAsync(dst, s.h_ids + x.b0, x.b1 - x.b0, hipMemcpyHostToDevice, s.copy));
+    hipStream_t cs = s.copy[k % s.copy.size()];
+    GMP_HIP(hipStreamWaitEvent(cs, s.freed[k], 0));
+    if (x.b1 > x.b0) GMP_HIP(hipMemcpyAsync(dst, s.h_ids + x.b0, x.b1 - x.b0, hipMemcpyHostToDevice, cs));
     s.streamed_bytes += x.b1 - x.b0;
     if (need_w) {
       uint32_t *wd = s.wring.as<uint32_t>() + (uint64_t)k * s.max_edges;
-      if (x.e1 > x.e0) GMP_HIP(hipMemcpyAsync(wd, s.h_w + x.e0, (x.e1 - x.e0) * 4, hipMemcpyHostToDevice, s.copy));
+      if (x.e1 > x.e0) GMP_HIP(hipMemcpyAsync(wd, s.h_w + x.e0, (x.e1 - x.e0) * 4, hipMemcpyHostToDevice, cs));
       s.streamed_bytes += (x.e1 - x.e0) * 4;
       wts = wd;
     }
-    GMP_HIP(hipEventRecord(s.ready[k], s.copy));
+    GMP_HIP(hipEventRecord(s.ready[k], cs));
     GMP_HIP(hipStreamWaitEvent(e.stream, s.ready[k], 0));
     ids = dst;
     d.slot = (int)k;
