@@ -510,13 +510,13 @@ __device__ inline BinLds<ACC> bin_lds(void *lds, uint32_t R, uint32_t ns) {
   return L;
 }
 
-constexpr uint32_t kRunIdle = 0x80000000u;  // run-table mark: the run's strip was not gathered
-
-// Sets every accumulator to init and builds the run table of bin task bt (all threads of
-// the block; thread tid owns strips [s0, s1)).  With IDLE, runs of strips the sweep did not
-// gather are marked kRunIdle (their messages were applied when they were made, and applying a
-// maximum again changes nothing); returns the number of messages of the others in the task
-// (every message without IDLE).
+// Sets every accumulator to init and builds the run table of bin task bt (all threads of the
+// block; thread tid owns strips [s0, s1)).  The task is the slice [bt.y, bt.z) of the bin's
+// messages in strip order; the table lists, per strip, where its part of the slice starts in
+// msg (rbase) and where it starts among the task's live messages (pre): with IDLE, strips the
+// sweep did not gather contribute nothing (their messages were applied when they were made,
+// and applying a maximum again changes nothing), so later loops visit live messages only.
+// Returns the number of live messages (the same in every thread).
 template <bool IDLE, class ACC>
 __device__ inline uint32_t bin_prepare(const StripArgs &a, const uint4 &bt, const BinLds<ACC> &L, ACC init,
                                        uint32_t *wtot, uint32_t *live, uint32_t s0, uint32_t s1) {
@@ -524,69 +524,83 @@ __device__ inline uint32_t bin_prepare(const StripArgs &a, const uint4 &bt, cons
   const uint32_t tid = threadIdx.x, lane = lane_id(), wv = wave_id();
   __syncthreads();  // the previous task is done with the LDS
   for (uint32_t i = tid; i < R; i += kIterBlock) L.acc[i] = init;
-  if (IDLE && tid == 0) *live = 0;
+  // run lengths and their prefix over the bin: the coordinates of bt.y and bt.z
   uint32_t sum = 0;
   for (uint32_t s = s0; s < s1; s++) {
     const uint32_t r0 = a.runs[(size_t)b * ns + s], r1 = a.runs[(size_t)(b + 1) * ns + s];
-    // gathered this sweep: flag t - 1; bins that already applied may have moved it to t
-    // (re-applying a strip's earlier messages is harmless), kNever + 1 wraps to 0
-    const bool idle = IDLE && a.sflag[s] + 1u < a.sweep;
-    L.rbase[s] = r0 | (idle ? kRunIdle : 0u);
+    L.rbase[s] = r0;
     L.pre[s] = r1 - r0;
     sum += r1 - r0;
   }
-  const uint32_t incl = wave_inclusive_scan(sum);
+  uint32_t incl = wave_inclusive_scan(sum);
   if (lane == kWave - 1) wtot[wv] = incl;
   __syncthreads();
   uint32_t run = incl - sum;
+  for (uint32_t w = 0; w < wv; w++) run += wtot[w];
+  // each strip's live part of the slice
+  uint32_t sum2 = 0;
+  for (uint32_t s = s0; s < s1; s++) {
+    const uint32_t p0 = run, p1 = run + L.pre[s];
+    run = p1;
+    const uint32_t q0 = max(p0, bt.y), q1 = min(p1, bt.z);
+    // gathered this sweep: flag t - 1; bins that already applied may have moved it to t
+    // (re-applying a strip's earlier messages is harmless), kNever + 1 wraps to 0
+    const bool idle = IDLE && a.sflag[s] + 1u < a.sweep;
+    const uint32_t len = (q1 > q0 && !idle) ? q1 - q0 : 0u;
+    L.rbase[s] += len ? q0 - p0 : 0u;
+    L.pre[s] = len;
+    sum2 += len;
+  }
+  __syncthreads();  // every thread has read wtot
+  incl = wave_inclusive_scan(sum2);
+  if (lane == kWave - 1) wtot[wv] = incl;
+  __syncthreads();
+  run = incl - sum2;
   for (uint32_t w = 0; w < wv; w++) run += wtot[w];
   for (uint32_t s = s0; s < s1; s++) {
     const uint32_t l = L.pre[s];
     L.pre[s] = run;
     run += l;
   }
-  if (tid == kIterBlock - 1) L.pre[ns] = run;
+  if (tid == kIterBlock - 1) {
+    L.pre[ns] = run;
+    *live = run;
+  }
   __syncthreads();
-  // chunk index of the slice: every kChunk-th message's run bounds the searches below
-  const uint32_t nch = (bt.z - bt.y + kChunk - 1) / kChunk;
+  const uint32_t total = *live;
+  // chunk index: the strip holding every kChunk-th live message bounds the searches below
+  const uint32_t nch = (total + kChunk - 1) / kChunk;
   for (uint32_t s = s0; s < s1; s++) {
-    const uint32_t p0 = max(L.pre[s], bt.y), p1 = min(L.pre[s + 1], bt.z);
-    for (uint32_t c = (p0 - bt.y + kChunk - 1) / kChunk; p0 < p1 && bt.y + c * kChunk < p1; c++) L.cidx[c] = s;
+    const uint32_t p0 = L.pre[s], p1 = L.pre[s + 1];
+    for (uint32_t c = (p0 + kChunk - 1) / kChunk; c * kChunk < p1; c++) L.cidx[c] = s;
   }
   if (tid == 0) L.cidx[nch] = ns - 1;
   __syncthreads();
-  if (!IDLE) return bt.z - bt.y;
-  // messages of gathered strips inside the slice [bt.y, bt.z)
-  uint32_t in = 0;
-  for (uint32_t s = s0; s < s1; s++)
-    if (!(L.rbase[s] & kRunIdle)) in += min(L.pre[s + 1], bt.z) - min(max(L.pre[s], bt.y), min(L.pre[s + 1], bt.z));
-  if (in) atomicAdd(live, in);
-  __syncthreads();
-  return *live;
+  return total;
 }
 
-// Calls add(row, message word) for every message [bt.y, bt.z) of the task (with IDLE, not for
-// messages of runs marked kRunIdle), kUnroll messages in flight per thread; the caller
-// synchronises afterwards.
-template <bool IDLE, class ACC, class F>
-__device__ inline void bin_messages(const StripArgs &a, const uint4 &bt, const BinLds<ACC> &L, F add) {
+// Calls add(row, message word) for the n live messages of the task (bin_prepare's count),
+// kUnroll messages in flight per thread; the caller synchronises afterwards.
+template <class ACC, class F>
+__device__ inline void bin_messages(const StripArgs &a, uint32_t n, const BinLds<ACC> &L, F add) {
   const uint32_t *msgw = reinterpret_cast<const uint32_t *>(a.msg);
-  for (uint32_t jb = bt.y + threadIdx.x; jb < bt.z; jb += kIterBlock * kUnroll) {
+  for (uint32_t jb = threadIdx.x; jb < n; jb += kIterBlock * kUnroll) {
     uint32_t m[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const uint32_t j = jb + u * kIterBlock;
       m[u] = 0xFFFFFFFFu;
-      if (j < bt.z) {
-        // the run holding message j: the last r with pre[r] <= j (never an empty run)
-        const uint32_t c = (j - bt.y) / kChunk;
+      if (j < n) {
+        // the strip holding live message j: the last s with pre[s] <= j (strips with no live
+        // message share the next one's pre and are never the last)
+        const uint32_t c = j / kChunk;
         uint32_t lo = L.cidx[c], hi = L.cidx[c + 1];
         while (lo < hi) {
           const uint32_t mid = (lo + hi + 1) >> 1;
           if (L.pre[mid] <= j) lo = mid;
           else hi = mid - 1;
         }
-        m[u] = IDLE && (L.rbase[lo] & kRunIdle) ? 0xFFFFFFFFu : L.rbase[lo] + (j - L.pre[lo]);
+        m[u] = L.rbase[lo] + (j - L.pre[lo]);
       }
     }
     uint32_t x[kUnroll], r[kUnroll];
@@ -615,8 +629,8 @@ __global__ __launch_bounds__(kIterBlock) void pr_bin_kernel(StripArgs a) {
   PrAcc pacc;
   for (uint32_t t = blockIdx.x; t < a.nbt; t += gridDim.x) {
     const uint4 bt = a.btask[t];
-    bin_prepare<false>(a, bt, L, 0ull, wtot, &live, s0, s1);
-    bin_messages<false>(a, bt, L, [&](uint32_t r, uint32_t x) { atomicAdd(&L.acc[r], fix_from_float(__uint_as_float(x))); });
+    const uint32_t n = bin_prepare<false>(a, bt, L, 0ull, wtot, &live, s0, s1);
+    bin_messages(a, n, L, [&](uint32_t r, uint32_t x) { atomicAdd(&L.acc[r], fix_from_float(__uint_as_float(x))); });
     __syncthreads();
     if (bt.w == kNoSlot) {
       const uint32_t v0 = bt.x << a.rlog;
@@ -674,9 +688,25 @@ __device__ inline void min_apply(const StripArgs &a, uint32_t v, uint32_t m, uin
   }
 }
 
+// Messages of strips gathered this sweep in the whole bin b (the same for every thread): reads
+// the strip flags and the run bounds of active strips only, so a bin that receives nothing skips
+// the run table and the LDS accumulators of bin_prepare.
+__device__ inline uint32_t bin_live_whole(const StripArgs &a, uint32_t b, uint32_t *qtot, uint32_t s0, uint32_t s1) {
+  __syncthreads();  // every thread is done reading qtot of the previous task
+  uint32_t n = 0;
+  for (uint32_t s = s0; s < s1; s++)
+    if (a.sflag[s] + 1u >= a.sweep) n += a.runs[(size_t)(b + 1) * a.ns + s] - a.runs[(size_t)b * a.ns + s];
+  n = read_lane_u32(wave_inclusive_scan(n), kWave - 1);
+  if (lane_id() == 0) qtot[wave_id()] = n;
+  __syncthreads();
+  uint32_t tot = 0;
+  for (uint32_t w = 0; w < kIterWaves; w++) tot += qtot[w];
+  return tot;
+}
+
 __global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
   extern __shared__ uint32_t lds2u[];
-  __shared__ uint32_t wtot[kIterWaves], live;
+  __shared__ uint32_t wtot[kIterWaves], qtot[kIterWaves], live;
   const uint32_t R = 1u << a.rlog, ns = a.ns;
   const BinLds<uint32_t> L = bin_lds<uint32_t>(lds2u, R, ns);
   const uint32_t tid = threadIdx.x;
@@ -687,12 +717,33 @@ __global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
     const uint4 bt = a.btask[t];
     // a whole bin with no message of a gathered strip cannot change; a split bin's slice still
     // writes its partial (zeros), which the split apply reads
-    if (bin_prepare<true>(a, bt, L, 0u, wtot, &live, s0, s1) == 0 && bt.w == kNoSlot) continue;
-    bin_messages<true>(a, bt, L, [&](uint32_t r, uint32_t x) { atomicMax(&L.acc[r], x); });
+    if (bt.w == kNoSlot && bin_live_whole(a, bt.x, qtot, s0, s1) == 0) continue;
+    const uint32_t n = bin_prepare<true>(a, bt, L, 0u, wtot, &live, s0, s1);
+    if (n == 0 && bt.w == kNoSlot) continue;
+    bin_messages(a, n, L, [&](uint32_t r, uint32_t x) { atomicMax(&L.acc[r], x); });
     __syncthreads();
     if (bt.w == kNoSlot) {
+      // rows without a message hold 0 and cannot change: only the others read their value, kUnroll
+      // loads in flight per thread
       const uint32_t v0 = bt.x << a.rlog, n = min(R, a.V - v0);
-      for (uint32_t i = tid; i < n; i += kIterBlock) min_apply(a, v0 + i, L.acc[i], changed);
+      for (uint32_t i0 = tid; i0 < n; i0 += kIterBlock * kUnroll) {
+        uint32_t m[kUnroll], old[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const uint32_t i = i0 + u * kIterBlock;
+          m[u] = i < n ? L.acc[i] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) old[u] = m[u] ? a.nlab[v0 + i0 + u * kIterBlock] : ~0u;
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          if (m[u] > old[u]) {
+            a.nlab[v0 + i0 + u * kIterBlock] = m[u];
+            a.sflag[(v0 + i0 + u * kIterBlock) >> a.wlog] = a.sweep;
+            changed++;
+          }
+        }
+      }
     } else {
       uint32_t *dst = reinterpret_cast<uint32_t *>(a.bpart) + ((size_t)bt.w << a.rlog);
       for (uint32_t i = tid; i < R; i += kIterBlock) dst[i] = L.acc[i];
