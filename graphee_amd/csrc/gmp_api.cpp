@@ -136,6 +136,10 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       in(0, 1, "window_packed");
       e->opt.window_packed = (int)value;
       break;
+    case GMP_OPT_PUSH_LOG:
+      in(0, 31, "push_log");
+      e->opt.push_log = (uint32_t)value;
+      break;
     default:
       fail(GMP_EINVAL, "gmp_set_option: unknown key " + std::to_string(key));
   }
@@ -159,6 +163,7 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_EDGE_BUDGET_MB: *value = (int64_t)e->opt.edge_budget_mb; break;
     case GMP_OPT_WINDOW_MB: *value = e->opt.window_mb; break;
     case GMP_OPT_WINDOW_PACKED: *value = e->opt.window_packed; break;
+    case GMP_OPT_PUSH_LOG: *value = e->opt.push_log; break;
     default: fail(GMP_EINVAL, "gmp_get_option: unknown key " + std::to_string(key));
   }
   GMP_API_END

@@ -247,6 +247,7 @@ struct Options {
   uint64_t edge_budget_mb = 0; // > 0: graphs whose edge data exceed it load semi-externally
   uint32_t window_mb = 64;     // semi-external windows: id bytes per window
   int window_packed = 1;       // semi-external windows: varint-packed ids
+  uint32_t push_log = 10;      // CC / Bellman-Ford: frontiers of at most V >> push_log push (0: sweeps only)
 };
 
 // Many graphs resident at once as one disjoint union (graph k owns union ids
@@ -394,18 +395,26 @@ void ensure_packed(Engine &e);
 // Builds e.g.strips for e.opt.strip_log / bin_log; returns false when the layout does not apply
 // (strips off, or more than 2^32 - 1 edges).
 bool ensure_strips(Engine &e);
-// Connected components by synchronous min-label sweeps over the strip layout: label (V entries
-// in work order, device) receives, per vertex, the least original id that reaches it along
-// edges -- GraphMP's CC rule iterated to its fixed point.  Returns false, having launched
-// nothing, when the layout does not apply; ms is the device time of the sweeps and edges the
-// edges gathered (a sweep skips strips none of whose vertices changed in the one before).
-bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms, uint64_t &edges);
-// Bellman-Ford by synchronous sweeps over the strip layout: dist (V entries in work order,
-// device) receives the shortest-path lengths from work id src (unit weights, or the graph's
-// weights), 0xFFFFFFFF where unreached; sums saturate at 0xFFFFFFFE like the frontier
-// relaxation.  Returns false, having launched nothing, when the layout does not apply; ms and
-// edges as for strip_cc.
-bool strip_sssp(Engine &e, uint32_t src, bool unit, uint32_t *dist, int &sweeps, double &ms, uint64_t &edges);
+// Min-type sweeps one at a time, for switching between sweeps and frontier pushes.  kind: 0 CC,
+// 1 unit-weight SSSP, 2 weighted SSSP.  c holds complemented values (c = ~x, 0 = unreached) in
+// work order; vstamp (V entries, or null) receives, per vertex, the number of the last sweep that
+// changed it.  sweep_open returns null, having launched nothing, when the layout does not apply.
+struct SweepRun;
+struct SweepRunDeleter {
+  void operator()(SweepRun *s) const;
+};
+using SweepRunPtr = std::unique_ptr<SweepRun, SweepRunDeleter>;
+SweepRunPtr sweep_open(Engine &e, int kind, uint32_t *c, uint32_t *vstamp);
+// Starting values: CC c = ~(original id), every strip active; SSSP only the source reached.
+void sweep_start(SweepRun &r, uint32_t src);
+// Continues from a frontier: the n vertices of front (device, work ids) changed and are not yet
+// propagated; every other strip is idle.
+void sweep_enter(SweepRun &r, const uint32_t *front, uint32_t n);
+// Runs one sweep; returns the number of vertices it changed.
+uint32_t sweep_step(SweepRun &r);
+uint32_t sweep_last(const SweepRun &r);
+uint64_t sweep_edges(SweepRun &r);
+void complement_values(Engine &e, uint32_t *x, uint32_t n);
 // Row id of every edge of a CSR (device array of E entries).
 void edge_row_ids(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off64, DevBuf &rowid);
 

@@ -105,6 +105,7 @@ struct StripArgs {
   const uint32_t *wt;      // edge weights in layout order (weighted SSSP), or
   const uint8_t *wt8;      // the same as bytes when every weight fits one
   uint32_t *sflag;         // per strip: the last sweep that changed one of its vertices
+  uint32_t *vstamp;        // per vertex: the last sweep that changed it (null: not kept)
   uint32_t sweep;          // this sweep's number, from 1
   unsigned long long *edges;  // edges gathered (tasks not skipped)
 };
@@ -684,6 +685,7 @@ __device__ inline void min_apply(const StripArgs &a, uint32_t v, uint32_t m, uin
   if (m > old) {
     a.nlab[v] = m;
     a.sflag[v >> a.wlog] = a.sweep;
+    if (a.vstamp) a.vstamp[v] = a.sweep;
     changed++;
   }
 }
@@ -738,8 +740,10 @@ __global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
           if (m[u] > old[u]) {
-            a.nlab[v0 + i0 + u * kIterBlock] = m[u];
-            a.sflag[(v0 + i0 + u * kIterBlock) >> a.wlog] = a.sweep;
+            const uint32_t v = v0 + i0 + u * kIterBlock;
+            a.nlab[v] = m[u];
+            a.sflag[v >> a.wlog] = a.sweep;
+            if (a.vstamp) a.vstamp[v] = a.sweep;
             changed++;
           }
         }
@@ -781,6 +785,17 @@ __global__ void min_init_kernel(uint32_t *c, const uint32_t *n2o, uint32_t n, ui
   for (uint32_t s = i0; s < ns; s += stride) sflag[s] = (src == kNever || s == (src >> wlog)) ? 0u : kNever;
   uint32_t *m = reinterpret_cast<uint32_t *>(msg);
   for (uint64_t j = i0; j < nseg; j += stride) m[j] = 0u;
+}
+
+// Entering sweeps from a frontier: every strip idle, then the strips of the frontier's vertices
+// (changed, not yet propagated) marked as changed in sweep `last`.
+__global__ void sflag_fill_kernel(uint32_t *sflag, uint32_t n, uint32_t value) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) sflag[s] = value;
+}
+
+__global__ void sflag_mark_kernel(const uint32_t *front, uint32_t n, uint32_t wlog, uint32_t last, uint32_t *sflag) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    sflag[front[i] >> wlog] = last;
 }
 
 __global__ void weights_to_bytes_kernel(const uint32_t *w, uint64_t n, uint8_t *b) {
@@ -1100,16 +1115,36 @@ void ensure_strip_weights(Engine &e) {
 
 // Synchronous sweeps of one min-type rule over the strip layout until nothing changes; c is
 // the complemented result in work order.
-bool strip_sweeps(Engine &e, int mode, uint32_t src, uint32_t *c, int &sweeps, double &ms, uint64_t &edges) {
-  if (!ensure_strips(e)) return false;
+}  // namespace
+
+struct SweepRun {
+  Engine &e;
+  int mode = kSweepCC;
+  StripArgs a{};
+  DevBuf changed, sflag, nedges;
+  uint32_t nsall = 0, gi = 1;
+  size_t lds2 = 0;
+  uint32_t last = 0;        // number of the last sweep run
+  bool msg_ready = false;   // messages zeroed
+  explicit SweepRun(Engine &en) : e(en) {}
+};
+
+void SweepRunDeleter::operator()(SweepRun *s) const { delete s; }
+
+SweepRunPtr sweep_open(Engine &e, int kind, uint32_t *c, uint32_t *vstamp) {
+  if (!ensure_strips(e)) return nullptr;
+  int mode = kind == 0 ? kSweepCC : kind == 1 ? kSweepUnit : kSweepWeighted;
   if (mode == kSweepWeighted) {
+    if (!e.g.work_in().w) return nullptr;
     ensure_strip_weights(e);
     if (e.g.strips.wt8.p) mode = kSweepWeighted8;
   }
+  SweepRunPtr sp(new SweepRun(e));
+  SweepRun &r = *sp;
+  r.mode = mode;
   const StripCsr &S = e.g.strips;
-  const uint32_t *n2o = e.g.work_identity ? nullptr : e.g.n2o.as<uint32_t>();  // built with the layout
   const uint32_t V = e.g.V, R = 1u << S.rlog;
-  const size_t lds2 = (size_t)4 * R + (2 * (size_t)S.ns + 1) * 4 + ((size_t)(1u << e.opt.bin_task_log) / kChunk + 2) * 4;
+  r.lds2 = (size_t)4 * R + (2 * (size_t)S.ns + 1) * 4 + ((size_t)(1u << e.opt.bin_task_log) / kChunk + 2) * 4;
   const void *gk = mode == kSweepCC          ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepCC>)
                    : mode == kSweepUnit      ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepUnit>)
                    : mode == kSweepWeighted8 ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepWeighted8>)
@@ -1117,17 +1152,16 @@ bool strip_sweeps(Engine &e, int mode, uint32_t src, uint32_t *c, int &sweeps, d
   if (hipFuncSetAttribute(gk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
     (void)hipGetLastError();
   if (hipFuncSetAttribute(reinterpret_cast<const void *>(&min_bin_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)lds2) != hipSuccess)
+                          (int)r.lds2) != hipSuccess)
     (void)hipGetLastError();
-  DevBuf changed, sflag, nedges;
-  changed.alloc(4);
-  nedges.alloc(8);
-  GMP_HIP(hipMemsetAsync(nedges.p, 0, 8, e.stream));
+  r.changed.alloc(4);
+  r.nedges.alloc(8);
+  GMP_HIP(hipMemsetAsync(r.nedges.p, 0, 8, e.stream));
   // a flag for every strip of the id range: vertices past the last strip holding an edge (no
   // out-edges in the degree order) still change and mark their strip
-  const uint32_t nsall = (uint32_t)(((uint64_t)V + (1ull << S.wlog) - 1) >> S.wlog);
-  sflag.alloc((size_t)std::max<uint32_t>(nsall, 1) * 4);
-  StripArgs a{};
+  r.nsall = (uint32_t)(((uint64_t)V + (1ull << S.wlog) - 1) >> S.wlog);
+  r.sflag.alloc((size_t)std::max<uint32_t>(r.nsall, 1) * 4);
+  StripArgs &a = r.a;
   a.code = S.code.as<uint16_t>();
   a.wtask = S.wtask.as<uint4>();
   a.gtask = S.gtask.as<uint4>();
@@ -1145,57 +1179,76 @@ bool strip_sweeps(Engine &e, int mode, uint32_t src, uint32_t *c, int &sweeps, d
   a.wlog = S.wlog;
   a.rlog = S.rlog;
   a.nlab = c;
-  a.changed = changed.as<uint32_t>();
+  a.changed = r.changed.as<uint32_t>();
   a.wt = S.wt.as<uint32_t>();
   a.wt8 = S.wt8.as<uint8_t>();
-  a.sflag = sflag.as<uint32_t>();
-  a.edges = nedges.as<unsigned long long>();
-  const uint32_t gi = (uint32_t)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, (uint64_t)e.num_cus * 8);
-  EventPair tot;
-  GMP_HIP(hipEventRecord(tot.a, e.stream));
-  min_init_kernel<<<gi, kBlock, 0, e.stream>>>(c, n2o, V, mode == kSweepCC ? kNever : src, a.sflag, nsall, S.wlog,
-                                               a.msg, S.nseg);
+  a.sflag = r.sflag.as<uint32_t>();
+  a.vstamp = vstamp;
+  a.edges = r.nedges.as<unsigned long long>();
+  r.gi = (uint32_t)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, (uint64_t)e.num_cus * 8);
+  return sp;
+}
+
+void sweep_start(SweepRun &r, uint32_t src) {
+  const StripCsr &S = r.e.g.strips;
+  const uint32_t *n2o = r.e.g.work_identity ? nullptr : r.e.g.n2o.as<uint32_t>();  // built with the layout
+  min_init_kernel<<<r.gi, kBlock, 0, r.e.stream>>>(r.a.nlab, n2o, r.a.V, r.mode == kSweepCC ? kNever : src, r.a.sflag,
+                                                   r.nsall, S.wlog, r.a.msg, S.nseg);
   GMP_CHECK_LAUNCH();
-  sweeps = 0;
-  for (;;) {
-    a.sweep = (uint32_t)sweeps + 1;
-    GMP_HIP(hipMemsetAsync(changed.p, 0, 4, e.stream));
-    if (mode == kSweepCC) min_strip_gather_kernel<kSweepCC><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
-    else if (mode == kSweepUnit) min_strip_gather_kernel<kSweepUnit><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
-    else if (mode == kSweepWeighted8)
-      min_strip_gather_kernel<kSweepWeighted8><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
-    else min_strip_gather_kernel<kSweepWeighted><<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
-    GMP_CHECK_LAUNCH();
-    min_bin_kernel<<<S.g2, kIterBlock, lds2, e.stream>>>(a);
-    GMP_CHECK_LAUNCH();
-    if (S.nsplit) {
-      min_split_apply_kernel<<<S.g3, kBlock, 0, e.stream>>>(a);
-      GMP_CHECK_LAUNCH();
-    }
-    uint32_t h = 0;
-    GMP_HIP(hipMemcpyAsync(&h, changed.p, 4, hipMemcpyDeviceToHost, e.stream));
-    GMP_HIP(hipStreamSynchronize(e.stream));
-    sweeps++;
-    if (h == 0) break;
+  r.msg_ready = true;
+  r.last = 0;
+}
+
+void sweep_enter(SweepRun &r, const uint32_t *front, uint32_t n) {
+  const StripCsr &S = r.e.g.strips;
+  if (!r.msg_ready) {
+    GMP_HIP(hipMemsetAsync(r.a.msg, 0, (size_t)S.nseg * 4, r.e.stream));
+    r.msg_ready = true;
   }
-  complement_kernel<<<gi, kBlock, 0, e.stream>>>(c, V);
+  sflag_fill_kernel<<<grid_for(r.nsall), kBlock, 0, r.e.stream>>>(r.a.sflag, r.nsall, kNever);
   GMP_CHECK_LAUNCH();
-  GMP_HIP(hipEventRecord(tot.b, e.stream));
-  GMP_HIP(hipMemcpyAsync(&edges, nedges.p, 8, hipMemcpyDeviceToHost, e.stream));
-  GMP_HIP(hipStreamSynchronize(e.stream));
-  ms = tot.ms();
-  return true;
+  if (n) {
+    sflag_mark_kernel<<<grid_for(n), kBlock, 0, r.e.stream>>>(front, n, S.wlog, r.last, r.a.sflag);
+    GMP_CHECK_LAUNCH();
+  }
 }
 
-}  // namespace
-
-bool strip_cc(Engine &e, uint32_t *label, int &sweeps, double &ms, uint64_t &edges) {
-  return strip_sweeps(e, kSweepCC, kNever, label, sweeps, ms, edges);
+uint32_t sweep_step(SweepRun &r) {
+  const StripCsr &S = r.e.g.strips;
+  StripArgs &a = r.a;
+  hipStream_t st = r.e.stream;
+  a.sweep = ++r.last;
+  GMP_HIP(hipMemsetAsync(r.changed.p, 0, 4, st));
+  if (r.mode == kSweepCC) min_strip_gather_kernel<kSweepCC><<<S.g1, kIterBlock, S.lds1, st>>>(a);
+  else if (r.mode == kSweepUnit) min_strip_gather_kernel<kSweepUnit><<<S.g1, kIterBlock, S.lds1, st>>>(a);
+  else if (r.mode == kSweepWeighted8) min_strip_gather_kernel<kSweepWeighted8><<<S.g1, kIterBlock, S.lds1, st>>>(a);
+  else min_strip_gather_kernel<kSweepWeighted><<<S.g1, kIterBlock, S.lds1, st>>>(a);
+  GMP_CHECK_LAUNCH();
+  min_bin_kernel<<<S.g2, kIterBlock, r.lds2, st>>>(a);
+  GMP_CHECK_LAUNCH();
+  if (S.nsplit) {
+    min_split_apply_kernel<<<S.g3, kBlock, 0, st>>>(a);
+    GMP_CHECK_LAUNCH();
+  }
+  uint32_t *h = static_cast<uint32_t *>(r.e.h_pinned);
+  GMP_HIP(hipMemcpyAsync(h, r.changed.p, 4, hipMemcpyDeviceToHost, st));
+  GMP_HIP(hipStreamSynchronize(st));
+  return *h;
 }
 
-bool strip_sssp(Engine &e, uint32_t src, bool unit, uint32_t *dist, int &sweeps, double &ms, uint64_t &edges) {
-  if (!unit && !e.g.work_in().w) return false;
-  return strip_sweeps(e, unit ? kSweepUnit : kSweepWeighted, src, dist, sweeps, ms, edges);
+uint32_t sweep_last(const SweepRun &r) { return r.last; }
+
+uint64_t sweep_edges(SweepRun &r) {
+  uint64_t *h = static_cast<uint64_t *>(r.e.h_pinned);
+  GMP_HIP(hipMemcpyAsync(h, r.nedges.p, 8, hipMemcpyDeviceToHost, r.e.stream));
+  GMP_HIP(hipStreamSynchronize(r.e.stream));
+  return *h;
+}
+
+void complement_values(Engine &e, uint32_t *x, uint32_t n) {
+  const uint32_t gi = (uint32_t)std::min<uint64_t>(((uint64_t)n + kBlock - 1) / kBlock, (uint64_t)e.num_cus * 8);
+  complement_kernel<<<std::max<uint32_t>(gi, 1), kBlock, 0, e.stream>>>(x, n);
+  GMP_CHECK_LAUNCH();
 }
 
 }  // namespace gmp

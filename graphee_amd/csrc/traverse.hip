@@ -532,6 +532,118 @@ void read_counters(Engine &e, Counters *dev, Counters &h) {
   std::memcpy(&h, e.h_pinned, sizeof(Counters));
 }
 
+// Vertices whose stamp equals `sweep`, appended to list (count = cnt->near_cnt).
+__global__ __launch_bounds__(kBlock) void collect_stamped_kernel(const uint32_t *vstamp, uint32_t V, uint32_t sweep,
+                                                                 uint32_t *list, Counters *cnt) {
+  for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < V; v0 += gridDim.x * blockDim.x) {
+    const uint32_t v = v0 + threadIdx.x;
+    const bool p = v < V && vstamp[v] == sweep;
+    const uint32_t pos = wave_append(p, &cnt->near_cnt);
+    if (p) list[pos] = v;
+  }
+}
+
+// Direction-optimizing min solve (CC labels, Bellman-Ford distances) on the work layout: frontier
+// pushes over the out-CSR while at most V >> push_log vertices are active, dense pull sweeps over
+// the source strips otherwise (GraphPS's sparse/dense switch, GraphPS/include/defines.h:9 and
+// GraphPS/graphps.h:336-390, on one GPU).  SSSP starts pushing from the source, CC sweeping over
+// every vertex; either may switch any number of times.  A sweep stamps the vertices it changes,
+// so its frontier is exact; entering sweeps marks only the frontier's strips.  Every phase keeps
+// relaxing until nothing drops, so the fixed point is the pure sweeps' or pure pushes'.  Returns
+// false, having launched nothing, when the strip layout does not apply; the values (plain, work
+// order) are left in e.s_a.
+template <class OffT>
+bool hybrid_min(Engine &e, int kind, uint32_t src, int &rounds, int &sweeps, double &ms, uint64_t &edges) {
+  Graph &g = e.g;
+  const uint32_t V = g.V;
+  TraverseBufs b = alloc_traverse(e, false);
+  e.s_e.alloc(std::max<size_t>(V, 1) * 4);
+  uint32_t *vstamp = e.s_e.as<uint32_t>();
+  const uint32_t T = e.opt.push_log ? (V >> e.opt.push_log) : 0u;  // largest frontier that pushes
+  SweepRunPtr sw = sweep_open(e, kind, b.dist, T ? vstamp : nullptr);
+  if (!sw) return false;
+  RelaxArgs<OffT> a{};
+  if (T) {
+    ensure_work_out(e);
+    ensure_hubs<OffT>(e);
+    const CsrView ov = g.work_out();
+    a.off = ov.offs<OffT>();
+    a.idx = ov.idx;
+    a.w = ov.w;
+  }
+  const int mode = kind == 0 ? kModeMinLabel : kind == 1 ? kModeUnit : kModeWeighted;
+  EventPair tot;
+  GMP_HIP(hipEventRecord(tot.a, e.stream));
+  GMP_HIP(hipMemsetAsync(b.cnt, 0, sizeof(Counters), e.stream));
+  if (T) {
+    fill_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(vstamp, V, 0u);
+    fill_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.stamp, V, 0u);
+    GMP_CHECK_LAUNCH();
+    a.dist = b.dist;
+    a.near_stamp = b.stamp;
+    a.threshold = kInf;
+    a.cnt = b.cnt;
+    a.V = V;
+    attach_hubs(e, a);
+  }
+  uint32_t *qa = b.q0, *qb = b.q1;
+  uint32_t n = 0, stamp = 1;
+  bool pushing = false;
+  if (kind == 0 || !T) {
+    sweep_start(*sw, src);  // complemented starting values, every strip (CC) or the source's active
+  } else {
+    fill_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.dist, V, kInf);
+    GMP_CHECK_LAUNCH();
+    const uint32_t zero = 0;
+    GMP_HIP(hipMemcpyAsync(b.dist + src, &zero, 4, hipMemcpyHostToDevice, e.stream));
+    GMP_HIP(hipMemcpyAsync(qa, &src, 4, hipMemcpyHostToDevice, e.stream));
+    n = 1;
+    pushing = true;
+  }
+  rounds = 0;
+  sweeps = 0;
+  Counters h{};
+  for (;;) {
+    if (pushing) {
+      if (n == 0) break;
+      if (n > T) {  // dense: continue with sweeps from the frontier
+        complement_values(e, b.dist, V);
+        sweep_enter(*sw, qa, n);
+        pushing = false;
+        continue;
+      }
+      a.front = qa;
+      a.nfront = n;
+      a.stamp = stamp++;
+      a.near_out = qb;
+      GMP_HIP(hipMemsetAsync(&b.cnt->near_cnt, 0, 8, e.stream));  // near_cnt, far_cnt
+      launch_relax<OffT>(e, mode, a);
+      read_counters(e, b.cnt, h);
+      n = h.near_cnt;
+      std::swap(qa, qb);
+      rounds++;
+    } else {
+      const uint32_t changed = sweep_step(*sw);
+      sweeps++;
+      if (changed == 0) break;
+      if (changed <= T) {  // sparse: push from the vertices this sweep changed
+        GMP_HIP(hipMemsetAsync(&b.cnt->near_cnt, 0, 4, e.stream));
+        collect_stamped_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(vstamp, V, sweep_last(*sw), qa, b.cnt);
+        GMP_CHECK_LAUNCH();
+        complement_values(e, b.dist, V);
+        n = changed;
+        pushing = true;
+      }
+    }
+  }
+  if (!pushing) complement_values(e, b.dist, V);
+  GMP_HIP(hipEventRecord(tot.b, e.stream));
+  read_counters(e, b.cnt, h);
+  edges = sweep_edges(*sw) + h.edges;
+  ms = tot.ms();
+  return true;
+}
+
 // Bucket epochs of delta-stepping.  FIXED: aligned buckets [k*delta, (k+1)*delta).  LEARNED:
 // the MLP picks each epoch's bucket [m, m + 8 * 2^c) from far-pile features (m = smallest live
 // far distance).  EXPLORE records traces: at checkpoint epochs, from a snapshot of the state,
@@ -546,23 +658,23 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
   Graph &g = e.g;
   const bool unit = p.unit_weights || !g.weighted;
   if (policy == kPolicyBF) {
-    // Bellman-Ford as dense sweeps over the source-strip layout when it applies (strips whose
-    // sources did not change are skipped); the frontier form below otherwise
+    // Bellman-Ford: frontier pushes while the frontier is small, sweeps over the source strips
+    // while it is large (when the strip layout applies); the frontier form below otherwise
     ensure_work(e);
-    e.s_a.alloc(std::max<size_t>(g.V, 1) * 4);  // finish_output maps through s_f
-    int sweeps = 0;
+    int rounds = 0, sweeps = 0;
     double ms = 0.0;
     uint64_t edges = 0;
-    if (strip_sssp(e, work_id(e, src), unit, e.s_a.as<uint32_t>(), sweeps, ms, edges)) {
+    if (hybrid_min<OffT>(e, unit ? 1 : 2, work_id(e, src), rounds, sweeps, ms, edges)) {
       finish_output(e, e.s_a.as<uint32_t>(), dist_out);
       if (st) {
         std::memset(st, 0, sizeof *st);
-        st->iterations = sweeps;
-        st->launches = sweeps;
+        st->iterations = rounds + sweeps;
+        st->launches = rounds + sweeps;
         st->ms_total = ms;
         st->ms_kernel = ms;
         st->edges = edges;
         st->bytes = edges * (unit ? 8 : 12);
+        st->epochs = sweeps;
       }
       return;
     }
@@ -780,23 +892,23 @@ template <class OffT>
 void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
   Graph &g = e.g;
   {
-    // dense sweeps over the source-strip layout when it applies: every sweep streams the edges
-    // once (2-byte codes, labels staged in LDS), which beats frontier relaxation on graphs of
-    // small diameter
-    e.s_a.alloc(std::max<size_t>(g.V, 1) * 4);  // finish_output maps through s_f
-    int sweeps = 0;
+    // dense sweeps over the source-strip layout while many labels change (every sweep streams the
+    // edges once: 2-byte codes, labels staged in LDS), frontier pushes for the sparse tail
+    ensure_work(e);
+    int rounds = 0, sweeps = 0;
     double ms = 0.0;
     uint64_t edges = 0;
-    if (strip_cc(e, e.s_a.as<uint32_t>(), sweeps, ms, edges)) {
+    if (hybrid_min<OffT>(e, 0, 0u, rounds, sweeps, ms, edges)) {
       finish_output(e, e.s_a.as<uint32_t>(), label_out);
       if (st) {
         std::memset(st, 0, sizeof *st);
-        st->iterations = sweeps;
-        st->launches = sweeps;
+        st->iterations = rounds + sweeps;
+        st->launches = rounds + sweeps;
         st->ms_total = ms;
         st->ms_kernel = ms;
         st->edges = edges;
         st->bytes = edges * 8;
+        st->epochs = sweeps;
       }
       return;
     }

@@ -41,8 +41,10 @@ extern "C" {
 #define GMP_APP_SSSP_W 4
 
 /* gmp_sssp methods */
-#define GMP_SSSP_BF 0      /* Bellman-Ford: sweeps over the source strips that skip unchanged
-                              strips, or frontier relaxation when strips are off */
+#define GMP_SSSP_BF 0      /* Bellman-Ford: frontier pushes while the frontier is small
+                              (GMP_OPT_PUSH_LOG), sweeps over the source strips that skip
+                              unchanged strips while it is large; frontier relaxation alone
+                              when strips are off */
 #define GMP_SSSP_DELTA 1   /* fixed-delta stepping */
 #define GMP_SSSP_LEARNED 2 /* delta chosen per bucket epoch by the trained MLP */
 
@@ -130,6 +132,10 @@ int gmp_destroy(gmp_engine *h);
 #define GMP_OPT_WINDOW_MB 11      /* semi-external: id bytes per window, 1..4096 (default 64) */
 #define GMP_OPT_WINDOW_PACKED 12  /* semi-external: 1 (default) windows hold delta + varint packed
                                      ids (decoded in the gather), 0 plain uint32 ids */
+#define GMP_OPT_PUSH_LOG 13        /* CC and Bellman-Ford: frontiers of at most V >> k vertices are
+                                     pushed over the out-edges, larger ones swept over the source
+                                     strips (GraphPS's sparse/dense switch); 0 = sweeps only,
+                                     0..31 (default 10).  Results do not depend on it. */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 

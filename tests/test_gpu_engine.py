@@ -234,6 +234,50 @@ def test_cc_sweeps_and_frontier_agree(engine, strip_log, bin_log, task_log):
             engine.set_option(k, v)
 
 
+@pytest.mark.parametrize("push_log", [0, 1, 3, 8])
+def test_push_sweep_switch_keeps_results(engine, push_log):
+    """Bellman-Ford and CC switching between frontier pushes (frontiers of at most V >> push_log
+    vertices) and strip sweeps (larger ones) -- push_log 0: sweeps only, 1: nearly all pushes --
+    give Dijkstra's distances, BFS levels and the union-find labels, also on a long path that
+    switches back and forth."""
+    saved = engine.get_option(gmp.OPT_PUSH_LOG)
+    engine.set_option(gmp.OPT_PUSH_LOG, push_log)
+    try:
+        g = gen(engine, 13, weighted=True)
+        V = g["outdeg"].size
+        ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+        for s in (engine.pick_source(5), engine.pick_source(6)):
+            say(f"push_log {push_log}: bf src={s}")
+            d, st = engine.sssp(s, method=gmp.SSSP_BF)
+            np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, ow, s))
+            d, _ = engine.sssp(s, method=gmp.SSSP_BF, unit_weights=True)
+            np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, None, s))
+            if push_log == 0:
+                assert st.epochs == st.iterations  # every step a sweep
+            if push_log == 1:
+                assert st.epochs < st.iterations  # some steps pushed
+        g = gen(engine, 13, undirected=True)
+        lab, st = engine.cc()
+        np.testing.assert_array_equal(lab, oracle.cc_unionfind(g["off"], g["idx"]))
+        # a path with side trees: sparse fronts, dense bursts, back to sparse
+        V = 40000
+        rng = np.random.default_rng(3)
+        a = np.concatenate([np.arange(0, 4000), rng.integers(4000, V, 30000)])
+        b = np.concatenate([np.arange(1, 4001), rng.integers(4000, V, 30000)])
+        src_, dst_ = np.concatenate([a, b]), np.concatenate([b, a])
+        off, idx = graphs.csr_from_edges(V, src_, dst_)
+        w = graphs.edge_weights(off, idx, 4)
+        engine.load_csr(off, idx, w)
+        lab, _ = engine.cc()
+        np.testing.assert_array_equal(lab, oracle.cc_unionfind(off, idx))
+        ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+        for s in (0, 2000, 39999):
+            d, _ = engine.sssp(s, method=gmp.SSSP_BF)
+            np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, ow, s))
+    finally:
+        engine.set_option(gmp.OPT_PUSH_LOG, saved)
+
+
 def test_off64_offsets_path(engine):
     g = gen(engine, 11, weighted=True, off64=True)
     V = g["outdeg"].size
