@@ -75,14 +75,33 @@ __device__ inline void gather_range(const PrArgs<OffT> &a, uint64_t e0, uint64_t
   const uint32_t lane = lane_id();
   if (e1 <= e0) return;
   if (COMP) {
-    for (uint64_t g = e0 / kVarintGroup; g * kVarintGroup < e1; g++) {
-      const uint32_t src = varint_decode_group(a.pk, g, a.E, scratch);
-      const uint64_t e = g * kVarintGroup + lane;
-      if (e >= e0 && e < e1) {
-        const float x = a.contrib_in[src];
+    // two-stage pipeline: group g's gathers and group g+2's bytes are in flight while group g+1
+    // decodes; lanes outside [e0, e1) load a harmless contrib_in[0]
+    const uint64_t gend = (e1 + kVarintGroup - 1) / kVarintGroup;
+    uint64_t g = e0 / kVarintGroup;
+    const VarintGroupWords w0 = varint_fetch(a.pk, g);
+    VarintGroupWords w1 = g + 1 < gend ? varint_fetch(a.pk, g + 1) : w0;
+    uint64_t e = g * kVarintGroup + lane;
+    bool ok = e >= e0 && e < e1;
+    uint32_t src = varint_decode_words(w0, a.pk, g, a.E, scratch);
+    float x = a.contrib_in[ok ? src : 0u];
+    for (; g < gend; g++) {
+      const VarintGroupWords w2 = g + 2 < gend ? varint_fetch(a.pk, g + 2) : w1;
+      const uint64_t en = e + kVarintGroup;
+      const bool okn = en >= e0 && en < e1;
+      float xn = 0.0f;
+      if (g + 1 < gend) {
+        const uint32_t sn = varint_decode_words(w1, a.pk, g + 1, a.E, scratch);
+        xn = a.contrib_in[okn ? sn : 0u];
+      }
+      if (ok) {
         if (TO_LDS) buf[e - e0] = x;
         else acc_add<KAHAN>(s, c, x);
       }
+      x = xn;
+      ok = okn;
+      e = en;
+      w1 = w2;
     }
     return;
   }
