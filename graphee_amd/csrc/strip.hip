@@ -106,6 +106,7 @@ struct StripArgs {
   const uint8_t *wt8;      // the same as bytes when every weight fits one
   uint32_t *sflag;         // per strip: the last sweep that changed one of its vertices
   uint32_t *vstamp;        // per vertex: the last sweep that changed it (null: not kept)
+  uint32_t *gflag;         // per strip: the last sweep that gathered it
   uint32_t sweep;          // this sweep's number, from 1
   unsigned long long *edges;  // edges gathered (tasks not skipped)
 };
@@ -446,11 +447,14 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
   float *mb = sv + W + wv * kRound;
   for (uint32_t t = blockIdx.x; t < a.ngt; t += gridDim.x) {
     const uint4 gt = a.gtask[t];
-    // a sweep skips strips none of whose vertices changed in the previous sweep: their
-    // messages from an earlier sweep (or the cleared ones) still hold
+    // a sweep skips strips none of whose vertices changed in the previous sweep; the bins apply
+    // the messages of gathered strips only
     if (MODE != kSweepPR) {
       if (a.sflag[gt.x] + 1u != a.sweep) continue;
-      if (threadIdx.x == 0) atomicAdd(a.edges, (unsigned long long)(a.wtask[gt.z - 1].y - a.wtask[gt.y].x));
+      if (threadIdx.x == 0) {
+        atomicAdd(a.edges, (unsigned long long)(a.wtask[gt.z - 1].y - a.wtask[gt.y].x));
+        a.gflag[gt.x] = a.sweep;
+      }
     }
     const uint32_t v0 = gt.x << a.wlog;
     const uint32_t n = min(W, a.V - v0);
@@ -544,9 +548,7 @@ __device__ inline uint32_t bin_prepare(const StripArgs &a, const uint4 &bt, cons
     const uint32_t p0 = run, p1 = run + L.pre[s];
     run = p1;
     const uint32_t q0 = max(p0, bt.y), q1 = min(p1, bt.z);
-    // gathered this sweep: flag t - 1; bins that already applied may have moved it to t
-    // (re-applying a strip's earlier messages is harmless), kNever + 1 wraps to 0
-    const bool idle = IDLE && a.sflag[s] + 1u < a.sweep;
+    const bool idle = IDLE && a.gflag[s] != a.sweep;  // not gathered in this sweep
     const uint32_t len = (q1 > q0 && !idle) ? q1 - q0 : 0u;
     L.rbase[s] += len ? q0 - p0 : 0u;
     L.pre[s] = len;
@@ -697,7 +699,7 @@ __device__ inline uint32_t bin_live_whole(const StripArgs &a, uint32_t b, uint32
   __syncthreads();  // every thread is done reading qtot of the previous task
   uint32_t n = 0;
   for (uint32_t s = s0; s < s1; s++)
-    if (a.sflag[s] + 1u >= a.sweep) n += a.runs[(size_t)(b + 1) * a.ns + s] - a.runs[(size_t)b * a.ns + s];
+    if (a.gflag[s] == a.sweep) n += a.runs[(size_t)(b + 1) * a.ns + s] - a.runs[(size_t)b * a.ns + s];
   n = read_lane_u32(wave_inclusive_scan(n), kWave - 1);
   if (lane_id() == 0) qtot[wave_id()] = n;
   __syncthreads();
@@ -776,15 +778,15 @@ __global__ __launch_bounds__(kBlock) void min_split_apply_kernel(StripArgs a) {
 }
 
 // Starting values: CC c = ~(original id) and every strip active; SSSP c = 0 (unreached) but
-// ~0 at the source, and only the source's strip active.  Messages start at 0 (neutral), so a
-// strip no sweep has gathered yet offers nothing.
+// ~0 at the source, and only the source's strip active.  No strip has been gathered yet.
 __global__ void min_init_kernel(uint32_t *c, const uint32_t *n2o, uint32_t n, uint32_t src, uint32_t *sflag,
-                                uint32_t ns, uint32_t wlog, float *msg, uint64_t nseg) {
+                                uint32_t *gflag, uint32_t ns, uint32_t wlog) {
   const uint32_t stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t i = i0; i < n; i += stride) c[i] = src == kNever ? ~(n2o ? n2o[i] : i) : (i == src ? ~0u : 0u);
-  for (uint32_t s = i0; s < ns; s += stride) sflag[s] = (src == kNever || s == (src >> wlog)) ? 0u : kNever;
-  uint32_t *m = reinterpret_cast<uint32_t *>(msg);
-  for (uint64_t j = i0; j < nseg; j += stride) m[j] = 0u;
+  for (uint32_t s = i0; s < ns; s += stride) {
+    sflag[s] = (src == kNever || s == (src >> wlog)) ? 0u : kNever;
+    gflag[s] = 0u;
+  }
 }
 
 // Entering sweeps from a frontier: every strip idle, then the strips of the frontier's vertices
@@ -1121,11 +1123,10 @@ struct SweepRun {
   Engine &e;
   int mode = kSweepCC;
   StripArgs a{};
-  DevBuf changed, sflag, nedges;
+  DevBuf changed, sflag, gflag, nedges;
   uint32_t nsall = 0, gi = 1;
   size_t lds2 = 0;
   uint32_t last = 0;        // number of the last sweep run
-  bool msg_ready = false;   // messages zeroed
   explicit SweepRun(Engine &en) : e(en) {}
 };
 
@@ -1161,6 +1162,7 @@ SweepRunPtr sweep_open(Engine &e, int kind, uint32_t *c, uint32_t *vstamp) {
   // out-edges in the degree order) still change and mark their strip
   r.nsall = (uint32_t)(((uint64_t)V + (1ull << S.wlog) - 1) >> S.wlog);
   r.sflag.alloc((size_t)std::max<uint32_t>(r.nsall, 1) * 4);
+  r.gflag.alloc((size_t)std::max<uint32_t>(r.nsall, 1) * 4);
   StripArgs &a = r.a;
   a.code = S.code.as<uint16_t>();
   a.wtask = S.wtask.as<uint4>();
@@ -1184,6 +1186,7 @@ SweepRunPtr sweep_open(Engine &e, int kind, uint32_t *c, uint32_t *vstamp) {
   a.wt8 = S.wt8.as<uint8_t>();
   a.sflag = r.sflag.as<uint32_t>();
   a.vstamp = vstamp;
+  a.gflag = r.gflag.as<uint32_t>();
   a.edges = r.nedges.as<unsigned long long>();
   r.gi = (uint32_t)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, (uint64_t)e.num_cus * 8);
   return sp;
@@ -1193,20 +1196,19 @@ void sweep_start(SweepRun &r, uint32_t src) {
   const StripCsr &S = r.e.g.strips;
   const uint32_t *n2o = r.e.g.work_identity ? nullptr : r.e.g.n2o.as<uint32_t>();  // built with the layout
   min_init_kernel<<<r.gi, kBlock, 0, r.e.stream>>>(r.a.nlab, n2o, r.a.V, r.mode == kSweepCC ? kNever : src, r.a.sflag,
-                                                   r.nsall, S.wlog, r.a.msg, S.nseg);
+                                                   r.a.gflag, r.nsall, S.wlog);
   GMP_CHECK_LAUNCH();
-  r.msg_ready = true;
   r.last = 0;
 }
 
 void sweep_enter(SweepRun &r, const uint32_t *front, uint32_t n) {
   const StripCsr &S = r.e.g.strips;
-  if (!r.msg_ready) {
-    GMP_HIP(hipMemsetAsync(r.a.msg, 0, (size_t)S.nseg * 4, r.e.stream));
-    r.msg_ready = true;
-  }
   sflag_fill_kernel<<<grid_for(r.nsall), kBlock, 0, r.e.stream>>>(r.a.sflag, r.nsall, kNever);
   GMP_CHECK_LAUNCH();
+  if (r.last == 0) {  // no sweep ran yet in this solve
+    sflag_fill_kernel<<<grid_for(r.nsall), kBlock, 0, r.e.stream>>>(r.a.gflag, r.nsall, 0u);
+    GMP_CHECK_LAUNCH();
+  }
   if (n) {
     sflag_mark_kernel<<<grid_for(n), kBlock, 0, r.e.stream>>>(front, n, S.wlog, r.last, r.a.sflag);
     GMP_CHECK_LAUNCH();
