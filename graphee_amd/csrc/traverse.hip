@@ -341,41 +341,60 @@ __global__ __launch_bounds__(64) void policy_kernel(const float *P, Counters *cn
 // Smallest distance among far entries still at or above the old threshold.
 __global__ __launch_bounds__(kBlock) void far_min_kernel(const uint32_t *far, uint32_t n, const uint32_t *dist,
                                                          uint32_t thr_old, Counters *cnt) {
+  __shared__ uint32_t sm[kWavesPerBlock];
   uint32_t m = kInf;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t d = dist[far[i]];
     if (d >= thr_old) m = min(m, d);
   }
   m = wave_reduce_min(m);
-  if (lane_id() == 0 && m != kInf) atomicMin(&cnt->far_min, m);
+  if (lane_id() == 0) sm[wave_id()] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWavesPerBlock; w++) m = min(m, sm[w]);
+    if (m != kInf) atomicMin(&cnt->far_min, m);
+  }
 }
 
-// Splits the far pile: entries settled below thr_old are dropped, entries below thr_new move
-// to the near frontier, the rest stay (compacted into far_next).
+// Splits the far pile at a new threshold: entries below the old threshold are settled and
+// dropped, entries below the new one join the near frontier (once per launch), the rest stay.
+// kSplitPer entries per thread, their loads in flight together; one atomic per block and list.
+constexpr uint32_t kSplitPer = 4;
 __global__ __launch_bounds__(kBlock) void far_split_kernel(const uint32_t *far, uint32_t n, const uint32_t *dist,
                                                            uint32_t thr_old, uint32_t thr_new, uint32_t *far_flag,
                                                            uint32_t *near_stamp, uint32_t stamp, uint32_t *near_out,
                                                            uint32_t *far_next, Counters *cnt) {
-  const uint32_t nfull = (n + kBlock - 1) / kBlock * kBlock;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nfull; i += gridDim.x * blockDim.x) {
-    bool to_near = false, keep = false;
-    uint32_t v = 0;
-    if (i < n) {
-      v = far[i];
-      const uint32_t d = dist[v];
-      if (d < thr_old) {
-        far_flag[v] = 0;
-      } else if (d < thr_new) {
-        far_flag[v] = 0;
-        to_near = atomicExch(&near_stamp[v], stamp) != stamp;
+  __shared__ uint32_t shn[kWavesPerBlock + 1], shk[kWavesPerBlock + 1];
+  const uint64_t per_block = (uint64_t)kBlock * kSplitPer;
+  for (uint64_t b0 = blockIdx.x * per_block; b0 < n; b0 += (uint64_t)gridDim.x * per_block) {
+    uint32_t v[kSplitPer], d[kSplitPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kSplitPer; k++) {
+      const uint64_t i = b0 + k * kBlock + threadIdx.x;
+      v[k] = i < n ? far[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kSplitPer; k++) d[k] = dist[v[k]];
+    uint32_t to_near = 0, keep = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSplitPer; k++) {
+      if (b0 + k * kBlock + threadIdx.x >= n) continue;
+      if (d[k] < thr_old) {
+        far_flag[v[k]] = 0;
+      } else if (d[k] < thr_new) {
+        far_flag[v[k]] = 0;
+        if (atomicExch(&near_stamp[v[k]], stamp) != stamp) to_near |= 1u << k;
       } else {
-        keep = true;
+        keep |= 1u << k;
       }
     }
-    const uint32_t sn = wave_append(to_near, &cnt->near_cnt);
-    if (to_near) near_out[sn] = v;
-    const uint32_t sf = wave_append(keep, &cnt->far_cnt);
-    if (keep) far_next[sf] = v;
+    uint32_t pn = block_append<kWavesPerBlock>((uint32_t)__popc(to_near), &cnt->near_cnt, shn);
+    uint32_t pk = block_append<kWavesPerBlock>((uint32_t)__popc(keep), &cnt->far_cnt, shk);
+#pragma unroll
+    for (uint32_t k = 0; k < kSplitPer; k++) {
+      if ((to_near >> k) & 1u) near_out[pn++] = v[k];
+      if ((keep >> k) & 1u) far_next[pk++] = v[k];
+    }
   }
 }
 

@@ -211,4 +211,28 @@ __device__ inline uint32_t wave_append(bool pred, uint32_t *count) {
   return base + (uint32_t)__popcll(below);
 }
 
+// Block-aggregated append: every thread of the block calls it (block-uniform control flow) with
+// its number of entries; returns the first of its consecutive slots, with one atomic on `count`
+// per block.  `sh` is WAVES + 1 words of LDS.
+template <int WAVES>
+__device__ inline uint32_t block_append(uint32_t mine, uint32_t *count, uint32_t *sh) {
+  const uint32_t lane = lane_id(), wv = wave_id();
+  const uint32_t incl = wave_scan_dpp(mine);
+  if (lane == kWave - 1) sh[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < WAVES; w++) {
+      const uint32_t x = sh[w];
+      sh[w] = t;
+      t += x;
+    }
+    sh[WAVES] = t ? atomicAdd(count, t) : 0u;
+  }
+  __syncthreads();
+  const uint32_t base = sh[WAVES] + sh[wv] + incl - mine;
+  __syncthreads();  // sh may be reused by the next call
+  return base;
+}
+
 }  // namespace gmp
