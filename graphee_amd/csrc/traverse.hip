@@ -88,6 +88,26 @@ struct RelaxArgs {
 
 constexpr int kRelaxUnroll = 4;  // edges per lane in flight in the relax loops
 
+// Appends of improved targets are staged in a per-wave LDS buffer and written out with one
+// atomic per flush (when the buffer could overflow on the next batch, and at the end of the
+// kernel), coalesced, instead of one atomic per wave and batch.
+constexpr uint32_t kAppendCap = 512;  // entries per wave and list
+struct WaveAppend {
+  uint32_t *buf;  // this wave's LDS slots
+  uint32_t n;     // entries held (the same in every lane)
+};
+
+__device__ inline void append_flush(WaveAppend &w, uint32_t *count, uint32_t *out) {
+  if (!w.n) return;
+  wave_sync();
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(count, w.n);
+  base = read_lane_u32(base, 0);
+  for (uint32_t i = lane_id(); i < w.n; i += kWave) out[base + i] = w.buf[i];
+  wave_sync();
+  w.n = 0;
+}
+
 // Relaxes up to kRelaxUnroll edges per lane (edge[q], source value du[q]; entries with bit q
 // of `live` clear are skipped) and appends the improved targets to the near frontier (first
 // time this launch) or the far pile.  Each phase is issued for all edges before its results
@@ -95,7 +115,8 @@ constexpr int kRelaxUnroll = 4;  // edges per lane in flight in the relax loops
 // edges, then the stamp or flag exchange of those whose atomicMin lowered the distance.
 template <class OffT, int MODE>
 __device__ inline void relax_batch(const RelaxArgs<OffT> &a, const uint64_t (&edge)[kRelaxUnroll],
-                                   const uint32_t (&du)[kRelaxUnroll], uint32_t live, unsigned long long &smask) {
+                                   const uint32_t (&du)[kRelaxUnroll], uint32_t live, unsigned long long &smask,
+                                   WaveAppend &wn, WaveAppend &wf) {
   uint32_t v[kRelaxUnroll], wt[kRelaxUnroll];
 #pragma unroll
   for (int q = 0; q < kRelaxUnroll; q++) {
@@ -141,38 +162,40 @@ __device__ inline void relax_batch(const RelaxArgs<OffT> &a, const uint64_t (&ed
     if (tn[q] && ex[q] != a.stamp) nmask |= 1u << q;
     if (tf[q] && ex[q] == 0u) fmask |= 1u << q;
   }
-  // wave-aggregated appends: one atomic per wave and list
-  const uint32_t lane = lane_id();
+  // staged appends (LDS), flushed with one atomic when the next batch could overflow them
   const uint32_t nn = (uint32_t)__popc(nmask), nf = (uint32_t)__popc(fmask);
-  const uint32_t in = wave_inclusive_scan(nn);
-  const uint32_t tnn = (uint32_t)__shfl((int)in, kWave - 1, kWave);
+  const uint32_t in = wave_scan_dpp(nn);
+  const uint32_t tnn = read_lane_u32(in, kWave - 1);
   if (tnn) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&a.cnt->near_cnt, tnn);
-    uint32_t pos = (uint32_t)__shfl((int)base, 0, kWave) + in - nn;
+    uint32_t pos = wn.n + in - nn;
 #pragma unroll
     for (int q = 0; q < kRelaxUnroll; q++)
       if ((nmask >> q) & 1u) {
-        a.near_out[pos++] = v[q];
+        wn.buf[pos++] = v[q];
         if (a.track_shards) smask |= 1ull << (uint32_t)(((uint64_t)v[q] * kMlpShards) / a.V);
       }
+    wn.n += tnn;
+    if (wn.n > kAppendCap - kWave * kRelaxUnroll) append_flush(wn, &a.cnt->near_cnt, a.near_out);
   }
   if (a.far_out) {
-    const uint32_t inf = wave_inclusive_scan(nf);
-    const uint32_t tff = (uint32_t)__shfl((int)inf, kWave - 1, kWave);
+    const uint32_t inf = wave_scan_dpp(nf);
+    const uint32_t tff = read_lane_u32(inf, kWave - 1);
     if (tff) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(&a.cnt->far_cnt, tff);
-      uint32_t pos = (uint32_t)__shfl((int)base, 0, kWave) + inf - nf;
+      uint32_t pos = wf.n + inf - nf;
 #pragma unroll
       for (int q = 0; q < kRelaxUnroll; q++)
-        if ((fmask >> q) & 1u) a.far_out[pos++] = v[q];
+        if ((fmask >> q) & 1u) wf.buf[pos++] = v[q];
+      wf.n += tff;
+      if (wf.n > kAppendCap - kWave * kRelaxUnroll) append_flush(wf, &a.cnt->far_cnt, a.far_out);
     }
   }
 }
 
 template <class OffT>
-__device__ inline void relax_flush_counts(const RelaxArgs<OffT> &a, unsigned long long edges, unsigned long long smask) {
+__device__ inline void relax_flush_counts(const RelaxArgs<OffT> &a, unsigned long long edges, unsigned long long smask,
+                                          WaveAppend &wn, WaveAppend &wf) {
+  append_flush(wn, &a.cnt->near_cnt, a.near_out);
+  if (a.far_out) append_flush(wf, &a.cnt->far_cnt, a.far_out);
   if (lane_id() == 0 && edges) atomicAdd(&a.cnt->edges, edges);
   if (a.track_shards) {
 #pragma unroll
@@ -186,9 +209,11 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
   __shared__ uint32_t s_start[kWavesPerBlock][kWave];
   __shared__ uint64_t s_base[kWavesPerBlock][kWave];
   __shared__ uint32_t s_val[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_app[kWavesPerBlock][2][kAppendCap];
   const uint32_t lane = lane_id(), wv = wave_id();
   const uint32_t nchunks = (a.nfront + kWave - 1) / kWave;
   unsigned long long edges = 0, smask = 0;
+  WaveAppend wn{s_app[wv][0], 0u}, wf{s_app[wv][1], 0u};
   if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt->hub_cnt = 0;  // hub_compact_kernel counts after us
   for (uint32_t c = blockIdx.x * kWavesPerBlock + wv; c < nchunks; c += gridDim.x * kWavesPerBlock) {
     const uint32_t i = c * kWave + lane;
@@ -213,8 +238,8 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
         deg = 0;
       }
     }
-    const uint32_t incl = wave_inclusive_scan(deg);
-    const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
+    const uint32_t incl = wave_scan_dpp(deg);
+    const uint32_t total = read_lane_u32(incl, kWave - 1);
     s_start[wv][lane] = incl - deg;
     s_base[wv][lane] = b;
     s_val[wv][lane] = du;
@@ -238,11 +263,11 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
         e[q] = s_base[wv][lo] + (jj - s_start[wv][lo]);
         du[q] = s_val[wv][lo];
       }
-      relax_batch<OffT, MODE>(a, e, du, live, smask);
+      relax_batch<OffT, MODE>(a, e, du, live, smask, wn, wf);
     }
     wave_sync();
   }
-  relax_flush_counts(a, edges, smask);
+  relax_flush_counts(a, edges, smask, wn, wf);
 }
 
 // Lists the slices of the hubs flagged by this launch's relax_kernel (one thread per slice).
@@ -260,9 +285,11 @@ __global__ __launch_bounds__(kBlock) void hub_compact_kernel(RelaxArgs<OffT> a) 
 // spread over the whole chip.
 template <class OffT, int MODE>
 __global__ __launch_bounds__(kBlock) void hub_relax_kernel(RelaxArgs<OffT> a) {
+  __shared__ uint32_t s_app[kWavesPerBlock][2][kAppendCap];
   const uint32_t lane = lane_id(), wv = wave_id();
   const uint32_t nlist = a.cnt->hub_cnt;
   unsigned long long edges = 0, smask = 0;
+  WaveAppend wn{s_app[wv][0], 0u}, wf{s_app[wv][1], 0u};
   for (uint32_t i = blockIdx.x * kWavesPerBlock + wv; i < nlist; i += gridDim.x * kWavesPerBlock) {
     const uint2 sl = a.slices[a.hub_list[i]];
     const uint32_t u = a.hub_ids[sl.x];
@@ -280,10 +307,10 @@ __global__ __launch_bounds__(kBlock) void hub_relax_kernel(RelaxArgs<OffT> a) {
         e[q] = j < e1 ? j : e0;
         dv[q] = du;
       }
-      relax_batch<OffT, MODE>(a, e, dv, live, smask);
+      relax_batch<OffT, MODE>(a, e, dv, live, smask, wn, wf);
     }
   }
-  relax_flush_counts(a, edges, smask);
+  relax_flush_counts(a, edges, smask, wn, wf);
 }
 
 // Histograms of the live far pile for the epoch features: distance above the pile minimum in
