@@ -96,6 +96,7 @@ def run_learned(eng, eval_scale, nsources=8, train_scales=(24, 25, 26), trace_so
     rng = np.random.default_rng(2700 + eval_scale)
     srcs = [eng.pick_source(int(rng.integers(1 << 30))) for _ in range(nsources)]
     eng.sssp(srcs[0], method=gmp_const("SSSP_DELTA"), delta=64, want_dist=False)  # builds layouts
+    eng.sssp(srcs[0], method=gmp_const("SSSP_BF"), want_dist=False)  # and the source strips
     t_fix = t_lrn = 0.0
     same = True
     widths = []

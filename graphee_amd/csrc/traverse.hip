@@ -578,6 +578,18 @@ void read_counters(Engine &e, Counters *dev, Counters &h) {
   std::memcpy(&h, e.h_pinned, sizeof(Counters));
 }
 
+// Shards (64ths of the id range) holding a vertex whose value differs between before and after,
+// OR-ed into cnt->shard_mask: the trace's shard mask of an epoch relaxed without shard tracking.
+__global__ __launch_bounds__(kBlock) void changed_shards_kernel(const uint32_t *before, const uint32_t *after,
+                                                                uint32_t V, Counters *cnt) {
+  unsigned long long m = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+    if (before[v] != after[v]) m |= 1ull << (uint32_t)(((uint64_t)v * kMlpShards) / V);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m |= (unsigned long long)__shfl_xor((long long)m, o, kWave);
+  if (lane_id() == 0 && m) atomicOr(&cnt->shard_mask, m);
+}
+
 // Vertices whose stamp equals `sweep`, appended to list (count = cnt->near_cnt).
 __global__ __launch_bounds__(kBlock) void collect_stamped_kernel(const uint32_t *vstamp, uint32_t V, uint32_t sweep,
                                                                  uint32_t *list, Counters *cnt) {
@@ -595,11 +607,14 @@ __global__ __launch_bounds__(kBlock) void collect_stamped_kernel(const uint32_t 
 // GraphPS/graphps.h:336-390, on one GPU).  SSSP starts pushing from the source, CC sweeping over
 // every vertex; either may switch any number of times.  A sweep stamps the vertices it changes,
 // so its frontier is exact; entering sweeps marks only the frontier's strips.  Every phase keeps
-// relaxing until nothing drops, so the fixed point is the pure sweeps' or pure pushes'.  Returns
+// relaxing until nothing drops, so the fixed point is the pure sweeps' or pure pushes'.  With
+// front0 set, the values already in e.s_a (plain) are continued from the n0 vertices of front0
+// (one of e.s_c, e.s_d: changed, not yet propagated) instead of the starting values.  Returns
 // false, having launched nothing, when the strip layout does not apply; the values (plain, work
 // order) are left in e.s_a.
 template <class OffT>
-bool hybrid_min(Engine &e, int kind, uint32_t src, int &rounds, int &sweeps, double &ms, uint64_t &edges) {
+bool hybrid_min(Engine &e, int kind, uint32_t src, uint32_t *front0, uint32_t n0, int &rounds, int &sweeps, double &ms,
+                uint64_t &edges) {
   Graph &g = e.g;
   const uint32_t V = g.V;
   TraverseBufs b = alloc_traverse(e, false);
@@ -635,7 +650,12 @@ bool hybrid_min(Engine &e, int kind, uint32_t src, int &rounds, int &sweeps, dou
   uint32_t *qa = b.q0, *qb = b.q1;
   uint32_t n = 0, stamp = 1;
   bool pushing = false;
-  if (kind == 0 || !T) {
+  if (front0) {  // continue: a frontier larger than T switches to sweeps at once
+    qa = front0;
+    qb = front0 == b.q0 ? b.q1 : b.q0;
+    n = n0;
+    pushing = true;
+  } else if (kind == 0 || !T) {
     sweep_start(*sw, src);  // complemented starting values, every strip (CC) or the source's active
   } else {
     fill_u32_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(b.dist, V, kInf);
@@ -691,12 +711,14 @@ bool hybrid_min(Engine &e, int kind, uint32_t src, int &rounds, int &sweeps, dou
 }
 
 // Bucket epochs of delta-stepping.  FIXED: aligned buckets [k*delta, (k+1)*delta).  LEARNED:
-// the MLP picks each epoch's bucket [m, m + 8 * 2^c) from far-pile features (m = smallest live
-// far distance).  EXPLORE records traces: at checkpoint epochs, from a snapshot of the state,
-// every width class runs the epoch and then the widest class runs to the end of the solve; the
+// the MLP picks each epoch's action from far-pile features (m = smallest live far distance):
+// class c < 7 the bucket [m, m + 8 * 2^c), class 7 (kFinishClass) the rest of the solve as
+// Bellman-Ford pushes and strip sweeps (hybrid_min) from the live far pile.  EXPLORE records traces: at checkpoint epochs, from a snapshot of the state,
+// every width class runs the epoch and then fixed width 64 runs to the end of the solve; the
 // class with the shortest such rollout is the label and the run continues with it.  Every
 // policy relaxes until no distance drops, so distances are the same.
 enum Policy { kPolicyBF = 0, kPolicyFixed = 1, kPolicyLearned = 2, kPolicyExplore = 3 };
+constexpr uint32_t kFinishClass = 7;
 
 template <class OffT>
 void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gmp_trace_row *rows, uint32_t cap,
@@ -710,7 +732,7 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
     int rounds = 0, sweeps = 0;
     double ms = 0.0;
     uint64_t edges = 0;
-    if (hybrid_min<OffT>(e, unit ? 1 : 2, work_id(e, src), rounds, sweeps, ms, edges)) {
+    if (hybrid_min<OffT>(e, unit ? 1 : 2, work_id(e, src), nullptr, 0u, rounds, sweeps, ms, edges)) {
       finish_output(e, e.s_a.as<uint32_t>(), dist_out);
       if (st) {
         std::memset(st, 0, sizeof *st);
@@ -807,6 +829,25 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
     const uint64_t t = (uint64_t)m + (8ull << c);
     return t >= kInf ? kInf : (uint32_t)t;
   };
+  // the finish action: every live far entry joins the frontier and hybrid_min relaxes to the
+  // fixed point (plain pushes without the strip layout); the edge counter stays cumulative
+  auto finish = [&]() {
+    read_counters(e, b.cnt, h);
+    const unsigned long long before = h.edges;
+    split(thr, kInf);
+    int r = 0, sw = 0;
+    double fms = 0.0;
+    uint64_t fed = 0;
+    if (hybrid_min<OffT>(e, unit ? 1 : 2, 0u, near_in, nnear, r, sw, fms, fed)) {
+      const unsigned long long tot_edges = before + fed;
+      GMP_HIP(hipMemcpy(&b.cnt->edges, &tot_edges, 8, hipMemcpyHostToDevice));
+      iters += r + sw;
+      nnear = 0;
+      nfar = 0;
+    } else {
+      relax_until_empty(false);
+    }
+  };
   // snapshot of the traversal state for trace exploration
   DevBuf sd, ss, sf, sfar, scnt;
   if (policy == kPolicyExplore) {
@@ -826,15 +867,20 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
     read_counters(e, b.cnt, h);
     return h.far_min;
   };
-  // trace rollout: one epoch of width class c, then the widest class to the end of the run
+  // trace rollout: one epoch of width class c, then width 64 (class 3, the fixed schedule's) to the
+  // end of the run
   auto run_rollout = [&](uint32_t m, uint32_t c) {
+    if (c == kFinishClass) {
+      finish();
+      return;
+    }
     for (;;) {
       split(thr, width_end(m, c));
       relax_until_empty(false);
       if (nfar == 0) return;
       m = far_min();
       if (m == kInf) return;
-      c = (uint32_t)kMlpClasses - 1;
+      c = 3u;
     }
   };
   // explored epochs: the first four, then powers of two
@@ -909,8 +955,16 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
       row.best = (int32_t)cur_class;
       GMP_HIP(hipMemsetAsync(&b.cnt->shard_mask, 0, 8, e.stream));
     }
-    split(thr_old, width_end(m, cur_class));
-    relax_until_empty(explore);
+    if (cur_class == kFinishClass) {
+      finish();
+      if (explore) {  // the finish does not track shards: take them from the changed distances
+        changed_shards_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(sd.as<uint32_t>(), b.dist, V, b.cnt);
+        GMP_CHECK_LAUNCH();
+      }
+    } else {
+      split(thr_old, width_end(m, cur_class));
+      relax_until_empty(explore);
+    }
     if (explore) {
       read_counters(e, b.cnt, h);
       row.shard_mask = h.shard_mask;
@@ -944,7 +998,7 @@ void cc_impl(Engine &e, uint32_t *label_out, gmp_stats *st) {
     int rounds = 0, sweeps = 0;
     double ms = 0.0;
     uint64_t edges = 0;
-    if (hybrid_min<OffT>(e, 0, 0u, rounds, sweeps, ms, edges)) {
+    if (hybrid_min<OffT>(e, 0, 0u, nullptr, 0u, rounds, sweeps, ms, edges)) {
       finish_output(e, e.s_a.as<uint32_t>(), label_out);
       if (st) {
         std::memset(st, 0, sizeof *st);

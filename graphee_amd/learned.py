@@ -2,15 +2,16 @@
 
 The engine records a trace row at checkpoint bucket epochs of SSSP (`Engine.sssp_trace`): the
 16 features of the far pile, and for every width class the time and relaxations of finishing the
-solve from that state (one epoch of the class, then the widest class), the fastest class and the
+solve from that state (one epoch of the class, then fixed width 64; class 7 finishes the solve
+with Bellman-Ford pushes and strip sweeps), the fastest class and the
 shards that became active.  `train` fits the network with the engine's own HIP
 forward/backward/Adam kernels; `Engine.sssp(method=SSSP_LEARNED)` then runs inference on the GPU
-at every epoch.  The network only picks bucket widths, so distances equal those of every other
-method.
+at every epoch.  The network only picks how the next epoch relaxes, so distances equal those of
+every other method.
 """
 import numpy as np
 
-from . import SSSP_LEARNED, Engine  # noqa: F401
+from . import SSSP_BF, SSSP_LEARNED, Engine  # noqa: F401
 
 
 def record(engine, graphs, sources_per_graph=2, seed=0):
@@ -19,6 +20,8 @@ def record(engine, graphs, sources_per_graph=2, seed=0):
     rng = np.random.default_rng(seed)
     for scale, ef, gseed in graphs:
         engine.generate_rmat(scale, ef, gseed, weighted=True)
+        # build the layouts every action uses (out-CSR, source strips) before anything is timed
+        engine.sssp(engine.pick_source(gseed), method=SSSP_BF, want_dist=False)
         for _ in range(sources_per_graph):
             src = engine.pick_source(int(rng.integers(1 << 30)))
             r, _, _ = engine.sssp_trace(src, want_dist=False)
