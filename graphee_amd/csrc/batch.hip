@@ -77,6 +77,7 @@ __device__ inline uint32_t instance_of(const uint32_t *slot_base, uint32_t n, ui
 
 constexpr int kRelaxUnroll = 4;    // edges per lane in flight in the relax loop
 constexpr uint32_t kLdsInst = 1024;  // instance slot bases kept in LDS up to this many instances
+constexpr uint32_t kAppendCap = 512;  // staged appends per wave
 
 // One round for every instance: reads queue[round&1] and appends improved slots to the other
 // queue; the last workgroup to finish flips the round and clears the consumed count.  A wave
@@ -89,6 +90,7 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
   __shared__ uint32_t s_val[kWavesPerBlock][kWave];
   __shared__ uint32_t s_sb[kWavesPerBlock][kWave];  // slot base minus union vertex base
   __shared__ uint32_t s_islot[kLdsInst];
+  __shared__ uint32_t s_app[kWavesPerBlock][kAppendCap];
   const bool lds_inst = a.ninst <= kLdsInst;
   if (lds_inst)
     for (uint32_t i = threadIdx.x; i < a.ninst; i += blockDim.x) s_islot[i] = a.inst_slot[i];
@@ -101,6 +103,7 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
   const uint32_t lane = lane_id(), wv = wave_id();
   const uint32_t nchunks = (nfront + kWave - 1) / kWave;
   unsigned long long edges = 0;
+  WaveAppend wa{s_app[wv], 0u};
   for (uint32_t c = blockIdx.x * kWavesPerBlock + wv; c < nchunks; c += gridDim.x * kWavesPerBlock) {
     const uint32_t i = c * kWave + lane;
     uint64_t b = 0;
@@ -115,8 +118,8 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
       du = a.dist[s];
       if (du == kInf) deg = 0;
     }
-    const uint32_t incl = wave_inclusive_scan(deg);
-    const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
+    const uint32_t incl = wave_scan_dpp(deg);
+    const uint32_t total = read_lane_u32(incl, kWave - 1);
     s_start[wv][lane] = incl - deg;
     s_base[wv][lane] = b;
     s_val[wv][lane] = du;
@@ -172,22 +175,22 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
 #pragma unroll
       for (int q = 0; q < kRelaxUnroll; q++)
         if (imp[q] && old[q] != round + 1) app |= 1u << q;
-      // append the lane's improved slots: one atomic per wave
+      // the lane's improved slots, staged in the wave's LDS buffer
       const uint32_t cnt = (uint32_t)__popc(app);
-      const uint32_t inc = wave_inclusive_scan(cnt);
-      const uint32_t tot = (uint32_t)__shfl((int)inc, kWave - 1, kWave);
-      uint32_t base = 0;
+      const uint32_t inc = wave_scan_dpp(cnt);
+      const uint32_t tot = read_lane_u32(inc, kWave - 1);
       if (tot) {
-        if (lane == 0) base = atomicAdd(&a.count[nxt], tot);
-        base = (uint32_t)__shfl((int)base, 0, kWave);
-      }
-      uint32_t pos = base + inc - cnt;
+        uint32_t pos = wa.n + inc - cnt;
 #pragma unroll
-      for (int q = 0; q < kRelaxUnroll; q++)
-        if ((app >> q) & 1u) out[pos++] = slot[q];
+        for (int q = 0; q < kRelaxUnroll; q++)
+          if ((app >> q) & 1u) wa.buf[pos++] = slot[q];
+        wa.n += tot;
+        if (wa.n > kAppendCap - kWave * kRelaxUnroll) append_flush(wa, &a.count[nxt], out);
+      }
     }
     wave_sync();
   }
+  append_flush(wa, &a.count[nxt], out);
   if (lane == 0 && edges) atomicAdd(a.edges, edges);
   __syncthreads();
   if (threadIdx.x == 0) {

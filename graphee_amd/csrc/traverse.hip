@@ -88,25 +88,8 @@ struct RelaxArgs {
 
 constexpr int kRelaxUnroll = 4;  // edges per lane in flight in the relax loops
 
-// Appends of improved targets are staged in a per-wave LDS buffer and written out with one
-// atomic per flush (when the buffer could overflow on the next batch, and at the end of the
-// kernel), coalesced, instead of one atomic per wave and batch.
+// Appends of improved targets are staged per wave in LDS (WaveAppend, wave.hpp).
 constexpr uint32_t kAppendCap = 512;  // entries per wave and list
-struct WaveAppend {
-  uint32_t *buf;  // this wave's LDS slots
-  uint32_t n;     // entries held (the same in every lane)
-};
-
-__device__ inline void append_flush(WaveAppend &w, uint32_t *count, uint32_t *out) {
-  if (!w.n) return;
-  wave_sync();
-  uint32_t base = 0;
-  if (lane_id() == 0) base = atomicAdd(count, w.n);
-  base = read_lane_u32(base, 0);
-  for (uint32_t i = lane_id(); i < w.n; i += kWave) out[base + i] = w.buf[i];
-  wave_sync();
-  w.n = 0;
-}
 
 // Relaxes up to kRelaxUnroll edges per lane (edge[q], source value du[q]; entries with bit q
 // of `live` clear are skipped) and appends the improved targets to the near frontier (first

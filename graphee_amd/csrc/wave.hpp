@@ -211,6 +211,25 @@ __device__ inline uint32_t wave_append(bool pred, uint32_t *count) {
   return base + (uint32_t)__popcll(below);
 }
 
+// Appends staged in a per-wave LDS buffer and written out with one atomic per flush (when the
+// buffer could overflow on the next batch, and at the end of the kernel), coalesced, instead of
+// one atomic per wave and batch: a single append counter serialises its atomics.
+struct WaveAppend {
+  uint32_t *buf;  // this wave's LDS slots
+  uint32_t n;     // entries held (the same in every lane)
+};
+
+__device__ inline void append_flush(WaveAppend &w, uint32_t *count, uint32_t *out) {
+  if (!w.n) return;
+  wave_sync();
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(count, w.n);
+  base = read_lane_u32(base, 0);
+  for (uint32_t i = lane_id(); i < w.n; i += kWave) out[base + i] = w.buf[i];
+  wave_sync();
+  w.n = 0;
+}
+
 // Block-aggregated append: every thread of the block calls it (block-uniform control flow) with
 // its number of entries; returns the first of its consecutive slots, with one atomic on `count`
 // per block.  `sh` is WAVES + 1 words of LDS.
