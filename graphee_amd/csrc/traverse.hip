@@ -573,14 +573,24 @@ __global__ __launch_bounds__(kBlock) void changed_shards_kernel(const uint32_t *
   if (lane_id() == 0 && m) atomicOr(&cnt->shard_mask, m);
 }
 
-// Vertices whose stamp equals `sweep`, appended to list (count = cnt->near_cnt).
+// Vertices whose stamp equals `sweep`, appended to list (count = cnt->near_cnt): kCollectPer
+// vertices per thread and one atomic per block and pass.
+constexpr uint32_t kCollectPer = 16;
 __global__ __launch_bounds__(kBlock) void collect_stamped_kernel(const uint32_t *vstamp, uint32_t V, uint32_t sweep,
                                                                  uint32_t *list, Counters *cnt) {
-  for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < V; v0 += gridDim.x * blockDim.x) {
-    const uint32_t v = v0 + threadIdx.x;
-    const bool p = v < V && vstamp[v] == sweep;
-    const uint32_t pos = wave_append(p, &cnt->near_cnt);
-    if (p) list[pos] = v;
+  __shared__ uint32_t sh[kWavesPerBlock + 1];
+  const uint64_t per_block = (uint64_t)kBlock * kCollectPer;
+  for (uint64_t b0 = blockIdx.x * per_block; b0 < V; b0 += (uint64_t)gridDim.x * per_block) {
+    uint32_t hit = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kCollectPer; k++) {
+      const uint64_t v = b0 + k * kBlock + threadIdx.x;
+      if (v < V && vstamp[v] == sweep) hit |= 1u << k;
+    }
+    uint32_t pos = block_append<kWavesPerBlock>((uint32_t)__popc(hit), &cnt->near_cnt, sh);
+#pragma unroll
+    for (uint32_t k = 0; k < kCollectPer; k++)
+      if ((hit >> k) & 1u) list[pos++] = (uint32_t)(b0 + k * kBlock + threadIdx.x);
   }
 }
 
