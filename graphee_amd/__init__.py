@@ -99,7 +99,7 @@ class TuneResult(ctypes.Structure):
                 ("vec_min", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32), ("trials", ctypes.c_uint32),
                 ("gather_ms_before", ctypes.c_double), ("gather_ms_after", ctypes.c_double),
                 ("strip_log", ctypes.c_uint32), ("bin_log", ctypes.c_uint32), ("bin_task_log", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("push_log", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -2,7 +2,7 @@
 // and the bin-task size of the strip layout (or the row gather, strip width 0), and for the row
 // gather the degree-bin threshold (vec_min), the LDS tile (stream_cap rows' edges per wave), the
 // hub chunk size and the workgroups per CU, on the loaded graph, by measuring iterations with
-// HIP events.  GraphMP fixes the matching
+// HIP events; then the push/sweep threshold of CC and Bellman-Ford by timing Bellman-Ford.  GraphMP fixes the matching
 // knobs at compile time (CMPNUM threads, GraphMP/system/Global.h:58); nothing here changes
 // results, only the task plan and the grid.
 //
@@ -29,6 +29,15 @@ double time_gather(Engine &e, int iters) {
   run_pagerank(e, p, nullptr, &st);  // warm (plan build, caches)
   run_pagerank(e, p, nullptr, &st);
   return st.launches ? st.ms_kernel / st.launches : 0.0;
+}
+
+// Bellman-Ford solve time from src (stored weights, or unit weights on an unweighted graph).
+double time_bf(Engine &e, uint32_t src) {
+  gmp_sssp_params p{GMP_SSSP_BF, 0, 0, 0};
+  gmp_stats st{};
+  run_sssp(e, src, p, nullptr, &st);  // warm (layouts)
+  run_sssp(e, src, p, nullptr, &st);
+  return st.ms_total;
 }
 
 uint32_t *field(Options &o, int k) {
@@ -94,6 +103,26 @@ void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
     if (!improved) break;
   }
   apply(e, best);
+  // the push/sweep switch of CC and Bellman-Ford (strip layout only): Bellman-Ford from the
+  // vertex with the most out-edges for each threshold
+  if (best.strip_log != 0 && e.g.V > 1) {
+    std::vector<uint32_t> od(e.g.V);
+    GMP_HIP(hipMemcpy(od.data(), e.g.out_deg.p, (size_t)e.g.V * 4, hipMemcpyDeviceToHost));
+    const uint32_t src = (uint32_t)(std::max_element(od.begin(), od.end()) - od.begin());
+    double best_bf = time_bf(e, src);
+    trials++;
+    for (uint32_t v : {0u, 6u, 8u, 10u, 12u, 14u}) {
+      if (v == best.push_log) continue;
+      e.opt.push_log = v;
+      const double ms = time_bf(e, src);
+      trials++;
+      if (ms > 0 && ms < best_bf * 0.98) {
+        best_bf = ms;
+        best.push_log = v;
+      }
+    }
+    e.opt.push_log = best.push_log;
+  }
   if (out) {
     std::memset(out, 0, sizeof *out);
     out->strip_log = best.strip_log;
@@ -107,6 +136,7 @@ void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
     out->trials = trials;
     out->gather_ms_before = start_ms;
     out->gather_ms_after = best_ms;
+    out->push_log = best.push_log;
   }
 }
 
@@ -116,7 +146,8 @@ void tune_save(Engine &e, const std::string &path) {
   f << "{\"reorder\": " << e.opt.reorder << ", \"stream_cap\": " << e.opt.stream_cap
     << ", \"row_cap\": " << e.opt.row_cap << ", \"chunk\": " << e.opt.chunk << ", \"vec_min\": " << e.opt.vec_min
     << ", \"wg_per_cu\": " << e.opt.wg_per_cu << ", \"strip_log\": " << e.opt.strip_log
-    << ", \"bin_log\": " << e.opt.bin_log << ", \"bin_task_log\": " << e.opt.bin_task_log << "}\n";
+    << ", \"bin_log\": " << e.opt.bin_log << ", \"bin_task_log\": " << e.opt.bin_task_log
+    << ", \"push_log\": " << e.opt.push_log << "}\n";
   if (!f) fail(GMP_EIO, path + ": short write");
 }
 
@@ -148,6 +179,7 @@ void tune_load(Engine &e, const std::string &path) {
   get("strip_log", 0, 15, o.strip_log);
   get("bin_log", 8, 14, o.bin_log);
   get("bin_task_log", 8, 22, o.bin_task_log);
+  get("push_log", 0, 31, o.push_log);
   if (o.strip_log != 0 && o.strip_log < 8) fail(GMP_EFORMAT, path + ": strip_log out of range");
   if ((int)reorder != e.opt.reorder) {
     e.g.work_ready = false;

@@ -141,12 +141,13 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 
 /* Autotuner (no reference counterpart: GraphMP's knobs are compile-time constants): times the
  * PageRank gather on the loaded graph for candidate bin thresholds, tile sizes, chunk sizes
- * and workgroup counts (coordinate descent, `iters` iterations per trial), applies the best
- * and reports it.  Results of every app are unchanged by any choice. */
+ * and workgroup counts (coordinate descent, `iters` iterations per trial), then Bellman-Ford
+ * for candidate GMP_OPT_PUSH_LOG values, applies the best and reports it.  Results of every
+ * app are unchanged by any choice. */
 typedef struct gmp_tune_result {
   uint32_t stream_cap, row_cap, chunk, vec_min, wg_per_cu, trials;
   double gather_ms_before, gather_ms_after; /* mean PageRank iteration time, first and best */
-  uint32_t strip_log, bin_log, bin_task_log, reserved;
+  uint32_t strip_log, bin_log, bin_task_log, push_log;
 } gmp_tune_result;
 int gmp_autotune(gmp_engine *h, int32_t iters, gmp_tune_result *out);
 /* Tunables as a flat JSON object, written after tuning and read at start-up. */

@@ -419,7 +419,7 @@ def test_source_strips_keep_results(engine, strip_log, bin_log, task_log):
 
 def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
     keys = (gmp.OPT_STREAM_CAP, gmp.OPT_ROW_CAP, gmp.OPT_CHUNK, gmp.OPT_WG_PER_CU, gmp.OPT_VEC_MIN,
-            gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG)
+            gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG, gmp.OPT_PUSH_LOG)
     saved = [engine.get_option(k) for k in keys]
     try:
         g = gen(engine, 14)
@@ -427,7 +427,9 @@ def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
         res = engine.autotune(iters=2)
         assert res.trials > 1 and res.gather_ms_after <= res.gather_ms_before
         assert [engine.get_option(k) for k in keys] == [res.stream_cap, res.row_cap, res.chunk, res.wg_per_cu,
-                                                         res.vec_min, res.strip_log, res.bin_log, res.bin_task_log]
+                                                         res.vec_min, res.strip_log, res.bin_log, res.bin_task_log,
+                                                         res.push_log]
+        assert res.push_log in (0, 6, 8, 10, 12, 14)
         assert res.strip_log in (0, 13, 14, 15)
         r, _ = engine.pagerank(max_iter=20)
         x, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
@@ -439,6 +441,7 @@ def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
         engine.tune_load(p)
         assert engine.get_option(gmp.OPT_CHUNK) == res.chunk
         assert engine.get_option(gmp.OPT_STRIP_LOG) == res.strip_log
+        assert engine.get_option(gmp.OPT_PUSH_LOG) == res.push_log
         p.write_text('{"chunk": 3}')
         with pytest.raises(gmp.GmpError) as ei:
             engine.tune_load(p)
