@@ -20,7 +20,7 @@ GRAPH_OFF64 = 1
 EDGES_WEIGHTED, EDGES_SYMMETRIZE = 1, 2
 OPT_REORDER, OPT_STREAM_CAP, OPT_ROW_CAP, OPT_CHUNK, OPT_WG_PER_CU, OPT_VEC_MIN = 1, 2, 3, 4, 5, 6
 OPT_STRIP_LOG, OPT_BIN_LOG, OPT_BIN_TASK_LOG = 7, 8, 9
-OPT_EDGE_BUDGET_MB, OPT_WINDOW_MB, OPT_WINDOW_PACKED, OPT_PUSH_LOG = 10, 11, 12, 13
+OPT_EDGE_BUDGET_MB, OPT_WINDOW_MB, OPT_WINDOW_PACKED, OPT_PUSH_LOG, OPT_CC_UNION_FIND = 10, 11, 12, 13, 14
 
 # every entry point include/gmp.h declares
 SYMBOLS = (

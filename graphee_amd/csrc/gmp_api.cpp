@@ -140,6 +140,10 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       in(0, 31, "push_log");
       e->opt.push_log = (uint32_t)value;
       break;
+    case GMP_OPT_CC_UNION_FIND:
+      in(0, 1, "cc_union_find");
+      e->opt.cc_union_find = (int)value;
+      break;
     default:
       fail(GMP_EINVAL, "gmp_set_option: unknown key " + std::to_string(key));
   }
@@ -164,6 +168,7 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_WINDOW_MB: *value = e->opt.window_mb; break;
     case GMP_OPT_WINDOW_PACKED: *value = e->opt.window_packed; break;
     case GMP_OPT_PUSH_LOG: *value = e->opt.push_log; break;
+    case GMP_OPT_CC_UNION_FIND: *value = e->opt.cc_union_find; break;
     default: fail(GMP_EINVAL, "gmp_get_option: unknown key " + std::to_string(key));
   }
   GMP_API_END
@@ -203,6 +208,7 @@ int gmp_load_edge_list(gmp_engine *h, const char *path, uint32_t V, uint32_t P, 
   e->loaded = false;
   gmp::graph_upload(*e, V, g.off[V], g.off.data(), g.idx.data(), g.w.empty() ? nullptr : g.w.data(), false);
   gmp::graph_finish(*e, np, bounds.data());
+  e->g.symmetric = (flags & GMP_EDGES_SYMMETRIZE) != 0;  // every edge was added both ways
   e->loaded = true;
   GMP_API_END
 }

@@ -248,6 +248,7 @@ struct Options {
   uint32_t window_mb = 64;     // semi-external windows: id bytes per window
   int window_packed = 1;       // semi-external windows: varint-packed ids
   uint32_t push_log = 10;      // CC / Bellman-Ford: frontiers of at most V >> push_log push (0: sweeps only)
+  int cc_union_find = 1;       // CC of a symmetric graph by sampled union-find (0: sweeps and pushes)
 };
 
 // Many graphs resident at once as one disjoint union (graph k owns union ids

@@ -136,6 +136,9 @@ int gmp_destroy(gmp_engine *h);
                                      pushed over the out-edges, larger ones swept over the source
                                      strips (GraphPS's sparse/dense switch); 0 = sweeps only,
                                      0..31 (default 10).  Results do not depend on it. */
+#define GMP_OPT_CC_UNION_FIND 14   /* 1 (default): gmp_cc on a symmetric graph (undirected R-MAT,
+                                     edge lists loaded with GMP_EDGES_SYMMETRIZE) by sampled
+                                     union-find; 0: min-label sweeps and pushes.  Same labels. */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 

@@ -214,6 +214,7 @@ def test_cc_sweeps_and_frontier_agree(engine, strip_log, bin_log, task_log):
     components with a long path (many sweeps) and with a hub."""
     for k, v in zip((gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG), (strip_log, bin_log, task_log)):
         engine.set_option(k, v)
+    engine.set_option(gmp.OPT_CC_UNION_FIND, 0)  # the sweeps, also on the symmetric R-MAT graph
     try:
         g = gen(engine, 12, undirected=True)
         say(f"cc strips {strip_log}/{bin_log}/{task_log}")
@@ -230,8 +231,45 @@ def test_cc_sweeps_and_frontier_agree(engine, strip_log, bin_log, task_log):
         np.testing.assert_array_equal(lab, oracle.cc_unionfind(off, idx))
         assert st.iterations >= (3000 if strip_log == 0 else 2)
     finally:
+        engine.set_option(gmp.OPT_CC_UNION_FIND, 1)
         for k, v in STRIP_DEFAULTS:
             engine.set_option(k, v)
+
+
+@pytest.mark.parametrize("scale,seed", [(10, 3), (13, 4), (15, 5)])
+def test_cc_union_find_on_symmetric_graphs(engine, tmp_path, scale, seed):
+    """CC of a symmetric graph by sampled union-find (the default there) gives the union-find
+    oracle's labels -- each component's least original id -- and the sweeps' labels; also for an
+    edge list loaded with symmetrize, and with the degree-ordered layout off."""
+    g = gen(engine, scale, seed=seed, undirected=True)
+    want = oracle.cc_unionfind(g["off"], g["idx"])
+    say(f"cc union-find s{scale}")
+    lab, st = engine.cc()
+    np.testing.assert_array_equal(lab, want)
+    assert st.edges < g["idx"].size  # the giant component skips its remaining edges
+    engine.set_option(gmp.OPT_CC_UNION_FIND, 0)
+    try:
+        lab2, _ = engine.cc()
+    finally:
+        engine.set_option(gmp.OPT_CC_UNION_FIND, 1)
+    np.testing.assert_array_equal(lab2, want)
+    engine.set_option(gmp.OPT_REORDER, 0)
+    try:
+        lab3, _ = engine.cc()
+    finally:
+        engine.set_option(gmp.OPT_REORDER, 1)
+    np.testing.assert_array_equal(lab3, want)
+    # many small components plus a path, from a symmetrized edge list
+    rng = np.random.default_rng(seed)
+    V = 5000
+    src = np.concatenate([rng.integers(0, V, 3000), np.arange(4000, 4999)])
+    dst = np.concatenate([rng.integers(0, V, 3000), np.arange(4001, 5000)])
+    p = tmp_path / "e.tsv"
+    p.write_text("".join(f"{a} {b}\n" for a, b in zip(src, dst)))
+    engine.load_edge_list(p, V=V, symmetrize=True)
+    h = engine.download()
+    lab4, _ = engine.cc()
+    np.testing.assert_array_equal(lab4, oracle.cc_unionfind(h["off"], h["idx"]))
 
 
 @pytest.mark.parametrize("push_log", [0, 1, 3, 8])
