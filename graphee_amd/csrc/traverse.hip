@@ -690,7 +690,8 @@ bool hybrid_min(Engine &e, int kind, uint32_t src, uint32_t *front0, uint32_t n0
         collect_stamped_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(vstamp, V, sweep_last(*sw), qa, b.cnt);
         GMP_CHECK_LAUNCH();
         complement_values(e, b.dist, V);
-        n = changed;
+        read_counters(e, b.cnt, h);
+        n = h.near_cnt;  // the frontier the list holds (every changed vertex is stamped once)
         pushing = true;
       }
     }

@@ -490,11 +490,16 @@ def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
 
 
 def test_option_range_checks(engine):
-    for k, v in ((gmp.OPT_STREAM_CAP, 8192), (gmp.OPT_REORDER, 2), (gmp.OPT_WG_PER_CU, 0), (99, 1)):
+    for k, v in ((gmp.OPT_STREAM_CAP, 8192), (gmp.OPT_REORDER, 2), (gmp.OPT_WG_PER_CU, 0), (99, 1),
+                 (gmp.OPT_PUSH_LOG, 32), (gmp.OPT_CC_UNION_FIND, 2), (gmp.OPT_WINDOW_MB, 0),
+                 (gmp.OPT_WINDOW_PACKED, 2), (gmp.OPT_EDGE_BUDGET_MB, -1)):
         with pytest.raises(gmp.GmpError) as ei:
             engine.set_option(k, v)
         assert ei.value.code == gmp.EINVAL
     assert engine.get_option(gmp.OPT_CHUNK) == 4096
+    # defaults of the traversal and semi-external options
+    assert [engine.get_option(k) for k in (gmp.OPT_PUSH_LOG, gmp.OPT_CC_UNION_FIND, gmp.OPT_EDGE_BUDGET_MB,
+                                           gmp.OPT_WINDOW_MB, gmp.OPT_WINDOW_PACKED)] == [10, 1, 0, 64, 1]
 
 
 # ---- shard files and edge cases --------------------------------------------------------
