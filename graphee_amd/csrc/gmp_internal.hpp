@@ -183,6 +183,8 @@ struct SemiExternal {
   DevBuf srcblk;             // uint64[nwin * blk_words]: source blocks of every window
   DevBuf stamp;              // uint32[nblk]: last sweep that changed a vertex of the block
   DevBuf active;             // uint8[nwin]
+  DevBuf first;              // uint32[2V]: every row's first two ids (union-find CC), built on demand
+  DevBuf wrow;               // uint32[nwin + 1]: first row of every window
   std::vector<hipStream_t> copy;  // copy streams of the ring slots
   std::vector<hipEvent_t> ready, freed;
   uint64_t slot_seq = 0;

@@ -22,6 +22,7 @@
 #include <cstring>
 
 #include "gmp_internal.hpp"
+#include "unionfind.hpp"
 #include "wave.hpp"
 
 namespace gmp {
@@ -982,126 +983,6 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
   }
 }
 
-// ---- connected components of a symmetric graph by sampled union-find ----
-// Afforest (Sutton, Ben-Nun, Barak: "Optimizing Parallel Graph Connectivity Computation via
-// Subgraph Sampling", IPDPS 2018): every vertex is linked to its first kAfSample neighbours, the
-// trees are compressed, the component holding most of a vertex sample is taken as the giant
-// one, and only vertices outside it link their remaining neighbours -- on a symmetric graph an
-// edge into the giant component is also an edge out of the other endpoint, so nothing is lost.
-// Trees hang from their least work id; the label of a component is its least original id, the
-// fixed point of GraphMP's min-label rule (cc.cpp:238-244).
-constexpr uint32_t kAfSample = 2;
-constexpr uint32_t kAfProbe = 1024;
-
-// Hooks the trees of u and v: the larger root is swung to the smaller with a compare-and-swap;
-// otherwise both climb.  Every value ever stored in p[x] is <= x, so whatever (possibly stale)
-// values the plain loads return, max(a, b) drops on every pass and the loop ends; after a failed
-// swing the climb starts from the value the compare-and-swap found, not from a reload.
-__device__ inline void af_link(uint32_t *p, uint32_t u, uint32_t v) {
-  uint32_t a = p[u], b = p[v];
-  while (a != b) {
-    const uint32_t hi = max(a, b), lo = min(a, b);
-    uint32_t ph = p[hi];
-    if (ph == lo) return;  // hi hangs from lo: the trees are one
-    if (ph == hi) {
-      ph = atomicCAS(&p[hi], hi, lo);
-      if (ph == hi) return;
-    }
-    a = p[ph];  // ph < hi
-    b = p[lo];
-  }
-}
-
-__global__ void af_init_kernel(uint32_t *p, uint32_t V) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) p[v] = v;
-}
-
-template <class OffT>
-__global__ void af_sample_kernel(const OffT *off, const uint32_t *idx, uint32_t V, uint32_t r, uint32_t *p) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
-    const uint64_t b = off[v];
-    if ((uint64_t)off[v + 1] - b > r) af_link(p, v, idx[b + r]);
-  }
-}
-
-// Pointer jumping until every vertex points at its root.
-__global__ void af_compress_kernel(uint32_t *p, uint32_t V) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
-    uint32_t x = p[v];
-    while (p[x] != x) x = p[x];
-    p[v] = x;
-  }
-}
-
-// Roots of kAfProbe vertices spread over the id range (after compression p is the root).
-__global__ void af_probe_kernel(const uint32_t *p, uint32_t V, uint32_t *out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kAfProbe) out[i] = p[(uint32_t)(((uint64_t)mix64(i + 1) >> 11) % V)];
-}
-
-// Vertices outside the giant component link their neighbours past the sampled ones; one wave
-// per vertex above a wave's worth of edges, one lane per vertex below.
-template <class OffT>
-__global__ __launch_bounds__(kBlock) void af_finish_kernel(const OffT *off, const uint32_t *idx, uint32_t V,
-                                                           uint32_t giant, uint32_t *p,
-                                                           unsigned long long *edges) {
-  const uint32_t lane = lane_id();
-  unsigned long long n = 0;
-  for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < V; v0 += gridDim.x * blockDim.x) {
-    const uint32_t v = v0 + threadIdx.x;
-    uint64_t b = 0, en = 0;
-    bool mine = false;
-    if (v < V && p[v] != giant) {
-      b = (uint64_t)off[v] + kAfSample;
-      en = off[v + 1];
-      mine = en > b;
-    }
-    const bool wide = mine && en - b > kWave;
-    if (mine && !wide) {
-      for (uint64_t k = b; k < en; k++) af_link(p, v, idx[k]);
-      n += en - b;
-    }
-    // long lists: the wave takes them one at a time, a lane per edge
-    uint64_t wmask = __ballot(wide);
-    while (wmask) {
-      const uint32_t l = (uint32_t)__ffsll((long long)wmask) - 1u;
-      wmask &= wmask - 1;
-      const uint32_t u = read_lane_u32(v, (int)l);
-      const uint64_t lb = ((uint64_t)read_lane_u32((uint32_t)(b >> 32), (int)l) << 32) | read_lane_u32((uint32_t)b, (int)l);
-      const uint64_t le = ((uint64_t)read_lane_u32((uint32_t)(en >> 32), (int)l) << 32) | read_lane_u32((uint32_t)en, (int)l);
-      for (uint64_t k = lb + lane; k < le; k += kWave) af_link(p, u, idx[k]);
-      if (lane == 0) n += le - lb;
-    }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) n += (unsigned long long)__shfl_xor((long long)n, o, kWave);
-  if (lane == 0 && n) atomicAdd(edges, n);
-}
-
-// minorig[root] = least original id of the component; then label[v] = minorig[p[v]].  The giant
-// component's minimum is reduced per block first (one atomic per block on its root).
-__global__ __launch_bounds__(kBlock) void af_min_kernel(const uint32_t *p, const uint32_t *n2o, uint32_t V,
-                                                        uint32_t giant, uint32_t *minorig) {
-  __shared__ uint32_t sm[kWavesPerBlock];
-  uint32_t g = kInf;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
-    const uint32_t r = p[v], o = n2o ? n2o[v] : v;
-    if (r == giant) g = min(g, o);
-    else atomicMin(&minorig[r], o);
-  }
-  g = wave_reduce_min(g);
-  if (lane_id() == 0) sm[wave_id()] = g;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < kWavesPerBlock; w++) g = min(g, sm[w]);
-    if (g != kInf) atomicMin(&minorig[giant], g);
-  }
-}
-
-__global__ void af_label_kernel(const uint32_t *p, const uint32_t *minorig, uint32_t V, uint32_t *label) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) label[v] = minorig[p[v]];
-}
-
 // Labels (work order, original-id values) into e.s_a.
 template <class OffT>
 void afforest(Engine &e, int &phases, double &ms, uint64_t &edges) {
@@ -1131,18 +1012,8 @@ void afforest(Engine &e, int &phases, double &ms, uint64_t &edges) {
   std::vector<uint32_t> roots(kAfProbe);
   GMP_HIP(hipMemcpyAsync(roots.data(), probe, kAfProbe * 4, hipMemcpyDeviceToHost, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
-  std::sort(roots.begin(), roots.end());
-  uint32_t giant = roots[0], best = 0;
-  for (size_t i = 0; i < roots.size();) {
-    size_t j = i;
-    while (j < roots.size() && roots[j] == roots[i]) j++;
-    if (j - i > best) {
-      best = (uint32_t)(j - i);
-      giant = roots[i];
-    }
-    i = j;
-  }
-  af_finish_kernel<OffT><<<gv, kBlock, 0, e.stream>>>(off, w.idx, V, giant, p, dedges);
+  const uint32_t giant = af_giant(roots);
+  af_finish_kernel<OffT><<<gv, kBlock, 0, e.stream>>>(off, w.idx, 0u, V, giant, p, dedges);
   af_compress_kernel<<<gv, kBlock, 0, e.stream>>>(p, V);
   // label first holds each root's least original id (slot = root), then every vertex takes its
   // root's: a root's slot is rewritten with its own value, other slots are never read as roots

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "gmp_internal.hpp"
+#include "unionfind.hpp"
 #include "varint.hpp"
 #include "wave.hpp"
 
@@ -310,6 +311,114 @@ void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st) {
     st->edges = edges;
     st->bytes = edges * 8;
     st->policy = 0;
+  }
+}
+
+// The first kAfSample column ids of rows [r0, r1) (ids by global edge index), kAfNone where a row
+// is shorter.
+template <class OffT>
+__global__ void first_ids_kernel(const OffT *off, const uint32_t *idx, uint32_t r0, uint32_t r1, uint32_t *first) {
+  for (uint32_t v = r0 + blockIdx.x * blockDim.x + threadIdx.x; v < r1; v += gridDim.x * blockDim.x) {
+    const uint64_t b = off[v], d = (uint64_t)off[v + 1] - b;
+#pragma unroll
+    for (uint32_t r = 0; r < kAfSample; r++) first[(uint64_t)v * kAfSample + r] = r < d ? idx[b + r] : kAfNone;
+  }
+}
+
+// flags[w] = 1 when a row of window w lies outside the giant component and has edges past the
+// sampled ones (one block per window).
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void finish_flags_kernel(const OffT *off, const uint32_t *p, uint32_t giant,
+                                                              const uint32_t *wrow, uint8_t *flags) {
+  const uint32_t r0 = wrow[blockIdx.x], r1 = wrow[blockIdx.x + 1];
+  int hit = 0;
+  for (uint32_t v = r0 + threadIdx.x; v < r1 && !hit; v += blockDim.x)
+    if (p[v] != giant && (uint64_t)off[v + 1] - (uint64_t)off[v] > kAfSample) hit = 1;
+  hit = __syncthreads_or(hit);
+  if (threadIdx.x == 0) flags[blockIdx.x] = hit ? 1 : 0;
+}
+
+// CC of a symmetric semi-external graph by sampled union-find: the sampled neighbours come from a
+// table of every row's first ids (extracted by one pass over the windows and kept: vertex-sized
+// data), and the finish phase streams only the windows flagged above.
+template <class OffT>
+void semi_cc_union_find(Engine &e, uint32_t *label_out, gmp_stats *st) {
+  Graph &g = e.g;
+  SemiExternal &s = semi_of(e);
+  const uint32_t V = g.V, nwin = (uint32_t)s.win.size();
+  const OffT *off = g.in.off.as<OffT>();
+  if (!s.first.p) {
+    s.first.alloc((size_t)V * kAfSample * 4);
+    std::vector<uint32_t> wrow(nwin + 1);
+    for (uint32_t w = 0; w < nwin; w++) wrow[w] = s.win[w].r0;
+    wrow[nwin] = V;
+    s.wrow.alloc((size_t)(nwin + 1) * 4);
+    GMP_HIP(hipMemcpy(s.wrow.p, wrow.data(), (size_t)(nwin + 1) * 4, hipMemcpyHostToDevice));
+    for (uint32_t w = 0; w < nwin; w++) {
+      const EdgeWindow &x = s.win[w];
+      const WindowDev d = window_begin(e, w, false, true);
+      first_ids_kernel<OffT><<<grid_for(x.r1 - x.r0), kBlock, 0, e.stream>>>(off, d.idx, x.r0, x.r1,
+                                                                              s.first.as<uint32_t>());
+      GMP_CHECK_LAUNCH();
+      window_end(e, d);
+    }
+    GMP_HIP(hipStreamSynchronize(e.stream));
+  }
+  e.s_a.alloc((size_t)V * 4);
+  e.s_b.alloc((size_t)V * 4);
+  e.s_c.alloc((size_t)kAfProbe * 4 + 8);
+  uint32_t *label = e.s_a.as<uint32_t>(), *p = e.s_b.as<uint32_t>(), *probe = e.s_c.as<uint32_t>();
+  unsigned long long *dedges = reinterpret_cast<unsigned long long *>(probe + kAfProbe);
+  const uint32_t gv = grid_for(V);
+  EventPair tot;
+  GMP_HIP(hipEventRecord(tot.a, e.stream));
+  af_init_kernel<<<gv, kBlock, 0, e.stream>>>(p, V);
+  GMP_CHECK_LAUNCH();
+  GMP_HIP(hipMemsetAsync(dedges, 0, 8, e.stream));
+  for (uint32_t r = 0; r < kAfSample; r++) {
+    af_sample_table_kernel<<<gv, kBlock, 0, e.stream>>>(s.first.as<uint32_t>(), V, r, p);
+    af_compress_kernel<<<gv, kBlock, 0, e.stream>>>(p, V);
+  }
+  af_probe_kernel<<<(kAfProbe + kBlock - 1) / kBlock, kBlock, 0, e.stream>>>(p, V, probe);
+  GMP_CHECK_LAUNCH();
+  std::vector<uint32_t> roots(kAfProbe);
+  GMP_HIP(hipMemcpyAsync(roots.data(), probe, kAfProbe * 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  const uint32_t giant = af_giant(roots);
+  finish_flags_kernel<OffT><<<nwin, kBlock, 0, e.stream>>>(off, p, giant, s.wrow.as<uint32_t>(), s.active.as<uint8_t>());
+  GMP_CHECK_LAUNCH();
+  std::vector<uint8_t> flags(nwin);
+  GMP_HIP(hipMemcpyAsync(flags.data(), s.active.p, nwin, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  for (uint32_t w = 0; w < nwin; w++) {
+    const EdgeWindow &x = s.win[w];
+    if (!flags[w]) {
+      s.windows_skipped++;
+      continue;
+    }
+    const WindowDev d = window_begin(e, w, false, true);
+    af_finish_kernel<OffT><<<grid_for(x.r1 - x.r0), kBlock, 0, e.stream>>>(off, d.idx, x.r0, x.r1, giant, p, dedges);
+    GMP_CHECK_LAUNCH();
+    window_end(e, d);
+  }
+  af_compress_kernel<<<gv, kBlock, 0, e.stream>>>(p, V);
+  fill_kernel<<<gv, kBlock, 0, e.stream>>>(label, V, kInf);
+  af_min_kernel<<<gv, kBlock, 0, e.stream>>>(p, nullptr, V, giant, label);
+  af_label_kernel<<<gv, kBlock, 0, e.stream>>>(p, label, V, label);
+  GMP_CHECK_LAUNCH();
+  GMP_HIP(hipEventRecord(tot.b, e.stream));
+  uint64_t h = 0;
+  GMP_HIP(hipMemcpyAsync(&h, dedges, 8, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  if (label_out) GMP_HIP(hipMemcpy(label_out, label, (size_t)V * 4, hipMemcpyDeviceToHost));
+  if (st) {
+    std::memset(st, 0, sizeof *st);
+    st->iterations = (int)kAfSample + 1;
+    st->launches = st->iterations;
+    st->ms_total = tot.ms();
+    st->ms_kernel = st->ms_total;
+    st->edges = (uint64_t)V * kAfSample + h;
+    st->bytes = st->edges * 8;
   }
 }
 
@@ -666,6 +775,11 @@ const uint8_t *window_activity(Engine &e, uint32_t sweep) {
 }
 
 void semi_cc(Engine &e, uint32_t *label_out, gmp_stats *st) {
+  if (e.g.symmetric && e.opt.cc_union_find) {
+    if (e.g.off64) semi_cc_union_find<uint64_t>(e, label_out, st);
+    else semi_cc_union_find<uint32_t>(e, label_out, st);
+    return;
+  }
   if (e.g.off64) semi_min<uint64_t>(e, kMinLabel, 0, label_out, st);
   else semi_min<uint32_t>(e, kMinLabel, 0, label_out, st);
 }

@@ -156,6 +156,34 @@ def test_windows_skip_idle_windows_exactly(packed):
         assert s.windows()["windows_skipped"] > w2["windows_skipped"]
 
 
+@pytest.mark.parametrize("packed", [False, True])
+def test_windows_union_find_cc_streams_what_it_needs(tmp_path, packed):
+    """CC of a symmetric semi-external graph by sampled union-find: two large components (the
+    second one's rows have more than two edges, so the finish phase streams their windows) and
+    isolated vertices, against the union-find oracle and the sweeps."""
+    rng = np.random.default_rng(8)
+    V = 200000
+    h = V // 2
+    a = np.concatenate([rng.integers(0, h - 1000, 400000), rng.integers(h, V - 1000, 400000)])
+    b = np.concatenate([rng.integers(0, h - 1000, 400000), rng.integers(h, V - 1000, 400000)])
+    p = tmp_path / "two.tsv"
+    p.write_text("".join(f"{x} {y}\n" for x, y in zip(a, b)))
+    with _engine(1, 1, packed) as s:
+        s.load_edge_list(p, V=V, symmetrize=True)
+        w0 = s.windows()
+        assert w0["windows"] >= 3
+        g = s.download()
+        want = oracle.cc_unionfind(g["off"], g["idx"])
+        lab, st = s.cc()
+        np.testing.assert_array_equal(lab, want)
+        assert st.edges > V * 2  # the other component's remaining edges were linked
+        lab2, _ = s.cc()  # the first-ids table is kept
+        np.testing.assert_array_equal(lab2, want)
+        s.set_option(gmp.OPT_CC_UNION_FIND, 0)
+        lab3, _ = s.cc()
+        np.testing.assert_array_equal(lab3, want)
+
+
 @pytest.mark.parametrize("app", [gmp.APP_PAGERANK, gmp.APP_SSSP, gmp.APP_CC, gmp.APP_SSSP_W])
 @pytest.mark.parametrize("packed", [False, True])
 def test_windows_compat_apps_bit_exact(app, packed):
