@@ -39,6 +39,12 @@ __global__ void shift_offsets_kernel(const OffT *off, uint32_t V, uint64_t ebase
   GRID_STRIDE32(r, V + 1) out[r] = (uint64_t)off[r] + ebase;
 }
 
+// out[vbase + i] = vbase + map[i] (map null: the identity).
+__global__ void shift_map_kernel(const uint32_t *map, uint32_t n, uint32_t vbase, uint32_t *out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[vbase + i] = vbase + (map ? map[i] : i);
+}
+
 __global__ void shift_ids_kernel(const uint32_t *idx, uint64_t E, uint32_t vbase, uint32_t *out) {
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < E; j += (uint64_t)gridDim.x * blockDim.x)
     out[j] = idx[j] + vbase;
@@ -469,9 +475,13 @@ void batch_generate(Engine &e, uint32_t ngraphs, const gmp_rmat_params *params) 
   } restore{e, saved};
   b.vbase.assign(ngraphs + 1, 0);
   b.ebase.assign(ngraphs + 1, 0);
-  std::vector<Csr> parts(ngraphs);
+  // each graph in its degree-ordered layout (hubs' values share cache lines in the PageRank
+  // gathers) for the in-CSR, and in its own numbering for the out-CSR of the SSSP pushes (spread
+  // ids keep the distance atomics of hub neighbours off shared lines)
+  std::vector<Csr> parts(ngraphs), orig(ngraphs);
+  std::vector<DevBuf> maps(ngraphs);
   std::vector<bool> off64(ngraphs);
-  bool weighted = true;
+  bool weighted = true, relabelled = false;
   for (uint32_t k = 0; k < ngraphs; k++) {
     graph_generate_rmat(e, params[k]);
     if ((uint64_t)b.vbase[k] + e.g.V >= 0xFFFFFFFFull) fail(GMP_EUNSUPPORTED, "batch exceeds 2^32 vertices");
@@ -479,37 +489,72 @@ void batch_generate(Engine &e, uint32_t ngraphs, const gmp_rmat_params *params) 
     b.ebase[k + 1] = b.ebase[k] + e.g.E;
     weighted = weighted && e.g.weighted;
     off64[k] = e.g.off64;
-    parts[k] = std::move(e.g.in);
+    graph_finish(e, 1, nullptr);
+    ensure_work(e);
+    if (e.g.work_identity) {
+      parts[k] = std::move(e.g.in);
+    } else {
+      relabelled = true;
+      parts[k] = std::move(e.g.win);
+      orig[k] = std::move(e.g.in);
+      maps[k] = std::move(e.g.o2n);
+    }
   }
   b.V = b.vbase[ngraphs];
   b.E = b.ebase[ngraphs];
   b.weighted = weighted;
-  b.in.off.alloc(((size_t)b.V + 1) * 8);
-  b.in.idx.alloc(std::max<uint64_t>(b.E, 1) * 4);
-  if (weighted) b.in.w.alloc(std::max<uint64_t>(b.E, 1) * 4);
+  b.o2n.alloc(std::max<size_t>(b.V, 1) * 4);
   for (uint32_t k = 0; k < ngraphs; k++) {
     const uint32_t Vk = b.vbase[k + 1] - b.vbase[k];
-    const uint64_t Ek = b.ebase[k + 1] - b.ebase[k];
-    uint64_t *o = b.in.off.as<uint64_t>() + b.vbase[k];
-    if (off64[k]) shift_offsets_kernel<uint64_t><<<grid_for(Vk + 1), kBlock, 0, e.stream>>>(parts[k].off.as<uint64_t>(), Vk, b.ebase[k], o);
-    else shift_offsets_kernel<uint32_t><<<grid_for(Vk + 1), kBlock, 0, e.stream>>>(parts[k].off.as<uint32_t>(), Vk, b.ebase[k], o);
-    GMP_CHECK_LAUNCH();
-    if (Ek) {
-      shift_ids_kernel<<<grid_for(Ek), kBlock, 0, e.stream>>>(parts[k].idx.as<uint32_t>(), Ek, b.vbase[k],
-                                                             b.in.idx.as<uint32_t>() + b.ebase[k]);
+    if (Vk) {
+      shift_map_kernel<<<grid_for(Vk), kBlock, 0, e.stream>>>(maps[k].as<uint32_t>(), Vk, b.vbase[k],
+                                                              b.o2n.as<uint32_t>());
       GMP_CHECK_LAUNCH();
-      if (weighted)
-        GMP_HIP(hipMemcpyAsync(b.in.w.as<uint32_t>() + b.ebase[k], parts[k].w.p, Ek * 4, hipMemcpyDeviceToDevice,
-                               e.stream));
     }
-    GMP_HIP(hipStreamSynchronize(e.stream));
-    parts[k] = Csr();
   }
-  b.in.valid = true;
+  b.h_o2n.resize(b.V);
+  GMP_HIP(hipMemcpyAsync(b.h_o2n.data(), b.o2n.p, (size_t)b.V * 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  maps.clear();
+  // the union of per-graph CSRs, ids shifted by the graphs' bases (the parts are released)
+  auto concat = [&](std::vector<Csr> &from, Csr &to) {
+    to.off.alloc(((size_t)b.V + 1) * 8);
+    to.idx.alloc(std::max<uint64_t>(b.E, 1) * 4);
+    if (weighted) to.w.alloc(std::max<uint64_t>(b.E, 1) * 4);
+    for (uint32_t k = 0; k < ngraphs; k++) {
+      const uint32_t Vk = b.vbase[k + 1] - b.vbase[k];
+      const uint64_t Ek = b.ebase[k + 1] - b.ebase[k];
+      uint64_t *o = to.off.as<uint64_t>() + b.vbase[k];
+      if (off64[k]) shift_offsets_kernel<uint64_t><<<grid_for(Vk + 1), kBlock, 0, e.stream>>>(from[k].off.as<uint64_t>(), Vk, b.ebase[k], o);
+      else shift_offsets_kernel<uint32_t><<<grid_for(Vk + 1), kBlock, 0, e.stream>>>(from[k].off.as<uint32_t>(), Vk, b.ebase[k], o);
+      GMP_CHECK_LAUNCH();
+      if (Ek) {
+        shift_ids_kernel<<<grid_for(Ek), kBlock, 0, e.stream>>>(from[k].idx.as<uint32_t>(), Ek, b.vbase[k],
+                                                               to.idx.as<uint32_t>() + b.ebase[k]);
+        GMP_CHECK_LAUNCH();
+        if (weighted)
+          GMP_HIP(hipMemcpyAsync(to.w.as<uint32_t>() + b.ebase[k], from[k].w.p, Ek * 4, hipMemcpyDeviceToDevice,
+                                 e.stream));
+      }
+      GMP_HIP(hipStreamSynchronize(e.stream));
+      from[k] = Csr();
+    }
+    to.valid = true;
+  };
+  concat(parts, b.in);
+  if (relabelled) {
+    for (uint32_t k = 0; k < ngraphs; k++)
+      if (!orig[k].idx.p && b.ebase[k + 1] > b.ebase[k]) fail(GMP_ESTATE, "batch: mixed graph layouts");
+    Csr whole;
+    concat(orig, whole);
+    transpose_csr(e, CsrView{whole.off.p, whole.idx.as<uint32_t>(), whole.w.as<uint32_t>()}, b.V, b.E, true, weighted,
+                  b.out);
+  } else {
+    transpose_csr(e, CsrView{b.in.off.p, b.in.idx.as<uint32_t>(), b.in.w.as<uint32_t>()}, b.V, b.E, true, weighted,
+                  b.out);
+  }
   b.outdeg.alloc(std::max<size_t>(b.V, 1) * 4);
   count_degrees(e, b.in.idx.as<uint32_t>(), b.E, b.V, b.outdeg.as<uint32_t>());
-  transpose_csr(e, CsrView{b.in.off.p, b.in.idx.as<uint32_t>(), b.in.w.as<uint32_t>()}, b.V, b.E, true, weighted,
-                b.out);
   b.h_off.resize((size_t)b.V + 1);
   GMP_HIP(hipMemcpy(b.h_off.data(), b.in.off.p, ((size_t)b.V + 1) * 8, hipMemcpyDeviceToHost));
   b.ready = true;
@@ -638,7 +683,7 @@ void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *see
         const uint32_t i = per[k][s0 + c];
         order.push_back(i);
         group_of.push_back(gid);
-        seed_local.push_back(seed[i]);
+        seed_local.push_back(b.h_o2n[b.vbase[k] + seed[i]] - b.vbase[k]);
       }
       rtot += (uint64_t)Vk * g.k;
       // tasks: the pieces of the hub rows, and runs of the other rows sized for the slots
@@ -751,7 +796,8 @@ void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *see
       const PprGroup &g = groups[group_of[gi]];
       const uint32_t col = gi - g.first, i = order[gi];
       float *dst = rank_out + obase[i];
-      for (uint32_t v = 0; v < g.V; v++) dst[v] = h[g.rbase + (uint64_t)v * g.k + col];
+      const uint32_t *map = b.h_o2n.data() + g.vbase;  // caller's local id -> stored union id
+      for (uint32_t v = 0; v < g.V; v++) dst[v] = h[g.rbase + (uint64_t)(map[v] - g.vbase) * g.k + col];
     }
   }
   if (st) {

@@ -1,4 +1,5 @@
 // C ABI of the engine: handle lifetime, graph loading, app entry points.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -397,18 +398,37 @@ int gmp_batch_download(gmp_engine *h, uint32_t graph, uint64_t *row_off, uint32_
   const gmp::BatchGraphs &b = e->batch;
   if (!b.ready) fail(GMP_ESTATE, "no batch loaded");
   if (graph + 1 >= b.vbase.size()) fail(GMP_EINVAL, "gmp_batch_download: graph out of range");
+  if (weights && !b.weighted) fail(GMP_EUNSUPPORTED, "batch graphs have no weights");
   const uint32_t v0 = b.vbase[graph], V = b.vbase[graph + 1] - v0;
   const uint64_t e0 = b.ebase[graph], E = b.ebase[graph + 1] - e0;
-  if (row_off)
-    for (uint32_t r = 0; r <= V; r++) row_off[r] = b.h_off[v0 + r] - e0;
-  if (col_idx && E) {
-    GMP_HIP(hipMemcpy(col_idx, b.in.idx.as<uint32_t>() + e0, E * 4, hipMemcpyDeviceToHost));
-    for (uint64_t k = 0; k < E; k++) col_idx[k] -= v0;
+  // the graph is stored relabelled (stored id = o2n[original id]); rows come back in the original
+  // numbering with every row's sources ascending, the order the graph was built in
+  const uint32_t *o2n = b.h_o2n.data() + v0;
+  std::vector<uint32_t> n2o(V), sidx(E), sw(b.weighted ? E : 0);
+  for (uint32_t v = 0; v < V; v++) n2o[o2n[v] - v0] = v;
+  if (E) {
+    GMP_HIP(hipMemcpy(sidx.data(), b.in.idx.as<uint32_t>() + e0, E * 4, hipMemcpyDeviceToHost));
+    if (b.weighted) GMP_HIP(hipMemcpy(sw.data(), b.in.w.as<uint32_t>() + e0, E * 4, hipMemcpyDeviceToHost));
   }
-  if (weights) {
-    if (!b.weighted) fail(GMP_EUNSUPPORTED, "batch graphs have no weights");
-    if (E) GMP_HIP(hipMemcpy(weights, b.in.w.as<uint32_t>() + e0, E * 4, hipMemcpyDeviceToHost));
+  std::vector<std::pair<uint32_t, uint32_t>> row;
+  uint64_t pos = 0;
+  for (uint32_t r = 0; r < V; r++) {
+    if (row_off) row_off[r] = pos;
+    const uint32_t s = o2n[r] - v0;
+    const uint64_t a = b.h_off[v0 + s] - e0, z = b.h_off[v0 + s + 1] - e0;
+    row.clear();
+    for (uint64_t k = a; k < z; k++) row.emplace_back(n2o[sidx[k] - v0], b.weighted ? sw[k] : 0u);
+    std::stable_sort(row.begin(), row.end(),
+                     [](const std::pair<uint32_t, uint32_t> &x, const std::pair<uint32_t, uint32_t> &y) {
+                       return x.first < y.first;
+                     });
+    for (const auto &q : row) {
+      if (col_idx) col_idx[pos] = q.first;
+      if (weights) weights[pos] = q.second;
+      pos++;
+    }
   }
+  if (row_off) row_off[V] = pos;
   GMP_API_END
 }
 

@@ -261,6 +261,10 @@ struct BatchGraphs {
   std::vector<uint64_t> h_off;        // union in-CSR offsets (host copy)
   Csr in, out;                        // union in-CSR (offsets always uint64) and its transpose
   DevBuf outdeg;                      // uint32[Vtot]
+  // each graph is stored in its degree-ordered layout (as the engine's work layout): o2n maps a
+  // union id in the caller's numbering to the stored one (identity when reordering is off)
+  DevBuf o2n;                         // uint32[Vtot]
+  std::vector<uint32_t> h_o2n;        // host copy
   uint32_t V = 0;
   uint64_t E = 0;
   bool weighted = false;
