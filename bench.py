@@ -34,8 +34,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+_T0 = time.perf_counter()
+
+
 def log(*a):
-    print("[bench]", *a, file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - _T0:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
 def load_traffic(scale, ef):

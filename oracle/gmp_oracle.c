@@ -303,11 +303,6 @@ static uint64_t orc_mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-static int cmp_u32(const void *a, const void *b) {
-  uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
-  return x < y ? -1 : (x > y);
-}
-
 static int has_u32(const uint32_t *a, int64_t n, uint32_t x) {
   int64_t lo = 0, hi = n;
   while (lo < hi) {
@@ -332,24 +327,36 @@ int orc_graphmp_baseline(int32_t app, int64_t V, int32_t P, const int64_t *bound
   /* per-shard sorted unique source sets: the exact stand-in for the per-shard bloom
    * filters built in GraphPS.h:225-245 (built before timing, as the reference builds its
    * filters during init) */
+  /* shard p's set: srcset[ss_off[2p], ss_off[2p+1]), inside the shard's own edge range */
   uint32_t *srcset = (uint32_t *)malloc((size_t)(E > 0 ? E : 1) * sizeof(uint32_t));
-  int64_t *ss_off = (int64_t *)malloc((size_t)(P + 1) * sizeof(int64_t));
+  int64_t *ss_off = (int64_t *)malloc((size_t)(2 * P) * sizeof(int64_t));
   uint32_t *active = (uint32_t *)malloc((size_t)V * sizeof(uint32_t));
-  if (!msgnew || !pactive || !order || !srcset || !ss_off || !active) {
-    free(msgnew); free(pactive); free(order); free(srcset); free(ss_off); free(active);
+  const int64_t nwords = (V + 63) / 64;
+  uint64_t *marks = (uint64_t *)calloc((size_t)threads * (size_t)nwords, sizeof(uint64_t));
+  if (!msgnew || !pactive || !order || !srcset || !ss_off || !active || !marks) {
+    free(msgnew); free(pactive); free(order); free(srcset); free(ss_off); free(active); free(marks);
     return ORC_ENOMEM;
   }
   if (app == ORC_APP_PAGERANK) contrib = (float *)malloc((size_t)V * sizeof(float));
-  ss_off[0] = 0;
+  /* sorted unique sources per shard: mark them in a bitmap over V, read the marks in order
+   * (shards in parallel, one bitmap per thread, cleared as it is read) */
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
   for (int32_t p = 0; p < P; p++) {
-    int64_t a = off[bounds[p]], b = off[bounds[p + 1]], o = ss_off[p];
-    memcpy(srcset + o, idx + a, (size_t)(b - a) * sizeof(uint32_t));
-    qsort(srcset + o, (size_t)(b - a), sizeof(uint32_t), cmp_u32);
-    int64_t n = 0;
-    for (int64_t q = 0; q < b - a; q++)
-      if (n == 0 || srcset[o + q] != srcset[o + n - 1]) srcset[o + n++] = srcset[o + q];
-    ss_off[p + 1] = o + n;
+    uint64_t *mk = marks + (size_t)omp_get_thread_num() * (size_t)nwords;
+    int64_t a = off[bounds[p]], b = off[bounds[p + 1]], n = 0;
+    for (int64_t q = a; q < b; q++) mk[idx[q] >> 6] |= 1ull << (idx[q] & 63u);
+    for (int64_t wi = 0; wi < nwords; wi++) {
+      uint64_t m = mk[wi];
+      mk[wi] = 0;
+      while (m) {
+        srcset[a + n++] = (uint32_t)(wi * 64 + __builtin_ctzll(m));
+        m &= m - 1;
+      }
+    }
+    ss_off[2 * p] = a;
+    ss_off[2 * p + 1] = a + n;
   }
+  free(marks);
   memcpy(msgnew, msg, (size_t)V * 4);
   memset(pactive, 1, (size_t)P);
   double t0 = omp_get_wtime();
@@ -422,7 +429,7 @@ int orc_graphmp_baseline(int32_t app, int64_t V, int32_t P, const int64_t *bound
       for (int32_t p = 0; p < P; p++) {
         uint8_t act = 0;
         for (int64_t a = 0; a < na && !act; a++)
-          act = (uint8_t)has_u32(srcset + ss_off[p], ss_off[p + 1] - ss_off[p], active[a]);
+          act = (uint8_t)has_u32(srcset + ss_off[2 * p], ss_off[2 * p + 1] - ss_off[2 * p], active[a]);
         pactive[p] = act;
       }
     } else {
