@@ -162,7 +162,8 @@ int gmp_tune_load(gmp_engine *h, const char *path);
  * GraphMP's shards (GraphMP/tools/format_spark.py:137-157, GraphMP/system/Comp.h:21-50).
  * row_off has V+1 entries (global offsets); shard_bounds has P+1 entries (NULL: the
  * equal-in-edge partitioner of format_spark.py:85-108 with P shards, P=0 -> 1).
- * weights may be NULL. */
+ * weights may be NULL.  Repeated sources in a row are accepted as parallel edges: each copy
+ * counts in out-degrees and sums, and the lightest copy bounds a distance. */
 int gmp_load_csr(gmp_engine *h, uint32_t V, uint64_t E, const uint64_t *row_off,
                  const uint32_t *col_idx, const uint32_t *weights, uint32_t P,
                  const uint32_t *shard_bounds);

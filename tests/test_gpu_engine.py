@@ -591,3 +591,85 @@ def test_load_edge_list_matches_the_converter(engine, tmp_path):
     gs = engine.download()
     lab, _ = engine.cc()
     np.testing.assert_array_equal(lab, oracle.cc_unionfind(gs["off"], gs["idx"]))
+
+
+def _multigraph(V, seed):
+    """Destination-row CSR with repeated edges (adjacent copies), self-loops, a source feeding
+    every vertex and a destination fed by every vertex (rows above the hub degree, 1024, in the
+    push and the gather) and a tail of isolated vertices; sources ascending within a row."""
+    rng = np.random.default_rng(seed)
+    n = V - 50  # the last 50 vertices have no edges
+    src = rng.integers(0, n, 8 * n)
+    dst = rng.integers(0, n, 8 * n)
+    loops = rng.integers(0, n, n // 4)
+    src = np.concatenate([src, loops, np.full(n, 7), np.arange(n), np.arange(0, n, 3)])
+    dst = np.concatenate([dst, loops, np.arange(n), np.full(n, 11), np.full((n + 2) // 3, 11)])
+    rep = np.where(rng.random(src.size) < 0.2, 2, 1)
+    src, dst = np.repeat(src, rep), np.repeat(dst, rep)
+    order = np.lexsort((src, dst))
+    src, dst = src[order], dst[order]
+    off = np.zeros(V + 1, dtype=np.int64)
+    np.add.at(off, dst + 1, 1)
+    return np.cumsum(off), src.astype(np.uint32)
+
+
+def test_multigraph_self_loops_and_hubs(engine):
+    """Repeated edges count once per copy (out-degrees, PageRank sums) and the lightest copy
+    wins (SSSP); self-loops and isolated vertices take the plain rules; every gather and push
+    path agrees with the oracle, the compat apps bit for bit."""
+    V = 3000
+    off, idx = _multigraph(V, 21)
+    assert int(np.diff(off)[11]) > 2048 and int((idx == 7).sum()) > 2048
+    w = graphs.edge_weights(off, idx, 22)
+    od = graphs.out_degree(V, idx)
+    say("multigraph pagerank")
+    engine.load_csr(off, idx, weights=w, P=3)
+    g = engine.download(weights=True)
+    np.testing.assert_array_equal(g["outdeg"], od)
+    x, _, _ = oracle.pr_f64(off, idx, od, max_iter=30)
+    for strip_log in (15, 0):
+        engine.set_option(gmp.OPT_STRIP_LOG, strip_log)
+        for compressed in ((False, True) if strip_log == 0 else (False,)):
+            r, st = engine.pagerank(max_iter=30, compressed=compressed)
+            assert st.iterations == 30
+            assert np.abs(r - x).sum() < PR_L1_TOL, (strip_log, compressed)
+    engine.set_option(gmp.OPT_STRIP_LOG, 15)
+    say("multigraph sssp")
+    ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+    for s in (7, 11, 0, V - 1):
+        want = oracle.dijkstra(ooff, oidx, ow, s)
+        for method in (gmp.SSSP_BF, gmp.SSSP_DELTA):
+            d, _ = engine.sssp(s, method=method)
+            np.testing.assert_array_equal(d, want, err_msg=f"source {s} method {method}")
+        du, _ = engine.sssp(s, unit_weights=True)
+        np.testing.assert_array_equal(du, oracle.dijkstra(ooff, oidx, np.ones_like(ow), s))
+    say("multigraph compat apps")
+    for app in (gmp.APP_PAGERANK, gmp.APP_SSSP, gmp.APP_CC, gmp.APP_SSSP_W):
+        vout = None
+        if app == gmp.APP_PAGERANK:
+            msg0, vout, cap = np.full(V, 1.0 / V, dtype=np.float32), od.astype(np.int32), 40
+        elif app == gmp.APP_CC:
+            msg0, cap = np.arange(V, dtype=np.int32), 1000
+        elif app == gmp.APP_SSSP:
+            msg0, cap = np.full(V, 10000, dtype=np.int32), 1000
+            msg0[7] = 0
+        else:
+            msg0, cap = np.full(V, 0xFFFFFFFF, dtype=np.uint32), 1000
+            msg0[7] = 0
+        msg, val, st = engine.run_app(app, msg0, cap, vertex_out=vout)
+        want, wval, it, changed = oracle.app_sync(app, off, idx, msg0, cap, w=w, vout=vout)
+        np.testing.assert_array_equal(msg.view(np.uint32), want.view(np.uint32), err_msg=f"app {app}")
+        np.testing.assert_array_equal(val.view(np.uint32), wval.view(np.uint32), err_msg=f"app {app}")
+        assert st.iterations == it and st.changed == changed[-1]
+    say("multigraph cc")
+    dst = np.repeat(np.arange(V), np.diff(off))
+    s2 = np.concatenate([idx.astype(np.int64), dst])
+    d2 = np.concatenate([dst, idx.astype(np.int64)])
+    order = np.lexsort((s2, d2))
+    soff = np.zeros(V + 1, dtype=np.int64)
+    np.add.at(soff, d2 + 1, 1)
+    soff, sidx = np.cumsum(soff), s2[order].astype(np.uint32)
+    engine.load_csr(soff, sidx)
+    lab, _ = engine.cc()
+    np.testing.assert_array_equal(lab, oracle.cc_unionfind(soff, sidx))
+    assert (lab[-50:] == np.arange(V - 50, V)).all()
