@@ -122,7 +122,9 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
       b = a.off[u];
       deg = (uint32_t)(a.off[u + 1] - b);
       du = a.dist[s];
-      if (du == kInf) deg = 0;
+      // a slot whose distance already dropped again in this round is queued for the next one,
+      // where it relaxes its edges from the lower distance: skip it here
+      if (du == kInf || a.stamp[s] == round + 1) deg = 0;
     }
     const uint32_t incl = wave_scan_dpp(deg);
     const uint32_t total = read_lane_u32(incl, kWave - 1);

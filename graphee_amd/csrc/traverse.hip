@@ -210,6 +210,9 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
       deg = (uint32_t)((uint64_t)a.off[u + 1] - b);
       du = a.dist[u];
       if (MODE != kModeMinLabel && du == kInf) deg = 0;
+      // a vertex whose value already dropped again in this launch is in the next frontier,
+      // where it relaxes its edges from the lower value: skip it here
+      if (a.near_stamp[u] == a.stamp) deg = 0;
       if (deg > kHubDegree && a.nhubs) {
         // a hub: its edges go to hub_relax_kernel's slices, spread over many waves
         uint32_t lo = 0, hi = a.nhubs - 1;
