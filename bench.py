@@ -265,8 +265,8 @@ def main():
                          "dispatch that scales the HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the engine runs on a single MI355X)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the engine runs on a "
+              "single MI355X (how it would shard over several is a design note, not code)", file=sys.stderr)
         return 2
 
     import graphee_amd as gmp
