@@ -44,8 +44,8 @@ constexpr int kIterWaves = kIterBlock / kWave;
 constexpr uint32_t kRound = 8 * kWave;         // edges per wave per round (8 per lane)
 constexpr uint32_t kWaveTaskEdges = 16 * kRound;  // edges per wave task
 constexpr int kAhead = 4;                      // rounds of codes in flight per wave
-constexpr int kUnroll = 8;                     // messages in flight per thread (bin kernel)
-constexpr uint32_t kChunk = 1024;              // bin kernel: messages per chunk-index entry
+constexpr int kUnroll = 16;                    // messages in flight per thread (bin kernel)
+constexpr uint32_t kChunk = 512;               // bin kernel: messages per chunk-index entry
 constexpr uint32_t kSplitRows = 1024;          // split apply: rows per block task
 constexpr uint32_t kGroupWaveTasks = 256;      // wave tasks per workgroup task (one staging)
 constexpr uint32_t kCodePad = kRound;          // codes read past the last edge stay in bounds
