@@ -292,6 +292,7 @@ struct Engine {
   bool loaded = false;
   // scratch reused across runs
   DevBuf s_a, s_b, s_c, s_d, s_e, s_f, s_part, s_scal;
+  DevBuf s_gaps;  // graph build: runs of empty rows left to a kernel of their own (count, then runs)
   void *h_pinned = nullptr;  // small pinned host buffer for per-iteration readbacks
 };
 

@@ -82,13 +82,44 @@ __global__ void rmat_keys_kernel(RmatConsts c, uint64_t m, uint64_t *keys, int u
   }
 }
 
-// off[r] = first j with row(key[j]) >= r, for r in [0, V]; key rows are nondecreasing.
+// A run of rows [r0, r1] whose offsets are all `val`.
+struct GapRun {
+  uint32_t r0, r1;
+  uint64_t val;
+};
+constexpr uint32_t kGapSerial = 256;     // rows an edge's thread fills itself
+constexpr uint32_t kGapPiece = 1u << 16;  // rows of one gap_fill_kernel block
+
+// off[r] = first j with row(key[j]) >= r, for r in [0, V]; key rows are nondecreasing.  The
+// thread of edge j (j = E: the end) owns rows (row(key[j-1]), row(key[j])]: up to kGapSerial of
+// them it writes itself, longer runs it leaves as pieces of kGapPiece rows in `gaps` (at most
+// cap of them; *ngaps counts them) for gap_fill_kernel.
 template <class OffT>
-__global__ void offsets_from_keys_kernel(const uint64_t *keys, uint64_t E, uint32_t shift, uint32_t V, OffT *off) {
+__global__ void offsets_from_keys_kernel(const uint64_t *keys, uint64_t E, uint32_t shift, uint32_t V, OffT *off,
+                                         GapRun *gaps, uint32_t *ngaps) {
   GRID_STRIDE(j, E + 1) {
-    int64_t r = j < E ? (int64_t)(keys[j] >> shift) : (int64_t)V;
-    int64_t rp = j > 0 ? (int64_t)(keys[j - 1] >> shift) : -1;
-    for (int64_t rr = rp + 1; rr <= r; rr++) off[rr] = (OffT)j;
+    const int64_t r = j < E ? (int64_t)(keys[j] >> shift) : (int64_t)V;
+    const int64_t rp = j > 0 ? (int64_t)(keys[j - 1] >> shift) : -1;
+    if (r - rp <= (int64_t)kGapSerial) {
+      for (int64_t rr = rp + 1; rr <= r; rr++) off[rr] = (OffT)j;
+    } else {
+      const uint32_t np = (uint32_t)((r - rp + kGapPiece - 1) / kGapPiece);
+      const uint32_t first = atomicAdd(ngaps, np);
+      for (uint32_t k = 0; k < np; k++) {
+        const int64_t a = rp + 1 + (int64_t)k * kGapPiece;
+        gaps[first + k] = GapRun{(uint32_t)a, (uint32_t)std::min<int64_t>(a + kGapPiece - 1, r), (uint64_t)j};
+      }
+    }
+  }
+}
+
+// One block per run left by offsets_from_keys_kernel.
+template <class OffT>
+__global__ void gap_fill_kernel(const GapRun *gaps, const uint32_t *ngaps, OffT *off) {
+  const uint32_t n = *ngaps;
+  for (uint32_t d = blockIdx.x; d < n; d += gridDim.x) {
+    const GapRun g = gaps[d];
+    for (uint64_t rr = (uint64_t)g.r0 + threadIdx.x; rr <= g.r1; rr += blockDim.x) off[rr] = (OffT)g.val;
   }
 }
 
@@ -164,10 +195,20 @@ void csr_from_sorted_keys(Engine &e, const uint64_t *keys, uint64_t E, uint32_t 
   c.off.alloc((size_t)(V + 1) * (off64 ? 8 : 4));
   c.idx.alloc(std::max<uint64_t>(E, 1) * 4);
   uint32_t g = grid_for(E + 1);
-  if (off64)
-    offsets_from_keys_kernel<uint64_t><<<g, kBlock, 0, e.stream>>>(keys, E, shift, V, c.off.as<uint64_t>());
-  else
-    offsets_from_keys_kernel<uint32_t><<<g, kBlock, 0, e.stream>>>(keys, E, shift, V, c.off.as<uint32_t>());
+  // runs longer than kGapSerial rows are disjoint, each over kGapSerial rows long: at most
+  // (V + 1) / kGapSerial of them, plus one piece boundary per kGapPiece rows
+  const size_t cap = ((size_t)V + 1) / kGapSerial + ((size_t)V + 1) / kGapPiece + 2;
+  e.s_gaps.alloc(256 + cap * sizeof(GapRun));
+  uint32_t *ngaps = e.s_gaps.as<uint32_t>();
+  GapRun *gaps = reinterpret_cast<GapRun *>(static_cast<char *>(e.s_gaps.p) + 256);
+  GMP_HIP(hipMemsetAsync(ngaps, 0, 4, e.stream));
+  if (off64) {
+    offsets_from_keys_kernel<uint64_t><<<g, kBlock, 0, e.stream>>>(keys, E, shift, V, c.off.as<uint64_t>(), gaps, ngaps);
+    gap_fill_kernel<uint64_t><<<1024, kBlock, 0, e.stream>>>(gaps, ngaps, c.off.as<uint64_t>());
+  } else {
+    offsets_from_keys_kernel<uint32_t><<<g, kBlock, 0, e.stream>>>(keys, E, shift, V, c.off.as<uint32_t>(), gaps, ngaps);
+    gap_fill_kernel<uint32_t><<<1024, kBlock, 0, e.stream>>>(gaps, ngaps, c.off.as<uint32_t>());
+  }
   GMP_CHECK_LAUNCH();
   if (E) {
     cols_from_keys_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(keys, E, (1ull << shift) - 1, c.idx.as<uint32_t>());

@@ -673,3 +673,36 @@ def test_multigraph_self_loops_and_hubs(engine):
     lab, _ = engine.cc()
     np.testing.assert_array_equal(lab, oracle.cc_unionfind(soff, sidx))
     assert (lab[-50:] == np.arange(V - 50, V)).all()
+
+
+def test_long_runs_of_isolated_vertices(engine):
+    """Edges only among ids 1000000..1000299 of 3000000 vertices: the out-CSR, the degree-ordered
+    layout and the strip layout are built over a million empty rows before the band and two
+    million after it (runs longer than one thread fills, left to the gap kernel)."""
+    rng = np.random.default_rng(41)
+    V, lo, n = 3_000_000, 1_000_000, 300
+    src = lo + rng.integers(0, n, 4000)
+    dst = lo + rng.integers(0, n, 4000)
+    off, idx = graphs.csr_from_edges(V, src, dst)
+    w = graphs.edge_weights(off, idx, 42)
+    say("isolated-vertex runs: load")
+    engine.load_csr(off, idx, weights=w, P=2)
+    g = engine.download(weights=True)
+    np.testing.assert_array_equal(g["off"], off)
+    np.testing.assert_array_equal(g["outdeg"], graphs.out_degree(V, idx))
+    say("isolated-vertex runs: sssp")
+    ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+    s = int(src[0])
+    want = oracle.dijkstra(ooff, oidx, ow, s)
+    for method in (gmp.SSSP_BF, gmp.SSSP_DELTA):
+        d, _ = engine.sssp(s, method=method)
+        np.testing.assert_array_equal(d, want)
+    say("isolated-vertex runs: pagerank")
+    x, _, _ = oracle.pr_f64(off, idx, graphs.out_degree(V, idx), max_iter=10)
+    r, _ = engine.pagerank(max_iter=10)
+    assert np.abs(r - x).sum() < PR_L1_TOL
+    say("isolated-vertex runs: cc")
+    soff, sidx = graphs.symmetrize(V, off, idx)
+    engine.load_csr(soff, sidx)
+    lab, _ = engine.cc()
+    np.testing.assert_array_equal(lab, oracle.cc_unionfind(soff, sidx))
