@@ -398,6 +398,20 @@ void pagerank_windows(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_st
     n1 += g1[w];
     n2 += g2[w];
   }
+  // windows in an order that spreads the streamed ones evenly among the cached ones (a cached
+  // one first), so the copy engines keep filling the ring while cached windows are gathered;
+  // every row's sum is the same in any order
+  std::vector<uint32_t> order;
+  order.reserve(nwin);
+  {
+    std::vector<uint32_t> cw, sw;
+    for (uint32_t w = 0; w < nwin; w++) (s.win[w].cache >= 0 ? cw : sw).push_back(w);
+    size_t ic = 0, is = 0;
+    while (ic < cw.size() || is < sw.size()) {
+      const bool take_s = is < sw.size() && (ic == cw.size() || (is + 1) * nwin <= (ic + is + 1) * sw.size());
+      order.push_back(take_s ? sw[is++] : cw[ic++]);
+    }
+  }
   const uint32_t np1 = std::max(n1, G), np2 = std::max<uint32_t>(n2, 1);
   e.s_a.alloc((size_t)V * 4);
   e.s_b.alloc((size_t)V * 4);
@@ -438,7 +452,8 @@ void pagerank_windows(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_st
   for (it = 0; it < max_iter; it++) {
     GMP_HIP(hipEventRecord(ev[2 + 2 * it], e.stream));
     uint32_t o1 = 0, o2 = 0;
-    for (uint32_t w = 0; w < nwin; w++) {
+    for (uint32_t wi = 0; wi < nwin; wi++) {
+      const uint32_t w = order[wi];
       const EdgeWindow &x = s.win[w];
       const WindowDev d = window_begin(e, w, false, false);
       PrArgs<OffT> a{};
