@@ -370,6 +370,9 @@ struct WindowDev {
 // Makes window `w` available to work enqueued next on e.stream: a cached window at once, a
 // streamed one by a copy on the copy stream that e.stream waits for.  plain: ids decoded to
 // uint32 (always available for plain streams; packed ones are decoded on e.stream).
+// Windows with the streamed ones spread evenly among the cached ones, a cached one first: the
+// copy engines keep filling the ring while cached windows are gathered.
+std::vector<uint32_t> window_order(const SemiExternal &s);
 WindowDev window_begin(Engine &e, uint32_t w, bool weights, bool plain);
 // Releases the ring slot of window_begin after the work enqueued so far on e.stream.
 void window_end(Engine &e, const WindowDev &d);

@@ -401,17 +401,7 @@ void pagerank_windows(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_st
   // windows in an order that spreads the streamed ones evenly among the cached ones (a cached
   // one first), so the copy engines keep filling the ring while cached windows are gathered;
   // every row's sum is the same in any order
-  std::vector<uint32_t> order;
-  order.reserve(nwin);
-  {
-    std::vector<uint32_t> cw, sw;
-    for (uint32_t w = 0; w < nwin; w++) (s.win[w].cache >= 0 ? cw : sw).push_back(w);
-    size_t ic = 0, is = 0;
-    while (ic < cw.size() || is < sw.size()) {
-      const bool take_s = is < sw.size() && (ic == cw.size() || (is + 1) * nwin <= (ic + is + 1) * sw.size());
-      order.push_back(take_s ? sw[is++] : cw[ic++]);
-    }
-  }
+  const std::vector<uint32_t> order = window_order(s);
   const uint32_t np1 = std::max(n1, G), np2 = std::max<uint32_t>(n2, 1);
   e.s_a.alloc((size_t)V * 4);
   e.s_b.alloc((size_t)V * 4);

@@ -266,6 +266,9 @@ void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st) {
   const uint32_t *tasks = s.tasks.as<uint32_t>();
   const HubRow *hubs = s.hubs.as<HubRow>();
   const bool weights = mode == kMinWeighted;
+  // values are updated in place, so a window may see this sweep's earlier windows; the minimum
+  // the sweeps converge to does not depend on the order
+  const std::vector<uint32_t> order = window_order(s);
   int sweeps = 0;
   uint64_t edges = 0;
   for (uint32_t sweep = 1;; sweep++) {
@@ -274,7 +277,8 @@ void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st) {
     for (uint32_t w = 0; w < nwin; w++) nact += act[w];
     if (nact == 0) break;
     a.sweep = sweep;
-    for (uint32_t w = 0; w < nwin; w++) {
+    for (uint32_t wi = 0; wi < nwin; wi++) {
+      const uint32_t w = order[wi];
       const EdgeWindow &x = s.win[w];
       if (!act[w]) {
         s.windows_skipped++;
@@ -692,6 +696,19 @@ void window_plans(Engine &e) {
   if (!hubs.empty()) GMP_HIP(hipMemcpy(s.hubs.p, hubs.data(), hubs.size() * sizeof(HubRow), hipMemcpyHostToDevice));
   if (!multi.empty()) GMP_HIP(hipMemcpy(s.multi.p, multi.data(), multi.size() * 4, hipMemcpyHostToDevice));
   std::copy(key, key + 4, s.plan_key);
+}
+
+std::vector<uint32_t> window_order(const SemiExternal &s) {
+  const uint32_t nwin = (uint32_t)s.win.size();
+  std::vector<uint32_t> cw, sw, order;
+  order.reserve(nwin);
+  for (uint32_t w = 0; w < nwin; w++) (s.win[w].cache >= 0 ? cw : sw).push_back(w);
+  size_t ic = 0, is = 0;
+  while (ic < cw.size() || is < sw.size()) {
+    const bool take_s = is < sw.size() && (ic == cw.size() || (is + 1) * nwin <= (ic + is + 1) * sw.size());
+    order.push_back(take_s ? sw[is++] : cw[ic++]);
+  }
+  return order;
 }
 
 WindowDev window_begin(Engine &e, uint32_t wi, bool weights, bool plain) {
