@@ -41,6 +41,32 @@ def log(*a):
     print(f"[bench {time.perf_counter() - _T0:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
+def cpu_threads():
+    """Threads for the CPU baseline: every CPU in this process's affinity set, capped by
+    OMP_NUM_THREADS when the environment sets it (a GPU host that grants each GPU job a share
+    of its cores says so there)."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(int(env), aff) if env.isdigit() and int(env) > 0 else aff
+
+
+def host_info():
+    """What the CPU baseline ran on: CPU model (lscpu's "Model name", read from /proc/cpuinfo),
+    logical CPUs of the host (nproc --all), CPUs this process may use, OMP_NUM_THREADS."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "threads_used": cpu_threads()}
+
+
 def load_traffic(scale, ef):
     """HBM bytes per PageRank iteration from the committed rocprofv3 PMC summary, if any."""
     p = os.path.join(ROOT, "profiles", f"pr_strip_rmat_s{scale}_ef{ef}_traffic.json")
@@ -119,6 +145,92 @@ def run_learned(eng, eval_scale, nsources=8, train_scales=(24, 25, 26), trace_so
             "trace_seconds": round(t_rec, 1), "train_seconds": round(t_train, 2), "train_loss": round(loss, 4),
             "train_class_accuracy": round(acc["class_accuracy"], 3),
             "shard_precision": round(acc["shard_precision"], 3), "shard_recall": round(acc["shard_recall"], 3)}
+
+
+def run_pagerank_s20(eng, threads):
+    """PageRank on directed R-MAT s20 ef16 in 16 destination-interval shards, exactly 20
+    iterations from 1/N, FP32 ranks: GraphMP's CPU engine shape (the oracle's OpenMP
+    restatement, timed) beside the GPU, each compared with FP64 -- the same 20 iterations in
+    FP64 (arithmetic error) and the FP64 fixed point (arithmetic + truncation) -- plus the
+    reference's own update rule (GraphMP/app/pagerank.cpp:42-59, run by gmp_run_app) bit for bit
+    against the oracle's restatement."""
+    import graphee_amd as gmp
+    import oracle
+    scale = 20
+    eng.generate_rmat(scale, 16, 1000 + scale, num_shards=16)
+    V, E, P, _ = eng.info()
+    g = eng.download()
+    msg0 = np.full(V, 1.0 / V, dtype=np.float32)
+    x_cpu, stc = oracle.graphmp_baseline(oracle.APP_PAGERANK, g["off"], g["idx"], g["bounds"], msg0, 20,
+                                         outdeg=g["outdeg"], threads=threads)
+    x20, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
+    eng.pagerank(damping=0.85, max_iter=20, want_ranks=False)
+    x_gpu, stg = eng.pagerank(damping=0.85, max_iter=20)
+    out = {"workload": f"pagerank-rmat-s{scale}-ef16, {P} shards, 20 iterations, FP32 ranks",
+           "vertices": V, "edges": E, "shards": P}
+    for name, x, secs, it in (("cpu", x_cpu, stc.seconds, stc.iterations), ("gpu", x_gpu, stg.ms_total / 1e3,
+                                                                           stg.iterations)):
+        d20 = np.abs(x.astype(np.float64) - x20)
+        fp = oracle.pr_f64_error(g["off"], g["idx"], g["outdeg"], x, threads=threads, max_iter=200, bound=1e-13)
+        out[name] = {"seconds": round(secs, 5), "iterations": it, "gteps": round(E * it / secs / 1e9, 3),
+                     "l1_vs_fp64_same_20_iterations": float(d20.sum()),
+                     "linf_vs_fp64_same_20_iterations": float(d20.max()),
+                     "l1_vs_fp64_fixed_point": fp["l1_error"], "linf_vs_fp64_fixed_point": fp["linf_error"],
+                     "fixed_point_reference_bound": fp["reference_l1_bound"]}
+    out["cpu"]["threads"] = stc.threads
+    out["gpu_speedup"] = round(stc.seconds / (stg.ms_total / 1e3), 2)
+    # the reference's own rule: sum in source order, (float)(rel*0.85 + 1.0/N), dangling mass dropped
+    vout = g["outdeg"].astype(np.int32)
+    m_gpu, _, _ = eng.run_app(gmp.APP_PAGERANK, msg0, 20, vertex_out=vout)
+    m_orc, _, it_orc, _ = oracle.app_sync(oracle.APP_PAGERANK, g["off"], g["idx"], msg0, 20, vout=vout)
+    out["graphmp_rule_bit_exact"] = bool(np.array_equal(m_gpu.view(np.uint32), m_orc.view(np.uint32)))
+    log(f"pagerank s{scale} x20: CPU {stc.seconds * 1e3:.1f} ms on {stc.threads} threads "
+        f"(L1/Linf vs FP64 {out['cpu']['l1_vs_fp64_same_20_iterations']:.2e}/"
+        f"{out['cpu']['linf_vs_fp64_same_20_iterations']:.2e}), GPU {stg.ms_total:.2f} ms "
+        f"({out['gpu']['l1_vs_fp64_same_20_iterations']:.2e}/{out['gpu']['linf_vs_fp64_same_20_iterations']:.2e}); "
+        f"GraphMP rule bit-exact: {out['graphmp_rule_bit_exact']}")
+    return out
+
+
+def run_sssp_s24(eng, threads):
+    """SSSP on directed R-MAT s24 ef16 with uint32 weights in [1,255] from a seeded source of
+    out-degree > 0: Bellman-Ford (pushes + strip sweeps, GraphMP's selective scheduling) and
+    fixed delta = 64 on the GPU, and the weighted GraphMP rule in the CPU engine restatement;
+    the three distance vectors must be identical."""
+    import graphee_amd as gmp
+    import oracle
+    scale = 24
+    eng.generate_rmat(scale, 16, 1000 + scale, weighted=True, num_shards=16)
+    V, E, _, _ = eng.info()
+    src = eng.pick_source(1000 + scale)
+    out = {"workload": f"sssp-rmat-s{scale}-ef16, weights 1..255", "vertices": V, "edges": E, "source": src}
+    dist = {}
+    for name, method in (("bf", gmp.SSSP_BF), ("delta64", gmp.SSSP_DELTA)):
+        eng.sssp(src, method=method, delta=64, want_dist=False)
+        d, st = eng.sssp(src, method=method, delta=64)
+        dist[name] = d
+        reached = int((d != 0xFFFFFFFF).sum())
+        out[name] = {"ms": round(st.ms_total, 3), "iterations": st.iterations, "edges_relaxed": st.edges,
+                     "work_gteps": round(st.edges / st.ms_total / 1e6, 3), "reached": reached,
+                     "yardstick_gbs": round(st.bytes / st.ms_total / 1e6, 1),
+                     "hbm_frac": round(st.bytes / st.ms_total / 1e6 / HBM_PEAK_GBS, 4)}
+    g = eng.download(weights=True)
+    # edges reachable from the source (Graph500-style GTEPS counts the reached part)
+    reach = dist["bf"] != 0xFFFFFFFF
+    e_reached = int(np.diff(g["off"].astype(np.int64))[reach].sum())
+    for name in ("bf", "delta64"):
+        out[name]["graph500_gteps"] = round(e_reached / out[name]["ms"] / 1e6, 3)
+    msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+    msg0[src] = 0
+    dc, stc = oracle.graphmp_baseline(oracle.APP_SSSP_W, g["off"], g["idx"], g["bounds"], msg0, 10000, w=g["w"],
+                                      threads=threads)
+    out["cpu"] = {"seconds": round(stc.seconds, 3), "threads": stc.threads, "sweeps": stc.iterations}
+    out["identical"] = bool(np.array_equal(dc, dist["bf"]) and np.array_equal(dc, dist["delta64"]))
+    out["gpu_bf_speedup"] = round(stc.seconds * 1e3 / out["bf"]["ms"], 2)
+    out["gpu_delta64_speedup"] = round(stc.seconds * 1e3 / out["delta64"]["ms"], 2)
+    log(f"sssp s{scale}: BF {out['bf']['ms']:.1f} ms, delta-64 {out['delta64']['ms']:.1f} ms, CPU "
+        f"{stc.seconds:.2f} s ({stc.threads} threads); identical: {out['identical']}")
+    return out
 
 
 def gmp_const(name):
@@ -250,6 +362,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample")
+    ap.add_argument("--fp64-steps", type=int, default=40,
+                    help="most FP64 CPU power iterations of the reference the PageRank errors are measured against")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--no-extra", action="store_true", help="skip the SSSP and CC measurements")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-instance runs")
@@ -377,12 +491,23 @@ def main():
         import oracle
         ranks, _ = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol)
         g = eng.download()
-        threads = min(len(os.sched_getaffinity(0)), 16)
+        threads = cpu_threads()
         l1, linf = oracle.pr_step_residual(g["off"], g["idx"], g["outdeg"], ranks, threads=threads)
+        t0 = time.perf_counter()
+        err = oracle.pr_f64_error(g["off"], g["idx"], g["outdeg"], ranks, threads=threads,
+                                  max_iter=args.fp64_steps, bound=1e-11)
         result["extra"]["pagerank_fp64_check"] = {
-            "step_residual_l1": l1, "step_residual_linf": linf, "l1_error_bound": l1 / 0.15,
-            "method": "one FP64 CPU step from the GPU ranks; ||x-x*||_1 <= ||T(x)-x||_1/(1-d)"}
-        log(f"FP64 check: ||T(x)-x||_1={l1:.3e} -> L1 error <= {l1 / 0.15:.3e}, Linf {linf:.3e}")
+            "l1_error": err["l1_error"], "linf_error": err["linf_error"],
+            "reference_l1_bound": err["reference_l1_bound"], "reference_steps": err["reference_steps"],
+            "reference_seconds": round(time.perf_counter() - t0, 1),
+            "l1_error_bound": l1 / 0.15, "step_residual_l1": l1, "step_residual_linf": linf,
+            "method": "errors: |x - x_ref| of the GPU ranks x against an FP64 CPU power iteration x_ref "
+                      "started from x and run until its a-posteriori bound ||x_ref - x*||_1 <= "
+                      "||x_ref - x_ref_prev||_1 d/(1-d) is reference_l1_bound (each error is exact to within "
+                      "it); l1_error_bound = ||T(x)-x||_1/(1-d) from one FP64 step (step_residual_*: T(x)-x)"}
+        log(f"FP64 check: L1 error {err['l1_error']:.3e}, Linf error {err['linf_error']:.3e} "
+            f"(reference within {err['reference_l1_bound']:.1e} after {err['reference_steps']} steps); "
+            f"a-posteriori L1 bound {l1 / 0.15:.3e}")
         msg0 = np.full(V, 1.0 / V, dtype=np.float32)
         _, st1 = oracle.graphmp_baseline(oracle.APP_PAGERANK, g["off"], g["idx"], g["bounds"], msg0, 1,
                                          outdeg=g["outdeg"], threads=threads)
@@ -394,7 +519,8 @@ def main():
             "value": round(cpu_gteps, 4), "unit": "GTEPS", "cores": stc.threads, "kind": "port",
             "sample": f"{stc.iterations} PageRank iterations of the GraphMP OpenMP engine restatement "
                       f"(16 shards, guided schedule, edges in RAM) on the same s{scale} graph, "
-                      f"{stc.seconds:.1f} s"}
+                      f"{stc.seconds:.1f} s",
+            "host": host_info()}
         log(f"cpu baseline: {cpu_gteps:.3f} GTEPS on {stc.threads} threads ({stc.iterations} iterations)")
         del g
 
@@ -425,7 +551,7 @@ def main():
             g = eng.download(weights=True)
             msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
             msg0[src] = 0
-            threads = min(len(os.sched_getaffinity(0)), 16)
+            threads = cpu_threads()
             dist, stc = oracle.graphmp_baseline(oracle.APP_SSSP_W, g["off"], g["idx"], g["bounds"], msg0, 10000,
                                                 w=g["w"], threads=threads)
             same = bool(np.array_equal(dist, gpu_dist) and np.array_equal(dist, bf_dist))
@@ -440,40 +566,53 @@ def main():
             del g, dist
         eng.generate_rmat(scale, ef, seed, undirected=True, num_shards=16)
         Vs, Es, _, _ = eng.info()
-        eng.cc(want_labels=False)
-        lab, st = eng.cc()  # a symmetric graph: sampled union-find
-        ncomp = int((lab == np.arange(Vs, dtype=np.uint32)).sum())
-        result["extra"]["cc"] = {"ms": round(st.ms_total, 3), "edges_symmetric": Es,
-                                 "algorithm": "sampled union-find (2 neighbours, giant component skipped)",
-                                 "phases": st.iterations, "gteps": round(Es / st.ms_total / 1e6, 3),
-                                 "edges_examined": st.edges, "components": ncomp}
-        log(f"cc: {st.ms_total:.1f} ms, union-find, {st.edges} edges examined, {ncomp} components")
-        # the same labels by min-label sweeps and pushes; yardstick 8 B per gathered edge
+        # config 3's CC: min-label propagation (GraphMP/app/cc.cpp:40-56 on the symmetrised graph),
+        # sweeps over the source strips with frontier pushes for the sparse tail; yardstick 8 B per
+        # gathered edge
         eng.set_option(gmp.OPT_CC_UNION_FIND, 0)
         eng.cc(want_labels=False)
-        lab2, st2 = eng.cc()
+        lab, st = eng.cc()
         eng.set_option(gmp.OPT_CC_UNION_FIND, 1)
-        result["extra"]["cc_sweeps"] = {"ms": round(st2.ms_total, 3), "iterations": st2.iterations,
-                                        "edges_relaxed": st2.edges,
-                                        "yardstick_gbs": round(st2.bytes / st2.ms_total / 1e6, 1),
-                                        "hbm_frac": round(st2.bytes / st2.ms_total / 1e6 / HBM_PEAK_GBS, 4),
-                                        "identical_to_union_find": bool(np.array_equal(lab, lab2))}
-        log(f"cc sweeps: {st2.ms_total:.1f} ms, {st2.iterations} iterations, identical: {np.array_equal(lab, lab2)}")
+        ncomp = int((lab == np.arange(Vs, dtype=np.uint32)).sum())
+        result["extra"]["cc"] = {"ms": round(st.ms_total, 3), "edges_symmetric": Es,
+                                 "algorithm": "min-label propagation: strip sweeps + frontier pushes",
+                                 "iterations": st.iterations, "sweeps": st.epochs,
+                                 "gteps": round(Es * st.epochs / st.ms_total / 1e6, 3),
+                                 "edges_gathered_or_pushed": st.edges,
+                                 "work_gteps": round(st.edges / st.ms_total / 1e6, 3),
+                                 "yardstick_gbs": round(st.bytes / st.ms_total / 1e6, 1),
+                                 "hbm_frac": round(st.bytes / st.ms_total / 1e6 / HBM_PEAK_GBS, 4),
+                                 "components": ncomp}
+        log(f"cc (min-label): {st.ms_total:.1f} ms, {st.iterations} iterations ({st.epochs} sweeps), "
+            f"{ncomp} components")
+        # the same labels by sampled union-find (not the config's algorithm: reported beside it)
+        eng.cc(want_labels=False)
+        lab2, st2 = eng.cc()
+        result["extra"]["cc_union_find"] = {
+            "ms": round(st2.ms_total, 3), "algorithm": "sampled union-find (2 neighbours, giant component skipped)",
+            "phases": st2.iterations, "edges_examined": st2.edges,
+            "gteps_on_edges_examined": round(st2.edges / st2.ms_total / 1e6, 3),
+            "identical_to_min_label": bool(np.array_equal(lab, lab2))}
+        log(f"cc union-find: {st2.ms_total:.1f} ms, {st2.edges} edges examined, identical: {np.array_equal(lab, lab2)}")
         del lab2
         if not args.no_cpu:
             import oracle
             g = eng.download()
-            threads = min(len(os.sched_getaffinity(0)), 16)
+            threads = cpu_threads()
             labc, stc = oracle.graphmp_baseline(oracle.APP_CC, g["off"], g["idx"], g["bounds"],
                                                 np.arange(Vs, dtype=np.uint32), 10000, threads=threads)
             result["extra"]["cc_cpu_baseline"] = {
                 "seconds": round(stc.seconds, 3), "threads": stc.threads, "sweeps": stc.iterations,
                 "edges_gathered": stc.edges, "identical_to_gpu": bool(np.array_equal(labc, lab)),
-                "gpu_speedup": round(stc.seconds * 1e3 / st.ms_total, 2)}
-            log(f"cc cpu baseline: {stc.seconds:.2f} s ({stc.iterations} sweeps) -> GPU is "
+                "gpu_min_label_speedup": round(stc.seconds * 1e3 / st.ms_total, 2),
+                "gpu_union_find_speedup": round(stc.seconds * 1e3 / st2.ms_total, 2)}
+            log(f"cc cpu baseline: {stc.seconds:.2f} s ({stc.iterations} sweeps) -> GPU min-label is "
                 f"{stc.seconds * 1e3 / st.ms_total:.1f}x; identical: {np.array_equal(labc, lab)}")
             del g, labc
 
+    if not args.no_extra and not args.no_cpu:
+        result["extra"]["pagerank_rmat_s20_x20"] = run_pagerank_s20(eng, cpu_threads())
+        result["extra"]["sssp_rmat_s24"] = run_sssp_s24(eng, cpu_threads())
     if not args.no_extra and not args.no_batched:
         result["extra"]["batched"] = run_batched(eng)
     if not args.no_extra and not args.no_learned:

@@ -1,5 +1,5 @@
 // Connected components (min label) as a GraphMP app on the gfx950 engine:
-// GraphMP/app/cc.cpp:260-273 on gmp::GraphPS.
+// GraphMP/app/cc.cpp:62-94 on gmp::GraphPS.
 //   usage: gmp_cc DATA_DIR VERTICES SHARDS MAX_ITER [OUT.npy]
 #include <cstdio>
 #include <cstdlib>

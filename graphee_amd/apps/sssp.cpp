@@ -1,6 +1,6 @@
-// SSSP as a GraphMP app on the gfx950 engine: GraphMP/app/sssp.cpp:167-179 on gmp::GraphPS
+// SSSP as a GraphMP app on the gfx950 engine: GraphMP/app/sssp.cpp:63-94 on gmp::GraphPS
 // (unit weights, GPS_INF = 10000 as in GraphMP/system/Global.h:50).  The source is a command
-// line argument instead of the hard-coded vertex 1 of sssp.cpp:176 (default 1).
+// line argument instead of the hard-coded vertex 1 of sssp.cpp:72 (default 1).
 //   usage: gmp_sssp DATA_DIR VERTICES SHARDS MAX_ITER [OUT.npy] [SOURCE]
 #include <cstdio>
 #include <cstdlib>

@@ -12,8 +12,8 @@
 // Compat apps (gmp_run_app) reproduce GraphMP's per-vertex rules bit for bit:
 //   pagerank.cpp:42-48  rel = sum in source order of Msg[u]/Out[u] in FP32, then
 //                       (float)(rel*0.85 + 1.0/N) in FP64 (no FMA contraction);
-//   sssp.cpp:144-150    min(Msg[v], Msg[u]+1) in int32;
-//   cc.cpp:238-244      min(Msg[v], Msg[u]) in int32;
+//   sssp.cpp:40-46      min(Msg[v], Msg[u]+1) in int32;
+//   cc.cpp:40-45        min(Msg[v], Msg[u]) in int32;
 // under the synchronous loop of GraphPS.h:256-386 (changed count against VertexValue,
 // stop when nothing changed and step > 1).
 #include <algorithm>

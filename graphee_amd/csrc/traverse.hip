@@ -12,7 +12,7 @@
 //
 // Results equal the reference's fixed points regardless of processing order: SSSP distances
 // are shortest-path lengths (min over relaxations), CC labels are the minimum vertex id of
-// each component (min-label rule of GraphMP/app/cc.cpp:238-244 on a symmetric graph).
+// each component (min-label rule of GraphMP/app/cc.cpp:40-45 on a symmetric graph).
 //
 // Delta-stepping (fixed or learned bucket width) is the near/far form: vertices whose new
 // distance lies below the current threshold join the near frontier, others the far pile;

@@ -7,7 +7,7 @@
 // and only vertices outside it link their remaining neighbours -- on a symmetric graph an edge
 // into the giant component is also an edge out of the other endpoint, so nothing is lost.  Trees
 // hang from their least (work) id; the label of a component is its least original id, the fixed
-// point of GraphMP's min-label rule (cc.cpp:238-244).
+// point of GraphMP's min-label rule (cc.cpp:40-45).
 #pragma once
 
 #include <algorithm>

@@ -32,8 +32,8 @@ extern "C" {
 #define GMP_EUNSUPPORTED 7 /* feature not available for this graph */
 
 /* GraphMP-compatible apps for gmp_run_app: the per-vertex rules of
- * GraphMP/app/pagerank.cpp:42-59 (float), sssp.cpp:144-162 (int32, +1 per edge) and
- * cc.cpp:238-255 (int32 min label), plus the weighted SSSP generalisation (uint32,
+ * GraphMP/app/pagerank.cpp:42-59 (float), sssp.cpp:40-57 (int32, +1 per edge) and
+ * cc.cpp:40-56 (int32 min label), plus the weighted SSSP generalisation (uint32,
  * saturating add of the per-edge weight). */
 #define GMP_APP_PAGERANK 1
 #define GMP_APP_SSSP 2

@@ -52,9 +52,11 @@ def lib():
                                            _I32, _I32, _P, _P]
         L.orc_pr_f64_step_residual.argtypes = [_I64, _P, _P, _P, ctypes.c_double, _P, _I32, _P, _P]
         L.orc_mlp_loss_grad.argtypes = [_P, _P, _P, _P, _I64, _P, _P]
+        L.orc_pr_f64_refine.argtypes = [_I64, _P, _P, _P, ctypes.c_double, _P, _I32, _I32, ctypes.c_double,
+                                        _P, _P, _P]
         for f in ("orc_partition_bounds", "orc_app_sync", "orc_pr_f64", "orc_sssp_dijkstra",
                   "orc_cc_unionfind", "orc_graphmp_baseline", "orc_pr_f64_step_residual",
-                  "orc_mlp_loss_grad"):
+                  "orc_mlp_loss_grad", "orc_pr_f64_refine"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -174,3 +176,23 @@ def pr_step_residual(off, idx, outdeg, x, d=0.85, threads=0):
     _check(lib().orc_pr_f64_step_residual(V, _ptr(off), _ptr(idx), _ptr(od), d, _ptr(x), threads,
                                           ctypes.byref(l1), ctypes.byref(li)), "step_residual")
     return l1.value, li.value
+
+
+def pr_f64_error(off, idx, outdeg, x, d=0.85, threads=0, max_iter=60, bound=1e-11):
+    """Error of the FP32 ranks x against the FP64 fixed point of the standard formula: FP64
+    power iterations from x until the reference's own a-posteriori L1 bound (last change *
+    d/(1-d)) is below `bound`.  Returns a dict with the L1 and L-inf distances of x to that
+    reference, the reference's bound (each distance is exact to within it) and the steps run."""
+    V = off.size - 1
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    od = np.ascontiguousarray(outdeg, dtype=np.uint32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    ref = np.zeros(V, dtype=np.float64)
+    it = ctypes.c_int32(0)
+    last = ctypes.c_double(0)
+    _check(lib().orc_pr_f64_refine(V, _ptr(off), _ptr(idx), _ptr(od), d, _ptr(x), threads, max_iter, bound,
+                                   _ptr(ref), ctypes.byref(it), ctypes.byref(last)), "pr_f64_refine")
+    err = np.abs(x.astype(np.float64) - ref)
+    return {"l1_error": float(err.sum()), "linf_error": float(err.max()) if V else 0.0,
+            "reference_l1_bound": last.value * d / (1.0 - d), "reference_steps": it.value}

@@ -12,7 +12,7 @@
  *   orc_app_sync           GraphMP/system/GraphPS.h:211-386 run() loop (synchronous
  *                          Msg/MsgNew swap, changed count against VertexValue, stop rule
  *                          changed==0 && step>1) with the three per-vertex update rules
- *                          GraphMP/app/pagerank.cpp:42-59, sssp.cpp:144-162, cc.cpp:238-255
+ *                          GraphMP/app/pagerank.cpp:42-59, sssp.cpp:40-57, cc.cpp:40-56
  *   orc_pr_f64             textbook FP64 power iteration (uniform or single-seed teleport,
  *                          dangling mass redistributed to the teleport vector)
  *   orc_sssp_dijkstra      textbook binary-heap Dijkstra (pins Bellman-Ford / delta-stepping)
@@ -20,8 +20,12 @@
  *   orc_graphmp_baseline   GraphPS.h:256-386 engine shape for timing: OpenMP guided over
  *                          destination-interval shards, per-shard pull gather, sync swap,
  *                          selective scheduling of shards whose sources changed
+ *   orc_pr_f64_step_residual  one FP64 step from FP32 ranks: ||T(x)-x||_1 and _inf
+ *   orc_pr_f64_refine      parallel FP64 power iteration from FP32 ranks to an a-posteriori
+ *                          bound, the reference for measured L1 / L-inf errors
+ *   orc_mlp_loss_grad      FP64 loss and gradient of the scheduling MLP
  *
- * Pinning (which test checks which function) is listed in the test files and the design notes.
+ * Pinning (which test checks which function) is listed in tests/test_oracle.py.
  * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
  */
 #include <stdint.h>
@@ -113,7 +117,7 @@ int orc_app_sync(int32_t app, int64_t V, const int64_t *off, const uint32_t *idx
         int32_t best = m[v];
         for (int64_t k = off[v]; k < off[v + 1]; k++) {
           int32_t t = m[idx[k]];
-          if (app == ORC_APP_SSSP) t = (int32_t)((uint32_t)t + 1u); /* sssp.cpp:147 */
+          if (app == ORC_APP_SSSP) t = (int32_t)((uint32_t)t + 1u); /* sssp.cpp:43 */
           if (best > t) best = t;
         }
         mn[v] = best;
@@ -251,7 +255,7 @@ int orc_sssp_dijkstra(int64_t V, const int64_t *ooff, const uint32_t *oidx, cons
 /* ------------------------------------------------------------------------------------ */
 /* Union-find over any edge set given as a CSR; label[v] = minimum vertex id in v's
  * (weakly) connected component -- the unique fixed point of min-label propagation on
- * a symmetric graph (GraphMP/app/cc.cpp:238-244). */
+ * a symmetric graph (GraphMP/app/cc.cpp:40-45). */
 static int64_t uf_find(int64_t *p, int64_t x) {
   while (p[x] != x) {
     p[x] = p[p[x]];
@@ -481,6 +485,52 @@ int orc_pr_f64_step_residual(int64_t V, const int64_t *off, const uint32_t *idx,
   *l1 = s1;
   *linf = mx;
   free(c);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* FP64 power iteration of the standard formula (uniform teleport, as orc_pr_f64) started from
+ * the FP32 ranks x0 and run in parallel, for measuring the error of x0 against the FP64 fixed
+ * point x*.  Iterates until the a-posteriori bound ||x_k - x*||_1 <= ||x_k - x_{k-1}||_1 * d/(1-d)
+ * drops below `bound` or max_iter steps ran; x receives x_k, *iters the steps, *last_change the
+ * final ||x_k - x_{k-1}||_1 (so the caller can state how far x_k itself may be from x*). */
+int orc_pr_f64_refine(int64_t V, const int64_t *off, const uint32_t *idx, const uint32_t *outdeg,
+                      double d, const float *x0, int32_t threads, int32_t max_iter, double bound,
+                      double *x, int32_t *iters, double *last_change) {
+  if (V <= 0) return ORC_EINVAL;
+  if (threads <= 0) threads = omp_get_max_threads();
+  double *c = (double *)malloc((size_t)V * sizeof(double));
+  double *xn = (double *)malloc((size_t)V * sizeof(double));
+  if (!c || !xn) { free(c); free(xn); return ORC_ENOMEM; }
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (int64_t v = 0; v < V; v++) x[v] = (double)x0[v];
+  const double t = 1.0 / (double)V;
+  int32_t it = 0;
+  double r = 0;
+  while (it < max_iter) {
+    double D = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : D)
+    for (int64_t u = 0; u < V; u++) {
+      if (outdeg[u] == 0) { D += x[u]; c[u] = 0; }
+      else c[u] = x[u] / (double)outdeg[u];
+    }
+    r = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096) reduction(+ : r)
+    for (int64_t v = 0; v < V; v++) {
+      double s = 0;
+      for (int64_t k = off[v]; k < off[v + 1]; k++) s += c[idx[k]];
+      xn[v] = (1.0 - d) * t + d * (s + D * t);
+      r += fabs(xn[v] - x[v]);
+    }
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t v = 0; v < V; v++) x[v] = xn[v];
+    it++;
+    if (r * d / (1.0 - d) < bound) break;
+  }
+  *iters = it;
+  *last_change = r;
+  free(c);
+  free(xn);
   return ORC_OK;
 }
 

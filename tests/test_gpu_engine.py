@@ -533,6 +533,69 @@ def test_load_shards_missing_file_is_an_error(engine, tmp_path):
     assert ei.value.code == gmp.EIO
 
 
+def _shard_words(path):
+    """The int32 words of a shard file: header [edge_num_raw, nnz, len(indptr), start, end],
+    indices, indptr (GraphMP/system/Comp.h:37-42)."""
+    return np.load(path).astype(np.int64)
+
+
+def _corrupt_shard(tmp_path, how):
+    """A two-shard directory of a 100-vertex graph whose shard 1 is damaged as `how` says."""
+    off, idx = graphs.random_graph(100, 300, seed=2)
+    gmp.write_shards(tmp_path, off, idx, np.array([0, 50, 100], dtype=np.uint32))
+    p = tmp_path / "1.edge.npy"
+    w = _shard_words(p)
+    nnz = int(w[1])
+    if how == "indptr_length":  # one indptr entry dropped: len(indptr) != end - start + 1
+        w = np.concatenate([w[:5], w[5:5 + nnz], w[5 + nnz:-1]])
+        w[2] -= 1
+        np.save(p, w.astype(np.int32))
+    elif how == "source_out_of_range":  # a source id equal to V
+        w[5] = 100
+        np.save(p, w.astype(np.int32))
+    elif how == "truncated_data":  # the file ends 40 bytes before its declared shape
+        raw = p.read_bytes()
+        p.write_bytes(raw[:-40])
+    elif how == "truncated_header":  # the file ends inside the .npy header
+        raw = p.read_bytes()
+        p.write_bytes(raw[:30])
+    elif how == "short_shard_header":  # fewer than the 5 shard-header words
+        np.save(p, np.array([3, 0, 1], dtype=np.int32))
+    elif how == "big_endian":  # cnpy.h:100-101 refuses big-endian data
+        np.save(p, w.astype(">i4"))
+    elif how == "two_dimensional":  # cnpy.h:90 reads 1-D arrays only
+        np.save(p, np.zeros((2, 3), dtype=np.int32))
+    elif how == "gap_between_shards":  # shard 1 starts one row after shard 0 ends
+        w[3] += 1
+        w = np.concatenate([w[:-1]])
+        w[2] -= 1
+        np.save(p, w.astype(np.int32))
+    else:
+        raise AssertionError(how)
+    return off, idx
+
+
+@pytest.mark.parametrize("how,code", [("indptr_length", "EFORMAT"), ("source_out_of_range", "EFORMAT"),
+                                      ("truncated_data", "EIO"), ("truncated_header", "EFORMAT"),
+                                      ("short_shard_header", "EFORMAT"), ("big_endian", "EFORMAT"),
+                                      ("two_dimensional", "EFORMAT"), ("gap_between_shards", "EFORMAT")])
+def test_load_shards_malformed_is_an_error(engine, tmp_path, how, code):
+    """Damaged shards give a status code through the C ABI (GraphMP aborts or asserts:
+    cnpy.h:122-133, Comp.h:37-42), and the engine still loads a good graph afterwards."""
+    say(f"malformed shard: {how}")
+    off, idx = _corrupt_shard(tmp_path, how)
+    with pytest.raises(gmp.GmpError) as ei:
+        engine.load_shards(tmp_path, 100, 2)
+    assert ei.value.code == getattr(gmp, code), (how, ei.value.code, str(ei.value))
+    good = tmp_path / "good"
+    good.mkdir()
+    gmp.write_shards(good, off, idx, np.array([0, 50, 100], dtype=np.uint32))
+    engine.load_shards(good, 100, 2)
+    g = engine.download()
+    np.testing.assert_array_equal(g["off"], off)
+    np.testing.assert_array_equal(g["idx"], idx)
+
+
 def test_edgeless_and_tiny_graphs(engine):
     say("edgeless graph")
     V = 5

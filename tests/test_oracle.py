@@ -93,7 +93,7 @@ def test_stop_rule_counts_three_steps_on_empty_graph():
 def test_sssp_compat_is_bfs_levels():
     V = 400
     off, idx = graphs.random_graph(V, 1200, seed=7)
-    msg0 = np.full(V, 10000, dtype=np.int32)  # GPS_INF, sssp.cpp:175-176
+    msg0 = np.full(V, 10000, dtype=np.int32)  # GPS_INF, sssp.cpp:71-72
     msg0[1] = 0
     msg, _, _, _ = oracle.app_sync(oracle.APP_SSSP, off, idx, msg0, 1000)
     A = graphs.scipy_adj(V, off, idx)
@@ -313,3 +313,29 @@ def test_step_residual_bounds_distance_to_fixed_point():
         assert true <= l1 / (1 - d) + 1e-12
         assert l1 >= (1 - d) * true - 1e-12   # ||T(x)-x|| >= (1-d)||x-x*|| as well
         assert 0 < linf <= l1
+
+
+def test_pr_f64_error_matches_dense_solve():
+    """The refined FP64 reference agrees with the dense linear solve of the fixed point, so
+    the L1 / L-inf errors it reports for perturbed FP32 ranks are the true ones (within the
+    reference's own bound); a reference stopped early would report a visibly different L-inf."""
+    V, d = 200, 0.85
+    off, idx = graphs.random_graph(V, 900, seed=43)
+    od = graphs.out_degree(V, idx)
+    A = graphs.scipy_adj(V, off, idx).toarray()
+    t = np.full(V, 1.0 / V)
+    P = np.zeros((V, V))
+    for u in range(V):
+        P[:, u] = A[u] / od[u] if od[u] else t
+    xs = np.linalg.solve(np.eye(V) - d * P, (1 - d) * t)
+    rng = np.random.default_rng(5)
+    x32 = (xs * (1 + 1e-4 * rng.standard_normal(V))).astype(np.float32)
+    e = oracle.pr_f64_error(off, idx, od, x32, d=d, threads=2, max_iter=400, bound=1e-13)
+    true = np.abs(x32.astype(np.float64) - xs)
+    assert e["reference_l1_bound"] < 1e-13
+    assert abs(e["l1_error"] - true.sum()) <= e["reference_l1_bound"] + 1e-15
+    assert abs(e["linf_error"] - true.max()) <= e["reference_l1_bound"] + 1e-15
+    assert e["linf_error"] < e["l1_error"]
+    # two steps only: the reference is still far off, and its bound says so
+    e2 = oracle.pr_f64_error(off, idx, od, x32, d=d, threads=2, max_iter=2, bound=1e-13)
+    assert e2["reference_steps"] == 2 and e2["reference_l1_bound"] > 1e-9
