@@ -107,129 +107,93 @@ def batched_workload(seed=5000):
     return specs, gs, gp, rng
 
 
-def run_learned(eng, eval_scale, nsources=8, train_scales=(24, 25, 26), trace_sources=2):
-    """Learned delta-stepping: traces from SSSP runs at train_scales, MLP trained on the GPU,
-    then fixed delta=64 vs the learned schedule on R-MAT eval_scale for nsources seeded sources
-    (distances must be identical)."""
+def run_learned(eng, eval_scale, nsources=8, train_scales=(22, 23, 24, 25, 26), trace_sources=3):
+    """Learned delta-stepping on R-MAT eval_scale ef16 (weights 1..255) for nsources seeded
+    sources: traces from SSSP runs on R-MAT train_scales, the MLP trained on the GPU, then the
+    learned schedule -- the eight bucket widths 8..1024 as classes -- against fixed delta = 64 at
+    identical distances; beside it every fixed width, plain Bellman-Ford, and the variant whose
+    class 7 finishes by Bellman-Ford (its own traces and network)."""
+    import graphee_amd as gmp
     from graphee_amd import learned
-    t0 = time.perf_counter()
-    rows = learned.record(eng, [(s, 16, 1000 + s) for s in train_scales], sources_per_graph=trace_sources, seed=27)
-    t_rec = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    loss = learned.train(eng, rows, epochs=400)
-    t_train = time.perf_counter() - t0
-    acc = learned.accuracy(eng, rows)
-    log(f"learned: {len(rows)} trace rows in {t_rec:.1f} s, trained in {t_train:.2f} s, loss {loss:.3f}, {acc}")
+    out = {"eval_scale": eval_scale, "sources": nsources, "train_scales": list(train_scales),
+           "action_space": "bucket widths 8, 16, ..., 1024"}
+    nets = {}
+    for variant, finish in (("widths", False), ("finish", True)):
+        t0 = time.perf_counter()
+        rows = learned.record(eng, [(s, 16, 1000 + s) for s in train_scales], sources_per_graph=trace_sources,
+                              seed=27, finish=finish)
+        t_rec = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        loss = learned.train(eng, rows, epochs=400)
+        t_train = time.perf_counter() - t0
+        acc = learned.accuracy(eng, rows)
+        nets[variant] = eng.mlp_get()
+        labels = learned.labels(rows)
+        out[f"train_{variant}"] = {
+            "trace_rows": int(len(rows)), "trace_seconds": round(t_rec, 1), "train_seconds": round(t_train, 2),
+            "train_loss": round(loss, 4), "label_histogram": np.bincount(labels, minlength=8).tolist(),
+            "class_accuracy": round(acc["class_accuracy"], 3), "shard_precision": round(acc["shard_precision"], 3),
+            "shard_recall": round(acc["shard_recall"], 3)}
+        log(f"learned ({variant}): {len(rows)} trace rows in {t_rec:.1f} s, trained in {t_train:.2f} s, "
+            f"loss {loss:.3f}, labels {np.bincount(labels, minlength=8).tolist()}, {acc}")
     eng.generate_rmat(eval_scale, 16, 1000 + eval_scale, weighted=True, num_shards=16)
     V, E, _, _ = eng.info()
+    out.update(vertices=V, edges=E)
     rng = np.random.default_rng(2700 + eval_scale)
     srcs = [eng.pick_source(int(rng.integers(1 << 30))) for _ in range(nsources)]
-    eng.sssp(srcs[0], method=gmp_const("SSSP_DELTA"), delta=64, want_dist=False)  # builds layouts
-    eng.sssp(srcs[0], method=gmp_const("SSSP_BF"), want_dist=False)  # and the source strips
-    t_fix = t_lrn = 0.0
+    eng.sssp(srcs[0], method=gmp.SSSP_DELTA, delta=64, want_dist=False)  # builds the layouts
+    eng.sssp(srcs[0], method=gmp.SSSP_BF, want_dist=False)
+
+    def learned_run(variant, s, want):
+        eng.mlp_set(nets[variant])
+        eng.set_option(gmp.OPT_LEARNED_FINISH, int(variant == "finish"))
+        try:
+            return eng.sssp(s, method=gmp.SSSP_LEARNED, want_dist=want)
+        finally:
+            eng.set_option(gmp.OPT_LEARNED_FINISH, 0)
+
+    ms = {k: 0.0 for k in ("fixed_delta64", "learned", "learned_finish", "bf")}
+    wall = dict(ms)
+    fixed = {8 << c: 0.0 for c in range(8)}
+    hist = {"learned": np.zeros(8, np.int64), "learned_finish": np.zeros(8, np.int64)}
     same = True
-    widths = []
     for s in srcs:
-        d0, s0 = eng.sssp(s, method=gmp_const("SSSP_DELTA"), delta=64)
-        d1, s1 = eng.sssp(s, method=gmp_const("SSSP_LEARNED"))
-        same = same and bool(np.array_equal(d0, d1))
-        t_fix += s0.ms_total
-        t_lrn += s1.ms_total
-        widths.append(s1.epochs)
-    log(f"learned s{eval_scale}: fixed-64 {t_fix:.1f} ms vs learned {t_lrn:.1f} ms over {nsources} sources "
-        f"-> {t_fix / t_lrn:.2f}x, identical distances: {same}")
-    return {"eval_scale": eval_scale, "vertices": V, "edges": E, "sources": nsources,
-            "fixed_delta64_ms": round(t_fix, 3), "learned_ms": round(t_lrn, 3),
-            "speedup": round(t_fix / t_lrn, 3), "identical_distances": same,
-            "learned_epochs": widths, "trace_rows": int(len(rows)), "train_scales": list(train_scales),
-            "trace_seconds": round(t_rec, 1), "train_seconds": round(t_train, 2), "train_loss": round(loss, 4),
-            "train_class_accuracy": round(acc["class_accuracy"], 3),
-            "shard_precision": round(acc["shard_precision"], 3), "shard_recall": round(acc["shard_recall"], 3)}
-
-
-def run_pagerank_s20(eng, threads):
-    """PageRank on directed R-MAT s20 ef16 in 16 destination-interval shards, exactly 20
-    iterations from 1/N, FP32 ranks: GraphMP's CPU engine shape (the oracle's OpenMP
-    restatement, timed) beside the GPU, each compared with FP64 -- the same 20 iterations in
-    FP64 (arithmetic error) and the FP64 fixed point (arithmetic + truncation) -- plus the
-    reference's own update rule (GraphMP/app/pagerank.cpp:42-59, run by gmp_run_app) bit for bit
-    against the oracle's restatement."""
-    import graphee_amd as gmp
-    import oracle
-    scale = 20
-    eng.generate_rmat(scale, 16, 1000 + scale, num_shards=16)
-    V, E, P, _ = eng.info()
-    g = eng.download()
-    msg0 = np.full(V, 1.0 / V, dtype=np.float32)
-    x_cpu, stc = oracle.graphmp_baseline(oracle.APP_PAGERANK, g["off"], g["idx"], g["bounds"], msg0, 20,
-                                         outdeg=g["outdeg"], threads=threads)
-    x20, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
-    eng.pagerank(damping=0.85, max_iter=20, want_ranks=False)
-    x_gpu, stg = eng.pagerank(damping=0.85, max_iter=20)
-    out = {"workload": f"pagerank-rmat-s{scale}-ef16, {P} shards, 20 iterations, FP32 ranks",
-           "vertices": V, "edges": E, "shards": P}
-    for name, x, secs, it in (("cpu", x_cpu, stc.seconds, stc.iterations), ("gpu", x_gpu, stg.ms_total / 1e3,
-                                                                           stg.iterations)):
-        d20 = np.abs(x.astype(np.float64) - x20)
-        fp = oracle.pr_f64_error(g["off"], g["idx"], g["outdeg"], x, threads=threads, max_iter=200, bound=1e-13)
-        out[name] = {"seconds": round(secs, 5), "iterations": it, "gteps": round(E * it / secs / 1e9, 3),
-                     "l1_vs_fp64_same_20_iterations": float(d20.sum()),
-                     "linf_vs_fp64_same_20_iterations": float(d20.max()),
-                     "l1_vs_fp64_fixed_point": fp["l1_error"], "linf_vs_fp64_fixed_point": fp["linf_error"],
-                     "fixed_point_reference_bound": fp["reference_l1_bound"]}
-    out["cpu"]["threads"] = stc.threads
-    out["gpu_speedup"] = round(stc.seconds / (stg.ms_total / 1e3), 2)
-    # the reference's own rule: sum in source order, (float)(rel*0.85 + 1.0/N), dangling mass dropped
-    vout = g["outdeg"].astype(np.int32)
-    m_gpu, _, _ = eng.run_app(gmp.APP_PAGERANK, msg0, 20, vertex_out=vout)
-    m_orc, _, it_orc, _ = oracle.app_sync(oracle.APP_PAGERANK, g["off"], g["idx"], msg0, 20, vout=vout)
-    out["graphmp_rule_bit_exact"] = bool(np.array_equal(m_gpu.view(np.uint32), m_orc.view(np.uint32)))
-    log(f"pagerank s{scale} x20: CPU {stc.seconds * 1e3:.1f} ms on {stc.threads} threads "
-        f"(L1/Linf vs FP64 {out['cpu']['l1_vs_fp64_same_20_iterations']:.2e}/"
-        f"{out['cpu']['linf_vs_fp64_same_20_iterations']:.2e}), GPU {stg.ms_total:.2f} ms "
-        f"({out['gpu']['l1_vs_fp64_same_20_iterations']:.2e}/{out['gpu']['linf_vs_fp64_same_20_iterations']:.2e}); "
-        f"GraphMP rule bit-exact: {out['graphmp_rule_bit_exact']}")
-    return out
-
-
-def run_sssp_s24(eng, threads):
-    """SSSP on directed R-MAT s24 ef16 with uint32 weights in [1,255] from a seeded source of
-    out-degree > 0: Bellman-Ford (pushes + strip sweeps, GraphMP's selective scheduling) and
-    fixed delta = 64 on the GPU, and the weighted GraphMP rule in the CPU engine restatement;
-    the three distance vectors must be identical."""
-    import graphee_amd as gmp
-    import oracle
-    scale = 24
-    eng.generate_rmat(scale, 16, 1000 + scale, weighted=True, num_shards=16)
-    V, E, _, _ = eng.info()
-    src = eng.pick_source(1000 + scale)
-    out = {"workload": f"sssp-rmat-s{scale}-ef16, weights 1..255", "vertices": V, "edges": E, "source": src}
-    dist = {}
-    for name, method in (("bf", gmp.SSSP_BF), ("delta64", gmp.SSSP_DELTA)):
-        eng.sssp(src, method=method, delta=64, want_dist=False)
-        d, st = eng.sssp(src, method=method, delta=64)
-        dist[name] = d
-        reached = int((d != 0xFFFFFFFF).sum())
-        out[name] = {"ms": round(st.ms_total, 3), "iterations": st.iterations, "edges_relaxed": st.edges,
-                     "work_gteps": round(st.edges / st.ms_total / 1e6, 3), "reached": reached,
-                     "yardstick_gbs": round(st.bytes / st.ms_total / 1e6, 1),
-                     "hbm_frac": round(st.bytes / st.ms_total / 1e6 / HBM_PEAK_GBS, 4)}
-    g = eng.download(weights=True)
-    # edges reachable from the source (Graph500-style GTEPS counts the reached part)
-    reach = dist["bf"] != 0xFFFFFFFF
-    e_reached = int(np.diff(g["off"].astype(np.int64))[reach].sum())
-    for name in ("bf", "delta64"):
-        out[name]["graph500_gteps"] = round(e_reached / out[name]["ms"] / 1e6, 3)
-    msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
-    msg0[src] = 0
-    dc, stc = oracle.graphmp_baseline(oracle.APP_SSSP_W, g["off"], g["idx"], g["bounds"], msg0, 10000, w=g["w"],
-                                      threads=threads)
-    out["cpu"] = {"seconds": round(stc.seconds, 3), "threads": stc.threads, "sweeps": stc.iterations}
-    out["identical"] = bool(np.array_equal(dc, dist["bf"]) and np.array_equal(dc, dist["delta64"]))
-    out["gpu_bf_speedup"] = round(stc.seconds * 1e3 / out["bf"]["ms"], 2)
-    out["gpu_delta64_speedup"] = round(stc.seconds * 1e3 / out["delta64"]["ms"], 2)
-    log(f"sssp s{scale}: BF {out['bf']['ms']:.1f} ms, delta-64 {out['delta64']['ms']:.1f} ms, CPU "
-        f"{stc.seconds:.2f} s ({stc.threads} threads); identical: {out['identical']}")
+        d0, _ = eng.sssp(s, method=gmp.SSSP_DELTA, delta=64)
+        for variant, key in (("widths", "learned"), ("finish", "learned_finish")):
+            d1, _ = learned_run(variant, s, True)
+            same = same and bool(np.array_equal(d0, d1))
+        db, _ = eng.sssp(s, method=gmp.SSSP_BF)
+        same = same and bool(np.array_equal(d0, db))
+        del d0, d1, db
+        # timed solves (distances stay on the device)
+        for key, fn in (("fixed_delta64", lambda: eng.sssp(s, method=gmp.SSSP_DELTA, delta=64, want_dist=False)),
+                        ("learned", lambda: learned_run("widths", s, False)),
+                        ("learned_finish", lambda: learned_run("finish", s, False)),
+                        ("bf", lambda: eng.sssp(s, method=gmp.SSSP_BF, want_dist=False))):
+            t0 = time.perf_counter()
+            _, st = fn()
+            wall[key] += (time.perf_counter() - t0) * 1e3
+            ms[key] += st.ms_total
+            if key in hist:
+                hist[key] += eng.sssp_actions()
+        for dl in fixed:
+            _, st = eng.sssp(s, method=gmp.SSSP_DELTA, delta=dl, want_dist=False)
+            fixed[dl] += st.ms_total
+    best = min(fixed, key=fixed.get)
+    out.update({
+        "fixed_delta64_ms": round(ms["fixed_delta64"], 3), "learned_ms": round(ms["learned"], 3),
+        "speedup": round(ms["fixed_delta64"] / ms["learned"], 3),
+        "learned_finish_ms": round(ms["learned_finish"], 3),
+        "learned_finish_speedup": round(ms["fixed_delta64"] / ms["learned_finish"], 3),
+        "bf_ms": round(ms["bf"], 3), "best_fixed_delta": best, "best_fixed_delta_ms": round(fixed[best], 3),
+        "fixed_delta_ms": {str(k): round(v, 3) for k, v in fixed.items()},
+        "wall_ms": {k: round(v, 3) for k, v in wall.items()},
+        "action_histogram": hist["learned"].tolist(), "action_histogram_finish": hist["learned_finish"].tolist(),
+        "identical_distances": same})
+    log(f"learned s{eval_scale} ({nsources} sources): fixed-64 {ms['fixed_delta64']:.1f} ms, learned widths "
+        f"{ms['learned']:.1f} ms ({out['speedup']:.2f}x, actions {hist['learned'].tolist()}), learned+finish "
+        f"{ms['learned_finish']:.1f} ms, BF {ms['bf']:.1f} ms, best fixed delta {best}: {fixed[best]:.1f} ms; "
+        f"identical distances: {same}")
     return out
 
 

@@ -21,6 +21,7 @@ EDGES_WEIGHTED, EDGES_SYMMETRIZE = 1, 2
 OPT_REORDER, OPT_STREAM_CAP, OPT_ROW_CAP, OPT_CHUNK, OPT_WG_PER_CU, OPT_VEC_MIN = 1, 2, 3, 4, 5, 6
 OPT_STRIP_LOG, OPT_BIN_LOG, OPT_BIN_TASK_LOG = 7, 8, 9
 OPT_EDGE_BUDGET_MB, OPT_WINDOW_MB, OPT_WINDOW_PACKED, OPT_PUSH_LOG, OPT_CC_UNION_FIND = 10, 11, 12, 13, 14
+OPT_DELTA_PULL_LOG, OPT_LEARNED_FINISH = 15, 16
 
 # every entry point include/gmp.h declares
 SYMBOLS = (
@@ -29,7 +30,7 @@ SYMBOLS = (
     "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_sssp", "gmp_cc",
     "gmp_set_option", "gmp_get_option", "gmp_batch_generate", "gmp_batch_info", "gmp_batch_download",
     "gmp_batch_sssp", "gmp_batch_ppr", "gmp_autotune", "gmp_tune_save", "gmp_tune_load",
-    "gmp_sssp_trace", "gmp_mlp_init", "gmp_mlp_set", "gmp_mlp_get", "gmp_mlp_forward", "gmp_mlp_loss_grad",
+    "gmp_sssp_trace", "gmp_sssp_actions", "gmp_mlp_init", "gmp_mlp_set", "gmp_mlp_get", "gmp_mlp_forward", "gmp_mlp_loss_grad",
     "gmp_mlp_train", "gmp_windows",
     "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards", "gmp_convert_edge_list",
 )
@@ -136,6 +137,7 @@ def lib():
         "gmp_autotune": ([_P, i32, ctypes.POINTER(TuneResult)], ctypes.c_int),
         "gmp_sssp_trace": ([_P, u32, ctypes.POINTER(SsspParams), _P, u32, _P, _P, ctypes.POINTER(Stats)],
                            ctypes.c_int),
+        "gmp_sssp_actions": ([_P, _P, u32], ctypes.c_int),
         "gmp_mlp_init": ([_P, u64], ctypes.c_int),
         "gmp_mlp_set": ([_P, _P], ctypes.c_int),
         "gmp_mlp_get": ([_P, _P], ctypes.c_int),
@@ -408,6 +410,12 @@ class Engine:
         check(lib().gmp_sssp_trace(self._h, source, ctypes.byref(p), _ptr(rows), cap, ctypes.byref(n), _ptr(out),
                                    ctypes.byref(st)))
         return rows[: n.value].copy(), out, st
+
+    def sssp_actions(self):
+        """Per-class counts of the bucket epochs the last learned SSSP run chose."""
+        c = np.zeros(MLP_CLASSES, dtype=np.uint32)
+        check(lib().gmp_sssp_actions(self._h, _ptr(c), MLP_CLASSES))
+        return c
 
     def mlp_init(self, seed):
         check(lib().gmp_mlp_init(self._h, seed))

@@ -145,6 +145,14 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       in(0, 1, "cc_union_find");
       e->opt.cc_union_find = (int)value;
       break;
+    case GMP_OPT_DELTA_PULL_LOG:
+      in(0, 31, "delta_pull_log");
+      e->opt.delta_pull_log = (uint32_t)value;
+      break;
+    case GMP_OPT_LEARNED_FINISH:
+      in(0, 1, "learned_finish");
+      e->opt.learned_finish = (int)value;
+      break;
     default:
       fail(GMP_EINVAL, "gmp_set_option: unknown key " + std::to_string(key));
   }
@@ -170,6 +178,8 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_WINDOW_PACKED: *value = e->opt.window_packed; break;
     case GMP_OPT_PUSH_LOG: *value = e->opt.push_log; break;
     case GMP_OPT_CC_UNION_FIND: *value = e->opt.cc_union_find; break;
+    case GMP_OPT_DELTA_PULL_LOG: *value = e->opt.delta_pull_log; break;
+    case GMP_OPT_LEARNED_FINISH: *value = e->opt.learned_finish; break;
     default: fail(GMP_EINVAL, "gmp_get_option: unknown key " + std::to_string(key));
   }
   GMP_API_END

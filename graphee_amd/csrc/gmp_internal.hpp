@@ -251,6 +251,8 @@ struct Options {
   int window_packed = 1;       // semi-external windows: varint-packed ids
   uint32_t push_log = 10;      // CC / Bellman-Ford: frontiers of at most V >> push_log push (0: sweeps only)
   int cc_union_find = 1;       // CC of a symmetric graph by sampled union-find (0: sweeps and pushes)
+  uint32_t delta_pull_log = 3; // delta-stepping rounds with more than E >> k frontier out-edges pull (0: never)
+  int learned_finish = 0;      // learned SSSP: class 7 finishes by Bellman-Ford instead of width 1024
 };
 
 // Many graphs resident at once as one disjoint union (graph k owns union ids
@@ -294,6 +296,7 @@ struct Engine {
   DevBuf s_a, s_b, s_c, s_d, s_e, s_f, s_part, s_scal;
   DevBuf s_gaps;  // graph build: runs of empty rows left to a kernel of their own (count, then runs)
   void *h_pinned = nullptr;  // small pinned host buffer for per-iteration readbacks
+  uint32_t last_actions[kMlpClasses] = {};  // classes the last learned SSSP run chose, per class
 };
 
 void set_last_error(const std::string &m);
@@ -424,6 +427,19 @@ void sweep_enter(SweepRun &r, const uint32_t *front, uint32_t n);
 // Runs one sweep; returns the number of vertices it changed.
 uint32_t sweep_step(SweepRun &r);
 uint32_t sweep_last(const SweepRun &r);
+// Delta-stepping pull rounds over the strip layout (plain distances, unit or stored weights):
+// the round relaxes exactly the out-edges of the near frontier -- the vertices whose near stamp
+// is fstamp (front lists them, for the strips to gather) -- like a push round would, with the
+// same appends: improved vertices below thr to near_out (stamp next_stamp, out-degrees summed
+// into near_deg), the others to far_out once (far_flag).  Nothing is synchronised.
+struct PullRound {
+  uint32_t *dist, *near_stamp, *far_flag, *near_out, *far_out, *near_cnt, *far_cnt, *changed;
+  const uint32_t *front, *deg;
+  unsigned long long *near_deg;
+  uint32_t nfront, fstamp, next_stamp, thr;
+};
+SweepRunPtr pull_open(Engine &e, bool unit);
+void pull_round(SweepRun &r, const PullRound &p);
 uint64_t sweep_edges(SweepRun &r);
 void complement_values(Engine &e, uint32_t *x, uint32_t n);
 // Row id of every edge of a CSR (device array of E entries).

@@ -298,6 +298,15 @@ float mlp_train(Engine &e, const float *X, const int32_t *y, const uint64_t *mas
 
 extern "C" {
 
+int gmp_sssp_actions(gmp_engine *h, uint32_t *counts, uint32_t n) {
+  GMP_API_BEGIN
+  if (!h || (n && !counts)) gmp::fail(GMP_EINVAL, "gmp_sssp_actions: null argument");
+  if (n > (uint32_t)gmp::kMlpClasses) gmp::fail(GMP_EINVAL, "gmp_sssp_actions: n above the class count");
+  const gmp::Engine &e = *reinterpret_cast<gmp::Engine *>(h);
+  for (uint32_t c = 0; c < n; c++) counts[c] = e.last_actions[c];
+  GMP_API_END
+}
+
 int gmp_mlp_init(gmp_engine *h, uint64_t seed) {
   GMP_API_BEGIN
   if (!h) gmp::fail(GMP_EINVAL, "null engine handle");

@@ -51,6 +51,7 @@ constexpr uint32_t kGroupWaveTasks = 256;      // wave tasks per workgroup task 
 constexpr uint32_t kCodePad = kRound;          // codes read past the last edge stay in bounds
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr uint32_t kNever = 0xFFFFFFFFu;       // strip flag: no source changed yet
+constexpr int kPullUnroll = 4;                 // pull rounds: rows per thread in flight (bin apply)
 
 // What a sweep over the strips computes.  PageRank sums contributions; the others keep
 // complemented values (c = ~x), so the least label or distance is the greatest c and 0 -- what
@@ -109,6 +110,15 @@ struct StripArgs {
   uint32_t *gflag;         // per strip: the last sweep that gathered it
   uint32_t sweep;          // this sweep's number, from 1
   unsigned long long *edges;  // edges gathered (tasks not skipped)
+  // delta-stepping pull rounds (fmask set): the values are plain distances in `dist`; a source
+  // offers only when fmask[u] == fstamp (it is in the round's near frontier), so the round
+  // relaxes exactly the frontier's out-edges; a destination whose distance drops below thr joins
+  // the next near frontier (near stamp next_stamp), one that stays at or above it the far pile
+  const uint32_t *fmask;
+  uint32_t fstamp, next_stamp, thr;
+  uint32_t *dist, *near_stamp, *near_out, *near_cnt, *far_flag, *far_out, *far_cnt;
+  const uint32_t *deg;          // out-degree by work id: the next frontier's out-edges
+  unsigned long long *near_deg; // are summed here
 };
 
 // ---------------------------------------------------------------- build kernels
@@ -459,7 +469,31 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
     const uint32_t v0 = gt.x << a.wlog;
     const uint32_t n = min(W, a.V - v0);
     __syncthreads();  // the previous task's readers are done
-    if (n == W && (v0 & 3u) == 0) {
+    if (MODE != kSweepPR && a.fmask) {
+      // pull round of delta-stepping: complemented distances of the frontier's sources, 0 (no
+      // offer) for every other source
+      uint32_t *su = reinterpret_cast<uint32_t *>(sv);
+      if (n == W && (v0 & 3u) == 0) {
+        const u32x4 *dd = reinterpret_cast<const u32x4 *>(a.dist + v0);
+        const u32x4 *ff = reinterpret_cast<const u32x4 *>(a.fmask + v0);
+        u32x4 *dst = reinterpret_cast<u32x4 *>(sv);
+        const uint32_t fs = a.fstamp;
+        for (uint32_t i = threadIdx.x; i < W / 4; i += kIterBlock) {
+          const u32x4 d = dd[i], f = ff[i];
+          u32x4 o;
+          o.x = f.x == fs ? ~d.x : 0u;
+          o.y = f.y == fs ? ~d.y : 0u;
+          o.z = f.z == fs ? ~d.z : 0u;
+          o.w = f.w == fs ? ~d.w : 0u;
+          dst[i] = o;
+        }
+      } else {
+        for (uint32_t i = threadIdx.x; i < n; i += kIterBlock) {
+          const uint32_t d = a.dist[v0 + i];
+          su[i] = a.fmask[v0 + i] == a.fstamp ? ~d : 0u;
+        }
+      }
+    } else if (n == W && (v0 & 3u) == 0) {
       // whole strip: 16-byte loads, four in flight per thread before their stores
       const u32x4 *src = reinterpret_cast<const u32x4 *>(vals + v0);
       u32x4 *dst = reinterpret_cast<u32x4 *>(sv);
@@ -682,6 +716,51 @@ __device__ inline void min_count_changed(uint32_t n, uint32_t *changed) {
   if (lane_id() == 0 && n) atomicAdd(changed, n);
 }
 
+// Pull round of delta-stepping: distance v takes the round's best offer m (complemented) when it
+// is shorter; the caller appends v to the next near frontier (*to_near) or the far pile (*to_far)
+// exactly as a push round would.  Each vertex is applied by one thread, so plain stores suffice.
+__device__ inline bool pull_apply(const StripArgs &a, uint32_t v, uint32_t m, uint32_t old, bool &to_near,
+                                  bool &to_far) {
+  to_near = to_far = false;
+  const uint32_t nd = ~m;
+  if (!m || nd >= old) return false;
+  a.dist[v] = nd;
+  if (nd < a.thr) {
+    a.near_stamp[v] = a.next_stamp;
+    to_near = true;
+  } else if (a.far_out && a.far_flag[v] == 0u) {
+    a.far_flag[v] = 1u;
+    to_far = true;
+  }
+  return true;
+}
+
+// End of a pull round's bin or split kernel: the block's changed count and the out-degree sum of
+// its near appends, one atomic each (every thread of the block calls it).
+template <int WAVES>
+__device__ inline void pull_count_block(const StripArgs &a, uint32_t changed, unsigned long long ndeg, uint32_t *sh) {
+  __shared__ unsigned long long shd[WAVES];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) ndeg += (unsigned long long)__shfl_xor((long long)ndeg, o, kWave);
+  changed = read_lane_u32(wave_scan_dpp(changed), kWave - 1);
+  __syncthreads();
+  if (lane_id() == 0) {
+    shd[wave_id()] = ndeg;
+    sh[wave_id()] = changed;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long d = 0;
+    uint32_t c = 0;
+    for (int w = 0; w < WAVES; w++) {
+      d += shd[w];
+      c += sh[w];
+    }
+    if (d) atomicAdd(a.near_deg, d);
+    if (c) atomicAdd(a.changed, c);
+  }
+}
+
 __device__ inline void min_apply(const StripArgs &a, uint32_t v, uint32_t m, uint32_t &changed) {
   const uint32_t old = a.nlab[v];
   if (m > old) {
@@ -710,13 +789,14 @@ __device__ inline uint32_t bin_live_whole(const StripArgs &a, uint32_t b, uint32
 
 __global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
   extern __shared__ uint32_t lds2u[];
-  __shared__ uint32_t wtot[kIterWaves], qtot[kIterWaves], live;
+  __shared__ uint32_t wtot[kIterWaves], qtot[kIterWaves], live, shp[kIterWaves + 1];
   const uint32_t R = 1u << a.rlog, ns = a.ns;
   const BinLds<uint32_t> L = bin_lds<uint32_t>(lds2u, R, ns);
   const uint32_t tid = threadIdx.x;
   const uint32_t per = (ns + kIterBlock - 1) / kIterBlock;
   const uint32_t s0 = min(tid * per, ns), s1 = min(s0 + per, ns);
   uint32_t changed = 0;
+  unsigned long long ndeg = 0;
   for (uint32_t t = blockIdx.x; t < a.nbt; t += gridDim.x) {
     const uint4 bt = a.btask[t];
     // a whole bin with no message of a gathered strip cannot change; a split bin's slice still
@@ -726,7 +806,41 @@ __global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
     if (n == 0 && bt.w == kNoSlot) continue;
     bin_messages(a, n, L, [&](uint32_t r, uint32_t x) { atomicMax(&L.acc[r], x); });
     __syncthreads();
-    if (bt.w == kNoSlot) {
+    if (bt.w == kNoSlot && a.fmask) {
+      // pull round: thread tid owns rows tid + k * kIterBlock (k < 16, bins of at most 2^14
+      // rows); the rows that join the near frontier or the far pile leave with one atomic per
+      // list and bin task
+      const uint32_t v0 = bt.x << a.rlog, nr = min(R, a.V - v0);
+      uint32_t nmask = 0, fmask = 0;
+      for (uint32_t k0 = 0; k0 * kIterBlock < nr; k0 += kPullUnroll) {
+        uint32_t m[kPullUnroll], old[kPullUnroll];
+#pragma unroll
+        for (int u = 0; u < kPullUnroll; u++) {
+          const uint32_t i = (k0 + u) * kIterBlock + tid;
+          m[u] = i < nr ? L.acc[i] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kPullUnroll; u++) old[u] = m[u] ? a.dist[v0 + (k0 + u) * kIterBlock + tid] : 0u;
+#pragma unroll
+        for (int u = 0; u < kPullUnroll; u++) {
+          const uint32_t v = v0 + (k0 + u) * kIterBlock + tid;
+          bool tn, tf;
+          if (pull_apply(a, v, m[u], old[u], tn, tf)) changed++;
+          nmask |= (tn ? 1u : 0u) << (k0 + u);
+          fmask |= (tf ? 1u : 0u) << (k0 + u);
+        }
+      }
+      uint32_t pn = block_append<kIterWaves>((uint32_t)__popc(nmask), a.near_cnt, shp);
+      uint32_t pf = a.far_out ? block_append<kIterWaves>((uint32_t)__popc(fmask), a.far_cnt, shp) : 0u;
+      for (uint32_t k = 0; nmask | fmask; k++, nmask >>= 1, fmask >>= 1) {
+        const uint32_t v = v0 + k * kIterBlock + tid;
+        if (nmask & 1u) {
+          a.near_out[pn++] = v;
+          ndeg += a.deg[v];
+        }
+        if (fmask & 1u) a.far_out[pf++] = v;
+      }
+    } else if (bt.w == kNoSlot) {
       // rows without a message hold 0 and cannot change: only the others read their value, kUnroll
       // loads in flight per thread
       const uint32_t v0 = bt.x << a.rlog, n = min(R, a.V - v0);
@@ -755,26 +869,50 @@ __global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
       for (uint32_t i = tid; i < R; i += kIterBlock) dst[i] = L.acc[i];
     }
   }
-  min_count_changed(changed, a.changed);
+  if (a.fmask) pull_count_block<kIterWaves>(a, changed, ndeg, shp);
+  else min_count_changed(changed, a.changed);
 }
 
 __global__ __launch_bounds__(kBlock) void min_split_apply_kernel(StripArgs a) {
   const uint32_t R = 1u << a.rlog;
   const uint32_t *part = reinterpret_cast<const uint32_t *>(a.bpart);
   const uint32_t nck = (R + kSplitRows - 1) / kSplitRows;  // row chunks per split bin
+  __shared__ uint32_t shp[kBlock / kWave + 1];
   uint32_t changed = 0;
+  unsigned long long ndeg = 0;
   for (uint32_t t = blockIdx.x; t < a.nsplit * nck; t += gridDim.x) {
     const uint4 sp = a.split[t / nck];
     const uint32_t r0 = (t % nck) * kSplitRows;
     const uint32_t v0 = sp.x << a.rlog;
     const uint32_t n = min(min(R, a.V - v0), r0 + kSplitRows);
+    if (a.fmask) {  // pull round: block-uniform steps, one atomic per list and step
+      for (uint32_t i0 = r0; i0 < n; i0 += kBlock) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t m = 0;
+        if (i < n)
+          for (uint32_t k = 0; k < sp.z; k++) m = max(m, part[((size_t)(sp.y + k) << a.rlog) + i]);
+        bool tn, tf;
+        if (pull_apply(a, v0 + i, m, m ? a.dist[v0 + i] : 0u, tn, tf)) changed++;
+        const uint32_t pn = block_append<kBlock / kWave>(tn ? 1u : 0u, a.near_cnt, shp);
+        if (tn) {
+          a.near_out[pn] = v0 + i;
+          ndeg += a.deg[v0 + i];
+        }
+        if (a.far_out) {
+          const uint32_t pf = block_append<kBlock / kWave>(tf ? 1u : 0u, a.far_cnt, shp);
+          if (tf) a.far_out[pf] = v0 + i;
+        }
+      }
+      continue;
+    }
     for (uint32_t i = r0 + threadIdx.x; i < n; i += kBlock) {
       uint32_t m = 0;
       for (uint32_t k = 0; k < sp.z; k++) m = max(m, part[((size_t)(sp.y + k) << a.rlog) + i]);
       min_apply(a, v0 + i, m, changed);
     }
   }
-  min_count_changed(changed, a.changed);
+  if (a.fmask) pull_count_block<kBlock / kWave>(a, changed, ndeg, shp);
+  else min_count_changed(changed, a.changed);
 }
 
 // Starting values: CC c = ~(original id) and every strip active; SSSP c = 0 (unreached) but
@@ -1239,6 +1377,54 @@ uint32_t sweep_step(SweepRun &r) {
 }
 
 uint32_t sweep_last(const SweepRun &r) { return r.last; }
+
+SweepRunPtr pull_open(Engine &e, bool unit) {
+  return sweep_open(e, unit ? 1 : 2, nullptr, nullptr);
+}
+
+void pull_round(SweepRun &r, const PullRound &p) {
+  const StripCsr &S = r.e.g.strips;
+  StripArgs &a = r.a;
+  hipStream_t st = r.e.stream;
+  a.sweep = ++r.last;
+  if (r.last == 1) {
+    sflag_fill_kernel<<<grid_for(r.nsall), kBlock, 0, st>>>(a.gflag, r.nsall, 0u);
+    GMP_CHECK_LAUNCH();
+  }
+  // only the strips holding a frontier vertex are gathered
+  sflag_fill_kernel<<<grid_for(r.nsall), kBlock, 0, st>>>(a.sflag, r.nsall, kNever);
+  GMP_CHECK_LAUNCH();
+  if (p.nfront) {
+    sflag_mark_kernel<<<grid_for(p.nfront), kBlock, 0, st>>>(p.front, p.nfront, S.wlog, r.last - 1, a.sflag);
+    GMP_CHECK_LAUNCH();
+  }
+  a.fmask = p.near_stamp;
+  a.fstamp = p.fstamp;
+  a.next_stamp = p.next_stamp;
+  a.thr = p.thr;
+  a.dist = p.dist;
+  a.near_stamp = p.near_stamp;
+  a.near_out = p.near_out;
+  a.near_cnt = p.near_cnt;
+  a.far_flag = p.far_flag;
+  a.far_out = p.far_out;
+  a.far_cnt = p.far_cnt;
+  a.deg = p.deg;
+  a.near_deg = p.near_deg;
+  a.changed = p.changed ? p.changed : r.changed.as<uint32_t>();
+  a.nlab = nullptr;
+  a.vstamp = nullptr;
+  if (r.mode == kSweepUnit) min_strip_gather_kernel<kSweepUnit><<<S.g1, kIterBlock, S.lds1, st>>>(a);
+  else if (r.mode == kSweepWeighted8) min_strip_gather_kernel<kSweepWeighted8><<<S.g1, kIterBlock, S.lds1, st>>>(a);
+  else min_strip_gather_kernel<kSweepWeighted><<<S.g1, kIterBlock, S.lds1, st>>>(a);
+  GMP_CHECK_LAUNCH();
+  min_bin_kernel<<<S.g2, kIterBlock, r.lds2, st>>>(a);
+  GMP_CHECK_LAUNCH();
+  if (S.nsplit) {
+    min_split_apply_kernel<<<S.g3, kBlock, 0, st>>>(a);
+    GMP_CHECK_LAUNCH();
+  }
+}
 
 uint64_t sweep_edges(SweepRun &r) {
   uint64_t *h = static_cast<uint64_t *>(r.e.h_pinned);

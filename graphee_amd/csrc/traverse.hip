@@ -40,6 +40,7 @@ enum RelaxMode { kModeWeighted = 0, kModeUnit = 1, kModeMinLabel = 2 };
 struct Counters {
   uint32_t near_cnt;
   uint32_t far_cnt;
+  unsigned long long near_deg;      // out-edges of the vertices appended to the near frontier
   uint32_t far_min;
   uint32_t choice;                  // policy output: bucket-width class
   uint32_t hub_cnt;                 // flagged hub slices of the current relax launch
@@ -85,6 +86,7 @@ struct RelaxArgs {
   const uint2 *slices;      // {hub index, slice number}
   uint32_t nslices;
   uint32_t *hub_list;       // flagged slices of this launch (hub_compact_kernel)
+  const uint32_t *deg;      // set: out-degrees, summed over the near appends into cnt->near_deg
 };
 
 constexpr int kRelaxUnroll = 4;  // edges per lane in flight in the relax loops
@@ -100,7 +102,7 @@ constexpr uint32_t kAppendCap = 512;  // entries per wave and list
 template <class OffT, int MODE>
 __device__ inline void relax_batch(const RelaxArgs<OffT> &a, const uint64_t (&edge)[kRelaxUnroll],
                                    const uint32_t (&du)[kRelaxUnroll], uint32_t live, unsigned long long &smask,
-                                   WaveAppend &wn, WaveAppend &wf) {
+                                   unsigned long long &ndeg, WaveAppend &wn, WaveAppend &wf) {
   uint32_t v[kRelaxUnroll], wt[kRelaxUnroll];
 #pragma unroll
   for (int q = 0; q < kRelaxUnroll; q++) {
@@ -157,6 +159,7 @@ __device__ inline void relax_batch(const RelaxArgs<OffT> &a, const uint64_t (&ed
       if ((nmask >> q) & 1u) {
         wn.buf[pos++] = v[q];
         if (a.track_shards) smask |= 1ull << (uint32_t)(((uint64_t)v[q] * kMlpShards) / a.V);
+        if (a.deg) ndeg += a.deg[v[q]];
       }
     wn.n += tnn;
     if (wn.n > kAppendCap - kWave * kRelaxUnroll) append_flush(wn, &a.cnt->near_cnt, a.near_out);
@@ -177,10 +180,15 @@ __device__ inline void relax_batch(const RelaxArgs<OffT> &a, const uint64_t (&ed
 
 template <class OffT>
 __device__ inline void relax_flush_counts(const RelaxArgs<OffT> &a, unsigned long long edges, unsigned long long smask,
-                                          WaveAppend &wn, WaveAppend &wf) {
+                                          unsigned long long ndeg, WaveAppend &wn, WaveAppend &wf) {
   append_flush(wn, &a.cnt->near_cnt, a.near_out);
   if (a.far_out) append_flush(wf, &a.cnt->far_cnt, a.far_out);
   if (lane_id() == 0 && edges) atomicAdd(&a.cnt->edges, edges);
+  if (a.deg) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ndeg += (unsigned long long)__shfl_xor((long long)ndeg, o, kWave);
+    if (lane_id() == 0 && ndeg) atomicAdd(&a.cnt->near_deg, ndeg);
+  }
   if (a.track_shards) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) smask |= (unsigned long long)__shfl_xor((long long)smask, o, kWave);
@@ -196,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
   __shared__ uint32_t s_app[kWavesPerBlock][2][kAppendCap];
   const uint32_t lane = lane_id(), wv = wave_id();
   const uint32_t nchunks = (a.nfront + kWave - 1) / kWave;
-  unsigned long long edges = 0, smask = 0;
+  unsigned long long edges = 0, smask = 0, ndeg = 0;
   WaveAppend wn{s_app[wv][0], 0u}, wf{s_app[wv][1], 0u};
   if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt->hub_cnt = 0;  // hub_compact_kernel counts after us
   for (uint32_t c = blockIdx.x * kWavesPerBlock + wv; c < nchunks; c += gridDim.x * kWavesPerBlock) {
@@ -250,11 +258,11 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(RelaxArgs<OffT> a) {
         e[q] = s_base[wv][lo] + (jj - s_start[wv][lo]);
         du[q] = s_val[wv][lo];
       }
-      relax_batch<OffT, MODE>(a, e, du, live, smask, wn, wf);
+      relax_batch<OffT, MODE>(a, e, du, live, smask, ndeg, wn, wf);
     }
     wave_sync();
   }
-  relax_flush_counts(a, edges, smask, wn, wf);
+  relax_flush_counts(a, edges, smask, ndeg, wn, wf);
 }
 
 // Lists the slices of the hubs flagged by this launch's relax_kernel (one thread per slice).
@@ -275,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void hub_relax_kernel(RelaxArgs<OffT> a) {
   __shared__ uint32_t s_app[kWavesPerBlock][2][kAppendCap];
   const uint32_t lane = lane_id(), wv = wave_id();
   const uint32_t nlist = a.cnt->hub_cnt;
-  unsigned long long edges = 0, smask = 0;
+  unsigned long long edges = 0, smask = 0, ndeg = 0;
   WaveAppend wn{s_app[wv][0], 0u}, wf{s_app[wv][1], 0u};
   for (uint32_t i = blockIdx.x * kWavesPerBlock + wv; i < nlist; i += gridDim.x * kWavesPerBlock) {
     const uint2 sl = a.slices[a.hub_list[i]];
@@ -294,10 +302,10 @@ __global__ __launch_bounds__(kBlock) void hub_relax_kernel(RelaxArgs<OffT> a) {
         e[q] = j < e1 ? j : e0;
         dv[q] = du;
       }
-      relax_batch<OffT, MODE>(a, e, dv, live, smask, wn, wf);
+      relax_batch<OffT, MODE>(a, e, dv, live, smask, ndeg, wn, wf);
     }
   }
-  relax_flush_counts(a, edges, smask, wn, wf);
+  relax_flush_counts(a, edges, smask, ndeg, wn, wf);
 }
 
 // Histograms of the live far pile for the epoch features: distance above the pile minimum in
@@ -377,9 +385,10 @@ constexpr uint32_t kSplitPer = 4;
 __global__ __launch_bounds__(kBlock) void far_split_kernel(const uint32_t *far, uint32_t n, const uint32_t *dist,
                                                            uint32_t thr_old, uint32_t thr_new, uint32_t *far_flag,
                                                            uint32_t *near_stamp, uint32_t stamp, uint32_t *near_out,
-                                                           uint32_t *far_next, Counters *cnt) {
+                                                           uint32_t *far_next, Counters *cnt, const uint32_t *deg) {
   __shared__ uint32_t shn[kWavesPerBlock + 1], shk[kWavesPerBlock + 1];
   const uint64_t per_block = (uint64_t)kBlock * kSplitPer;
+  unsigned long long ndeg = 0;
   for (uint64_t b0 = blockIdx.x * per_block; b0 < n; b0 += (uint64_t)gridDim.x * per_block) {
     uint32_t v[kSplitPer], d[kSplitPer];
 #pragma unroll
@@ -406,9 +415,17 @@ __global__ __launch_bounds__(kBlock) void far_split_kernel(const uint32_t *far, 
     uint32_t pk = block_append<kWavesPerBlock>((uint32_t)__popc(keep), &cnt->far_cnt, shk);
 #pragma unroll
     for (uint32_t k = 0; k < kSplitPer; k++) {
-      if ((to_near >> k) & 1u) near_out[pn++] = v[k];
+      if ((to_near >> k) & 1u) {
+        near_out[pn++] = v[k];
+        if (deg) ndeg += deg[v[k]];
+      }
       if ((keep >> k) & 1u) far_next[pk++] = v[k];
     }
+  }
+  if (deg) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ndeg += (unsigned long long)__shfl_xor((long long)ndeg, o, kWave);
+    if (lane_id() == 0 && ndeg) atomicAdd(&cnt->near_deg, ndeg);
   }
 }
 
@@ -709,14 +726,15 @@ bool hybrid_min(Engine &e, int kind, uint32_t src, uint32_t *front0, uint32_t n0
 }
 
 // Bucket epochs of delta-stepping.  FIXED: aligned buckets [k*delta, (k+1)*delta).  LEARNED:
-// the MLP picks each epoch's action from far-pile features (m = smallest live far distance):
-// class c < 7 the bucket [m, m + 8 * 2^c), class 7 (kFinishClass) the rest of the solve as
-// Bellman-Ford pushes and strip sweeps (hybrid_min) from the live far pile.  EXPLORE records traces: at checkpoint epochs, from a snapshot of the state,
-// every width class runs the epoch and then fixed width 64 runs to the end of the solve; the
-// class with the shortest such rollout is the label and the run continues with it.  Every
-// policy relaxes until no distance drops, so distances are the same.
+// the MLP picks each epoch's bucket from far-pile features (m = smallest live far distance):
+// class c the bucket [m, m + 8 * 2^c), widths 8 .. 1024; with the learned-finish option, class 7
+// instead finishes the solve as Bellman-Ford pushes and strip sweeps (hybrid_min) from the live
+// far pile.  EXPLORE records traces: at checkpoint epochs, from a snapshot of the state, every
+// class runs the epoch and then fixed width 64 runs to the end of the solve; the class with the
+// shortest such rollout is the label and the run continues with it.  Every policy relaxes until
+// no distance drops, so distances are the same.
 enum Policy { kPolicyBF = 0, kPolicyFixed = 1, kPolicyLearned = 2, kPolicyExplore = 3 };
-constexpr uint32_t kFinishClass = 7;
+constexpr uint32_t kFinishClass = 7;  // the finishing class, when the option enables it
 
 template <class OffT>
 void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gmp_trace_row *rows, uint32_t cap,
@@ -748,11 +766,15 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
   ensure_work_out(e);
   const int mode = unit ? kModeUnit : kModeWeighted;
   if (policy == kPolicyLearned && !e.mlp.trained) policy = kPolicyFixed;  // untrained network: plain schedule
+  if (policy == kPolicyLearned) std::fill(std::begin(e.last_actions), std::end(e.last_actions), 0u);
   const bool bucketed = policy != kPolicyBF;
   const uint32_t delta = p.delta ? p.delta : 64u;
   const uint32_t V = g.V;
   src = work_id(e, src);
   ensure_hubs<OffT>(e);
+  // pull rounds need the source-strip layout (and its weights): opened before the clock starts
+  SweepRunPtr pr;
+  if (e.opt.delta_pull_log && g.E) pr = pull_open(e, unit);
   TraverseBufs b = alloc_traverse(e, true);
   const CsrView ov = g.work_out();
   const OffT *ooff = ov.offs<OffT>();
@@ -791,8 +813,12 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
   a.cnt = b.cnt;
   a.V = V;
   attach_hubs(e, a);
-  int iters = 0, epochs = 0;
+  int iters = 0, epochs = 0, pulls = 0;
   Counters h{};
+  // rounds whose frontier has more than E >> delta_pull_log out-edges pull over the source strips
+  const uint64_t pull_edges = e.opt.delta_pull_log ? (g.E >> e.opt.delta_pull_log) : ~0ull;
+  if (pr) a.deg = g.work_deg();
+  uint64_t front_deg = 0;  // out-edges of the current near frontier (when pull rounds are possible)
 
   auto relax_until_empty = [&](bool track) {
     while (nnear) {
@@ -803,23 +829,46 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
       a.threshold = thr;
       a.far_out = bucketed ? far_cur + nfar : nullptr;
       a.track_shards = track ? 1 : 0;
-      GMP_HIP(hipMemsetAsync(&b.cnt->near_cnt, 0, 8, e.stream));  // near_cnt, far_cnt
-      launch_relax<OffT>(e, mode, a);
+      GMP_HIP(hipMemsetAsync(&b.cnt->near_cnt, 0, 16, e.stream));  // near_cnt, far_cnt, near_deg
+      if (pr && front_deg > pull_edges && !track) {
+        // pull: the frontier's vertices carry the stamp of the launch that appended them
+        PullRound q{};
+        q.dist = b.dist;
+        q.near_stamp = b.stamp;
+        q.far_flag = b.farflag;
+        q.near_out = near_out;
+        q.far_out = a.far_out;
+        q.near_cnt = &b.cnt->near_cnt;
+        q.far_cnt = &b.cnt->far_cnt;
+        q.near_deg = &b.cnt->near_deg;
+        q.front = near_in;
+        q.deg = a.deg;
+        q.nfront = nnear;
+        q.fstamp = a.stamp - 1u;
+        q.next_stamp = a.stamp;
+        q.thr = thr;
+        pull_round(*pr, q);
+        pulls++;
+      } else {
+        launch_relax<OffT>(e, mode, a);
+      }
       read_counters(e, b.cnt, h);
       nnear = h.near_cnt;
       nfar += h.far_cnt;
+      front_deg = h.near_deg;
       std::swap(near_in, near_out);
       iters++;
     }
   };
   auto split = [&](uint32_t thr_old, uint32_t thr_new) {
-    GMP_HIP(hipMemsetAsync(&b.cnt->near_cnt, 0, 8, e.stream));
+    GMP_HIP(hipMemsetAsync(&b.cnt->near_cnt, 0, 16, e.stream));
     far_split_kernel<<<grid_fill(e, nfar), kBlock, 0, e.stream>>>(far_cur, nfar, b.dist, thr_old, thr_new, b.farflag,
-                                                                 b.stamp, stamp++, near_in, far_nxt, b.cnt);
+                                                                 b.stamp, stamp++, near_in, far_nxt, b.cnt, a.deg);
     GMP_CHECK_LAUNCH();
     read_counters(e, b.cnt, h);
     nnear = h.near_cnt;
     nfar = h.far_cnt;
+    front_deg = h.near_deg;
     std::swap(far_cur, far_nxt);
     thr = thr_new;
   };
@@ -868,7 +917,7 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
   // trace rollout: one epoch of width class c, then width 64 (class 3, the fixed schedule's) to the
   // end of the run
   auto run_rollout = [&](uint32_t m, uint32_t c) {
-    if (c == kFinishClass) {
+    if (c == kFinishClass && e.opt.learned_finish) {
       finish();
       return;
     }
@@ -953,7 +1002,8 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
       row.best = (int32_t)cur_class;
       GMP_HIP(hipMemsetAsync(&b.cnt->shard_mask, 0, 8, e.stream));
     }
-    if (cur_class == kFinishClass) {
+    if (policy == kPolicyLearned) e.last_actions[cur_class]++;
+    if (cur_class == kFinishClass && e.opt.learned_finish) {
       finish();
       if (explore) {  // the finish does not track shards: take them from the changed distances
         changed_shards_kernel<<<grid_fill(e, V), kBlock, 0, e.stream>>>(sd.as<uint32_t>(), b.dist, V, b.cnt);
@@ -972,6 +1022,7 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
   GMP_HIP(hipEventRecord(tot.b, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
   read_counters(e, b.cnt, h);
+  const uint64_t pulled = pr ? sweep_edges(*pr) : 0;  // edges of the strips the pull rounds gathered
   finish_output(e, b.dist, dist_out);
   if (st) {
     std::memset(st, 0, sizeof *st);
@@ -979,10 +1030,11 @@ void sssp_impl(Engine &e, uint32_t src, const gmp_sssp_params &p, int policy, gm
     st->launches = iters;
     st->ms_total = tot.ms();
     st->ms_kernel = st->ms_total;
-    st->edges = h.edges;
-    st->bytes = h.edges * (unit ? 8 : 12);
+    st->edges = h.edges + pulled;
+    st->bytes = st->edges * (unit ? 8 : 12);
     st->epochs = epochs;
     st->policy = policy;
+    st->changed = (uint64_t)pulls;
   }
 }
 

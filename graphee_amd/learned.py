@@ -1,31 +1,40 @@
 """Learned delta-stepping schedule: record traces, train the MLP on the GPU, run it.
 
 The engine records a trace row at checkpoint bucket epochs of SSSP (`Engine.sssp_trace`): the
-16 features of the far pile, and for every width class the time and relaxations of finishing the
-solve from that state (one epoch of the class, then fixed width 64; class 7 finishes the solve
-with Bellman-Ford pushes and strip sweeps), the fastest class and the
-shards that became active.  `train` fits the network with the engine's own HIP
+16 features of the far pile, and for every class the time and relaxations of finishing the
+solve from that state (one epoch of the class, then fixed width 64), the fastest class and the
+shards that became active.  The eight classes are the bucket widths 8, 16, ..., 1024; with
+`finish=True` (the engine's learned-finish option) class 7 instead finishes the solve as
+Bellman-Ford pushes and strip sweeps.  `train` fits the network with the engine's own HIP
 forward/backward/Adam kernels; `Engine.sssp(method=SSSP_LEARNED)` then runs inference on the GPU
 at every epoch.  The network only picks how the next epoch relaxes, so distances equal those of
 every other method.
 """
 import numpy as np
 
-from . import SSSP_BF, SSSP_LEARNED, Engine  # noqa: F401
+from . import OPT_LEARNED_FINISH, SSSP_BF, SSSP_DELTA, SSSP_LEARNED, Engine  # noqa: F401
 
 
-def record(engine, graphs, sources_per_graph=2, seed=0):
-    """Traces from SSSP runs on R-MAT graphs given as (scale, edge_factor, seed) tuples."""
+def record(engine, graphs, sources_per_graph=2, seed=0, finish=False):
+    """Traces from SSSP runs on R-MAT graphs given as (scale, edge_factor, seed) tuples; with
+    finish=True class 7 is the Bellman-Ford finish instead of width 1024."""
     rows = []
     rng = np.random.default_rng(seed)
-    for scale, ef, gseed in graphs:
-        engine.generate_rmat(scale, ef, gseed, weighted=True)
-        # build the layouts every action uses (out-CSR, source strips) before anything is timed
-        engine.sssp(engine.pick_source(gseed), method=SSSP_BF, want_dist=False)
-        for _ in range(sources_per_graph):
-            src = engine.pick_source(int(rng.integers(1 << 30)))
-            r, _, _ = engine.sssp_trace(src, want_dist=False)
-            rows.append(r)
+    saved = engine.get_option(OPT_LEARNED_FINISH)
+    engine.set_option(OPT_LEARNED_FINISH, int(finish))
+    try:
+        for scale, ef, gseed in graphs:
+            engine.generate_rmat(scale, ef, gseed, weighted=True)
+            # build the layouts every action uses (out-CSR, hub table, source strips) first
+            src0 = engine.pick_source(gseed)
+            engine.sssp(src0, method=SSSP_BF, want_dist=False)
+            engine.sssp(src0, method=SSSP_DELTA, delta=64, want_dist=False)
+            for _ in range(sources_per_graph):
+                src = engine.pick_source(int(rng.integers(1 << 30)))
+                r, _, _ = engine.sssp_trace(src, want_dist=False)
+                rows.append(r)
+    finally:
+        engine.set_option(OPT_LEARNED_FINISH, saved)
     return np.concatenate(rows) if rows else np.zeros(0)
 
 

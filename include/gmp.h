@@ -139,6 +139,15 @@ int gmp_destroy(gmp_engine *h);
 #define GMP_OPT_CC_UNION_FIND 14   /* 1 (default): gmp_cc on a symmetric graph (undirected R-MAT,
                                      edge lists loaded with GMP_EDGES_SYMMETRIZE) by sampled
                                      union-find; 0: min-label sweeps and pushes.  Same labels. */
+#define GMP_OPT_DELTA_PULL_LOG 15  /* delta-stepping SSSP (fixed and learned): a round whose near
+                                     frontier has more than E >> k out-edges relaxes them by a pull
+                                     over the source strips (only the frontier's sources offer)
+                                     instead of pushes; 0 = pushes only, 0..31 (default 3).  The
+                                     relaxations, and the distances, are the same. */
+#define GMP_OPT_LEARNED_FINISH 16  /* GMP_SSSP_LEARNED and gmp_sssp_trace: 0 (default) the eight classes
+                                     are bucket widths 8 .. 1024; 1: class 7 instead finishes the
+                                     solve as Bellman-Ford (pushes and strip sweeps) from the live
+                                     far pile.  Distances are the same. */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 
@@ -233,10 +242,10 @@ int gmp_cc(gmp_engine *h, uint32_t *label_out, gmp_stats *st);
  * every other method.  Until it is trained or set, GMP_SSSP_LEARNED runs fixed delta.
  *
  * gmp_sssp_trace runs SSSP while recording a row at explored bucket epochs (the first four,
- * then every power of two): the 16 features, and for every width class the device time and
- * relaxations of finishing the run from that state with one epoch of that width followed by
- * the widest class; the best class (fastest finish), which the run then uses, and the shards
- * (64ths of the vertex range) that joined the frontier in that epoch. */
+ * then every power of two): the 16 features, and for every class the device time and
+ * relaxations of finishing the run from that state with one epoch of that class followed by
+ * width 64; the best class (fastest finish), which the run then uses, and the shards (64ths of
+ * the vertex range) that joined the frontier in that epoch. */
 #define GMP_MLP_IN 16
 #define GMP_MLP_HIDDEN 64
 #define GMP_MLP_CLASSES 8
@@ -252,6 +261,9 @@ typedef struct gmp_trace_row {
 } gmp_trace_row;
 int gmp_sssp_trace(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, gmp_trace_row *rows,
                    uint32_t cap, uint32_t *nrows, uint32_t *dist_out, gmp_stats *st);
+/* Per-class counts of the bucket epochs the last GMP_SSSP_LEARNED run of this engine chose
+ * (counts[c], c < n <= GMP_MLP_CLASSES). */
+int gmp_sssp_actions(gmp_engine *h, uint32_t *counts, uint32_t n);
 int gmp_mlp_init(gmp_engine *h, uint64_t seed);
 int gmp_mlp_set(gmp_engine *h, const float *params);
 int gmp_mlp_get(gmp_engine *h, float *params);

@@ -188,6 +188,39 @@ def test_sssp_bellman_ford_sweeps_and_frontier_agree(engine, strip_log, bin_log,
             engine.set_option(k, v)
 
 
+@pytest.mark.parametrize("pull_log", [0, 2, 31])
+@pytest.mark.parametrize("strip_log,bin_log,task_log", [(15, 14, 20), (8, 8, 8), (11, 10, 10)])
+def test_delta_stepping_pull_rounds_keep_distances(engine, pull_log, strip_log, bin_log, task_log):
+    """Delta-stepping rounds that pull over the source strips (only the near frontier's sources
+    offer, the bins append to the next frontier and the far pile as a push would) in place of
+    pushes -- pull_log 0: never, 2: rounds above E/4 frontier out-edges, 31: every round -- give
+    Dijkstra's distances for narrow, medium and wide buckets, stored and unit weights, and on
+    strip widths, bin sizes and split bins of every kind."""
+    keys = (gmp.OPT_DELTA_PULL_LOG, gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG)
+    saved = [engine.get_option(k) for k in keys]
+    for k, v in zip(keys, (pull_log, strip_log, bin_log, task_log)):
+        engine.set_option(k, v)
+    try:
+        g = gen(engine, 13, weighted=True)
+        V = g["outdeg"].size
+        ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+        for s in (engine.pick_source(3), engine.pick_source(4)):
+            want = oracle.dijkstra(ooff, oidx, ow, s)
+            for delta in (1, 64, 1000):
+                say(f"pull_log {pull_log} strips {strip_log}/{bin_log}/{task_log}: delta {delta} src={s}")
+                d, st = engine.sssp(s, method=gmp.SSSP_DELTA, delta=delta)
+                np.testing.assert_array_equal(d, want, err_msg=f"delta {delta}")
+                if pull_log == 0:
+                    assert st.changed == 0  # no pull round
+                if pull_log == 31:
+                    assert st.changed > 0  # pull rounds ran
+            d, _ = engine.sssp(s, method=gmp.SSSP_DELTA, delta=8, unit_weights=True)
+            np.testing.assert_array_equal(d, oracle.dijkstra(ooff, oidx, None, s))
+    finally:
+        for k, v in zip(keys, saved):
+            engine.set_option(k, v)
+
+
 def test_sssp_unit_weights_is_bfs(engine):
     g = gen(engine, 12)
     V = g["outdeg"].size
@@ -704,6 +737,10 @@ def test_multigraph_self_loops_and_hubs(engine):
         for method in (gmp.SSSP_BF, gmp.SSSP_DELTA):
             d, _ = engine.sssp(s, method=method)
             np.testing.assert_array_equal(d, want, err_msg=f"source {s} method {method}")
+        engine.set_option(gmp.OPT_DELTA_PULL_LOG, 31)  # every delta-stepping round pulls
+        d, _ = engine.sssp(s, method=gmp.SSSP_DELTA, delta=32)
+        engine.set_option(gmp.OPT_DELTA_PULL_LOG, 3)
+        np.testing.assert_array_equal(d, want, err_msg=f"source {s} pulled")
         du, _ = engine.sssp(s, unit_weights=True)
         np.testing.assert_array_equal(du, oracle.dijkstra(ooff, oidx, np.ones_like(ow), s))
     say("multigraph compat apps")
