@@ -22,12 +22,13 @@ OPT_REORDER, OPT_STREAM_CAP, OPT_ROW_CAP, OPT_CHUNK, OPT_WG_PER_CU, OPT_VEC_MIN 
 OPT_STRIP_LOG, OPT_BIN_LOG, OPT_BIN_TASK_LOG = 7, 8, 9
 OPT_EDGE_BUDGET_MB, OPT_WINDOW_MB, OPT_WINDOW_PACKED, OPT_PUSH_LOG, OPT_CC_UNION_FIND = 10, 11, 12, 13, 14
 OPT_DELTA_PULL_LOG, OPT_LEARNED_FINISH = 15, 16
+NOALLOC_MAX_VERTICES = 1 << 22
 
 # every entry point include/gmp.h declares
 SYMBOLS = (
     "gmp_last_error", "gmp_abi_version", "gmp_create", "gmp_destroy", "gmp_load_csr",
     "gmp_load_shards", "gmp_load_edge_list", "gmp_generate_rmat", "gmp_graph_info", "gmp_graph_download",
-    "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_sssp", "gmp_cc",
+    "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_pagerank_noalloc", "gmp_sssp", "gmp_cc",
     "gmp_set_option", "gmp_get_option", "gmp_batch_generate", "gmp_batch_info", "gmp_batch_download",
     "gmp_batch_sssp", "gmp_batch_ppr", "gmp_autotune", "gmp_tune_save", "gmp_tune_load",
     "gmp_sssp_trace", "gmp_sssp_actions", "gmp_mlp_init", "gmp_mlp_set", "gmp_mlp_get", "gmp_mlp_forward", "gmp_mlp_loss_grad",
@@ -154,6 +155,8 @@ def lib():
         "gmp_get_option": ([_P, i32, _P], ctypes.c_int),
         "gmp_run_app": ([_P, i32, i32, _P, _P, _P, _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_pagerank": ([_P, ctypes.POINTER(PrParams), _P, ctypes.POINTER(Stats)], ctypes.c_int),
+        "gmp_pagerank_noalloc": ([_P, u32, u64, _P, _P, _P, ctypes.POINTER(PrParams), _P, ctypes.POINTER(Stats)],
+                                 ctypes.c_int),
         "gmp_sssp": ([_P, u32, ctypes.POINTER(SsspParams), _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_cc": ([_P, _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_partition_bounds": ([_P, u32, u32, _P], ctypes.c_int),
@@ -390,6 +393,17 @@ class Engine:
         st = Stats()
         check(lib().gmp_pagerank(self._h, ctypes.byref(p), _ptr(out), ctypes.byref(st)))
         return out, st
+
+    def pagerank_noalloc(self, V, E, d_row_off, d_col_idx, d_out_degree, d_rank_out, damping=0.85, max_iter=20,
+                         tol=0.0, seed_vertex=-1, kahan=True):
+        """gmp_pagerank_noalloc: the CSR and the rank buffer are device addresses (ints) owned by
+        the caller; nothing is allocated."""
+        p = PrParams(damping, max_iter, tol, seed_vertex, 0 if kahan else PR_NO_KAHAN, 0)
+        st = Stats()
+        check(lib().gmp_pagerank_noalloc(self._h, V, E, ctypes.c_void_p(d_row_off), ctypes.c_void_p(d_col_idx),
+                                         ctypes.c_void_p(d_out_degree), ctypes.byref(p), ctypes.c_void_p(d_rank_out),
+                                         ctypes.byref(st)))
+        return st
 
     def sssp(self, source, method=SSSP_BF, delta=64, unit_weights=False, want_dist=True):
         V = self.info()[0]

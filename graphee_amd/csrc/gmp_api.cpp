@@ -49,9 +49,14 @@ int gmp_create(int32_t device, gmp_engine **out) {
     e->num_cus = cus > 0 ? cus : 256;
     GMP_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     GMP_HIP(hipHostMalloc(&e->h_pinned, 256, hipHostMallocDefault));
+    GMP_HIP(hipEventCreate(&e->na_ev[0]));
+    GMP_HIP(hipEventCreate(&e->na_ev[1]));
+    gmp::noalloc_attach(*e);  // the scratch array of gmp_pagerank_noalloc on this device
   } catch (...) {
     if (e->stream) (void)hipStreamDestroy(e->stream);
     if (e->h_pinned) (void)hipHostFree(e->h_pinned);
+    for (hipEvent_t &x : e->na_ev)
+      if (x) (void)hipEventDestroy(x);
     delete e;
     throw;
   }
@@ -66,6 +71,8 @@ int gmp_destroy(gmp_engine *h) {
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   if (e->h_pinned) (void)hipHostFree(e->h_pinned);
+  for (hipEvent_t x : e->na_ev)
+    if (x) (void)hipEventDestroy(x);
   hipStream_t s = e->stream;
   delete e;
   if (s) (void)hipStreamDestroy(s);

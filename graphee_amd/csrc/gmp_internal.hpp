@@ -297,6 +297,8 @@ struct Engine {
   DevBuf s_gaps;  // graph build: runs of empty rows left to a kernel of their own (count, then runs)
   void *h_pinned = nullptr;  // small pinned host buffer for per-iteration readbacks
   uint32_t last_actions[kMlpClasses] = {};  // classes the last learned SSSP run chose, per class
+  void *scratch = nullptr;       // device address of noalloc.hip's scratch array on this device
+  hipEvent_t na_ev[2] = {};      // timing events of gmp_pagerank_noalloc (created with the engine)
 };
 
 void set_last_error(const std::string &m);
@@ -457,6 +459,11 @@ void mlp_forward(Engine &e, const float *X, uint32_t n, float *Y);
 float mlp_loss_grad(Engine &e, const float *X, const int32_t *y, const uint64_t *mask, uint32_t n, float *grad);
 float mlp_train(Engine &e, const float *X, const int32_t *y, const uint64_t *mask, uint32_t n, int32_t epochs,
                 float lr);
+
+// ---- noalloc.hip ----
+// Resolves the device address of the scratch array on e.device (current) into e.scratch, once per
+// device.
+void noalloc_attach(Engine &e);
 
 // ---- pagerank.hip ----
 void run_pagerank(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st);

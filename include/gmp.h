@@ -226,6 +226,20 @@ int gmp_run_app(gmp_engine *h, int32_t app, int32_t max_iter, const void *msg_in
 
 /* ---- engine apps -------------------------------------------------------------------- */
 int gmp_pagerank(gmp_engine *h, const gmp_pr_params *p, float *rank_out, gmp_stats *st);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (work under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): gmp_pagerank's
+ * standard PageRank (same parameters and formula; GMP_PR_COMPRESSED is refused) over a
+ * destination-row CSR the caller holds in DEVICE memory -- d_row_off uint64[V+1], d_col_idx
+ * uint32[E], d_out_degree uint32[V] -- with the ranks written to the caller's device buffer
+ * d_rank_out float[V].  It calls no hipMalloc/hipFree: its working memory is a fixed per-GPU
+ * device array sized for GMP_NOALLOC_MAX_VERTICES vertices (48 MiB), resolved by gmp_create and
+ * kept until the process ends; larger V is refused with GMP_EINVAL.  Calls on one GPU share that
+ * array: calls that overlap on one GPU (two engines or threads) race.  Runs on the engine's stream;
+ * returns after the last iteration finished (with tol > 0 it waits for every iteration's residual). */
+#define GMP_NOALLOC_MAX_VERTICES 4194304u
+int gmp_pagerank_noalloc(gmp_engine *h, uint32_t V, uint64_t E, const uint64_t *d_row_off,
+                         const uint32_t *d_col_idx, const uint32_t *d_out_degree, const gmp_pr_params *p,
+                         float *d_rank_out, gmp_stats *st);
 int gmp_sssp(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, uint32_t *dist_out,
              gmp_stats *st);
 /* Connected components of a symmetric graph: label = minimum vertex id in the component
