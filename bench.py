@@ -197,12 +197,141 @@ def run_learned(eng, eval_scale, nsources=8, train_scales=(22, 23, 24, 25, 26), 
     return out
 
 
+def run_pagerank_s20(eng, threads):
+    """PageRank on directed R-MAT s20 ef16 in 16 destination-interval shards, exactly 20
+    iterations from 1/N, FP32 ranks: GraphMP's CPU engine shape (the oracle's OpenMP
+    restatement, timed) beside the GPU, each compared with FP64 -- the same 20 iterations in
+    FP64 (arithmetic error) and the FP64 fixed point (arithmetic + truncation) -- plus the
+    reference's own update rule (GraphMP/app/pagerank.cpp:42-59, run by gmp_run_app) bit for bit
+    against the oracle's restatement."""
+    import graphee_amd as gmp
+    import oracle
+    scale = 20
+    eng.generate_rmat(scale, 16, 1000 + scale, num_shards=16)
+    V, E, P, _ = eng.info()
+    g = eng.download()
+    msg0 = np.full(V, 1.0 / V, dtype=np.float32)
+    x_cpu, stc = oracle.graphmp_baseline(oracle.APP_PAGERANK, g["off"], g["idx"], g["bounds"], msg0, 20,
+                                         outdeg=g["outdeg"], threads=threads)
+    x20, _, _ = oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=20)
+    eng.pagerank(damping=0.85, max_iter=20, want_ranks=False)
+    x_gpu, stg = eng.pagerank(damping=0.85, max_iter=20)
+    out = {"workload": f"pagerank-rmat-s{scale}-ef16, {P} shards, 20 iterations, FP32 ranks",
+           "vertices": V, "edges": E, "shards": P}
+    for name, x, secs, it in (("cpu", x_cpu, stc.seconds, stc.iterations),
+                              ("gpu", x_gpu, stg.ms_total / 1e3, stg.iterations)):
+        d20 = np.abs(x.astype(np.float64) - x20)
+        fp = oracle.pr_f64_error(g["off"], g["idx"], g["outdeg"], x, threads=threads, max_iter=200, bound=1e-13)
+        out[name] = {"seconds": round(secs, 5), "iterations": it, "gteps": round(E * it / secs / 1e9, 3),
+                     "l1_vs_fp64_same_20_iterations": float(d20.sum()),
+                     "linf_vs_fp64_same_20_iterations": float(d20.max()),
+                     "l1_vs_fp64_fixed_point": fp["l1_error"], "linf_vs_fp64_fixed_point": fp["linf_error"],
+                     "fixed_point_reference_bound": fp["reference_l1_bound"]}
+    out["cpu"]["threads"] = stc.threads
+    out["gpu_speedup"] = round(stc.seconds / (stg.ms_total / 1e3), 2)
+    # the reference's own rule: sum in source order, (float)(rel*0.85 + 1.0/N), dangling mass dropped
+    vout = g["outdeg"].astype(np.int32)
+    m_gpu, _, _ = eng.run_app(gmp.APP_PAGERANK, msg0, 20, vertex_out=vout)
+    m_orc, _, _, _ = oracle.app_sync(oracle.APP_PAGERANK, g["off"], g["idx"], msg0, 20, vout=vout)
+    out["graphmp_rule_bit_exact"] = bool(np.array_equal(m_gpu.view(np.uint32), m_orc.view(np.uint32)))
+    log(f"pagerank s{scale} x20: CPU {stc.seconds * 1e3:.1f} ms on {stc.threads} threads "
+        f"(L1/Linf vs FP64 {out['cpu']['l1_vs_fp64_same_20_iterations']:.2e}/"
+        f"{out['cpu']['linf_vs_fp64_same_20_iterations']:.2e}), GPU {stg.ms_total:.2f} ms "
+        f"({out['gpu']['l1_vs_fp64_same_20_iterations']:.2e}/{out['gpu']['linf_vs_fp64_same_20_iterations']:.2e}); "
+        f"GraphMP rule bit-exact: {out['graphmp_rule_bit_exact']}")
+    return out
+
+
+def run_sssp_s24(eng, threads):
+    """SSSP on directed R-MAT s24 ef16 with uint32 weights in [1,255] from a seeded source of
+    out-degree > 0: Bellman-Ford (pushes + strip sweeps, GraphMP's selective scheduling) and
+    fixed delta = 64 on the GPU, and the weighted GraphMP rule in the CPU engine restatement;
+    the three distance vectors must be identical."""
+    import graphee_amd as gmp
+    import oracle
+    scale = 24
+    eng.generate_rmat(scale, 16, 1000 + scale, weighted=True, num_shards=16)
+    V, E, _, _ = eng.info()
+    src = eng.pick_source(1000 + scale)
+    out = {"workload": f"sssp-rmat-s{scale}-ef16, weights 1..255", "vertices": V, "edges": E, "source": src}
+    dist = {}
+    for name, method in (("bf", gmp.SSSP_BF), ("delta64", gmp.SSSP_DELTA)):
+        eng.sssp(src, method=method, delta=64, want_dist=False)
+        d, st = eng.sssp(src, method=method, delta=64)
+        dist[name] = d
+        out[name] = {"ms": round(st.ms_total, 3), "iterations": st.iterations, "edges_examined": st.edges,
+                     "work_gteps": round(st.edges / st.ms_total / 1e6, 3), "reached": int((d != 0xFFFFFFFF).sum()),
+                     "yardstick_gbs": round(st.bytes / st.ms_total / 1e6, 1),
+                     "hbm_frac": round(st.bytes / st.ms_total / 1e6 / HBM_PEAK_GBS, 4)}
+    g = eng.download(weights=True)
+    # Graph500-style GTEPS counts the edges of the reached vertices
+    reach = dist["bf"] != 0xFFFFFFFF
+    e_reached = int(np.diff(g["off"].astype(np.int64))[reach].sum())
+    for name in ("bf", "delta64"):
+        out[name]["graph500_gteps"] = round(e_reached / out[name]["ms"] / 1e6, 3)
+    msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+    msg0[src] = 0
+    dc, stc = oracle.graphmp_baseline(oracle.APP_SSSP_W, g["off"], g["idx"], g["bounds"], msg0, 10000, w=g["w"],
+                                      threads=threads)
+    out["cpu"] = {"seconds": round(stc.seconds, 3), "threads": stc.threads, "sweeps": stc.iterations}
+    out["identical"] = bool(np.array_equal(dc, dist["bf"]) and np.array_equal(dc, dist["delta64"]))
+    out["gpu_bf_speedup"] = round(stc.seconds * 1e3 / out["bf"]["ms"], 2)
+    out["gpu_delta64_speedup"] = round(stc.seconds * 1e3 / out["delta64"]["ms"], 2)
+    log(f"sssp s{scale}: BF {out['bf']['ms']:.1f} ms, delta-64 {out['delta64']['ms']:.1f} ms, CPU "
+        f"{stc.seconds:.2f} s ({stc.threads} threads); identical: {out['identical']}")
+    return out
+
+
 def gmp_const(name):
     import graphee_amd
     return getattr(graphee_amd, name)
 
 
-def run_batched(eng):
+def verify_batched(eng, specs, gs, srcs, dists, gp, seeds, ranks, threads):
+    """One SSSP and one personalised-PageRank instance per distinct graph scale (the first
+    instance on a graph of that scale) checked against the oracle: distances bit for bit against
+    Dijkstra, ranks within 1e-6 L1 of FP64 personalised PageRank (30 iterations, damping 0.85)."""
+    import oracle
+    from oracle import rmat_ref
+    out = {"sssp_checked": 0, "sssp_identical": 0, "ppr_checked": 0, "ppr_max_l1": 0.0, "scales": []}
+    cache = {}
+
+    def graph(g):
+        if g not in cache:
+            cache.clear()
+            off, idx, w = eng.batch_download(g)
+            cache[g] = (off, idx, w)
+        return cache[g]
+
+    for scale in sorted({s["scale"] for s in specs}):
+        out["scales"].append(scale)
+        on = [i for i, g in enumerate(gs) if specs[g]["scale"] == scale]
+        if on:
+            i = on[0]
+            off, idx, w = graph(int(gs[i]))
+            V = off.size - 1
+            ooff, oidx, ow = rmat_ref.transpose(V, off, idx, w)
+            want = oracle.dijkstra(ooff, oidx, ow, int(srcs[i]))
+            out["sssp_checked"] += 1
+            out["sssp_identical"] += int(np.array_equal(dists[i], want))
+            del ooff, oidx, ow, want
+        on = [i for i, g in enumerate(gp) if specs[g]["scale"] == scale]
+        if on:
+            i = on[0]
+            off, idx, w = graph(int(gp[i]))
+            V = off.size - 1
+            od = np.bincount(idx, minlength=V).astype(np.uint32)
+            x, _, _ = oracle.pr_f64(off, idx, od, d=0.85, max_iter=30, seed=int(seeds[i]), threads=threads)
+            l1 = float(np.abs(ranks[i].astype(np.float64) - x).sum())
+            out["ppr_checked"] += 1
+            out["ppr_max_l1"] = max(out["ppr_max_l1"], l1)
+    out["verified"] = bool(out["sssp_identical"] == out["sssp_checked"] and out["ppr_max_l1"] <= 1e-6)
+    return out
+
+
+def run_batched(eng, verify=True):
+    """32 R-MAT graphs of scale 14..22 and edge factor 8..32 (weights 1..255) resident at once;
+    256 SSSP sources and 256 personalised-PageRank seeds (30 iterations) as batched launches."""
     specs, gs, gp, rng = batched_workload()
     t0 = time.perf_counter()
     eng.batch_generate(specs)
@@ -211,15 +340,22 @@ def run_batched(eng):
     srcs = np.array([rng.integers(0, sizes[g][0]) for g in gs], dtype=np.uint32)
     seeds = np.array([rng.integers(0, sizes[g][0]) for g in gp], dtype=np.uint32)
     eng.batch_sssp(gs[:4], srcs[:4])  # warm the kernels
-    _, st = eng.batch_sssp(gs, srcs)
+    dists, st = eng.batch_sssp(gs, srcs)
     sssp = {"instances": 256, "ms": round(st.ms_total, 3), "rounds": st.iterations, "graph_launches": st.launches,
             "edges_relaxed": st.edges, "work_gteps": round(st.edges / st.ms_total / 1e6, 3)}
-    _, st2 = eng.batch_ppr(gp, seeds, damping=0.85, iters=30)
+    ranks, st2 = eng.batch_ppr(gp, seeds, damping=0.85, iters=30)
     ppr = {"instances": 256, "iterations": 30, "ms": round(st2.ms_total, 3),
            "instance_edges": st2.edges, "gteps": round(st2.edges / st2.ms_total / 1e6, 3)}
     log(f"batched: {sum(s[1] for s in sizes)} edges in 32 graphs (built in {gen_s:.1f} s); "
         f"SSSP x256 {st.ms_total:.1f} ms, PPR x256 {st2.ms_total:.1f} ms")
-    return {"graphs": 32, "total_edges": int(sum(s[1] for s in sizes)), "sssp": sssp, "ppr": ppr}
+    res = {"graphs": 32, "total_edges": int(sum(s[1] for s in sizes)), "sssp": sssp, "ppr": ppr}
+    if verify:
+        t0 = time.perf_counter()
+        res["verification"] = verify_batched(eng, specs, gs, srcs, dists, gp, seeds, ranks, cpu_threads())
+        res["verification"]["seconds"] = round(time.perf_counter() - t0, 1)
+        res["verified"] = res["verification"]["verified"]
+        log(f"batched verification: {res['verification']}")
+    return res
 
 
 PCIE5_X16_GBS = 64.0  # MI355X host link, PCIe 5.0 x16: published 128 GB/s both ways, 64 per direction
@@ -462,6 +598,7 @@ def main():
                                   max_iter=args.fp64_steps, bound=1e-11)
         result["extra"]["pagerank_fp64_check"] = {
             "l1_error": err["l1_error"], "linf_error": err["linf_error"],
+            "linf_error_upper_bound": err["linf_error"] + err["reference_l1_bound"],
             "reference_l1_bound": err["reference_l1_bound"], "reference_steps": err["reference_steps"],
             "reference_seconds": round(time.perf_counter() - t0, 1),
             "l1_error_bound": l1 / 0.15, "step_residual_l1": l1, "step_residual_linf": linf,
@@ -578,7 +715,7 @@ def main():
         result["extra"]["pagerank_rmat_s20_x20"] = run_pagerank_s20(eng, cpu_threads())
         result["extra"]["sssp_rmat_s24"] = run_sssp_s24(eng, cpu_threads())
     if not args.no_extra and not args.no_batched:
-        result["extra"]["batched"] = run_batched(eng)
+        result["extra"]["batched"] = run_batched(eng, verify=not args.no_cpu)
     if not args.no_extra and not args.no_learned:
         result["extra"]["learned_sssp"] = run_learned(eng, args.learn_scale)
 

@@ -45,7 +45,7 @@ def lib():
         L.orc_partition_bounds.argtypes = [_P, _I64, _I32, _P]
         L.orc_app_sync.argtypes = [_I32, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P]
         L.orc_pr_f64.argtypes = [_I64, _P, _P, _P, ctypes.c_double, _I32, ctypes.c_double,
-                                 _I64, _P, _P, _P, _P]
+                                 _I64, _P, _P, _P, _P, _I32]
         L.orc_sssp_dijkstra.argtypes = [_I64, _P, _P, _P, ctypes.c_uint32, _P]
         L.orc_cc_unionfind.argtypes = [_I64, _P, _P, _P]
         L.orc_graphmp_baseline.argtypes = [_I32, _I64, _I32, _P, _P, _P, _P, _P, ctypes.c_float,
@@ -95,7 +95,7 @@ def app_sync(app, off, idx, msg_init, max_iter, w=None, vout=None):
     return msg, value, it.value, changed[: it.value]
 
 
-def pr_f64(off, idx, outdeg, d=0.85, max_iter=100, tol=0.0, seed=-1, x0=None):
+def pr_f64(off, idx, outdeg, d=0.85, max_iter=100, tol=0.0, seed=-1, x0=None, threads=1):
     V = off.size - 1
     off = np.ascontiguousarray(off, dtype=np.int64)
     idx = np.ascontiguousarray(idx, dtype=np.uint32)
@@ -105,7 +105,7 @@ def pr_f64(off, idx, outdeg, d=0.85, max_iter=100, tol=0.0, seed=-1, x0=None):
     it = ctypes.c_int32(0)
     r = ctypes.c_double(0)
     _check(lib().orc_pr_f64(V, _ptr(off), _ptr(idx), _ptr(od), d, max_iter, tol, seed,
-                            _ptr(x0a), _ptr(x), ctypes.byref(it), ctypes.byref(r)), "pr_f64")
+                            _ptr(x0a), _ptr(x), ctypes.byref(it), ctypes.byref(r), threads), "pr_f64")
     return x, it.value, r.value
 
 

@@ -155,11 +155,14 @@ int orc_app_sync(int32_t app, int64_t V, const int64_t *off, const uint32_t *idx
 /* FP64 power iteration.  x_{k+1}[v] = (1-d) t[v] + d (sum_{u->v} x_k[u]/outdeg[u] + D_k t[v]),
  * D_k = sum over dangling u of x_k[u]; t = uniform 1/V, or e_seed when seed >= 0.
  * Stops after max_iter iterations, or as soon as the L1 change ||x_{k+1}-x_k||_1 < tol
- * (tol > 0).  *iters = iterations executed, *resid = the last L1 change. */
+ * (tol > 0).  *iters = iterations executed, *resid = the last L1 change.  threads > 1 runs the
+ * two loops with OpenMP (the per-vertex sums keep their order; only D and the residual are summed
+ * in another order). */
 int orc_pr_f64(int64_t V, const int64_t *off, const uint32_t *idx, const uint32_t *outdeg,
                double d, int32_t max_iter, double tol, int64_t seed, const double *x0,
-               double *x, int32_t *iters, double *resid) {
+               double *x, int32_t *iters, double *resid, int32_t threads) {
   if (V <= 0) return ORC_EINVAL;
+  if (threads <= 0) threads = 1;
   double *c = (double *)malloc((size_t)V * sizeof(double));
   double *xn = (double *)malloc((size_t)V * sizeof(double));
   if (!c || !xn) { free(c); free(xn); return ORC_ENOMEM; }
@@ -168,11 +171,13 @@ int orc_pr_f64(int64_t V, const int64_t *off, const uint32_t *idx, const uint32_
   double r = 0;
   for (it = 0; it < max_iter;) {
     double D = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : D) if (threads > 1)
     for (int64_t u = 0; u < V; u++) {
       if (outdeg[u] == 0) { D += x[u]; c[u] = 0; }
       else c[u] = x[u] / (double)outdeg[u];
     }
     r = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096) reduction(+ : r) if (threads > 1)
     for (int64_t v = 0; v < V; v++) {
       double s = 0;
       for (int64_t k = off[v]; k < off[v + 1]; k++) s += c[idx[k]];
