@@ -329,9 +329,11 @@ def verify_batched(eng, specs, gs, srcs, dists, gp, seeds, ranks, threads):
     return out
 
 
-def run_batched(eng, verify=True):
+def run_batched(eng, verify=True, tune=True):
     """32 R-MAT graphs of scale 14..22 and edge factor 8..32 (weights 1..255) resident at once;
-    256 SSSP sources and 256 personalised-PageRank seeds (30 iterations) as batched launches."""
+    256 SSSP sources and 256 personalised-PageRank seeds (30 iterations): the batch autotuner's
+    pass over the batched kernels' knobs, each kind alone, and both kinds in one replayed graph
+    (gmp_batch_run, whose outputs are the ones verified)."""
     specs, gs, gp, rng = batched_workload()
     t0 = time.perf_counter()
     eng.batch_generate(specs)
@@ -339,16 +341,26 @@ def run_batched(eng, verify=True):
     sizes = [eng.batch_info(g) for g in range(32)]
     srcs = np.array([rng.integers(0, sizes[g][0]) for g in gs], dtype=np.uint32)
     seeds = np.array([rng.integers(0, sizes[g][0]) for g in gp], dtype=np.uint32)
+    res = {"graphs": 32, "total_edges": int(sum(s[1] for s in sizes))}
+    if tune:
+        t0 = time.perf_counter()
+        res["tuned"] = eng.batch_autotune(gs, srcs, gp, seeds, iters=30)
+        res["tuned"]["seconds"] = round(time.perf_counter() - t0, 1)
+        log(f"batch autotune: {res['tuned']}")
     eng.batch_sssp(gs[:4], srcs[:4])  # warm the kernels
-    dists, st = eng.batch_sssp(gs, srcs)
-    sssp = {"instances": 256, "ms": round(st.ms_total, 3), "rounds": st.iterations, "graph_launches": st.launches,
-            "edges_relaxed": st.edges, "work_gteps": round(st.edges / st.ms_total / 1e6, 3)}
-    ranks, st2 = eng.batch_ppr(gp, seeds, damping=0.85, iters=30)
-    ppr = {"instances": 256, "iterations": 30, "ms": round(st2.ms_total, 3),
-           "instance_edges": st2.edges, "gteps": round(st2.edges / st2.ms_total / 1e6, 3)}
-    log(f"batched: {sum(s[1] for s in sizes)} edges in 32 graphs (built in {gen_s:.1f} s); "
-        f"SSSP x256 {st.ms_total:.1f} ms, PPR x256 {st2.ms_total:.1f} ms")
-    res = {"graphs": 32, "total_edges": int(sum(s[1] for s in sizes)), "sssp": sssp, "ppr": ppr}
+    _, st = eng.batch_sssp(gs, srcs)
+    res["sssp"] = {"instances": 256, "ms": round(st.ms_total, 3), "rounds": st.iterations,
+                   "graph_launches": st.launches, "edges_relaxed": st.edges,
+                   "work_gteps": round(st.edges / st.ms_total / 1e6, 3)}
+    _, st2 = eng.batch_ppr(gp, seeds, damping=0.85, iters=30)
+    res["ppr"] = {"instances": 256, "iterations": 30, "ms": round(st2.ms_total, 3),
+                  "instance_edges": st2.edges, "gteps": round(st2.edges / st2.ms_total / 1e6, 3)}
+    dists, ranks, st3, st4 = eng.batch_run(gs, srcs, gp, seeds, damping=0.85, iters=30)
+    res["combined"] = {"ms": round(st3.ms_total, 3), "graph_launches": st3.launches,
+                       "separate_ms": round(st.ms_total + st2.ms_total, 3),
+                       "speedup_over_separate": round((st.ms_total + st2.ms_total) / st3.ms_total, 3)}
+    log(f"batched: {res['total_edges']} edges in 32 graphs (built in {gen_s:.1f} s); SSSP x256 {st.ms_total:.1f} ms, "
+        f"PPR x256 {st2.ms_total:.1f} ms, both in one graph {st3.ms_total:.1f} ms")
     if verify:
         t0 = time.perf_counter()
         res["verification"] = verify_batched(eng, specs, gs, srcs, dists, gp, seeds, ranks, cpu_threads())

@@ -23,6 +23,8 @@ OPT_STRIP_LOG, OPT_BIN_LOG, OPT_BIN_TASK_LOG = 7, 8, 9
 OPT_EDGE_BUDGET_MB, OPT_WINDOW_MB, OPT_WINDOW_PACKED, OPT_PUSH_LOG, OPT_CC_UNION_FIND = 10, 11, 12, 13, 14
 OPT_DELTA_PULL_LOG, OPT_LEARNED_FINISH = 15, 16
 NOALLOC_MAX_VERTICES = 1 << 22
+(OPT_BATCH_SSSP_UNROLL, OPT_BATCH_SSSP_GRID, OPT_BATCH_PPR_PIECE, OPT_BATCH_PPR_RUN_ROWS, OPT_BATCH_PPR_RUN_EDGES,
+ OPT_BATCH_PPR_LOADS, OPT_BATCH_PPR_GRID) = range(17, 24)
 
 # every entry point include/gmp.h declares
 SYMBOLS = (
@@ -30,7 +32,7 @@ SYMBOLS = (
     "gmp_load_shards", "gmp_load_edge_list", "gmp_generate_rmat", "gmp_graph_info", "gmp_graph_download",
     "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_pagerank_noalloc", "gmp_sssp", "gmp_cc",
     "gmp_set_option", "gmp_get_option", "gmp_batch_generate", "gmp_batch_info", "gmp_batch_download",
-    "gmp_batch_sssp", "gmp_batch_ppr", "gmp_autotune", "gmp_tune_save", "gmp_tune_load",
+    "gmp_batch_sssp", "gmp_batch_ppr", "gmp_batch_run", "gmp_batch_autotune", "gmp_autotune", "gmp_tune_save", "gmp_tune_load",
     "gmp_sssp_trace", "gmp_sssp_actions", "gmp_mlp_init", "gmp_mlp_set", "gmp_mlp_get", "gmp_mlp_forward", "gmp_mlp_loss_grad",
     "gmp_mlp_train", "gmp_windows",
     "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards", "gmp_convert_edge_list",
@@ -101,7 +103,18 @@ class TuneResult(ctypes.Structure):
                 ("vec_min", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32), ("trials", ctypes.c_uint32),
                 ("gather_ms_before", ctypes.c_double), ("gather_ms_after", ctypes.c_double),
                 ("strip_log", ctypes.c_uint32), ("bin_log", ctypes.c_uint32), ("bin_task_log", ctypes.c_uint32),
-                ("push_log", ctypes.c_uint32)]
+                ("push_log", ctypes.c_uint32), ("delta_pull_log", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class BatchTuneResult(ctypes.Structure):
+    _fields_ = [("sssp_unroll", ctypes.c_uint32), ("sssp_grid_per_cu", ctypes.c_uint32),
+                ("ppr_piece", ctypes.c_uint32), ("ppr_run_rows", ctypes.c_uint32), ("ppr_run_edges", ctypes.c_uint32),
+                ("ppr_loads", ctypes.c_uint32), ("ppr_grid_per_cu", ctypes.c_uint32), ("trials", ctypes.c_uint32),
+                ("sssp_ms_before", ctypes.c_double), ("sssp_ms_after", ctypes.c_double),
+                ("ppr_ms_before", ctypes.c_double), ("ppr_ms_after", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -152,6 +165,10 @@ def lib():
         "gmp_batch_download": ([_P, u32, _P, _P, _P], ctypes.c_int),
         "gmp_batch_sssp": ([_P, u32, _P, _P, _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_batch_ppr": ([_P, u32, _P, _P, ctypes.c_float, i32, _P, ctypes.POINTER(Stats)], ctypes.c_int),
+        "gmp_batch_run": ([_P, u32, _P, _P, u32, _P, _P, ctypes.c_float, i32, _P, _P, ctypes.POINTER(Stats),
+                           ctypes.POINTER(Stats)], ctypes.c_int),
+        "gmp_batch_autotune": ([_P, u32, _P, _P, u32, _P, _P, ctypes.c_float, i32, ctypes.POINTER(BatchTuneResult)],
+                               ctypes.c_int),
         "gmp_get_option": ([_P, i32, _P], ctypes.c_int),
         "gmp_run_app": ([_P, i32, i32, _P, _P, _P, _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_pagerank": ([_P, ctypes.POINTER(PrParams), _P, ctypes.POINTER(Stats)], ctypes.c_int),
@@ -368,6 +385,24 @@ class Engine:
         check(lib().gmp_batch_ppr(self._h, graphs.size, _ptr(graphs), _ptr(seeds), damping, iters, _ptr(out),
                                   ctypes.byref(st)))
         return self._split(graphs, out, np.float32), st
+
+    def batch_run(self, sssp_graphs, sources, ppr_graphs, seeds, damping=0.85, iters=30):
+        """SSSP and PPR instances in one replayed graph (gmp_batch_run); returns
+        (distances per SSSP instance, ranks per PPR instance, SSSP stats, PPR stats)."""
+        sg, so, pg, se = _u32(sssp_graphs), _u32(sources), _u32(ppr_graphs), _u32(seeds)
+        ds = np.zeros(sum(self.batch_info(int(g))[0] for g in sg), dtype=np.uint32)
+        rk = np.zeros(sum(self.batch_info(int(g))[0] for g in pg), dtype=np.float32)
+        st_s, st_p = Stats(), Stats()
+        check(lib().gmp_batch_run(self._h, sg.size, _ptr(sg), _ptr(so), pg.size, _ptr(pg), _ptr(se), damping, iters,
+                                  _ptr(ds), _ptr(rk), ctypes.byref(st_s), ctypes.byref(st_p)))
+        return self._split(sg, ds, np.uint32), self._split(pg, rk, np.float32), st_s, st_p
+
+    def batch_autotune(self, sssp_graphs, sources, ppr_graphs, seeds, damping=0.85, iters=30):
+        sg, so, pg, se = _u32(sssp_graphs), _u32(sources), _u32(ppr_graphs), _u32(seeds)
+        r = BatchTuneResult()
+        check(lib().gmp_batch_autotune(self._h, sg.size, _ptr(sg), _ptr(so), pg.size, _ptr(pg), _ptr(se), damping,
+                                       iters, ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in r._fields_}
 
     # apps
     def run_app(self, app, msg_init, max_iter, vertex_out=None):

@@ -2,7 +2,9 @@
 // and the bin-task size of the strip layout (or the row gather, strip width 0), and for the row
 // gather the degree-bin threshold (vec_min), the LDS tile (stream_cap rows' edges per wave), the
 // hub chunk size and the workgroups per CU, on the loaded graph, by measuring iterations with
-// HIP events; then the push/sweep threshold of CC and Bellman-Ford by timing Bellman-Ford.  GraphMP fixes the matching
+// HIP events; then the push/sweep threshold of CC and Bellman-Ford by timing Bellman-Ford, and the
+// pull threshold of delta-stepping by timing delta = 64; the batched kernels' knobs have their own
+// search (batch_autotune), on the caller's instances.  GraphMP fixes the matching
 // knobs at compile time (CMPNUM threads, GraphMP/system/Global.h:58); nothing here changes
 // results, only the task plan and the grid.
 //
@@ -36,6 +38,15 @@ double time_bf(Engine &e, uint32_t src) {
   gmp_sssp_params p{GMP_SSSP_BF, 0, 0, 0};
   gmp_stats st{};
   run_sssp(e, src, p, nullptr, &st);  // warm (layouts)
+  run_sssp(e, src, p, nullptr, &st);
+  return st.ms_total;
+}
+
+// Fixed delta = 64 solve time from src.
+double time_delta(Engine &e, uint32_t src) {
+  gmp_sssp_params p{GMP_SSSP_DELTA, 64, 0, 0};
+  gmp_stats st{};
+  run_sssp(e, src, p, nullptr, &st);
   run_sssp(e, src, p, nullptr, &st);
   return st.ms_total;
 }
@@ -122,6 +133,21 @@ void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
       }
     }
     e.opt.push_log = best.push_log;
+    // the pull threshold of delta-stepping, timed on delta = 64 from the same vertex (weighted
+    // graphs, or unit weights)
+    double best_ds = time_delta(e, src);
+    trials++;
+    for (uint32_t v : {0u, 2u, 3u, 4u, 5u, 6u}) {
+      if (v == best.delta_pull_log) continue;
+      e.opt.delta_pull_log = v;
+      const double ms = time_delta(e, src);
+      trials++;
+      if (ms > 0 && ms < best_ds * 0.98) {
+        best_ds = ms;
+        best.delta_pull_log = v;
+      }
+    }
+    e.opt.delta_pull_log = best.delta_pull_log;
   }
   if (out) {
     std::memset(out, 0, sizeof *out);
@@ -137,6 +163,75 @@ void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
     out->gather_ms_before = start_ms;
     out->gather_ms_after = best_ms;
     out->push_log = best.push_log;
+    out->delta_pull_log = best.delta_pull_log;
+  }
+}
+
+// Batched kernels: coordinate descent over each kind's knobs, two sweeps, a trial = one batched
+// SSSP run or a few batched PPR iterations on the caller's instances, timed with HIP events.
+void batch_autotune(Engine &e, uint32_t ns, const uint32_t *sgraph, const uint32_t *source, uint32_t np,
+                    const uint32_t *pgraph, const uint32_t *seed, float damping, int32_t iters,
+                    gmp_batch_tune_result *out) {
+  if (!e.batch.ready) fail(GMP_ESTATE, "no batch loaded");
+  if (ns == 0 && np == 0) fail(GMP_EINVAL, "gmp_batch_autotune needs instances");
+  const int ti = iters > 0 ? std::min(iters, 6) : 6;
+  int trials = 0;
+  auto run = [&](bool sssp) {
+    gmp_stats st{};
+    if (sssp) batch_sssp(e, ns, sgraph, source, nullptr, &st);
+    else batch_ppr(e, np, pgraph, seed, damping, ti, nullptr, &st);
+    trials++;
+    return sssp ? st.ms_total : st.ms_total / ti;
+  };
+  // descend over the knobs of one kind; returns {first, best} time
+  auto descend = [&](bool sssp, const std::vector<std::pair<uint32_t *, std::vector<uint32_t>>> &knobs) {
+    run(sssp);  // warm (kernel code objects, caches)
+    double best = run(sssp);
+    const double first = best;
+    for (int sweep = 0; sweep < 2; sweep++) {
+      bool improved = false;
+      for (const auto &k : knobs) {
+        const uint32_t keep = *k.first;
+        uint32_t chosen = keep;
+        for (uint32_t v : k.second) {
+          if (v == keep) continue;
+          *k.first = v;
+          const double ms = run(sssp);
+          if (ms > 0 && ms < best * 0.98) {
+            best = ms;
+            chosen = v;
+            improved = true;
+          }
+        }
+        *k.first = chosen;
+      }
+      if (!improved) break;
+    }
+    return std::make_pair(first, best);
+  };
+  Options &o = e.opt;
+  std::pair<double, double> ts{0.0, 0.0}, tp{0.0, 0.0};
+  if (ns) ts = descend(true, {{&o.bsssp_unroll, {2, 4, 8}}, {&o.bsssp_grid, {1, 2, 4, 8, 16}}});
+  if (np)
+    tp = descend(false, {{&o.ppr_piece, {256, 512, 1024, 2048, 4096}},
+                         {&o.ppr_run_rows, {8, 16, 32, 64, 128}},
+                         {&o.ppr_run_edges, {512, 1024, 2048, 4096, 8192}},
+                         {&o.ppr_loads, {4, 8, 16}},
+                         {&o.ppr_grid, {2, 4, 8, 16, 32}}});
+  if (out) {
+    std::memset(out, 0, sizeof *out);
+    out->sssp_unroll = o.bsssp_unroll;
+    out->sssp_grid_per_cu = o.bsssp_grid;
+    out->ppr_piece = o.ppr_piece;
+    out->ppr_run_rows = o.ppr_run_rows;
+    out->ppr_run_edges = o.ppr_run_edges;
+    out->ppr_loads = o.ppr_loads;
+    out->ppr_grid_per_cu = o.ppr_grid;
+    out->trials = (uint32_t)trials;
+    out->sssp_ms_before = ts.first;
+    out->sssp_ms_after = ts.second;
+    out->ppr_ms_before = tp.first;
+    out->ppr_ms_after = tp.second;
   }
 }
 
@@ -147,7 +242,11 @@ void tune_save(Engine &e, const std::string &path) {
     << ", \"row_cap\": " << e.opt.row_cap << ", \"chunk\": " << e.opt.chunk << ", \"vec_min\": " << e.opt.vec_min
     << ", \"wg_per_cu\": " << e.opt.wg_per_cu << ", \"strip_log\": " << e.opt.strip_log
     << ", \"bin_log\": " << e.opt.bin_log << ", \"bin_task_log\": " << e.opt.bin_task_log
-    << ", \"push_log\": " << e.opt.push_log << "}\n";
+    << ", \"push_log\": " << e.opt.push_log << ", \"delta_pull_log\": " << e.opt.delta_pull_log
+    << ", \"batch_sssp_unroll\": " << e.opt.bsssp_unroll << ", \"batch_sssp_grid\": " << e.opt.bsssp_grid
+    << ", \"batch_ppr_piece\": " << e.opt.ppr_piece << ", \"batch_ppr_run_rows\": " << e.opt.ppr_run_rows
+    << ", \"batch_ppr_run_edges\": " << e.opt.ppr_run_edges << ", \"batch_ppr_loads\": " << e.opt.ppr_loads
+    << ", \"batch_ppr_grid\": " << e.opt.ppr_grid << "}\n";
   if (!f) fail(GMP_EIO, path + ": short write");
 }
 
@@ -180,6 +279,18 @@ void tune_load(Engine &e, const std::string &path) {
   get("bin_log", 8, 14, o.bin_log);
   get("bin_task_log", 8, 22, o.bin_task_log);
   get("push_log", 0, 31, o.push_log);
+  get("delta_pull_log", 0, 31, o.delta_pull_log);
+  get("batch_sssp_unroll", 2, 8, o.bsssp_unroll);
+  get("batch_sssp_grid", 1, 64, o.bsssp_grid);
+  get("batch_ppr_piece", 64, 65536, o.ppr_piece);
+  get("batch_ppr_run_rows", 1, 1024, o.ppr_run_rows);
+  get("batch_ppr_run_edges", 64, 65536, o.ppr_run_edges);
+  get("batch_ppr_loads", 4, 16, o.ppr_loads);
+  get("batch_ppr_grid", 1, 64, o.ppr_grid);
+  if (o.bsssp_unroll != 2 && o.bsssp_unroll != 4 && o.bsssp_unroll != 8)
+    fail(GMP_EFORMAT, path + ": batch_sssp_unroll must be 2, 4 or 8");
+  if (o.ppr_loads != 4 && o.ppr_loads != 8 && o.ppr_loads != 16)
+    fail(GMP_EFORMAT, path + ": batch_ppr_loads must be 4, 8 or 16");
   if (o.strip_log != 0 && o.strip_log < 8) fail(GMP_EFORMAT, path + ": strip_log out of range");
   if ((int)reorder != e.opt.reorder) {
     e.g.work_ready = false;

@@ -81,15 +81,16 @@ __device__ inline uint32_t instance_of(const uint32_t *slot_base, uint32_t n, ui
   return lo;
 }
 
-constexpr int kRelaxUnroll = 4;    // edges per lane in flight in the relax loop
 constexpr uint32_t kLdsInst = 1024;  // instance slot bases kept in LDS up to this many instances
 constexpr uint32_t kAppendCap = 512;  // staged appends per wave
 
 // One round for every instance: reads queue[round&1] and appends improved slots to the other
 // queue; the last workgroup to finish flips the round and clears the consumed count.  A wave
-// takes 64 frontier slots and expands their edges kRelaxUnroll per lane at a time: the edge,
-// weight and distance loads of those edges are issued together (edges past the end reload the
-// wave's first edge and are dropped), then the improving ones are applied with atomics.
+// takes 64 frontier slots and expands their edges kRelaxUnroll per lane at a time (a tunable:
+// 2, 4 or 8): the edge, weight and distance loads of those edges are issued together (edges past
+// the end reload the wave's first edge and are dropped), then the improving ones are applied
+// with atomics.
+template <int kRelaxUnroll>
 __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
   __shared__ uint32_t s_start[kWavesPerBlock][kWave];
   __shared__ uint64_t s_base[kWavesPerBlock][kWave];
@@ -239,20 +240,18 @@ __global__ void batch_sssp_seed_kernel(const uint32_t *slots, uint32_t n, uint32
 //
 // A group holds up to 64 instances of one graph; lane l of a wave serves instance l % kk of
 // the group (kk = instances rounded up to a power of two) in slot l / kk, so a wave runs
-// 64 / kk slots.  Rows of in-degree up to kHubDegree are handed out in runs, one row per slot
-// at a time; a slot walks its row's in-edges with kLoads source loads in flight per lane and a
-// Kahan sum per lane, and writes the row's k ranks as one contiguous vector.  Rows above
-// kHubDegree are cut into pieces of kHubDegree edges spread over the slots of a wave; each
-// piece leaves an FP64 partial per instance, and ppr_hub_kernel adds a row's pieces in order.
+// 64 / kk slots.  Rows of in-degree up to the piece size are handed out in runs (of at most
+// run_rows rows and run_edges edges per slot), one row per slot at a time; a slot walks its
+// row's in-edges with kLoads source loads in flight per lane (a tunable: 4, 8 or 16) and a Kahan
+// sum per lane, and writes the row's k ranks as one contiguous vector.  Longer rows are cut into
+// pieces of `piece` edges spread over the slots of a wave; each piece leaves an FP64 partial per
+// instance, and ppr_hub_kernel adds a row's pieces in order.  The piece size, run sizes, loads in
+// flight and grid are the autotuner's (gmp_batch_autotune).
 // Waves take tasks (runs and pieces of every group) from one queue, so groups of any size and
 // instance count share the chip without a cost model.  The dangling mass of the new ranks
 // (needed by the next iteration's teleport) is summed per instance in an exact 61-bit fixed
 // point: the sum does not depend on which wave took which task, so results are reproducible.
 
-constexpr uint32_t kHubDegree = 1024;  // rows above this are split into pieces of this size
-constexpr int kLoads = 8;              // source loads in flight per lane
-constexpr uint32_t kRunRows = 32;      // rows per slot in a row run
-constexpr uint32_t kRunEdges = 2048;   // edges per slot in a row run
 constexpr uint32_t kCols = 64;         // partials are stored 64 instances wide
 
 struct PprGroup {      // up to 64 instances on one graph
@@ -273,7 +272,7 @@ struct PprTask {       // a run of rows (part == kNoPiece) or one piece of a hub
 };
 constexpr uint32_t kNoPiece = 0xFFFFFFFFu;
 
-struct PprHub {        // a row above kHubDegree
+struct PprHub {        // a row above the piece size
   uint32_t group;
   uint32_t row;        // group-local
   uint32_t part0;      // partial slots [part0, part0 + nparts)
@@ -298,11 +297,19 @@ struct PprArgs {
   unsigned long long *dang_next;    // ... and of the new ranks (fixed point, cleared beforehand)
   double *part;                     // [parts][kCols]: FP64 sums of hub pieces
   float d;
+  uint32_t piece;                   // edges per hub piece
+  // set when the iterations share a replayed graph with other work: an iteration whose number
+  // (*iter) has reached iter_limit does nothing, and ppr_advance_kernel counts iterations
+  uint32_t *iter;
+  uint32_t iter_limit;
 };
+
+__device__ inline bool ppr_skip(const PprArgs &a) { return a.iter && *a.iter >= a.iter_limit; }
 
 // Kahan-sums contrib_in over the in-edges [e, e1) of a row, taking every step-th edge.  The
 // loads are unconditional -- an edge past e1 reloads edge e, a lane without an instance reads
 // column 0 -- so all kLoads of a batch are in flight together; the extra values are dropped.
+template <int kLoads>
 __device__ inline void ppr_sum_edges(const PprArgs &a, const PprGroup &g, uint32_t inst, bool on, uint64_t e,
                                      uint64_t e1, uint32_t step, float &s, float &c) {
   const float *col = a.contrib_in + g.rbase + (on ? inst : 0u);
@@ -348,7 +355,9 @@ __device__ inline void ppr_flush_dangling(const PprArgs &a, const PprGroup &g, u
   acc = 0ull;
 }
 
+template <int kLoads>
 __global__ __launch_bounds__(kBlock) void ppr_gather_kernel(PprArgs a) {
+  if (ppr_skip(a)) return;
   const uint32_t lane = lane_id();
   uint32_t gid = 0xFFFFFFFFu;
   PprGroup g{};
@@ -385,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void ppr_gather_kernel(PprArgs a) {
         const uint64_t n0 = a.off[q], n1 = a.off[q + 1];
         const uint32_t nod = a.outdeg[q];
         float s = 0.0f, c = 0.0f;
-        ppr_sum_edges(a, g, inst, on, e0, e1, 1, s, c);
+        ppr_sum_edges<kLoads>(a, g, inst, on, e0, e1, 1, s, c);
         if (on) dacc += ppr_finish_row(a, g, inst, r, s - c, seed, tele, od);
         e0 = n0;
         e1 = n1;
@@ -393,10 +402,10 @@ __global__ __launch_bounds__(kBlock) void ppr_gather_kernel(PprArgs a) {
       }
     } else {
       const uint32_t u = g.vbase + tk.row;
-      const uint64_t e0 = a.off[u] + (uint64_t)tk.n * kHubDegree;
-      const uint64_t e1 = min(a.off[u + 1], e0 + kHubDegree);
+      const uint64_t e0 = a.off[u] + (uint64_t)tk.n * a.piece;
+      const uint64_t e1 = min(a.off[u + 1], e0 + a.piece);
       float s = 0.0f, c = 0.0f;
-      ppr_sum_edges(a, g, inst, on, e0 + slot, e1, S, s, c);
+      ppr_sum_edges<kLoads>(a, g, inst, on, e0 + slot, e1, S, s, c);
       // the slots' sums of one instance, combined in a fixed butterfly order
       float hi = s, lo = -c;
       for (uint32_t o = kWave >> 1; o >= g.kk; o >>= 1) {
@@ -411,6 +420,7 @@ __global__ __launch_bounds__(kBlock) void ppr_gather_kernel(PprArgs a) {
 
 // One wave per hub row: its pieces' partials in piece order, then the rank update.
 __global__ __launch_bounds__(kBlock) void ppr_hub_kernel(PprArgs a) {
+  if (ppr_skip(a)) return;
   const uint32_t lane = lane_id();
   const uint32_t nw = gridDim.x * kWavesPerBlock;
   for (uint32_t h = blockIdx.x * kWavesPerBlock + wave_id(); h < a.nhubs; h += nw) {
@@ -429,9 +439,16 @@ __global__ __launch_bounds__(kBlock) void ppr_hub_kernel(PprArgs a) {
 }
 
 // Clears the task queue and the dangling sums the next gather accumulates into.
-__global__ void ppr_reset_kernel(uint32_t *next, unsigned long long *dang_next, uint32_t n) {
+__global__ void ppr_reset_kernel(uint32_t *next, unsigned long long *dang_next, uint32_t n, const uint32_t *iter,
+                                 uint32_t iter_limit) {
+  if (iter && *iter >= iter_limit) return;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dang_next[i] = 0ull;
   if (blockIdx.x == 0 && threadIdx.x == 0) *next = 0u;
+}
+
+// Counts a finished iteration (one thread; after the iteration's kernels on the same stream).
+__global__ void ppr_advance_kernel(uint32_t *iter, uint32_t iter_limit) {
+  if (threadIdx.x == 0 && *iter < iter_limit) *iter += 1u;
 }
 
 // r = e_seed, c = r / outdeg over the group's [V][k] block (blockIdx.y = group), and the
@@ -562,81 +579,313 @@ void batch_generate(Engine &e, uint32_t ngraphs, const gmp_rmat_params *params) 
   b.ready = true;
 }
 
-void batch_sssp(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *source, uint32_t *dist_out,
-                gmp_stats *st) {
+namespace {
+
+// ---- prepared runs ---------------------------------------------------------------------
+// A batched SSSP or PPR run split into its set-up (instances checked, device buffers filled), its
+// start (initial values) and its rounds or iterations, so that both kinds can be captured
+// into one HIP graph by gmp_batch_run as well as run alone.
+
+struct SsspRun {
+  uint32_t n = 0, T = 0;
+  std::vector<uint32_t> slot;
+  DevBuf d_slot, d_vb, d_seed, dist, stamp, q0, q1, cnt;
+  BatchSsspArgs a{};
+  uint32_t grid = 0;
+};
+
+void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *source, SsspRun &r) {
   BatchGraphs &b = e.batch;
   if (!b.ready) fail(GMP_ESTATE, "no batch loaded");
   if (!b.weighted) fail(GMP_EUNSUPPORTED, "batched SSSP needs weighted graphs");
   if (n == 0 || !graph || !source) fail(GMP_EINVAL, "batched SSSP needs instances");
   const uint32_t G = (uint32_t)b.vbase.size() - 1;
-  std::vector<uint32_t> slot(n + 1, 0), vb(n), seeds(n);
+  r.n = n;
+  r.slot.assign(n + 1, 0);
+  std::vector<uint32_t> vb(n), seeds(n);
   for (uint32_t i = 0; i < n; i++) {
     if (graph[i] >= G) fail(GMP_EINVAL, "instance graph out of range");
     const uint32_t Vk = b.vbase[graph[i] + 1] - b.vbase[graph[i]];
     if (source[i] >= Vk) fail(GMP_EINVAL, "instance source out of range");
-    if ((uint64_t)slot[i] + Vk >= 0xFFFFFFFFull) fail(GMP_EUNSUPPORTED, "batch exceeds 2^32 distance slots");
-    slot[i + 1] = slot[i] + Vk;
+    if ((uint64_t)r.slot[i] + Vk >= 0xFFFFFFFFull) fail(GMP_EUNSUPPORTED, "batch exceeds 2^32 distance slots");
+    r.slot[i + 1] = r.slot[i] + Vk;
     vb[i] = b.vbase[graph[i]];
-    seeds[i] = slot[i] + source[i];
+    seeds[i] = r.slot[i] + source[i];
   }
-  const uint32_t T = slot[n];
-  DevBuf d_slot, d_vb, d_seed, dist, stamp, q0, q1, cnt;
-  d_slot.alloc((n + 1) * 4);
-  d_vb.alloc(n * 4);
-  d_seed.alloc(n * 4);
-  dist.alloc((size_t)T * 4);
-  stamp.alloc((size_t)T * 4);
-  q0.alloc((size_t)T * 4);
-  q1.alloc((size_t)T * 4);
-  cnt.alloc(64);
-  GMP_HIP(hipMemcpyAsync(d_slot.p, slot.data(), (n + 1) * 4, hipMemcpyHostToDevice, e.stream));
-  GMP_HIP(hipMemcpyAsync(d_vb.p, vb.data(), n * 4, hipMemcpyHostToDevice, e.stream));
-  GMP_HIP(hipMemcpyAsync(d_seed.p, seeds.data(), n * 4, hipMemcpyHostToDevice, e.stream));
-  BatchSsspArgs a{};
+  r.T = r.slot[n];
+  r.d_slot.alloc((n + 1) * 4);
+  r.d_vb.alloc(n * 4);
+  r.d_seed.alloc(n * 4);
+  r.dist.alloc((size_t)r.T * 4);
+  r.stamp.alloc((size_t)r.T * 4);
+  r.q0.alloc((size_t)r.T * 4);
+  r.q1.alloc((size_t)r.T * 4);
+  r.cnt.alloc(64);
+  GMP_HIP(hipMemcpyAsync(r.d_slot.p, r.slot.data(), (n + 1) * 4, hipMemcpyHostToDevice, e.stream));
+  GMP_HIP(hipMemcpyAsync(r.d_vb.p, vb.data(), n * 4, hipMemcpyHostToDevice, e.stream));
+  GMP_HIP(hipMemcpyAsync(r.d_seed.p, seeds.data(), n * 4, hipMemcpyHostToDevice, e.stream));
+  BatchSsspArgs &a = r.a;
   a.off = b.out.off.as<uint64_t>();
   a.idx = b.out.idx.as<uint32_t>();
   a.w = b.out.w.as<uint32_t>();
-  a.inst_slot = d_slot.as<uint32_t>();
-  a.inst_vbase = d_vb.as<uint32_t>();
+  a.inst_slot = r.d_slot.as<uint32_t>();
+  a.inst_vbase = r.d_vb.as<uint32_t>();
   a.ninst = n;
-  a.dist = dist.as<uint32_t>();
-  a.stamp = stamp.as<uint32_t>();
-  a.queue0 = q0.as<uint32_t>();
-  a.queue1 = q1.as<uint32_t>();
-  a.count = cnt.as<uint32_t>();
-  a.ticket = cnt.as<uint32_t>() + 4;
-  a.edges = reinterpret_cast<unsigned long long *>(cnt.as<uint32_t>() + 8);
-  const uint32_t grid = (uint32_t)e.num_cus * 4;
+  a.dist = r.dist.as<uint32_t>();
+  a.stamp = r.stamp.as<uint32_t>();
+  a.queue0 = r.q0.as<uint32_t>();
+  a.queue1 = r.q1.as<uint32_t>();
+  a.count = r.cnt.as<uint32_t>();
+  a.ticket = r.cnt.as<uint32_t>() + 4;
+  a.edges = reinterpret_cast<unsigned long long *>(r.cnt.as<uint32_t>() + 8);
+  r.grid = std::max<uint32_t>(1, (uint32_t)e.num_cus * e.opt.bsssp_grid);
+}
 
+void sssp_start(Engine &e, SsspRun &r, hipStream_t s) {
+  batch_sssp_init_kernel<<<grid_for(r.T), kBlock, 0, s>>>(r.a.dist, r.a.stamp, r.T);
+  batch_sssp_seed_kernel<<<grid_for(r.n), kBlock, 0, s>>>(r.d_seed.as<uint32_t>(), r.n, r.a.dist, r.a.queue0,
+                                                          r.a.count, r.a.ticket, r.a.edges);
+  GMP_CHECK_LAUNCH();
+}
+
+void sssp_rounds(Engine &e, SsspRun &r, hipStream_t s, int rounds) {
+  for (int k = 0; k < rounds; k++) {
+    switch (e.opt.bsssp_unroll) {
+      case 2: batch_relax_kernel<2><<<r.grid, kBlock, 0, s>>>(r.a); break;
+      case 8: batch_relax_kernel<8><<<r.grid, kBlock, 0, s>>>(r.a); break;
+      default: batch_relax_kernel<4><<<r.grid, kBlock, 0, s>>>(r.a); break;
+    }
+    GMP_CHECK_LAUNCH();
+  }
+}
+
+void sssp_results(Engine &e, SsspRun &r, uint32_t *dist_out, uint64_t &edges) {
+  unsigned long long h = 0;
+  GMP_HIP(hipMemcpy(&h, r.a.edges, 8, hipMemcpyDeviceToHost));
+  edges = h;
+  if (dist_out) GMP_HIP(hipMemcpy(dist_out, r.a.dist, (size_t)r.T * 4, hipMemcpyDeviceToHost));
+}
+
+struct PprRun {
+  uint32_t n = 0, ng = 0;
+  uint64_t rtot = 0, instance_edges = 0;
+  std::vector<PprGroup> groups;
+  std::vector<uint32_t> group_of, order;
+  DevBuf d_groups, d_gof, d_seed, d_tasks, d_hubs, d_next, d_dang, d_part, rank0, rank1, con0, con1, d_iter;
+  PprArgs a{};
+  float *rank[2] = {nullptr, nullptr};
+  float *con[2] = {nullptr, nullptr};
+  unsigned long long *dang[2] = {nullptr, nullptr};
+  uint32_t gather_grid = 0, hub_grid = 0, init_x = 1;
+};
+
+void ppr_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *seed, float damping, PprRun &r) {
+  BatchGraphs &b = e.batch;
+  if (!b.ready) fail(GMP_ESTATE, "no batch loaded");
+  if (n == 0 || !graph || !seed) fail(GMP_EINVAL, "batched PageRank needs instances");
+  if (!(damping >= 0.0f && damping <= 1.0f)) fail(GMP_EINVAL, "bad damping");
+  const uint32_t G = (uint32_t)b.vbase.size() - 1;
+  const uint32_t piece = e.opt.ppr_piece, run_rows = e.opt.ppr_run_rows, run_edges = e.opt.ppr_run_edges;
+  // group instances by graph, at most 64 per group; group order keeps caller order per graph
+  std::vector<std::vector<uint32_t>> per(G);
+  for (uint32_t i = 0; i < n; i++) {
+    if (graph[i] >= G) fail(GMP_EINVAL, "instance graph out of range");
+    if (seed[i] >= b.vbase[graph[i] + 1] - b.vbase[graph[i]]) fail(GMP_EINVAL, "instance seed out of range");
+    per[graph[i]].push_back(i);
+    r.instance_edges += b.ebase[graph[i] + 1] - b.ebase[graph[i]];
+  }
+  r.n = n;
+  std::vector<uint32_t> seed_local;
+  std::vector<PprTask> pieces, runs;
+  std::vector<PprHub> hubs;
+  uint32_t nparts = 0;
+  for (uint32_t k = 0; k < G; k++) {
+    const uint32_t Vk = b.vbase[k + 1] - b.vbase[k];
+    const uint64_t *off = b.h_off.data() + b.vbase[k];
+    for (size_t s0 = 0; s0 < per[k].size(); s0 += kCols) {
+      PprGroup g{};
+      g.vbase = b.vbase[k];
+      g.V = Vk;
+      g.k = (uint32_t)std::min<size_t>(kCols, per[k].size() - s0);
+      g.kk = 1;
+      while (g.kk < g.k) g.kk <<= 1;
+      g.rbase = r.rtot;
+      g.first = (uint32_t)r.order.size();
+      const uint32_t gid = (uint32_t)r.groups.size();
+      for (uint32_t c = 0; c < g.k; c++) {
+        const uint32_t i = per[k][s0 + c];
+        r.order.push_back(i);
+        r.group_of.push_back(gid);
+        seed_local.push_back(b.h_o2n[b.vbase[k] + seed[i]] - b.vbase[k]);
+      }
+      r.rtot += (uint64_t)Vk * g.k;
+      // tasks: the pieces of the hub rows, and runs of the other rows sized for the slots
+      const uint32_t S = kWave / g.kk;
+      for (uint32_t row = 0; row < Vk; row++) {
+        const uint64_t d = off[row + 1] - off[row];
+        if (d <= piece) continue;
+        const uint32_t np = (uint32_t)((d + piece - 1) / piece);
+        hubs.push_back(PprHub{gid, row, nparts, np});
+        for (uint32_t q = 0; q < np; q++) pieces.push_back(PprTask{gid, row, q, nparts + q});
+        nparts += np;
+      }
+      for (uint32_t r0 = 0; r0 < Vk;) {
+        uint32_t r1 = r0;
+        uint64_t edges = 0;
+        while (r1 < Vk && r1 - r0 < run_rows * S) {
+          const uint64_t d = off[r1 + 1] - off[r1];
+          if (d > piece || (r1 > r0 && edges + d > (uint64_t)run_edges * S)) break;
+          edges += d;
+          r1++;
+        }
+        if (r1 > r0) runs.push_back(PprTask{gid, r0, r1 - r0, kNoPiece});
+        r0 = r1 + (r1 < Vk && off[r1 + 1] - off[r1] > piece ? 1 : 0);
+      }
+      r.groups.push_back(g);
+    }
+  }
+  // the queue: hub pieces first (the longest tasks start early), then the runs
+  std::vector<PprTask> tasks(pieces);
+  tasks.insert(tasks.end(), runs.begin(), runs.end());
+  r.ng = (uint32_t)r.groups.size();
+  r.d_groups.alloc(r.ng * sizeof(PprGroup));
+  r.d_gof.alloc(n * 4);
+  r.d_seed.alloc(n * 4);
+  r.d_tasks.alloc(std::max<size_t>(tasks.size(), 1) * sizeof(PprTask));
+  r.d_hubs.alloc(std::max<size_t>(hubs.size(), 1) * sizeof(PprHub));
+  r.d_next.alloc(4);
+  r.d_iter.alloc(4);
+  r.d_dang.alloc((size_t)2 * n * 8);  // two fixed-point dangling sums per instance, ping-pong
+  r.d_part.alloc(std::max<size_t>(nparts, 1) * kCols * 8);
+  r.rank0.alloc(std::max<uint64_t>(r.rtot, 1) * 4);
+  r.rank1.alloc(std::max<uint64_t>(r.rtot, 1) * 4);
+  r.con0.alloc(std::max<uint64_t>(r.rtot, 1) * 4);
+  r.con1.alloc(std::max<uint64_t>(r.rtot, 1) * 4);
+  GMP_HIP(hipMemcpyAsync(r.d_groups.p, r.groups.data(), r.ng * sizeof(PprGroup), hipMemcpyHostToDevice, e.stream));
+  GMP_HIP(hipMemcpyAsync(r.d_gof.p, r.group_of.data(), n * 4, hipMemcpyHostToDevice, e.stream));
+  GMP_HIP(hipMemcpyAsync(r.d_seed.p, seed_local.data(), n * 4, hipMemcpyHostToDevice, e.stream));
+  if (!tasks.empty())
+    GMP_HIP(hipMemcpyAsync(r.d_tasks.p, tasks.data(), tasks.size() * sizeof(PprTask), hipMemcpyHostToDevice, e.stream));
+  if (!hubs.empty())
+    GMP_HIP(hipMemcpyAsync(r.d_hubs.p, hubs.data(), hubs.size() * sizeof(PprHub), hipMemcpyHostToDevice, e.stream));
+  r.rank[0] = r.rank0.as<float>();
+  r.rank[1] = r.rank1.as<float>();
+  r.con[0] = r.con0.as<float>();
+  r.con[1] = r.con1.as<float>();
+  r.dang[0] = r.d_dang.as<unsigned long long>();
+  r.dang[1] = r.dang[0] + n;
+  PprArgs &a = r.a;
+  a.off = b.in.off.as<uint64_t>();
+  a.idx = b.in.idx.as<uint32_t>();
+  a.outdeg = b.outdeg.as<uint32_t>();
+  a.groups = r.d_groups.as<PprGroup>();
+  a.tasks = r.d_tasks.as<PprTask>();
+  a.ntasks = (uint32_t)tasks.size();
+  a.next = r.d_next.as<uint32_t>();
+  a.hubs = r.d_hubs.as<PprHub>();
+  a.nhubs = (uint32_t)hubs.size();
+  a.seed_local = r.d_seed.as<uint32_t>();
+  a.part = r.d_part.as<double>();
+  a.d = damping;
+  a.piece = piece;
+  r.gather_grid = (uint32_t)std::max(1, e.num_cus) * e.opt.ppr_grid;
+  r.hub_grid = std::max<uint32_t>(1, std::min<uint32_t>((a.nhubs + kWavesPerBlock - 1) / kWavesPerBlock,
+                                                         (uint32_t)e.num_cus * 8));
+  for (const PprGroup &g : r.groups)
+    r.init_x = std::max<uint32_t>(r.init_x, (uint32_t)std::min<uint64_t>(((uint64_t)g.V * g.k + kBlock - 1) / kBlock, 1024));
+}
+
+__global__ void set_u32_kernel(uint32_t *p, uint32_t v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+void ppr_start(Engine &e, PprRun &r, hipStream_t s) {
+  ppr_init_kernel<<<dim3(r.init_x, r.ng), kBlock, 0, s>>>(r.a.groups, r.a.seed_local, r.a.outdeg, r.rank[0], r.con[0],
+                                                          r.dang[0]);
+  set_u32_kernel<<<1, 64, 0, s>>>(r.d_iter.as<uint32_t>(), 0u);
+  GMP_CHECK_LAUNCH();
+}
+
+// Iterations it0 .. it0 + count - 1 (ping-pong buffers by parity).  guard: each iteration checks
+// the iteration counter against `limit` and counts itself (for graphs replayed past the end).
+void ppr_iterations(Engine &e, PprRun &r, hipStream_t s, int it0, int count, bool guard, uint32_t limit) {
+  for (int it = it0; it < it0 + count; it++) {
+    const int cur = it & 1;
+    PprArgs ai = r.a;
+    ai.contrib_in = r.con[cur];
+    ai.rank_out = r.rank[cur ^ 1];
+    ai.contrib_out = r.con[cur ^ 1];
+    ai.dang = r.dang[cur];
+    ai.dang_next = r.dang[cur ^ 1];
+    ai.iter = guard ? r.d_iter.as<uint32_t>() : nullptr;
+    ai.iter_limit = limit;
+    ppr_reset_kernel<<<1, kBlock, 0, s>>>(ai.next, ai.dang_next, r.n, ai.iter, limit);
+    switch (e.opt.ppr_loads) {
+      case 4: ppr_gather_kernel<4><<<r.gather_grid, kBlock, 0, s>>>(ai); break;
+      case 16: ppr_gather_kernel<16><<<r.gather_grid, kBlock, 0, s>>>(ai); break;
+      default: ppr_gather_kernel<8><<<r.gather_grid, kBlock, 0, s>>>(ai); break;
+    }
+    if (r.a.nhubs) ppr_hub_kernel<<<r.hub_grid, kBlock, 0, s>>>(ai);
+    if (guard) ppr_advance_kernel<<<1, 64, 0, s>>>(ai.iter, limit);
+    GMP_CHECK_LAUNCH();
+  }
+}
+
+void ppr_results(Engine &e, PprRun &r, const uint32_t *graph, int iters, float *rank_out) {
+  if (!rank_out) return;
+  BatchGraphs &b = e.batch;
+  const float *fin = r.rank[iters & 1];
+  std::vector<float> h((size_t)r.rtot);
+  GMP_HIP(hipMemcpy(h.data(), fin, (size_t)r.rtot * 4, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> obase(r.n + 1, 0);
+  for (uint32_t i = 0; i < r.n; i++) obase[i + 1] = obase[i] + (b.vbase[graph[i] + 1] - b.vbase[graph[i]]);
+  for (uint32_t gi = 0; gi < r.n; gi++) {
+    const PprGroup &g = r.groups[r.group_of[gi]];
+    const uint32_t col = gi - g.first, i = r.order[gi];
+    float *dst = rank_out + obase[i];
+    const uint32_t *map = b.h_o2n.data() + g.vbase;  // caller's local id -> stored union id
+    for (uint32_t v = 0; v < g.V; v++) dst[v] = h[g.rbase + (uint64_t)(map[v] - g.vbase) * g.k + col];
+  }
+}
+
+// Reads the SSSP counters ({|queue0|, |queue1|, round}) and the PPR iteration count.
+void read_batch_state(Engine &e, const SsspRun *sr, const PprRun *pr, uint32_t h[4], uint32_t &it) {
+  uint32_t *p = static_cast<uint32_t *>(e.h_pinned);
+  if (sr) GMP_HIP(hipMemcpyAsync(p, sr->a.count, 16, hipMemcpyDeviceToHost, e.stream));
+  if (pr) GMP_HIP(hipMemcpyAsync(p + 4, pr->d_iter.p, 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  if (sr) std::memcpy(h, p, 16);
+  if (pr) it = p[4];
+}
+
+}  // namespace
+
+void batch_sssp(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *source, uint32_t *dist_out,
+                gmp_stats *st) {
+  SsspRun r;
+  sssp_prepare(e, n, graph, source, r);
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
-  batch_sssp_init_kernel<<<grid_for(T), kBlock, 0, e.stream>>>(a.dist, a.stamp, T);
-  batch_sssp_seed_kernel<<<grid_for(n), kBlock, 0, e.stream>>>(d_seed.as<uint32_t>(), n, a.dist, a.queue0, a.count,
-                                                              a.ticket, a.edges);
-  GMP_CHECK_LAUNCH();
+  sssp_start(e, r, e.stream);
   // capture two rounds; the counters live on the device, so replays need no host input
   GraphExecGuard ge;
   GMP_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-  batch_relax_kernel<<<grid, kBlock, 0, e.stream>>>(a);
-  batch_relax_kernel<<<grid, kBlock, 0, e.stream>>>(a);
+  sssp_rounds(e, r, e.stream, 2);
   GMP_HIP(hipStreamEndCapture(e.stream, &ge.graph));
   GMP_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
-  uint32_t h[4] = {0, 0, 0, 0};
+  uint32_t h[4] = {0, 0, 0, 0}, unused = 0;
   int launches = 0;
   for (;;) {
     GMP_HIP(hipGraphLaunch(ge.exec, e.stream));
     launches++;
-    GMP_HIP(hipMemcpyAsync(e.h_pinned, a.count, 16, hipMemcpyDeviceToHost, e.stream));
-    GMP_HIP(hipStreamSynchronize(e.stream));
-    std::memcpy(h, e.h_pinned, 16);
+    read_batch_state(e, &r, nullptr, h, unused);
     if (h[h[2] & 1u] == 0) break;
     if (launches > (1 << 22)) fail(GMP_ESTATE, "batched SSSP did not converge");
   }
   GMP_HIP(hipEventRecord(tot.b, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
-  unsigned long long edges = 0;
-  GMP_HIP(hipMemcpy(&edges, a.edges, 8, hipMemcpyDeviceToHost));
-  if (dist_out) GMP_HIP(hipMemcpy(dist_out, a.dist, (size_t)T * 4, hipMemcpyDeviceToHost));
+  uint64_t edges = 0;
+  sssp_results(e, r, dist_out, edges);
   if (st) {
     std::memset(st, 0, sizeof *st);
     st->iterations = (int32_t)h[2];
@@ -650,135 +899,15 @@ void batch_sssp(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *so
 
 void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *seed, float damping, int32_t iters,
                float *rank_out, gmp_stats *st) {
-  BatchGraphs &b = e.batch;
-  if (!b.ready) fail(GMP_ESTATE, "no batch loaded");
-  if (n == 0 || !graph || !seed) fail(GMP_EINVAL, "batched PageRank needs instances");
-  if (!(damping >= 0.0f && damping <= 1.0f) || iters < 0) fail(GMP_EINVAL, "bad damping or iteration count");
-  const uint32_t G = (uint32_t)b.vbase.size() - 1;
-  // group instances by graph, at most 64 per group; group order keeps caller order per graph
-  std::vector<std::vector<uint32_t>> per(G);
-  for (uint32_t i = 0; i < n; i++) {
-    if (graph[i] >= G) fail(GMP_EINVAL, "instance graph out of range");
-    if (seed[i] >= b.vbase[graph[i] + 1] - b.vbase[graph[i]]) fail(GMP_EINVAL, "instance seed out of range");
-    per[graph[i]].push_back(i);
-  }
-  std::vector<PprGroup> groups;
-  std::vector<uint32_t> group_of, seed_local, order;  // per group-order instance
-  std::vector<PprTask> pieces, runs;
-  std::vector<PprHub> hubs;
-  uint64_t rtot = 0;
-  uint32_t nparts = 0;
-  for (uint32_t k = 0; k < G; k++) {
-    const uint32_t Vk = b.vbase[k + 1] - b.vbase[k];
-    const uint64_t *off = b.h_off.data() + b.vbase[k];
-    for (size_t s0 = 0; s0 < per[k].size(); s0 += kCols) {
-      PprGroup g{};
-      g.vbase = b.vbase[k];
-      g.V = Vk;
-      g.k = (uint32_t)std::min<size_t>(kCols, per[k].size() - s0);
-      g.kk = 1;
-      while (g.kk < g.k) g.kk <<= 1;
-      g.rbase = rtot;
-      g.first = (uint32_t)order.size();
-      const uint32_t gid = (uint32_t)groups.size();
-      for (uint32_t c = 0; c < g.k; c++) {
-        const uint32_t i = per[k][s0 + c];
-        order.push_back(i);
-        group_of.push_back(gid);
-        seed_local.push_back(b.h_o2n[b.vbase[k] + seed[i]] - b.vbase[k]);
-      }
-      rtot += (uint64_t)Vk * g.k;
-      // tasks: the pieces of the hub rows, and runs of the other rows sized for the slots
-      const uint32_t S = kWave / g.kk;
-      for (uint32_t r = 0; r < Vk; r++) {
-        const uint64_t d = off[r + 1] - off[r];
-        if (d <= kHubDegree) continue;
-        const uint32_t np = (uint32_t)((d + kHubDegree - 1) / kHubDegree);
-        hubs.push_back(PprHub{gid, r, nparts, np});
-        for (uint32_t q = 0; q < np; q++) pieces.push_back(PprTask{gid, r, q, nparts + q});
-        nparts += np;
-      }
-      for (uint32_t r0 = 0; r0 < Vk;) {
-        uint32_t r1 = r0;
-        uint64_t edges = 0;
-        while (r1 < Vk && r1 - r0 < kRunRows * S) {
-          const uint64_t d = off[r1 + 1] - off[r1];
-          if (d > kHubDegree || (r1 > r0 && edges + d > (uint64_t)kRunEdges * S)) break;
-          edges += d;
-          r1++;
-        }
-        if (r1 > r0) runs.push_back(PprTask{gid, r0, r1 - r0, kNoPiece});
-        r0 = r1 + (r1 < Vk && off[r1 + 1] - off[r1] > kHubDegree ? 1 : 0);
-      }
-      groups.push_back(g);
-    }
-  }
-  // the queue: hub pieces first (the longest tasks start early), then the runs
-  std::vector<PprTask> tasks(pieces);
-  tasks.insert(tasks.end(), runs.begin(), runs.end());
-  const uint32_t ng = (uint32_t)groups.size();
-  DevBuf d_groups, d_gof, d_seed, d_tasks, d_hubs, d_next, d_dang, d_part, rank0, rank1, con0, con1;
-  d_groups.alloc(ng * sizeof(PprGroup));
-  d_gof.alloc(n * 4);
-  d_seed.alloc(n * 4);
-  d_tasks.alloc(std::max<size_t>(tasks.size(), 1) * sizeof(PprTask));
-  d_hubs.alloc(std::max<size_t>(hubs.size(), 1) * sizeof(PprHub));
-  d_next.alloc(4);
-  d_dang.alloc((size_t)2 * n * 8);  // two fixed-point dangling sums per instance, ping-pong
-  d_part.alloc(std::max<size_t>(nparts, 1) * kCols * 8);
-  rank0.alloc(std::max<uint64_t>(rtot, 1) * 4);
-  rank1.alloc(std::max<uint64_t>(rtot, 1) * 4);
-  con0.alloc(std::max<uint64_t>(rtot, 1) * 4);
-  con1.alloc(std::max<uint64_t>(rtot, 1) * 4);
-  GMP_HIP(hipMemcpyAsync(d_groups.p, groups.data(), ng * sizeof(PprGroup), hipMemcpyHostToDevice, e.stream));
-  GMP_HIP(hipMemcpyAsync(d_gof.p, group_of.data(), n * 4, hipMemcpyHostToDevice, e.stream));
-  GMP_HIP(hipMemcpyAsync(d_seed.p, seed_local.data(), n * 4, hipMemcpyHostToDevice, e.stream));
-  if (!tasks.empty())
-    GMP_HIP(hipMemcpyAsync(d_tasks.p, tasks.data(), tasks.size() * sizeof(PprTask), hipMemcpyHostToDevice, e.stream));
-  if (!hubs.empty())
-    GMP_HIP(hipMemcpyAsync(d_hubs.p, hubs.data(), hubs.size() * sizeof(PprHub), hipMemcpyHostToDevice, e.stream));
-  float *rank[2] = {rank0.as<float>(), rank1.as<float>()};
-  float *con[2] = {con0.as<float>(), con1.as<float>()};
-  unsigned long long *dang[2] = {d_dang.as<unsigned long long>(), d_dang.as<unsigned long long>() + n};
-  PprArgs a{};
-  a.off = b.in.off.as<uint64_t>();
-  a.idx = b.in.idx.as<uint32_t>();
-  a.outdeg = b.outdeg.as<uint32_t>();
-  a.groups = d_groups.as<PprGroup>();
-  a.tasks = d_tasks.as<PprTask>();
-  a.ntasks = (uint32_t)tasks.size();
-  a.next = d_next.as<uint32_t>();
-  a.hubs = d_hubs.as<PprHub>();
-  a.nhubs = (uint32_t)hubs.size();
-  a.seed_local = d_seed.as<uint32_t>();
-  a.part = d_part.as<double>();
-  a.d = damping;
-  const uint32_t gather_grid = (uint32_t)std::max(1, e.num_cus) * 8;
-  const uint32_t hub_grid = std::max<uint32_t>(1, std::min<uint32_t>((a.nhubs + kWavesPerBlock - 1) / kWavesPerBlock,
-                                                                      (uint32_t)e.num_cus * 8));
-  uint32_t init_x = 1;
-  for (const PprGroup &g : groups)
-    init_x = std::max<uint32_t>(init_x, (uint32_t)std::min<uint64_t>(((uint64_t)g.V * g.k + kBlock - 1) / kBlock, 1024));
-
+  if (iters < 0) fail(GMP_EINVAL, "bad iteration count");
+  PprRun r;
+  ppr_prepare(e, n, graph, seed, damping, r);
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
-  ppr_init_kernel<<<dim3(init_x, ng), kBlock, 0, e.stream>>>(a.groups, a.seed_local, a.outdeg, rank[0], con[0],
-                                                              dang[0]);
-  GMP_CHECK_LAUNCH();
+  ppr_start(e, r, e.stream);
   GraphExecGuard ge;
   GMP_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-  for (int it = 0; it < iters; it++) {
-    const int cur = it & 1;
-    PprArgs ai = a;
-    ai.contrib_in = con[cur];
-    ai.rank_out = rank[cur ^ 1];
-    ai.contrib_out = con[cur ^ 1];
-    ai.dang = dang[cur];
-    ai.dang_next = dang[cur ^ 1];
-    ppr_reset_kernel<<<1, kBlock, 0, e.stream>>>(ai.next, ai.dang_next, n);
-    ppr_gather_kernel<<<gather_grid, kBlock, 0, e.stream>>>(ai);
-    if (a.nhubs) ppr_hub_kernel<<<hub_grid, kBlock, 0, e.stream>>>(ai);
-  }
+  ppr_iterations(e, r, e.stream, 0, iters, false, 0u);
   GMP_HIP(hipStreamEndCapture(e.stream, &ge.graph));
   if (iters > 0) {
     GMP_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
@@ -786,29 +915,95 @@ void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *see
   }
   GMP_HIP(hipEventRecord(tot.b, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
-  const float *fin = rank[iters & 1];
-  uint64_t total_edges = 0;
-  for (uint32_t i = 0; i < n; i++) total_edges += b.ebase[graph[i] + 1] - b.ebase[graph[i]];
-  if (rank_out) {
-    std::vector<float> h((size_t)rtot);
-    GMP_HIP(hipMemcpy(h.data(), fin, (size_t)rtot * 4, hipMemcpyDeviceToHost));
-    std::vector<uint64_t> obase(n + 1, 0);
-    for (uint32_t i = 0; i < n; i++) obase[i + 1] = obase[i] + (b.vbase[graph[i] + 1] - b.vbase[graph[i]]);
-    for (uint32_t gi = 0; gi < n; gi++) {
-      const PprGroup &g = groups[group_of[gi]];
-      const uint32_t col = gi - g.first, i = order[gi];
-      float *dst = rank_out + obase[i];
-      const uint32_t *map = b.h_o2n.data() + g.vbase;  // caller's local id -> stored union id
-      for (uint32_t v = 0; v < g.V; v++) dst[v] = h[g.rbase + (uint64_t)(map[v] - g.vbase) * g.k + col];
-    }
-  }
+  ppr_results(e, r, graph, iters, rank_out);
   if (st) {
     std::memset(st, 0, sizeof *st);
     st->iterations = iters;
     st->launches = 1;
     st->ms_total = tot.ms();
     st->ms_kernel = st->ms_total;
-    st->edges = total_edges * (uint64_t)iters;
+    st->edges = r.instance_edges * (uint64_t)iters;
+  }
+}
+
+// SSSP and PPR instances in one replayed HIP graph: a fork into two branches -- two SSSP rounds on
+// one stream, two PPR iterations (guarded by the iteration counter) on the other -- joined again,
+// replayed until the SSSP frontier is empty and every PPR iteration ran.  The branches run at the
+// same time, so the PPR gathers fill the chip while the SSSP rounds wait on their atomics.
+void batch_run(Engine &e, uint32_t ns, const uint32_t *sgraph, const uint32_t *source, uint32_t np,
+               const uint32_t *pgraph, const uint32_t *seed, float damping, int32_t iters, uint32_t *dist_out,
+               float *rank_out, gmp_stats *st_sssp, gmp_stats *st_ppr, double *ms_total) {
+  if (iters < 0) fail(GMP_EINVAL, "bad iteration count");
+  if (ns == 0 && np == 0) fail(GMP_EINVAL, "batched run needs instances");
+  SsspRun sr;
+  PprRun pr;
+  if (ns) sssp_prepare(e, ns, sgraph, source, sr);
+  if (np) ppr_prepare(e, np, pgraph, seed, damping, pr);
+  const bool has_ppr = np && iters > 0;
+  struct StreamGuard {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    ~StreamGuard() {
+      if (fork) (void)hipEventDestroy(fork);
+      if (join) (void)hipEventDestroy(join);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } sg;
+  GMP_HIP(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
+  GMP_HIP(hipEventCreateWithFlags(&sg.fork, hipEventDisableTiming));
+  GMP_HIP(hipEventCreateWithFlags(&sg.join, hipEventDisableTiming));
+  EventPair tot;
+  GMP_HIP(hipEventRecord(tot.a, e.stream));
+  if (ns) sssp_start(e, sr, e.stream);
+  if (np) ppr_start(e, pr, e.stream);
+  GraphExecGuard ge;
+  GMP_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+  GMP_HIP(hipEventRecord(sg.fork, e.stream));
+  GMP_HIP(hipStreamWaitEvent(sg.s, sg.fork, 0));
+  if (ns) sssp_rounds(e, sr, e.stream, 2);
+  if (has_ppr) ppr_iterations(e, pr, sg.s, 0, 2, true, (uint32_t)iters);
+  GMP_HIP(hipEventRecord(sg.join, sg.s));
+  GMP_HIP(hipStreamWaitEvent(e.stream, sg.join, 0));
+  GMP_HIP(hipStreamEndCapture(e.stream, &ge.graph));
+  GMP_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
+  uint32_t h[4] = {0, 0, 0, 0}, it = 0;
+  int launches = 0;
+  for (;;) {
+    GMP_HIP(hipGraphLaunch(ge.exec, e.stream));
+    launches++;
+    read_batch_state(e, ns ? &sr : nullptr, has_ppr ? &pr : nullptr, h, it);
+    const bool sssp_done = !ns || h[h[2] & 1u] == 0;
+    const bool ppr_done = !has_ppr || it >= (uint32_t)iters;
+    if (sssp_done && ppr_done) break;
+    if (launches > (1 << 22)) fail(GMP_ESTATE, "batched run did not converge");
+  }
+  GMP_HIP(hipEventRecord(tot.b, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  const double ms = tot.ms();
+  if (ms_total) *ms_total = ms;
+  if (ns) {
+    uint64_t edges = 0;
+    sssp_results(e, sr, dist_out, edges);
+    if (st_sssp) {
+      std::memset(st_sssp, 0, sizeof *st_sssp);
+      st_sssp->iterations = (int32_t)h[2];
+      st_sssp->launches = launches;
+      st_sssp->ms_total = ms;
+      st_sssp->ms_kernel = ms;
+      st_sssp->edges = edges;
+      st_sssp->bytes = edges * 12;
+    }
+  }
+  if (np) {
+    ppr_results(e, pr, pgraph, iters, rank_out);
+    if (st_ppr) {
+      std::memset(st_ppr, 0, sizeof *st_ppr);
+      st_ppr->iterations = iters;
+      st_ppr->launches = launches;
+      st_ppr->ms_total = ms;
+      st_ppr->ms_kernel = ms;
+      st_ppr->edges = pr.instance_edges * (uint64_t)iters;
+    }
   }
 }
 

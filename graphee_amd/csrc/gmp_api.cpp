@@ -160,6 +160,34 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       in(0, 1, "learned_finish");
       e->opt.learned_finish = (int)value;
       break;
+    case GMP_OPT_BATCH_SSSP_UNROLL:
+      if (value != 2 && value != 4 && value != 8) fail(GMP_EINVAL, "gmp_set_option: batch_sssp_unroll must be 2, 4 or 8");
+      e->opt.bsssp_unroll = (uint32_t)value;
+      break;
+    case GMP_OPT_BATCH_SSSP_GRID:
+      in(1, 64, "batch_sssp_grid");
+      e->opt.bsssp_grid = (uint32_t)value;
+      break;
+    case GMP_OPT_BATCH_PPR_PIECE:
+      in(64, 65536, "batch_ppr_piece");
+      e->opt.ppr_piece = (uint32_t)value;
+      break;
+    case GMP_OPT_BATCH_PPR_RUN_ROWS:
+      in(1, 1024, "batch_ppr_run_rows");
+      e->opt.ppr_run_rows = (uint32_t)value;
+      break;
+    case GMP_OPT_BATCH_PPR_RUN_EDGES:
+      in(64, 65536, "batch_ppr_run_edges");
+      e->opt.ppr_run_edges = (uint32_t)value;
+      break;
+    case GMP_OPT_BATCH_PPR_LOADS:
+      if (value != 4 && value != 8 && value != 16) fail(GMP_EINVAL, "gmp_set_option: batch_ppr_loads must be 4, 8 or 16");
+      e->opt.ppr_loads = (uint32_t)value;
+      break;
+    case GMP_OPT_BATCH_PPR_GRID:
+      in(1, 64, "batch_ppr_grid");
+      e->opt.ppr_grid = (uint32_t)value;
+      break;
     default:
       fail(GMP_EINVAL, "gmp_set_option: unknown key " + std::to_string(key));
   }
@@ -187,6 +215,13 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_CC_UNION_FIND: *value = e->opt.cc_union_find; break;
     case GMP_OPT_DELTA_PULL_LOG: *value = e->opt.delta_pull_log; break;
     case GMP_OPT_LEARNED_FINISH: *value = e->opt.learned_finish; break;
+    case GMP_OPT_BATCH_SSSP_UNROLL: *value = e->opt.bsssp_unroll; break;
+    case GMP_OPT_BATCH_SSSP_GRID: *value = e->opt.bsssp_grid; break;
+    case GMP_OPT_BATCH_PPR_PIECE: *value = e->opt.ppr_piece; break;
+    case GMP_OPT_BATCH_PPR_RUN_ROWS: *value = e->opt.ppr_run_rows; break;
+    case GMP_OPT_BATCH_PPR_RUN_EDGES: *value = e->opt.ppr_run_edges; break;
+    case GMP_OPT_BATCH_PPR_LOADS: *value = e->opt.ppr_loads; break;
+    case GMP_OPT_BATCH_PPR_GRID: *value = e->opt.ppr_grid; break;
     default: fail(GMP_EINVAL, "gmp_get_option: unknown key " + std::to_string(key));
   }
   GMP_API_END
@@ -464,6 +499,27 @@ int gmp_batch_ppr(gmp_engine *h, uint32_t n, const uint32_t *graph, const uint32
   Engine *e = check(h);
   GMP_HIP(hipSetDevice(e->device));
   gmp::batch_ppr(*e, n, graph, seed, damping, iters, rank_out, st);
+  GMP_API_END
+}
+
+int gmp_batch_run(gmp_engine *h, uint32_t ns, const uint32_t *sssp_graph, const uint32_t *source, uint32_t np,
+                  const uint32_t *ppr_graph, const uint32_t *seed, float damping, int32_t iters,
+                  uint32_t *dist_out, float *rank_out, gmp_stats *st_sssp, gmp_stats *st_ppr) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  GMP_HIP(hipSetDevice(e->device));
+  gmp::batch_run(*e, ns, sssp_graph, source, np, ppr_graph, seed, damping, iters, dist_out, rank_out, st_sssp,
+                 st_ppr, nullptr);
+  GMP_API_END
+}
+
+int gmp_batch_autotune(gmp_engine *h, uint32_t ns, const uint32_t *sssp_graph, const uint32_t *source,
+                       uint32_t np, const uint32_t *ppr_graph, const uint32_t *seed, float damping, int32_t iters,
+                       gmp_batch_tune_result *out) {
+  GMP_API_BEGIN
+  Engine *e = check(h);
+  GMP_HIP(hipSetDevice(e->device));
+  gmp::batch_autotune(*e, ns, sssp_graph, source, np, ppr_graph, seed, damping, iters, out);
   GMP_API_END
 }
 

@@ -253,6 +253,14 @@ struct Options {
   int cc_union_find = 1;       // CC of a symmetric graph by sampled union-find (0: sweeps and pushes)
   uint32_t delta_pull_log = 3; // delta-stepping rounds with more than E >> k frontier out-edges pull (0: never)
   int learned_finish = 0;      // learned SSSP: class 7 finishes by Bellman-Ford instead of width 1024
+  // batched runs (batch.hip; gmp_batch_autotune writes these)
+  uint32_t bsssp_unroll = 4;     // SSSP relax: edges per lane in flight (2, 4, 8)
+  uint32_t bsssp_grid = 4;       // SSSP relax: workgroups per CU
+  uint32_t ppr_piece = 1024;     // PPR: rows with more in-edges are cut into pieces of this many
+  uint32_t ppr_run_rows = 32;    // PPR: rows per slot in a row run
+  uint32_t ppr_run_edges = 2048; // PPR: edges per slot in a row run
+  uint32_t ppr_loads = 8;        // PPR: source loads in flight per lane (4, 8, 16)
+  uint32_t ppr_grid = 8;         // PPR gather: workgroups per CU
 };
 
 // Many graphs resident at once as one disjoint union (graph k owns union ids
@@ -483,6 +491,12 @@ void batch_sssp(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *so
                 gmp_stats *st);
 void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *seed, float damping, int32_t iters,
                float *rank_out, gmp_stats *st);
+void batch_run(Engine &e, uint32_t ns, const uint32_t *sgraph, const uint32_t *source, uint32_t np,
+               const uint32_t *pgraph, const uint32_t *seed, float damping, int32_t iters, uint32_t *dist_out,
+               float *rank_out, gmp_stats *st_sssp, gmp_stats *st_ppr, double *ms_total);
+void batch_autotune(Engine &e, uint32_t ns, const uint32_t *sgraph, const uint32_t *source, uint32_t np,
+                    const uint32_t *pgraph, const uint32_t *seed, float damping, int32_t iters,
+                    gmp_batch_tune_result *out);
 
 // ---- shared constants of the synthetic-input definition ----
 constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define GMP_ABI_VERSION 2
+#define GMP_ABI_VERSION 3
 
 /* status codes */
 #define GMP_OK 0
@@ -148,18 +148,30 @@ int gmp_destroy(gmp_engine *h);
                                      are bucket widths 8 .. 1024; 1: class 7 instead finishes the
                                      solve as Bellman-Ford (pushes and strip sweeps) from the live
                                      far pile.  Distances are the same. */
+/* Batched runs (gmp_batch_*); written by gmp_batch_autotune.  Results are the same for every
+ * value (SSSP distances bit for bit; PPR ranks within rounding of the summation order). */
+#define GMP_OPT_BATCH_SSSP_UNROLL 17  /* batched SSSP: edges per lane in flight, 2, 4 (default) or 8 */
+#define GMP_OPT_BATCH_SSSP_GRID 18    /* batched SSSP: relax workgroups per CU, 1..64 (default 4) */
+#define GMP_OPT_BATCH_PPR_PIECE 19    /* batched PPR: rows with more in-edges than this are cut into
+                                        pieces of this many, 64..65536 (default 1024) */
+#define GMP_OPT_BATCH_PPR_RUN_ROWS 20 /* batched PPR: rows per wave slot in a row run, 1..1024 (32) */
+#define GMP_OPT_BATCH_PPR_RUN_EDGES 21 /* batched PPR: edges per wave slot in a row run, 64..65536 (2048) */
+#define GMP_OPT_BATCH_PPR_LOADS 22    /* batched PPR: source loads in flight per lane, 4, 8 (default) or 16 */
+#define GMP_OPT_BATCH_PPR_GRID 23     /* batched PPR: gather workgroups per CU, 1..64 (default 8) */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 
 /* Autotuner (no reference counterpart: GraphMP's knobs are compile-time constants): times the
  * PageRank gather on the loaded graph for candidate bin thresholds, tile sizes, chunk sizes
  * and workgroup counts (coordinate descent, `iters` iterations per trial), then Bellman-Ford
- * for candidate GMP_OPT_PUSH_LOG values, applies the best and reports it.  Results of every
- * app are unchanged by any choice. */
+ * for candidate GMP_OPT_PUSH_LOG values and delta = 64 stepping for candidate
+ * GMP_OPT_DELTA_PULL_LOG values, applies the best and reports it.  Results of every app are
+ * unchanged by any choice. */
 typedef struct gmp_tune_result {
   uint32_t stream_cap, row_cap, chunk, vec_min, wg_per_cu, trials;
   double gather_ms_before, gather_ms_after; /* mean PageRank iteration time, first and best */
   uint32_t strip_log, bin_log, bin_task_log, push_log;
+  uint32_t delta_pull_log, reserved;
 } gmp_tune_result;
 int gmp_autotune(gmp_engine *h, int32_t iters, gmp_tune_result *out);
 /* Tunables as a flat JSON object, written after tuning and read at start-up. */
@@ -304,6 +316,26 @@ int gmp_batch_sssp(gmp_engine *h, uint32_t n, const uint32_t *graph, const uint3
                    gmp_stats *st);
 int gmp_batch_ppr(gmp_engine *h, uint32_t n, const uint32_t *graph, const uint32_t *seed, float damping,
                   int32_t iters, float *rank_out, gmp_stats *st);
+/* SSSP instances (ns) and personalised-PageRank instances (np) in one replayed HIP graph whose two
+ * branches run at the same time on the device: two SSSP rounds and two PPR iterations per replay,
+ * replayed until the frontier is empty and `iters` iterations ran.  Same outputs as
+ * gmp_batch_sssp and gmp_batch_ppr; both stats carry the combined run's time.  Either count may
+ * be 0. */
+int gmp_batch_run(gmp_engine *h, uint32_t ns, const uint32_t *sssp_graph, const uint32_t *source, uint32_t np,
+                  const uint32_t *ppr_graph, const uint32_t *seed, float damping, int32_t iters,
+                  uint32_t *dist_out, float *rank_out, gmp_stats *st_sssp, gmp_stats *st_ppr);
+/* Autotuner for the batched kernels (no reference counterpart): coordinate descent over the
+ * GMP_OPT_BATCH_* values, timing gmp_batch_sssp on the given SSSP instances and a few
+ * iterations of gmp_batch_ppr on the given PPR instances; applies and reports the best. */
+typedef struct gmp_batch_tune_result {
+  uint32_t sssp_unroll, sssp_grid_per_cu, ppr_piece, ppr_run_rows, ppr_run_edges, ppr_loads, ppr_grid_per_cu,
+      trials;
+  double sssp_ms_before, sssp_ms_after; /* one batched SSSP run, first and best */
+  double ppr_ms_before, ppr_ms_after;   /* batched PPR, per iteration, first and best */
+} gmp_batch_tune_result;
+int gmp_batch_autotune(gmp_engine *h, uint32_t ns, const uint32_t *sssp_graph, const uint32_t *source,
+                       uint32_t np, const uint32_t *ppr_graph, const uint32_t *seed, float damping, int32_t iters,
+                       gmp_batch_tune_result *out);
 
 /* ---- host-side format utilities (no GPU needed) -----------------------------------
  * Equal-in-edge partitioner, format_spark.py:85-108. */

@@ -110,3 +110,80 @@ def test_batch_layouts_agree(engine):
     for i in range(gsel.size):
         np.testing.assert_array_equal(d1[i], d0[i])
         assert np.abs(r1[i].astype(np.float64) - r0[i]).sum() < 1e-6, i
+
+
+def test_batch_run_one_graph_for_both_kinds(engine):
+    """SSSP and PPR instances in one replayed graph with two concurrent branches: the distances
+    equal Dijkstra's and the ranks equal gmp_batch_ppr's bit for bit (same kernels, order-free
+    reductions), for an odd iteration count too; either kind may be absent."""
+    parts = _batch(engine, SPECS)
+    rng = np.random.default_rng(5)
+    gs = np.array([1, 1, 3, 0, 2, 4, 1, 3], dtype=np.uint32)
+    srcs = np.array([rng.integers(0, parts[g][0].size - 1) for g in gs], dtype=np.uint32)
+    gp = np.array([2] * 9 + [1, 0, 3], dtype=np.uint32)
+    seeds = np.array([rng.integers(0, parts[g][0].size - 1) for g in gp], dtype=np.uint32)
+    for iters in (7, 30):
+        dists, ranks, st_s, st_p = engine.batch_run(gs, srcs, gp, seeds, damping=0.85, iters=iters)
+        assert st_p.iterations == iters and st_s.ms_total == st_p.ms_total > 0
+        alone, _ = engine.batch_ppr(gp, seeds, damping=0.85, iters=iters)
+        for i in range(gp.size):
+            np.testing.assert_array_equal(ranks[i].view(np.uint32), alone[i].view(np.uint32), err_msg=f"ppr {i}")
+        for i, (g, s) in enumerate(zip(gs, srcs)):
+            off, idx, w = parts[g]
+            V = off.size - 1
+            ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+            np.testing.assert_array_equal(dists[i], oracle.dijkstra(ooff, oidx, ow, int(s)), err_msg=f"sssp {i}")
+    d_only, r_none, _, _ = engine.batch_run(gs[:2], srcs[:2], [], [], iters=5)
+    assert len(r_none) == 0
+    off, idx, w = parts[gs[0]]
+    ooff, oidx, ow = graphs.transpose(off.size - 1, off, idx, w)
+    np.testing.assert_array_equal(d_only[0], oracle.dijkstra(ooff, oidx, ow, int(srcs[0])))
+    _, r_only, _, _ = engine.batch_run([], [], gp[:3], seeds[:3], iters=4)
+    alone, _ = engine.batch_ppr(gp[:3], seeds[:3], damping=0.85, iters=4)
+    np.testing.assert_array_equal(np.concatenate(r_only).view(np.uint32), np.concatenate(alone).view(np.uint32))
+
+
+def test_batch_autotune_keeps_results(engine):
+    """The batch tuner times every knob value on the given instances and applies the best; any
+    value gives Dijkstra's distances and ranks within 1e-6 L1 of FP64 (forced extremes too)."""
+    import graphee_amd as gmp
+    parts = _batch(engine, SPECS)
+    rng = np.random.default_rng(6)
+    gs = np.array([1, 1, 3, 0, 2], dtype=np.uint32)
+    srcs = np.array([rng.integers(0, parts[g][0].size - 1) for g in gs], dtype=np.uint32)
+    gp = np.array([2] * 5 + [1, 3], dtype=np.uint32)
+    seeds = np.array([rng.integers(0, parts[g][0].size - 1) for g in gp], dtype=np.uint32)
+    keys = (gmp.OPT_BATCH_SSSP_UNROLL, gmp.OPT_BATCH_SSSP_GRID, gmp.OPT_BATCH_PPR_PIECE, gmp.OPT_BATCH_PPR_RUN_ROWS,
+            gmp.OPT_BATCH_PPR_RUN_EDGES, gmp.OPT_BATCH_PPR_LOADS, gmp.OPT_BATCH_PPR_GRID)
+    saved = [engine.get_option(k) for k in keys]
+
+    def check_results():
+        d, _ = engine.batch_sssp(gs, srcs)
+        r, _ = engine.batch_ppr(gp, seeds, damping=0.85, iters=15)
+        for i, (g, s) in enumerate(zip(gs, srcs)):
+            off, idx, w = parts[g]
+            ooff, oidx, ow = graphs.transpose(off.size - 1, off, idx, w)
+            np.testing.assert_array_equal(d[i], oracle.dijkstra(ooff, oidx, ow, int(s)))
+        for i, (g, s) in enumerate(zip(gp, seeds)):
+            off, idx, _ = parts[g]
+            V = off.size - 1
+            x, _, _ = oracle.pr_f64(off, idx, graphs.out_degree(V, idx), d=0.85, max_iter=15, seed=int(s))
+            assert np.abs(r[i].astype(np.float64) - x).sum() < 1e-6, i
+
+    try:
+        res = engine.batch_autotune(gs, srcs, gp, seeds, iters=15)
+        print("batch tune", res, flush=True)
+        assert res["trials"] > 10 and res["sssp_ms_after"] <= res["sssp_ms_before"]
+        assert [engine.get_option(k) for k in keys] == [res["sssp_unroll"], res["sssp_grid_per_cu"], res["ppr_piece"],
+                                                        res["ppr_run_rows"], res["ppr_run_edges"], res["ppr_loads"],
+                                                        res["ppr_grid_per_cu"]]
+        check_results()
+        for vals in ((2, 1, 64, 1, 64, 4, 1), (8, 64, 65536, 1024, 65536, 16, 64)):
+            for k, v in zip(keys, vals):
+                engine.set_option(k, v)
+            check_results()
+        with pytest.raises(gmp.GmpError):
+            engine.set_option(gmp.OPT_BATCH_PPR_LOADS, 5)
+    finally:
+        for k, v in zip(keys, saved):
+            engine.set_option(k, v)
