@@ -461,6 +461,56 @@ def run_semi(scale, ef, seed, ref, iters=20):
                 f"host-to-device ({r['h2d_frac_of_peak']:.2f} of PCIe 5.0 x16, {r['h2d_frac_of_pinned_copy']:.2f} of a "
                 f"pinned copy), L1 vs resident {r['l1_vs_resident']:.2e}")
             del x, runs
+    out["sssp_prefetch"] = run_semi_sssp(scale, ef, seed)
+    return out
+
+
+def run_semi_sssp(scale, ef, seed, budget_mb=512):
+    """Weighted SSSP over the windows of a semi-external R-MAT graph (ids packed, weights plain,
+    both streamed): exact window activation against the learned prefetch, whose window predictor
+    is trained on traces of two other sources' solves on the same graph; distances must agree."""
+    import graphee_amd as gmp
+    from graphee_amd import learned
+    out = {"workload": f"sssp-rmat-s{scale}-ef{ef}, weights 1..255, edges in pinned host memory, "
+                       f"{budget_mb} MiB device budget"}
+    with gmp.Engine(0) as s:
+        s.set_option(gmp.OPT_EDGE_BUDGET_MB, budget_mb)
+        s.set_option(gmp.OPT_WINDOW_MB, 64)
+        s.set_option(gmp.OPT_WINDOW_PACKED, 1)
+        s.generate_rmat(scale, ef, seed, weighted=True, num_shards=16)
+        rng = np.random.default_rng(seed + 7)
+        train_src = [s.pick_source(int(rng.integers(1 << 30))) for _ in range(2)]
+        src = s.pick_source(seed)
+        t0 = time.perf_counter()
+        rows = learned.record_windows(s, kinds=(2,), sources=train_src)
+        loss = learned.train_windows(s, rows, epochs=400)
+        acc = learned.window_accuracy(s, rows)
+        out["predictor"] = {"trace_rows": int(len(rows)), "train_loss": round(loss, 4),
+                            "shard_precision": round(acc["shard_precision"], 3),
+                            "shard_recall": round(acc["shard_recall"], 3),
+                            "seconds": round(time.perf_counter() - t0, 1)}
+        dist = {}
+        for name, pf in (("exact", 0), ("learned_prefetch", 1)):
+            s.set_option(gmp.OPT_WINDOW_PREFETCH, pf)
+            w0 = s.windows()
+            d, st = s.sssp(src)
+            w1 = s.windows()
+            dist[name] = d
+            out[name] = {"ms": round(st.ms_total, 3), "sweeps": st.iterations,
+                         "ms_per_sweep": round(st.ms_total / max(st.iterations, 1), 3),
+                         "streamed_gb": round((w1["streamed_bytes"] - w0["streamed_bytes"]) / 1e9, 3),
+                         "windows_run": w1["windows_run"] - w0["windows_run"],
+                         "windows_skipped": w1["windows_skipped"] - w0["windows_skipped"],
+                         "prefetched": w1["prefetched"] - w0["prefetched"],
+                         "prefetch_wasted_gb": round((w1["prefetch_wasted_bytes"] - w0["prefetch_wasted_bytes"]) / 1e9, 3)}
+        s.set_option(gmp.OPT_WINDOW_PREFETCH, 0)
+        out["identical_distances"] = bool(np.array_equal(dist["exact"], dist["learned_prefetch"]))
+        out["speedup"] = round(out["exact"]["ms"] / out["learned_prefetch"]["ms"], 3)
+    log(f"semi-external sssp: exact {out['exact']['ms']:.1f} ms ({out['exact']['sweeps']} sweeps, "
+        f"{out['exact']['streamed_gb']} GB), learned prefetch {out['learned_prefetch']['ms']:.1f} ms "
+        f"({out['learned_prefetch']['streamed_gb']} GB, {out['learned_prefetch']['prefetched']} prefetched, "
+        f"{out['learned_prefetch']['prefetch_wasted_gb']} GB wasted); identical: {out['identical_distances']}; "
+        f"predictor {out['predictor']}")
     return out
 
 

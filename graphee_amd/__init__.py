@@ -25,6 +25,7 @@ OPT_DELTA_PULL_LOG, OPT_LEARNED_FINISH = 15, 16
 NOALLOC_MAX_VERTICES = 1 << 22
 (OPT_BATCH_SSSP_UNROLL, OPT_BATCH_SSSP_GRID, OPT_BATCH_PPR_PIECE, OPT_BATCH_PPR_RUN_ROWS, OPT_BATCH_PPR_RUN_EDGES,
  OPT_BATCH_PPR_LOADS, OPT_BATCH_PPR_GRID) = range(17, 24)
+OPT_MLP_NET, OPT_WINDOW_PREFETCH = 24, 25
 
 # every entry point include/gmp.h declares
 SYMBOLS = (
@@ -33,7 +34,7 @@ SYMBOLS = (
     "gmp_pick_source", "gmp_run_app", "gmp_pagerank", "gmp_pagerank_noalloc", "gmp_sssp", "gmp_cc",
     "gmp_set_option", "gmp_get_option", "gmp_batch_generate", "gmp_batch_info", "gmp_batch_download",
     "gmp_batch_sssp", "gmp_batch_ppr", "gmp_batch_run", "gmp_batch_autotune", "gmp_autotune", "gmp_tune_save", "gmp_tune_load",
-    "gmp_sssp_trace", "gmp_sssp_actions", "gmp_mlp_init", "gmp_mlp_set", "gmp_mlp_get", "gmp_mlp_forward", "gmp_mlp_loss_grad",
+    "gmp_sssp_trace", "gmp_sssp_actions", "gmp_window_trace", "gmp_mlp_init", "gmp_mlp_set", "gmp_mlp_get", "gmp_mlp_forward", "gmp_mlp_loss_grad",
     "gmp_mlp_train", "gmp_windows",
     "gmp_partition_bounds", "gmp_npy_read_u32", "gmp_npy_write_i32", "gmp_write_shards", "gmp_convert_edge_list",
 )
@@ -92,7 +93,8 @@ class WindowStats(ctypes.Structure):
     _fields_ = [("windows", ctypes.c_uint32), ("cached", ctypes.c_uint32), ("ring_slots", ctypes.c_uint32),
                 ("packed", ctypes.c_uint32), ("host_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
                 ("streamed_bytes", ctypes.c_uint64), ("windows_run", ctypes.c_uint64),
-                ("windows_skipped", ctypes.c_uint64)]
+                ("windows_skipped", ctypes.c_uint64), ("prefetched", ctypes.c_uint64),
+                ("prefetch_wasted_bytes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -152,6 +154,7 @@ def lib():
         "gmp_sssp_trace": ([_P, u32, ctypes.POINTER(SsspParams), _P, u32, _P, _P, ctypes.POINTER(Stats)],
                            ctypes.c_int),
         "gmp_sssp_actions": ([_P, _P, u32], ctypes.c_int),
+        "gmp_window_trace": ([_P, i32, u32, _P, u32, _P, ctypes.POINTER(Stats)], ctypes.c_int),
         "gmp_mlp_init": ([_P, u64], ctypes.c_int),
         "gmp_mlp_set": ([_P, _P], ctypes.c_int),
         "gmp_mlp_get": ([_P, _P], ctypes.c_int),
@@ -459,6 +462,15 @@ class Engine:
         check(lib().gmp_sssp_trace(self._h, source, ctypes.byref(p), _ptr(rows), cap, ctypes.byref(n), _ptr(out),
                                    ctypes.byref(st)))
         return rows[: n.value].copy(), out, st
+
+    def window_trace(self, kind, source=0, cap=1 << 12):
+        """gmp_window_trace: min sweeps over a semi-external graph (kind 0 labels, 1 unit, 2
+        weighted distances) recording the window predictor's rows."""
+        rows = np.zeros(cap, dtype=TRACE_DTYPE)
+        n = ctypes.c_uint32()
+        st = Stats()
+        check(lib().gmp_window_trace(self._h, kind, source, _ptr(rows), cap, ctypes.byref(n), ctypes.byref(st)))
+        return rows[: n.value].copy(), st
 
     def sssp_actions(self):
         """Per-class counts of the bucket epochs the last learned SSSP run chose."""

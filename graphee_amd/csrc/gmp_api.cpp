@@ -188,6 +188,14 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       in(1, 64, "batch_ppr_grid");
       e->opt.ppr_grid = (uint32_t)value;
       break;
+    case GMP_OPT_MLP_NET:
+      in(0, 1, "mlp_net");
+      e->net_sel = (int)value;
+      break;
+    case GMP_OPT_WINDOW_PREFETCH:
+      in(0, 1, "window_prefetch");
+      e->opt.window_prefetch = (int)value;
+      break;
     default:
       fail(GMP_EINVAL, "gmp_set_option: unknown key " + std::to_string(key));
   }
@@ -222,6 +230,8 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_BATCH_PPR_RUN_EDGES: *value = e->opt.ppr_run_edges; break;
     case GMP_OPT_BATCH_PPR_LOADS: *value = e->opt.ppr_loads; break;
     case GMP_OPT_BATCH_PPR_GRID: *value = e->opt.ppr_grid; break;
+    case GMP_OPT_MLP_NET: *value = e->net_sel; break;
+    case GMP_OPT_WINDOW_PREFETCH: *value = e->opt.window_prefetch; break;
     default: fail(GMP_EINVAL, "gmp_get_option: unknown key " + std::to_string(key));
   }
   GMP_API_END
@@ -499,6 +509,17 @@ int gmp_batch_ppr(gmp_engine *h, uint32_t n, const uint32_t *graph, const uint32
   Engine *e = check(h);
   GMP_HIP(hipSetDevice(e->device));
   gmp::batch_ppr(*e, n, graph, seed, damping, iters, rank_out, st);
+  GMP_API_END
+}
+
+int gmp_window_trace(gmp_engine *h, int32_t kind, uint32_t source, gmp_trace_row *rows, uint32_t cap,
+                     uint32_t *nrows, gmp_stats *st) {
+  GMP_API_BEGIN
+  Engine *e = check_loaded(h);
+  if (!rows || !nrows) fail(GMP_EINVAL, "gmp_window_trace: null output");
+  if (!e->g.semi) fail(GMP_EUNSUPPORTED, "gmp_window_trace: the graph is resident, not semi-external");
+  GMP_HIP(hipSetDevice(e->device));
+  gmp::semi_trace(*e, kind, source, rows, cap, nrows, st);
   GMP_API_END
 }
 

@@ -189,6 +189,12 @@ struct SemiExternal {
   std::vector<hipEvent_t> ready, freed;
   uint64_t slot_seq = 0;
   uint64_t streamed_bytes = 0, windows_run = 0, windows_skipped = 0;  // since the load
+  // learned prefetch (GMP_OPT_WINDOW_PREFETCH): the window predictor's inputs and outputs
+  DevBuf prev;               // uint8[nwin]: the activity flags of the sweep before
+  DevBuf pred;               // uint8[nwin]: predicted activity of the sweep after next
+  DevBuf wcnt;               // uint32[16]: counts behind the predictor's features
+  DevBuf wfeat;              // float[16]: the last features
+  uint64_t prefetched = 0, prefetch_wasted_bytes = 0;  // windows copied ahead, bytes of those not run
   SemiExternal() = default;
   SemiExternal(const SemiExternal &) = delete;
   SemiExternal &operator=(const SemiExternal &) = delete;
@@ -261,6 +267,7 @@ struct Options {
   uint32_t ppr_run_edges = 2048; // PPR: edges per slot in a row run
   uint32_t ppr_loads = 8;        // PPR: source loads in flight per lane (4, 8, 16)
   uint32_t ppr_grid = 8;         // PPR gather: workgroups per CU
+  int window_prefetch = 0;       // semi-external sweeps: learned window prefetch and order
 };
 
 // Many graphs resident at once as one disjoint union (graph k owns union ids
@@ -297,7 +304,10 @@ struct Engine {
   hipStream_t stream = nullptr;
   Graph g;
   BatchGraphs batch;
-  Mlp mlp;
+  Mlp mlp;      // delta-stepping bucket schedule
+  Mlp mlp_win;  // semi-external window predictor
+  int net_sel = 0;  // network the gmp_mlp_* calls act on (GMP_OPT_MLP_NET)
+  Mlp &net() { return net_sel ? mlp_win : mlp; }
   Options opt;
   bool loaded = false;
   // scratch reused across runs
@@ -389,6 +399,21 @@ std::vector<uint32_t> window_order(const SemiExternal &s);
 WindowDev window_begin(Engine &e, uint32_t w, bool weights, bool plain);
 // Releases the ring slot of window_begin after the work enqueued so far on e.stream.
 void window_end(Engine &e, const WindowDev &d);
+// Learned prefetch: issues the copy of streamed window w into the next ring slot now and returns
+// the slot (-1 for a cached window); window_begin_slot then uses that slot in place of a copy, or
+// window_release gives it back unused (its bytes counted as wasted).
+int window_prefetch(Engine &e, uint32_t w, bool weights);
+WindowDev window_begin_slot(Engine &e, uint32_t w, int slot, bool weights, bool plain);
+void window_release(Engine &e, uint32_t w, int slot, bool weights);
+// window_activity plus the window predictor: pred (nwin) receives the windows it expects to be
+// active in the sweep after, feat (16) the features it read.
+const uint8_t *window_activity_predicted(Engine &e, uint32_t sweep, uint8_t *pred, float *feat);
+// Bit j: an active window overlaps shard j (rows [j V / 64, (j + 1) V / 64)).
+uint64_t window_shard_mask(const SemiExternal &s, const uint8_t *act, uint32_t V);
+void window_rows(Engine &e);  // s.wrow
+// Min sweeps (kind 0 labels, 1 unit, 2 weighted distances) recording the window predictor's trace.
+void semi_trace(Engine &e, int kind, uint32_t src, gmp_trace_row *rows, uint32_t cap, uint32_t *nrows,
+                gmp_stats *st);
 // Gather plans of the windows for the current plan options.
 void window_plans(Engine &e);
 // Activation: flags[w] = 1 when a source block of window w holds a vertex changed in `sweep`;

@@ -1,12 +1,16 @@
 // The scheduling MLP: 16 features -> 64 hidden (ReLU) -> 72 outputs = 8 bucket-width logits
 // (delta in {8, 16, ..., 1024}) + 64 shard-activity logits, FP32, trained and run on the GPU by
-// the kernels below.  It only chooses delta-stepping bucket widths (and scores shards), never
-// which edges are relaxed, so SSSP distances are identical with or without it.
+// the kernels below.  An engine holds two networks of this shape -- the delta-stepping schedule
+// (bucket widths; its shard outputs predict the degree bands an epoch touches) and the
+// semi-external window predictor (shard outputs: the 64ths of the row range whose windows the
+// sweep after next runs, for prefetch and order) -- and GMP_OPT_MLP_NET selects the one the
+// gmp_mlp_* calls act on.  Neither ever changes which edges are relaxed or gathered, so results
+// are identical with or without them.
 //
 // Parameters, 5768 floats: W1[16][64] (input-major), b1[64], W2[64][72], b2[72].
-// Loss per sample: cross-entropy of softmax(y[0:8]) against the best width class, plus the
-// mean binary cross-entropy of sigmoid(y[8:72]) against the 64-bit shard-activity mask;
-// averaged over the batch.  Every reduction runs in a fixed order (bit-reproducible).
+// Loss per sample: cross-entropy of softmax(y[0:8]) against the best width class (none for a
+// row labelled -1), plus the mean binary cross-entropy of sigmoid(y[8:72]) against the 64-bit
+// shard-activity mask; averaged over the batch.  Every reduction runs in a fixed order (bit-reproducible).
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -55,9 +59,10 @@ __global__ __launch_bounds__(kBlock) void mlp_output_grad_kernel(const float *Y,
     float z = 0;
     for (int c = 0; c < kMlpClasses; c++) z += expf(y[c] - m);
     const int lab = label[s];
-    float l = logf(z) + m - y[lab];
     const float inv_n = 1.0f / (float)n;
-    for (int c = 0; c < kMlpClasses; c++) d[c] = (expf(y[c] - m) / z - (c == lab ? 1.0f : 0.0f)) * inv_n;
+    // a row without a class (label < 0) trains the shard outputs only
+    float l = lab >= 0 ? logf(z) + m - y[lab] : 0.0f;
+    for (int c = 0; c < kMlpClasses; c++) d[c] = lab >= 0 ? (expf(y[c] - m) / z - (c == lab ? 1.0f : 0.0f)) * inv_n : 0.0f;
     const uint64_t msk = mask[s];
     float b = 0;
     for (int j = 0; j < kMlpShards; j++) {
@@ -165,9 +170,9 @@ struct MlpBatch {
   }
 };
 
-// Full-batch loss and gradient into e.mlp.grad; returns the mean loss.
+// Full-batch loss and gradient into the selected network's gradient; returns the mean loss.
 float loss_grad(Engine &e, MlpBatch &b, uint32_t n) {
-  Mlp &m = e.mlp;
+  Mlp &m = e.net();
   mlp_forward_kernel<<<grid_for(n), kBlock, 0, e.stream>>>(m.params.as<float>(), b.X.as<float>(), n, b.H.as<float>(),
                                                          b.Y.as<float>());
   mlp_output_grad_kernel<<<grid_for(n), kBlock, 0, e.stream>>>(b.Y.as<float>(), b.y.as<int32_t>(),
@@ -189,7 +194,7 @@ float loss_grad(Engine &e, MlpBatch &b, uint32_t n) {
 
 void upload_batch(Engine &e, MlpBatch &b, const float *X, const int32_t *y, const uint64_t *mask, uint32_t n) {
   for (uint32_t s = 0; s < n; s++)
-    if (y[s] < 0 || y[s] >= kMlpClasses) fail(GMP_EINVAL, "mlp: label out of range");
+    if (y[s] < -1 || y[s] >= kMlpClasses) fail(GMP_EINVAL, "mlp: label out of range");
   b.alloc(n);
   GMP_HIP(hipMemcpyAsync(b.X.p, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, e.stream));
   GMP_HIP(hipMemcpyAsync(b.y.p, y, (size_t)n * 4, hipMemcpyHostToDevice, e.stream));
@@ -199,7 +204,7 @@ void upload_batch(Engine &e, MlpBatch &b, const float *X, const int32_t *y, cons
 }  // namespace
 
 void mlp_ensure(Engine &e) {
-  Mlp &m = e.mlp;
+  Mlp &m = e.net();
   if (m.params.p) return;
   m.params.alloc(kMlpParams * 4);
   m.grad.alloc(kMlpParams * 4);
@@ -211,7 +216,7 @@ void mlp_ensure(Engine &e) {
 
 // He-style uniform initialisation from the SplitMix64 stream of `seed`.
 void mlp_init(Engine &e, uint64_t seed) {
-  Mlp &m = e.mlp;
+  Mlp &m = e.net();
   if (!m.params.p) {
     m.params.alloc(kMlpParams * 4);
     m.grad.alloc(kMlpParams * 4);
@@ -233,7 +238,7 @@ void mlp_init(Engine &e, uint64_t seed) {
 }
 
 void mlp_set(Engine &e, const float *params) {
-  Mlp &m = e.mlp;
+  Mlp &m = e.net();
   if (!m.params.p) mlp_ensure(e);
   GMP_HIP(hipMemcpy(m.params.p, params, kMlpParams * 4, hipMemcpyHostToDevice));
   GMP_HIP(hipMemset(m.adam_m.p, 0, kMlpParams * 4));
@@ -244,7 +249,7 @@ void mlp_set(Engine &e, const float *params) {
 
 void mlp_get(Engine &e, float *params) {
   mlp_ensure(e);
-  GMP_HIP(hipMemcpy(params, e.mlp.params.p, kMlpParams * 4, hipMemcpyDeviceToHost));
+  GMP_HIP(hipMemcpy(params, e.net().params.p, kMlpParams * 4, hipMemcpyDeviceToHost));
 }
 
 void mlp_forward(Engine &e, const float *X, uint32_t n, float *Y) {
@@ -254,7 +259,7 @@ void mlp_forward(Engine &e, const float *X, uint32_t n, float *Y) {
   dX.alloc((size_t)n * kMlpIn * 4);
   dY.alloc((size_t)n * kMlpOut * 4);
   GMP_HIP(hipMemcpyAsync(dX.p, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, e.stream));
-  mlp_forward_kernel<<<grid_for(n), kBlock, 0, e.stream>>>(e.mlp.params.as<float>(), dX.as<float>(), n, nullptr,
+  mlp_forward_kernel<<<grid_for(n), kBlock, 0, e.stream>>>(e.net().params.as<float>(), dX.as<float>(), n, nullptr,
                                                          dY.as<float>());
   GMP_CHECK_LAUNCH();
   GMP_HIP(hipMemcpyAsync(Y, dY.p, (size_t)n * kMlpOut * 4, hipMemcpyDeviceToHost, e.stream));
@@ -267,7 +272,7 @@ float mlp_loss_grad(Engine &e, const float *X, const int32_t *y, const uint64_t 
   MlpBatch b;
   upload_batch(e, b, X, y, mask, n);
   const float l = loss_grad(e, b, n);
-  if (grad) GMP_HIP(hipMemcpy(grad, e.mlp.grad.p, kMlpParams * 4, hipMemcpyDeviceToHost));
+  if (grad) GMP_HIP(hipMemcpy(grad, e.net().grad.p, kMlpParams * 4, hipMemcpyDeviceToHost));
   return l;
 }
 
@@ -277,7 +282,7 @@ float mlp_train(Engine &e, const float *X, const int32_t *y, const uint64_t *mas
   if (n == 0) fail(GMP_EINVAL, "mlp: empty batch");
   MlpBatch b;
   upload_batch(e, b, X, y, mask, n);
-  Mlp &m = e.mlp;
+  Mlp &m = e.net();
   float l = 0;
   const float b1 = 0.9f, b2 = 0.999f;
   for (int32_t ep = 0; ep < epochs; ep++) {

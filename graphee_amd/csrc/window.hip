@@ -148,6 +148,85 @@ __global__ __launch_bounds__(kBlock) void window_activity_kernel(const uint64_t 
   if (threadIdx.x == 0) active[blockIdx.x] = hit ? 1 : 0;
 }
 
+// Counts behind the window predictor's features, added into cnt[0..14] (cleared before): [0..7]
+// active windows in each eighth of the window list, [8] active windows, [9] windows active in the
+// sweep before, [10] source blocks changed in sweep `last`, [11..14] those per quarter of the blocks.
+__global__ __launch_bounds__(kBlock) void window_count_kernel(const uint8_t *active, const uint8_t *prev,
+                                                              uint32_t nwin, const uint32_t *stamp, uint32_t nblk,
+                                                              uint32_t last, uint32_t *cnt) {
+  __shared__ uint32_t sh[16];
+  if (threadIdx.x < 16) sh[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t w = i0; w < nwin; w += stride) {
+    if (active[w]) {
+      atomicAdd(&sh[(uint32_t)((uint64_t)w * 8 / nwin)], 1u);
+      atomicAdd(&sh[8], 1u);
+    }
+    if (prev[w]) atomicAdd(&sh[9], 1u);
+  }
+  for (uint32_t b = i0; b < nblk; b += stride)
+    if (stamp[b] == last) {
+      atomicAdd(&sh[10], 1u);
+      atomicAdd(&sh[11 + (uint32_t)((uint64_t)b * 4 / nblk)], 1u);
+    }
+  __syncthreads();
+  if (threadIdx.x < 15 && sh[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], sh[threadIdx.x]);
+}
+
+// The window predictor (one block of 64 threads): the 16 features of the sweep -- active fraction
+// of each eighth of the windows, overall and in the sweep before, changed fraction of the source
+// blocks, overall and per quarter, log of the sweep number -- through the network's hidden layer to
+// its 64 shard logits (shard j = rows [j V / 64, (j + 1) V / 64)); a window is predicted active in
+// the sweep after next when a shard it overlaps has a positive logit.  Writes the features too.
+__global__ __launch_bounds__(64) void window_predict_kernel(const float *P, const uint32_t *cnt, uint32_t nwin,
+                                                            uint32_t nblk, uint32_t sweep, const uint32_t *wrow,
+                                                            uint32_t V, uint8_t *pred, float *feat) {
+  __shared__ float f[kMlpIn], hsh[kMlpHid], lg[kMlpShards];
+  const uint32_t j = threadIdx.x;
+  if (j == 0) {
+    for (uint32_t i = 0; i < 8; i++) {
+      const uint32_t lo = (i * nwin + 7) / 8, hi = ((i + 1) * nwin + 7) / 8;
+      f[i] = hi > lo ? (float)cnt[i] / (float)(hi - lo) : 0.0f;
+    }
+    f[8] = (float)cnt[8] / (float)max(nwin, 1u);
+    f[9] = (float)cnt[9] / (float)max(nwin, 1u);
+    f[10] = (float)cnt[10] / (float)max(nblk, 1u);
+    for (uint32_t q = 0; q < 4; q++) {
+      const uint32_t lo = (uint32_t)(((uint64_t)q * nblk + 3) / 4), hi = (uint32_t)(((uint64_t)(q + 1) * nblk + 3) / 4);
+      f[11 + q] = hi > lo ? (float)cnt[11 + q] / (float)(hi - lo) : 0.0f;
+    }
+    f[15] = log2f(1.0f + (float)sweep) / 6.0f;
+  }
+  __syncthreads();
+  if (j < (uint32_t)kMlpIn) feat[j] = f[j];
+  if (!P) {  // no network yet (trace recording): features only
+    for (uint32_t w = j; w < nwin; w += 64) pred[w] = 0;
+    return;
+  }
+  const float *W1 = P, *b1 = P + kMlpIn * kMlpHid, *W2 = b1 + kMlpHid, *b2 = W2 + kMlpHid * kMlpOut;
+  float a = b1[j];
+  for (int i = 0; i < kMlpIn; i++) a = fmaf(f[i], W1[i * kMlpHid + j], a);
+  hsh[j] = a > 0.0f ? a : 0.0f;
+  __syncthreads();
+  float y = b2[kMlpClasses + j];
+  for (int k = 0; k < kMlpHid; k++) y = fmaf(hsh[k], W2[k * kMlpOut + kMlpClasses + j], y);
+  lg[j] = y;
+  __syncthreads();
+  for (uint32_t w = j; w < nwin; w += 64) {
+    const uint32_t r0 = wrow[w], r1 = wrow[w + 1];
+    const uint32_t s0 = (uint32_t)((uint64_t)r0 * kMlpShards / V);
+    const uint32_t s1 = r1 > r0 ? (uint32_t)((uint64_t)(r1 - 1) * kMlpShards / V) : s0;
+    bool on = false;
+    for (uint32_t s = s0; s <= s1 && s < (uint32_t)kMlpShards; s++) on = on || lg[s] > 0.0f;
+    pred[w] = on ? 1 : 0;
+  }
+}
+
+__global__ void copy_u8_kernel(const uint8_t *src, uint8_t *dst, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 template <class OffT>
 struct MinWin {
   const OffT *off;      // row offsets, shifted to the window's first row
@@ -233,15 +312,24 @@ void launch_min(Engine &e, const MinWin<OffT> &a, uint32_t grid) {
   GMP_CHECK_LAUNCH();
 }
 
-// Min-type sweeps over the windows until no window is active.
+// Min-type sweeps over the windows until no window is active.  With learned prefetch (a trained
+// window predictor and GMP_OPT_WINDOW_PREFETCH), the windows the predictor expects in the next
+// sweep have their copies issued as soon as this sweep is enqueued -- before the host waits for
+// the next sweep's exact activation -- and run first; a prefetched window the activation does
+// not want is dropped (its bytes counted as wasted).  With `rows` set, a trace row is recorded
+// per sweep: the predictor's features and the shard mask of the windows active one sweep later.
 template <class OffT>
-void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st) {
+void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st, gmp_trace_row *rows = nullptr,
+              uint32_t cap = 0, uint32_t *nrows = nullptr) {
   Graph &g = e.g;
   SemiExternal &s = semi_of(e);
   window_plans(e);
   const uint32_t V = g.V, nwin = (uint32_t)s.win.size();
   e.s_a.alloc((size_t)V * 4);
   uint32_t *val = e.s_a.as<uint32_t>();
+  const bool learn = e.opt.window_prefetch && e.mlp_win.trained && s.nslots > 0;
+  const bool predict = learn || rows;
+  if (nrows) *nrows = 0;
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
   if (mode == kMinLabel) {
@@ -269,39 +357,75 @@ void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st) {
   // values are updated in place, so a window may see this sweep's earlier windows; the minimum
   // the sweeps converge to does not depend on the order
   const std::vector<uint32_t> order = window_order(s);
+  std::vector<uint8_t> pred(nwin, 0);
+  std::vector<int> pf_slot(nwin, -1);  // ring slot of a window copied ahead, -1 otherwise
+  std::vector<uint32_t> pf;             // those windows, in the order their copies were issued
+  float feat[kMlpIn] = {}, prev_feat[kMlpIn] = {};
+  bool have_prev = false;
   int sweeps = 0;
   uint64_t edges = 0;
+  auto run_window = [&](uint32_t w, const WindowDev &d) {
+    const EdgeWindow &x = s.win[w];
+    a.off = off + x.r0;
+    a.idx = d.idx;
+    a.w = d.w;
+    a.r0 = x.r0;
+    a.tasks = tasks + 2ull * x.task0;
+    a.ntasks = x.ntasks;
+    a.hubs = hubs + x.hub0;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(G, (x.ntasks + kWavesPerBlock - 1) / kWavesPerBlock));
+    if (x.ntasks) {
+      if (mode == kMinLabel) launch_min<kMinLabel>(e, a, grid);
+      else if (mode == kMinUnit) launch_min<kMinUnit>(e, a, grid);
+      else launch_min<kMinWeighted>(e, a, grid);
+    }
+    window_end(e, d);
+    edges += x.e1 - x.e0;
+  };
   for (uint32_t sweep = 1;; sweep++) {
-    const uint8_t *act = window_activity(e, sweep - 1);
+    const uint8_t *act = predict ? window_activity_predicted(e, sweep - 1, pred.data(), feat)
+                                 : window_activity(e, sweep - 1);
     uint32_t nact = 0;
     for (uint32_t w = 0; w < nwin; w++) nact += act[w];
-    if (nact == 0) break;
+    if (rows && have_prev && *nrows < cap) {  // the features of the sweep before and what followed
+      gmp_trace_row r{};
+      std::memcpy(r.feat, prev_feat, sizeof r.feat);
+      r.best = -1;
+      r.epoch = sweep;
+      r.shard_mask = window_shard_mask(s, act, V);
+      rows[(*nrows)++] = r;
+    }
+    std::memcpy(prev_feat, feat, sizeof feat);
+    have_prev = true;
     a.sweep = sweep;
+    // windows copied ahead first: run the active ones, drop the others
+    for (uint32_t w : pf) {
+      if (act[w] && nact) run_window(w, window_begin_slot(e, w, pf_slot[w], weights, true));
+      else window_release(e, w, pf_slot[w], weights);
+    }
+    if (nact == 0) break;
     for (uint32_t wi = 0; wi < nwin; wi++) {
       const uint32_t w = order[wi];
-      const EdgeWindow &x = s.win[w];
+      if (pf_slot[w] >= 0) continue;
       if (!act[w]) {
         s.windows_skipped++;
         continue;
       }
-      const WindowDev d = window_begin(e, w, weights, true);
-      a.off = off + x.r0;
-      a.idx = d.idx;
-      a.w = d.w;
-      a.r0 = x.r0;
-      a.tasks = tasks + 2ull * x.task0;
-      a.ntasks = x.ntasks;
-      a.hubs = hubs + x.hub0;
-      const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(G, (x.ntasks + kWavesPerBlock - 1) / kWavesPerBlock));
-      if (x.ntasks) {
-        if (mode == kMinLabel) launch_min<kMinLabel>(e, a, grid);
-        else if (mode == kMinUnit) launch_min<kMinUnit>(e, a, grid);
-        else launch_min<kMinWeighted>(e, a, grid);
-      }
-      window_end(e, d);
-      edges += x.e1 - x.e0;
+      run_window(w, window_begin(e, w, weights, true));
     }
+    for (uint32_t w : pf) pf_slot[w] = -1;
+    pf.clear();
     sweeps++;
+    if (learn) {
+      // the windows the predictor expects after this sweep: their copies start now, into the ring
+      // slots this sweep's last windows release
+      for (uint32_t wi = 0; wi < nwin && pf.size() < s.nslots; wi++) {
+        const uint32_t w = order[wi];
+        if (!pred[w] || s.win[w].cache >= 0) continue;
+        pf_slot[w] = window_prefetch(e, w, weights);
+        pf.push_back(w);
+      }
+    }
   }
   GMP_HIP(hipEventRecord(tot.b, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
@@ -314,7 +438,7 @@ void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st) {
     st->ms_kernel = st->ms_total;
     st->edges = edges;
     st->bytes = edges * 8;
-    st->policy = 0;
+    st->policy = learn ? 1 : 0;
   }
 }
 
@@ -711,33 +835,42 @@ std::vector<uint32_t> window_order(const SemiExternal &s) {
   return order;
 }
 
-WindowDev window_begin(Engine &e, uint32_t wi, bool weights, bool plain) {
+namespace {
+
+// Copies window x's ids (and weights) into the next ring slot on that slot's copy stream, after
+// the slot's previous window is done with it; returns the slot.
+uint32_t ring_copy(SemiExternal &s, const EdgeWindow &x, bool need_w) {
+  const uint32_t k = (uint32_t)(s.slot_seq++ % s.nslots);
+  uint8_t *dst = s.ring.as<uint8_t>() + (uint64_t)k * s.slot_bytes;
+  hipStream_t cs = s.copy[k % s.copy.size()];
+  GMP_HIP(hipStreamWaitEvent(cs, s.freed[k], 0));
+  if (x.b1 > x.b0) GMP_HIP(hipMemcpyAsync(dst, s.h_ids + x.b0, x.b1 - x.b0, hipMemcpyHostToDevice, cs));
+  s.streamed_bytes += x.b1 - x.b0;
+  if (need_w) {
+    uint32_t *wd = s.wring.as<uint32_t>() + (uint64_t)k * s.max_edges;
+    if (x.e1 > x.e0) GMP_HIP(hipMemcpyAsync(wd, s.h_w + x.e0, (x.e1 - x.e0) * 4, hipMemcpyHostToDevice, cs));
+    s.streamed_bytes += (x.e1 - x.e0) * 4;
+  }
+  GMP_HIP(hipEventRecord(s.ready[k], cs));
+  return k;
+}
+
+// Device pointers of window wi for work enqueued next on e.stream: from the cache (slot < 0) or
+// from ring slot `slot`, whose copy the stream waits for.
+WindowDev window_use(Engine &e, uint32_t wi, int slot, bool need_w, bool plain) {
   SemiExternal &s = semi_of(e);
   const EdgeWindow &x = s.win[wi];
-  const bool need_w = weights && s.h_w;
   WindowDev d;
   const uint8_t *ids;
   const uint32_t *wts = nullptr;
-  if (x.cache >= 0) {
+  if (slot < 0) {
     ids = s.cache.as<uint8_t>() + x.cache;
     if (need_w) wts = s.wcache.as<uint32_t>() + (x.e0 - s.wcache_e0);
   } else {
-    const uint32_t k = (uint32_t)(s.slot_seq++ % s.nslots);
-    uint8_t *dst = s.ring.as<uint8_t>() + (uint64_t)k * s.slot_bytes;
-    hipStream_t cs = s.copy[k % s.copy.size()];
-    GMP_HIP(hipStreamWaitEvent(cs, s.freed[k], 0));
-    if (x.b1 > x.b0) GMP_HIP(hipMemcpyAsync(dst, s.h_ids + x.b0, x.b1 - x.b0, hipMemcpyHostToDevice, cs));
-    s.streamed_bytes += x.b1 - x.b0;
-    if (need_w) {
-      uint32_t *wd = s.wring.as<uint32_t>() + (uint64_t)k * s.max_edges;
-      if (x.e1 > x.e0) GMP_HIP(hipMemcpyAsync(wd, s.h_w + x.e0, (x.e1 - x.e0) * 4, hipMemcpyHostToDevice, cs));
-      s.streamed_bytes += (x.e1 - x.e0) * 4;
-      wts = wd;
-    }
-    GMP_HIP(hipEventRecord(s.ready[k], cs));
-    GMP_HIP(hipStreamWaitEvent(e.stream, s.ready[k], 0));
-    ids = dst;
-    d.slot = (int)k;
+    GMP_HIP(hipStreamWaitEvent(e.stream, s.ready[slot], 0));
+    ids = s.ring.as<uint8_t>() + (uint64_t)slot * s.slot_bytes;
+    if (need_w) wts = s.wring.as<uint32_t>() + (uint64_t)slot * s.max_edges;
+    d.slot = slot;
   }
   if (s.packed) {
     d.pk = Packed{shift_back<uint8_t>(ids, x.b0), s.gbyte.as<uint64_t>(), s.gmask.as<uint64_t>()};
@@ -760,6 +893,38 @@ WindowDev window_begin(Engine &e, uint32_t wi, bool weights, bool plain) {
   if (wts) d.w = shift_back<uint32_t>(wts, 4 * x.e0);
   s.windows_run++;
   return d;
+}
+
+}  // namespace
+
+WindowDev window_begin(Engine &e, uint32_t wi, bool weights, bool plain) {
+  SemiExternal &s = semi_of(e);
+  const EdgeWindow &x = s.win[wi];
+  const bool need_w = weights && s.h_w;
+  const int slot = x.cache >= 0 ? -1 : (int)ring_copy(s, x, need_w);
+  return window_use(e, wi, slot, need_w, plain);
+}
+
+int window_prefetch(Engine &e, uint32_t wi, bool weights) {
+  SemiExternal &s = semi_of(e);
+  const EdgeWindow &x = s.win[wi];
+  if (x.cache >= 0) return -1;
+  s.prefetched++;
+  return (int)ring_copy(s, x, weights && s.h_w);
+}
+
+WindowDev window_begin_slot(Engine &e, uint32_t wi, int slot, bool weights, bool plain) {
+  SemiExternal &s = semi_of(e);
+  return window_use(e, wi, slot, weights && s.h_w, plain);
+}
+
+void window_release(Engine &e, uint32_t wi, int slot, bool weights) {
+  SemiExternal &s = semi_of(e);
+  if (slot < 0) return;
+  const EdgeWindow &x = s.win[wi];
+  s.prefetch_wasted_bytes += (x.b1 - x.b0) + (weights && s.h_w ? (x.e1 - x.e0) * 4 : 0);
+  s.windows_skipped++;
+  GMP_HIP(hipEventRecord(s.freed[slot], e.stream));
 }
 
 void window_end(Engine &e, const WindowDev &d) {
@@ -791,6 +956,61 @@ const uint8_t *window_activity(Engine &e, uint32_t sweep) {
   return flags.data();
 }
 
+void window_rows(Engine &e) {
+  SemiExternal &s = semi_of(e);
+  if (s.wrow.p) return;
+  const uint32_t nwin = (uint32_t)s.win.size();
+  std::vector<uint32_t> wrow(nwin + 1);
+  for (uint32_t w = 0; w < nwin; w++) wrow[w] = s.win[w].r0;
+  wrow[nwin] = e.g.V;
+  s.wrow.alloc((size_t)(nwin + 1) * 4);
+  GMP_HIP(hipMemcpy(s.wrow.p, wrow.data(), (size_t)(nwin + 1) * 4, hipMemcpyHostToDevice));
+}
+
+const uint8_t *window_activity_predicted(Engine &e, uint32_t sweep, uint8_t *pred, float *feat) {
+  SemiExternal &s = semi_of(e);
+  const uint32_t nwin = (uint32_t)s.win.size();
+  static thread_local std::vector<uint8_t> flags;
+  flags.assign(nwin, 0);
+  if (!nwin) return flags.data();
+  window_rows(e);
+  if (!s.prev.p) {
+    s.prev.alloc(nwin);
+    s.pred.alloc(nwin);
+    s.wcnt.alloc(16 * 4);
+    s.wfeat.alloc(16 * 4);
+    GMP_HIP(hipMemsetAsync(s.prev.p, 0, nwin, e.stream));
+  }
+  if (sweep == 0) GMP_HIP(hipMemsetAsync(s.prev.p, 0, nwin, e.stream));  // no sweep before the first
+  window_activity_kernel<<<nwin, kBlock, 0, e.stream>>>(s.srcblk.as<uint64_t>(), s.blk_words,
+                                                        s.stamp.as<uint32_t>(), sweep, s.active.as<uint8_t>());
+  fill_kernel<<<1, 64, 0, e.stream>>>(s.wcnt.as<uint32_t>(), 16, 0u);
+  const uint32_t gc = grid_for(std::max<uint32_t>(s.nblk, nwin));
+  window_count_kernel<<<std::min<uint32_t>(gc, 1024), kBlock, 0, e.stream>>>(
+      s.active.as<uint8_t>(), s.prev.as<uint8_t>(), nwin, s.stamp.as<uint32_t>(), s.nblk, sweep, s.wcnt.as<uint32_t>());
+  window_predict_kernel<<<1, 64, 0, e.stream>>>(e.mlp_win.params.as<float>(), s.wcnt.as<uint32_t>(), nwin, s.nblk,
+                                                sweep + 1, s.wrow.as<uint32_t>(), e.g.V, s.pred.as<uint8_t>(),
+                                                s.wfeat.as<float>());
+  copy_u8_kernel<<<grid_for(nwin), kBlock, 0, e.stream>>>(s.active.as<uint8_t>(), s.prev.as<uint8_t>(), nwin);
+  GMP_CHECK_LAUNCH();
+  GMP_HIP(hipMemcpyAsync(flags.data(), s.active.p, nwin, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipMemcpyAsync(pred, s.pred.p, nwin, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipMemcpyAsync(feat, s.wfeat.p, 16 * 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  return flags.data();
+}
+
+uint64_t window_shard_mask(const SemiExternal &s, const uint8_t *act, uint32_t V) {
+  uint64_t m = 0;
+  for (size_t w = 0; w < s.win.size(); w++) {
+    if (!act[w] || s.win[w].r1 <= s.win[w].r0) continue;
+    const uint32_t s0 = (uint32_t)((uint64_t)s.win[w].r0 * kMlpShards / V);
+    const uint32_t s1 = (uint32_t)((uint64_t)(s.win[w].r1 - 1) * kMlpShards / V);
+    for (uint32_t j = s0; j <= s1 && j < (uint32_t)kMlpShards; j++) m |= 1ull << j;
+  }
+  return m;
+}
+
 void semi_cc(Engine &e, uint32_t *label_out, gmp_stats *st) {
   if (e.g.symmetric && e.opt.cc_union_find) {
     if (e.g.off64) semi_cc_union_find<uint64_t>(e, label_out, st);
@@ -799,6 +1019,16 @@ void semi_cc(Engine &e, uint32_t *label_out, gmp_stats *st) {
   }
   if (e.g.off64) semi_min<uint64_t>(e, kMinLabel, 0, label_out, st);
   else semi_min<uint32_t>(e, kMinLabel, 0, label_out, st);
+}
+
+void semi_trace(Engine &e, int kind, uint32_t src, gmp_trace_row *rows, uint32_t cap, uint32_t *nrows,
+                gmp_stats *st) {
+  if (kind < 0 || kind > 2) fail(GMP_EINVAL, "gmp_window_trace: kind must be 0 (labels), 1 or 2 (distances)");
+  if (kind && src >= e.g.V) fail(GMP_EINVAL, "source vertex out of range");
+  if (kind == 2 && !e.g.weighted) fail(GMP_EINVAL, "gmp_window_trace: the graph has no weights");
+  const int mode = kind == 0 ? kMinLabel : kind == 1 ? kMinUnit : kMinWeighted;
+  if (e.g.off64) semi_min<uint64_t>(e, mode, src, nullptr, st, rows, cap, nrows);
+  else semi_min<uint32_t>(e, mode, src, nullptr, st, rows, cap, nrows);
 }
 
 void semi_sssp(Engine &e, uint32_t src, bool unit, uint32_t *dist_out, gmp_stats *st) {
@@ -843,6 +1073,8 @@ void semi_info(Engine &e, gmp_window_stats *out) {
   out->streamed_bytes = s.streamed_bytes;
   out->windows_run = s.windows_run;
   out->windows_skipped = s.windows_skipped;
+  out->prefetched = s.prefetched;
+  out->prefetch_wasted_bytes = s.prefetch_wasted_bytes;
 }
 
 }  // namespace gmp

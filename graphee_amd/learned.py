@@ -12,7 +12,7 @@ every other method.
 """
 import numpy as np
 
-from . import OPT_LEARNED_FINISH, SSSP_BF, SSSP_DELTA, SSSP_LEARNED, Engine  # noqa: F401
+from . import OPT_LEARNED_FINISH, OPT_MLP_NET, SSSP_BF, SSSP_DELTA, SSSP_LEARNED, Engine  # noqa: F401
 
 
 def record(engine, graphs, sources_per_graph=2, seed=0, finish=False):
@@ -85,3 +85,51 @@ def accuracy(engine, rows):
     prec = tp / max(1.0, float(pred.sum()))
     rec = tp / max(1.0, float(bits.sum()))
     return {"class_accuracy": acc, "shard_precision": prec, "shard_recall": rec}
+
+
+# ---- the semi-external window predictor (the engine's second network) ----------------------
+#
+# On a semi-external graph, min sweeps (CC labels, SSSP distances) run only the windows whose
+# sources changed in the sweep before; the host learns which ones only after that sweep ends.
+# The window predictor sees 16 features of a sweep's exact activation (active fraction per eighth
+# of the windows, changed fraction of the source blocks, ...) and scores the 64 shards of the row
+# range for the sweep after; with GMP_OPT_WINDOW_PREFETCH the windows of its shards are copied
+# ahead and run first.  Its rows carry no class (label -1): only the shard outputs train.
+
+
+def record_windows(engine, kinds=(0, 2), sources=(0,)):
+    """Trace rows of the window predictor on the engine's semi-external graph: for each kind (0
+    labels, 1 unit, 2 weighted distances) and source."""
+    rows = []
+    for kind in kinds:
+        for s in (sources if kind else (0,)):
+            r, _ = engine.window_trace(kind, int(s))
+            rows.append(r)
+    return np.concatenate(rows) if rows else np.zeros(0)
+
+
+def train_windows(engine, rows, epochs=400, lr=3e-3, init_seed=11):
+    """Fits the window predictor (network 1) on window trace rows; network 0 stays selected."""
+    X = np.ascontiguousarray(rows["feat"], dtype=np.float32)
+    y = np.full(len(rows), -1, dtype=np.int32)
+    m = np.ascontiguousarray(rows["shard_mask"], dtype=np.uint64)
+    engine.set_option(OPT_MLP_NET, 1)
+    try:
+        engine.mlp_init(init_seed)
+        return engine.mlp_train(X, y, m, epochs=epochs, lr=lr)
+    finally:
+        engine.set_option(OPT_MLP_NET, 0)
+
+
+def window_accuracy(engine, rows):
+    """Shard precision and recall of the window predictor on window trace rows."""
+    engine.set_option(OPT_MLP_NET, 1)
+    try:
+        Y = engine.mlp_forward(np.ascontiguousarray(rows["feat"], dtype=np.float32))
+    finally:
+        engine.set_option(OPT_MLP_NET, 0)
+    pred = Y[:, 8:] > 0
+    m = np.asarray(rows["shard_mask"], dtype=np.uint64)
+    bits = ((m[:, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)).astype(bool)
+    tp = float((pred & bits).sum())
+    return {"shard_precision": tp / max(1.0, float(pred.sum())), "shard_recall": tp / max(1.0, float(bits.sum()))}

@@ -158,6 +158,13 @@ int gmp_destroy(gmp_engine *h);
 #define GMP_OPT_BATCH_PPR_RUN_EDGES 21 /* batched PPR: edges per wave slot in a row run, 64..65536 (2048) */
 #define GMP_OPT_BATCH_PPR_LOADS 22    /* batched PPR: source loads in flight per lane, 4, 8 (default) or 16 */
 #define GMP_OPT_BATCH_PPR_GRID 23     /* batched PPR: gather workgroups per CU, 1..64 (default 8) */
+#define GMP_OPT_MLP_NET 24  /* the network the gmp_mlp_* calls act on: 0 (default) the delta-stepping
+                              schedule of GMP_SSSP_LEARNED, 1 the semi-external window predictor of
+                              GMP_OPT_WINDOW_PREFETCH */
+#define GMP_OPT_WINDOW_PREFETCH 25 /* semi-external min sweeps (gmp_sssp, gmp_cc): 1 = the trained
+                              window predictor's shard outputs pick the windows whose copies start
+                              before the next sweep's activation is known, and those windows run
+                              first; 0 (default) = exact activation only.  Results are the same. */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 
@@ -220,6 +227,8 @@ typedef struct gmp_window_stats {
   uint64_t streamed_bytes; /* host-to-device bytes copied for windows */
   uint64_t windows_run;
   uint64_t windows_skipped;
+  uint64_t prefetched;            /* windows copied ahead by the learned prefetch */
+  uint64_t prefetch_wasted_bytes; /* bytes of prefetched windows that then did not run */
 } gmp_window_stats;
 int gmp_windows(gmp_engine *h, gmp_window_stats *out);
 int gmp_graph_download(gmp_engine *h, uint64_t *row_off, uint32_t *col_idx, uint32_t *weights,
@@ -290,6 +299,13 @@ int gmp_sssp_trace(gmp_engine *h, uint32_t source, const gmp_sssp_params *p, gmp
 /* Per-class counts of the bucket epochs the last GMP_SSSP_LEARNED run of this engine chose
  * (counts[c], c < n <= GMP_MLP_CLASSES). */
 int gmp_sssp_actions(gmp_engine *h, uint32_t *counts, uint32_t n);
+/* The window predictor's trace (no reference counterpart), on a semi-external graph: min sweeps
+ * over the windows -- kind 0 labels (CC), 1 unit-weight or 2 weighted distances from source --
+ * recording one row per sweep: the predictor's 16 features of that sweep (best = -1: no class)
+ * and, as shard_mask, the 64ths of the row range holding a window active one sweep later (the
+ * windows the prefetch should have copied).  Train with GMP_OPT_MLP_NET = 1. */
+int gmp_window_trace(gmp_engine *h, int32_t kind, uint32_t source, gmp_trace_row *rows, uint32_t cap,
+                     uint32_t *nrows, gmp_stats *st);
 int gmp_mlp_init(gmp_engine *h, uint64_t seed);
 int gmp_mlp_set(gmp_engine *h, const float *params);
 int gmp_mlp_get(gmp_engine *h, float *params);

@@ -542,8 +542,8 @@ int orc_pr_f64_refine(int64_t V, const int64_t *off, const uint32_t *idx, const 
 /* ------------------------------------------------------------------------------------ */
 /* FP64 reference of the scheduling MLP (the engine trains it in FP32 on the GPU):
  * h = relu(x W1 + b1), y = h W2 + b2, 16 -> 64 -> 72.  Loss per sample = cross-entropy of
- * softmax(y[0:8]) against label + mean over 64 shards of the logistic loss of y[8+j] against
- * mask bit j; averaged over n.  Parameters: W1[16][64], b1[64], W2[64][72], b2[72]. */
+ * softmax(y[0:8]) against label (no class term when label < 0) + mean over 64 shards of the
+ * logistic loss of y[8+j] against mask bit j; averaged over n.  Parameters: W1[16][64], b1[64], W2[64][72], b2[72]. */
 #define ORC_IN 16
 #define ORC_HID 64
 #define ORC_CLS 8
@@ -574,8 +574,10 @@ int orc_mlp_loss_grad(const double *P, const float *X, const int32_t *label, con
     double m = y[0], z = 0;
     for (int c = 1; c < ORC_CLS; c++) m = y[c] > m ? y[c] : m;
     for (int c = 0; c < ORC_CLS; c++) z += exp(y[c] - m);
-    L += log(z) + m - y[label[s]];
-    for (int c = 0; c < ORC_CLS; c++) dy[c] = (exp(y[c] - m) / z - (c == label[s])) / (double)n;
+    /* label < 0: a row without a class trains the shard outputs only */
+    if (label[s] >= 0) L += log(z) + m - y[label[s]];
+    for (int c = 0; c < ORC_CLS; c++)
+      dy[c] = label[s] >= 0 ? (exp(y[c] - m) / z - (c == label[s])) / (double)n : 0.0;
     double b = 0;
     for (int j = 0; j < ORC_SH; j++) {
       const double t = (double)((mask[s] >> j) & 1u), v = y[ORC_CLS + j];

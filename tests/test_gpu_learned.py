@@ -27,6 +27,7 @@ def test_mlp_forward_and_gradient_match_fp64(engine):
     engine.mlp_set(P)
     np.testing.assert_array_equal(engine.mlp_get(), P)
     X, y, m = _batch(300, 5)
+    y[::7] = -1  # rows without a class train the shard outputs only
     loss, g = engine.mlp_loss_grad(X, y, m)
     want_loss, want_g = oracle.mlp_loss_grad(P.astype(np.float64), X, y, m)
     assert abs(loss - want_loss) < 1e-5 * max(1.0, abs(want_loss))

@@ -249,3 +249,50 @@ def test_windows_load_paths_and_unsupported_runs(tmp_path):
     with _engine(64, 1, True) as s:
         s.load_csr(off, idx)
         assert s.windows()["windows"] == 0
+
+
+@pytest.mark.parametrize("packed", [False, True])
+def test_windows_learned_prefetch_keeps_results(packed):
+    """The window predictor (the engine's second network), trained on traces of this graph's
+    sweeps, picks windows whose copies start before the next sweep's activation is known, and
+    they run first: labels and distances stay the resident engine's and Dijkstra's, prefetched
+    windows are counted (and those the activation then skips, as wasted bytes), and the trace's
+    shard masks cover the windows that ran."""
+    from graphee_amd import learned
+    V = 1 << 22
+    off, idx = _local_graph(V, 32)
+    w = graphs.edge_weights(off, idx, 9)
+    src = V // 3 + 1
+    with _engine() as r, _engine(1, 1, packed) as s:
+        for e in (r, s):
+            e.load_csr(off, idx, weights=w)
+        nwin = s.windows()["windows"]
+        assert nwin >= 8
+        say(f"window trace: {nwin} windows")
+        rows = learned.record_windows(s, kinds=(0, 2), sources=(src, 7))
+        assert len(rows) >= 4 and np.all(rows["best"] == -1)
+        assert rows["shard_mask"].any()
+        loss = learned.train_windows(s, rows, epochs=300)
+        acc = learned.window_accuracy(s, rows)
+        say(f"window predictor: {len(rows)} rows, loss {loss:.3f}, {acc}")
+        a_lab, _ = r.cc()
+        a_dist, _ = r.sssp(src)
+        ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+        want = oracle.dijkstra(ooff, oidx, ow, src)
+        np.testing.assert_array_equal(a_dist, want)
+        for prefetch in (0, 1):
+            s.set_option(gmp.OPT_WINDOW_PREFETCH, prefetch)
+            w0 = s.windows()
+            b_lab, st1 = s.cc()
+            b_dist, st2 = s.sssp(src)
+            w1 = s.windows()
+            np.testing.assert_array_equal(a_lab, b_lab)
+            np.testing.assert_array_equal(want, b_dist)
+            say(f"prefetch={prefetch}: prefetched {w1['prefetched'] - w0['prefetched']}, wasted "
+                f"{w1['prefetch_wasted_bytes'] - w0['prefetch_wasted_bytes']} B, streamed "
+                f"{w1['streamed_bytes'] - w0['streamed_bytes']} B, {st1.ms_total:.2f} + {st2.ms_total:.2f} ms")
+            if prefetch:
+                assert w1["prefetched"] > w0["prefetched"] and st2.policy == 1
+            else:
+                assert w1["prefetched"] == w0["prefetched"] and st2.policy == 0
+        s.set_option(gmp.OPT_WINDOW_PREFETCH, 0)

@@ -339,3 +339,26 @@ def test_pr_f64_error_matches_dense_solve():
     # two steps only: the reference is still far off, and its bound says so
     e2 = oracle.pr_f64_error(off, idx, od, x32, d=d, threads=2, max_iter=2, bound=1e-13)
     assert e2["reference_steps"] == 2 and e2["reference_l1_bound"] > 1e-9
+
+
+def test_mlp_rows_without_class_train_shards_only():
+    """A row labelled -1 carries no class term: at zero parameters its loss is the shard term
+    alone (log 2), and the gradient of a mixed batch still matches central differences."""
+    P = np.zeros(oracle.MLP_PARAMS)
+    X, y, m = _mlp_batch(6, 11)
+    y = y.copy()
+    y[:] = -1
+    loss, g = oracle.mlp_loss_grad(P, X, y, m)
+    assert abs(loss - np.log(2.0)) < 1e-12
+    assert np.all(g[16 * 64 + 64: 16 * 64 + 64 + 64 * 72].reshape(64, 72)[:, :8] == 0)
+    rng = np.random.default_rng(12)
+    P = rng.normal(0, 0.3, oracle.MLP_PARAMS)
+    y[::2] = np.arange(3)
+    _, g = oracle.mlp_loss_grad(P, X, y, m)
+    for k in rng.choice(oracle.MLP_PARAMS, 40, replace=False):
+        h = 1e-6
+        Pp, Pm = P.copy(), P.copy()
+        Pp[k] += h
+        Pm[k] -= h
+        fd = (oracle.mlp_loss_grad(Pp, X, y, m, want_grad=False)[0] - oracle.mlp_loss_grad(Pm, X, y, m, want_grad=False)[0]) / (2 * h)
+        assert abs(fd - g[k]) < 1e-6 * max(1.0, abs(fd)), k
