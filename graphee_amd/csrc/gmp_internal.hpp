@@ -108,7 +108,8 @@ struct StripCsr {
   uint32_t wlog = 0, rlog = 0;  // strips of 2^wlog sources, bins of 2^rlog destinations
   uint32_t ns = 0, nbins = 0;   // strips up to the last one holding an edge; destination bins
   uint64_t nseg = 0;            // segments = messages per iteration
-  DevBuf code;                  // uint16[E+pad]: source offset in its strip, bit 15 ends a segment
+  DevBuf code;                  // uint16[E+pad]: source offset in its strip
+  DevBuf endb;                  // uint8[(E+pad)/8]: bit k of byte i ends the segment of edge 8 i + k
   DevBuf wtask;                 // uint32x4[nwt]: {first edge, end edge, first segment, strip}
   DevBuf gtask;                 // uint32x4[ngt]: {strip, first wave task, end wave task, 0}
   DevBuf seg_row;               // uint16[nseg]: destination offset inside its bin

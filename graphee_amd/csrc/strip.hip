@@ -20,8 +20,9 @@
 //                           (their slices leave fixed-point partials).
 //
 // Segments end where the strip or destination changes and at every wave-task boundary, so
-// each wave task starts with a fresh segment.  Bit 15 of an edge's code marks a segment's
-// last edge, hence strips of at most 2^15 sources.
+// each wave task starts with a fresh segment.  While the layout is built, bit 15 of an edge's
+// code marks a segment's last edge (hence strips of at most 2^15 sources); the finished layout
+// keeps those bits in a byte per 8 edges and the codes as bare offsets.
 //
 // Build (once per graph and layout options): a 64-bit key per edge, strip << (b + wlog) |
 // dst << wlog | source offset (b = id bits), an LSD radix sort, segment ends, an exclusive
@@ -85,7 +86,8 @@ struct StripArgs {
   float d;
   uint32_t seed;
   // layout
-  const uint16_t *code;
+  const uint16_t *code;   // source offsets in the strip
+  const uint8_t *endb;    // bit k of byte i: edge 8 i + k ends its segment
   const uint4 *wtask;
   const uint4 *gtask;
   uint32_t ngt;
@@ -167,6 +169,21 @@ __global__ void task_seg_kernel(uint4 *wt, uint32_t n, const uint32_t *segid) {
   GS64(t, n) wt[t].z = segid[wt[t].x];
 }
 
+// Moves the end bits out of the codes: endb[i] bit k = bit 15 of code[8 i + k], which is then
+// cleared, so a code is the bare source offset (n = groups of 8 codes, padding included).
+__global__ void split_ends_kernel(uint16_t *code, uint64_t n, uint8_t *endb) {
+  GS64(i, n) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint16_t c = code[8 * i + k];
+      m |= (uint32_t)(c >> 15) << k;
+      code[8 * i + k] = (uint16_t)(c & 0x7FFFu);
+    }
+    endb[i] = (uint8_t)m;
+  }
+}
+
 // at each segment's last edge: the destination inside its bin, and the full destination
 __global__ void seg_rows_kernel(const uint64_t *keys, const uint16_t *code, const uint32_t *segid, uint64_t E,
                                 uint32_t wlog, uint32_t b, uint32_t rmask, uint16_t *seg_row, uint32_t *seg_full) {
@@ -226,6 +243,12 @@ __device__ inline u32x4 load_round(const StripArgs &a, uint32_t b0, uint32_t r, 
   return cv;
 }
 
+// End bits of the lane's 8 edges of round r (zero outside [e0, e1)).
+__device__ inline uint32_t load_round_ends(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1) {
+  const uint32_t q = b0 + r * kRound + 8 * lane_id();
+  return q < e1 && q + 8 > e0 ? (uint32_t)__builtin_nontemporal_load(a.endb + q / 8) : 0u;
+}
+
 // Message buffer of one wave in LDS: messages [base, base + fill) of the wave task wait there
 // until a round would overflow it, then leave with coalesced stores.
 struct MsgBuf {
@@ -246,23 +269,22 @@ __device__ inline void flush_messages(const StripArgs &a, MsgBuf &B) {
 // The messages of the segments that end in the round go to the wave's buffer in segment
 // order; the sum of the segment left open stays in `carry`.
 template <bool FULL>
-__device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &B, u32x4 cv, uint32_t q, uint32_t e0,
-                                   uint32_t e1, uint64_t upto, double &carry) {
+__device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &B, u32x4 cv, uint32_t eb, uint32_t q,
+                                   uint32_t e0, uint32_t e1, uint64_t upto, double &carry) {
   const uint32_t lane = lane_id();
   const bool live = q < e1 && q + 8 > e0;
   float x[8];
-  uint32_t ends = 0;
+  uint32_t ends = eb;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const uint32_t c = (cv[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
-    const float v = sv[c & 0x7FFFu];
+    const float v = sv[c];
     if (FULL) {
-      x[k] = v;  // a dead lane (past e1) holds zero codes: no ends; its sum is dropped below
-      ends |= (c >> 15) << k;
+      x[k] = v;  // a dead lane (past e1) holds zero codes and no ends; its sum is dropped below
     } else {
       const bool ok = q + k >= e0 && q + k < e1;
       x[k] = ok ? v : 0.0f;
-      if (ok && (c & 0x8000u)) ends |= 1u << k;
+      if (!ok) ends &= ~(1u << k);
     }
   }
   const uint32_t cnt = (uint32_t)__popc(ends);
@@ -280,31 +302,32 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &
   const uint32_t spare = B.fill + total + min(lane, kRound - 1 - min(B.fill + total, kRound - 1));
   const uint32_t slot0 = ends ? o0 : spare;
   const uint32_t kf = ends ? (uint32_t)__ffs(ends) - 1u : 8u;
-  // the sum restarts after an end: run = fma(run, keep, x) with keep = 1.0, or 0.0 after an
-  // end -- exactly run + x, or exactly x (run is finite) -- one FMA in place of two selects
+  // A lane's sums (at most 8 edges) are taken in FP32 and restart after an end: run =
+  // fma(run, keep, x) with keep = 1.0, or 0.0 after an end -- exactly the rounded run + x, or
+  // exactly x -- one FMA in place of two selects.  Sums across lanes and rounds are FP64.
   const uint32_t nends = ~ends;
-  double run = 0.0, first = 0.0;
+  float run = 0.0f, first = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     if (k == 0) {
-      run = (double)x[0];
+      run = x[0];
     } else {
-      const uint32_t keep_hi = (uint32_t)(-(int32_t)((nends >> (k - 1)) & 1u)) & 0x3FF00000u;
-      run = fma(run, __hiloint2double((int)keep_hi, 0), (double)x[k]);
+      const uint32_t keep = (0u - ((nends >> (k - 1)) & 1u)) & 0x3F800000u;
+      run = fmaf(run, __uint_as_float(keep), x[k]);
     }
     if ((uint32_t)k == kf) first = run;
-    B.mb[slot0 + (uint32_t)__popc(ends & ((1u << k) - 1u))] = (float)run;
+    B.mb[slot0 + (uint32_t)__popc(ends & ((1u << k) - 1u))] = run;
   }
-  run *= __hiloint2double((int)((uint32_t)(-(int32_t)((nends >> 7) & 1u)) & 0x3FF00000u), 0);
-  if (FULL && !live) run = 0.0;
+  run *= __uint_as_float((0u - ((nends >> 7) & 1u)) & 0x3F800000u);
+  if (FULL && !live) run = 0.0f;
   // the open sum through each lane; the previous round's carry joins where no segment
   // started since the round began
   const bool head = ends != 0;
-  double open = wave_segmented_scan_dpp_short(run, head);
+  double open = wave_segmented_scan_dpp_short((double)run, head);
   if (!(__ballot(head) & upto)) open += carry;
   double cin = dpp_f64<kDppWaveShr1>(open);
   if (lane == 0) cin = carry;
-  if (head) B.mb[o0] = (float)(cin + first);
+  if (head) B.mb[o0] = (float)(cin + (double)first);
   carry = read_lane_f64(open, kWave - 1);
   B.fill += total;
 }
@@ -314,16 +337,16 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &
 // an unreached source), a segment's message is the greatest offer, and the open segment's
 // value carries over lanes and rounds by a segmented maximum.
 template <bool FULL, int MODE>
-__device__ inline void min_round(const StripArgs &a, const uint32_t *sv, MsgBuf &B, u32x4 cv, u32x4 w0, u32x4 w1,
-                                 uint32_t q, uint32_t e0, uint32_t e1, uint64_t upto, uint32_t &carry) {
+__device__ inline void min_round(const StripArgs &a, const uint32_t *sv, MsgBuf &B, u32x4 cv, uint32_t eb, u32x4 w0,
+                                 u32x4 w1, uint32_t q, uint32_t e0, uint32_t e1, uint64_t upto, uint32_t &carry) {
   const uint32_t lane = lane_id();
   const bool live = q < e1 && q + 8 > e0;
   uint32_t x[8];
-  uint32_t ends = 0;
+  uint32_t ends = eb;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const uint32_t c = (cv[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
-    uint32_t v = sv[c & 0x7FFFu];
+    uint32_t v = sv[c];
     if (MODE != kSweepCC) {
       const uint32_t w = MODE == kSweepWeighted    ? (k < 4 ? w0[k] : w1[k - 4])
                          : MODE == kSweepWeighted8 ? (w0[k >> 2] >> (8 * (k & 3))) & 0xFFu
@@ -334,11 +357,10 @@ __device__ inline void min_round(const StripArgs &a, const uint32_t *sv, MsgBuf 
     }
     if (FULL) {
       x[k] = v;
-      ends |= (c >> 15) << k;
     } else {
       const bool ok = q + k >= e0 && q + k < e1;
       x[k] = ok ? v : 0u;
-      if (ok && (c & 0x8000u)) ends |= 1u << k;
+      if (!ok) ends &= ~(1u << k);
     }
   }
   const uint32_t cnt = (uint32_t)__popc(ends);
@@ -407,9 +429,11 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
   typename std::conditional<MODE == kSweepPR, double, uint32_t>::type carry = 0;  // the open segment's value
   MsgBuf B{mb, seg, 0u};
   u32x4 buf[kAhead], wa[kAhead], wb[kAhead];
+  uint32_t eba[kAhead];
 #pragma unroll
   for (int i = 0; i < kAhead; i++) {
     buf[i] = load_round(a, b0, i, e0, e1);
+    eba[i] = load_round_ends(a, b0, i, e0, e1);
     if (MODE == kSweepWeighted) load_round_w(a, b0, i, e0, e1, wa[i], wb[i]);
     if (MODE == kSweepWeighted8) load_round_w8(a, b0, i, e0, e1, wa[i]);
   }
@@ -418,7 +442,9 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
     for (int i = 0; i < kAhead; i++) {
       if (r + i < nr) {
         const u32x4 cv = buf[i];
+        const uint32_t eb = eba[i];
         buf[i] = load_round(a, b0, r + i + kAhead, e0, e1);
+        eba[i] = load_round_ends(a, b0, r + i + kAhead, e0, e1);
         u32x4 w0{}, w1{};
         if (MODE == kSweepWeighted) {
           w0 = wa[i];
@@ -435,11 +461,11 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
         const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
         if constexpr (MODE != kSweepPR) {
           const uint32_t *su = reinterpret_cast<const uint32_t *>(sv);
-          if (cut) min_round<false, MODE>(a, su, B, cv, w0, w1, q, e0, e1, upto, carry);
-          else min_round<true, MODE>(a, su, B, cv, w0, w1, q, e0, e1, upto, carry);
+          if (cut) min_round<false, MODE>(a, su, B, cv, eb, w0, w1, q, e0, e1, upto, carry);
+          else min_round<true, MODE>(a, su, B, cv, eb, w0, w1, q, e0, e1, upto, carry);
         } else {
-          if (cut) strip_round<false>(a, sv, B, cv, q, e0, e1, upto, carry);
-          else strip_round<true>(a, sv, B, cv, q, e0, e1, upto, carry);
+          if (cut) strip_round<false>(a, sv, B, cv, eb, q, e0, e1, upto, carry);
+          else strip_round<true>(a, sv, B, cv, eb, q, e0, e1, upto, carry);
         }
       }
     }
@@ -1080,6 +1106,11 @@ bool ensure_strips(Engine &e) {
   strip_seg_kernel<<<grid_for((uint64_t)ns + 1), kBlock, 0, e.stream>>>(soff.as<uint32_t>(), segid.as<uint32_t>(), ns,
                                                                          E, nseg, sseg.as<uint32_t>());
   GMP_CHECK_LAUNCH();
+  // the end bits leave the codes for a byte per 8 edges (the padding's bytes stay zero)
+  const uint64_t ngrp = (E + kCodePad) / 8;
+  S.endb.alloc(ngrp);
+  split_ends_kernel<<<grid_for(ngrp), kBlock, 0, e.stream>>>(S.code.as<uint16_t>(), ngrp, S.endb.as<uint8_t>());
+  GMP_CHECK_LAUNCH();
   GMP_HIP(hipStreamSynchronize(e.stream));
   kA.release();
   kB.release();
@@ -1165,6 +1196,7 @@ void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2)
   a.d = io.d;
   a.seed = io.seed;
   a.code = S.code.as<uint16_t>();
+  a.endb = S.endb.as<uint8_t>();
   a.wtask = S.wtask.as<uint4>();
   a.gtask = S.gtask.as<uint4>();
   a.ngt = S.ngt;
@@ -1303,6 +1335,7 @@ SweepRunPtr sweep_open(Engine &e, int kind, uint32_t *c, uint32_t *vstamp) {
   r.gflag.alloc((size_t)std::max<uint32_t>(r.nsall, 1) * 4);
   StripArgs &a = r.a;
   a.code = S.code.as<uint16_t>();
+  a.endb = S.endb.as<uint8_t>();
   a.wtask = S.wtask.as<uint4>();
   a.gtask = S.gtask.as<uint4>();
   a.ngt = S.ngt;
