@@ -235,6 +235,57 @@ __global__ void bin_count_kernel(const uint32_t *runs, uint32_t ns, uint32_t nbi
 
 // ---------------------------------------------------------------- iteration kernels
 
+// The gather kernels hold no static LDS (the host checks), so their dynamic LDS -- the strip's
+// values, then the message buffers -- starts at LDS address 0 and a byte offset is an address.
+typedef __attribute__((address_space(3))) float lds_f32;
+__device__ inline float lds_ld(uint32_t byte) { return *reinterpret_cast<const lds_f32 *>((uintptr_t)byte); }
+__device__ inline void lds_st(uint32_t byte, float v) { *reinterpret_cast<lds_f32 *>((uintptr_t)byte) = v; }
+
+// Byte offsets of the strip values the low and high 16-bit codes of a word name (one SDWA shift
+// each: the compiler folds the mask of the low half into the shift otherwise).
+__device__ inline uint32_t code_lo4(uint32_t w) {
+  uint32_t r;
+  asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+      : "=v"(r)
+      : "v"(w));
+  return r;
+}
+__device__ inline uint32_t code_hi4(uint32_t w) {
+  uint32_t r;
+  asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1"
+      : "=v"(r)
+      : "v"(w));
+  return r;
+}
+
+// All ones when bit K of nends is set (edge K is not an end), else 0.  Written as the instruction
+// itself: the compiler rewrites the 1.0-or-0 factor built from it into a compare and a select,
+// and it converts a signed 1-bit field extract to float as if it were unsigned.
+template <int K>
+__device__ inline uint32_t nonend_mask(uint32_t nends) {
+  uint32_t m;
+  asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(m) : "v"(nends), "n"(K));
+  return m;
+}
+
+// a + 4 m (one instruction; m is 0 or all ones, so this is a or a - 4)
+__device__ inline uint32_t lshl2_add(uint32_t m, uint32_t a) {
+  uint32_t r;
+  asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(r) : "v"(m), "v"(a));
+  return r;
+}
+
+// Edge K of a lane's PageRank round (strip_round): the run restarts after an end, and the store
+// goes to the slot of the lane's next message.
+template <int K>
+__device__ inline void pr_edge(float (&x)[8], float &run, uint32_t nends, uint32_t &A) {
+  const uint32_t m = nonend_mask<K - 1>(nends);
+  run = fmaf(run, __uint_as_float(m & 0x3F800000u), x[K]);
+  A = lshl2_add(m, A);
+  lds_st(A + 4u * K, run);
+  x[K] = run;
+}
+
 // Codes of round r of a wave task (8 per lane); zero outside [e0, e1).
 __device__ inline u32x4 load_round(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1) {
   const uint32_t q = b0 + r * kRound + 8 * lane_id();
@@ -258,35 +309,47 @@ struct MsgBuf {
 
 __device__ inline void flush_messages(const StripArgs &a, MsgBuf &B) {
   wave_sync();
-  for (uint32_t i = lane_id(); i < B.fill; i += kWave) a.msg[B.base + i] = B.mb[i];
+  // four LDS reads in flight before their stores
+  uint32_t i = lane_id();
+  for (; i + 3 * kWave < B.fill; i += 4 * kWave) {
+    const float m0 = B.mb[i], m1 = B.mb[i + kWave], m2 = B.mb[i + 2 * kWave], m3 = B.mb[i + 3 * kWave];
+    a.msg[B.base + i] = m0;
+    a.msg[B.base + i + kWave] = m1;
+    a.msg[B.base + i + 2 * kWave] = m2;
+    a.msg[B.base + i + 3 * kWave] = m3;
+  }
+  for (; i < B.fill; i += kWave) a.msg[B.base + i] = B.mb[i];
   wave_sync();
   B.base += B.fill;
   B.fill = 0;
 }
 
-// One round: 8 consecutive edges per lane from q.  FULL: every lane's 8 edges lie inside
-// [e0, e1) or entirely outside it (only the first and last rounds of a strip can cut a lane).
-// The messages of the segments that end in the round go to the wave's buffer in segment
-// order; the sum of the segment left open stays in `carry`.
+// One PageRank round: 8 consecutive edges per lane from q (codes cv, end bits eb), the wave's
+// message buffer at LDS byte address mbb.  FULL: every lane's 8 edges lie inside [e0, e1) or
+// entirely past e1 (only the first and last rounds of a task can cut a lane).  The messages of
+// the segments that end in the round go to the buffer in segment order; the sum of the segment
+// left open stays in `carry`.
 template <bool FULL>
-__device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &B, u32x4 cv, uint32_t eb, uint32_t q,
-                                   uint32_t e0, uint32_t e1, uint64_t upto, double &carry) {
+__device__ inline void strip_round(const StripArgs &a, MsgBuf &B, uint32_t mbb, u32x4 cv, uint32_t eb, uint32_t q,
+                                   uint32_t e0, uint32_t e1, double &carry) {
   const uint32_t lane = lane_id();
-  const bool live = q < e1 && q + 8 > e0;
   float x[8];
   uint32_t ends = eb;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const uint32_t c = (cv[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
-    const float v = sv[c];
+    const uint32_t w = cv[k >> 1];
+    const float v = lds_ld((k & 1) ? code_hi4(w) : code_lo4(w));
     if (FULL) {
-      x[k] = v;  // a dead lane (past e1) holds zero codes and no ends; its sum is dropped below
+      x[k] = v;
     } else {
       const bool ok = q + k >= e0 && q + k < e1;
       x[k] = ok ? v : 0.0f;
       if (!ok) ends &= ~(1u << k);
     }
   }
+  // a lane past e1 (last round) reads codes of the padding or the next task: it ends nothing,
+  // and its sums reach only the carry out of the task's last round, which is dropped
+  if (FULL && q >= e1) ends = 0u;
   const uint32_t cnt = (uint32_t)__popc(ends);
   const uint32_t incl = wave_scan_dpp(cnt);
   const uint32_t total = read_lane_u32(incl, kWave - 1);
@@ -301,34 +364,40 @@ __device__ inline void strip_round(const StripArgs &a, const float *sv, MsgBuf &
   // buffer has room (lanes writing one LDS address would serialise).
   const uint32_t spare = B.fill + total + min(lane, kRound - 1 - min(B.fill + total, kRound - 1));
   const uint32_t slot0 = ends ? o0 : spare;
-  const uint32_t kf = ends ? (uint32_t)__ffs(ends) - 1u : 8u;
-  // A lane's sums (at most 8 edges) are taken in FP32 and restart after an end: run =
-  // fma(run, keep, x) with keep = 1.0, or 0.0 after an end -- exactly the rounded run + x, or
-  // exactly x -- one FMA in place of two selects.  Sums across lanes and rounds are FP64.
+  // Edge k's slot address is mbb + 4 (slot0 + k - non-ends before k) = A + 4 k: A drops by 4 after
+  // every non-end and 4 k is the store's immediate offset.  The lane's sums (at most 8 edges) run
+  // in FP32 and restart after an end: run = fma(run, keep, x) with keep = 1.0 (after a non-end)
+  // or 0 -- exactly the rounded run + x, or exactly x.  Sums across lanes and rounds: below.
   const uint32_t nends = ~ends;
-  float run = 0.0f, first = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    if (k == 0) {
-      run = x[0];
-    } else {
-      const uint32_t keep = (0u - ((nends >> (k - 1)) & 1u)) & 0x3F800000u;
-      run = fmaf(run, __uint_as_float(keep), x[k]);
-    }
-    if ((uint32_t)k == kf) first = run;
-    B.mb[slot0 + (uint32_t)__popc(ends & ((1u << k) - 1u))] = run;
-  }
-  run *= __uint_as_float((0u - ((nends >> 7) & 1u)) & 0x3F800000u);
-  if (FULL && !live) run = 0.0f;
-  // the open sum through each lane; the previous round's carry joins where no segment
-  // started since the round began
-  const bool head = ends != 0;
-  double open = wave_segmented_scan_dpp_short((double)run, head);
-  if (!(__ballot(head) & upto)) open += carry;
-  double cin = dpp_f64<kDppWaveShr1>(open);
-  if (lane == 0) cin = carry;
-  if (head) B.mb[o0] = (float)(cin + (double)first);
-  carry = read_lane_f64(open, kWave - 1);
+  uint32_t A = mbb + 4u * slot0;
+  float run = x[0];
+  lds_st(A, run);
+  pr_edge<1>(x, run, nends, A);
+  pr_edge<2>(x, run, nends, A);
+  pr_edge<3>(x, run, nends, A);
+  pr_edge<4>(x, run, nends, A);
+  pr_edge<5>(x, run, nends, A);
+  pr_edge<6>(x, run, nends, A);
+  pr_edge<7>(x, run, nends, A);
+  run *= __uint_as_float(nonend_mask<7>(nends) & 0x3F800000u);  // the open tail (none after an end at 7)
+  // the lane's first message before the carry in: its run at the first end (3-level select)
+  const uint32_t kf = (uint32_t)__builtin_ctz(ends | 0x80000000u);
+  const float t0 = (kf & 1u) ? x[1] : x[0], t1 = (kf & 1u) ? x[3] : x[2];
+  const float t2 = (kf & 1u) ? x[5] : x[4], t3 = (kf & 1u) ? x[7] : x[6];
+  const float u0 = (kf & 2u) ? t1 : t0, u1 = (kf & 2u) ? t3 : t2;
+  const float first = (kf & 4u) ? u1 : u0;
+  // The open sums through each lane (FP32, segments start at lanes with ends); a head lane's
+  // message adds the open sum through the lane before it, and the carry of the rounds before
+  // joins the round's first head lane.  Head lanes (re)write their first slot; the others
+  // write their free slot.  The carry stays in FP64.
+  const uint64_t hm = __ballot(ends != 0u);
+  const float open = wave_segmented_scan_f32(run, hm);
+  const float cin = __builtin_amdgcn_update_dpp(0.0f, open, kDppWaveShr1, 0xF, 0xF, true);
+  const bool firsthead = __builtin_amdgcn_inverse_ballot_w64(hm & (0ull - hm));
+  const double m0 = (double)cin + (double)first;
+  lds_st(mbb + 4u * slot0, (float)(firsthead ? m0 + carry : m0));
+  const double tail = (double)__builtin_bit_cast(float, read_lane_u32(__builtin_bit_cast(uint32_t, open), kWave - 1));
+  carry = hm ? tail : carry + tail;
   B.fill += total;
 }
 
@@ -417,16 +486,66 @@ __device__ inline void load_round_w(const StripArgs &a, uint32_t b0, uint32_t r,
   }
 }
 
+// PageRank messages of edges [e0, e1) whose first segment has index seg; every segment ends
+// inside.  The loads of codes and end bits run kAhead rounds ahead of their use; a round's loads
+// are whole (the layout's padding keeps those of the last round in bounds).
+__device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint32_t e0, uint32_t e1,
+                                    uint32_t seg) {
+  const uint32_t lane = lane_id();
+  const uint32_t b0 = e0 & ~7u;
+  const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
+  const u32x4 *cb = reinterpret_cast<const u32x4 *>(a.code + b0);  // a round is kWave words of 8 codes
+  const uint8_t *ebb = a.endb + b0 / 8;
+  double carry = 0.0;  // the open segment's sum
+  MsgBuf B{mb, seg, 0u};
+  u32x4 buf[kAhead];
+  uint32_t eba[kAhead];
+#pragma unroll
+  for (int i = 0; i < kAhead; i++) {
+    buf[i] = u32x4{0u, 0u, 0u, 0u};
+    eba[i] = 0u;
+    if ((uint32_t)i < nr) {
+      buf[i] = __builtin_nontemporal_load(cb + (uint32_t)i * kWave + lane);
+      eba[i] = __builtin_nontemporal_load(ebb + (uint32_t)i * kWave + lane);
+    }
+  }
+  for (uint32_t r = 0; r < nr; r += kAhead) {
+#pragma unroll
+    for (int i = 0; i < kAhead; i++) {
+      if (r + i < nr) {
+        const u32x4 cv = buf[i];
+        const uint32_t eb = eba[i];
+        const uint32_t rn = r + i + kAhead;
+        if (rn < nr) {
+          buf[i] = __builtin_nontemporal_load(cb + rn * kWave + lane);
+          eba[i] = __builtin_nontemporal_load(ebb + rn * kWave + lane);
+        }
+        const uint32_t base = b0 + (r + i) * kRound;
+        const uint32_t q = base + 8 * lane;
+        // lanes are cut only where e0 or e1 is not a multiple of 8 inside this round
+        const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
+        if (cut) strip_round<false>(a, B, mbb, cv, eb, q, e0, e1, carry);
+        else strip_round<true>(a, B, mbb, cv, eb, q, e0, e1, carry);
+      }
+    }
+  }
+  flush_messages(a, B);
+}
+
 // Messages of edges [e0, e1) whose first segment has index seg; every segment ends inside.
 // The code loads run kAhead rounds ahead of their use.
 template <int MODE>
-__device__ inline void strip_wave_task(const StripArgs &a, const float *sv, float *mb, uint32_t e0, uint32_t e1,
-                                       uint32_t seg) {
+__device__ inline void strip_wave_task(const StripArgs &a, const float *sv, float *mb, uint32_t mbb, uint32_t e0,
+                                       uint32_t e1, uint32_t seg) {
+  if constexpr (MODE == kSweepPR) {
+    pr_wave_task(a, mb, mbb, e0, e1, seg);
+    return;
+  }
   const uint32_t lane = lane_id();
   const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
   const uint32_t b0 = e0 & ~7u;
   const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
-  typename std::conditional<MODE == kSweepPR, double, uint32_t>::type carry = 0;  // the open segment's value
+  uint32_t carry = 0;  // the open segment's value
   MsgBuf B{mb, seg, 0u};
   u32x4 buf[kAhead], wa[kAhead], wb[kAhead];
   uint32_t eba[kAhead];
@@ -459,14 +578,9 @@ __device__ inline void strip_wave_task(const StripArgs &a, const float *sv, floa
         const uint32_t q = base + 8 * lane;
         // lanes are cut only where e0 or e1 is not a multiple of 8 inside this round
         const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
-        if constexpr (MODE != kSweepPR) {
-          const uint32_t *su = reinterpret_cast<const uint32_t *>(sv);
-          if (cut) min_round<false, MODE>(a, su, B, cv, eb, w0, w1, q, e0, e1, upto, carry);
-          else min_round<true, MODE>(a, su, B, cv, eb, w0, w1, q, e0, e1, upto, carry);
-        } else {
-          if (cut) strip_round<false>(a, sv, B, cv, eb, q, e0, e1, upto, carry);
-          else strip_round<true>(a, sv, B, cv, eb, q, e0, e1, upto, carry);
-        }
+        const uint32_t *su = reinterpret_cast<const uint32_t *>(sv);
+        if (cut) min_round<false, MODE>(a, su, B, cv, eb, w0, w1, q, e0, e1, upto, carry);
+        else min_round<true, MODE>(a, su, B, cv, eb, w0, w1, q, e0, e1, upto, carry);
       }
     }
   }
@@ -481,6 +595,7 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
   const uint32_t *vals = MODE != kSweepPR ? a.nlab : reinterpret_cast<const uint32_t *>(a.contrib_in);
   const uint32_t wv = wave_id();
   float *mb = sv + W + wv * kRound;
+  const uint32_t mbb = 4u * (W + wv * kRound);  // mb as an LDS address
   for (uint32_t t = blockIdx.x; t < a.ngt; t += gridDim.x) {
     const uint4 gt = a.gtask[t];
     // a sweep skips strips none of whose vertices changed in the previous sweep; the bins apply
@@ -539,7 +654,7 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
     __syncthreads();
     for (uint32_t w = gt.y + wv; w < gt.z; w += kIterWaves) {
       const uint4 wt = a.wtask[w];
-      strip_wave_task<MODE>(a, sv, mb, wt.x, wt.y, wt.z);
+      strip_wave_task<MODE>(a, sv, mb, mbb, wt.x, wt.y, wt.z);
     }
   }
 }
@@ -670,8 +785,9 @@ __device__ inline void bin_messages(const StripArgs &a, uint32_t n, const BinLds
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       if (m[u] != 0xFFFFFFFFu) {
-        x[u] = __builtin_nontemporal_load(&msgw[m[u]]);
-        r[u] = __builtin_nontemporal_load(&a.seg_row[m[u]]);
+        // plain loads: the lines at a run's borders are shared with the neighbouring bins
+        x[u] = msgw[m[u]];
+        r[u] = a.seg_row[m[u]];
       }
     }
 #pragma unroll
@@ -690,7 +806,7 @@ __global__ __launch_bounds__(kIterBlock) void pr_bin_kernel(StripArgs a) {
   const uint32_t per = (ns + kIterBlock - 1) / kIterBlock;
   const uint32_t s0 = min(tid * per, ns), s1 = min(s0 + per, ns);
   PrAcc pacc;
-  for (uint32_t t = blockIdx.x; t < a.nbt; t += gridDim.x) {
+  for (uint32_t t = xcd_block_id(); t < a.nbt; t += gridDim.x) {
     const uint4 bt = a.btask[t];
     const uint32_t n = bin_prepare<false>(a, bt, L, 0ull, wtot, &live, s0, s1);
     bin_messages(a, n, L, [&](uint32_t r, uint32_t x) { atomicAdd(&L.acc[r], fix_from_float(__uint_as_float(x))); });
@@ -823,7 +939,7 @@ __global__ __launch_bounds__(kIterBlock) void min_bin_kernel(StripArgs a) {
   const uint32_t s0 = min(tid * per, ns), s1 = min(s0 + per, ns);
   uint32_t changed = 0;
   unsigned long long ndeg = 0;
-  for (uint32_t t = blockIdx.x; t < a.nbt; t += gridDim.x) {
+  for (uint32_t t = xcd_block_id(); t < a.nbt; t += gridDim.x) {
     const uint4 bt = a.btask[t];
     // a whole bin with no message of a gathered strip cannot change; a split bin's slice still
     // writes its partial (zeros), which the split apply reads
@@ -978,6 +1094,14 @@ int lds_limit(Engine &e) {
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, e.device) != hipSuccess || v <= 0)
     v = 65536;
   return v;
+}
+
+// The gather kernels address their dynamic LDS from 0, which holds while they declare no static
+// LDS (the dynamic part follows the static part).
+void check_gather_lds(const void *kernel) {
+  hipFuncAttributes fa{};
+  GMP_HIP(hipFuncGetAttributes(&fa, kernel));
+  if (fa.sharedSizeBytes != 0) fail(GMP_EHIP, "strip gather kernel holds static LDS");
 }
 
 }  // namespace
@@ -1173,6 +1297,7 @@ bool ensure_strips(Engine &e) {
   S.g3 = std::max<uint32_t>(1, std::min<uint32_t>(S.nsplit * ((R + kSplitRows - 1) / kSplitRows), cus * 4));
   // raise the dynamic-LDS ceiling where the runtime keeps one; a refusal leaves the launch
   // itself to report a size the device cannot hold
+  check_gather_lds(reinterpret_cast<const void *>(&pr_strip_gather_kernel));
   if (hipFuncSetAttribute(reinterpret_cast<const void *>(&pr_strip_gather_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
     (void)hipGetLastError();
@@ -1320,6 +1445,7 @@ SweepRunPtr sweep_open(Engine &e, int kind, uint32_t *c, uint32_t *vstamp) {
                    : mode == kSweepUnit      ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepUnit>)
                    : mode == kSweepWeighted8 ? reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepWeighted8>)
                                              : reinterpret_cast<const void *>(&min_strip_gather_kernel<kSweepWeighted>);
+  check_gather_lds(gk);
   if (hipFuncSetAttribute(gk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
     (void)hipGetLastError();
   if (hipFuncSetAttribute(reinterpret_cast<const void *>(&min_bin_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -19,6 +19,16 @@ __device__ inline void wave_sync() {
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
+// gfx950 deals the workgroups of a launch round-robin over its 8 XCDs, each with its own L2.
+// xcd_block_id() renumbers them so that each XCD holds a contiguous range of ids (when the grid
+// is a multiple of 8): tasks taken in id order then put neighbouring tasks -- which share the
+// cache lines at their borders -- on one L2.
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_block_id() {
+  const uint32_t b = blockIdx.x, g = gridDim.x;
+  return g % kXcds ? b : (b % kXcds) * (g / kXcds) + b / kXcds;
+}
+
 // Wave index inside the workgroup, made provably uniform so loads indexed by it are
 // scalar loads.
 __device__ inline uint32_t wave_id() {
@@ -155,6 +165,34 @@ __device__ inline double wave_segmented_scan_dpp_short(double x, bool head) {
     seg_scan_step<kDppRowBcast31, 0xC>(x, f);
   }
   return x;
+}
+
+// One step of wave_segmented_scan_f32: lanes whose flag (bit l of f) is set keep x, the others
+// add the value the DPP control moves in (one DPP add and one select on a scalar mask).
+template <int CTRL, int ROWS = 0xF>
+__device__ inline float seg_scan_step_f32(float x, uint64_t f) {
+  const float y = __builtin_amdgcn_update_dpp(0.0f, x, CTRL, ROWS, 0xF, true);
+  return __builtin_amdgcn_inverse_ballot_w64(f) ? x : x + y;
+}
+
+// Inclusive segmented FP32 sum over the 64 lanes with the step pattern of wave_scan_dpp; bit l
+// of `heads` starts a segment at lane l.  The flags (a segment start inside a lane's window)
+// follow on the scalar unit: a row shift by s moves flag bits up by s inside each row of 16; a
+// row broadcast sets a whole row when the source lane's flag is set.
+__device__ inline float wave_segmented_scan_f32(float x, uint64_t heads) {
+  constexpr uint64_t kRow = 0x0001000100010001ull;  // lane 0 of every row
+  uint64_t f = heads;
+  x = seg_scan_step_f32<kDppRowShr1>(x, f);
+  f |= (f << 1) & ~(kRow * 0x1ull);
+  x = seg_scan_step_f32<kDppRowShr2>(x, f);
+  f |= (f << 2) & ~(kRow * 0x3ull);
+  x = seg_scan_step_f32<kDppRowShr4>(x, f);
+  f |= (f << 4) & ~(kRow * 0xFull);
+  x = seg_scan_step_f32<kDppRowShr8>(x, f);
+  f |= (f << 8) & ~(kRow * 0xFFull);
+  x = seg_scan_step_f32<kDppRowBcast15, 0xA>(x, f);  // lane 15 into row 1, lane 47 into row 3
+  f |= ((f >> 15) & 1ull ? 0x00000000FFFF0000ull : 0ull) | ((f >> 47) & 1ull ? 0xFFFF000000000000ull : 0ull);
+  return seg_scan_step_f32<kDppRowBcast31, 0xC>(x, f);  // lane 31 into rows 2 and 3
 }
 
 template <int CTRL, int ROWS = 0xF>
