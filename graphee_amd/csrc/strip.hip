@@ -286,39 +286,26 @@ __device__ inline void pr_edge(float (&x)[8], float &run, uint32_t nends, uint32
   x[K] = run;
 }
 
-// Codes of round r of a wave task (8 per lane); zero outside [e0, e1).
-__device__ inline u32x4 load_round(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1) {
-  const uint32_t q = b0 + r * kRound + 8 * lane_id();
-  u32x4 cv = {0u, 0u, 0u, 0u};
-  if (q < e1 && q + 8 > e0) cv = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.code + q));
-  return cv;
-}
-
-// End bits of the lane's 8 edges of round r (zero outside [e0, e1)).
-__device__ inline uint32_t load_round_ends(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1) {
-  const uint32_t q = b0 + r * kRound + 8 * lane_id();
-  return q < e1 && q + 8 > e0 ? (uint32_t)__builtin_nontemporal_load(a.endb + q / 8) : 0u;
-}
-
 // Message buffer of one wave in LDS: messages [base, base + fill) of the wave task wait there
 // until a round would overflow it, then leave with coalesced stores.
 struct MsgBuf {
+  float *msg;  // the message array
   float *mb;
   uint32_t base, fill;
 };
 
-__device__ inline void flush_messages(const StripArgs &a, MsgBuf &B) {
+__device__ inline void flush_messages(MsgBuf &B) {
   wave_sync();
   // four LDS reads in flight before their stores
   uint32_t i = lane_id();
   for (; i + 3 * kWave < B.fill; i += 4 * kWave) {
     const float m0 = B.mb[i], m1 = B.mb[i + kWave], m2 = B.mb[i + 2 * kWave], m3 = B.mb[i + 3 * kWave];
-    a.msg[B.base + i] = m0;
-    a.msg[B.base + i + kWave] = m1;
-    a.msg[B.base + i + 2 * kWave] = m2;
-    a.msg[B.base + i + 3 * kWave] = m3;
+    B.msg[B.base + i] = m0;
+    B.msg[B.base + i + kWave] = m1;
+    B.msg[B.base + i + 2 * kWave] = m2;
+    B.msg[B.base + i + 3 * kWave] = m3;
   }
-  for (; i < B.fill; i += kWave) a.msg[B.base + i] = B.mb[i];
+  for (; i < B.fill; i += kWave) B.msg[B.base + i] = B.mb[i];
   wave_sync();
   B.base += B.fill;
   B.fill = 0;
@@ -330,7 +317,7 @@ __device__ inline void flush_messages(const StripArgs &a, MsgBuf &B) {
 // the segments that end in the round go to the buffer in segment order; the sum of the segment
 // left open stays in `carry`.
 template <bool FULL>
-__device__ inline void strip_round(const StripArgs &a, MsgBuf &B, uint32_t mbb, u32x4 cv, uint32_t eb, uint32_t q,
+__device__ inline void strip_round(MsgBuf &B, uint32_t mbb, u32x4 cv, uint32_t eb, uint32_t q,
                                    uint32_t e0, uint32_t e1, double &carry) {
   const uint32_t lane = lane_id();
   float x[8];
@@ -354,7 +341,7 @@ __device__ inline void strip_round(const StripArgs &a, MsgBuf &B, uint32_t mbb, 
   const uint32_t incl = wave_scan_dpp(cnt);
   const uint32_t total = read_lane_u32(incl, kWave - 1);
   // room for the round plus a free slot (needed only when some edge is not an end)
-  if (B.fill + total >= kRound) flush_messages(a, B);
+  if (B.fill + total >= kRound) flush_messages(B);
   const uint32_t o0 = B.fill + incl - cnt;  // this lane's first slot
   // Every edge writes slot slot0 + (ends before it).  In a lane with ends, an end writes its
   // message and a non-end writes the slot of the lane's next message (which overwrites it
@@ -401,28 +388,40 @@ __device__ inline void strip_round(const StripArgs &a, MsgBuf &B, uint32_t mbb, 
   B.fill += total;
 }
 
+// Edge K of a lane's sweep round (min_round): the running maximum restarts after an end (m = 0
+// clears it; 0 is neutral), and the store goes to the slot of the lane's next message.
+template <int K>
+__device__ inline void min_edge(uint32_t (&x)[8], uint32_t &run, uint32_t nends, uint32_t &A) {
+  const uint32_t m = nonend_mask<K - 1>(nends);
+  run = max(run & m, x[K]);
+  A = lshl2_add(m, A);
+  lds_st(A + 4u * K, __uint_as_float(run));
+  x[K] = run;
+}
+
 // The round of the CC and SSSP sweeps: the value an edge offers its destination is the source's
 // complemented label (CC) or the complement of min(distance + weight, 2^32 - 2) (SSSP; 0 from
 // an unreached source), a segment's message is the greatest offer, and the open segment's
-// value carries over lanes and rounds by a segmented maximum.
+// value carries over lanes and rounds by a segmented maximum.  Slots, FULL and the layout of
+// the round as in strip_round; w0, w1 hold the lane's 8 weights (MODE kSweepWeighted) or its
+// 8 byte weights in w0.x, w0.y (kSweepWeighted8).
 template <bool FULL, int MODE>
-__device__ inline void min_round(const StripArgs &a, const uint32_t *sv, MsgBuf &B, u32x4 cv, uint32_t eb, u32x4 w0,
-                                 u32x4 w1, uint32_t q, uint32_t e0, uint32_t e1, uint64_t upto, uint32_t &carry) {
+__device__ inline void min_round(MsgBuf &B, uint32_t mbb, u32x4 cv, uint32_t eb, u32x4 w0, u32x4 w1, uint32_t q,
+                                 uint32_t e0, uint32_t e1, uint32_t &carry) {
   const uint32_t lane = lane_id();
-  const bool live = q < e1 && q + 8 > e0;
   uint32_t x[8];
   uint32_t ends = eb;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const uint32_t c = (cv[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
-    uint32_t v = sv[c];
+    const uint32_t cw = cv[k >> 1];
+    uint32_t v = __float_as_uint(lds_ld((k & 1) ? code_hi4(cw) : code_lo4(cw)));
     if (MODE != kSweepCC) {
       const uint32_t w = MODE == kSweepWeighted    ? (k < 4 ? w0[k] : w1[k - 4])
                          : MODE == kSweepWeighted8 ? (w0[k >> 2] >> (8 * (k & 3))) & 0xFFu
                                                    : 1u;
-      const uint32_t d = ~v;
-      const uint32_t t = d > 0xFFFFFFFEu - w ? 0xFFFFFFFEu : d + w;
-      v = v ? ~t : 0u;
+      // in complements c = ~d: ~min(d + w, 2^32 - 2) is c - w, or 1 where d + w saturates
+      // (c <= w); an unreached source (c = 0) offers nothing
+      v = min(v, max(__builtin_elementwise_sub_sat(v, w), 1u));
     }
     if (FULL) {
       x[k] = v;
@@ -432,58 +431,92 @@ __device__ inline void min_round(const StripArgs &a, const uint32_t *sv, MsgBuf 
       if (!ok) ends &= ~(1u << k);
     }
   }
+  if (FULL && q >= e1) ends = 0u;  // past the task's last edge: only the dropped carry sees it
   const uint32_t cnt = (uint32_t)__popc(ends);
   const uint32_t incl = wave_scan_dpp(cnt);
   const uint32_t total = read_lane_u32(incl, kWave - 1);
-  if (B.fill + total >= kRound) flush_messages(a, B);
+  if (B.fill + total >= kRound) flush_messages(B);
   const uint32_t o0 = B.fill + incl - cnt;
-  // slots as in strip_round
   const uint32_t spare = B.fill + total + min(lane, kRound - 1 - min(B.fill + total, kRound - 1));
   const uint32_t slot0 = ends ? o0 : spare;
-  const uint32_t kf = ends ? (uint32_t)__ffs(ends) - 1u : 8u;
-  uint32_t *mb = reinterpret_cast<uint32_t *>(B.mb);
   const uint32_t nends = ~ends;
-  uint32_t run = 0, first = 0;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    run = k == 0 ? x[0] : max(run & (0u - ((nends >> (k - 1)) & 1u)), x[k]);  // restart after an end
-    if ((uint32_t)k == kf) first = run;
-    mb[slot0 + (uint32_t)__popc(ends & ((1u << k) - 1u))] = run;
-  }
-  run &= 0u - ((nends >> 7) & 1u);
-  if (!live) run = 0u;
-  const bool head = ends != 0;
-  uint32_t open = wave_segmented_max_dpp_short(run, head);
-  if (!(__ballot(head) & upto)) open = max(open, carry);
-  uint32_t cin = dpp_u32<kDppWaveShr1>(open);
-  if (lane == 0) cin = carry;
-  if (head) mb[o0] = max(cin, first);
-  carry = read_lane_u32(open, kWave - 1);
+  uint32_t A = mbb + 4u * slot0;
+  uint32_t run = x[0];
+  lds_st(A, __uint_as_float(run));
+  min_edge<1>(x, run, nends, A);
+  min_edge<2>(x, run, nends, A);
+  min_edge<3>(x, run, nends, A);
+  min_edge<4>(x, run, nends, A);
+  min_edge<5>(x, run, nends, A);
+  min_edge<6>(x, run, nends, A);
+  min_edge<7>(x, run, nends, A);
+  run &= nonend_mask<7>(nends);
+  const uint32_t kf = (uint32_t)__builtin_ctz(ends | 0x80000000u);
+  const uint32_t t0 = (kf & 1u) ? x[1] : x[0], t1 = (kf & 1u) ? x[3] : x[2];
+  const uint32_t t2 = (kf & 1u) ? x[5] : x[4], t3 = (kf & 1u) ? x[7] : x[6];
+  const uint32_t u0 = (kf & 2u) ? t1 : t0, u1 = (kf & 2u) ? t3 : t2;
+  const uint32_t first = (kf & 4u) ? u1 : u0;
+  const uint64_t hm = __ballot(ends != 0u);
+  const uint32_t open = wave_segmented_max_u32(run, hm);
+  const uint32_t cin = dpp_u32<kDppWaveShr1>(open);
+  const bool firsthead = __builtin_amdgcn_inverse_ballot_w64(hm & (0ull - hm));
+  const uint32_t m0 = max(cin, first);
+  lds_st(mbb + 4u * slot0, __uint_as_float(firsthead ? max(m0, carry) : m0));
+  const uint32_t tail = read_lane_u32(open, kWave - 1);
+  carry = hm ? tail : max(carry, tail);
   B.fill += total;
 }
 
-// Byte weights of round r (8 per lane, one 8-byte load into w0.x, w0.y); zero outside [e0, e1).
-__device__ inline void load_round_w8(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1,
-                                     u32x4 &w0) {
-  const uint32_t q = b0 + r * kRound + 8 * lane_id();
-  w0 = u32x4{0u, 0u, 0u, 0u};
-  if (q < e1 && q + 8 > e0) {
-    const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(a.wt8 + q));
-    w0.x = v.x;
-    w0.y = v.y;
+// Sweep messages of edges [e0, e1) whose first segment has index seg (every segment ends
+// inside); loads as in pr_wave_task, weights with them.
+template <int MODE>
+__device__ inline void min_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint32_t e0, uint32_t e1,
+                                     uint32_t seg) {
+  const uint32_t lane = lane_id();
+  const uint32_t b0 = e0 & ~7u;
+  const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
+  const u32x4 *cb = reinterpret_cast<const u32x4 *>(a.code + b0);
+  const uint8_t *ebb = a.endb + b0 / 8;
+  const u32x4 *wb = reinterpret_cast<const u32x4 *>(a.wt + b0);   // two words of 4 weights per lane
+  const u32x2 *w8b = reinterpret_cast<const u32x2 *>(a.wt8 + b0);  // 8 byte weights per lane
+  uint32_t carry = 0;  // the open segment's value
+  MsgBuf B{a.msg, mb, seg, 0u};
+  u32x4 buf[kAhead], wa[kAhead], wc[kAhead];
+  uint32_t eba[kAhead];
+  auto load = [&](int i, uint32_t r) {
+    buf[i] = __builtin_nontemporal_load(cb + r * kWave + lane);
+    eba[i] = __builtin_nontemporal_load(ebb + r * kWave + lane);
+    if (MODE == kSweepWeighted) {
+      wa[i] = __builtin_nontemporal_load(wb + 2 * (r * kWave + lane));
+      wc[i] = __builtin_nontemporal_load(wb + 2 * (r * kWave + lane) + 1);
+    }
+    if (MODE == kSweepWeighted8) {
+      const u32x2 v = __builtin_nontemporal_load(w8b + r * kWave + lane);
+      wa[i] = u32x4{v.x, v.y, 0u, 0u};
+    }
+  };
+#pragma unroll
+  for (int i = 0; i < kAhead; i++) {
+    buf[i] = wa[i] = wc[i] = u32x4{0u, 0u, 0u, 0u};
+    eba[i] = 0u;
+    if ((uint32_t)i < nr) load(i, (uint32_t)i);
   }
-}
-
-// Weights of round r of a wave task (8 per lane, two 16-byte loads); zero outside [e0, e1).
-__device__ inline void load_round_w(const StripArgs &a, uint32_t b0, uint32_t r, uint32_t e0, uint32_t e1, u32x4 &w0,
-                                    u32x4 &w1) {
-  const uint32_t q = b0 + r * kRound + 8 * lane_id();
-  w0 = u32x4{0u, 0u, 0u, 0u};
-  w1 = w0;
-  if (q < e1 && q + 8 > e0) {
-    w0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.wt + q));
-    w1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.wt + q + 4));
+  for (uint32_t r = 0; r < nr; r += kAhead) {
+#pragma unroll
+    for (int i = 0; i < kAhead; i++) {
+      if (r + i < nr) {
+        const u32x4 cv = buf[i], w0 = wa[i], w1 = wc[i];
+        const uint32_t eb = eba[i];
+        if (r + i + kAhead < nr) load(i, r + i + kAhead);
+        const uint32_t base = b0 + (r + i) * kRound;
+        const uint32_t q = base + 8 * lane;
+        const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
+        if (cut) min_round<false, MODE>(B, mbb, cv, eb, w0, w1, q, e0, e1, carry);
+        else min_round<true, MODE>(B, mbb, cv, eb, w0, w1, q, e0, e1, carry);
+      }
+    }
   }
+  flush_messages(B);
 }
 
 // PageRank messages of edges [e0, e1) whose first segment has index seg; every segment ends
@@ -497,7 +530,7 @@ __device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb,
   const u32x4 *cb = reinterpret_cast<const u32x4 *>(a.code + b0);  // a round is kWave words of 8 codes
   const uint8_t *ebb = a.endb + b0 / 8;
   double carry = 0.0;  // the open segment's sum
-  MsgBuf B{mb, seg, 0u};
+  MsgBuf B{a.msg, mb, seg, 0u};
   u32x4 buf[kAhead];
   uint32_t eba[kAhead];
 #pragma unroll
@@ -524,67 +557,20 @@ __device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb,
         const uint32_t q = base + 8 * lane;
         // lanes are cut only where e0 or e1 is not a multiple of 8 inside this round
         const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
-        if (cut) strip_round<false>(a, B, mbb, cv, eb, q, e0, e1, carry);
-        else strip_round<true>(a, B, mbb, cv, eb, q, e0, e1, carry);
+        if (cut) strip_round<false>(B, mbb, cv, eb, q, e0, e1, carry);
+        else strip_round<true>(B, mbb, cv, eb, q, e0, e1, carry);
       }
     }
   }
-  flush_messages(a, B);
+  flush_messages(B);
 }
 
-// Messages of edges [e0, e1) whose first segment has index seg; every segment ends inside.
-// The code loads run kAhead rounds ahead of their use.
+// Messages of edges [e0, e1) whose first segment has index seg.
 template <int MODE>
-__device__ inline void strip_wave_task(const StripArgs &a, const float *sv, float *mb, uint32_t mbb, uint32_t e0,
-                                       uint32_t e1, uint32_t seg) {
-  if constexpr (MODE == kSweepPR) {
-    pr_wave_task(a, mb, mbb, e0, e1, seg);
-    return;
-  }
-  const uint32_t lane = lane_id();
-  const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
-  const uint32_t b0 = e0 & ~7u;
-  const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
-  uint32_t carry = 0;  // the open segment's value
-  MsgBuf B{mb, seg, 0u};
-  u32x4 buf[kAhead], wa[kAhead], wb[kAhead];
-  uint32_t eba[kAhead];
-#pragma unroll
-  for (int i = 0; i < kAhead; i++) {
-    buf[i] = load_round(a, b0, i, e0, e1);
-    eba[i] = load_round_ends(a, b0, i, e0, e1);
-    if (MODE == kSweepWeighted) load_round_w(a, b0, i, e0, e1, wa[i], wb[i]);
-    if (MODE == kSweepWeighted8) load_round_w8(a, b0, i, e0, e1, wa[i]);
-  }
-  for (uint32_t r = 0; r < nr; r += kAhead) {
-#pragma unroll
-    for (int i = 0; i < kAhead; i++) {
-      if (r + i < nr) {
-        const u32x4 cv = buf[i];
-        const uint32_t eb = eba[i];
-        buf[i] = load_round(a, b0, r + i + kAhead, e0, e1);
-        eba[i] = load_round_ends(a, b0, r + i + kAhead, e0, e1);
-        u32x4 w0{}, w1{};
-        if (MODE == kSweepWeighted) {
-          w0 = wa[i];
-          w1 = wb[i];
-          load_round_w(a, b0, r + i + kAhead, e0, e1, wa[i], wb[i]);
-        }
-        if (MODE == kSweepWeighted8) {
-          w0 = wa[i];
-          load_round_w8(a, b0, r + i + kAhead, e0, e1, wa[i]);
-        }
-        const uint32_t base = b0 + (r + i) * kRound;
-        const uint32_t q = base + 8 * lane;
-        // lanes are cut only where e0 or e1 is not a multiple of 8 inside this round
-        const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
-        const uint32_t *su = reinterpret_cast<const uint32_t *>(sv);
-        if (cut) min_round<false, MODE>(a, su, B, cv, eb, w0, w1, q, e0, e1, upto, carry);
-        else min_round<true, MODE>(a, su, B, cv, eb, w0, w1, q, e0, e1, upto, carry);
-      }
-    }
-  }
-  flush_messages(a, B);
+__device__ inline void strip_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint32_t e0, uint32_t e1,
+                                       uint32_t seg) {
+  if constexpr (MODE == kSweepPR) pr_wave_task(a, mb, mbb, e0, e1, seg);
+  else min_wave_task<MODE>(a, mb, mbb, e0, e1, seg);
 }
 
 // One strip's values (contributions, or complemented labels for CC) staged in LDS, then the
@@ -654,7 +640,7 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
     __syncthreads();
     for (uint32_t w = gt.y + wv; w < gt.z; w += kIterWaves) {
       const uint4 wt = a.wtask[w];
-      strip_wave_task<MODE>(a, sv, mb, mbb, wt.x, wt.y, wt.z);
+      strip_wave_task<MODE>(a, mb, mbb, wt.x, wt.y, wt.z);
     }
   }
 }

@@ -123,48 +123,16 @@ __device__ inline uint32_t wave_scan_dpp(uint32_t x) {
   return x;
 }
 
-template <int CTRL, int ROWS = 0xF>
-__device__ inline void seg_scan_step(double &x, uint32_t &f) {
-  const double y = dpp_f64<CTRL, ROWS>(x);
-  const uint32_t fy = dpp_u32<CTRL, ROWS>(f);
-  if (!f) x += y;
-  f |= fy;
+// Segment flags of the scans below (bit l: a segment starts inside lane l's window): after a
+// row shift by S the flags move up by S inside each row of 16; after the row broadcast of lane
+// 15 into rows 1 and 3 a row takes the flag of its source lane.
+template <int S>
+__device__ inline uint64_t seg_flags_shift(uint64_t f) {
+  constexpr uint64_t kRow = 0x0001000100010001ull;  // lane 0 of every row
+  return f | ((f << S) & ~(kRow * ((1ull << S) - 1)));
 }
-
-// Inclusive segmented sum over the 64 lanes (same step pattern as wave_scan_dpp); a lane with
-// `head` starts a new segment.  The combination tree is fixed, so FP64 results are
-// deterministic.
-__device__ inline double wave_segmented_scan_dpp(double x, bool head) {
-  uint32_t f = head ? 1u : 0u;
-  seg_scan_step<kDppRowShr1>(x, f);
-  seg_scan_step<kDppRowShr2>(x, f);
-  seg_scan_step<kDppRowShr4>(x, f);
-  seg_scan_step<kDppRowShr8>(x, f);
-  seg_scan_step<kDppRowBcast15, 0xA>(x, f);
-  seg_scan_step<kDppRowBcast31, 0xC>(x, f);
-  return x;
-}
-
-// The same segmented sum, skipping the steps no lane needs: after the row shift by s a lane is
-// complete when its window holds a segment start or reaches its row's first lane; the row
-// broadcasts run only when some lane of rows 1-3 has no start in its row prefix.  A skipped
-// step would leave every lane unchanged, so the result equals the full scan's.
-__device__ inline double wave_segmented_scan_dpp_short(double x, bool head) {
-  const uint32_t lane = lane_id(), rl = lane & 15u;
-  uint32_t f = head ? 1u : 0u;
-  seg_scan_step<kDppRowShr1>(x, f);
-  if (__ballot(!f && rl >= 2u)) {
-    seg_scan_step<kDppRowShr2>(x, f);
-    if (__ballot(!f && rl >= 4u)) {
-      seg_scan_step<kDppRowShr4>(x, f);
-      if (__ballot(!f && rl >= 8u)) seg_scan_step<kDppRowShr8>(x, f);
-    }
-  }
-  if (__ballot(!f && lane >= 16u)) {
-    seg_scan_step<kDppRowBcast15, 0xA>(x, f);
-    seg_scan_step<kDppRowBcast31, 0xC>(x, f);
-  }
-  return x;
+__device__ inline uint64_t seg_flags_bcast15(uint64_t f) {
+  return f | ((f >> 15) & 1ull ? 0x00000000FFFF0000ull : 0ull) | ((f >> 47) & 1ull ? 0xFFFF000000000000ull : 0ull);
 }
 
 // One step of wave_segmented_scan_f32: lanes whose flag (bit l of f) is set keep x, the others
@@ -176,51 +144,42 @@ __device__ inline float seg_scan_step_f32(float x, uint64_t f) {
 }
 
 // Inclusive segmented FP32 sum over the 64 lanes with the step pattern of wave_scan_dpp; bit l
-// of `heads` starts a segment at lane l.  The flags (a segment start inside a lane's window)
-// follow on the scalar unit: a row shift by s moves flag bits up by s inside each row of 16; a
-// row broadcast sets a whole row when the source lane's flag is set.
+// of `heads` starts a segment at lane l.  The flags follow on the scalar unit.  The combination
+// tree is fixed, so results are deterministic.
 __device__ inline float wave_segmented_scan_f32(float x, uint64_t heads) {
-  constexpr uint64_t kRow = 0x0001000100010001ull;  // lane 0 of every row
   uint64_t f = heads;
   x = seg_scan_step_f32<kDppRowShr1>(x, f);
-  f |= (f << 1) & ~(kRow * 0x1ull);
+  f = seg_flags_shift<1>(f);
   x = seg_scan_step_f32<kDppRowShr2>(x, f);
-  f |= (f << 2) & ~(kRow * 0x3ull);
+  f = seg_flags_shift<2>(f);
   x = seg_scan_step_f32<kDppRowShr4>(x, f);
-  f |= (f << 4) & ~(kRow * 0xFull);
+  f = seg_flags_shift<4>(f);
   x = seg_scan_step_f32<kDppRowShr8>(x, f);
-  f |= (f << 8) & ~(kRow * 0xFFull);
+  f = seg_flags_shift<8>(f);
   x = seg_scan_step_f32<kDppRowBcast15, 0xA>(x, f);  // lane 15 into row 1, lane 47 into row 3
-  f |= ((f >> 15) & 1ull ? 0x00000000FFFF0000ull : 0ull) | ((f >> 47) & 1ull ? 0xFFFF000000000000ull : 0ull);
-  return seg_scan_step_f32<kDppRowBcast31, 0xC>(x, f);  // lane 31 into rows 2 and 3
+  return seg_scan_step_f32<kDppRowBcast31, 0xC>(x, seg_flags_bcast15(f));  // lane 31 into rows 2, 3
 }
 
 template <int CTRL, int ROWS = 0xF>
-__device__ inline void seg_max_step(uint32_t &x, uint32_t &f) {
-  const uint32_t y = dpp_u32<CTRL, ROWS>(x);
-  const uint32_t fy = dpp_u32<CTRL, ROWS>(f);
-  if (!f) x = max(x, y);
-  f |= fy;
+__device__ inline uint32_t seg_max_step_u32(uint32_t x, uint64_t f) {
+  const uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xF, true);
+  return __builtin_amdgcn_inverse_ballot_w64(f) ? x : max(x, y);
 }
 
-// Inclusive segmented maximum over the 64 lanes with the same early exits as
-// wave_segmented_scan_dpp_short; 0 (what the DPP moves in past a row's start) is neutral.
-__device__ inline uint32_t wave_segmented_max_dpp_short(uint32_t x, bool head) {
-  const uint32_t lane = lane_id(), rl = lane & 15u;
-  uint32_t f = head ? 1u : 0u;
-  seg_max_step<kDppRowShr1>(x, f);
-  if (__ballot(!f && rl >= 2u)) {
-    seg_max_step<kDppRowShr2>(x, f);
-    if (__ballot(!f && rl >= 4u)) {
-      seg_max_step<kDppRowShr4>(x, f);
-      if (__ballot(!f && rl >= 8u)) seg_max_step<kDppRowShr8>(x, f);
-    }
-  }
-  if (__ballot(!f && lane >= 16u)) {
-    seg_max_step<kDppRowBcast15, 0xA>(x, f);
-    seg_max_step<kDppRowBcast31, 0xC>(x, f);
-  }
-  return x;
+// Inclusive segmented maximum over the 64 lanes with the steps and flags of
+// wave_segmented_scan_f32; 0 (what the DPP moves in past a row's start) is neutral.
+__device__ inline uint32_t wave_segmented_max_u32(uint32_t x, uint64_t heads) {
+  uint64_t f = heads;
+  x = seg_max_step_u32<kDppRowShr1>(x, f);
+  f = seg_flags_shift<1>(f);
+  x = seg_max_step_u32<kDppRowShr2>(x, f);
+  f = seg_flags_shift<2>(f);
+  x = seg_max_step_u32<kDppRowShr4>(x, f);
+  f = seg_flags_shift<4>(f);
+  x = seg_max_step_u32<kDppRowShr8>(x, f);
+  f = seg_flags_shift<8>(f);
+  x = seg_max_step_u32<kDppRowBcast15, 0xA>(x, f);
+  return seg_max_step_u32<kDppRowBcast31, 0xC>(x, seg_flags_bcast15(f));
 }
 
 __device__ inline uint32_t read_lane_u32(uint32_t x, int lane) {
