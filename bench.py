@@ -535,7 +535,8 @@ def main():
     ap.add_argument("--strip-log", type=int, default=15,
                     help="PageRank source strips of 2^k ids (0: the row gather over the in-CSR)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="skip the autotuner pass over the strip layout before the warm-up")
+                    help="skip the autotuner passes (the strip layout before the warm-up, the batched "
+                         "kernels' knobs in the batched leg)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size "
                          "dispatch that scales the HBM byte counters under rocprofv3 --pmc)")
@@ -777,7 +778,7 @@ def main():
         result["extra"]["pagerank_rmat_s20_x20"] = run_pagerank_s20(eng, cpu_threads())
         result["extra"]["sssp_rmat_s24"] = run_sssp_s24(eng, cpu_threads())
     if not args.no_extra and not args.no_batched:
-        result["extra"]["batched"] = run_batched(eng, verify=not args.no_cpu)
+        result["extra"]["batched"] = run_batched(eng, verify=not args.no_cpu, tune=args.autotune)
     if not args.no_extra and not args.no_learned:
         result["extra"]["learned_sssp"] = run_learned(eng, args.learn_scale)
 
