@@ -3,11 +3,11 @@
 // persistent launch per round, with the rounds captured in a HIP graph.
 //
 // The graphs are stored as one disjoint union (graph k owns union ids [vbase_k, vbase_{k+1})),
-// in-CSR for PageRank and out-CSR for SSSP.  An SSSP instance i on graph k owns the distance
-// slots [slot_i, slot_i + V_k); a frontier entry is a slot, so the frontiers of all instances
-// share one queue and one relax launch advances every instance.  Personalised-PageRank
-// instances on the same graph share the edge stream: their ranks are stored instance-minor
-// ([v][k]) so one gathered source yields the contributions of all k instances.
+// in-CSR for PageRank and out-CSR for SSSP.  Instances on the same graph share the edge stream
+// and keep their values instance-minor ([v][k]): SSSP groups of up to 32 instances relax an edge
+// once for every instance active at its source (a frontier entry is a group vertex with a mask
+// of instances), so the frontiers of all instances share one queue and one relax launch advances
+// every instance; one gathered PageRank source yields the contributions of all k instances.
 //
 // HIP graphs have no conditional nodes, so SSSP captures two rounds (ping-pong queues; the last
 // workgroup of each round flips the round counter and clears the consumed queue's count --
@@ -55,58 +55,79 @@ inline uint32_t grid_for(uint64_t n) {
 }
 
 // ---- SSSP ------------------------------------------------------------------------------
+//
+// Instances are grouped by graph, at most kGroupInst to a group.  A group keeps the distances of
+// its k instances vertex-major: instance j's distance at vertex v is dist[dslot + v * kk + j],
+// kk = k rounded up to a power of two, so the instances of one graph relax an edge with one read
+// of the edge and one cache line of destination distances.  A frontier entry is a group vertex
+// (gv = gvbase + v, each at most once per round); mask[gv] holds the instances whose distance at
+// v dropped in the round before.  Two mask arrays alternate: a round reads and clears the one
+// its entries were queued with and ORs the instances it improves into the other; the OR that
+// finds a vertex's mask empty queues the vertex.
 
-struct BatchSsspArgs {
-  const uint64_t *off;         // union out-CSR offsets
-  const uint32_t *idx;         // union ids
-  const uint32_t *w;
-  const uint32_t *inst_slot;   // [n+1] slot bases
-  const uint32_t *inst_vbase;  // [n] union vertex base of the instance's graph
-  uint32_t ninst;
-  uint32_t *dist;              // [total slots]
-  uint32_t *stamp;             // [total slots]
-  uint32_t *queue0, *queue1;
-  uint32_t *count;             // [0]=|queue0|, [1]=|queue1|, [2]=round
-  uint32_t *ticket;
-  unsigned long long *edges;
+constexpr uint32_t kGroupInst = 32;   // instances per group (bits of a mask word)
+constexpr uint32_t kLdsGroups = 256;  // group table kept in LDS up to this many groups
+constexpr uint32_t kAppendCap = 512;  // staged appends per wave
+
+struct BatchGroup {
+  uint32_t gvbase;  // first group vertex
+  uint32_t vbase;   // union vertex base of the group's graph
+  uint32_t dslot;   // first distance slot
+  uint32_t kklog;   // log2 of the distance stride kk
 };
 
-__device__ inline uint32_t instance_of(const uint32_t *slot_base, uint32_t n, uint32_t s) {
-  uint32_t lo = 0, hi = n - 1;  // last i with slot_base[i] <= s
+struct BatchSsspArgs {
+  const uint64_t *off;        // union out-CSR offsets
+  const uint32_t *idx;        // union ids
+  const uint32_t *w;
+  const BatchGroup *groups;
+  uint32_t ngroups;
+  uint32_t *dist;             // group-major, vertex-major inside a group
+  uint32_t *mask0, *mask1;    // per group vertex, alternating by round
+  uint32_t *queue0, *queue1;  // group vertices
+  uint32_t *count;            // [0]=|queue0|, [1]=|queue1|, [2]=round
+  uint32_t *ticket;
+  unsigned long long *edges;  // instance edges relaxed
+};
+
+// the last group whose first group vertex is <= gv
+__device__ inline uint32_t group_of(const BatchGroup *g, uint32_t n, uint32_t gv) {
+  uint32_t lo = 0, hi = n - 1;
   while (lo < hi) {
     const uint32_t mid = (lo + hi + 1) >> 1;
-    if (slot_base[mid] <= s) lo = mid;
+    if (g[mid].gvbase <= gv) lo = mid;
     else hi = mid - 1;
   }
   return lo;
 }
 
-constexpr uint32_t kLdsInst = 1024;  // instance slot bases kept in LDS up to this many instances
-constexpr uint32_t kAppendCap = 512;  // staged appends per wave
-
-// One round for every instance: reads queue[round&1] and appends improved slots to the other
-// queue; the last workgroup to finish flips the round and clears the consumed count.  A wave
-// takes 64 frontier slots and expands their edges kRelaxUnroll per lane at a time (a tunable:
-// 2, 4 or 8): the edge, weight and distance loads of those edges are issued together (edges past
-// the end reload the wave's first edge and are dropped), then the improving ones are applied
-// with atomics.
+// One round for every instance: reads queue[round&1] and appends improved group vertices to
+// the other queue; the last workgroup to finish flips the round and clears the consumed count.
+// A wave takes 64 frontier entries and expands their edges kRelaxUnroll per lane at a time (a
+// tunable: 2, 4 or 8); a lane walks the instances of its edge's entry: for each, the improving
+// distance goes in by atomicMin, and one atomicOr per edge marks the instances improved at the
+// destination.
 template <int kRelaxUnroll>
 __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
   __shared__ uint32_t s_start[kWavesPerBlock][kWave];
   __shared__ uint64_t s_base[kWavesPerBlock][kWave];
-  __shared__ uint32_t s_val[kWavesPerBlock][kWave];
-  __shared__ uint32_t s_sb[kWavesPerBlock][kWave];  // slot base minus union vertex base
-  __shared__ uint32_t s_islot[kLdsInst];
+  __shared__ uint32_t s_mask[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_src[kWavesPerBlock][kWave];  // distance slot of the entry's instance 0
+  __shared__ uint32_t s_grp[kWavesPerBlock][kWave];
+  __shared__ BatchGroup s_g[kLdsGroups];
   __shared__ uint32_t s_app[kWavesPerBlock][kAppendCap];
-  const bool lds_inst = a.ninst <= kLdsInst;
-  if (lds_inst)
-    for (uint32_t i = threadIdx.x; i < a.ninst; i += blockDim.x) s_islot[i] = a.inst_slot[i];
+  const bool lds_g = a.ngroups <= kLdsGroups;
+  if (lds_g)
+    for (uint32_t i = threadIdx.x; i < a.ngroups; i += blockDim.x) s_g[i] = a.groups[i];
   __syncthreads();
+  const BatchGroup *G = lds_g ? s_g : a.groups;
   const uint32_t round = a.count[2];
   const uint32_t cur = round & 1u, nxt = cur ^ 1u;
   const uint32_t nfront = a.count[cur];
   const uint32_t *front = cur ? a.queue1 : a.queue0;
   uint32_t *out = cur ? a.queue0 : a.queue1;
+  uint32_t *mcur = cur ? a.mask1 : a.mask0;
+  uint32_t *mnxt = cur ? a.mask0 : a.mask1;
   const uint32_t lane = lane_id(), wv = wave_id();
   const uint32_t nchunks = (nfront + kWave - 1) / kWave;
   unsigned long long edges = 0;
@@ -114,30 +135,33 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
   for (uint32_t c = blockIdx.x * kWavesPerBlock + wv; c < nchunks; c += gridDim.x * kWavesPerBlock) {
     const uint32_t i = c * kWave + lane;
     uint64_t b = 0;
-    uint32_t deg = 0, du = kInf, sb = 0;
+    uint32_t deg = 0, m = 0, src = 0, gi = 0;
     if (i < nfront) {
-      const uint32_t s = front[i];
-      const uint32_t inst = lds_inst ? instance_of(s_islot, a.ninst, s) : instance_of(a.inst_slot, a.ninst, s);
-      const uint32_t u = a.inst_vbase[inst] + (s - a.inst_slot[inst]);
-      sb = a.inst_slot[inst] - a.inst_vbase[inst];
+      const uint32_t gv = front[i];
+      gi = group_of(G, a.ngroups, gv);
+      const BatchGroup g = G[gi];
+      const uint32_t v = gv - g.gvbase, u = g.vbase + v;
+      m = mcur[gv];
+      mcur[gv] = 0u;
+      // instances whose distance at v dropped again in this round are queued for the next one,
+      // where they relax v's edges from the lower distance: skip them here
+      m &= ~mnxt[gv];
       b = a.off[u];
-      deg = (uint32_t)(a.off[u + 1] - b);
-      du = a.dist[s];
-      // a slot whose distance already dropped again in this round is queued for the next one,
-      // where it relaxes its edges from the lower distance: skip it here
-      if (du == kInf || a.stamp[s] == round + 1) deg = 0;
+      deg = m ? (uint32_t)(a.off[u + 1] - b) : 0u;
+      src = g.dslot + (v << g.kklog);
+      edges += (unsigned long long)deg * __popc(m);
     }
     const uint32_t incl = wave_scan_dpp(deg);
     const uint32_t total = read_lane_u32(incl, kWave - 1);
     s_start[wv][lane] = incl - deg;
     s_base[wv][lane] = b;
-    s_val[wv][lane] = du;
-    s_sb[wv][lane] = sb;
+    s_mask[wv][lane] = m;
+    s_src[wv][lane] = src;
+    s_grp[wv][lane] = gi;
     wave_sync();
-    edges += total;
     for (uint32_t jb = 0; jb < total; jb += kWave * kRelaxUnroll) {
       uint64_t e[kRelaxUnroll];
-      uint32_t src[kRelaxUnroll];
+      uint32_t own[kRelaxUnroll];
 #pragma unroll
       for (int q = 0; q < kRelaxUnroll; q++) {
         const uint32_t j = jb + lane + (uint32_t)q * kWave;
@@ -149,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
           if (s_start[wv][mid] <= jj) lo = mid;
           else hi = mid - 1;
         }
-        src[q] = lo;
+        own[q] = lo;
         e[q] = s_base[wv][lo] + (jj - s_start[wv][lo]);
       }
       uint32_t to[kRelaxUnroll], wt[kRelaxUnroll];
@@ -158,33 +182,35 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
         to[q] = a.idx[e[q]];
         wt[q] = a.w[e[q]];
       }
-      uint32_t slot[kRelaxUnroll], nd[kRelaxUnroll], cd[kRelaxUnroll];
+      uint32_t app = 0, gvt[kRelaxUnroll];
 #pragma unroll
       for (int q = 0; q < kRelaxUnroll; q++) {
-        slot[q] = s_sb[wv][src[q]] + to[q];
-        const uint64_t t = (uint64_t)s_val[wv][src[q]] + wt[q];
-        nd[q] = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
-        cd[q] = a.dist[slot[q]];
-      }
-      // nd < cd (read this round) means the slot's distance drops this round, by this lane or
-      // by another one, so the slot belongs in the next frontier whichever atomicMin wins; the
-      // stamp exchange lets exactly one lane append it.  All atomics are issued before any
-      // result is read.
-      bool imp[kRelaxUnroll];
-      uint32_t old[kRelaxUnroll];
-#pragma unroll
-      for (int q = 0; q < kRelaxUnroll; q++) {
-        imp[q] = jb + lane + (uint32_t)q * kWave < total && nd[q] < cd[q];
-        if (imp[q]) {
-          atomicMin(&a.dist[slot[q]], nd[q]);
-          old[q] = atomicExch(&a.stamp[slot[q]], round + 1);
+        gvt[q] = 0u;
+        if (jb + lane + (uint32_t)q * kWave >= total) continue;
+        const uint32_t o = own[q];
+        const BatchGroup g = G[s_grp[wv][o]];
+        const uint32_t tl = to[q] - g.vbase;
+        const uint32_t tb = g.dslot + (tl << g.kklog), sb = s_src[wv][o];
+        uint32_t mo = s_mask[wv][o], imp = 0;
+        // nd < cd (read this round) means the distance drops this round, by this lane or by
+        // another one, so the instance belongs in the next frontier whichever atomicMin wins
+        // (waiting for atomicMin's old value instead costs more than the marks it saves)
+        while (mo) {
+          const uint32_t k = (uint32_t)__builtin_ctz(mo);
+          mo &= mo - 1u;
+          const uint64_t t = (uint64_t)a.dist[sb + k] + wt[q];
+          const uint32_t nd = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
+          if (nd < a.dist[tb + k]) {
+            atomicMin(&a.dist[tb + k], nd);
+            imp |= 1u << k;
+          }
+        }
+        if (imp) {
+          gvt[q] = g.gvbase + tl;
+          if (atomicOr(&mnxt[gvt[q]], imp) == 0u) app |= 1u << q;  // the vertex's first mark
         }
       }
-      uint32_t app = 0;
-#pragma unroll
-      for (int q = 0; q < kRelaxUnroll; q++)
-        if (imp[q] && old[q] != round + 1) app |= 1u << q;
-      // the lane's improved slots, staged in the wave's LDS buffer
+      // the lane's newly queued group vertices, staged in the wave's LDS buffer
       const uint32_t cnt = (uint32_t)__popc(app);
       const uint32_t inc = wave_scan_dpp(cnt);
       const uint32_t tot = read_lane_u32(inc, kWave - 1);
@@ -192,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
         uint32_t pos = wa.n + inc - cnt;
 #pragma unroll
         for (int q = 0; q < kRelaxUnroll; q++)
-          if ((app >> q) & 1u) wa.buf[pos++] = slot[q];
+          if ((app >> q) & 1u) wa.buf[pos++] = gvt[q];
         wa.n += tot;
         if (wa.n > kAppendCap - kWave * kRelaxUnroll) append_flush(wa, &a.count[nxt], out);
       }
@@ -200,6 +226,7 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
     wave_sync();
   }
   append_flush(wa, &a.count[nxt], out);
+  edges = wave_reduce_sum_u64(edges);
   if (lane == 0 && edges) atomicAdd(a.edges, edges);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -214,25 +241,44 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
   }
 }
 
-__global__ void batch_sssp_init_kernel(uint32_t *dist, uint32_t *stamp, uint32_t total) {
-  GRID_STRIDE32(i, total) {
-    dist[i] = kInf;
-    stamp[i] = 0;
+// Every distance unreached, every mask empty, the counters cleared.
+__global__ void batch_sssp_init_kernel(uint32_t *dist, uint32_t total, uint32_t *mask0, uint32_t *mask1,
+                                       uint32_t ngv, uint32_t *count, uint32_t *ticket, unsigned long long *edges) {
+  GRID_STRIDE32(i, total) dist[i] = kInf;
+  GRID_STRIDE32(i, ngv) {
+    mask0[i] = 0u;
+    mask1[i] = 0u;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    count[0] = count[1] = count[2] = 0u;
+    *ticket = 0u;
+    *edges = 0ull;
   }
 }
 
-__global__ void batch_sssp_seed_kernel(const uint32_t *slots, uint32_t n, uint32_t *dist, uint32_t *queue0,
-                                       uint32_t *count, uint32_t *ticket, unsigned long long *edges) {
+// The sources: seed i = (distance slot, group vertex, instance bit); instances of one group
+// that share a source vertex queue it once.
+__global__ void batch_sssp_seed_kernel(const uint4 *seeds, uint32_t n, uint32_t *dist, uint32_t *mask0,
+                                       uint32_t *queue0, uint32_t *count) {
   GRID_STRIDE32(i, n) {
-    dist[slots[i]] = 0;
-    queue0[i] = slots[i];
+    const uint4 sd = seeds[i];
+    dist[sd.x] = 0u;
+    if (atomicOr(&mask0[sd.y], sd.z) == 0u) queue0[atomicAdd(&count[0], 1u)] = sd.y;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    count[0] = n;
-    count[1] = 0;
-    count[2] = 0;
-    *ticket = 0;
-    *edges = 0;
+}
+
+// Distances back in instance order: out[oslot_i + v] = dist[dslot_i + v * kk_i + j_i].
+__global__ void batch_sssp_out_kernel(const uint32_t *dist, const uint4 *inst, const uint32_t *oslot, uint32_t n,
+                                      uint32_t total, uint32_t *out) {
+  GRID_STRIDE32(s, total) {
+    uint32_t lo = 0, hi = n - 1;  // last instance with oslot <= s
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (oslot[mid] <= s) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint4 in = inst[lo];  // (dslot, kklog, j, -)
+    out[s] = dist[in.x + ((s - oslot[lo]) << in.y) + in.z];
   }
 }
 
@@ -587,11 +633,11 @@ namespace {
 // into one HIP graph by gmp_batch_run as well as run alone.
 
 struct SsspRun {
-  uint32_t n = 0, T = 0;
-  std::vector<uint32_t> slot;
-  DevBuf d_slot, d_vb, d_seed, dist, stamp, q0, q1, cnt;
+  uint32_t n = 0, T = 0, ngv = 0, D = 0;  // instances, output slots, group vertices, distance slots
+  std::vector<uint32_t> oslot;            // [n+1] output slot bases (instance order)
+  DevBuf d_groups, d_seed, d_inst, d_oslot, dist, mask0, mask1, q0, q1, cnt;
   BatchSsspArgs a{};
-  uint32_t grid = 0;
+  uint32_t grid = 0, nseed = 0;
 };
 
 void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *source, SsspRun &r) {
@@ -601,38 +647,66 @@ void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *
   if (n == 0 || !graph || !source) fail(GMP_EINVAL, "batched SSSP needs instances");
   const uint32_t G = (uint32_t)b.vbase.size() - 1;
   r.n = n;
-  r.slot.assign(n + 1, 0);
-  std::vector<uint32_t> vb(n), seeds(n);
+  r.oslot.assign(n + 1, 0);
+  std::vector<std::vector<uint32_t>> per(G);
   for (uint32_t i = 0; i < n; i++) {
     if (graph[i] >= G) fail(GMP_EINVAL, "instance graph out of range");
     const uint32_t Vk = b.vbase[graph[i] + 1] - b.vbase[graph[i]];
     if (source[i] >= Vk) fail(GMP_EINVAL, "instance source out of range");
-    if ((uint64_t)r.slot[i] + Vk >= 0xFFFFFFFFull) fail(GMP_EUNSUPPORTED, "batch exceeds 2^32 distance slots");
-    r.slot[i + 1] = r.slot[i] + Vk;
-    vb[i] = b.vbase[graph[i]];
-    seeds[i] = r.slot[i] + source[i];
+    if ((uint64_t)r.oslot[i] + Vk >= 0xFFFFFFFFull) fail(GMP_EUNSUPPORTED, "batch exceeds 2^32 distance slots");
+    r.oslot[i + 1] = r.oslot[i] + Vk;
+    per[graph[i]].push_back(i);
   }
-  r.T = r.slot[n];
-  r.d_slot.alloc((n + 1) * 4);
-  r.d_vb.alloc(n * 4);
-  r.d_seed.alloc(n * 4);
-  r.dist.alloc((size_t)r.T * 4);
-  r.stamp.alloc((size_t)r.T * 4);
-  r.q0.alloc((size_t)r.T * 4);
-  r.q1.alloc((size_t)r.T * 4);
+  r.T = r.oslot[n];
+  // groups of at most kGroupInst instances of one graph, in graph order
+  std::vector<BatchGroup> groups;
+  std::vector<uint4> seeds(n), inst(n);
+  uint64_t gv = 0, ds = 0;
+  for (uint32_t k = 0; k < G; k++) {
+    const uint32_t Vk = b.vbase[k + 1] - b.vbase[k];
+    for (size_t s0 = 0; s0 < per[k].size(); s0 += kGroupInst) {
+      const uint32_t cnt = (uint32_t)std::min<size_t>(kGroupInst, per[k].size() - s0);
+      uint32_t kklog = 0;
+      while ((1u << kklog) < cnt) kklog++;
+      if (gv + Vk >= 0xFFFFFFFFull || ds + ((uint64_t)Vk << kklog) >= 0xFFFFFFFFull)
+        fail(GMP_EUNSUPPORTED, "batch exceeds 2^32 distance slots");
+      groups.push_back(BatchGroup{(uint32_t)gv, b.vbase[k], (uint32_t)ds, kklog});
+      for (uint32_t j = 0; j < cnt; j++) {
+        const uint32_t i = per[k][s0 + j];
+        seeds[i] = uint4{(uint32_t)ds + (source[i] << kklog) + j, (uint32_t)gv + source[i], 1u << j, 0u};
+        inst[i] = uint4{(uint32_t)ds, kklog, j, 0u};
+      }
+      gv += Vk;
+      ds += (uint64_t)Vk << kklog;
+    }
+  }
+  r.ngv = (uint32_t)gv;
+  r.D = (uint32_t)ds;
+  r.nseed = n;
+  r.d_groups.alloc(groups.size() * sizeof(BatchGroup));
+  r.d_seed.alloc(n * sizeof(uint4));
+  r.d_inst.alloc(n * sizeof(uint4));
+  r.d_oslot.alloc((n + 1) * 4);
+  r.dist.alloc((size_t)r.D * 4);
+  r.mask0.alloc((size_t)r.ngv * 4);
+  r.mask1.alloc((size_t)r.ngv * 4);
+  r.q0.alloc((size_t)r.ngv * 4);
+  r.q1.alloc((size_t)r.ngv * 4);
   r.cnt.alloc(64);
-  GMP_HIP(hipMemcpyAsync(r.d_slot.p, r.slot.data(), (n + 1) * 4, hipMemcpyHostToDevice, e.stream));
-  GMP_HIP(hipMemcpyAsync(r.d_vb.p, vb.data(), n * 4, hipMemcpyHostToDevice, e.stream));
-  GMP_HIP(hipMemcpyAsync(r.d_seed.p, seeds.data(), n * 4, hipMemcpyHostToDevice, e.stream));
+  GMP_HIP(hipMemcpyAsync(r.d_groups.p, groups.data(), groups.size() * sizeof(BatchGroup), hipMemcpyHostToDevice,
+                         e.stream));
+  GMP_HIP(hipMemcpyAsync(r.d_seed.p, seeds.data(), n * sizeof(uint4), hipMemcpyHostToDevice, e.stream));
+  GMP_HIP(hipMemcpyAsync(r.d_inst.p, inst.data(), n * sizeof(uint4), hipMemcpyHostToDevice, e.stream));
+  GMP_HIP(hipMemcpyAsync(r.d_oslot.p, r.oslot.data(), (n + 1) * 4, hipMemcpyHostToDevice, e.stream));
   BatchSsspArgs &a = r.a;
   a.off = b.out.off.as<uint64_t>();
   a.idx = b.out.idx.as<uint32_t>();
   a.w = b.out.w.as<uint32_t>();
-  a.inst_slot = r.d_slot.as<uint32_t>();
-  a.inst_vbase = r.d_vb.as<uint32_t>();
-  a.ninst = n;
+  a.groups = r.d_groups.as<BatchGroup>();
+  a.ngroups = (uint32_t)groups.size();
   a.dist = r.dist.as<uint32_t>();
-  a.stamp = r.stamp.as<uint32_t>();
+  a.mask0 = r.mask0.as<uint32_t>();
+  a.mask1 = r.mask1.as<uint32_t>();
   a.queue0 = r.q0.as<uint32_t>();
   a.queue1 = r.q1.as<uint32_t>();
   a.count = r.cnt.as<uint32_t>();
@@ -642,9 +716,10 @@ void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *
 }
 
 void sssp_start(Engine &e, SsspRun &r, hipStream_t s) {
-  batch_sssp_init_kernel<<<grid_for(r.T), kBlock, 0, s>>>(r.a.dist, r.a.stamp, r.T);
-  batch_sssp_seed_kernel<<<grid_for(r.n), kBlock, 0, s>>>(r.d_seed.as<uint32_t>(), r.n, r.a.dist, r.a.queue0,
-                                                          r.a.count, r.a.ticket, r.a.edges);
+  batch_sssp_init_kernel<<<grid_for(std::max(r.D, r.ngv)), kBlock, 0, s>>>(r.a.dist, r.D, r.a.mask0, r.a.mask1, r.ngv,
+                                                                            r.a.count, r.a.ticket, r.a.edges);
+  batch_sssp_seed_kernel<<<grid_for(r.nseed), kBlock, 0, s>>>(r.d_seed.as<uint4>(), r.nseed, r.a.dist, r.a.mask0,
+                                                              r.a.queue0, r.a.count);
   GMP_CHECK_LAUNCH();
 }
 
@@ -663,7 +738,16 @@ void sssp_results(Engine &e, SsspRun &r, uint32_t *dist_out, uint64_t &edges) {
   unsigned long long h = 0;
   GMP_HIP(hipMemcpy(&h, r.a.edges, 8, hipMemcpyDeviceToHost));
   edges = h;
-  if (dist_out) GMP_HIP(hipMemcpy(dist_out, r.a.dist, (size_t)r.T * 4, hipMemcpyDeviceToHost));
+  if (dist_out) {
+    DevBuf out;
+    out.alloc((size_t)r.T * 4);
+    batch_sssp_out_kernel<<<grid_for(r.T), kBlock, 0, e.stream>>>(r.a.dist, r.d_inst.as<uint4>(),
+                                                                  r.d_oslot.as<uint32_t>(), r.n, r.T,
+                                                                  out.as<uint32_t>());
+    GMP_CHECK_LAUNCH();
+    GMP_HIP(hipMemcpyAsync(dist_out, out.p, (size_t)r.T * 4, hipMemcpyDeviceToHost, e.stream));
+    GMP_HIP(hipStreamSynchronize(e.stream));
+  }
 }
 
 struct PprRun {

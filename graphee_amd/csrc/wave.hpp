@@ -76,6 +76,12 @@ __device__ inline double wave_reduce_sum(double x) {
   return x;
 }
 
+__device__ inline unsigned long long wave_reduce_sum_u64(unsigned long long x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+  return x;
+}
+
 __device__ inline uint32_t wave_reduce_min(uint32_t x) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, kWave));
