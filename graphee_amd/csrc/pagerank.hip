@@ -356,7 +356,9 @@ void compat_windows_step(Engine &e, int app, int step, uint32_t grid_cap, double
   const uint32_t nwin = (uint32_t)s.win.size();
   const uint8_t *act = nullptr;
   if (minapp && step > 0) act = window_activity(e, (uint32_t)step);
-  for (uint32_t w = 0; w < nwin; w++) {
+  // messages are double-buffered, so the order of the windows changes nothing; window_order
+  // spreads the streamed windows between the cached ones, keeping the copy engines busy
+  for (const uint32_t w : window_order(s)) {
     const EdgeWindow &x = s.win[w];
     if (act && !act[w]) {
       GMP_HIP(hipMemcpyAsync(static_cast<uint32_t *>(msgnew) + x.r0, static_cast<const uint32_t *>(msg) + x.r0,

@@ -24,6 +24,7 @@
 
 #include "gmp_internal.hpp"
 #include "unionfind.hpp"
+#include "window_order.hpp"
 #include "varint.hpp"
 #include "wave.hpp"
 
@@ -482,7 +483,7 @@ void semi_cc_union_find(Engine &e, uint32_t *label_out, gmp_stats *st) {
     wrow[nwin] = V;
     s.wrow.alloc((size_t)(nwin + 1) * 4);
     GMP_HIP(hipMemcpy(s.wrow.p, wrow.data(), (size_t)(nwin + 1) * 4, hipMemcpyHostToDevice));
-    for (uint32_t w = 0; w < nwin; w++) {
+    for (const uint32_t w : window_order(s)) {  // rows are independent: any window order
       const EdgeWindow &x = s.win[w];
       const WindowDev d = window_begin(e, w, false, true);
       first_ids_kernel<OffT><<<grid_for(x.r1 - x.r0), kBlock, 0, e.stream>>>(off, d.idx, x.r0, x.r1,
@@ -518,7 +519,8 @@ void semi_cc_union_find(Engine &e, uint32_t *label_out, gmp_stats *st) {
   std::vector<uint8_t> flags(nwin);
   GMP_HIP(hipMemcpyAsync(flags.data(), s.active.p, nwin, hipMemcpyDeviceToHost, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
-  for (uint32_t w = 0; w < nwin; w++) {
+  // the links' order changes the forest but not the components, whose least ids become labels
+  for (const uint32_t w : window_order(s)) {
     const EdgeWindow &x = s.win[w];
     if (!flags[w]) {
       s.windows_skipped++;
@@ -823,16 +825,9 @@ void window_plans(Engine &e) {
 }
 
 std::vector<uint32_t> window_order(const SemiExternal &s) {
-  const uint32_t nwin = (uint32_t)s.win.size();
-  std::vector<uint32_t> cw, sw, order;
-  order.reserve(nwin);
-  for (uint32_t w = 0; w < nwin; w++) (s.win[w].cache >= 0 ? cw : sw).push_back(w);
-  size_t ic = 0, is = 0;
-  while (ic < cw.size() || is < sw.size()) {
-    const bool take_s = is < sw.size() && (ic == cw.size() || (is + 1) * nwin <= (ic + is + 1) * sw.size());
-    order.push_back(take_s ? sw[is++] : cw[ic++]);
-  }
-  return order;
+  std::vector<bool> cached(s.win.size());
+  for (size_t w = 0; w < s.win.size(); w++) cached[w] = s.win[w].cache >= 0;
+  return interleave_windows(cached);
 }
 
 namespace {
