@@ -120,6 +120,7 @@ struct StripCsr {
   DevBuf bpart;                 // uint64[nslots << rlog]: fixed-point sums of split bins
   DevBuf wt;                    // uint32[E+pad]: edge weights in layout order (weighted SSSP sweeps)
   DevBuf wt8;                   // uint8[E+pad]: the same as bytes, in place of wt when all fit
+  DevBuf gq;                    // uint32[2 + g1]: next workgroup task, ticket, each workgroup's task
   bool wt_valid = false;
   uint32_t nwt = 0, ngt = 0, nbt = 0, nsplit = 0, nslots = 0;
   uint32_t lds1 = 0, lds2 = 0;  // dynamic LDS bytes of the gather and bin kernels

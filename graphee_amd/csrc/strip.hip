@@ -91,6 +91,7 @@ struct StripArgs {
   const uint4 *wtask;
   const uint4 *gtask;
   uint32_t ngt;
+  uint32_t *gq;           // workgroup-task queue: [0] next task, [1] ticket, [2 + b] workgroup b's task
   float *msg;
   const uint16_t *seg_row;
   const uint32_t *runs;
@@ -582,7 +583,15 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
   const uint32_t wv = wave_id();
   float *mb = sv + W + wv * kRound;
   const uint32_t mbb = 4u * (W + wv * kRound);  // mb as an LDS address
-  for (uint32_t t = blockIdx.x; t < a.ngt; t += gridDim.x) {
+  // Workgroups take tasks from a queue (heaviest first), so they finish together; the task index
+  // reaches the whole workgroup through its slot in gq (the LDS is full).
+  volatile uint32_t *slot = a.gq + 2 + blockIdx.x;
+  for (;;) {
+    if (threadIdx.x == 0) *slot = atomicAdd(a.gq, 1u);
+    __syncthreads();
+    const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)*slot);
+    __syncthreads();  // every thread has read the slot before the next task is taken
+    if (t >= a.ngt) break;
     const uint4 gt = a.gtask[t];
     // a sweep skips strips none of whose vertices changed in the previous sweep; the bins apply
     // the messages of gathered strips only
@@ -641,6 +650,14 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
     for (uint32_t w = gt.y + wv; w < gt.z; w += kIterWaves) {
       const uint4 wt = a.wtask[w];
       strip_wave_task<MODE>(a, mb, mbb, wt.x, wt.y, wt.z);
+    }
+  }
+  // the last workgroup to leave resets the queue for the next launch
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(a.gq + 1, 1u) == gridDim.x - 1) {
+      atomicExch(a.gq, 0u);
+      atomicExch(a.gq + 1, 0u);
     }
   }
 }
@@ -1165,7 +1182,7 @@ bool ensure_strips(Engine &e) {
     for (uint32_t k = w0; k < (uint32_t)wt.size(); k += kGroupWaveTasks)
       gt.push_back(make_uint4(s, k, std::min<uint32_t>(k + kGroupWaveTasks, (uint32_t)wt.size()), 0u));
   }
-  // heaviest workgroup tasks first: the static round robin then balances well
+  // heaviest workgroup tasks first: the gather kernels' queue hands them out longest first
   std::stable_sort(gt.begin(), gt.end(), [&](const uint4 &p, const uint4 &q) {
     return wt[p.z - 1].y - wt[p.y].x > wt[q.z - 1].y - wt[q.y].x;
   });
@@ -1279,6 +1296,8 @@ bool ensure_strips(Engine &e) {
   const uint32_t per1 = (uint32_t)std::min<size_t>(2, std::max<size_t>(1, ldsmax / S.lds1));
   const uint32_t per2 = (uint32_t)std::min<size_t>(2, std::max<size_t>(1, ldsmax / (S.lds2 + 512)));
   S.g1 = std::max<uint32_t>(1, std::min<uint32_t>(S.ngt, cus * per1));
+  S.gq.alloc((size_t)(2 + S.g1) * 4);
+  GMP_HIP(hipMemsetAsync(S.gq.p, 0, (size_t)(2 + S.g1) * 4, e.stream));
   S.g2 = std::max<uint32_t>(1, std::min<uint32_t>(S.nbt, cus * per2));
   S.g3 = std::max<uint32_t>(1, std::min<uint32_t>(S.nsplit * ((R + kSplitRows - 1) / kSplitRows), cus * 4));
   // raise the dynamic-LDS ceiling where the runtime keeps one; a refusal leaves the launch
@@ -1310,6 +1329,7 @@ void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2)
   a.endb = S.endb.as<uint8_t>();
   a.wtask = S.wtask.as<uint4>();
   a.gtask = S.gtask.as<uint4>();
+  a.gq = S.gq.as<uint32_t>();
   a.ngt = S.ngt;
   a.msg = S.msg.as<float>();
   a.seg_row = S.seg_row.as<uint16_t>();
@@ -1450,6 +1470,7 @@ SweepRunPtr sweep_open(Engine &e, int kind, uint32_t *c, uint32_t *vstamp) {
   a.endb = S.endb.as<uint8_t>();
   a.wtask = S.wtask.as<uint4>();
   a.gtask = S.gtask.as<uint4>();
+  a.gq = S.gq.as<uint32_t>();
   a.ngt = S.ngt;
   a.msg = S.msg.as<float>();
   a.seg_row = S.seg_row.as<uint16_t>();
