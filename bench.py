@@ -565,6 +565,7 @@ def main():
         tuned = eng.autotune(iters=5).as_dict()
         tuned["seconds"] = round(time.perf_counter() - t0, 1)
         log(f"autotune: strip 2^{tuned['strip_log']}, bin 2^{tuned['bin_log']}, task 2^{tuned['bin_task_log']}, "
+            f"2^{tuned['strip_task_log']} wave tasks per strip task, "
             f"push frontiers <= V >> {tuned['push_log']}, "
             f"{tuned['gather_ms_before']:.3f} -> {tuned['gather_ms_after']:.3f} ms/iteration in {tuned['seconds']} s")
 

@@ -26,6 +26,7 @@ NOALLOC_MAX_VERTICES = 1 << 22
 (OPT_BATCH_SSSP_UNROLL, OPT_BATCH_SSSP_GRID, OPT_BATCH_PPR_PIECE, OPT_BATCH_PPR_RUN_ROWS, OPT_BATCH_PPR_RUN_EDGES,
  OPT_BATCH_PPR_LOADS, OPT_BATCH_PPR_GRID) = range(17, 24)
 OPT_MLP_NET, OPT_WINDOW_PREFETCH = 24, 25
+OPT_STRIP_TASK_LOG = 26
 
 # every entry point include/gmp.h declares
 SYMBOLS = (
@@ -105,7 +106,7 @@ class TuneResult(ctypes.Structure):
                 ("vec_min", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32), ("trials", ctypes.c_uint32),
                 ("gather_ms_before", ctypes.c_double), ("gather_ms_after", ctypes.c_double),
                 ("strip_log", ctypes.c_uint32), ("bin_log", ctypes.c_uint32), ("bin_task_log", ctypes.c_uint32),
-                ("push_log", ctypes.c_uint32), ("delta_pull_log", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("push_log", ctypes.c_uint32), ("delta_pull_log", ctypes.c_uint32), ("strip_task_log", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

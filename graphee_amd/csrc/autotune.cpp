@@ -56,18 +56,20 @@ uint32_t *field(Options &o, int k) {
     case 0: return &o.strip_log;
     case 1: return &o.bin_log;
     case 2: return &o.bin_task_log;
-    case 3: return &o.vec_min;
-    case 4: return &o.stream_cap;
-    case 5: return &o.chunk;
-    case 6: return &o.wg_per_cu;
+    case 3: return &o.strip_task_log;
+    case 4: return &o.vec_min;
+    case 5: return &o.stream_cap;
+    case 6: return &o.chunk;
+    case 7: return &o.wg_per_cu;
     default: return &o.row_cap;
   }
 }
-constexpr int kStripKnobs = 3;  // fields 0..2 shape the strip layout, the rest the row gather
+constexpr int kStripKnobs = 4;  // fields 0..3 shape the strip layout, the rest the row gather
 
 // Makes o the engine's options, dropping the strip layout when its shape changes.
 void apply(Engine &e, const Options &o) {
-  if (o.strip_log != e.opt.strip_log || o.bin_log != e.opt.bin_log || o.bin_task_log != e.opt.bin_task_log)
+  if (o.strip_log != e.opt.strip_log || o.bin_log != e.opt.bin_log || o.bin_task_log != e.opt.bin_task_log ||
+      o.strip_task_log != e.opt.strip_task_log)
     e.g.strips.valid = false;
   e.opt = o;
 }
@@ -81,6 +83,7 @@ void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
       {0, 13, 14, 15},                  // strip_log (0: row gather)
       {12, 13, 14},                     // bin_log
       {18, 19, 20, 21},                 // bin_task_log
+      {4, 5, 6, 7, 8},                  // strip_task_log
       {16, 32, 64, 128, 256, 512},      // vec_min
       {512, 1024, 2048, 4096},          // stream_cap
       {1024, 2048, 4096, 8192, 16384},  // chunk
@@ -164,6 +167,7 @@ void autotune(Engine &e, int32_t iters, gmp_tune_result *out) {
     out->gather_ms_after = best_ms;
     out->push_log = best.push_log;
     out->delta_pull_log = best.delta_pull_log;
+    out->strip_task_log = best.strip_task_log;
   }
 }
 
@@ -242,6 +246,7 @@ void tune_save(Engine &e, const std::string &path) {
     << ", \"row_cap\": " << e.opt.row_cap << ", \"chunk\": " << e.opt.chunk << ", \"vec_min\": " << e.opt.vec_min
     << ", \"wg_per_cu\": " << e.opt.wg_per_cu << ", \"strip_log\": " << e.opt.strip_log
     << ", \"bin_log\": " << e.opt.bin_log << ", \"bin_task_log\": " << e.opt.bin_task_log
+    << ", \"strip_task_log\": " << e.opt.strip_task_log
     << ", \"push_log\": " << e.opt.push_log << ", \"delta_pull_log\": " << e.opt.delta_pull_log
     << ", \"batch_sssp_unroll\": " << e.opt.bsssp_unroll << ", \"batch_sssp_grid\": " << e.opt.bsssp_grid
     << ", \"batch_ppr_piece\": " << e.opt.ppr_piece << ", \"batch_ppr_run_rows\": " << e.opt.ppr_run_rows
@@ -278,6 +283,7 @@ void tune_load(Engine &e, const std::string &path) {
   get("strip_log", 0, 15, o.strip_log);
   get("bin_log", 8, 14, o.bin_log);
   get("bin_task_log", 8, 22, o.bin_task_log);
+  get("strip_task_log", 0, 12, o.strip_task_log);
   get("push_log", 0, 31, o.push_log);
   get("delta_pull_log", 0, 31, o.delta_pull_log);
   get("batch_sssp_unroll", 2, 8, o.bsssp_unroll);

@@ -48,7 +48,6 @@ constexpr int kAhead = 4;                      // rounds of codes in flight per 
 constexpr int kUnroll = 16;                    // messages in flight per thread (bin kernel)
 constexpr uint32_t kChunk = 512;               // bin kernel: messages per chunk-index entry
 constexpr uint32_t kSplitRows = 1024;          // split apply: rows per block task
-constexpr uint32_t kGroupWaveTasks = 256;      // wave tasks per workgroup task (one staging)
 constexpr uint32_t kCodePad = kRound;          // codes read past the last edge stay in bounds
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr uint32_t kNever = 0xFFFFFFFFu;       // strip flag: no source changed yet
@@ -1168,6 +1167,7 @@ bool ensure_strips(Engine &e) {
   GMP_HIP(hipMemcpyAsync(hso.data(), soff.p, hso.size() * 4, hipMemcpyDeviceToHost, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
   std::vector<uint4> wt, gt;
+  const uint32_t gwt = 1u << e.opt.strip_task_log;  // wave tasks per workgroup task (one staging)
   for (uint32_t s = 0; s < ns; s++) {
     const uint32_t e0 = hso[s], e1 = hso[s + 1];
     if (e0 == e1) continue;
@@ -1179,8 +1179,8 @@ bool ensure_strips(Engine &e) {
       wt.push_back(make_uint4((uint32_t)x, (uint32_t)nx, 0u, s));
       x = nx;
     }
-    for (uint32_t k = w0; k < (uint32_t)wt.size(); k += kGroupWaveTasks)
-      gt.push_back(make_uint4(s, k, std::min<uint32_t>(k + kGroupWaveTasks, (uint32_t)wt.size()), 0u));
+    for (uint32_t k = w0; k < (uint32_t)wt.size(); k += gwt)
+      gt.push_back(make_uint4(s, k, std::min<uint32_t>(k + gwt, (uint32_t)wt.size()), 0u));
   }
   // heaviest workgroup tasks first: the gather kernels' queue hands them out longest first
   std::stable_sort(gt.begin(), gt.end(), [&](const uint4 &p, const uint4 &q) {

@@ -165,6 +165,9 @@ int gmp_destroy(gmp_engine *h);
                               window predictor's shard outputs pick the windows whose copies start
                               before the next sweep's activation is known, and those windows run
                               first; 0 (default) = exact activation only.  Results are the same. */
+#define GMP_OPT_STRIP_TASK_LOG 26 /* strip gathers (PageRank, sweeps, pull rounds): 2^k wave tasks of
+                              8192 edges per workgroup task, each task staging its strip once;
+                              0..12 (default 6).  Results do not depend on it. */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 
@@ -178,7 +181,7 @@ typedef struct gmp_tune_result {
   uint32_t stream_cap, row_cap, chunk, vec_min, wg_per_cu, trials;
   double gather_ms_before, gather_ms_after; /* mean PageRank iteration time, first and best */
   uint32_t strip_log, bin_log, bin_task_log, push_log;
-  uint32_t delta_pull_log, reserved;
+  uint32_t delta_pull_log, strip_task_log;
 } gmp_tune_result;
 int gmp_autotune(gmp_engine *h, int32_t iters, gmp_tune_result *out);
 /* Tunables as a flat JSON object, written after tuning and read at start-up. */

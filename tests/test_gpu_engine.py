@@ -445,16 +445,18 @@ def test_compressed_gather_is_bit_identical(engine, reorder):
         engine.set_option(gmp.OPT_STRIP_LOG, 15)
 
 
-STRIP_DEFAULTS = ((gmp.OPT_STRIP_LOG, 15), (gmp.OPT_BIN_LOG, 14), (gmp.OPT_BIN_TASK_LOG, 20))
+STRIP_DEFAULTS = ((gmp.OPT_STRIP_LOG, 15), (gmp.OPT_BIN_LOG, 14), (gmp.OPT_BIN_TASK_LOG, 20),
+                  (gmp.OPT_STRIP_TASK_LOG, 6))
 
 
-@pytest.mark.parametrize("strip_log,bin_log,task_log", [(0, 13, 20), (8, 8, 8), (11, 10, 10), (15, 13, 20),
-                                                        (15, 14, 8), (15, 14, 22)])
-def test_source_strips_keep_results(engine, strip_log, bin_log, task_log):
+@pytest.mark.parametrize("strip_log,bin_log,task_log,gtask_log", [(0, 13, 20, 6), (8, 8, 8, 0), (11, 10, 10, 12),
+                                                                  (15, 13, 20, 6), (15, 14, 8, 2), (15, 14, 22, 9)])
+def test_source_strips_keep_results(engine, strip_log, bin_log, task_log, gtask_log):
     """The source-strip iteration (LDS-staged strips, per-segment messages, fixed-point bin
-    sums, split bins) matches FP64 for every strip width, bin size and split size, and is
-    bit-reproducible; strip_log 0 selects the row gather."""
-    for k, v in zip((gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG), (strip_log, bin_log, task_log)):
+    sums, split bins) matches FP64 for every strip width, bin size, split size and workgroup
+    task size, and is bit-reproducible; strip_log 0 selects the row gather."""
+    for k, v in zip((gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG, gmp.OPT_STRIP_TASK_LOG),
+                    (strip_log, bin_log, task_log, gtask_log)):
         engine.set_option(k, v)
     try:
         for scale, kw in ((13, {}), (12, dict(undirected=True)), (11, dict(off64=True))):
@@ -488,9 +490,30 @@ def test_source_strips_keep_results(engine, strip_log, bin_log, task_log):
             engine.set_option(k, v)
 
 
+def test_strip_task_size_changes_nothing(engine):
+    """Workgroup tasks only decide which workgroup stages a strip and walks which wave tasks: the
+    messages, their sums and the ranks, CC labels and distances are the same bits for any size."""
+    g = gen(engine, 14, undirected=True)
+    out = []
+    engine.set_option(gmp.OPT_CC_UNION_FIND, 0)  # CC by strip sweeps
+    try:
+        for k in (0, 3, 6, 12):
+            engine.set_option(gmp.OPT_STRIP_TASK_LOG, k)
+            r, _ = engine.pagerank(max_iter=15)
+            lab, _ = engine.cc()
+            out.append((r, lab))
+    finally:
+        engine.set_option(gmp.OPT_STRIP_TASK_LOG, 6)
+        engine.set_option(gmp.OPT_CC_UNION_FIND, 1)
+    for r, lab in out[1:]:
+        np.testing.assert_array_equal(r.view(np.uint32), out[0][0].view(np.uint32))
+        np.testing.assert_array_equal(lab, out[0][1])
+    assert np.abs(out[0][0].astype(np.float64) - oracle.pr_f64(g["off"], g["idx"], g["outdeg"], max_iter=15)[0]).sum() < PR_L1_TOL
+
+
 def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
     keys = (gmp.OPT_STREAM_CAP, gmp.OPT_ROW_CAP, gmp.OPT_CHUNK, gmp.OPT_WG_PER_CU, gmp.OPT_VEC_MIN,
-            gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG, gmp.OPT_PUSH_LOG)
+            gmp.OPT_STRIP_LOG, gmp.OPT_BIN_LOG, gmp.OPT_BIN_TASK_LOG, gmp.OPT_PUSH_LOG, gmp.OPT_STRIP_TASK_LOG)
     saved = [engine.get_option(k) for k in keys]
     try:
         g = gen(engine, 14)
@@ -499,7 +522,7 @@ def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
         assert res.trials > 1 and res.gather_ms_after <= res.gather_ms_before
         assert [engine.get_option(k) for k in keys] == [res.stream_cap, res.row_cap, res.chunk, res.wg_per_cu,
                                                          res.vec_min, res.strip_log, res.bin_log, res.bin_task_log,
-                                                         res.push_log]
+                                                         res.push_log, res.strip_task_log]
         assert res.push_log in (0, 6, 8, 10, 12, 14)
         assert res.strip_log in (0, 13, 14, 15)
         r, _ = engine.pagerank(max_iter=20)
@@ -513,6 +536,7 @@ def test_autotune_keeps_results_and_round_trips(engine, tmp_path):
         assert engine.get_option(gmp.OPT_CHUNK) == res.chunk
         assert engine.get_option(gmp.OPT_STRIP_LOG) == res.strip_log
         assert engine.get_option(gmp.OPT_PUSH_LOG) == res.push_log
+        assert engine.get_option(gmp.OPT_STRIP_TASK_LOG) == res.strip_task_log
         p.write_text('{"chunk": 3}')
         with pytest.raises(gmp.GmpError) as ei:
             engine.tune_load(p)
