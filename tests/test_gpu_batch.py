@@ -45,6 +45,32 @@ def test_batch_sssp_matches_dijkstra(engine):
         np.testing.assert_array_equal(dists[i], oracle.dijkstra(ooff, oidx, ow, int(s)), err_msg=f"instance {i}")
 
 
+def test_batch_sssp_groups_of_many_instances(engine):
+    """Instances of one graph relax together in groups of up to 32 (vertex-major distances,
+    frontier entries with instance masks): 70 instances on one graph make groups of 32, 32 and 6,
+    some share a source (inside a group and across groups), a 3-instance group is padded to 4, a
+    graph has a single instance, and the instances are interleaved in the caller's order."""
+    parts = _batch(engine, SPECS)
+    rng = np.random.default_rng(11)
+    V1 = parts[1][0].size - 1
+    many = [(1, int(x)) for x in rng.integers(0, V1, 60)] + [(1, 7)] * 5 + [(1, 0)] * 5
+    rest = [(4, 3), (4, 3), (4, int(rng.integers(0, parts[4][0].size - 1))), (0, 1)]
+    inst = many + rest
+    order = rng.permutation(len(inst))
+    gsel = np.array([inst[i][0] for i in order], dtype=np.uint32)
+    srcs = np.array([inst[i][1] for i in order], dtype=np.uint32)
+    dists, st = engine.batch_sssp(gsel, srcs)
+    assert st.iterations > 0
+    ref = {}
+    for i, (g, s) in enumerate(zip(gsel, srcs)):
+        key = (int(g), int(s))
+        if key not in ref:
+            off, idx, w = parts[g]
+            ooff, oidx, ow = graphs.transpose(off.size - 1, off, idx, w)
+            ref[key] = oracle.dijkstra(ooff, oidx, ow, int(s))
+        np.testing.assert_array_equal(dists[i], ref[key], err_msg=f"instance {i} graph {g} source {s}")
+
+
 def test_batch_ppr_matches_fp64(engine):
     parts = _batch(engine, SPECS)
     rng = np.random.default_rng(2)
