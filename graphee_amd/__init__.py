@@ -26,7 +26,7 @@ NOALLOC_MAX_VERTICES = 1 << 22
 (OPT_BATCH_SSSP_UNROLL, OPT_BATCH_SSSP_GRID, OPT_BATCH_PPR_PIECE, OPT_BATCH_PPR_RUN_ROWS, OPT_BATCH_PPR_RUN_EDGES,
  OPT_BATCH_PPR_LOADS, OPT_BATCH_PPR_GRID) = range(17, 24)
 OPT_MLP_NET, OPT_WINDOW_PREFETCH = 24, 25
-OPT_STRIP_TASK_LOG = 26
+OPT_STRIP_TASK_LOG, OPT_SELECTIVE = 26, 27
 
 # every entry point include/gmp.h declares
 SYMBOLS = (

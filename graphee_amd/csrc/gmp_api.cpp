@@ -132,6 +132,10 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       if ((uint32_t)value != e->opt.bin_task_log) e->g.strips.valid = false;
       e->opt.bin_task_log = (uint32_t)value;
       break;
+    case GMP_OPT_SELECTIVE:
+      in(0, 1, "selective");
+      e->opt.selective = value != 0;
+      break;
     case GMP_OPT_STRIP_TASK_LOG:
       in(0, 12, "strip_task_log");
       if ((uint32_t)value != e->opt.strip_task_log) e->g.strips.valid = false;
@@ -222,6 +226,7 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_BIN_LOG: *value = e->opt.bin_log; break;
     case GMP_OPT_BIN_TASK_LOG: *value = e->opt.bin_task_log; break;
     case GMP_OPT_STRIP_TASK_LOG: *value = e->opt.strip_task_log; break;
+    case GMP_OPT_SELECTIVE: *value = e->opt.selective ? 1 : 0; break;
     case GMP_OPT_EDGE_BUDGET_MB: *value = (int64_t)e->opt.edge_budget_mb; break;
     case GMP_OPT_WINDOW_MB: *value = e->opt.window_mb; break;
     case GMP_OPT_WINDOW_PACKED: *value = e->opt.window_packed; break;

@@ -255,6 +255,7 @@ struct Options {
   uint32_t bin_log = 14;       // PageRank destination bins of up to 2^bin_log rows, 8..14
   uint32_t bin_task_log = 20;  // PageRank bins with more than 2^k messages are split, 8..22
   uint32_t strip_task_log = 6; // strip gathers: 2^k wave tasks per workgroup task (one staging), 0..12
+  bool selective = true;       // sweeps skip strips and windows whose sources did not change
   uint64_t edge_budget_mb = 0; // > 0: graphs whose edge data exceed it load semi-externally
   uint32_t window_mb = 64;     // semi-external windows: id bytes per window
   int window_packed = 1;       // semi-external windows: varint-packed ids

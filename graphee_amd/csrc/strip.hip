@@ -1525,6 +1525,8 @@ uint32_t sweep_step(SweepRun &r) {
   hipStream_t st = r.e.stream;
   a.sweep = ++r.last;
   GMP_HIP(hipMemsetAsync(r.changed.p, 0, 4, st));
+  // without selective scheduling every strip counts as changed in the sweep before
+  if (!r.e.opt.selective) sflag_fill_kernel<<<grid_for(r.nsall), kBlock, 0, st>>>(a.sflag, r.nsall, a.sweep - 1u);
   if (r.mode == kSweepCC) min_strip_gather_kernel<kSweepCC><<<S.g1, kIterBlock, S.lds1, st>>>(a);
   else if (r.mode == kSweepUnit) min_strip_gather_kernel<kSweepUnit><<<S.g1, kIterBlock, S.lds1, st>>>(a);
   else if (r.mode == kSweepWeighted8) min_strip_gather_kernel<kSweepWeighted8><<<S.g1, kIterBlock, S.lds1, st>>>(a);
@@ -1557,10 +1559,11 @@ void pull_round(SweepRun &r, const PullRound &p) {
     sflag_fill_kernel<<<grid_for(r.nsall), kBlock, 0, st>>>(a.gflag, r.nsall, 0u);
     GMP_CHECK_LAUNCH();
   }
-  // only the strips holding a frontier vertex are gathered
-  sflag_fill_kernel<<<grid_for(r.nsall), kBlock, 0, st>>>(a.sflag, r.nsall, kNever);
+  // only the strips holding a frontier vertex are gathered (every strip without selective
+  // scheduling: the others offer nothing, so the round relaxes the same edges)
+  sflag_fill_kernel<<<grid_for(r.nsall), kBlock, 0, st>>>(a.sflag, r.nsall, r.e.opt.selective ? kNever : r.last - 1);
   GMP_CHECK_LAUNCH();
-  if (p.nfront) {
+  if (p.nfront && r.e.opt.selective) {
     sflag_mark_kernel<<<grid_for(p.nfront), kBlock, 0, st>>>(p.front, p.nfront, S.wlog, r.last - 1, a.sflag);
     GMP_CHECK_LAUNCH();
   }

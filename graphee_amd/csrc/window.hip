@@ -399,16 +399,18 @@ void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st, g
     std::memcpy(prev_feat, feat, sizeof feat);
     have_prev = true;
     a.sweep = sweep;
-    // windows copied ahead first: run the active ones, drop the others
+    // windows copied ahead first: run the active ones, drop the others.  The sweeps end when no
+    // window is active (no value changed in the sweep before); without selective scheduling
+    // every window runs until then
     for (uint32_t w : pf) {
-      if (act[w] && nact) run_window(w, window_begin_slot(e, w, pf_slot[w], weights, true));
+      if ((act[w] || !e.opt.selective) && nact) run_window(w, window_begin_slot(e, w, pf_slot[w], weights, true));
       else window_release(e, w, pf_slot[w], weights);
     }
     if (nact == 0) break;
     for (uint32_t wi = 0; wi < nwin; wi++) {
       const uint32_t w = order[wi];
       if (pf_slot[w] >= 0) continue;
-      if (!act[w]) {
+      if (!act[w] && e.opt.selective) {
         s.windows_skipped++;
         continue;
       }

@@ -168,6 +168,11 @@ int gmp_destroy(gmp_engine *h);
 #define GMP_OPT_STRIP_TASK_LOG 26 /* strip gathers (PageRank, sweeps, pull rounds): 2^k wave tasks of
                               8192 edges per workgroup task, each task staging its strip once;
                               0..12 (default 6).  Results do not depend on it. */
+#define GMP_OPT_SELECTIVE 27 /* 1 (default): sweeps (CC, Bellman-Ford, delta-stepping pulls, and the
+                              semi-external windows of every min app) skip the strips and windows
+                              none of whose sources changed in the sweep before -- GraphMP's
+                              selective scheduling (GraphMP/system/GraphPS.h:318-368), exact; 0:
+                              every strip and window every sweep.  Results are the same. */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 

@@ -490,6 +490,42 @@ def test_source_strips_keep_results(engine, strip_log, bin_log, task_log, gtask_
             engine.set_option(k, v)
 
 
+def test_selective_scheduling_changes_nothing(engine):
+    """Schedule independence: with GraphMP's selective scheduling off (every strip in every sweep)
+    CC labels, Bellman-Ford and delta-stepping distances (pull rounds on every round), sweep
+    counts and changed counts equal the selective runs', which gather fewer edges."""
+    keys = (gmp.OPT_SELECTIVE, gmp.OPT_CC_UNION_FIND, gmp.OPT_PUSH_LOG, gmp.OPT_DELTA_PULL_LOG, gmp.OPT_STRIP_LOG)
+    saved = [engine.get_option(k) for k in keys]
+    try:
+        engine.set_option(gmp.OPT_STRIP_LOG, 10)  # 16 strips at scale 14, so there is something to skip
+        engine.set_option(gmp.OPT_CC_UNION_FIND, 0)
+        engine.set_option(gmp.OPT_PUSH_LOG, 0)  # sweeps only
+        engine.set_option(gmp.OPT_DELTA_PULL_LOG, 31)  # every delta-stepping round pulls
+        runs = {}
+        for sel in (1, 0):
+            engine.set_option(gmp.OPT_SELECTIVE, sel)
+            g = gen(engine, 14, undirected=True)
+            lab, stc = engine.cc()
+            g = gen(engine, 14, weighted=True)
+            src = engine.pick_source(5)
+            d, stb = engine.sssp(src, method=gmp.SSSP_BF)
+            dd, std = engine.sssp(src, method=gmp.SSSP_DELTA, delta=64)
+            runs[sel] = (lab, stc, d, stb, dd, std, g, src)
+        (l1, c1, d1, b1, e1, s1, g, src), (l0, c0, d0, b0, e0, s0, _, _) = runs[1], runs[0]
+        np.testing.assert_array_equal(l1, l0)
+        np.testing.assert_array_equal(d1, d0)
+        np.testing.assert_array_equal(e1, e0)
+        assert (c1.iterations, c1.changed) == (c0.iterations, c0.changed)
+        assert (b1.iterations, b1.changed) == (b0.iterations, b0.changed)
+        assert c1.edges < c0.edges and b1.edges < b0.edges
+        V = g["outdeg"].size
+        ooff, oidx, ow = graphs.transpose(V, g["off"], g["idx"], g["w"])
+        np.testing.assert_array_equal(d1, oracle.dijkstra(ooff, oidx, ow, src))
+    finally:
+        for k, v in zip(keys, saved):
+            engine.set_option(k, v)
+
+
 def test_strip_task_size_changes_nothing(engine):
     """Workgroup tasks only decide which workgroup stages a strip and walks which wave tasks: the
     messages, their sums and the ranks, CC labels and distances are the same bits for any size."""

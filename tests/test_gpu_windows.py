@@ -155,6 +155,35 @@ def test_windows_skip_idle_windows_exactly(packed):
         assert s.windows()["windows_skipped"] > w2["windows_skipped"]
 
 
+def test_windows_selective_off_runs_every_window():
+    """With selective scheduling off every window runs in every sweep and step, and the results,
+    stop steps and changed counts equal the selective runs'."""
+    V = 1 << 20
+    off, idx = _local_graph(V, 32)
+    w = graphs.edge_weights(off, idx, 9)
+    with _engine(1, 1, True) as s:
+        s.load_csr(off, idx, weights=w)
+        nwin = s.windows()["windows"]
+        out = {}
+        for sel in (1, 0):
+            s.set_option(gmp.OPT_SELECTIVE, sel)
+            w0 = s.windows()
+            lab, stc = s.cc()
+            d, _ = s.sssp(V // 2 + 3)
+            m, v, sa = s.run_app(gmp.APP_CC, np.arange(V, dtype=np.int32), 1000)
+            w1 = s.windows()
+            out[sel] = (lab, stc, d, m, v, sa, w1["windows_skipped"] - w0["windows_skipped"])
+        s.set_option(gmp.OPT_SELECTIVE, 1)
+    (l1, c1, d1, m1, v1, a1, k1), (l0, c0, d0, m0, v0, a0, k0) = out[1], out[0]
+    np.testing.assert_array_equal(l1, l0)
+    np.testing.assert_array_equal(d1, d0)
+    np.testing.assert_array_equal(m1, m0)
+    np.testing.assert_array_equal(v1, v0)
+    assert (a1.iterations, a1.changed) == (a0.iterations, a0.changed)
+    assert c1.iterations == c0.iterations
+    assert k1 > 0 and k0 == 0 and nwin >= 3
+
+
 @pytest.mark.parametrize("packed", [False, True])
 def test_windows_union_find_cc_streams_what_it_needs(tmp_path, packed):
     """CC of a symmetric semi-external graph by sampled union-find: two large components (the
