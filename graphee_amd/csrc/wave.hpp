@@ -107,13 +107,6 @@ __device__ inline uint32_t dpp_u32(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xF, true);
 }
 
-template <int CTRL, int ROWS = 0xF>
-__device__ inline double dpp_f64(double x) {
-  const uint64_t b = (uint64_t)__double_as_longlong(x);
-  const uint64_t lo = dpp_u32<CTRL, ROWS>((uint32_t)b), hi = dpp_u32<CTRL, ROWS>((uint32_t)(b >> 32));
-  return __longlong_as_double((long long)((hi << 32) | lo));
-}
-
 constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
 constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
 
@@ -192,11 +185,6 @@ __device__ inline uint32_t read_lane_u32(uint32_t x, int lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
 }
 
-__device__ inline double read_lane_f64(double x, int lane) {
-  const uint64_t b = (uint64_t)__double_as_longlong(x);
-  const uint64_t lo = read_lane_u32((uint32_t)b, lane), hi = read_lane_u32((uint32_t)(b >> 32), lane);
-  return __longlong_as_double((long long)((hi << 32) | lo));
-}
 
 // Wave-aggregated append: lanes with `pred` get consecutive slots of `list`, with one
 // atomic per wave on `count` (ballot + popcount prefix).
