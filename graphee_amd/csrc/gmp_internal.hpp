@@ -110,7 +110,9 @@ struct StripCsr {
   uint64_t nseg = 0;            // segments = messages per iteration
   DevBuf code;                  // uint16[E+pad]: source offset in its strip
   DevBuf endb;                  // uint8[(E+pad)/8]: bit k of byte i ends the segment of edge 8 i + k
-  DevBuf wtask;                 // uint32x4[nwt]: {first edge, end edge, first segment, strip}
+  DevBuf sbase;                 // uint64[ns]: the strip's first edge rounded down to a multiple of 8
+  DevBuf wtask;                 // uint32x4[nwt]: {first edge, end edge (both from the strip's sbase),
+                                //   first segment, strip}
   DevBuf gtask;                 // uint32x4[ngt]: {strip, first wave task, end wave task, 0}
   DevBuf seg_row;               // uint16[nseg]: destination offset inside its bin
   DevBuf runs;                  // uint32[(nbins+1)*ns]: [b*ns+s] = first segment of strip s in bin >= b

@@ -86,6 +86,7 @@ struct StripArgs {
   uint32_t seed;
   // layout
   const uint16_t *code;   // source offsets in the strip
+  const uint64_t *sbase;  // per strip: the edge its wave tasks' edge offsets count from
   const uint8_t *endb;    // bit k of byte i: edge 8 i + k ends its segment
   const uint4 *wtask;
   const uint4 *gtask;
@@ -135,11 +136,11 @@ __global__ void strip_keys_kernel(const uint32_t *idx, const uint32_t *rowid, ui
 }
 
 // off[s] = first edge whose strip >= s, s in [0, ns]
-__global__ void strip_offsets_kernel(const uint64_t *keys, uint64_t E, uint32_t sh, uint32_t ns, uint32_t *off) {
+__global__ void strip_offsets_kernel(const uint64_t *keys, uint64_t E, uint32_t sh, uint32_t ns, uint64_t *off) {
   GS64(i, E + 1) {
     const int64_t s = i < E ? (int64_t)(keys[i] >> sh) : (int64_t)ns;
     const int64_t sp = i > 0 ? (int64_t)(keys[i - 1] >> sh) : -1;
-    for (int64_t q = sp + 1; q <= s; q++) off[q] = (uint32_t)i;
+    for (int64_t q = sp + 1; q <= s; q++) off[q] = i;
   }
 }
 
@@ -154,9 +155,9 @@ __global__ void codes_kernel(const uint64_t *keys, uint64_t E, uint32_t wlog, ui
 }
 
 // every wave task starts a segment: end one at the edge before it
-__global__ void task_ends_kernel(const uint4 *wt, uint32_t n, uint16_t *code) {
+__global__ void task_ends_kernel(const uint4 *wt, const uint64_t *sbase, uint32_t n, uint16_t *code) {
   GS64(t, n) {
-    const uint32_t e0 = wt[t].x;
+    const uint64_t e0 = sbase[wt[t].w] + wt[t].x;
     if (e0 > 0) code[e0 - 1] |= (uint16_t)0x8000u;
   }
 }
@@ -165,8 +166,8 @@ __global__ void end_flags_kernel(const uint16_t *code, uint64_t E, uint32_t *fla
   GS64(j, E) flag[j] = code[j] >> 15;
 }
 
-__global__ void task_seg_kernel(uint4 *wt, uint32_t n, const uint32_t *segid) {
-  GS64(t, n) wt[t].z = segid[wt[t].x];
+__global__ void task_seg_kernel(uint4 *wt, const uint64_t *sbase, uint32_t n, const uint32_t *segid) {
+  GS64(t, n) wt[t].z = segid[sbase[wt[t].w] + wt[t].x];
 }
 
 // Moves the end bits out of the codes: endb[i] bit k = bit 15 of code[8 i + k], which is then
@@ -199,7 +200,7 @@ __global__ void seg_rows_kernel(const uint64_t *keys, const uint16_t *code, cons
 }
 
 // sseg[s] = first segment of strip s (an empty strip gets its successor's), sseg[ns] = nseg
-__global__ void strip_seg_kernel(const uint32_t *soff, const uint32_t *segid, uint32_t ns, uint64_t E,
+__global__ void strip_seg_kernel(const uint64_t *soff, const uint32_t *segid, uint32_t ns, uint64_t E,
                                  uint32_t nseg, uint32_t *sseg) {
   GS64(s, (uint64_t)ns + 1) sseg[s] = (s < ns && soff[s] < E) ? segid[soff[s]] : nseg;
 }
@@ -470,15 +471,15 @@ __device__ inline void min_round(MsgBuf &B, uint32_t mbb, u32x4 cv, uint32_t eb,
 // Sweep messages of edges [e0, e1) whose first segment has index seg (every segment ends
 // inside); loads as in pr_wave_task, weights with them.
 template <int MODE>
-__device__ inline void min_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint32_t e0, uint32_t e1,
-                                     uint32_t seg) {
+__device__ inline void min_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint64_t sb, uint32_t e0,
+                                     uint32_t e1, uint32_t seg) {
   const uint32_t lane = lane_id();
   const uint32_t b0 = e0 & ~7u;
   const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
-  const u32x4 *cb = reinterpret_cast<const u32x4 *>(a.code + b0);
-  const uint8_t *ebb = a.endb + b0 / 8;
-  const u32x4 *wb = reinterpret_cast<const u32x4 *>(a.wt + b0);   // two words of 4 weights per lane
-  const u32x2 *w8b = reinterpret_cast<const u32x2 *>(a.wt8 + b0);  // 8 byte weights per lane
+  const u32x4 *cb = reinterpret_cast<const u32x4 *>(a.code + sb + b0);
+  const uint8_t *ebb = a.endb + (sb + b0) / 8;
+  const u32x4 *wb = reinterpret_cast<const u32x4 *>(a.wt + sb + b0);   // two words of 4 weights per lane
+  const u32x2 *w8b = reinterpret_cast<const u32x2 *>(a.wt8 + sb + b0);  // 8 byte weights per lane
   uint32_t carry = 0;  // the open segment's value
   MsgBuf B{a.msg, mb, seg, 0u};
   u32x4 buf[kAhead], wa[kAhead], wc[kAhead];
@@ -522,13 +523,13 @@ __device__ inline void min_wave_task(const StripArgs &a, float *mb, uint32_t mbb
 // PageRank messages of edges [e0, e1) whose first segment has index seg; every segment ends
 // inside.  The loads of codes and end bits run kAhead rounds ahead of their use; a round's loads
 // are whole (the layout's padding keeps those of the last round in bounds).
-__device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint32_t e0, uint32_t e1,
-                                    uint32_t seg) {
+__device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint64_t sb, uint32_t e0,
+                                    uint32_t e1, uint32_t seg) {
   const uint32_t lane = lane_id();
   const uint32_t b0 = e0 & ~7u;
   const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
-  const u32x4 *cb = reinterpret_cast<const u32x4 *>(a.code + b0);  // a round is kWave words of 8 codes
-  const uint8_t *ebb = a.endb + b0 / 8;
+  const u32x4 *cb = reinterpret_cast<const u32x4 *>(a.code + sb + b0);  // a round is kWave words of 8 codes
+  const uint8_t *ebb = a.endb + (sb + b0) / 8;
   double carry = 0.0;  // the open segment's sum
   MsgBuf B{a.msg, mb, seg, 0u};
   u32x4 buf[kAhead];
@@ -567,10 +568,10 @@ __device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb,
 
 // Messages of edges [e0, e1) whose first segment has index seg.
 template <int MODE>
-__device__ inline void strip_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint32_t e0, uint32_t e1,
-                                       uint32_t seg) {
-  if constexpr (MODE == kSweepPR) pr_wave_task(a, mb, mbb, e0, e1, seg);
-  else min_wave_task<MODE>(a, mb, mbb, e0, e1, seg);
+__device__ inline void strip_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint64_t sb, uint32_t e0,
+                                       uint32_t e1, uint32_t seg) {
+  if constexpr (MODE == kSweepPR) pr_wave_task(a, mb, mbb, sb, e0, e1, seg);
+  else min_wave_task<MODE>(a, mb, mbb, sb, e0, e1, seg);
 }
 
 // One strip's values (contributions, or complemented labels for CC) staged in LDS, then the
@@ -646,9 +647,10 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
       for (uint32_t i = threadIdx.x; i < n; i += kIterBlock) reinterpret_cast<uint32_t *>(sv)[i] = vals[v0 + i];
     }
     __syncthreads();
+    const uint64_t sb = a.sbase[gt.x];
     for (uint32_t w = gt.y + wv; w < gt.z; w += kIterWaves) {
       const uint4 wt = a.wtask[w];
-      strip_wave_task<MODE>(a, mb, mbb, wt.x, wt.y, wt.z);
+      strip_wave_task<MODE>(a, mb, mbb, sb, wt.x, wt.y, wt.z);
     }
   }
   // the last workgroup to leave resets the queue for the next launch
@@ -1115,7 +1117,7 @@ bool ensure_strips(Engine &e) {
   Graph &g = e.g;
   StripCsr &S = g.strips;
   const uint32_t wlog = e.opt.strip_log;
-  if (wlog == 0 || g.E == 0 || g.E >= 0xFFFFFFFFull - kCodePad) return false;
+  if (wlog == 0 || g.E == 0) return false;
   if (S.valid) return true;
   const uint32_t V = g.V;
   const uint64_t E = g.E;
@@ -1160,23 +1162,27 @@ bool ensure_strips(Engine &e) {
 
   // strip edge ranges -> wave tasks and workgroup tasks
   DevBuf soff;
-  soff.alloc(((size_t)ns + 1) * 4);
-  strip_offsets_kernel<<<grid_for(E + 1), kBlock, 0, e.stream>>>(keys, E, sh, ns, soff.as<uint32_t>());
+  soff.alloc(((size_t)ns + 1) * 8);
+  strip_offsets_kernel<<<grid_for(E + 1), kBlock, 0, e.stream>>>(keys, E, sh, ns, soff.as<uint64_t>());
   GMP_CHECK_LAUNCH();
-  std::vector<uint32_t> hso((size_t)ns + 1);
-  GMP_HIP(hipMemcpyAsync(hso.data(), soff.p, hso.size() * 4, hipMemcpyDeviceToHost, e.stream));
+  std::vector<uint64_t> hso((size_t)ns + 1), hsb(std::max<uint32_t>(ns, 1), 0);
+  GMP_HIP(hipMemcpyAsync(hso.data(), soff.p, hso.size() * 8, hipMemcpyDeviceToHost, e.stream));
   GMP_HIP(hipStreamSynchronize(e.stream));
   std::vector<uint4> wt, gt;
   const uint32_t gwt = 1u << e.opt.strip_task_log;  // wave tasks per workgroup task (one staging)
   for (uint32_t s = 0; s < ns; s++) {
-    const uint32_t e0 = hso[s], e1 = hso[s + 1];
+    const uint64_t e0 = hso[s], e1 = hso[s + 1];
+    // wave tasks count edges from the strip's base (a multiple of 8, for whole 16-byte code
+    // loads), so a graph may have any number of edges while a strip stays under 2^32
+    hsb[s] = e0 & ~7ull;
     if (e0 == e1) continue;
+    if (e1 - hsb[s] >= 0xFFFFFFFFull - kCodePad) return false;
     const uint32_t w0 = (uint32_t)wt.size();
     // task boundaries inside a strip fall on multiples of 8, so only the strip's own ends can
     // cut a lane's 8 edges
     for (uint64_t x = e0; x < e1;) {
       const uint64_t nx = std::min<uint64_t>(((x & ~7ull) + kWaveTaskEdges), e1);
-      wt.push_back(make_uint4((uint32_t)x, (uint32_t)nx, 0u, s));
+      wt.push_back(make_uint4((uint32_t)(x - hsb[s]), (uint32_t)(nx - hsb[s]), 0u, s));
       x = nx;
     }
     for (uint32_t k = w0; k < (uint32_t)wt.size(); k += gwt)
@@ -1191,6 +1197,8 @@ bool ensure_strips(Engine &e) {
   S.wtask.alloc((size_t)S.nwt * sizeof(uint4));
   S.gtask.alloc((size_t)S.ngt * sizeof(uint4));
   GMP_HIP(hipMemcpyAsync(S.wtask.p, wt.data(), (size_t)S.nwt * sizeof(uint4), hipMemcpyHostToDevice, e.stream));
+  S.sbase.alloc(hsb.size() * 8);
+  GMP_HIP(hipMemcpyAsync(S.sbase.p, hsb.data(), hsb.size() * 8, hipMemcpyHostToDevice, e.stream));
   GMP_HIP(hipMemcpyAsync(S.gtask.p, gt.data(), (size_t)S.ngt * sizeof(uint4), hipMemcpyHostToDevice, e.stream));
 
   // codes and segment ids
@@ -1198,13 +1206,28 @@ bool ensure_strips(Engine &e) {
   GMP_HIP(hipMemsetAsync(S.code.p, 0, (E + kCodePad) * 2, e.stream));
   codes_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(keys, E, wlog, S.code.as<uint16_t>());
   GMP_CHECK_LAUNCH();
-  task_ends_kernel<<<grid_for(S.nwt), kBlock, 0, e.stream>>>(S.wtask.as<uint4>(), S.nwt, S.code.as<uint16_t>());
+  task_ends_kernel<<<grid_for(S.nwt), kBlock, 0, e.stream>>>(S.wtask.as<uint4>(), S.sbase.as<uint64_t>(), S.nwt,
+                                                             S.code.as<uint16_t>());
   GMP_CHECK_LAUNCH();
   DevBuf flag, segid;
   flag.alloc(E * 4);
   segid.alloc(E * 4);
   end_flags_kernel<<<grid_for(E), kBlock, 0, e.stream>>>(S.code.as<uint16_t>(), E, flag.as<uint32_t>());
   GMP_CHECK_LAUNCH();
+  {  // segment ids and message slots are 32-bit: count the segments in 64 bits first
+    DevBuf total, rtmp;
+    total.alloc(8);
+    size_t rb = 0;
+    GMP_HIP(rocprim::reduce(nullptr, rb, flag.as<uint32_t>(), total.as<uint64_t>(), (uint64_t)0, E,
+                            rocprim::plus<uint64_t>(), e.stream));
+    rtmp.alloc(rb);
+    GMP_HIP(rocprim::reduce(rtmp.p, rb, flag.as<uint32_t>(), total.as<uint64_t>(), (uint64_t)0, E,
+                            rocprim::plus<uint64_t>(), e.stream));
+    uint64_t hseg = 0;
+    GMP_HIP(hipMemcpyAsync(&hseg, total.p, 8, hipMemcpyDeviceToHost, e.stream));
+    GMP_HIP(hipStreamSynchronize(e.stream));
+    if (hseg >= 0xFFFFFFFFull) return false;
+  }
   size_t tb2 = 0;
   GMP_HIP(rocprim::exclusive_scan(nullptr, tb2, flag.as<uint32_t>(), segid.as<uint32_t>(), 0u, E,
                                   rocprim::plus<uint32_t>(), e.stream));
@@ -1219,7 +1242,8 @@ bool ensure_strips(Engine &e) {
   tmp2.release();
   const uint32_t nseg = lastseg + 1;  // the last edge always ends a segment
   S.nseg = nseg;
-  task_seg_kernel<<<grid_for(S.nwt), kBlock, 0, e.stream>>>(S.wtask.as<uint4>(), S.nwt, segid.as<uint32_t>());
+  task_seg_kernel<<<grid_for(S.nwt), kBlock, 0, e.stream>>>(S.wtask.as<uint4>(), S.sbase.as<uint64_t>(), S.nwt,
+                                                            segid.as<uint32_t>());
   GMP_CHECK_LAUNCH();
 
   // segment destinations and the (bin, strip) run table
@@ -1230,7 +1254,7 @@ bool ensure_strips(Engine &e) {
                                                         b, R - 1, S.seg_row.as<uint16_t>(), segfull.as<uint32_t>());
   GMP_CHECK_LAUNCH();
   sseg.alloc(((size_t)ns + 1) * 4);
-  strip_seg_kernel<<<grid_for((uint64_t)ns + 1), kBlock, 0, e.stream>>>(soff.as<uint32_t>(), segid.as<uint32_t>(), ns,
+  strip_seg_kernel<<<grid_for((uint64_t)ns + 1), kBlock, 0, e.stream>>>(soff.as<uint64_t>(), segid.as<uint32_t>(), ns,
                                                                          E, nseg, sseg.as<uint32_t>());
   GMP_CHECK_LAUNCH();
   // the end bits leave the codes for a byte per 8 edges (the padding's bytes stay zero)
@@ -1328,6 +1352,7 @@ void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2)
   a.code = S.code.as<uint16_t>();
   a.endb = S.endb.as<uint8_t>();
   a.wtask = S.wtask.as<uint4>();
+  a.sbase = S.sbase.as<uint64_t>();
   a.gtask = S.gtask.as<uint4>();
   a.gq = S.gq.as<uint32_t>();
   a.ngt = S.ngt;
@@ -1469,6 +1494,7 @@ SweepRunPtr sweep_open(Engine &e, int kind, uint32_t *c, uint32_t *vstamp) {
   a.code = S.code.as<uint16_t>();
   a.endb = S.endb.as<uint8_t>();
   a.wtask = S.wtask.as<uint4>();
+  a.sbase = S.sbase.as<uint64_t>();
   a.gtask = S.gtask.as<uint4>();
   a.gq = S.gq.as<uint32_t>();
   a.ngt = S.ngt;
