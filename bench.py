@@ -242,6 +242,48 @@ def run_pagerank_s20(eng, threads):
     return out
 
 
+def run_big(scale=28, ef=17, verify=True, threads=16):
+    """PageRank on R-MAT s28 ef17 (4.5e9 edges, past 2^32) resident in HBM: the strip layout
+    (edge offsets counted from per-strip bases) against the row gather on the same graph, and
+    the FP64 one-step bound of the converged strip ranks."""
+    import graphee_amd as gmp
+    e = gmp.Engine(0)
+    try:
+        t0 = time.perf_counter()
+        e.generate_rmat(scale, ef, 7000 + scale, num_shards=16)
+        V, E, _, _ = e.info()
+        gen_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        e.pagerank(max_iter=3, want_ranks=False)  # builds the strip layout
+        build_s = time.perf_counter() - t0
+        r1, st1 = e.pagerank(max_iter=100, tol=1e-7)
+        ms1 = st1.ms_kernel / st1.launches
+        e.set_option(gmp.OPT_STRIP_LOG, 0)
+        e.pagerank(max_iter=3, want_ranks=False)
+        r0, st0 = e.pagerank(max_iter=st1.iterations, tol=0.0)
+        ms0 = st0.ms_kernel / st0.launches
+        out = {"workload": f"pagerank-rmat-s{scale}-ef{ef}-tol1e-07", "vertices": V, "edges": E,
+               "edges_over_2e32": round(E / 2**32, 3), "generated_s": round(gen_s, 1),
+               "strip_layout_build_s": round(build_s, 1), "iterations": st1.iterations,
+               "strip_ms_per_iteration": round(ms1, 3), "strip_gteps": round(E / ms1 / 1e6, 1),
+               "strip_index_bytes_per_edge": round(st1.index_bytes / E, 3),
+               "row_gather_ms_per_iteration": round(ms0, 3), "strip_speedup": round(ms0 / ms1, 2),
+               "l1_strip_vs_row_gather": float(np.abs(r1.astype(np.float64) - r0).sum())}
+        if verify:
+            import oracle
+            g = e.download()
+            l1, _ = oracle.pr_step_residual(g["off"], g["idx"], g["outdeg"], r1, threads=threads)
+            out["fp64_step_residual_l1"] = l1
+            out["fp64_l1_error_bound"] = l1 / 0.15
+            del g
+        log(f"pagerank s{scale} ef{ef} ({E} edges = {E / 2**32:.2f} x 2^32): strips {ms1:.2f} ms/iteration "
+            f"({E / ms1 / 1e6:.0f} GTEPS), row gather {ms0:.2f} ms, L1 between them {out['l1_strip_vs_row_gather']:.2e}"
+            + (f", FP64 L1 bound {out['fp64_l1_error_bound']:.2e}" if verify else ""))
+        return out
+    finally:
+        e.close()
+
+
 def run_sssp_s24(eng, threads):
     """SSSP on directed R-MAT s24 ef16 with uint32 weights in [1,255] from a seeded source of
     out-degree > 0: Bellman-Ford (pushes + strip sweeps, GraphMP's selective scheduling) and
@@ -531,6 +573,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-instance runs")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-schedule comparison")
     ap.add_argument("--no-semi", action="store_true", help="skip the semi-external (host-resident edges) runs")
+    ap.add_argument("--no-big", action="store_true", help="skip PageRank on R-MAT s28 ef17 (4.5e9 edges)")
     ap.add_argument("--learn-scale", type=int, default=27, help="R-MAT scale of the learned-SSSP evaluation")
     ap.add_argument("--strip-log", type=int, default=15,
                     help="PageRank source strips of 2^k ids (0: the row gather over the in-CSR)")
@@ -784,6 +827,8 @@ def main():
         result["extra"]["learned_sssp"] = run_learned(eng, args.learn_scale)
 
     eng.close()
+    if not args.no_extra and not args.no_big:
+        result["extra"]["pagerank_beyond_2e32_edges"] = run_big(verify=not args.no_cpu, threads=cpu_threads())
     if semi_ref is not None:
         # one engine in the process, like a semi-external user: its stream and two copy streams fit
         # the device's hardware queues (GPU_MAX_HW_QUEUES, 4 by default), so copies overlap gathers
