@@ -27,6 +27,7 @@ NOALLOC_MAX_VERTICES = 1 << 22
  OPT_BATCH_PPR_LOADS, OPT_BATCH_PPR_GRID) = range(17, 24)
 OPT_MLP_NET, OPT_WINDOW_PREFETCH = 24, 25
 OPT_STRIP_TASK_LOG, OPT_SELECTIVE = 26, 27
+OPT_BATCH_SSSP_SCAN_LOG, OPT_BATCH_SSSP_HUB_LOG = 28, 29
 
 # every entry point include/gmp.h declares
 SYMBOLS = (
@@ -117,7 +118,8 @@ class BatchTuneResult(ctypes.Structure):
                 ("ppr_piece", ctypes.c_uint32), ("ppr_run_rows", ctypes.c_uint32), ("ppr_run_edges", ctypes.c_uint32),
                 ("ppr_loads", ctypes.c_uint32), ("ppr_grid_per_cu", ctypes.c_uint32), ("trials", ctypes.c_uint32),
                 ("sssp_ms_before", ctypes.c_double), ("sssp_ms_after", ctypes.c_double),
-                ("ppr_ms_before", ctypes.c_double), ("ppr_ms_after", ctypes.c_double)]
+                ("ppr_ms_before", ctypes.c_double), ("ppr_ms_after", ctypes.c_double),
+                ("sssp_scan_log", ctypes.c_uint32), ("sssp_hub_log", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

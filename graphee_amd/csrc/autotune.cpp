@@ -215,7 +215,11 @@ void batch_autotune(Engine &e, uint32_t ns, const uint32_t *sgraph, const uint32
   };
   Options &o = e.opt;
   std::pair<double, double> ts{0.0, 0.0}, tp{0.0, 0.0};
-  if (ns) ts = descend(true, {{&o.bsssp_unroll, {2, 4, 8}}, {&o.bsssp_grid, {1, 2, 4, 8, 16}}});
+  if (ns)
+    ts = descend(true, {{&o.bsssp_scan_log, {0, 4, 6, 8, 10, 12}},
+                        {&o.bsssp_hub_log, {0, 8, 10, 11, 12, 14}},
+                        {&o.bsssp_unroll, {2, 4, 8}},
+                        {&o.bsssp_grid, {1, 2, 4, 8, 16}}});
   if (np)
     tp = descend(false, {{&o.ppr_piece, {256, 512, 1024, 2048, 4096}},
                          {&o.ppr_run_rows, {8, 16, 32, 64, 128}},
@@ -226,6 +230,8 @@ void batch_autotune(Engine &e, uint32_t ns, const uint32_t *sgraph, const uint32
     std::memset(out, 0, sizeof *out);
     out->sssp_unroll = o.bsssp_unroll;
     out->sssp_grid_per_cu = o.bsssp_grid;
+    out->sssp_scan_log = o.bsssp_scan_log;
+    out->sssp_hub_log = o.bsssp_hub_log;
     out->ppr_piece = o.ppr_piece;
     out->ppr_run_rows = o.ppr_run_rows;
     out->ppr_run_edges = o.ppr_run_edges;
@@ -249,6 +255,7 @@ void tune_save(Engine &e, const std::string &path) {
     << ", \"strip_task_log\": " << e.opt.strip_task_log
     << ", \"push_log\": " << e.opt.push_log << ", \"delta_pull_log\": " << e.opt.delta_pull_log
     << ", \"batch_sssp_unroll\": " << e.opt.bsssp_unroll << ", \"batch_sssp_grid\": " << e.opt.bsssp_grid
+    << ", \"batch_sssp_scan_log\": " << e.opt.bsssp_scan_log << ", \"batch_sssp_hub_log\": " << e.opt.bsssp_hub_log
     << ", \"batch_ppr_piece\": " << e.opt.ppr_piece << ", \"batch_ppr_run_rows\": " << e.opt.ppr_run_rows
     << ", \"batch_ppr_run_edges\": " << e.opt.ppr_run_edges << ", \"batch_ppr_loads\": " << e.opt.ppr_loads
     << ", \"batch_ppr_grid\": " << e.opt.ppr_grid << "}\n";
@@ -288,6 +295,8 @@ void tune_load(Engine &e, const std::string &path) {
   get("delta_pull_log", 0, 31, o.delta_pull_log);
   get("batch_sssp_unroll", 2, 8, o.bsssp_unroll);
   get("batch_sssp_grid", 1, 64, o.bsssp_grid);
+  get("batch_sssp_scan_log", 0, 31, o.bsssp_scan_log);
+  get("batch_sssp_hub_log", 0, 24, o.bsssp_hub_log);
   get("batch_ppr_piece", 64, 65536, o.ppr_piece);
   get("batch_ppr_run_rows", 1, 1024, o.ppr_run_rows);
   get("batch_ppr_run_edges", 64, 65536, o.ppr_run_edges);
@@ -295,6 +304,7 @@ void tune_load(Engine &e, const std::string &path) {
   get("batch_ppr_grid", 1, 64, o.ppr_grid);
   if (o.bsssp_unroll != 2 && o.bsssp_unroll != 4 && o.bsssp_unroll != 8)
     fail(GMP_EFORMAT, path + ": batch_sssp_unroll must be 2, 4 or 8");
+  if (o.bsssp_hub_log != 0 && o.bsssp_hub_log < 6) fail(GMP_EFORMAT, path + ": batch_sssp_hub_log out of range");
   if (o.ppr_loads != 4 && o.ppr_loads != 8 && o.ppr_loads != 16)
     fail(GMP_EFORMAT, path + ": batch_ppr_loads must be 4, 8 or 16");
   if (o.strip_log != 0 && o.strip_log < 8) fail(GMP_EFORMAT, path + ": strip_log out of range");

@@ -64,6 +64,16 @@ inline uint32_t grid_for(uint64_t n) {
 // v dropped in the round before.  Two mask arrays alternate: a round reads and clears the one
 // its entries were queued with and ORs the instances it improves into the other; the OR that
 // finds a vertex's mask empty queues the vertex.
+//
+// A round is two launches.  batch_relax_kernel takes the frontier from the queue while it is
+// small, and from the mask array itself, in group-vertex order, once it holds more than
+// ngv >> scan_log entries: then the waves work through one group at a time, so the random
+// destination lines they touch are those of one group's distances (tens to hundreds of MB, held
+// by the memory-side cache) instead of the whole batch's.  Entries with more than 2^hub_log
+// out-edges are not expanded by the wave that finds them: they go to a hub list, and
+// batch_hub_kernel relaxes their edges in pieces of 2^hub_log spread over every wave, so a hub
+// (a source of round 1, a popular vertex of a dense round) does not hold one wave while the
+// others wait at the end of the round.
 
 constexpr uint32_t kGroupInst = 32;   // instances per group (bits of a mask word)
 constexpr uint32_t kLdsGroups = 256;  // group table kept in LDS up to this many groups
@@ -88,6 +98,11 @@ struct BatchSsspArgs {
   uint32_t *count;            // [0]=|queue0|, [1]=|queue1|, [2]=round
   uint32_t *ticket;
   unsigned long long *edges;  // instance edges relaxed
+  uint4 *hubs;                // the round's hub entries: (group vertex, mask, first piece, group)
+  unsigned long long *hubcnt; // hub entries << 32 | pieces, of the round
+  uint32_t ngv;               // group vertices
+  uint32_t scan_min;          // frontiers of more entries walk the mask array (ngv >> scan_log)
+  uint32_t hub_log;           // entries with more than 2^hub_log out-edges go to the hub list (0: none)
 };
 
 // the last group whose first group vertex is <= gv
@@ -101,11 +116,49 @@ __device__ inline uint32_t group_of(const BatchGroup *g, uint32_t n, uint32_t gv
   return lo;
 }
 
-// One round for every instance: reads queue[round&1] and appends improved group vertices to
-// the other queue; the last workgroup to finish flips the round and clears the consumed count.
-// A wave takes 64 frontier entries and expands their edges kRelaxUnroll per lane at a time (a
-// tunable: 2, 4 or 8); a lane walks the instances of its edge's entry: for each, the improving
-// distance goes in by atomicMin, and one atomicOr per edge marks the instances improved at the
+// Relaxes edge (to, weight wt) of a frontier entry for the instances in mo: sb and tb are the
+// distance slots of instance 0 at the source and at the destination.  Returns the instances
+// whose distance at the destination drops: nd < the distance read this round means it drops this
+// round, by this lane or by another one, so the instance belongs in the next frontier whichever
+// atomicMin wins (waiting for atomicMin's old value instead costs more than the marks it saves).
+__device__ inline uint32_t relax_instances(uint32_t *dist, uint32_t sb, uint32_t tb, uint32_t mo, uint32_t wt) {
+  uint32_t imp = 0;
+  while (mo) {
+    const uint32_t k = (uint32_t)__builtin_ctz(mo);
+    mo &= mo - 1u;
+    const uint64_t t = (uint64_t)dist[sb + k] + wt;
+    const uint32_t nd = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
+    if (nd < dist[tb + k]) {
+      atomicMin(&dist[tb + k], nd);
+      imp |= 1u << k;
+    }
+  }
+  return imp;
+}
+
+// Stages the group vertices this lane queued (bit q of app: gvt[q]) in the wave's LDS buffer,
+// flushing it to the next queue when the following batch could overflow it.
+template <int kN>
+__device__ inline void stage_appends(WaveAppend &wa, uint32_t app, const uint32_t (&gvt)[kN], uint32_t *count,
+                                     uint32_t *out) {
+  const uint32_t cnt = (uint32_t)__popc(app);
+  const uint32_t inc = wave_scan_dpp(cnt);
+  const uint32_t tot = read_lane_u32(inc, kWave - 1);
+  if (!tot) return;
+  uint32_t pos = wa.n + inc - cnt;
+#pragma unroll
+  for (int q = 0; q < kN; q++)
+    if ((app >> q) & 1u) wa.buf[pos++] = gvt[q];
+  wa.n += tot;
+  if (wa.n > kAppendCap - kWave * kN) append_flush(wa, count, out);
+}
+
+// First half of a round for every instance: takes the frontier (queue[round&1], or the mask
+// array in group-vertex order when the frontier is large), clears the entries' masks, defers
+// hub entries to the hub list, and appends improved group vertices to the other queue.  A wave
+// takes 64 frontier entries and expands their edges kRelaxUnroll per lane at a time (a tunable:
+// 2, 4 or 8); a lane walks the instances of its edge's entry: for each, the improving distance
+// goes in by atomicMin, and one atomicOr per edge marks the instances improved at the
 // destination.
 template <int kRelaxUnroll>
 __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
@@ -129,19 +182,27 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
   uint32_t *mcur = cur ? a.mask1 : a.mask0;
   uint32_t *mnxt = cur ? a.mask0 : a.mask1;
   const uint32_t lane = lane_id(), wv = wave_id();
-  const uint32_t nchunks = (nfront + kWave - 1) / kWave;
+  const bool scan = nfront > a.scan_min;
+  const uint32_t nchunks = ((scan ? a.ngv : nfront) + kWave - 1) / kWave;
   unsigned long long edges = 0;
   WaveAppend wa{s_app[wv], 0u};
   for (uint32_t c = blockIdx.x * kWavesPerBlock + wv; c < nchunks; c += gridDim.x * kWavesPerBlock) {
     const uint32_t i = c * kWave + lane;
     uint64_t b = 0;
-    uint32_t deg = 0, m = 0, src = 0, gi = 0;
-    if (i < nfront) {
-      const uint32_t gv = front[i];
+    uint32_t deg = 0, m = 0, src = 0, gi = 0, gv = 0;
+    if (scan) {
+      if (i < a.ngv) {
+        gv = i;
+        m = mcur[gv];
+      }
+    } else if (i < nfront) {
+      gv = front[i];
+      m = mcur[gv];
+    }
+    if (m) {
       gi = group_of(G, a.ngroups, gv);
       const BatchGroup g = G[gi];
       const uint32_t v = gv - g.gvbase, u = g.vbase + v;
-      m = mcur[gv];
       mcur[gv] = 0u;
       // instances whose distance at v dropped again in this round are queued for the next one,
       // where they relax v's edges from the lower distance: skip them here
@@ -150,6 +211,12 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
       deg = m ? (uint32_t)(a.off[u + 1] - b) : 0u;
       src = g.dslot + (v << g.kklog);
       edges += (unsigned long long)deg * __popc(m);
+      if (a.hub_log && deg > (1u << a.hub_log)) {  // relaxed by batch_hub_kernel, in pieces
+        const uint32_t np = (deg + (1u << a.hub_log) - 1u) >> a.hub_log;
+        const unsigned long long old = atomicAdd(a.hubcnt, (1ull << 32) | np);
+        a.hubs[old >> 32] = uint4{gv, m, (uint32_t)old, gi};
+        deg = 0;
+      }
     }
     const uint32_t incl = wave_scan_dpp(deg);
     const uint32_t total = read_lane_u32(incl, kWave - 1);
@@ -190,51 +257,104 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
         const uint32_t o = own[q];
         const BatchGroup g = G[s_grp[wv][o]];
         const uint32_t tl = to[q] - g.vbase;
-        const uint32_t tb = g.dslot + (tl << g.kklog), sb = s_src[wv][o];
-        uint32_t mo = s_mask[wv][o], imp = 0;
-        // nd < cd (read this round) means the distance drops this round, by this lane or by
-        // another one, so the instance belongs in the next frontier whichever atomicMin wins
-        // (waiting for atomicMin's old value instead costs more than the marks it saves)
-        while (mo) {
-          const uint32_t k = (uint32_t)__builtin_ctz(mo);
-          mo &= mo - 1u;
-          const uint64_t t = (uint64_t)a.dist[sb + k] + wt[q];
-          const uint32_t nd = t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
-          if (nd < a.dist[tb + k]) {
-            atomicMin(&a.dist[tb + k], nd);
-            imp |= 1u << k;
-          }
-        }
+        const uint32_t imp = relax_instances(a.dist, s_src[wv][o], g.dslot + (tl << g.kklog), s_mask[wv][o], wt[q]);
         if (imp) {
           gvt[q] = g.gvbase + tl;
           if (atomicOr(&mnxt[gvt[q]], imp) == 0u) app |= 1u << q;  // the vertex's first mark
         }
       }
-      // the lane's newly queued group vertices, staged in the wave's LDS buffer
-      const uint32_t cnt = (uint32_t)__popc(app);
-      const uint32_t inc = wave_scan_dpp(cnt);
-      const uint32_t tot = read_lane_u32(inc, kWave - 1);
-      if (tot) {
-        uint32_t pos = wa.n + inc - cnt;
-#pragma unroll
-        for (int q = 0; q < kRelaxUnroll; q++)
-          if ((app >> q) & 1u) wa.buf[pos++] = gvt[q];
-        wa.n += tot;
-        if (wa.n > kAppendCap - kWave * kRelaxUnroll) append_flush(wa, &a.count[nxt], out);
-      }
+      stage_appends(wa, app, gvt, &a.count[nxt], out);
     }
     wave_sync();
   }
   append_flush(wa, &a.count[nxt], out);
   edges = wave_reduce_sum_u64(edges);
   if (lane == 0 && edges) atomicAdd(a.edges, edges);
+}
+
+// The last hub entry whose first piece is <= p, for a wave (p uniform): a 64-way search, each
+// step comparing the first pieces at 64 evenly spaced entries of the interval left (hubs[0]'s
+// first piece is 0, and first pieces grow with the entry).
+__device__ inline uint32_t hub_of_piece(const uint4 *hubs, uint32_t nhub, uint32_t p) {
+  const uint32_t lane = lane_id();
+  uint32_t lo = 0, n = nhub;  // the entry is in [lo, lo + n); hubs[lo] qualifies
+  while (n > 1) {
+    const uint32_t step = (n + kWave - 1) / kWave;
+    const uint32_t i = lo + lane * step;
+    const uint64_t le = __ballot(i < lo + n && hubs[i].z <= p);
+    const uint32_t k = 63u - (uint32_t)__clzll(le);
+    const uint32_t nlo = lo + k * step;
+    n = min(step, lo + n - nlo);
+    lo = nlo;
+  }
+  return lo;
+}
+
+// Second half of a round: the out-edges of the hub entries batch_relax_kernel deferred, in
+// pieces of 2^hub_log edges.  Pieces are numbered hub by hub (a hub entry's first piece comes
+// from the same atomic as its slot, so first pieces grow with the slot), and the waves take
+// them interleaved: at any time the grid works on neighbouring pieces, of hubs the relax
+// kernel met at about the same time (in group-vertex order in a dense round), so the
+// destination lines it touches belong to few groups.  The last workgroup to finish flips the
+// round and clears the consumed queue's count and the hub list.
+template <int kRelaxUnroll>
+__global__ __launch_bounds__(kBlock) void batch_hub_kernel(BatchSsspArgs a) {
+  __shared__ BatchGroup s_g[kLdsGroups];
+  __shared__ uint32_t s_app[kWavesPerBlock][kAppendCap];
+  const bool lds_g = a.ngroups <= kLdsGroups;
+  if (lds_g)
+    for (uint32_t i = threadIdx.x; i < a.ngroups; i += blockDim.x) s_g[i] = a.groups[i];
+  __syncthreads();
+  const BatchGroup *G = lds_g ? s_g : a.groups;
+  const uint32_t round = a.count[2];
+  const uint32_t cur = round & 1u, nxt = cur ^ 1u;
+  uint32_t *out = cur ? a.queue0 : a.queue1;
+  uint32_t *mnxt = cur ? a.mask0 : a.mask1;
+  const uint32_t lane = lane_id(), wv = wave_id();
+  const unsigned long long hc = *a.hubcnt;
+  const uint32_t nhub = (uint32_t)(hc >> 32), npiece = (uint32_t)hc;
+  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  WaveAppend wa{s_app[wv], 0u};
+  for (uint32_t p = blockIdx.x * kWavesPerBlock + wv; p < npiece; p += nwaves) {
+    const uint4 hub = a.hubs[hub_of_piece(a.hubs, nhub, p)];  // (group vertex, mask, first piece, group)
+    const BatchGroup g = G[hub.w];
+    const uint32_t v = hub.x - g.gvbase, u = g.vbase + v, sb = g.dslot + (v << g.kklog);
+    const uint64_t eb = a.off[u], ee = a.off[u + 1];
+    const uint64_t e0 = eb + ((uint64_t)(p - hub.z) << a.hub_log);
+    const uint64_t e1 = min(e0 + (1ull << a.hub_log), ee);
+    for (uint64_t jb = e0; jb < e1; jb += (uint64_t)kWave * kRelaxUnroll) {
+      uint32_t to[kRelaxUnroll], wt[kRelaxUnroll];
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++) {
+        const uint64_t e = jb + lane + (uint64_t)q * kWave;
+        const uint64_t ee2 = e < e1 ? e : e0;
+        to[q] = a.idx[ee2];
+        wt[q] = a.w[ee2];
+      }
+      uint32_t app = 0, gvt[kRelaxUnroll];
+#pragma unroll
+      for (int q = 0; q < kRelaxUnroll; q++) {
+        gvt[q] = 0u;
+        if (jb + lane + (uint64_t)q * kWave >= e1) continue;
+        const uint32_t tl = to[q] - g.vbase;
+        const uint32_t imp = relax_instances(a.dist, sb, g.dslot + (tl << g.kklog), hub.y, wt[q]);
+        if (imp) {
+          gvt[q] = g.gvbase + tl;
+          if (atomicOr(&mnxt[gvt[q]], imp) == 0u) app |= 1u << q;  // the vertex's first mark
+        }
+      }
+      stage_appends(wa, app, gvt, &a.count[nxt], out);
+    }
+  }
+  append_flush(wa, &a.count[nxt], out);
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
     const uint32_t t = atomicAdd(a.ticket, 1u);
-    if (t == gridDim.x - 1) {  // every workgroup has read count[cur]
+    if (t == gridDim.x - 1) {  // every workgroup has read count[cur] and the hub list
       a.count[cur] = 0;
       a.count[2] = round + 1;
+      *a.hubcnt = 0ull;
       *a.ticket = 0;
       __threadfence();
     }
@@ -243,7 +363,8 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
 
 // Every distance unreached, every mask empty, the counters cleared.
 __global__ void batch_sssp_init_kernel(uint32_t *dist, uint32_t total, uint32_t *mask0, uint32_t *mask1,
-                                       uint32_t ngv, uint32_t *count, uint32_t *ticket, unsigned long long *edges) {
+                                       uint32_t ngv, uint32_t *count, uint32_t *ticket, unsigned long long *edges,
+                                       unsigned long long *hubcnt) {
   GRID_STRIDE32(i, total) dist[i] = kInf;
   GRID_STRIDE32(i, ngv) {
     mask0[i] = 0u;
@@ -253,6 +374,7 @@ __global__ void batch_sssp_init_kernel(uint32_t *dist, uint32_t total, uint32_t 
     count[0] = count[1] = count[2] = 0u;
     *ticket = 0u;
     *edges = 0ull;
+    *hubcnt = 0ull;
   }
 }
 
@@ -635,7 +757,7 @@ namespace {
 struct SsspRun {
   uint32_t n = 0, T = 0, ngv = 0, D = 0;  // instances, output slots, group vertices, distance slots
   std::vector<uint32_t> oslot;            // [n+1] output slot bases (instance order)
-  DevBuf d_groups, d_seed, d_inst, d_oslot, dist, mask0, mask1, q0, q1, cnt;
+  DevBuf d_groups, d_seed, d_inst, d_oslot, dist, mask0, mask1, q0, q1, cnt, hubs;
   BatchSsspArgs a{};
   uint32_t grid = 0, nseed = 0;
 };
@@ -661,7 +783,8 @@ void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *
   // groups of at most kGroupInst instances of one graph, in graph order
   std::vector<BatchGroup> groups;
   std::vector<uint4> seeds(n), inst(n);
-  uint64_t gv = 0, ds = 0;
+  uint64_t gv = 0, ds = 0, hub_cap = 1, pieces = 0;
+  const uint32_t hub_log = e.opt.bsssp_hub_log;
   for (uint32_t k = 0; k < G; k++) {
     const uint32_t Vk = b.vbase[k + 1] - b.vbase[k];
     for (size_t s0 = 0; s0 < per[k].size(); s0 += kGroupInst) {
@@ -671,6 +794,12 @@ void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *
       if (gv + Vk >= 0xFFFFFFFFull || ds + ((uint64_t)Vk << kklog) >= 0xFFFFFFFFull)
         fail(GMP_EUNSUPPORTED, "batch exceeds 2^32 distance slots");
       groups.push_back(BatchGroup{(uint32_t)gv, b.vbase[k], (uint32_t)ds, kklog});
+      // a round's hub entries: group vertices of more than 2^hub_log out-edges, at most E >> hub_log
+      const uint64_t Ek = b.ebase[k + 1] - b.ebase[k];
+      if (hub_log) {
+        hub_cap += Ek >> hub_log;
+        pieces += (Ek >> hub_log) * 2;
+      }
       for (uint32_t j = 0; j < cnt; j++) {
         const uint32_t i = per[k][s0 + j];
         seeds[i] = uint4{(uint32_t)ds + (source[i] << kklog) + j, (uint32_t)gv + source[i], 1u << j, 0u};
@@ -680,6 +809,7 @@ void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *
       ds += (uint64_t)Vk << kklog;
     }
   }
+  if (pieces >= 0xFFFFFFFFull) fail(GMP_EUNSUPPORTED, "batch: too many hub pieces per round");
   r.ngv = (uint32_t)gv;
   r.D = (uint32_t)ds;
   r.nseed = n;
@@ -693,6 +823,7 @@ void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *
   r.q0.alloc((size_t)r.ngv * 4);
   r.q1.alloc((size_t)r.ngv * 4);
   r.cnt.alloc(64);
+  r.hubs.alloc(hub_cap * sizeof(uint4));
   GMP_HIP(hipMemcpyAsync(r.d_groups.p, groups.data(), groups.size() * sizeof(BatchGroup), hipMemcpyHostToDevice,
                          e.stream));
   GMP_HIP(hipMemcpyAsync(r.d_seed.p, seeds.data(), n * sizeof(uint4), hipMemcpyHostToDevice, e.stream));
@@ -712,12 +843,18 @@ void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *
   a.count = r.cnt.as<uint32_t>();
   a.ticket = r.cnt.as<uint32_t>() + 4;
   a.edges = reinterpret_cast<unsigned long long *>(r.cnt.as<uint32_t>() + 8);
+  a.hubcnt = reinterpret_cast<unsigned long long *>(r.cnt.as<uint32_t>() + 10);
+  a.hubs = r.hubs.as<uint4>();
+  a.ngv = r.ngv;
+  a.scan_min = e.opt.bsssp_scan_log ? r.ngv >> e.opt.bsssp_scan_log : 0xFFFFFFFFu;
+  a.hub_log = hub_log;
   r.grid = std::max<uint32_t>(1, (uint32_t)e.num_cus * e.opt.bsssp_grid);
 }
 
 void sssp_start(Engine &e, SsspRun &r, hipStream_t s) {
   batch_sssp_init_kernel<<<grid_for(std::max(r.D, r.ngv)), kBlock, 0, s>>>(r.a.dist, r.D, r.a.mask0, r.a.mask1, r.ngv,
-                                                                            r.a.count, r.a.ticket, r.a.edges);
+                                                                            r.a.count, r.a.ticket, r.a.edges,
+                                                                            r.a.hubcnt);
   batch_sssp_seed_kernel<<<grid_for(r.nseed), kBlock, 0, s>>>(r.d_seed.as<uint4>(), r.nseed, r.a.dist, r.a.mask0,
                                                               r.a.queue0, r.a.count);
   GMP_CHECK_LAUNCH();
@@ -726,9 +863,18 @@ void sssp_start(Engine &e, SsspRun &r, hipStream_t s) {
 void sssp_rounds(Engine &e, SsspRun &r, hipStream_t s, int rounds) {
   for (int k = 0; k < rounds; k++) {
     switch (e.opt.bsssp_unroll) {
-      case 2: batch_relax_kernel<2><<<r.grid, kBlock, 0, s>>>(r.a); break;
-      case 8: batch_relax_kernel<8><<<r.grid, kBlock, 0, s>>>(r.a); break;
-      default: batch_relax_kernel<4><<<r.grid, kBlock, 0, s>>>(r.a); break;
+      case 2:
+        batch_relax_kernel<2><<<r.grid, kBlock, 0, s>>>(r.a);
+        batch_hub_kernel<2><<<r.grid, kBlock, 0, s>>>(r.a);
+        break;
+      case 8:
+        batch_relax_kernel<8><<<r.grid, kBlock, 0, s>>>(r.a);
+        batch_hub_kernel<8><<<r.grid, kBlock, 0, s>>>(r.a);
+        break;
+      default:
+        batch_relax_kernel<4><<<r.grid, kBlock, 0, s>>>(r.a);
+        batch_hub_kernel<4><<<r.grid, kBlock, 0, s>>>(r.a);
+        break;
     }
     GMP_CHECK_LAUNCH();
   }

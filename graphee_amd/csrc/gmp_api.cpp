@@ -177,6 +177,14 @@ int gmp_set_option(gmp_engine *h, int32_t key, int64_t value) {
       in(1, 64, "batch_sssp_grid");
       e->opt.bsssp_grid = (uint32_t)value;
       break;
+    case GMP_OPT_BATCH_SSSP_SCAN_LOG:
+      in(0, 31, "batch_sssp_scan_log");
+      e->opt.bsssp_scan_log = (uint32_t)value;
+      break;
+    case GMP_OPT_BATCH_SSSP_HUB_LOG:
+      if (value != 0) in(6, 24, "batch_sssp_hub_log");
+      e->opt.bsssp_hub_log = (uint32_t)value;
+      break;
     case GMP_OPT_BATCH_PPR_PIECE:
       in(64, 65536, "batch_ppr_piece");
       e->opt.ppr_piece = (uint32_t)value;
@@ -236,6 +244,8 @@ int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value) {
     case GMP_OPT_LEARNED_FINISH: *value = e->opt.learned_finish; break;
     case GMP_OPT_BATCH_SSSP_UNROLL: *value = e->opt.bsssp_unroll; break;
     case GMP_OPT_BATCH_SSSP_GRID: *value = e->opt.bsssp_grid; break;
+    case GMP_OPT_BATCH_SSSP_SCAN_LOG: *value = e->opt.bsssp_scan_log; break;
+    case GMP_OPT_BATCH_SSSP_HUB_LOG: *value = e->opt.bsssp_hub_log; break;
     case GMP_OPT_BATCH_PPR_PIECE: *value = e->opt.ppr_piece; break;
     case GMP_OPT_BATCH_PPR_RUN_ROWS: *value = e->opt.ppr_run_rows; break;
     case GMP_OPT_BATCH_PPR_RUN_EDGES: *value = e->opt.ppr_run_edges; break;

@@ -268,6 +268,8 @@ struct Options {
   // batched runs (batch.hip; gmp_batch_autotune writes these)
   uint32_t bsssp_unroll = 4;     // SSSP relax: edges per lane in flight (2, 4, 8)
   uint32_t bsssp_grid = 4;       // SSSP relax: workgroups per CU
+  uint32_t bsssp_scan_log = 8;   // SSSP: frontiers above ngv >> k walk the mask array (0: always the queue)
+  uint32_t bsssp_hub_log = 11;   // SSSP: entries above 2^k out-edges are relaxed in pieces (0: never)
   uint32_t ppr_piece = 1024;     // PPR: rows with more in-edges are cut into pieces of this many
   uint32_t ppr_run_rows = 32;    // PPR: rows per slot in a row run
   uint32_t ppr_run_edges = 2048; // PPR: edges per slot in a row run

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define GMP_ABI_VERSION 3
+#define GMP_ABI_VERSION 4
 
 /* status codes */
 #define GMP_OK 0
@@ -173,6 +173,13 @@ int gmp_destroy(gmp_engine *h);
                               none of whose sources changed in the sweep before -- GraphMP's
                               selective scheduling (GraphMP/system/GraphPS.h:318-368), exact; 0:
                               every strip and window every sweep.  Results are the same. */
+#define GMP_OPT_BATCH_SSSP_SCAN_LOG 28 /* batched SSSP: a round whose frontier holds more than
+                              (group vertices) >> k entries takes it from the mask array in
+                              group-vertex order, one group at a time, instead of from the queue;
+                              0..31 (default 8), 0 = always the queue */
+#define GMP_OPT_BATCH_SSSP_HUB_LOG 29 /* batched SSSP: frontier entries with more than 2^k out-edges
+                              are relaxed by a second launch in pieces of 2^k edges spread over the
+                              grid; 6..24 (default 11), 0 = never */
 int gmp_set_option(gmp_engine *h, int32_t key, int64_t value);
 int gmp_get_option(gmp_engine *h, int32_t key, int64_t *value);
 
@@ -356,6 +363,7 @@ typedef struct gmp_batch_tune_result {
       trials;
   double sssp_ms_before, sssp_ms_after; /* one batched SSSP run, first and best */
   double ppr_ms_before, ppr_ms_after;   /* batched PPR, per iteration, first and best */
+  uint32_t sssp_scan_log, sssp_hub_log;
 } gmp_batch_tune_result;
 int gmp_batch_autotune(gmp_engine *h, uint32_t ns, const uint32_t *sssp_graph, const uint32_t *source,
                        uint32_t np, const uint32_t *ppr_graph, const uint32_t *seed, float damping, int32_t iters,

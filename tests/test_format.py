@@ -21,7 +21,7 @@ def test_library_exports_every_declared_symbol():
     L = gmp.lib()
     for name in declared:
         assert hasattr(L, name), name
-    assert L.gmp_abi_version() == 3
+    assert L.gmp_abi_version() == 4
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.uint32])

@@ -180,7 +180,8 @@ def test_batch_autotune_keeps_results(engine):
     gp = np.array([2] * 5 + [1, 3], dtype=np.uint32)
     seeds = np.array([rng.integers(0, parts[g][0].size - 1) for g in gp], dtype=np.uint32)
     keys = (gmp.OPT_BATCH_SSSP_UNROLL, gmp.OPT_BATCH_SSSP_GRID, gmp.OPT_BATCH_PPR_PIECE, gmp.OPT_BATCH_PPR_RUN_ROWS,
-            gmp.OPT_BATCH_PPR_RUN_EDGES, gmp.OPT_BATCH_PPR_LOADS, gmp.OPT_BATCH_PPR_GRID)
+            gmp.OPT_BATCH_PPR_RUN_EDGES, gmp.OPT_BATCH_PPR_LOADS, gmp.OPT_BATCH_PPR_GRID,
+            gmp.OPT_BATCH_SSSP_SCAN_LOG, gmp.OPT_BATCH_SSSP_HUB_LOG)
     saved = [engine.get_option(k) for k in keys]
 
     def check_results():
@@ -202,14 +203,18 @@ def test_batch_autotune_keeps_results(engine):
         assert res["trials"] > 10 and res["sssp_ms_after"] <= res["sssp_ms_before"]
         assert [engine.get_option(k) for k in keys] == [res["sssp_unroll"], res["sssp_grid_per_cu"], res["ppr_piece"],
                                                         res["ppr_run_rows"], res["ppr_run_edges"], res["ppr_loads"],
-                                                        res["ppr_grid_per_cu"]]
+                                                        res["ppr_grid_per_cu"], res["sssp_scan_log"],
+                                                        res["sssp_hub_log"]]
         check_results()
-        for vals in ((2, 1, 64, 1, 64, 4, 1), (8, 64, 65536, 1024, 65536, 16, 64)):
+        # the queue and no hub list; the mask array every round and hubs above 64 out-edges
+        for vals in ((2, 1, 64, 1, 64, 4, 1, 0, 0), (8, 64, 65536, 1024, 65536, 16, 64, 31, 6)):
             for k, v in zip(keys, vals):
                 engine.set_option(k, v)
             check_results()
         with pytest.raises(gmp.GmpError):
             engine.set_option(gmp.OPT_BATCH_PPR_LOADS, 5)
+        with pytest.raises(gmp.GmpError):
+            engine.set_option(gmp.OPT_BATCH_SSSP_HUB_LOG, 5)
     finally:
         for k, v in zip(keys, saved):
             engine.set_option(k, v)
