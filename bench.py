@@ -315,12 +315,20 @@ def run_sssp_s24(eng, threads):
     msg0[src] = 0
     dc, stc = oracle.graphmp_baseline(oracle.APP_SSSP_W, g["off"], g["idx"], g["bounds"], msg0, 10000, w=g["w"],
                                       threads=threads)
-    out["cpu"] = {"seconds": round(stc.seconds, 3), "threads": stc.threads, "sweeps": stc.iterations}
+    out["cpu"] = {"seconds": round(stc.seconds, 3), "threads": stc.threads, "sweeps": stc.iterations,
+                  "shards_computed": int(stc.shards)}
+    # the same engine with GraphMP's per-shard bloom filters choosing the shards of sparse steps
+    # (GraphPS.h:349-368): the reference's own activation, false positives included
+    db, stb = oracle.graphmp_baseline(oracle.APP_SSSP_W, g["off"], g["idx"], g["bounds"], msg0, 10000, w=g["w"],
+                                      threads=threads, bloom=True)
+    out["cpu_bloom"] = {"seconds": round(stb.seconds, 3), "sweeps": stb.iterations, "shards_computed": int(stb.shards),
+                        "identical": bool(np.array_equal(db, dc))}
     out["identical"] = bool(np.array_equal(dc, dist["bf"]) and np.array_equal(dc, dist["delta64"]))
     out["gpu_bf_speedup"] = round(stc.seconds * 1e3 / out["bf"]["ms"], 2)
     out["gpu_delta64_speedup"] = round(stc.seconds * 1e3 / out["delta64"]["ms"], 2)
     log(f"sssp s{scale}: BF {out['bf']['ms']:.1f} ms, delta-64 {out['delta64']['ms']:.1f} ms, CPU "
-        f"{stc.seconds:.2f} s ({stc.threads} threads); identical: {out['identical']}")
+        f"{stc.seconds:.2f} s ({stc.threads} threads, {int(stc.shards)} shard computations; with GraphMP's bloom "
+        f"filters {stb.seconds:.2f} s, {int(stb.shards)}); identical: {out['identical'] and out['cpu_bloom']['identical']}")
     return out
 
 

@@ -29,7 +29,11 @@ _I32 = ctypes.c_int32
 
 class BaselineStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("threads", ctypes.c_int32),
-                ("seconds", ctypes.c_double), ("edges", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("edges", ctypes.c_double), ("shards", ctypes.c_double)]
+
+
+class BloomShape(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("m", ctypes.c_uint64), ("salt", ctypes.c_uint32 * 16)]
 
 
 def build():
@@ -49,14 +53,16 @@ def lib():
         L.orc_sssp_dijkstra.argtypes = [_I64, _P, _P, _P, ctypes.c_uint32, _P]
         L.orc_cc_unionfind.argtypes = [_I64, _P, _P, _P]
         L.orc_graphmp_baseline.argtypes = [_I32, _I64, _I32, _P, _P, _P, _P, _P, ctypes.c_float,
-                                           _I32, _I32, _P, _P]
+                                           _I32, _I32, _I32, _P, _P]
+        L.orc_bf_params.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, _P]
+        L.orc_bf_probe.argtypes = [_P, _P, _I64, _P, _I64, _P]
         L.orc_pr_f64_step_residual.argtypes = [_I64, _P, _P, _P, ctypes.c_double, _P, _I32, _P, _P]
         L.orc_mlp_loss_grad.argtypes = [_P, _P, _P, _P, _I64, _P, _P]
         L.orc_pr_f64_refine.argtypes = [_I64, _P, _P, _P, ctypes.c_double, _P, _I32, _I32, ctypes.c_double,
                                         _P, _P, _P]
         for f in ("orc_partition_bounds", "orc_app_sync", "orc_pr_f64", "orc_sssp_dijkstra",
                   "orc_cc_unionfind", "orc_graphmp_baseline", "orc_pr_f64_step_residual",
-                  "orc_mlp_loss_grad", "orc_pr_f64_refine"):
+                  "orc_mlp_loss_grad", "orc_pr_f64_refine", "orc_bf_params", "orc_bf_probe"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -130,8 +136,9 @@ def cc_unionfind(off, idx):
 
 
 def graphmp_baseline(app, off, idx, bounds, msg_init, max_iter, w=None, outdeg=None,
-                     damping=0.85, threads=0):
-    """Timed CPU GraphMP engine; returns (msg, BaselineStats)."""
+                     damping=0.85, threads=0, bloom=False):
+    """Timed CPU GraphMP engine; returns (msg, BaselineStats).  bloom: sparse steps activate
+    shards by GraphMP's per-shard bloom filters instead of exact source sets."""
     V = off.size - 1
     off = np.ascontiguousarray(off, dtype=np.int64)
     idx = np.ascontiguousarray(idx, dtype=np.uint32)
@@ -142,9 +149,27 @@ def graphmp_baseline(app, off, idx, bounds, msg_init, max_iter, w=None, outdeg=N
     od = None if outdeg is None else np.ascontiguousarray(outdeg, dtype=np.uint32)
     st = BaselineStats()
     _check(lib().orc_graphmp_baseline(app, V, bounds.size - 1, _ptr(bounds), _ptr(off), _ptr(idx),
-                                      _ptr(wv), _ptr(od), damping, max_iter, threads, _ptr(msg),
-                                      ctypes.byref(st)), "baseline")
+                                      _ptr(wv), _ptr(od), damping, max_iter, threads, int(bool(bloom)),
+                                      _ptr(msg), ctypes.byref(st)), "baseline")
     return msg, st
+
+
+def bloom_shape(n=1000000, p=0.01, seed=0xA5A5A5A5):
+    """GraphMP's bloom-filter sizing and salts for n projected elements at false-positive
+    probability p (GraphPS.h:99-108 uses the defaults)."""
+    f = BloomShape()
+    _check(lib().orc_bf_params(n, p, seed, ctypes.byref(f)), "bloom parameters")
+    return f
+
+
+def bloom_probe(shape, keys, probes):
+    """Inserts keys into one filter of `shape` and returns contains() for every probe."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    probes = np.ascontiguousarray(probes, dtype=np.uint32)
+    hit = np.zeros(probes.size, dtype=np.uint8)
+    _check(lib().orc_bf_probe(ctypes.byref(shape), _ptr(keys), keys.size, _ptr(probes), probes.size, _ptr(hit)),
+           "bloom probe")
+    return hit.astype(bool)
 
 
 MLP_PARAMS = 16 * 64 + 64 + 64 * 72 + 72

@@ -19,7 +19,11 @@
  *   orc_cc_unionfind       textbook union-find, label = minimum id in the component
  *   orc_graphmp_baseline   GraphPS.h:256-386 engine shape for timing: OpenMP guided over
  *                          destination-interval shards, per-shard pull gather, sync swap,
- *                          selective scheduling of shards whose sources changed
+ *                          selective scheduling of shards whose sources changed, by exact
+ *                          source sets or by GraphMP's per-shard bloom filters
+ *   orc_bf_*               the vendored bloom filter GraphMP uses for that activation
+ *                          (GraphMP/system/bloom_filter.hpp) with GraphPS.h:99-108's
+ *                          parameters: sizing, salts, the hash of a 4-byte key
  *   orc_pr_f64_step_residual  one FP64 step from FP32 ranks: ||T(x)-x||_1 and _inf
  *   orc_pr_f64_refine      parallel FP64 power iteration from FP32 ranks to an a-posteriori
  *                          bound, the reference for measured L1 / L-inf errors
@@ -304,7 +308,94 @@ typedef struct {
   int32_t threads;
   double seconds;
   double edges;
+  double shards; /* shard computations, summed over the steps */
 } orc_baseline_stats;
+
+/* ---- GraphMP's per-shard bloom filters -------------------------------------------------
+ * GraphPS.h:99-108 gives every shard a filter of projected_element_count 1000000,
+ * false_positive_probability 0.01 and random_seed 0xA5A5A5A5; GraphPS.h:225-245 inserts the
+ * source of every edge of the shard (int32 keys); GraphPS.h:349-368 activates a shard when its
+ * filter contains any changed vertex.  The filter (bloom_filter.hpp) restated:
+ *   sizing (:107-157): the k in 1, 2, ..., 999 whose m(k) = -k n / ln(1 - p^(1/k)) is
+ *     smallest (first minimum), m truncated to an integer and rounded up to whole bytes;
+ *   salts (:466-510, k <= the predefined count): the first k predefined salts, then in
+ *     place, i = 0..k-1, salt[i] = salt[i] * salt[(i+3) % k] + (uint32)seed', with
+ *     seed' = seed * 0xA5A5A5A5 + 1 (the constructor, :176-185), all in 32-bit words;
+ *   hash of a 4-byte key (hash_ap, :550-608: no 8-byte rounds, one 4-byte step with an even
+ *     step counter): h = salt; h ^= ~((h << 11) + (key ^ (h >> 5)));
+ *   bit (compute_indices :460-464, insert :259-270, contains :300-318): index h mod m, byte
+ *     index / 8, bit index mod 8.
+ * The salt table has to match the reference's word for word; only the first 16 entries are
+ * carried, enough for every false-positive probability down to 1e-4 at any element count
+ * (k = 14).  Exact bits of this restatement are "parity unpinned" (no reference build);
+ * the tests pin the filter's behaviour (no false negatives, the designed false-positive
+ * rate, the textbook optimum of k and m). */
+#define ORC_BF_SALTS 16
+static const uint32_t orc_bf_predef[ORC_BF_SALTS] = {
+    0xAAAAAAAAu, 0x55555555u, 0x33333333u, 0xCCCCCCCCu, 0x66666666u, 0x99999999u, 0xB5B5B5B5u, 0x4B4B4B4Bu,
+    0xAA55AA55u, 0x55335533u, 0x33CC33CCu, 0xCC66CC66u, 0x66996699u, 0x99B599B5u, 0xB54BB54Bu, 0x4BAA4BAAu};
+
+typedef struct {
+  uint32_t k;
+  uint64_t m;                     /* bits */
+  uint32_t salt[ORC_BF_SALTS];
+} orc_bf_shape;
+
+int orc_bf_params(uint64_t n, double p, uint64_t seed, orc_bf_shape *out) {
+  if (!out || n == 0 || !(p > 0.0 && p < 1.0)) return ORC_EINVAL;
+  double min_m = INFINITY, min_k = 0.0;
+  for (double k = 1.0; k < 1000.0; k += 1.0) {
+    const double cur = (-k * (double)n) / log(1.0 - pow(p, 1.0 / k));
+    if (cur < min_m) {
+      min_m = cur;
+      min_k = k;
+    }
+  }
+  uint64_t m = (uint64_t)min_m;
+  if (m % 8) m += 8 - m % 8;
+  const uint32_t k = (uint32_t)min_k;
+  if (k < 1 || k > ORC_BF_SALTS || m == 0) return ORC_EINVAL;
+  out->k = k;
+  out->m = m;
+  const uint32_t s2 = (uint32_t)(seed * 0xA5A5A5A5ull + 1ull);
+  for (uint32_t i = 0; i < k; i++) out->salt[i] = orc_bf_predef[i];
+  for (uint32_t i = 0; i < k; i++) out->salt[i] = out->salt[i] * out->salt[(i + 3) % k] + s2;
+  return ORC_OK;
+}
+
+static inline uint64_t orc_bf_bit(const orc_bf_shape *f, uint32_t salt, uint32_t key) {
+  uint32_t h = salt;
+  h ^= ~((h << 11) + (key ^ (h >> 5)));
+  return (uint64_t)h % f->m;
+}
+
+static inline void orc_bf_insert(const orc_bf_shape *f, uint8_t *bits, uint32_t key) {
+  for (uint32_t i = 0; i < f->k; i++) {
+    const uint64_t b = orc_bf_bit(f, f->salt[i], key);
+    bits[b >> 3] |= (uint8_t)(1u << (b & 7u));
+  }
+}
+
+static inline int orc_bf_contains(const orc_bf_shape *f, const uint8_t *bits, uint32_t key) {
+  for (uint32_t i = 0; i < f->k; i++) {
+    const uint64_t b = orc_bf_bit(f, f->salt[i], key);
+    if (!(bits[b >> 3] & (1u << (b & 7u)))) return 0;
+  }
+  return 1;
+}
+
+/* One filter of shape f over keys[0..n) (bits: m / 8 bytes, cleared here), then `q` probes:
+ * hit[j] = contains(probe[j]).  Exposes the filter to the tests. */
+int orc_bf_probe(const orc_bf_shape *f, const uint32_t *keys, int64_t n, const uint32_t *probe, int64_t q,
+                 uint8_t *hit) {
+  if (!f || f->k == 0 || f->k > ORC_BF_SALTS || f->m == 0) return ORC_EINVAL;
+  uint8_t *bits = (uint8_t *)calloc((size_t)(f->m / 8), 1);
+  if (!bits) return ORC_ENOMEM;
+  for (int64_t i = 0; i < n; i++) orc_bf_insert(f, bits, keys[i]);
+  for (int64_t j = 0; j < q; j++) hit[j] = (uint8_t)orc_bf_contains(f, bits, probe[j]);
+  free(bits);
+  return ORC_OK;
+}
 
 static uint64_t orc_mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -325,8 +416,17 @@ static int has_u32(const uint32_t *a, int64_t n, uint32_t x) {
 int orc_graphmp_baseline(int32_t app, int64_t V, int32_t P, const int64_t *bounds,
                          const int64_t *off, const uint32_t *idx, const uint32_t *w,
                          const uint32_t *outdeg, float damping, int32_t max_iter,
-                         int32_t threads, void *msg, orc_baseline_stats *st) {
+                         int32_t threads, int32_t bloom, void *msg, orc_baseline_stats *st) {
   if (V <= 0 || P <= 0) return ORC_EINVAL;
+  /* bloom != 0: GraphMP's filters decide which shards a sparse step activates (a superset of
+   * the exact choice: false positives add shards, never drop one) */
+  orc_bf_shape bf;
+  uint8_t *bfbits = NULL;
+  if (bloom) {
+    if (orc_bf_params(1000000ull, 0.01, 0xA5A5A5A5ull, &bf) != ORC_OK) return ORC_EINVAL;
+    bfbits = (uint8_t *)calloc((size_t)P, (size_t)(bf.m / 8));
+    if (!bfbits) return ORC_ENOMEM;
+  }
   if (threads <= 0) threads = omp_get_max_threads();
   int64_t E = off[V];
   void *msgnew = malloc((size_t)V * 4);
@@ -366,10 +466,18 @@ int orc_graphmp_baseline(int32_t app, int64_t V, int32_t P, const int64_t *bound
     ss_off[2 * p + 1] = a + n;
   }
   free(marks);
+  if (bloom) {
+    /* GraphPS.h:225-245: every edge's source into its shard's filter (during init) */
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (int32_t p = 0; p < P; p++) {
+      uint8_t *bits = bfbits + (size_t)p * (size_t)(bf.m / 8);
+      for (int64_t q = off[bounds[p]]; q < off[bounds[p + 1]]; q++) orc_bf_insert(&bf, bits, idx[q]);
+    }
+  }
   memcpy(msgnew, msg, (size_t)V * 4);
   memset(pactive, 1, (size_t)P);
   double t0 = omp_get_wtime();
-  double edges = 0;
+  double edges = 0, shards = 0;
   int32_t step;
   uint64_t shuf = 0x5eed;
   for (step = 0; step < max_iter; step++) {
@@ -391,13 +499,14 @@ int orc_graphmp_baseline(int32_t app, int64_t V, int32_t P, const int64_t *bound
       base = (float)((1.0 - damping) / (double)V + damping * D / (double)V);
     }
     int64_t nchanged = 0;
-    double e_it = 0;
-#pragma omp parallel for num_threads(threads) schedule(guided) reduction(+ : nchanged, e_it)
+    double e_it = 0, s_it = 0;
+#pragma omp parallel for num_threads(threads) schedule(guided) reduction(+ : nchanged, e_it, s_it)
     for (int32_t k = 0; k < P; k++) {
       int32_t p = order[k];
       if (!pactive[p]) continue;
       int64_t s = bounds[p], e = bounds[p + 1];
       e_it += (double)(off[e] - off[s]);
+      s_it += 1.0;
       if (app == ORC_APP_PAGERANK) {
         const float *m = (const float *)msg;
         float *mn = (float *)msgnew;
@@ -423,6 +532,7 @@ int orc_graphmp_baseline(int32_t app, int64_t V, int32_t P, const int64_t *bound
       }
     }
     edges += e_it;
+    shards += s_it;
     /* GraphPS.h:316-368: collect changed vertices while swapping when the changed ratio
      * is below BF_THRE, then keep only shards holding a changed source */
     double ratio = (double)nchanged / (double)V;
@@ -437,8 +547,13 @@ int orc_graphmp_baseline(int32_t app, int64_t V, int32_t P, const int64_t *bound
 #pragma omp parallel for num_threads(threads) schedule(static)
       for (int32_t p = 0; p < P; p++) {
         uint8_t act = 0;
-        for (int64_t a = 0; a < na && !act; a++)
-          act = (uint8_t)has_u32(srcset + ss_off[2 * p], ss_off[2 * p + 1] - ss_off[2 * p], active[a]);
+        if (bloom) {
+          const uint8_t *bits = bfbits + (size_t)p * (size_t)(bf.m / 8);
+          for (int64_t a = 0; a < na && !act; a++) act = (uint8_t)orc_bf_contains(&bf, bits, active[a]);
+        } else {
+          for (int64_t a = 0; a < na && !act; a++)
+            act = (uint8_t)has_u32(srcset + ss_off[2 * p], ss_off[2 * p + 1] - ss_off[2 * p], active[a]);
+        }
         pactive[p] = act;
       }
     } else {
@@ -454,8 +569,9 @@ int orc_graphmp_baseline(int32_t app, int64_t V, int32_t P, const int64_t *bound
     st->threads = threads;
     st->seconds = t1 - t0;
     st->edges = edges;
+    st->shards = shards;
   }
-  free(msgnew); free(contrib); free(pactive); free(order); free(srcset); free(ss_off); free(active);
+  free(msgnew); free(contrib); free(pactive); free(order); free(srcset); free(ss_off); free(active); free(bfbits);
   return ORC_OK;
 }
 

@@ -45,6 +45,35 @@ def test_batch_sssp_matches_dijkstra(engine):
         np.testing.assert_array_equal(dists[i], oracle.dijkstra(ooff, oidx, ow, int(s)), err_msg=f"instance {i}")
 
 
+@pytest.mark.parametrize("scan_log,hub_log", [(0, 0), (31, 0), (0, 6), (31, 6), (4, 8)])
+def test_batch_sssp_frontier_walk_and_hub_pieces(engine, scan_log, hub_log):
+    """The frontier taken from the queue or from the mask array in group-vertex order (scan_log 31:
+    every round walks the masks), and entries above 2^hub_log out-edges relaxed in pieces by the
+    second launch (hub_log 6: hubs of 65 out-edges and more, many pieces each): every instance
+    keeps Dijkstra's distances, alone and in the combined launch with PPR instances."""
+    import graphee_amd as gmp
+    parts = _batch(engine, SPECS)
+    rng = np.random.default_rng(21)
+    gsel = np.array([1] * 40 + [3] * 9 + [2, 0, 4], dtype=np.uint32)
+    srcs = np.array([rng.integers(0, parts[g][0].size - 1) for g in gsel], dtype=np.uint32)
+    keys = (gmp.OPT_BATCH_SSSP_SCAN_LOG, gmp.OPT_BATCH_SSSP_HUB_LOG)
+    saved = [engine.get_option(k) for k in keys]
+    try:
+        engine.set_option(gmp.OPT_BATCH_SSSP_SCAN_LOG, scan_log)
+        engine.set_option(gmp.OPT_BATCH_SSSP_HUB_LOG, hub_log)
+        dists, st = engine.batch_sssp(gsel, srcs)
+        both, _, _, _ = engine.batch_run(gsel, srcs, [1, 2], [3, 4], iters=3)
+    finally:
+        for k, v in zip(keys, saved):
+            engine.set_option(k, v)
+    for i, (g, s) in enumerate(zip(gsel, srcs)):
+        off, idx, w = parts[g]
+        ooff, oidx, ow = graphs.transpose(off.size - 1, off, idx, w)
+        ref = oracle.dijkstra(ooff, oidx, ow, int(s))
+        np.testing.assert_array_equal(dists[i], ref, err_msg=f"instance {i}")
+        np.testing.assert_array_equal(both[i], ref, err_msg=f"combined, instance {i}")
+
+
 def test_batch_sssp_groups_of_many_instances(engine):
     """Instances of one graph relax together in groups of up to 32 (vertex-major distances,
     frontier entries with instance masks): 70 instances on one graph make groups of 32, 32 and 6,

@@ -201,6 +201,62 @@ def test_baseline_apps_match_textbook():
     np.testing.assert_array_equal(lab, oracle.cc_unionfind(soff, sidx))
 
 
+def test_bloom_filter_sizing_and_rates():
+    """GraphMP's per-shard filter (1e6 projected elements, false-positive probability 0.01):
+    k and m against the textbook optimum (m* = -n ln p / ln^2 2, k* = (m/n) ln 2), no false
+    negatives, and the false-positive rate at the design load against (1 - exp(-k n / m))^k.
+    The reference hash of a 4-byte key is a bijection close to affine in the key for each salt,
+    so its k bits are correlated: past the design load the rate leaves the independent-hash
+    model (0.86 against 0.83 at 5x), which is the reference's behaviour, not checked here."""
+    import math
+    f = oracle.bloom_shape()
+    n, p = 1000000, 0.01
+    m_star = -n * math.log(p) / math.log(2) ** 2
+    assert f.k == round(f.m / n * math.log(2)) == 7
+    assert f.m % 8 == 0 and m_star <= f.m <= m_star * 1.001
+    salts = list(f.salt[:f.k])
+    assert len(set(salts)) == f.k and 0 not in salts
+    rng = np.random.default_rng(41)
+    keys = rng.permutation(1 << 24).astype(np.uint32)
+    ins, probe = keys[:n], keys[-200000:]
+    hit = oracle.bloom_probe(f, ins, np.concatenate([ins, probe]))
+    assert hit[:n].all()  # no false negatives
+    fp = hit[n:].mean()
+    expect = (1 - math.exp(-f.k * n / f.m)) ** f.k
+    assert abs(fp - expect) < 0.002 and abs(fp - p) < 0.002, fp
+
+
+def test_baseline_bloom_activation_keeps_results():
+    """Sparse steps activating shards by GraphMP's bloom filters (GraphPS.h:349-368) give the
+    exact activation's values and iterations, and compute a superset of its shards."""
+    V = 5000
+    off, idx = graphs.random_graph(V, 20000, seed=31)
+    w = graphs.edge_weights(off, idx, seed=32)
+    many = oracle.partition_bounds(np.diff(off), 200)
+    msg0 = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+    msg0[0] = 0
+    d_ex, st_ex = oracle.graphmp_baseline(oracle.APP_SSSP_W, off, idx, many, msg0, 10000, w=w)
+    d_bf, st_bf = oracle.graphmp_baseline(oracle.APP_SSSP_W, off, idx, many, msg0, 10000, w=w, bloom=True)
+    np.testing.assert_array_equal(d_ex, d_bf)
+    assert st_ex.iterations == st_bf.iterations
+    assert st_ex.shards <= st_bf.shards <= st_bf.iterations * 200 and st_ex.edges <= st_bf.edges
+    assert st_ex.shards < st_ex.iterations * 200  # the exact choice skipped shards
+    soff, sidx = graphs.symmetrize(V, off, idx)
+    sb = oracle.partition_bounds(np.diff(soff), 50)
+    l_ex, c_ex = oracle.graphmp_baseline(oracle.APP_CC, soff, sidx, sb, np.arange(V, dtype=np.uint32), 10000)
+    l_bf, c_bf = oracle.graphmp_baseline(oracle.APP_CC, soff, sidx, sb, np.arange(V, dtype=np.uint32), 10000,
+                                         bloom=True)
+    np.testing.assert_array_equal(l_ex, l_bf)
+    assert c_ex.iterations == c_bf.iterations and c_ex.shards <= c_bf.shards
+    od = graphs.out_degree(V, idx)
+    pr0 = np.full(V, 1.0 / V, dtype=np.float32)
+    b7 = oracle.partition_bounds(np.diff(off), 7)
+    p_ex, s_ex = oracle.graphmp_baseline(oracle.APP_PAGERANK, off, idx, b7, pr0, 5, outdeg=od)
+    p_bf, s_bf = oracle.graphmp_baseline(oracle.APP_PAGERANK, off, idx, b7, pr0, 5, outdeg=od, bloom=True)
+    np.testing.assert_array_equal(p_ex.view(np.uint32), p_bf.view(np.uint32))
+    assert s_ex.shards == s_bf.shards == 5 * 7
+
+
 # ---- synthetic-input definition --------------------------------------------------------
 
 @pytest.mark.parametrize("scale", [1, 2, 5, 12, 17])
