@@ -54,6 +54,17 @@ inline uint32_t grid_for(uint64_t n) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + kBlock - 1) / kBlock, 1u << 20));
 }
 
+// flag = 1 when some weight needs more than a byte (a race only writes the same value)
+__global__ void weight_over_byte_kernel(const uint32_t *w, uint64_t n, uint32_t *flag) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    if (w[i] > 0xFFu) *flag = 1u;
+}
+
+__global__ void batch_weights_to_bytes_kernel(const uint32_t *w, uint64_t n, uint8_t *b) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    b[i] = (uint8_t)w[i];
+}
+
 // ---- SSSP ------------------------------------------------------------------------------
 //
 // Instances are grouped by graph, at most kGroupInst to a group.  A group keeps the distances of
@@ -89,7 +100,8 @@ struct BatchGroup {
 struct BatchSsspArgs {
   const uint64_t *off;        // union out-CSR offsets
   const uint32_t *idx;        // union ids
-  const uint32_t *w;
+  const uint32_t *w;          // weights, or null when
+  const uint8_t *w8;          // every weight fits a byte (R-MAT weights are 1..255)
   const BatchGroup *groups;
   uint32_t ngroups;
   uint32_t *dist;             // group-major, vertex-major inside a group
@@ -247,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void batch_relax_kernel(BatchSsspArgs a) {
 #pragma unroll
       for (int q = 0; q < kRelaxUnroll; q++) {
         to[q] = a.idx[e[q]];
-        wt[q] = a.w[e[q]];
+        wt[q] = a.w8 ? (uint32_t)a.w8[e[q]] : a.w[e[q]];
       }
       uint32_t app = 0, gvt[kRelaxUnroll];
 #pragma unroll
@@ -329,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void batch_hub_kernel(BatchSsspArgs a) {
         const uint64_t e = jb + lane + (uint64_t)q * kWave;
         const uint64_t ee2 = e < e1 ? e : e0;
         to[q] = a.idx[ee2];
-        wt[q] = a.w[ee2];
+        wt[q] = a.w8 ? (uint32_t)a.w8[ee2] : a.w[ee2];
       }
       uint32_t app = 0, gvt[kRelaxUnroll];
 #pragma unroll
@@ -740,6 +752,26 @@ void batch_generate(Engine &e, uint32_t ngraphs, const gmp_rmat_params *params) 
     transpose_csr(e, CsrView{b.in.off.p, b.in.idx.as<uint32_t>(), b.in.w.as<uint32_t>()}, b.V, b.E, true, weighted,
                   b.out);
   }
+  // the SSSP pushes read one byte per weight when every weight fits one
+  b.w8 = DevBuf();
+  if (weighted && b.E) {
+    DevBuf flag;
+    flag.alloc(4);
+    GMP_HIP(hipMemsetAsync(flag.p, 0, 4, e.stream));
+    weight_over_byte_kernel<<<grid_for(b.E), kBlock, 0, e.stream>>>(b.out.w.as<uint32_t>(), b.E, flag.as<uint32_t>());
+    GMP_CHECK_LAUNCH();
+    uint32_t over = 1;
+    GMP_HIP(hipMemcpyAsync(&over, flag.p, 4, hipMemcpyDeviceToHost, e.stream));
+    GMP_HIP(hipStreamSynchronize(e.stream));
+    if (!over) {
+      b.w8.alloc(b.E);
+      batch_weights_to_bytes_kernel<<<grid_for(b.E), kBlock, 0, e.stream>>>(b.out.w.as<uint32_t>(), b.E,
+                                                                            b.w8.as<uint8_t>());
+      GMP_CHECK_LAUNCH();
+      GMP_HIP(hipStreamSynchronize(e.stream));
+      b.out.w.release();
+    }
+  }
   b.outdeg.alloc(std::max<size_t>(b.V, 1) * 4);
   count_degrees(e, b.in.idx.as<uint32_t>(), b.E, b.V, b.outdeg.as<uint32_t>());
   b.h_off.resize((size_t)b.V + 1);
@@ -833,6 +865,7 @@ void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *
   a.off = b.out.off.as<uint64_t>();
   a.idx = b.out.idx.as<uint32_t>();
   a.w = b.out.w.as<uint32_t>();
+  a.w8 = b.w8.as<uint8_t>();
   a.groups = r.d_groups.as<BatchGroup>();
   a.ngroups = (uint32_t)groups.size();
   a.dist = r.dist.as<uint32_t>();

@@ -285,6 +285,7 @@ struct BatchGraphs {
   std::vector<uint64_t> ebase;        // G+1
   std::vector<uint64_t> h_off;        // union in-CSR offsets (host copy)
   Csr in, out;                        // union in-CSR (offsets always uint64) and its transpose
+  DevBuf w8;                          // uint8[E]: out's weights as bytes, in place of out.w, when all fit
   DevBuf outdeg;                      // uint32[Vtot]
   // each graph is stored in its degree-ordered layout (as the engine's work layout): o2n maps a
   // union id in the caller's numbering to the stored one (identity when reordering is off)
