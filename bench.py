@@ -516,8 +516,8 @@ def run_semi(scale, ef, seed, ref, iters=20):
 
 
 def run_semi_sssp(scale, ef, seed, budget_mb=512):
-    """Weighted SSSP over the windows of a semi-external R-MAT graph (ids packed, weights plain,
-    both streamed): exact window activation against the learned prefetch, whose window predictor
+    """Weighted SSSP over the windows of a semi-external R-MAT graph (ids packed, weights as
+    bytes, both streamed): exact window activation against the learned prefetch, whose window predictor
     is trained on traces of two other sources' solves on the same graph; distances must agree."""
     import graphee_amd as gmp
     from graphee_amd import learned

@@ -175,10 +175,14 @@ struct SemiExternal {
   uint64_t host_bytes = 0, cache_bytes = 0, budget = 0;
   std::vector<EdgeWindow> win;
   uint8_t *h_ids = nullptr;  // pinned: packed bytes (+ padding) or uint32 ids
-  uint32_t *h_w = nullptr;   // pinned: weights
+  uint32_t *h_w = nullptr;   // pinned: weights as uint32, or
+  uint8_t *h_w8 = nullptr;   // pinned: weights as bytes, when every weight fits one
+  uint32_t wsize = 4;        // bytes per weight in the host array, the ring and the cache
+  uint64_t wslot_bytes = 0;  // weight bytes of one ring slot
+  bool has_w() const { return h_w || h_w8; }
   DevBuf gbyte, gmask;       // group index of the packed stream
   DevBuf ring, wring;        // nslots slots of ids and of weights
-  DevBuf cache, wcache;      // cached windows (ids at EdgeWindow::cache, weights at 4*(e0 - first cached edge))
+  DevBuf cache, wcache;      // cached windows (ids at EdgeWindow::cache, weights at wsize*(e0 - first cached edge))
   uint64_t wcache_e0 = 0;
   DevBuf plain;              // one window's decoded ids
   DevBuf tasks, hubs, multi; // gather plans of all windows, rows relative to the window
@@ -396,7 +400,8 @@ void semi_setup(Engine &e, const uint32_t *idx, const uint32_t *w);
 struct WindowDev {
   const uint32_t *idx = nullptr;
   Packed pk{};
-  const uint32_t *w = nullptr;
+  const uint32_t *w = nullptr;   // weights by global edge, or
+  const uint8_t *w8 = nullptr;   // the same as bytes
   int slot = -1;
 };
 // Makes window `w` available to work enqueued next on e.stream: a cached window at once, a

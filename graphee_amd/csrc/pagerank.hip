@@ -268,7 +268,8 @@ __global__ __launch_bounds__(kFinalBlock) void pr_finalize_kernel(const double *
 // `sweep` (window activation of semi-external graphs).
 template <class OffT, int APP>
 __global__ __launch_bounds__(kBlock) void compat_step_kernel(const OffT *off, const uint32_t *idx,
-                                                             const uint32_t *w, const int32_t *vout, uint32_t v0,
+                                                             const uint32_t *w, const uint8_t *w8,
+                                                             const int32_t *vout, uint32_t v0,
                                                              uint32_t v1, double invN, const void *msg_v,
                                                              void *msgnew_v, void *value_v,
                                                              unsigned long long *changed, uint32_t *stamp,
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void compat_step_kernel(const OffT *off, co
       const uint32_t *msg = (const uint32_t *)msg_v;
       uint32_t best = msg[v];
       for (uint64_t k = k0; k < k1; k++) {
-        const uint64_t t = (uint64_t)msg[idx[k]] + (uint64_t)w[k];
+        const uint64_t t = (uint64_t)msg[idx[k]] + (uint64_t)(w8 ? (uint32_t)w8[k] : w[k]);
         if (t < (uint64_t)best) best = (uint32_t)t;
       }
       ((uint32_t *)msgnew_v)[v] = best;
@@ -322,6 +323,7 @@ __global__ __launch_bounds__(kBlock) void compat_step_kernel(const OffT *off, co
 
 struct CompatRows {
   const uint32_t *idx, *w;  // ids and weights by global edge
+  const uint8_t *w8;        // the weights as bytes (semi-external windows whose weights all fit one)
   uint32_t v0, v1;
   uint32_t *stamp;
   uint32_t blk_log, sweep;
@@ -332,7 +334,7 @@ void launch_compat(Engine &e, int app, uint32_t grid, double invN, const void *m
                    void *value, const int32_t *vout, unsigned long long *changed, const CompatRows &r) {
   const OffT *off = e.g.in.off.as<OffT>();
 #define GMP_COMPAT(APP)                                                                                      \
-  compat_step_kernel<OffT, APP><<<grid, kBlock, 0, e.stream>>>(off, r.idx, r.w, vout, r.v0, r.v1, invN, msg, \
+  compat_step_kernel<OffT, APP><<<grid, kBlock, 0, e.stream>>>(off, r.idx, r.w, r.w8, vout, r.v0, r.v1, invN, msg, \
                                                                msgnew, value, changed, r.stamp, r.blk_log,  \
                                                                r.sweep)
   switch (app) {
@@ -367,7 +369,8 @@ void compat_windows_step(Engine &e, int app, int step, uint32_t grid_cap, double
       continue;
     }
     const WindowDev d = window_begin(e, w, app == GMP_APP_SSSP_W, true);
-    CompatRows r{d.idx, d.w, x.r0, x.r1, minapp ? s.stamp.as<uint32_t>() : nullptr, s.blk_log, (uint32_t)step + 1};
+    CompatRows r{d.idx, d.w, d.w8, x.r0, x.r1, minapp ? s.stamp.as<uint32_t>() : nullptr, s.blk_log,
+                 (uint32_t)step + 1};
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((x.r1 - x.r0 + kBlock - 1) / kBlock, grid_cap));
     launch_compat<OffT>(e, app, grid, invN, msg, msgnew, value, vout, changed, r);
     window_end(e, d);
@@ -737,7 +740,7 @@ void compat_impl(Engine &e, int app, int max_iter, const void *msg_init, const i
     if (g.semi) {
       compat_windows_step<OffT>(e, app, step, grid, invN, msg[cur], msg[cur ^ 1], value, dvout, changed);
     } else {
-      const CompatRows all{g.in.idx.as<uint32_t>(), g.in.w.as<uint32_t>(), 0u, V, nullptr, 0u, 0u};
+      const CompatRows all{g.in.idx.as<uint32_t>(), g.in.w.as<uint32_t>(), nullptr, 0u, V, nullptr, 0u, 0u};
       launch_compat<OffT>(e, app, grid, invN, msg[cur], msg[cur ^ 1], value, dvout, changed, all);
     }
     GMP_HIP(hipMemcpyAsync(&nch, changed, 8, hipMemcpyDeviceToHost, e.stream));

@@ -17,6 +17,7 @@
 // GraphMP's bloom-filter test (GraphPS.h:318-351) made exact at block granularity: no window
 // whose sources changed is ever skipped.
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <thread>
@@ -232,7 +233,8 @@ template <class OffT>
 struct MinWin {
   const OffT *off;      // row offsets, shifted to the window's first row
   const uint32_t *idx;  // ids by global edge
-  const uint32_t *w;    // weights by global edge (kMinWeighted)
+  const uint32_t *w;    // weights by global edge (kMinWeighted), or
+  const uint8_t *w8;    // the same as bytes
   uint32_t *val;        // labels or distances, global ids
   uint32_t r0;          // window's first row
   const uint32_t *tasks;
@@ -248,7 +250,7 @@ __device__ inline uint32_t min_candidate(const MinWin<OffT> &a, uint64_t j) {
   const uint32_t d = a.val[a.idx[j]];
   if (MODE == kMinLabel) return d;
   if (d == kInf) return kInf;
-  const uint64_t t = (uint64_t)d + (MODE == kMinWeighted ? a.w[j] : 1u);
+  const uint64_t t = (uint64_t)d + (MODE == kMinWeighted ? (a.w8 ? (uint32_t)a.w8[j] : a.w[j]) : 1u);
   return t > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)t;
 }
 
@@ -370,6 +372,7 @@ void semi_min(Engine &e, int mode, uint32_t src, uint32_t *out, gmp_stats *st, g
     a.off = off + x.r0;
     a.idx = d.idx;
     a.w = d.w;
+    a.w8 = d.w8;
     a.r0 = x.r0;
     a.tasks = tasks + 2ull * x.task0;
     a.ntasks = x.ntasks;
@@ -563,6 +566,7 @@ SemiExternal::~SemiExternal() {
   for (auto c : copy) (void)hipStreamDestroy(c);
   if (h_ids) (void)hipHostFree(h_ids);
   if (h_w) (void)hipHostFree(h_w);
+  if (h_w8) (void)hipHostFree(h_w8);
 }
 
 bool semi_wanted(const Engine &e, uint64_t E, bool weighted) {
@@ -657,9 +661,28 @@ void semi_setup(Engine &e, const uint32_t *idx, const uint32_t *w) {
   }
   s.host_bytes = stream_bytes;
   if (w) {
-    s.h_w = reinterpret_cast<uint32_t *>(pinned_alloc(E * 4));
-    parallel_ranges(E, 1 << 22, [&](uint64_t a, uint64_t b) { std::memcpy(s.h_w + a, w + a, (b - a) * 4); });
-    s.host_bytes += E * 4;
+    // one byte per weight when every weight fits (R-MAT weights are 1..255): a quarter of the
+    // weight bytes to stream and to cache
+    std::vector<uint8_t> over(host_threads() + 1, 0);
+    std::atomic<uint32_t> part{0};
+    parallel_ranges(E, 1 << 22, [&](uint64_t a, uint64_t b) {
+      uint32_t m = 0;
+      for (uint64_t i = a; i < b; i++) m |= w[i];
+      over[part++] = m > 0xFFu;
+    });
+    const bool fits = std::find(over.begin(), over.end(), (uint8_t)1) == over.end();
+    if (fits) {
+      s.wsize = 1;
+      s.h_w8 = pinned_alloc(std::max<uint64_t>(E, 1));
+      parallel_ranges(E, 1 << 22, [&](uint64_t a, uint64_t b) {
+        for (uint64_t i = a; i < b; i++) s.h_w8[i] = (uint8_t)w[i];
+      });
+    } else {
+      s.wsize = 4;
+      s.h_w = reinterpret_cast<uint32_t *>(pinned_alloc(E * 4));
+      parallel_ranges(E, 1 << 22, [&](uint64_t a, uint64_t b) { std::memcpy(s.h_w + a, w + a, (b - a) * 4); });
+    }
+    s.host_bytes += E * s.wsize;
   }
   // windows: consecutive rows up to the target id bytes (a longer row makes a window alone)
   const uint64_t target = (uint64_t)e.opt.window_mb << 20;
@@ -701,13 +724,14 @@ void semi_setup(Engine &e, const uint32_t *idx, const uint32_t *w) {
   s.slot_bytes = (s.slot_bytes + 255) & ~255ull;
   // device: ring slots, decode buffer, cache of the leading windows that fit the budget
   s.nslots = kRingSlots;
-  const uint64_t wslot = w ? s.max_edges * 4 : 0;
+  const uint64_t wslot = w ? (s.max_edges * s.wsize + 255) & ~255ull : 0;
+  s.wslot_bytes = wslot;
   const uint64_t ring_bytes = s.nslots * (s.slot_bytes + wslot) + (s.packed ? s.max_edges * 4 : 0);
   uint64_t cache_ids = 0, cache_w = 0;
   uint32_t ncached = 0;
   while (ncached < nwin) {
     const EdgeWindow &x = s.win[ncached];
-    const uint64_t ids = ((x.b1 - x.b0) + 255) & ~255ull, wb = w ? (x.e1 - x.e0) * 4 : 0;
+    const uint64_t ids = ((x.b1 - x.b0) + 255) & ~255ull, wb = w ? (x.e1 - x.e0) * s.wsize : 0;
     if (ring_bytes + cache_ids + cache_w + ids + wb > s.budget) break;
     s.win[ncached].cache = (int64_t)cache_ids;
     cache_ids += ids;
@@ -728,7 +752,8 @@ void semi_setup(Engine &e, const uint32_t *idx, const uint32_t *w) {
     if (w && cache_w) {
       s.wcache.alloc(cache_w);
       s.wcache_e0 = 0;
-      GMP_HIP(hipMemcpy(s.wcache.p, s.h_w, cache_w, hipMemcpyHostToDevice));
+      GMP_HIP(hipMemcpy(s.wcache.p, s.h_w8 ? (const void *)s.h_w8 : (const void *)s.h_w, cache_w,
+                        hipMemcpyHostToDevice));
     }
   }
   // out-degrees: ids counted on the device in chunks
@@ -844,9 +869,11 @@ uint32_t ring_copy(SemiExternal &s, const EdgeWindow &x, bool need_w) {
   if (x.b1 > x.b0) GMP_HIP(hipMemcpyAsync(dst, s.h_ids + x.b0, x.b1 - x.b0, hipMemcpyHostToDevice, cs));
   s.streamed_bytes += x.b1 - x.b0;
   if (need_w) {
-    uint32_t *wd = s.wring.as<uint32_t>() + (uint64_t)k * s.max_edges;
-    if (x.e1 > x.e0) GMP_HIP(hipMemcpyAsync(wd, s.h_w + x.e0, (x.e1 - x.e0) * 4, hipMemcpyHostToDevice, cs));
-    s.streamed_bytes += (x.e1 - x.e0) * 4;
+    uint8_t *wd = s.wring.as<uint8_t>() + (uint64_t)k * s.wslot_bytes;
+    const uint8_t *hw = s.h_w8 ? s.h_w8 : reinterpret_cast<const uint8_t *>(s.h_w);
+    if (x.e1 > x.e0)
+      GMP_HIP(hipMemcpyAsync(wd, hw + x.e0 * s.wsize, (x.e1 - x.e0) * s.wsize, hipMemcpyHostToDevice, cs));
+    s.streamed_bytes += (x.e1 - x.e0) * s.wsize;
   }
   GMP_HIP(hipEventRecord(s.ready[k], cs));
   return k;
@@ -859,14 +886,14 @@ WindowDev window_use(Engine &e, uint32_t wi, int slot, bool need_w, bool plain) 
   const EdgeWindow &x = s.win[wi];
   WindowDev d;
   const uint8_t *ids;
-  const uint32_t *wts = nullptr;
+  const uint8_t *wts = nullptr;  // the window's first weight
   if (slot < 0) {
     ids = s.cache.as<uint8_t>() + x.cache;
-    if (need_w) wts = s.wcache.as<uint32_t>() + (x.e0 - s.wcache_e0);
+    if (need_w) wts = s.wcache.as<uint8_t>() + (x.e0 - s.wcache_e0) * s.wsize;
   } else {
     GMP_HIP(hipStreamWaitEvent(e.stream, s.ready[slot], 0));
     ids = s.ring.as<uint8_t>() + (uint64_t)slot * s.slot_bytes;
-    if (need_w) wts = s.wring.as<uint32_t>() + (uint64_t)slot * s.max_edges;
+    if (need_w) wts = s.wring.as<uint8_t>() + (uint64_t)slot * s.wslot_bytes;
     d.slot = slot;
   }
   if (s.packed) {
@@ -887,7 +914,10 @@ WindowDev window_use(Engine &e, uint32_t wi, int slot, bool need_w, bool plain) 
   } else {
     d.idx = shift_back<uint32_t>(ids, x.b0);
   }
-  if (wts) d.w = shift_back<uint32_t>(wts, 4 * x.e0);
+  if (wts) {
+    if (s.wsize == 1) d.w8 = shift_back<uint8_t>(wts, x.e0);
+    else d.w = shift_back<uint32_t>(wts, 4 * x.e0);
+  }
   s.windows_run++;
   return d;
 }
@@ -897,7 +927,7 @@ WindowDev window_use(Engine &e, uint32_t wi, int slot, bool need_w, bool plain) 
 WindowDev window_begin(Engine &e, uint32_t wi, bool weights, bool plain) {
   SemiExternal &s = semi_of(e);
   const EdgeWindow &x = s.win[wi];
-  const bool need_w = weights && s.h_w;
+  const bool need_w = weights && s.has_w();
   const int slot = x.cache >= 0 ? -1 : (int)ring_copy(s, x, need_w);
   return window_use(e, wi, slot, need_w, plain);
 }
@@ -907,19 +937,19 @@ int window_prefetch(Engine &e, uint32_t wi, bool weights) {
   const EdgeWindow &x = s.win[wi];
   if (x.cache >= 0) return -1;
   s.prefetched++;
-  return (int)ring_copy(s, x, weights && s.h_w);
+  return (int)ring_copy(s, x, weights && s.has_w());
 }
 
 WindowDev window_begin_slot(Engine &e, uint32_t wi, int slot, bool weights, bool plain) {
   SemiExternal &s = semi_of(e);
-  return window_use(e, wi, slot, weights && s.h_w, plain);
+  return window_use(e, wi, slot, weights && s.has_w(), plain);
 }
 
 void window_release(Engine &e, uint32_t wi, int slot, bool weights) {
   SemiExternal &s = semi_of(e);
   if (slot < 0) return;
   const EdgeWindow &x = s.win[wi];
-  s.prefetch_wasted_bytes += (x.b1 - x.b0) + (weights && s.h_w ? (x.e1 - x.e0) * 4 : 0);
+  s.prefetch_wasted_bytes += (x.b1 - x.b0) + (weights && s.has_w() ? (x.e1 - x.e0) * s.wsize : 0);
   s.windows_skipped++;
   GMP_HIP(hipEventRecord(s.freed[slot], e.stream));
 }
@@ -1052,8 +1082,9 @@ void semi_download(Engine &e, uint32_t *col_idx, uint32_t *weights) {
     }
   }
   if (weights) {
-    if (!s.h_w) fail(GMP_EUNSUPPORTED, "the graph has no weights");
-    std::memcpy(weights, s.h_w, E * 4);
+    if (!s.has_w()) fail(GMP_EUNSUPPORTED, "the graph has no weights");
+    if (s.h_w8) std::copy(s.h_w8, s.h_w8 + E, weights);
+    else std::memcpy(weights, s.h_w, E * 4);
   }
 }
 

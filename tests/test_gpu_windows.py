@@ -279,6 +279,34 @@ def test_windows_load_paths_and_unsupported_runs(tmp_path):
         assert s.windows()["windows"] == 0
 
 
+@pytest.mark.parametrize("wmax", [255, 100000])
+def test_windows_weights_as_bytes_or_words(wmax):
+    """Weights that all fit a byte stream and cache as bytes, wider ones as 32-bit words: the
+    host bytes say which, the download gives the loaded weights back, and weighted SSSP keeps
+    Dijkstra's distances through streamed and cached windows."""
+    say(f"semi-external weights up to {wmax}")
+    rng = np.random.default_rng(9)
+    V, E = 100000, 3000000
+    off, idx = graphs.csr_from_edges(V, rng.integers(0, V, E), rng.integers(0, V, E))
+    w = rng.integers(1, wmax + 1, idx.size).astype(np.uint32)
+    ooff, oidx, ow = graphs.transpose(V, off, idx, w)
+    src = int(np.argmax(graphs.out_degree(V, idx)))
+    want = oracle.dijkstra(ooff, oidx, ow, src)
+    for budget in (1, 8):  # nothing cached / some windows cached
+        with _engine(budget, 1, True) as s:
+            s.load_csr(off, idx, weights=w, P=3)
+            st = s.windows()
+            assert st["windows"] >= 3
+            per_edge = 1 if wmax <= 255 else 4
+            id_bytes = st["host_bytes"] - per_edge * idx.size
+            assert 0 < id_bytes <= 4 * idx.size
+            h = s.download(weights=True)
+            np.testing.assert_array_equal(h["w"], w)
+            for method in (gmp.SSSP_BF, gmp.SSSP_DELTA):
+                d, _ = s.sssp(src, method=method)
+                np.testing.assert_array_equal(d, want)
+
+
 @pytest.mark.parametrize("packed", [False, True])
 def test_windows_learned_prefetch_keeps_results(packed):
     """The window predictor (the engine's second network), trained on traces of this graph's
