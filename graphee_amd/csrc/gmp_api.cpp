@@ -31,6 +31,37 @@ std::string join(const char *dir, const std::string &name) {
 
 }  // namespace
 
+namespace gmp {
+
+hipEvent_t *engine_events(Engine &e, size_t n) {
+  while (e.ev_pool.size() < n) {
+    hipEvent_t x = nullptr;
+    GMP_HIP(hipEventCreate(&x));
+    e.ev_pool.push_back(x);
+  }
+  return e.ev_pool.data();
+}
+
+double *engine_host_doubles(Engine &e, size_t n, double **dev) {
+  if (e.h_hist_n < n) {
+    if (e.h_hist) (void)hipHostFree(e.h_hist);
+    e.h_hist = nullptr;
+    e.d_hist = nullptr;
+    e.h_hist_n = 0;
+    void *p = nullptr;
+    GMP_HIP(hipHostMalloc(&p, n * sizeof(double), hipHostMallocMapped));
+    e.h_hist = static_cast<double *>(p);
+    void *d = nullptr;
+    GMP_HIP(hipHostGetDevicePointer(&d, p, 0));
+    e.d_hist = static_cast<double *>(d);
+    e.h_hist_n = n;
+  }
+  *dev = e.d_hist;
+  return e.h_hist;
+}
+
+}  // namespace gmp
+
 extern "C" {
 
 int gmp_create(int32_t device, gmp_engine **out) {
@@ -71,6 +102,8 @@ int gmp_destroy(gmp_engine *h) {
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   if (e->h_pinned) (void)hipHostFree(e->h_pinned);
+  if (e->h_hist) (void)hipHostFree(e->h_hist);
+  for (hipEvent_t x : e->ev_pool) (void)hipEventDestroy(x);
   for (hipEvent_t x : e->na_ev)
     if (x) (void)hipEventDestroy(x);
   hipStream_t s = e->stream;

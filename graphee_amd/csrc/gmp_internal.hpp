@@ -327,10 +327,19 @@ struct Engine {
   DevBuf s_a, s_b, s_c, s_d, s_e, s_f, s_part, s_scal;
   DevBuf s_gaps;  // graph build: runs of empty rows left to a kernel of their own (count, then runs)
   void *h_pinned = nullptr;  // small pinned host buffer for per-iteration readbacks
+  std::vector<hipEvent_t> ev_pool;  // timing events, created on first use and kept
+  double *h_hist = nullptr;         // pinned, device-mapped: per-iteration PageRank residuals
+  double *d_hist = nullptr;         // its device address
+  size_t h_hist_n = 0;
   uint32_t last_actions[kMlpClasses] = {};  // classes the last learned SSSP run chose, per class
   void *scratch = nullptr;       // device address of noalloc.hip's scratch array on this device
   hipEvent_t na_ev[2] = {};      // timing events of gmp_pagerank_noalloc (created with the engine)
 };
+
+// The first n events of the engine's pool (created as needed, destroyed with the engine).
+hipEvent_t *engine_events(Engine &e, size_t n);
+// n doubles of pinned host memory that kernels write through *dev (kept with the engine).
+double *engine_host_doubles(Engine &e, size_t n, double **dev);
 
 void set_last_error(const std::string &m);
 const char *last_error();

@@ -220,11 +220,14 @@ __global__ __launch_bounds__(kBlock) void pr_init_kernel(uint32_t V, uint32_t se
 
 // Sums partials in a fixed order (n1 from the gather, n2 from the hub combine), records
 // the iteration's residual and dangling mass, and prepares the next base.
+// With tol > 0 it also raises sc->done once the residual is below tol (it < 0, the set-up call,
+// clears it), after which the iteration launches queued behind it do nothing.
 __global__ __launch_bounds__(kFinalBlock) void pr_finalize_kernel(const double *pr1, const double *pd1, uint32_t n1,
                                                                   const double *pr2, const double *pd2, uint32_t n2,
                                                                   uint32_t V, float d, int personalised,
-                                                                  PrScal *sc, double *hist, int32_t it) {
+                                                                  PrScal *sc, double *hist, int32_t it, double tol) {
   __shared__ double sr[kFinalBlock], sd[kFinalBlock];
+  if (it >= 0 && sc->done) return;
   double r = 0, dd = 0;
   for (uint32_t i = threadIdx.x; i < n1; i += kFinalBlock) {
     r += pr1[i];
@@ -260,6 +263,7 @@ __global__ __launch_bounds__(kFinalBlock) void pr_finalize_kernel(const double *
       hist[2 * it] = sr[0];
       hist[2 * it + 1] = D;
     }
+    sc->done = (it >= 0 && tol > 0.0 && sr[0] < tol) ? 1 : 0;
   }
 }
 
@@ -439,7 +443,7 @@ void pagerank_windows(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_st
   pr_init_kernel<<<G, kBlock, 0, e.stream>>>(V, seed, outdeg, rank[0], contrib[0], pr1, pd1);
   GMP_CHECK_LAUNCH();
   pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, G, pr2, pd2, 0, V, p.damping, personalised, sc,
-                                                      nullptr, -1);
+                                                      nullptr, -1, 0.0);
   GMP_CHECK_LAUNCH();
   const size_t lds = (size_t)kWavesPerBlock * (g.plan.stream_cap + (comp ? kVarintScratchWords : 0u)) * sizeof(float);
   int cur = 0, it = 0;
@@ -496,7 +500,7 @@ void pagerank_windows(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_st
     }
     GMP_HIP(hipEventRecord(ev[3 + 2 * it], e.stream));
     pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, n1, pr2, pd2, n2, V, p.damping, personalised, sc,
-                                                        hist, it);
+                                                        hist, it, 0.0);
     GMP_CHECK_LAUNCH();
     cur ^= 1;
     if (p.tol > 0) {
@@ -568,6 +572,8 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   // edges fit 32-bit indices; the row gather serves the other cases
   const bool strips = !comp && kahan && ensure_strips(e);
   const uint32_t np = std::max(G, strips ? strip_partials(e) : 0u);
+  // with a tolerance, the strip kernels run one iteration ahead of the host's residual check
+  const bool ahead = strips && p.tol > 0;
 
   // buffers: 2 x rank, 2 x contrib, hub partials, FP64 partials, scalars, history
   e.s_a.alloc((size_t)V * 4);
@@ -582,16 +588,11 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   double *pr1 = e.s_part.as<double>(), *pd1 = pr1 + np;
   double *pr2 = pd1 + np, *pd2 = pr2 + np;
   PrScal *sc = e.s_scal.as<PrScal>();
-  double *hist = reinterpret_cast<double *>(e.s_scal.as<char>() + sizeof(PrScal));
-
-  std::vector<hipEvent_t> ev(2 * (size_t)max_iter + 2);
-  for (auto &x : ev) GMP_HIP(hipEventCreate(&x));
-  struct EvGuard {
-    std::vector<hipEvent_t> &v;
-    ~EvGuard() {
-      for (auto x : v) (void)hipEventDestroy(x);
-    }
-  } evg{ev};
+  // the residual history lives in pinned host memory, written by the finalize kernel, so the host
+  // reads an iteration's residual as soon as that kernel is done, without a copy behind later work
+  double *hist = nullptr;
+  double *hh = engine_host_doubles(e, 2 * (size_t)max_iter + 2, &hist);
+  hipEvent_t *ev = engine_events(e, 2 * (size_t)max_iter + 2);
 
   PrArgs<OffT> a{};
   const CsrView wv = g.work_in();
@@ -614,7 +615,7 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   pr_init_kernel<<<G, kBlock, 0, e.stream>>>(V, seed, a.outdeg, rank[0], contrib[0], pr1, pd1);
   GMP_CHECK_LAUNCH();
   pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, G, pr2, pd2, 0, V, p.damping, personalised, sc,
-                                                      nullptr, -1);
+                                                      nullptr, -1, 0.0);
   GMP_CHECK_LAUNCH();
   if (comp) {
     a.pk = Packed{g.packed.bytes.as<uint8_t>(), g.packed.gbyte.as<uint64_t>(), g.packed.gmask.as<uint64_t>()};
@@ -654,16 +655,22 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     }
     GMP_HIP(hipEventRecord(ev[3 + 2 * it], e.stream));
     pr_finalize_kernel<<<1, kFinalBlock, 0, e.stream>>>(pr1, pd1, nfin1, pr2, pd2, nfin2, V, p.damping, personalised,
-                                                        sc, hist, it);
+                                                        sc, hist, it, ahead ? (double)p.tol : 0.0);
     GMP_CHECK_LAUNCH();
     cur ^= 1;
-    if (p.tol > 0) {
-      double h[2];
-      GMP_HIP(hipMemcpyAsync(h, hist + 2 * it, sizeof h, hipMemcpyDeviceToHost, e.stream));
+    if (p.tol > 0 && !ahead) {
       GMP_HIP(hipStreamSynchronize(e.stream));
-      resid = h[0];
-      if (resid < p.tol) {
+      if (hh[2 * it] < p.tol) {
         it++;
+        break;
+      }
+    } else if (p.tol > 0 && it >= 1) {
+      // iteration it is queued: read iteration it - 1's residual once its finalize is done (the
+      // event recorded at the start of iteration it); when it converged, iteration it finds
+      // sc->done set and does nothing, and its buffers keep iteration it - 1's ranks
+      GMP_HIP(hipEventSynchronize(ev[2 + 2 * it]));
+      if (hh[2 * (it - 1)] < p.tol) {
+        cur ^= 1;
         break;
       }
     }
@@ -679,10 +686,8 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   float tot = 0;
   GMP_HIP(hipEventElapsedTime(&tot, ev[0], ev[1]));
   if (it > 0) {
-    double h[2];
-    GMP_HIP(hipMemcpy(h, hist + 2 * (it - 1), sizeof h, hipMemcpyDeviceToHost));
-    resid = h[0];
-    dang = h[1];
+    resid = hh[2 * (it - 1)];
+    dang = hh[2 * (it - 1) + 1];
   }
   if (rank_out) {
     e.s_f.alloc((size_t)V * 4);

@@ -13,6 +13,7 @@ struct PrScal {
   float extra;   // added to the seed vertex only
   double resid;  // L1 change of the last iteration
   double dang;   // dangling mass of the current ranks
+  int32_t done;  // the solve has converged: iteration launches queued after that do nothing
 };
 
 struct PrAcc {

@@ -663,8 +663,10 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
   }
 }
 
+// An iteration queued after the solve converged (sc->done) does nothing, in every kernel of it.
 __global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a) {
   extern __shared__ float sv[];  // contributions of the task's strip, then a message buffer per wave
+  if (a.sc->done) return;
   strip_gather<kSweepPR>(a, sv);
 }
 
@@ -803,6 +805,7 @@ __device__ inline void bin_messages(const StripArgs &a, uint32_t n, const BinLds
 __global__ __launch_bounds__(kIterBlock) void pr_bin_kernel(StripArgs a) {
   extern __shared__ unsigned long long lds2[];
   __shared__ uint32_t wtot[kIterWaves], live;
+  if (a.sc->done) return;
   const uint32_t R = 1u << a.rlog, ns = a.ns;
   const BinLds<unsigned long long> L = bin_lds<unsigned long long>(lds2, R, ns);
   const uint32_t tid = threadIdx.x;
@@ -829,6 +832,7 @@ __global__ __launch_bounds__(kIterBlock) void pr_bin_kernel(StripArgs a) {
 }
 
 __global__ __launch_bounds__(kBlock) void pr_split_apply_kernel(StripArgs a) {
+  if (a.sc->done) return;
   const float base = a.sc->base, extra = a.sc->extra;
   const uint32_t R = 1u << a.rlog;
   PrAcc pacc;
