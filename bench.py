@@ -690,17 +690,38 @@ def main():
             "yardstick_gbs": round(bytes_launch / kms_r / 1e6, 1),
             "strip_speedup": round(kms_r / (avg_launch_s * 1e3), 3)}
         log(f"pagerank row gather: {kms_r:.3f} ms/iteration ({kms_r / (avg_launch_s * 1e3):.2f}x the strip path)")
-        eng.pagerank(damping=0.85, max_iter=3, compressed=True, want_ranks=False)  # builds the packed lists
+        # compressed ids: on the strip layout its codes bit-packed per gather round (the default
+        # with a strip width), in the row gather delta/varint column ids (strip layout off)
+        eng.pagerank(damping=0.85, max_iter=3, compressed=True, want_ranks=False)  # packs the codes
+        rc, stc = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, compressed=True)
         _, stc = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, compressed=True,
                               want_ranks=False)
+        rp, stp = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol)
+        same = bool(np.array_equal(rc.view(np.uint32), rp.view(np.uint32))) and stc.iterations == stp.iterations
+        del rc, rp
         kms_c = stc.ms_kernel / max(stc.launches, 1)
         result["extra"]["pagerank_compressed"] = {
+            "layout": "source strips, codes bit-packed per 512-edge round" if strip_now else "row gather, varint ids",
             "ms_per_solve": round(stc.ms_total, 3), "iterations": stc.iterations,
             "gteps": round(E * stc.iterations / stc.ms_total / 1e6, 3),
-            "gather_ms": round(kms_c, 4), "yardstick_gbs": round(bytes_launch / kms_c / 1e6, 1),
-            "index_bytes": stc.index_bytes, "plain_index_bytes": 4 * E,
+            "ms_per_iteration": round(kms_c, 4), "yardstick_gbs": round(bytes_launch / kms_c / 1e6, 1),
+            "bit_identical_to_uncompressed": same,
+            "index_bytes": stc.index_bytes, "uncompressed_index_bytes": stp.index_bytes,
             "index_bytes_per_edge": round(stc.index_bytes / E, 3)}
-        log(f"pagerank compressed: {stc.index_bytes / E:.2f} B/edge column ids, gather {kms_c:.3f} ms/launch")
+        log(f"pagerank compressed: {stc.index_bytes / E:.3f} B/edge index ({stp.index_bytes / E:.3f} uncompressed), "
+            f"{kms_c:.3f} ms/iteration, bit-identical {same}")
+        if strip_now:
+            eng.set_option(gmp.OPT_STRIP_LOG, 0)
+            eng.pagerank(damping=0.85, max_iter=3, compressed=True, want_ranks=False)  # builds the varint lists
+            _, stv = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, compressed=True,
+                                  want_ranks=False)
+            eng.set_option(gmp.OPT_STRIP_LOG, strip_now)
+            kms_v = stv.ms_kernel / max(stv.launches, 1)
+            result["extra"]["pagerank_compressed_row_gather"] = {
+                "ms_per_iteration": round(kms_v, 4), "iterations": stv.iterations,
+                "yardstick_gbs": round(bytes_launch / kms_v / 1e6, 1),
+                "index_bytes": stv.index_bytes, "index_bytes_per_edge": round(stv.index_bytes / E, 3)}
+            log(f"pagerank compressed row gather: {stv.index_bytes / E:.3f} B/edge varint ids, {kms_v:.3f} ms/iteration")
 
     if not args.no_cpu:
         import oracle

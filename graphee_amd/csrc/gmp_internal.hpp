@@ -123,6 +123,15 @@ struct StripCsr {
   DevBuf wt;                    // uint32[E+pad]: edge weights in layout order (weighted SSSP sweeps)
   DevBuf wt8;                   // uint8[E+pad]: the same as bytes, in place of wt when all fit
   DevBuf gq;                    // uint32[2 + g1]: next workgroup task, ticket, each workgroup's task
+  // packed codes (GMP_PR_COMPRESSED on this layout): one block per 512-edge gather round of each
+  // strip -- 8 words of per-lane widths, and the lanes' groups of 8 codes bit-packed
+  DevBuf pk;                    // uint32 stream of the groups, padded
+  DevBuf pkhdr;                 // uint32[8 blocks]: the widths
+  DevBuf pkoff;                 // uint32[blocks]: first word of each block
+  DevBuf pkblk;                 // uint32[ns]: first block of each strip
+  uint64_t pk_words = 0;        // words of the stream (padding not counted)
+  uint32_t pk_nblk = 0;
+  bool pk_valid = false;
   bool wt_valid = false;
   uint32_t nwt = 0, ngt = 0, nbt = 0, nsplit = 0, nslots = 0;
   uint32_t lds1 = 0, lds2 = 0;  // dynamic LDS bytes of the gather and bin kernels
@@ -467,6 +476,9 @@ void ensure_packed(Engine &e);
 // Builds e.g.strips for e.opt.strip_log / bin_log; returns false when the layout does not apply
 // (strips off, or more than 2^32 - 1 edges).
 bool ensure_strips(Engine &e);
+// The packed form of the layout's codes (built from them on the host); false when there is no
+// strip layout or the stream would pass 2^32 words.
+bool ensure_strip_pack(Engine &e);
 // Min-type sweeps one at a time, for switching between sweeps and frontier pushes.  kind: 0 CC,
 // 1 unit-weight SSSP, 2 weighted SSSP.  c holds complemented values (c = ~x, 0 = unreached) in
 // work order; vstamp (V entries, or null) receives, per vertex, the number of the last sweep that

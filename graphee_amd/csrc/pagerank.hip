@@ -565,12 +565,14 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   const uint32_t Gh = pl.nmulti ? std::min<uint32_t>((pl.nmulti + kWavesPerBlock - 1) / kWavesPerBlock, G) : 0;
   const int max_iter = std::max(0, p.max_iter);
 
-  const bool comp = (p.flags & GMP_PR_COMPRESSED) != 0;
   const bool kahan = !(p.flags & GMP_PR_NO_KAHAN);
+  // source-strip layout: the default for compensated PageRank on graphs whose edges fit 32-bit
+  // indices (compressed: its codes bit-packed); the row gather serves the other cases
+  // (compressed: varint column ids)
+  const bool packed = (p.flags & GMP_PR_COMPRESSED) && kahan && ensure_strip_pack(e);
+  const bool strips = packed || (!(p.flags & GMP_PR_COMPRESSED) && kahan && ensure_strips(e));
+  const bool comp = (p.flags & GMP_PR_COMPRESSED) && !strips;
   if (comp) ensure_packed(e);
-  // source-strip layout: the default for compensated, uncompressed PageRank on graphs whose
-  // edges fit 32-bit indices; the row gather serves the other cases
-  const bool strips = !comp && kahan && ensure_strips(e);
   const uint32_t np = std::max(G, strips ? strip_partials(e) : 0u);
   // with a tolerance, the strip kernels run one iteration ahead of the host's residual check
   const bool ahead = strips && p.tol > 0;
@@ -635,7 +637,7 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     uint32_t nfin1 = G, nfin2 = Gh;
     if (strips) {
       const PrStripIO io{a.outdeg, contrib[cur], rank[cur], rank[cur ^ 1], contrib[cur ^ 1], sc, p.damping, seed,
-                         pr1, pd1, pr2, pd2};
+                         pr1, pd1, pr2, pd2, packed};
       strip_iteration(e, io, nfin1, nfin2);
     } else {
       if (comp) {
@@ -707,7 +709,11 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     st->changed = 0;
     const uint64_t o = g.off64 ? 8 : 4;
     st->bytes = (uint64_t)it * (8 * g.E + (o + 16) * (uint64_t)V);
-    st->index_bytes = comp ? g.packed.nbytes + 16 * g.packed.ngroups : strips ? 2 * g.E : 4 * g.E;
+    const StripCsr &S = g.strips;
+    st->index_bytes = packed ? 4 * (S.pk_words + 9 * (uint64_t)S.pk_nblk)
+                      : comp ? g.packed.nbytes + 16 * g.packed.ngroups
+                      : strips ? 2 * g.E
+                               : 4 * g.E;
   }
 }
 

@@ -116,6 +116,7 @@ struct PrStripIO {
   uint32_t seed;
   double *pr1, *pd1;  // FP64 residual/dangling partials of the bin kernel, one per block
   double *pr2, *pd2;  // and of the split-bin kernel
+  bool packed;        // the gather reads the packed codes (ensure_strip_pack)
 };
 
 // FP64 partials that strip_iteration may write into each of pr1, pd1, pr2, pd2.

@@ -56,7 +56,8 @@ constexpr int kPullUnroll = 4;                 // pull rounds: rows per thread i
 // What a sweep over the strips computes.  PageRank sums contributions; the others keep
 // complemented values (c = ~x), so the least label or distance is the greatest c and 0 -- what
 // DPP moves in past a row's start, and the value of an unreached vertex -- is neutral.
-enum SweepMode { kSweepPR = 0, kSweepCC = 1, kSweepUnit = 2, kSweepWeighted = 3, kSweepWeighted8 = 4 };
+enum SweepMode { kSweepPR = 0, kSweepCC = 1, kSweepUnit = 2, kSweepWeighted = 3, kSweepWeighted8 = 4, kSweepPRPacked = 5 };
+constexpr bool is_pr(int mode) { return mode == kSweepPR || mode == kSweepPRPacked; }
 
 #define GS64(i, n) \
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
@@ -88,6 +89,10 @@ struct StripArgs {
   const uint16_t *code;   // source offsets in the strip
   const uint64_t *sbase;  // per strip: the edge its wave tasks' edge offsets count from
   const uint8_t *endb;    // bit k of byte i: edge 8 i + k ends its segment
+  const uint32_t *pk;     // packed codes (kSweepPRPacked): the stream,
+  const uint32_t *pkhdr;  // each block's 8 width words,
+  const uint32_t *pkoff;  // each block's first word,
+  const uint32_t *pkblk;  // each strip's first block
   const uint4 *wtask;
   const uint4 *gtask;
   uint32_t ngt;
@@ -485,14 +490,14 @@ __device__ inline void min_wave_task(const StripArgs &a, float *mb, uint32_t mbb
   u32x4 buf[kAhead], wa[kAhead], wc[kAhead];
   uint32_t eba[kAhead];
   auto load = [&](int i, uint32_t r) {
-    buf[i] = __builtin_nontemporal_load(cb + r * kWave + lane);
-    eba[i] = __builtin_nontemporal_load(ebb + r * kWave + lane);
+    buf[i] = cb[r * kWave + lane];
+    eba[i] = ebb[r * kWave + lane];
     if (MODE == kSweepWeighted) {
-      wa[i] = __builtin_nontemporal_load(wb + 2 * (r * kWave + lane));
-      wc[i] = __builtin_nontemporal_load(wb + 2 * (r * kWave + lane) + 1);
+      wa[i] = wb[2 * (r * kWave + lane)];
+      wc[i] = wb[2 * (r * kWave + lane) + 1];
     }
     if (MODE == kSweepWeighted8) {
-      const u32x2 v = __builtin_nontemporal_load(w8b + r * kWave + lane);
+      const u32x2 v = w8b[r * kWave + lane];
       wa[i] = u32x4{v.x, v.y, 0u, 0u};
     }
   };
@@ -522,7 +527,10 @@ __device__ inline void min_wave_task(const StripArgs &a, float *mb, uint32_t mbb
 
 // PageRank messages of edges [e0, e1) whose first segment has index seg; every segment ends
 // inside.  The loads of codes and end bits run kAhead rounds ahead of their use; a round's loads
-// are whole (the layout's padding keeps those of the last round in bounds).
+// are whole (the layout's padding keeps those of the last round in bounds).  They are plain
+// loads: a round's codes start anywhere in a 128-byte line (a strip starts at any multiple of 8
+// edges), so its first and last lines are shared with the rounds before and after, and end bytes
+// are half a line a round; streaming loads fetched those lines twice (6 % more bytes).
 __device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint64_t sb, uint32_t e0,
                                     uint32_t e1, uint32_t seg) {
   const uint32_t lane = lane_id();
@@ -539,8 +547,8 @@ __device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb,
     buf[i] = u32x4{0u, 0u, 0u, 0u};
     eba[i] = 0u;
     if ((uint32_t)i < nr) {
-      buf[i] = __builtin_nontemporal_load(cb + (uint32_t)i * kWave + lane);
-      eba[i] = __builtin_nontemporal_load(ebb + (uint32_t)i * kWave + lane);
+      buf[i] = cb[(uint32_t)i * kWave + lane];
+      eba[i] = ebb[(uint32_t)i * kWave + lane];
     }
   }
   for (uint32_t r = 0; r < nr; r += kAhead) {
@@ -551,8 +559,8 @@ __device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb,
         const uint32_t eb = eba[i];
         const uint32_t rn = r + i + kAhead;
         if (rn < nr) {
-          buf[i] = __builtin_nontemporal_load(cb + rn * kWave + lane);
-          eba[i] = __builtin_nontemporal_load(ebb + rn * kWave + lane);
+          buf[i] = cb[rn * kWave + lane];
+          eba[i] = ebb[rn * kWave + lane];
         }
         const uint32_t base = b0 + (r + i) * kRound;
         const uint32_t q = base + 8 * lane;
@@ -566,11 +574,158 @@ __device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb,
   flush_messages(B);
 }
 
-// Messages of edges [e0, e1) whose first segment has index seg.
+// ---- packed codes (kSweepPRPacked) ----------------------------------------------------------
+// A block serves one 512-edge round of a strip: 8 header words (a separate array) hold the 64
+// lanes' widths (4 bits each, lane l in bits 4 (l % 8) of word l / 8); the block's words in the
+// stream hold the lanes' groups bit after bit, least significant bit first.  A group is the lane's first code (15 bits) and then, for
+// edges 1..7, a field: the code itself (15 bits) where the edge starts a segment (the end byte
+// says the edge before it ends one), else the code minus the code before it (codes ascend inside
+// a segment), at the group's width.  A lane finds its group from the sizes of the groups before
+// it (a wave prefix sum) and gets its 8 codes back exactly, so the round proceeds as with the
+// plain codes and the sums are the same bits.
+
+// Bits of a group of width b whose end byte is eb.
+__device__ inline uint32_t pk_group_bits(uint32_t b, uint32_t eb) {
+  const uint32_t nst = (uint32_t)__popc(eb & 0x7Fu);
+  return 15u + 7u * b + nst * (15u - b);
+}
+
+// The 8 codes (two to a word, as the plain layout holds them) of the group whose first bit is
+// bit sh of w0; w0..w4 are the five words from there (a group has at most 120 bits).  Fields go
+// in pairs (two fit one word); after a pair the window shifts by their widths and keeps only the
+// words the fields left can fill (15 bits each at most).
+__device__ inline u32x4 pk_unpack(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t w4, uint32_t sh,
+                                  uint32_t b, uint32_t eb) {
+  uint32_t d0 = __builtin_amdgcn_alignbit(w1, w0, sh), d1 = __builtin_amdgcn_alignbit(w2, w1, sh);
+  uint32_t d2 = __builtin_amdgcn_alignbit(w3, w2, sh), d3 = __builtin_amdgcn_alignbit(w4, w3, sh);
+  const uint32_t neb = ~eb;
+  const int bm = (int)b - 15;
+  uint32_t c0 = d0 & 0x7FFFu;
+  // fields 1..7 from bit 15: 105 bits at most
+  d0 = __builtin_amdgcn_alignbit(d1, d0, 15u);
+  d1 = __builtin_amdgcn_alignbit(d2, d1, 15u);
+  d2 = __builtin_amdgcn_alignbit(d3, d2, 15u);
+  d3 >>= 15;
+  // field j: k = 1 for a difference (edge j - 1 does not end a segment) of width b, else a code
+  // of 15 bits; code j = k * code (j - 1) + field
+  auto keep = [&](int j) { return __builtin_amdgcn_ubfe(neb, (uint32_t)(j - 1), 1u); };
+  auto next = [](uint32_t c, uint32_t k, uint32_t v) {  // c * k + v (k is 0 or 1)
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(c), "v"(k), "v"(v));
+    return r;
+  };
+  auto width = [&](uint32_t k) { return (uint32_t)(__mul24((int)k, bm) + 15); };
+  const uint32_t k1 = keep(1), k2 = keep(2), k3 = keep(3), k4 = keep(4), k5 = keep(5), k6 = keep(6), k7 = keep(7);
+  const uint32_t x1 = width(k1), x2 = width(k2), x3 = width(k3), x4 = width(k4), x5 = width(k5), x6 = width(k6),
+                 x7 = width(k7);
+  const uint32_t c1 = next(c0, k1, __builtin_amdgcn_ubfe(d0, 0u, x1));
+  const uint32_t c2 = next(c1, k2, __builtin_amdgcn_ubfe(d0, x1, x2));
+  uint32_t s = x1 + x2;  // 75 bits left at most
+  d0 = __builtin_amdgcn_alignbit(d1, d0, s);
+  d1 = __builtin_amdgcn_alignbit(d2, d1, s);
+  d2 = __builtin_amdgcn_alignbit(d3, d2, s);
+  const uint32_t c3 = next(c2, k3, __builtin_amdgcn_ubfe(d0, 0u, x3));
+  const uint32_t c4 = next(c3, k4, __builtin_amdgcn_ubfe(d0, x3, x4));
+  s = x3 + x4;  // 45 bits left at most
+  d0 = __builtin_amdgcn_alignbit(d1, d0, s);
+  d1 = __builtin_amdgcn_alignbit(d2, d1, s);
+  const uint32_t c5 = next(c4, k5, __builtin_amdgcn_ubfe(d0, 0u, x5));
+  const uint32_t c6 = next(c5, k6, __builtin_amdgcn_ubfe(d0, x5, x6));
+  s = x5 + x6;  // 15 bits left at most
+  d0 = __builtin_amdgcn_alignbit(d1, d0, s);
+  const uint32_t c7 = next(c6, k7, __builtin_amdgcn_ubfe(d0, 0u, x7));
+  return u32x4{c0 | (c1 << 16), c2 | (c3 << 16), c4 | (c5 << 16), c6 | (c7 << 16)};
+}
+
+// A round's group words and what decodes them.
+struct PkRound {
+  uint32_t w0, w1, w2, w3, w4, sh, bw, eb;
+};
+
+// pr_wave_task over packed codes; round r of the task is block blk + r.  The task's width words
+// (8 a round, 16 rounds at most) are loaded at its start, two a lane, and handed to the lanes
+// that need them by ds_bpermute; a round's end bytes are loaded 6 rounds ahead of its use, its
+// group words (whose address needs the widths) 2 rounds ahead.
+__device__ inline void pr_wave_task_packed(const StripArgs &a, float *mb, uint32_t mbb, uint64_t sb, uint32_t e0,
+                                           uint32_t e1, uint32_t seg, uint32_t blk) {
+  const uint32_t lane = lane_id();
+  const uint32_t b0 = e0 & ~7u;
+  const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;  // at most kWaveTaskEdges / kRound + 1
+  const uint8_t *ebb = a.endb + (sb + b0) / 8;
+  // the task's block offsets, one per lane, and its width words, two per lane
+  const uint32_t pov = lane < nr ? a.pkoff[blk + lane] : 0u;
+  const uint32_t *hp = a.pkhdr + (size_t)blk * 8u;
+  const uint32_t hv0 = lane < 8u * nr ? hp[lane] : 0u;
+  const uint32_t hv1 = lane + kWave < 8u * nr ? hp[lane + kWave] : 0u;
+  const int hsrc = (int)(4u * (lane >> 3));  // ds_bpermute byte address of the lane's word in round 0
+  double carry = 0.0;
+  MsgBuf B{a.msg, mb, seg, 0u};
+  uint32_t eba[kAhead], hda[kAhead], boa[kAhead];  // round q's head in slot q % kAhead
+  auto load_head = [&](int i, uint32_t r) {
+    eba[i] = ebb[r * kWave + lane];
+    boa[i] = read_lane_u32(pov, (int)r);
+    hda[i] = (uint32_t)__builtin_amdgcn_ds_bpermute(hsrc + (int)(32u * (r & 7u)), (int)(r < 8u ? hv0 : hv1));
+  };
+  PkRound G[2];  // round q's groups in G[q % 2]
+  auto load_groups = [&](PkRound &g, int i) {
+    g.eb = eba[i];
+    g.bw = (hda[i] >> (4u * (lane & 7u))) & 15u;
+    const uint32_t bits = pk_group_bits(g.bw, g.eb);
+    const uint32_t o = wave_scan_dpp(bits) - bits;
+    const uint32_t *p = a.pk + boa[i] + (o >> 5);
+    g.sh = o & 31u;
+    // plain loads: a block's first and last lines are shared with the rounds before and after,
+    // which find them in L2 (streaming loads fetched them again, 6 % more bytes)
+    g.w0 = p[0];
+    g.w1 = p[1];
+    g.w2 = p[2];
+    g.w3 = p[3];
+    g.w4 = p[4];
+  };
+#pragma unroll
+  for (int i = 0; i < kAhead; i++) {
+    eba[i] = hda[i] = boa[i] = 0u;
+    if ((uint32_t)i < nr) load_head(i, (uint32_t)i);
+  }
+  G[0] = G[1] = PkRound{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+    if ((uint32_t)i < nr) {
+      load_groups(G[i], i);
+      if ((uint32_t)(i + kAhead) < nr) load_head(i, (uint32_t)(i + kAhead));
+    }
+  static_assert(kAhead % 2 == 0, "round parity must match the slot's");
+  for (uint32_t r = 0; r < nr; r += kAhead) {
+#pragma unroll
+    for (int i = 0; i < kAhead; i++) {
+      const uint32_t rr = r + (uint32_t)i;
+      if (rr < nr) {
+        PkRound &g = G[i & 1];
+        const u32x4 cv = pk_unpack(g.w0, g.w1, g.w2, g.w3, g.w4, g.sh, g.bw, g.eb);
+        const uint32_t ebr = g.eb;
+        if (rr + 2 < nr) {  // round rr + 2's groups, then its head slot refilled kAhead ahead
+          const int in = (i + 2) % kAhead;
+          load_groups(g, in);
+          if (rr + 2 + kAhead < nr) load_head(in, rr + 2 + kAhead);
+        }
+        const uint32_t base = b0 + rr * kRound;
+        const uint32_t q = base + 8 * lane;
+        const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
+        if (cut) strip_round<false>(B, mbb, cv, ebr, q, e0, e1, carry);
+        else strip_round<true>(B, mbb, cv, ebr, q, e0, e1, carry);
+      }
+    }
+  }
+  flush_messages(B);
+}
+
+// Messages of edges [e0, e1) whose first segment has index seg (blk: the task's first block of
+// packed codes).
 template <int MODE>
 __device__ inline void strip_wave_task(const StripArgs &a, float *mb, uint32_t mbb, uint64_t sb, uint32_t e0,
-                                       uint32_t e1, uint32_t seg) {
+                                       uint32_t e1, uint32_t seg, uint32_t blk) {
   if constexpr (MODE == kSweepPR) pr_wave_task(a, mb, mbb, sb, e0, e1, seg);
+  else if constexpr (MODE == kSweepPRPacked) pr_wave_task_packed(a, mb, mbb, sb, e0, e1, seg, blk);
   else min_wave_task<MODE>(a, mb, mbb, sb, e0, e1, seg);
 }
 
@@ -579,7 +734,7 @@ __device__ inline void strip_wave_task(const StripArgs &a, float *mb, uint32_t m
 template <int MODE>
 __device__ inline void strip_gather(const StripArgs &a, float *sv) {
   const uint32_t W = 1u << a.wlog;
-  const uint32_t *vals = MODE != kSweepPR ? a.nlab : reinterpret_cast<const uint32_t *>(a.contrib_in);
+  const uint32_t *vals = !is_pr(MODE) ? a.nlab : reinterpret_cast<const uint32_t *>(a.contrib_in);
   const uint32_t wv = wave_id();
   float *mb = sv + W + wv * kRound;
   const uint32_t mbb = 4u * (W + wv * kRound);  // mb as an LDS address
@@ -595,7 +750,7 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
     const uint4 gt = a.gtask[t];
     // a sweep skips strips none of whose vertices changed in the previous sweep; the bins apply
     // the messages of gathered strips only
-    if (MODE != kSweepPR) {
+    if (!is_pr(MODE)) {
       if (a.sflag[gt.x] + 1u != a.sweep) continue;
       if (threadIdx.x == 0) {
         atomicAdd(a.edges, (unsigned long long)(a.wtask[gt.z - 1].y - a.wtask[gt.y].x));
@@ -605,7 +760,7 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
     const uint32_t v0 = gt.x << a.wlog;
     const uint32_t n = min(W, a.V - v0);
     __syncthreads();  // the previous task's readers are done
-    if (MODE != kSweepPR && a.fmask) {
+    if (!is_pr(MODE) && a.fmask) {
       // pull round of delta-stepping: complemented distances of the frontier's sources, 0 (no
       // offer) for every other source
       uint32_t *su = reinterpret_cast<uint32_t *>(sv);
@@ -648,9 +803,12 @@ __device__ inline void strip_gather(const StripArgs &a, float *sv) {
     }
     __syncthreads();
     const uint64_t sb = a.sbase[gt.x];
+    const uint32_t blk = MODE == kSweepPRPacked ? a.pkblk[gt.x] : 0u;
     for (uint32_t w = gt.y + wv; w < gt.z; w += kIterWaves) {
       const uint4 wt = a.wtask[w];
-      strip_wave_task<MODE>(a, mb, mbb, sb, wt.x, wt.y, wt.z);
+      // a task's rounds start at a multiple of kRound from the strip's base (the first task at
+      // 0, the others at multiples of kWaveTaskEdges), one packed block each
+      strip_wave_task<MODE>(a, mb, mbb, sb, wt.x, wt.y, wt.z, blk + (wt.x & ~7u) / kRound);
     }
   }
   // the last workgroup to leave resets the queue for the next launch
@@ -668,6 +826,98 @@ __global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a
   extern __shared__ float sv[];  // contributions of the task's strip, then a message buffer per wave
   if (a.sc->done) return;
   strip_gather<kSweepPR>(a, sv);
+}
+
+// The same gather over the packed codes.
+__global__ __launch_bounds__(kIterBlock) void pr_strip_gather_packed_kernel(StripArgs a) {
+  extern __shared__ float sv[];
+  if (a.sc->done) return;
+  strip_gather<kSweepPRPacked>(a, sv);
+}
+
+// ---- packing the codes (one wave per block; block k covers edges blkpos[k] .. + kRound - 1)
+
+// Lane group at edge q (a multiple of 8): its codes, end byte and width.
+struct PkGroup {
+  uint32_t c[8];
+  uint32_t eb, b;
+  bool desc;  // a code below the one before it inside a segment (the layout never has one)
+};
+
+__device__ inline PkGroup pk_group(const uint16_t *code, const uint8_t *endb, uint64_t q) {
+  PkGroup g;
+  const u32x4 v = *reinterpret_cast<const u32x4 *>(code + q);
+#pragma unroll
+  for (int k = 0; k < 8; k++) g.c[k] = (k & 1) ? v[k >> 1] >> 16 : v[k >> 1] & 0xFFFFu;
+  g.eb = endb[q / 8];
+  uint32_t m = 0;  // the width of the widest difference is the width of their OR
+  g.desc = false;
+#pragma unroll
+  for (int j = 1; j < 8; j++)
+    if (!((g.eb >> (j - 1)) & 1u)) {
+      m |= g.c[j] - g.c[j - 1];
+      g.desc |= g.c[j] < g.c[j - 1];
+    }
+  g.b = g.desc ? 15u : m ? 32u - (uint32_t)__clz(m) : 0u;
+  return g;
+}
+
+// words[k] = the words of block k; *bad set where codes descend inside a segment
+__global__ void pk_size_kernel(const uint16_t *code, const uint8_t *endb, const uint64_t *blkpos, uint32_t nblk,
+                               uint32_t *words, uint32_t *bad) {
+  const uint32_t lane = lane_id(), wpb = blockDim.x / kWave;
+  for (uint32_t k = blockIdx.x * wpb + threadIdx.x / kWave; k < nblk; k += gridDim.x * wpb) {
+    const PkGroup g = pk_group(code, endb, blkpos[k] + 8u * lane);
+    if (g.desc) atomicOr(bad, 1u);
+    const uint32_t tot = read_lane_u32(wave_scan_dpp(pk_group_bits(g.b, g.eb)), kWave - 1);
+    if (lane == 0) words[k] = (tot + 31u) / 32u;
+  }
+}
+
+// v (below 2^w) at bit pos of the wave's block image w
+__device__ inline void pk_put(uint32_t *w, uint32_t &pos, uint32_t v, uint32_t wd) {
+  if (!wd) return;
+  const uint32_t s = pos & 31u;
+  atomicOr(&w[pos >> 5], v << s);
+  if (s + wd > 32u) atomicOr(&w[(pos >> 5) + 1], v >> (32u - s));
+  pos += wd;
+}
+
+constexpr uint32_t kPkBlockWords = 8 + (kWave * 120 + 31) / 32;  // a block's most words (248)
+
+// Block k to pk + off[k], its widths to hdr + 8 k: assembled in LDS (groups share words; the
+// image's first 8 words are the widths), then stored whole.
+__global__ __launch_bounds__(kBlock) void pk_write_kernel(const uint16_t *code, const uint8_t *endb,
+                                                          const uint64_t *blkpos, const uint32_t *off, uint32_t nblk,
+                                                          uint32_t *pk, uint32_t *hdr) {
+  __shared__ uint32_t img[kBlock / kWave][kPkBlockWords];
+  const uint32_t lane = lane_id(), wv = threadIdx.x / kWave, wpb = blockDim.x / kWave;
+  uint32_t *w = img[wv];
+  for (uint32_t k = blockIdx.x * wpb + wv; k < nblk; k += gridDim.x * wpb) {
+    for (uint32_t i = lane; i < kPkBlockWords; i += kWave) w[i] = 0u;
+    wave_sync();
+    const PkGroup g = pk_group(code, endb, blkpos[k] + 8u * lane);
+    const uint32_t bits = pk_group_bits(g.b, g.eb);
+    const uint32_t incl = wave_scan_dpp(bits);
+    const uint32_t tot = read_lane_u32(incl, kWave - 1);
+    atomicOr(&w[lane >> 3], g.b << (4u * (lane & 7u)));
+    uint32_t pos = 256u + incl - bits;
+    pk_put(w, pos, g.c[0], 15u);
+#pragma unroll
+    for (int j = 1; j < 8; j++) {
+      const bool st = (g.eb >> (j - 1)) & 1u;
+      pk_put(w, pos, st ? g.c[j] : g.c[j] - g.c[j - 1], st ? 15u : g.b);
+    }
+    wave_sync();
+    const uint32_t nw = (tot + 31u) / 32u;
+    if (lane < 8u) hdr[(size_t)k * 8u + lane] = w[lane];
+    for (uint32_t i = lane; i < nw; i += kWave) pk[off[k] + i] = w[8u + i];
+    wave_sync();
+  }
+}
+
+__global__ void narrow_offsets_kernel(const uint64_t *in, uint32_t n, uint32_t *out) {
+  GS64(i, n) out[i] = (uint32_t)in[i];
 }
 
 // complemented labels or distances of the task's strip, then the message buffers
@@ -1376,7 +1626,16 @@ void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2)
   a.pd1 = io.pd1;
   a.pr2 = io.pr2;
   a.pd2 = io.pd2;
-  pr_strip_gather_kernel<<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+  if (io.packed) {
+    if (!S.pk_valid) fail(GMP_ESTATE, "packed strip codes not built");
+    a.pk = S.pk.as<uint32_t>();
+    a.pkhdr = S.pkhdr.as<uint32_t>();
+    a.pkoff = S.pkoff.as<uint32_t>();
+    a.pkblk = S.pkblk.as<uint32_t>();
+    pr_strip_gather_packed_kernel<<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+  } else {
+    pr_strip_gather_kernel<<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+  }
   GMP_CHECK_LAUNCH();
   pr_bin_kernel<<<S.g2, kIterBlock, S.lds2, e.stream>>>(a);
   GMP_CHECK_LAUNCH();
@@ -1387,6 +1646,82 @@ void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2)
     GMP_CHECK_LAUNCH();
     n2 = S.g3;
   }
+}
+
+bool ensure_strip_pack(Engine &e) {
+  if (!ensure_strips(e)) return false;
+  StripCsr &S = e.g.strips;
+  if (S.pk_valid) return true;
+  const uint64_t E = e.g.E;
+  // blocks: each strip's rounds from its base, up to its last wave task's end
+  std::vector<uint4> wt(S.nwt);
+  std::vector<uint64_t> sb(S.ns);
+  GMP_HIP(hipMemcpyAsync(wt.data(), S.wtask.p, (size_t)S.nwt * sizeof(uint4), hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipMemcpyAsync(sb.data(), S.sbase.p, (size_t)S.ns * 8, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  std::vector<uint32_t> send(S.ns, 0u);
+  for (const uint4 &t : wt) {
+    // the gather finds a task's first block at its first edge / kRound (rounded down)
+    if ((t.x & ~7u) % kRound) return false;
+    send[t.w] = std::max(send[t.w], t.y);
+  }
+  std::vector<uint32_t> pkblk(S.ns);
+  std::vector<uint64_t> pos;
+  for (uint32_t s = 0; s < S.ns; s++) {
+    pkblk[s] = (uint32_t)pos.size();
+    const uint32_t nb = (send[s] + kRound - 1) / kRound;
+    for (uint32_t r = 0; r < nb; r++) pos.push_back(sb[s] + (uint64_t)r * kRound);
+  }
+  if (pos.empty() || pos.size() >= 0xFFFFFFFFull || pos.back() + kRound > E + kCodePad) return false;
+  const uint32_t nblk = (uint32_t)pos.size();
+  DevBuf dpos, words, off64, tmp, bad;
+  dpos.alloc((size_t)nblk * 8);
+  words.alloc((size_t)nblk * 4);
+  bad.alloc(4);
+  GMP_HIP(hipMemsetAsync(bad.p, 0, 4, e.stream));
+  off64.alloc((size_t)nblk * 8);
+  GMP_HIP(hipMemcpyAsync(dpos.p, pos.data(), (size_t)nblk * 8, hipMemcpyHostToDevice, e.stream));
+  const uint32_t wg = std::max<uint32_t>(1, std::min<uint32_t>((nblk + 3) / 4, 1u << 16));
+  pk_size_kernel<<<wg, kBlock, 0, e.stream>>>(S.code.as<uint16_t>(), S.endb.as<uint8_t>(), dpos.as<uint64_t>(), nblk,
+                                             words.as<uint32_t>(), bad.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  size_t tb = 0;
+  GMP_HIP(rocprim::exclusive_scan(nullptr, tb, words.as<uint32_t>(), off64.as<uint64_t>(), (uint64_t)0, nblk,
+                                  rocprim::plus<uint64_t>(), e.stream));
+  tmp.alloc(tb);
+  GMP_HIP(rocprim::exclusive_scan(tmp.p, tb, words.as<uint32_t>(), off64.as<uint64_t>(), (uint64_t)0, nblk,
+                                  rocprim::plus<uint64_t>(), e.stream));
+  uint64_t lo = 0;
+  uint32_t lw = 0, hbad = 0;
+  GMP_HIP(hipMemcpyAsync(&lo, off64.as<uint64_t>() + nblk - 1, 8, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipMemcpyAsync(&lw, words.as<uint32_t>() + nblk - 1, 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, e.stream));
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  if (hbad) return false;  // the differences would not be codes
+  const uint64_t total = lo + lw;
+  constexpr uint32_t kPkPad = 8;  // a group's five-word read from the last block stays in the stream
+  if (total + kPkPad >= 0xFFFFFFFFull) return false;
+  S.pk.alloc((total + kPkPad) * 4);
+  S.pkoff.alloc((size_t)nblk * 4);
+  S.pkhdr.alloc((size_t)nblk * 32);
+  S.pkblk.alloc((size_t)S.ns * 4);
+  GMP_HIP(hipMemsetAsync(S.pk.as<uint32_t>() + total, 0, kPkPad * 4, e.stream));
+  narrow_offsets_kernel<<<grid_for(nblk), kBlock, 0, e.stream>>>(off64.as<uint64_t>(), nblk, S.pkoff.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  GMP_HIP(hipMemcpyAsync(S.pkblk.p, pkblk.data(), (size_t)S.ns * 4, hipMemcpyHostToDevice, e.stream));
+  pk_write_kernel<<<wg, kBlock, 0, e.stream>>>(S.code.as<uint16_t>(), S.endb.as<uint8_t>(), dpos.as<uint64_t>(),
+                                              S.pkoff.as<uint32_t>(), nblk, S.pk.as<uint32_t>(),
+                                              S.pkhdr.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  check_gather_lds(reinterpret_cast<const void *>(&pr_strip_gather_packed_kernel));
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&pr_strip_gather_packed_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
+    (void)hipGetLastError();
+  S.pk_words = total;
+  S.pk_nblk = nblk;
+  S.pk_valid = true;
+  return true;
 }
 
 namespace {

@@ -490,6 +490,56 @@ def test_source_strips_keep_results(engine, strip_log, bin_log, task_log, gtask_
             engine.set_option(k, v)
 
 
+@pytest.mark.parametrize("strip_log,task_log,gtask_log", [(8, 8, 0), (11, 10, 12), (15, 20, 6), (15, 14, 2)])
+def test_packed_strip_codes_are_bit_identical(engine, strip_log, task_log, gtask_log):
+    """Compressed PageRank on the strip layout (source codes bit-packed per 512-edge round)
+    gives exactly the plain strip iteration's ranks, with and without a seed and a tolerance,
+    on graphs whose code differences need every width: consecutive sources (width 0), repeated
+    edges (a difference of -1, width 15), one-edge segments, strips of a few edges."""
+    for k, v in zip((gmp.OPT_STRIP_LOG, gmp.OPT_BIN_TASK_LOG, gmp.OPT_STRIP_TASK_LOG),
+                    (strip_log, task_log, gtask_log)):
+        engine.set_option(k, v)
+    rng = np.random.default_rng(strip_log)
+    try:
+        cases = [(f"rmat scale={scale} {kw}", (scale, kw)) for scale, kw in
+                 ((13, {}), (12, dict(undirected=True)), (11, dict(off64=True)))]
+        V = 40000
+        src = np.concatenate([np.arange(1, V), rng.integers(0, V, 100000), np.arange(2, 9000),
+                              np.repeat(rng.integers(0, V, 300), 7), np.arange(1000, 31000)])
+        dst = np.concatenate([np.zeros(V - 1, np.int64), rng.integers(0, V, 100000), np.ones(8998, np.int64),
+                              np.repeat(rng.integers(0, V, 300), 7), np.full(30000, 7, np.int64)])
+        off, idx = graphs.csr_from_edges(V, src, dst)
+        cases.append(("hub, pairs, repeats, a run of 30000 consecutive sources", (off, idx)))
+        V = 70001  # few edges per strip, E not a multiple of 8
+        src = rng.integers(0, V, 9997)
+        dst = rng.integers(0, V, 9997)
+        off, idx = graphs.csr_from_edges(V, src, dst)
+        cases.append(("sparse", (off, idx)))
+        for name, what in cases:
+            if isinstance(what[0], int):
+                g = gen(engine, what[0], **what[1])
+                off, idx = g["off"], g["idx"]
+            else:
+                off, idx = what
+                engine.load_csr(off, idx)
+            outdeg = graphs.out_degree(len(off) - 1, idx)
+            say(f"packed strips {strip_log}/{task_log}/{gtask_log}: {name}")
+            seed = int(np.argmax(outdeg))
+            for kw in (dict(max_iter=20), dict(max_iter=20, seed_vertex=seed), dict(max_iter=100, tol=1e-6)):
+                r0, s0 = engine.pagerank(**kw)
+                r1, s1 = engine.pagerank(compressed=True, **kw)
+                np.testing.assert_array_equal(r1.view(np.uint32), r0.view(np.uint32))
+                assert s1.iterations == s0.iterations
+                assert s0.index_bytes == 2 * len(idx)
+                assert 0 < s1.index_bytes
+            x, _, _ = oracle.pr_f64(off, idx, outdeg, max_iter=20)
+            r1, _ = engine.pagerank(max_iter=20, compressed=True)
+            assert np.abs(r1.astype(np.float64) - x).sum() < PR_L1_TOL
+    finally:
+        for k, v in STRIP_DEFAULTS:
+            engine.set_option(k, v)
+
+
 def test_selective_scheduling_changes_nothing(engine):
     """Schedule independence: with GraphMP's selective scheduling off (every strip in every sweep)
     CC labels, Bellman-Ford and delta-stepping distances (pull rounds on every round), sweep
