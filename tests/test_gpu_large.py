@@ -1,7 +1,8 @@
 """A graph past 2^32 edges resident in HBM: R-MAT s28 ef17 (4.5e9 edges).  The source-strip
 layout counts each strip's edges from a 64-bit base, so it runs such graphs; its PageRank must
 equal the row gather's (4-byte ids, 64-bit row offsets) to rounding, and stay within 1e-6 L1 of
-the FP64 fixed point by a one-step bound."""
+the FP64 fixed point by a one-step bound.  Its packed codes (compressed ids) give the same ranks
+bit for bit."""
 import numpy as np
 import pytest
 
@@ -19,6 +20,11 @@ def test_strips_past_2e32_edges():
         assert E > 2**32
         r1, st1 = e.pagerank(max_iter=100, tol=1e-7)
         assert st1.index_bytes == 2 * E  # the strip layout ran
+        rc, stc = e.pagerank(max_iter=100, tol=1e-7, compressed=True)
+        assert 0 < stc.index_bytes < 2 * E  # the packed codes ran
+        assert stc.iterations == st1.iterations
+        np.testing.assert_array_equal(rc.view(np.uint32), r1.view(np.uint32))
+        del rc
         e.set_option(gmp.OPT_STRIP_LOG, 0)
         r0, st0 = e.pagerank(max_iter=st1.iterations, tol=0.0)
         assert st0.index_bytes == 4 * E
