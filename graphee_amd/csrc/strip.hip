@@ -56,7 +56,14 @@ constexpr int kPullUnroll = 4;                 // pull rounds: rows per thread i
 // What a sweep over the strips computes.  PageRank sums contributions; the others keep
 // complemented values (c = ~x), so the least label or distance is the greatest c and 0 -- what
 // DPP moves in past a row's start, and the value of an unreached vertex -- is neutral.
-enum SweepMode { kSweepPR = 0, kSweepCC = 1, kSweepUnit = 2, kSweepWeighted = 3, kSweepWeighted8 = 4, kSweepPRPacked = 5 };
+enum SweepMode {
+  kSweepPR = 0,
+  kSweepCC = 1,
+  kSweepUnit = 2,
+  kSweepWeighted = 3,
+  kSweepWeighted8 = 4,
+  kSweepPRPacked = 5,  // PageRank over the packed codes
+};
 constexpr bool is_pr(int mode) { return mode == kSweepPR || mode == kSweepPRPacked; }
 
 #define GS64(i, n) \
@@ -577,12 +584,12 @@ __device__ inline void pr_wave_task(const StripArgs &a, float *mb, uint32_t mbb,
 // ---- packed codes (kSweepPRPacked) ----------------------------------------------------------
 // A block serves one 512-edge round of a strip: 8 header words (a separate array) hold the 64
 // lanes' widths (4 bits each, lane l in bits 4 (l % 8) of word l / 8); the block's words in the
-// stream hold the lanes' groups bit after bit, least significant bit first.  A group is the lane's first code (15 bits) and then, for
-// edges 1..7, a field: the code itself (15 bits) where the edge starts a segment (the end byte
-// says the edge before it ends one), else the code minus the code before it (codes ascend inside
-// a segment), at the group's width.  A lane finds its group from the sizes of the groups before
-// it (a wave prefix sum) and gets its 8 codes back exactly, so the round proceeds as with the
-// plain codes and the sums are the same bits.
+// stream hold the lanes' groups bit after bit, least significant bit first.  A group is the
+// lane's first code (15 bits) and then, for edges 1..7, a field: the code itself (15 bits) where
+// the edge starts a segment (the end byte says the edge before it ends one), else the code minus
+// the code before it (codes ascend inside a segment), at the group's width.  A lane finds its
+// group from the sizes of the groups before it (a wave prefix sum) and gets its 8 codes back
+// exactly, so the round proceeds as with the plain codes and the sums are the same bits.
 
 // Bits of a group of width b whose end byte is eb.
 __device__ inline uint32_t pk_group_bits(uint32_t b, uint32_t eb) {
@@ -650,8 +657,9 @@ __device__ inline void pr_wave_task_packed(const StripArgs &a, float *mb, uint32
                                            uint32_t e1, uint32_t seg, uint32_t blk) {
   const uint32_t lane = lane_id();
   const uint32_t b0 = e0 & ~7u;
-  const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;  // at most kWaveTaskEdges / kRound + 1
+  const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;  // at most kWaveTaskEdges / kRound (the packer checks)
   const uint8_t *ebb = a.endb + (sb + b0) / 8;
+  static_assert(kWaveTaskEdges / kRound <= 16, "a task's width words must fit two words a lane");
   // the task's block offsets, one per lane, and its width words, two per lane
   const uint32_t pov = lane < nr ? a.pkoff[blk + lane] : 0u;
   const uint32_t *hp = a.pkhdr + (size_t)blk * 8u;
@@ -1661,8 +1669,9 @@ bool ensure_strip_pack(Engine &e) {
   GMP_HIP(hipStreamSynchronize(e.stream));
   std::vector<uint32_t> send(S.ns, 0u);
   for (const uint4 &t : wt) {
-    // the gather finds a task's first block at its first edge / kRound (rounded down)
-    if ((t.x & ~7u) % kRound) return false;
+    // the gather finds a task's first block at its first edge / kRound (rounded down) and holds
+    // the widths of at most kWaveTaskEdges / kRound rounds
+    if ((t.x & ~7u) % kRound || t.y - (t.x & ~7u) > kWaveTaskEdges) return false;
     send[t.w] = std::max(send[t.w], t.y);
   }
   std::vector<uint32_t> pkblk(S.ns);
