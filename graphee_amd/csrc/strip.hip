@@ -329,16 +329,23 @@ __device__ inline void flush_messages(MsgBuf &B) {
 // entirely past e1 (only the first and last rounds of a task can cut a lane).  The messages of
 // the segments that end in the round go to the buffer in segment order; the sum of the segment
 // left open stays in `carry`.
-template <bool FULL>
-__device__ inline void strip_round(MsgBuf &B, uint32_t mbb, u32x4 cv, uint32_t eb, uint32_t q,
+// LDS byte offset of code k of a lane's 8: two to a word as the plain layout holds them, or one
+// to a word (the packed codes' decoder).
+struct WordCodes {
+  uint32_t c[8];
+};
+__device__ inline uint32_t code_addr(u32x4 cv, int k) { return (k & 1) ? code_hi4(cv[k >> 1]) : code_lo4(cv[k >> 1]); }
+__device__ inline uint32_t code_addr(const WordCodes &cv, int k) { return cv.c[k] << 2; }
+
+template <bool FULL, class Codes>
+__device__ inline void strip_round(MsgBuf &B, uint32_t mbb, Codes cv, uint32_t eb, uint32_t q,
                                    uint32_t e0, uint32_t e1, double &carry) {
   const uint32_t lane = lane_id();
   float x[8];
   uint32_t ends = eb;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const uint32_t w = cv[k >> 1];
-    const float v = lds_ld((k & 1) ? code_hi4(w) : code_lo4(w));
+    const float v = lds_ld(code_addr(cv, k));
     if (FULL) {
       x[k] = v;
     } else {
@@ -597,22 +604,16 @@ __device__ inline uint32_t pk_group_bits(uint32_t b, uint32_t eb) {
   return 15u + 7u * b + nst * (15u - b);
 }
 
-// The 8 codes (two to a word, as the plain layout holds them) of the group whose first bit is
-// bit sh of w0; w0..w4 are the five words from there (a group has at most 120 bits).  Fields go
-// in pairs (two fit one word); after a pair the window shifts by their widths and keeps only the
-// words the fields left can fill (15 bits each at most).
-__device__ inline u32x4 pk_unpack(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t w4, uint32_t sh,
-                                  uint32_t b, uint32_t eb) {
+// The 8 codes of the group whose first bit is bit sh of w0; w0..w4 are the five words from there
+// (a group has at most 120 bits).  Fields go in pairs (two fit one word: the first code with
+// field 1, then fields 2-3, 4-5, 6-7); after a pair the window shifts by their widths and keeps
+// only the words the fields left can fill (15 bits each at most).
+__device__ inline WordCodes pk_unpack(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t w4, uint32_t sh,
+                                      uint32_t b, uint32_t eb) {
   uint32_t d0 = __builtin_amdgcn_alignbit(w1, w0, sh), d1 = __builtin_amdgcn_alignbit(w2, w1, sh);
   uint32_t d2 = __builtin_amdgcn_alignbit(w3, w2, sh), d3 = __builtin_amdgcn_alignbit(w4, w3, sh);
   const uint32_t neb = ~eb;
   const int bm = (int)b - 15;
-  uint32_t c0 = d0 & 0x7FFFu;
-  // fields 1..7 from bit 15: 105 bits at most
-  d0 = __builtin_amdgcn_alignbit(d1, d0, 15u);
-  d1 = __builtin_amdgcn_alignbit(d2, d1, 15u);
-  d2 = __builtin_amdgcn_alignbit(d3, d2, 15u);
-  d3 >>= 15;
   // field j: k = 1 for a difference (edge j - 1 does not end a segment) of width b, else a code
   // of 15 bits; code j = k * code (j - 1) + field
   auto keep = [&](int j) { return __builtin_amdgcn_ubfe(neb, (uint32_t)(j - 1), 1u); };
@@ -625,23 +626,25 @@ __device__ inline u32x4 pk_unpack(uint32_t w0, uint32_t w1, uint32_t w2, uint32_
   const uint32_t k1 = keep(1), k2 = keep(2), k3 = keep(3), k4 = keep(4), k5 = keep(5), k6 = keep(6), k7 = keep(7);
   const uint32_t x1 = width(k1), x2 = width(k2), x3 = width(k3), x4 = width(k4), x5 = width(k5), x6 = width(k6),
                  x7 = width(k7);
-  const uint32_t c1 = next(c0, k1, __builtin_amdgcn_ubfe(d0, 0u, x1));
-  const uint32_t c2 = next(c1, k2, __builtin_amdgcn_ubfe(d0, x1, x2));
-  uint32_t s = x1 + x2;  // 75 bits left at most
+  WordCodes o;
+  o.c[0] = d0 & 0x7FFFu;
+  o.c[1] = next(o.c[0], k1, __builtin_amdgcn_ubfe(d0, 15u, x1));
+  uint32_t s = 15u + x1;  // 90 bits left at most
   d0 = __builtin_amdgcn_alignbit(d1, d0, s);
   d1 = __builtin_amdgcn_alignbit(d2, d1, s);
   d2 = __builtin_amdgcn_alignbit(d3, d2, s);
-  const uint32_t c3 = next(c2, k3, __builtin_amdgcn_ubfe(d0, 0u, x3));
-  const uint32_t c4 = next(c3, k4, __builtin_amdgcn_ubfe(d0, x3, x4));
-  s = x3 + x4;  // 45 bits left at most
+  o.c[2] = next(o.c[1], k2, __builtin_amdgcn_ubfe(d0, 0u, x2));
+  o.c[3] = next(o.c[2], k3, __builtin_amdgcn_ubfe(d0, x2, x3));
+  s = x2 + x3;  // 60 bits left at most
   d0 = __builtin_amdgcn_alignbit(d1, d0, s);
   d1 = __builtin_amdgcn_alignbit(d2, d1, s);
-  const uint32_t c5 = next(c4, k5, __builtin_amdgcn_ubfe(d0, 0u, x5));
-  const uint32_t c6 = next(c5, k6, __builtin_amdgcn_ubfe(d0, x5, x6));
-  s = x5 + x6;  // 15 bits left at most
+  o.c[4] = next(o.c[3], k4, __builtin_amdgcn_ubfe(d0, 0u, x4));
+  o.c[5] = next(o.c[4], k5, __builtin_amdgcn_ubfe(d0, x4, x5));
+  s = x4 + x5;  // 30 bits left at most
   d0 = __builtin_amdgcn_alignbit(d1, d0, s);
-  const uint32_t c7 = next(c6, k7, __builtin_amdgcn_ubfe(d0, 0u, x7));
-  return u32x4{c0 | (c1 << 16), c2 | (c3 << 16), c4 | (c5 << 16), c6 | (c7 << 16)};
+  o.c[6] = next(o.c[5], k6, __builtin_amdgcn_ubfe(d0, 0u, x6));
+  o.c[7] = next(o.c[6], k7, __builtin_amdgcn_ubfe(d0, x6, x7));
+  return o;
 }
 
 // A round's group words and what decodes them.
@@ -666,6 +669,7 @@ __device__ inline void pr_wave_task_packed(const StripArgs &a, float *mb, uint32
   const uint32_t hv0 = lane < 8u * nr ? hp[lane] : 0u;
   const uint32_t hv1 = lane + kWave < 8u * nr ? hp[lane + kWave] : 0u;
   const int hsrc = (int)(4u * (lane >> 3));  // ds_bpermute byte address of the lane's word in round 0
+  const uint32_t hsh = 4u * (lane & 7u);      // the lane's width in that word
   double carry = 0.0;
   MsgBuf B{a.msg, mb, seg, 0u};
   uint32_t eba[kAhead], hda[kAhead], boa[kAhead];  // round q's head in slot q % kAhead
@@ -677,7 +681,7 @@ __device__ inline void pr_wave_task_packed(const StripArgs &a, float *mb, uint32
   PkRound G[2];  // round q's groups in G[q % 2]
   auto load_groups = [&](PkRound &g, int i) {
     g.eb = eba[i];
-    g.bw = (hda[i] >> (4u * (lane & 7u))) & 15u;
+    g.bw = __builtin_amdgcn_ubfe(hda[i], hsh, 4u);
     const uint32_t bits = pk_group_bits(g.bw, g.eb);
     const uint32_t o = wave_scan_dpp(bits) - bits;
     const uint32_t *p = a.pk + boa[i] + (o >> 5);
@@ -709,7 +713,7 @@ __device__ inline void pr_wave_task_packed(const StripArgs &a, float *mb, uint32
       const uint32_t rr = r + (uint32_t)i;
       if (rr < nr) {
         PkRound &g = G[i & 1];
-        const u32x4 cv = pk_unpack(g.w0, g.w1, g.w2, g.w3, g.w4, g.sh, g.bw, g.eb);
+        const WordCodes cv = pk_unpack(g.w0, g.w1, g.w2, g.w3, g.w4, g.sh, g.bw, g.eb);
         const uint32_t ebr = g.eb;
         if (rr + 2 < nr) {  // round rr + 2's groups, then its head slot refilled kAhead ahead
           const int in = (i + 2) % kAhead;
