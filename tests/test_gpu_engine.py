@@ -515,6 +515,9 @@ def test_packed_strip_codes_are_bit_identical(engine, strip_log, task_log, gtask
         dst = rng.integers(0, V, 9997)
         off, idx = graphs.csr_from_edges(V, src, dst)
         cases.append(("sparse", (off, idx)))
+        for V, src, dst in ((3, [0, 1, 2, 0], [1, 2, 0, 2]), (2, [0], [1]), (9, [4] * 9, list(range(9)))):
+            off, idx = graphs.csr_from_edges(V, np.array(src), np.array(dst))
+            cases.append((f"tiny V={V} E={len(src)}", (off, idx)))
         for name, what in cases:
             if isinstance(what[0], int):
                 g = gen(engine, what[0], **what[1])
@@ -835,7 +838,7 @@ def test_multigraph_self_loops_and_hubs(engine):
     x, _, _ = oracle.pr_f64(off, idx, od, max_iter=30)
     for strip_log in (15, 0):
         engine.set_option(gmp.OPT_STRIP_LOG, strip_log)
-        for compressed in ((False, True) if strip_log == 0 else (False,)):
+        for compressed in (False, True):
             r, st = engine.pagerank(max_iter=30, compressed=compressed)
             assert st.iterations == 30
             assert np.abs(r - x).sum() < PR_L1_TOL, (strip_log, compressed)
