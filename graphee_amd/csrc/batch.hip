@@ -884,7 +884,7 @@ void sssp_prepare(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *
   r.grid = std::max<uint32_t>(1, (uint32_t)e.num_cus * e.opt.bsssp_grid);
 }
 
-void sssp_start(Engine &e, SsspRun &r, hipStream_t s) {
+void sssp_start(SsspRun &r, hipStream_t s) {
   batch_sssp_init_kernel<<<grid_for(std::max(r.D, r.ngv)), kBlock, 0, s>>>(r.a.dist, r.D, r.a.mask0, r.a.mask1, r.ngv,
                                                                             r.a.count, r.a.ticket, r.a.edges,
                                                                             r.a.hubcnt);
@@ -1062,7 +1062,7 @@ __global__ void set_u32_kernel(uint32_t *p, uint32_t v) {
   if (threadIdx.x == 0) *p = v;
 }
 
-void ppr_start(Engine &e, PprRun &r, hipStream_t s) {
+void ppr_start(PprRun &r, hipStream_t s) {
   ppr_init_kernel<<<dim3(r.init_x, r.ng), kBlock, 0, s>>>(r.a.groups, r.a.seed_local, r.a.outdeg, r.rank[0], r.con[0],
                                                           r.dang[0]);
   set_u32_kernel<<<1, 64, 0, s>>>(r.d_iter.as<uint32_t>(), 0u);
@@ -1129,7 +1129,7 @@ void batch_sssp(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *so
   sssp_prepare(e, n, graph, source, r);
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
-  sssp_start(e, r, e.stream);
+  sssp_start(r, e.stream);
   // capture two rounds; the counters live on the device, so replays need no host input
   GraphExecGuard ge;
   GMP_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
@@ -1167,7 +1167,7 @@ void batch_ppr(Engine &e, uint32_t n, const uint32_t *graph, const uint32_t *see
   ppr_prepare(e, n, graph, seed, damping, r);
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
-  ppr_start(e, r, e.stream);
+  ppr_start(r, e.stream);
   GraphExecGuard ge;
   GMP_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
   ppr_iterations(e, r, e.stream, 0, iters, false, 0u);
@@ -1217,8 +1217,8 @@ void batch_run(Engine &e, uint32_t ns, const uint32_t *sgraph, const uint32_t *s
   GMP_HIP(hipEventCreateWithFlags(&sg.join, hipEventDisableTiming));
   EventPair tot;
   GMP_HIP(hipEventRecord(tot.a, e.stream));
-  if (ns) sssp_start(e, sr, e.stream);
-  if (np) ppr_start(e, pr, e.stream);
+  if (ns) sssp_start(sr, e.stream);
+  if (np) ppr_start(pr, e.stream);
   GraphExecGuard ge;
   GMP_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
   GMP_HIP(hipEventRecord(sg.fork, e.stream));

@@ -274,10 +274,12 @@ void rebuild_csr(Engine &e, const CsrView &src, uint32_t V, uint64_t E, bool off
 void download_offsets(Engine &e, const Csr &c, uint32_t V, bool off64, std::vector<uint64_t> &h) {
   h.resize((size_t)V + 1);
   if (off64) {
-    GMP_HIP(hipMemcpy(h.data(), c.off.p, (size_t)(V + 1) * 8, hipMemcpyDeviceToHost));
+    GMP_HIP(hipMemcpyAsync(h.data(), c.off.p, (size_t)(V + 1) * 8, hipMemcpyDeviceToHost, e.stream));
+    GMP_HIP(hipStreamSynchronize(e.stream));
   } else {
     std::vector<uint32_t> o32((size_t)V + 1);
-    GMP_HIP(hipMemcpy(o32.data(), c.off.p, (size_t)(V + 1) * 4, hipMemcpyDeviceToHost));
+    GMP_HIP(hipMemcpyAsync(o32.data(), c.off.p, (size_t)(V + 1) * 4, hipMemcpyDeviceToHost, e.stream));
+    GMP_HIP(hipStreamSynchronize(e.stream));
     for (size_t r = 0; r <= V; r++) h[r] = o32[r];
   }
 }

@@ -359,7 +359,6 @@ void compat_windows_step(Engine &e, int app, int step, uint32_t grid_cap, double
                          void *value, const int32_t *vout, unsigned long long *changed) {
   SemiExternal &s = *e.g.semi;
   const bool minapp = app != GMP_APP_PAGERANK;
-  const uint32_t nwin = (uint32_t)s.win.size();
   const uint8_t *act = nullptr;
   if (minapp && step > 0 && e.opt.selective) act = window_activity(e, (uint32_t)step);  // the stop rule is the changed count
   // messages are double-buffered, so the order of the windows changes nothing; window_order
