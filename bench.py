@@ -711,6 +711,26 @@ def main():
         log(f"pagerank compressed: {stc.index_bytes / E:.3f} B/edge index ({stp.index_bytes / E:.3f} uncompressed), "
             f"{kms_c:.3f} ms/iteration, bit-identical {same}")
         if strip_now:
+            # the strip gather over each edge's 4-byte source id, the index exactly as the in-CSR
+            # stores it (the yardstick's 4 B per edge), in place of the 2-byte offset in the strip
+            eng.pagerank(damping=0.85, max_iter=3, strip_ids=True, want_ranks=False)  # builds the ids
+            ri, sti = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, strip_ids=True)
+            _, sti = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, strip_ids=True,
+                                  want_ranks=False)
+            rp, stp = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol)
+            same_i = bool(np.array_equal(ri.view(np.uint32), rp.view(np.uint32))) and sti.iterations == stp.iterations
+            del ri, rp
+            kms_i = sti.ms_kernel / max(sti.launches, 1)
+            result["extra"]["pagerank_strip_ids"] = {
+                "layout": "source strips, 4-byte source ids", "ms_per_solve": round(sti.ms_total, 3),
+                "iterations": sti.iterations, "gteps": round(E * sti.iterations / sti.ms_total / 1e6, 3),
+                "ms_per_iteration": round(kms_i, 4), "yardstick_gbs": round(bytes_launch / kms_i / 1e6, 1),
+                "yardstick_frac": round(bytes_launch / kms_i / 1e6 / HBM_PEAK_GBS, 4),
+                "bit_identical_to_codes": same_i, "index_bytes": sti.index_bytes,
+                "index_bytes_per_edge": round(sti.index_bytes / E, 3)}
+            log(f"pagerank strip ids: {sti.index_bytes / E:.3f} B/edge index, {kms_i:.3f} ms/iteration = "
+                f"{bytes_launch / kms_i / 1e6:.0f} GB/s on the yardstick, bit-identical {same_i}")
+        if strip_now:
             eng.set_option(gmp.OPT_STRIP_LOG, 0)
             eng.pagerank(damping=0.85, max_iter=3, compressed=True, want_ranks=False)  # builds the varint lists
             _, stv = eng.pagerank(damping=0.85, max_iter=args.max_iter, tol=args.tol, compressed=True,

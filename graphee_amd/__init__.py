@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libgmp.so")
 OK, EINVAL, EIO, EFORMAT, EHIP, ENOMEM, ESTATE, EUNSUPPORTED = range(8)
 APP_PAGERANK, APP_SSSP, APP_CC, APP_SSSP_W = 1, 2, 3, 4
 SSSP_BF, SSSP_DELTA, SSSP_LEARNED = 0, 1, 2
-PR_NO_KAHAN, PR_COMPRESSED = 1, 2
+PR_NO_KAHAN, PR_COMPRESSED, PR_STRIP_IDS = 1, 2, 4
 GRAPH_OFF64 = 1
 EDGES_WEIGHTED, EDGES_SYMMETRIZE = 1, 2
 OPT_REORDER, OPT_STREAM_CAP, OPT_ROW_CAP, OPT_CHUNK, OPT_WG_PER_CU, OPT_VEC_MIN = 1, 2, 3, 4, 5, 6
@@ -426,9 +426,9 @@ class Engine:
         return msg, val, st
 
     def pagerank(self, damping=0.85, max_iter=20, tol=0.0, seed_vertex=-1, kahan=True,
-                 compressed=False, want_ranks=True):
+                 compressed=False, want_ranks=True, strip_ids=False):
         V = self.info()[0]
-        flags = (0 if kahan else PR_NO_KAHAN) | (PR_COMPRESSED if compressed else 0)
+        flags = (0 if kahan else PR_NO_KAHAN) | (PR_COMPRESSED if compressed else 0) | (PR_STRIP_IDS if strip_ids else 0)
         p = PrParams(damping, max_iter, tol, seed_vertex, flags, 0)
         out = np.zeros(V, dtype=np.float32) if want_ranks else None
         st = Stats()

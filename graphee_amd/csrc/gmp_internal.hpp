@@ -129,6 +129,9 @@ struct StripCsr {
   DevBuf pkhdr;                 // uint32[8 blocks]: the widths
   DevBuf pkoff;                 // uint32[blocks]: first word of each block
   DevBuf pkblk;                 // uint32[ns]: first block of each strip
+  DevBuf ids;                   // uint32[E+pad]: each edge's source work id in layout order (the
+                                //   index as the in-CSR stores it: GMP_PR_STRIP_IDS)
+  bool ids_valid = false;
   uint64_t pk_words = 0;        // words of the stream (padding not counted)
   uint32_t pk_nblk = 0;
   bool pk_valid = false;
@@ -479,6 +482,9 @@ bool ensure_strips(Engine &e);
 // The packed form of the layout's codes (built from them on the host); false when there is no
 // strip layout or the stream would pass 2^32 words.
 bool ensure_strip_pack(Engine &e);
+// The layout's edges as 4-byte source work ids (built from the codes on the device); false when
+// there is no strip layout.
+bool ensure_strip_ids(Engine &e);
 // Min-type sweeps one at a time, for switching between sweeps and frontier pushes.  kind: 0 CC,
 // 1 unit-weight SSSP, 2 weighted SSSP.  c holds complemented values (c = ~x, 0 = unreached) in
 // work order; vstamp (V entries, or null) receives, per vertex, the number of the last sweep that

@@ -572,6 +572,8 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
   const bool strips = packed || (!(p.flags & GMP_PR_COMPRESSED) && kahan && ensure_strips(e));
   const bool comp = (p.flags & GMP_PR_COMPRESSED) && !strips;
   if (comp) ensure_packed(e);
+  // the strip layout's edges as 4-byte source ids in place of the 2-byte codes (same sums)
+  const bool ids = strips && !packed && (p.flags & GMP_PR_STRIP_IDS) && ensure_strip_ids(e);
   const uint32_t np = std::max(G, strips ? strip_partials(e) : 0u);
   // with a tolerance, the strip kernels run one iteration ahead of the host's residual check
   const bool ahead = strips && p.tol > 0;
@@ -636,7 +638,7 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     uint32_t nfin1 = G, nfin2 = Gh;
     if (strips) {
       const PrStripIO io{a.outdeg, contrib[cur], rank[cur], rank[cur ^ 1], contrib[cur ^ 1], sc, p.damping, seed,
-                         pr1, pd1, pr2, pd2, packed};
+                         pr1, pd1, pr2, pd2, packed, ids};
       strip_iteration(e, io, nfin1, nfin2);
     } else {
       if (comp) {
@@ -711,6 +713,7 @@ void pagerank_impl(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats
     const StripCsr &S = g.strips;
     st->index_bytes = packed ? 4 * (S.pk_words + 9 * (uint64_t)S.pk_nblk)
                       : comp ? g.packed.nbytes + 16 * g.packed.ngroups
+                      : ids  ? 4 * g.E
                       : strips ? 2 * g.E
                                : 4 * g.E;
   }
@@ -781,6 +784,8 @@ void compat_impl(Engine &e, int app, int max_iter, const void *msg_init, const i
 void run_pagerank(Engine &e, const gmp_pr_params &p, float *rank_out, gmp_stats *st) {
   if (!(p.damping >= 0.0f && p.damping <= 1.0f)) fail(GMP_EINVAL, "damping must be in [0, 1]");
   if (p.seed_vertex >= (int64_t)e.g.V) fail(GMP_EINVAL, "seed vertex out of range");
+  if ((p.flags & GMP_PR_STRIP_IDS) && (p.flags & GMP_PR_COMPRESSED))
+    fail(GMP_EINVAL, "GMP_PR_STRIP_IDS reads uncompressed ids: not with GMP_PR_COMPRESSED");
   if (e.g.off64) pagerank_impl<uint64_t>(e, p, rank_out, st);
   else pagerank_impl<uint32_t>(e, p, rank_out, st);
 }

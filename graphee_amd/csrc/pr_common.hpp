@@ -117,6 +117,7 @@ struct PrStripIO {
   double *pr1, *pd1;  // FP64 residual/dangling partials of the bin kernel, one per block
   double *pr2, *pd2;  // and of the split-bin kernel
   bool packed;        // the gather reads the packed codes (ensure_strip_pack)
+  bool ids;           // the gather reads 4-byte source ids (ensure_strip_ids)
 };
 
 // FP64 partials that strip_iteration may write into each of pr1, pd1, pr2, pd2.

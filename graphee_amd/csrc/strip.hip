@@ -63,8 +63,9 @@ enum SweepMode {
   kSweepWeighted = 3,
   kSweepWeighted8 = 4,
   kSweepPRPacked = 5,  // PageRank over the packed codes
+  kSweepPRIds = 6,     // PageRank over 4-byte source ids
 };
-constexpr bool is_pr(int mode) { return mode == kSweepPR || mode == kSweepPRPacked; }
+constexpr bool is_pr(int mode) { return mode == kSweepPR || mode == kSweepPRPacked || mode == kSweepPRIds; }
 
 #define GS64(i, n) \
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
@@ -100,6 +101,7 @@ struct StripArgs {
   const uint32_t *pkhdr;  // each block's 8 width words,
   const uint32_t *pkoff;  // each block's first word,
   const uint32_t *pkblk;  // each strip's first block
+  const uint32_t *ids;    // source work ids in layout order (kSweepPRIds)
   const uint4 *wtask;
   const uint4 *gtask;
   uint32_t ngt;
@@ -171,6 +173,18 @@ __global__ void task_ends_kernel(const uint4 *wt, const uint64_t *sbase, uint32_
   GS64(t, n) {
     const uint64_t e0 = sbase[wt[t].w] + wt[t].x;
     if (e0 > 0) code[e0 - 1] |= (uint16_t)0x8000u;
+  }
+}
+
+// ids[i]: the source work id of edge i in layout order, the strip's first id plus the edge's
+// code; a block per wave task
+__global__ void strip_ids_kernel(const uint4 *wt, const uint64_t *sbase, uint32_t n, uint32_t wlog,
+                                 const uint16_t *code, uint32_t *ids) {
+  for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
+    const uint4 w = wt[t];
+    const uint64_t b = sbase[w.w];
+    const uint32_t s0 = w.w << wlog, m = (1u << wlog) - 1u;
+    for (uint32_t i = w.x + threadIdx.x; i < w.y; i += blockDim.x) ids[b + i] = s0 | (code[b + i] & m);
   }
 }
 
@@ -336,6 +350,14 @@ struct WordCodes {
 };
 __device__ inline uint32_t code_addr(u32x4 cv, int k) { return (k & 1) ? code_hi4(cv[k >> 1]) : code_lo4(cv[k >> 1]); }
 __device__ inline uint32_t code_addr(const WordCodes &cv, int k) { return cv.c[k] << 2; }
+// 4-byte source ids: the low wlog bits (the offset in the strip) times 4, m4 = (2^wlog - 1) * 4.
+// The mask also keeps ids read past a task's end (padding, the next strip's edges) inside the
+// strip's values.
+struct IdWords {
+  u32x4 lo, hi;
+  uint32_t m4;
+};
+__device__ inline uint32_t code_addr(const IdWords &cv, int k) { return ((k < 4 ? cv.lo[k] : cv.hi[k - 4]) << 2) & cv.m4; }
 
 template <bool FULL, class Codes>
 __device__ inline void strip_round(MsgBuf &B, uint32_t mbb, Codes cv, uint32_t eb, uint32_t q,
@@ -731,6 +753,53 @@ __device__ inline void pr_wave_task_packed(const StripArgs &a, float *mb, uint32
   flush_messages(B);
 }
 
+// pr_wave_task over 4-byte source ids: a lane's 8 ids are two 16-byte loads (a round reads 2 KiB
+// of ids against 1 KiB of codes); the sums are the same bits as over the codes.
+__device__ inline void pr_wave_task_ids(const StripArgs &a, float *mb, uint32_t mbb, uint64_t sb, uint32_t e0,
+                                        uint32_t e1, uint32_t seg) {
+  const uint32_t lane = lane_id();
+  const uint32_t b0 = e0 & ~7u;
+  const uint32_t nr = (e1 - b0 + kRound - 1) / kRound;
+  const u32x4 *ib = reinterpret_cast<const u32x4 *>(a.ids + sb + b0);  // a round is 2 kWave words of 4 ids
+  const uint8_t *ebb = a.endb + (sb + b0) / 8;
+  const uint32_t m4 = ((1u << a.wlog) - 1u) << 2;
+  double carry = 0.0;
+  MsgBuf B{a.msg, mb, seg, 0u};
+  u32x4 blo[kAhead], bhi[kAhead];
+  uint32_t eba[kAhead];
+#pragma unroll
+  for (int i = 0; i < kAhead; i++) {
+    blo[i] = bhi[i] = u32x4{0u, 0u, 0u, 0u};
+    eba[i] = 0u;
+    if ((uint32_t)i < nr) {
+      blo[i] = ib[2 * ((uint32_t)i * kWave + lane)];
+      bhi[i] = ib[2 * ((uint32_t)i * kWave + lane) + 1];
+      eba[i] = ebb[(uint32_t)i * kWave + lane];
+    }
+  }
+  for (uint32_t r = 0; r < nr; r += kAhead) {
+#pragma unroll
+    for (int i = 0; i < kAhead; i++) {
+      if (r + i < nr) {
+        const IdWords cv{blo[i], bhi[i], m4};
+        const uint32_t eb = eba[i];
+        const uint32_t rn = r + i + kAhead;
+        if (rn < nr) {
+          blo[i] = ib[2 * (rn * kWave + lane)];
+          bhi[i] = ib[2 * (rn * kWave + lane) + 1];
+          eba[i] = ebb[rn * kWave + lane];
+        }
+        const uint32_t base = b0 + (r + i) * kRound;
+        const uint32_t q = base + 8 * lane;
+        const bool cut = ((e0 & 7u) && base == b0) || ((e1 & 7u) && base + kRound > e1);
+        if (cut) strip_round<false>(B, mbb, cv, eb, q, e0, e1, carry);
+        else strip_round<true>(B, mbb, cv, eb, q, e0, e1, carry);
+      }
+    }
+  }
+  flush_messages(B);
+}
+
 // Messages of edges [e0, e1) whose first segment has index seg (blk: the task's first block of
 // packed codes).
 template <int MODE>
@@ -738,6 +807,7 @@ __device__ inline void strip_wave_task(const StripArgs &a, float *mb, uint32_t m
                                        uint32_t e1, uint32_t seg, uint32_t blk) {
   if constexpr (MODE == kSweepPR) pr_wave_task(a, mb, mbb, sb, e0, e1, seg);
   else if constexpr (MODE == kSweepPRPacked) pr_wave_task_packed(a, mb, mbb, sb, e0, e1, seg, blk);
+  else if constexpr (MODE == kSweepPRIds) pr_wave_task_ids(a, mb, mbb, sb, e0, e1, seg);
   else min_wave_task<MODE>(a, mb, mbb, sb, e0, e1, seg);
 }
 
@@ -841,6 +911,12 @@ __global__ __launch_bounds__(kIterBlock) void pr_strip_gather_kernel(StripArgs a
 }
 
 // The same gather over the packed codes.
+__global__ __launch_bounds__(kIterBlock) void pr_strip_gather_ids_kernel(StripArgs a) {
+  extern __shared__ float sv[];
+  if (a.sc->done) return;
+  strip_gather<kSweepPRIds>(a, sv);
+}
+
 __global__ __launch_bounds__(kIterBlock) void pr_strip_gather_packed_kernel(StripArgs a) {
   extern __shared__ float sv[];
   if (a.sc->done) return;
@@ -1645,6 +1721,10 @@ void strip_iteration(Engine &e, const PrStripIO &io, uint32_t &n1, uint32_t &n2)
     a.pkoff = S.pkoff.as<uint32_t>();
     a.pkblk = S.pkblk.as<uint32_t>();
     pr_strip_gather_packed_kernel<<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
+  } else if (io.ids) {
+    if (!S.ids_valid) fail(GMP_ESTATE, "strip source ids not built");
+    a.ids = S.ids.as<uint32_t>();
+    pr_strip_gather_ids_kernel<<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
   } else {
     pr_strip_gather_kernel<<<S.g1, kIterBlock, S.lds1, e.stream>>>(a);
   }
@@ -1734,6 +1814,26 @@ bool ensure_strip_pack(Engine &e) {
   S.pk_words = total;
   S.pk_nblk = nblk;
   S.pk_valid = true;
+  return true;
+}
+
+bool ensure_strip_ids(Engine &e) {
+  if (!ensure_strips(e)) return false;
+  StripCsr &S = e.g.strips;
+  if (S.ids_valid) return true;
+  const uint64_t E = e.g.E;
+  // kCodePad entries past the end keep the last round's 16-byte loads in bounds
+  S.ids.alloc((E + kCodePad) * 4);
+  GMP_HIP(hipMemsetAsync(S.ids.p, 0, (E + kCodePad) * 4, e.stream));
+  strip_ids_kernel<<<std::min<uint32_t>(S.nwt, 1u << 20), kBlock, 0, e.stream>>>(
+      S.wtask.as<uint4>(), S.sbase.as<uint64_t>(), S.nwt, S.wlog, S.code.as<uint16_t>(), S.ids.as<uint32_t>());
+  GMP_CHECK_LAUNCH();
+  GMP_HIP(hipStreamSynchronize(e.stream));
+  check_gather_lds(reinterpret_cast<const void *>(&pr_strip_gather_ids_kernel));
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&pr_strip_gather_ids_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)S.lds1) != hipSuccess)
+    (void)hipGetLastError();
+  S.ids_valid = true;
   return true;
 }
 

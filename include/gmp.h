@@ -51,6 +51,9 @@ extern "C" {
 /* gmp_pr_params.flags */
 #define GMP_PR_NO_KAHAN 1u   /* plain FP32 gather sum instead of the compensated one */
 #define GMP_PR_COMPRESSED 2u /* gather from the delta/varint neighbour lists */
+#define GMP_PR_STRIP_IDS 4u  /* source-strip gather reads each edge's 4-byte source id, as the
+                                in-CSR stores it, in place of its 2-byte offset in the strip
+                                (same ranks, bit for bit; not with GMP_PR_COMPRESSED) */
 
 typedef struct gmp_engine gmp_engine;
 

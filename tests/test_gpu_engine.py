@@ -495,7 +495,9 @@ def test_packed_strip_codes_are_bit_identical(engine, strip_log, task_log, gtask
     """Compressed PageRank on the strip layout (source codes bit-packed per 512-edge round)
     gives exactly the plain strip iteration's ranks, with and without a seed and a tolerance,
     on graphs whose code differences need every width: consecutive sources (width 0), repeated
-    edges (a difference of -1, width 15), one-edge segments, strips of a few edges."""
+    edges (a difference of -1, width 15), one-edge segments, strips of a few edges.  So does the
+    gather over 4-byte source ids (GMP_PR_STRIP_IDS), whose last rounds read ids of the next
+    strip or of the padding past a task's end."""
     for k, v in zip((gmp.OPT_STRIP_LOG, gmp.OPT_BIN_TASK_LOG, gmp.OPT_STRIP_TASK_LOG),
                     (strip_log, task_log, gtask_log)):
         engine.set_option(k, v)
@@ -535,6 +537,10 @@ def test_packed_strip_codes_are_bit_identical(engine, strip_log, task_log, gtask
                 assert s1.iterations == s0.iterations
                 assert s0.index_bytes == 2 * len(idx)
                 assert 0 < s1.index_bytes
+                r2, s2 = engine.pagerank(strip_ids=True, **kw)
+                np.testing.assert_array_equal(r2.view(np.uint32), r0.view(np.uint32))
+                assert s2.iterations == s0.iterations
+                assert s2.index_bytes == 4 * len(idx)
             x, _, _ = oracle.pr_f64(off, idx, outdeg, max_iter=20)
             r1, _ = engine.pagerank(max_iter=20, compressed=True)
             assert np.abs(r1.astype(np.float64) - x).sum() < PR_L1_TOL
@@ -765,6 +771,9 @@ def test_argument_errors(engine):
     assert ei.value.code == gmp.EINVAL
     with pytest.raises(gmp.GmpError) as ei:
         engine.load_csr(np.array([0, 1, 1], dtype=np.uint64), np.array([9], dtype=np.uint32))
+    assert ei.value.code == gmp.EINVAL
+    with pytest.raises(gmp.GmpError) as ei:  # 4-byte ids are the uncompressed form
+        engine.pagerank(compressed=True, strip_ids=True)
     assert ei.value.code == gmp.EINVAL
     with gmp.Engine(0) as fresh:
         with pytest.raises(gmp.GmpError) as ei:

@@ -1,8 +1,8 @@
 """A graph past 2^32 edges resident in HBM: R-MAT s28 ef17 (4.5e9 edges).  The source-strip
 layout counts each strip's edges from a 64-bit base, so it runs such graphs; its PageRank must
 equal the row gather's (4-byte ids, 64-bit row offsets) to rounding, and stay within 1e-6 L1 of
-the FP64 fixed point by a one-step bound.  Its packed codes (compressed ids) give the same ranks
-bit for bit."""
+the FP64 fixed point by a one-step bound.  Its packed codes (compressed ids) and its 4-byte source
+ids give the same ranks bit for bit."""
 import numpy as np
 import pytest
 
@@ -25,6 +25,10 @@ def test_strips_past_2e32_edges():
         assert stc.iterations == st1.iterations
         np.testing.assert_array_equal(rc.view(np.uint32), r1.view(np.uint32))
         del rc
+        ri, sti = e.pagerank(max_iter=100, tol=1e-7, strip_ids=True)
+        assert sti.index_bytes == 4 * E  # the 4-byte source ids ran
+        np.testing.assert_array_equal(ri.view(np.uint32), r1.view(np.uint32))
+        del ri
         e.set_option(gmp.OPT_STRIP_LOG, 0)
         r0, st0 = e.pagerank(max_iter=st1.iterations, tol=0.0)
         assert st0.index_bytes == 4 * E
