@@ -92,14 +92,17 @@ __device__ inline void block_store_acc(PrAcc acc, double *pr, double *pd) {
 // exact and independent of order, so unordered accumulation stays deterministic.
 constexpr int kFixBits = 61;
 
+// Without branches (a message's conversion sits inside the bin kernel's unrolled loop): x >= 2,
+// and any bit pattern with the sign set, saturates to 2^62 through the clamp to 2.0f's bits;
+// zero and subnormals (< 2^-126, below the grid) shift right by 63 to 0.
 __device__ inline unsigned long long fix_from_float(float x) {
-  const uint32_t bits = __float_as_uint(x);
-  const uint32_t ex = bits >> 23;  // x >= 0: no sign bit
-  if (ex == 0) return 0ull;        // zero or subnormal (< 2^-126): below the fixed-point grid
+  const uint32_t bits = min(__float_as_uint(x), 0x40000000u);
+  const int32_t ex = (int32_t)(bits >> 23);
   const unsigned long long m = (bits & 0x7FFFFFu) | 0x800000u;
-  if (ex >= 128) return 1ull << 62;  // x >= 2 never occurs for a contribution; saturate
   // x = m * 2^(ex-150); x * 2^61 = m * 2^(ex-89)
-  return ex >= 89 ? m << (ex - 89) : m >> min(89u - ex, 63u);
+  const unsigned long long l = m << (uint32_t)(max(ex, 89) - 89);
+  const unsigned long long r = m >> (uint32_t)min(max(89 - ex, 0), 63);
+  return ex >= 89 ? l : r;
 }
 
 __device__ inline double fix_to_double(unsigned long long s) { return (double)s * 0x1p-61; }
